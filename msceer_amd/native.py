@@ -1,0 +1,103 @@
+"""ctypes binding of libmsceer_hip.so (the C ABI in include/msceer.h).
+
+The library is loaded from this package directory.  There is no fallback:
+if the library is missing or no GPU is present, calls raise.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+
+PKG = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(PKG, "libmsceer_hip.so")
+
+DEVICE_PTRS = 0x1
+SKIP_LDIR = 0x2
+TIMING = 0x4
+
+_P = ctypes.c_void_p
+_I64 = ctypes.c_int64
+_U32 = ctypes.c_uint32
+_I32 = ctypes.c_int32
+
+# Every exported symbol with (restype, argtypes); include/msceer.h is the source of truth
+# and tests check that the two agree.
+SIGNATURES = {
+    "msceer_version": (ctypes.c_int, []),
+    "msceer_strerror": (ctypes.c_char_p, [ctypes.c_int]),
+    "msceer_create": (ctypes.c_int, [ctypes.POINTER(_P), ctypes.c_int]),
+    "msceer_destroy": (ctypes.c_int, [_P]),
+    "msceer_set_stream": (ctypes.c_int, [_P, _P]),
+    "msceer_last_timing": (ctypes.c_int, [_P, _P]),
+    "msceer_lstar_grad": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _U32]),
+}
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [
+        ("lstar_ms", ctypes.c_float),
+        ("ldir_ms", ctypes.c_float),
+        ("total_ms", ctypes.c_float),
+        ("ldir_rounds", ctypes.c_int32),
+        ("critical", ctypes.c_int64 * 4),
+    ]
+
+
+class MsceerError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load the HIP library (raises if it was not built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise MsceerError(f"{LIB_PATH} is missing: run `python -m msceer_amd.build` first")
+        handle = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+    return _lib
+
+
+def check(rc: int, what: str = "") -> None:
+    if rc != 0:
+        msg = lib().msceer_strerror(rc).decode()
+        raise MsceerError(f"{what}: {msg} ({rc})")
+
+
+class Context:
+    """A msceer_ctx on one device (owns a HIP stream and device scratch)."""
+
+    def __init__(self, device: int = 0):
+        self._ptr = _P()
+        check(lib().msceer_create(ctypes.byref(self._ptr), device), "msceer_create")
+
+    @property
+    def handle(self):
+        return self._ptr
+
+    def close(self):
+        if self._ptr:
+            lib().msceer_destroy(self._ptr)
+            self._ptr = _P()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_stream(self, stream_handle: int | None):
+        check(lib().msceer_set_stream(self._ptr, _P(stream_handle) if stream_handle else None),
+              "msceer_set_stream")
+
+    def last_timing(self) -> Timing:
+        t = Timing()
+        check(lib().msceer_last_timing(self._ptr, ctypes.byref(t)), "msceer_last_timing")
+        return t
