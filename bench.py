@@ -1,0 +1,158 @@
+#!/usr/bin/env python3
+"""Benchmark: steepest_lstar .grad (bit-exact lower-star gradient) on one MI355X.
+
+Workload: 512x512x512 float32, N(0,1) white noise smoothed by a Gaussian
+(sigma 2 voxels, seed 202).  One step = one full .grad (lower-star pairing +
+ldir passes) computed from values already resident in HBM into a device
+buffer.  Prints one JSON line.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--size N] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "Mvert/s: lower-star grad (bit-exact); accurate labels plain vs learned"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def read_traffic(name: str):
+    """HBM bytes per launch from a committed rocprofv3 PMC summary (profiles/<name>)."""
+    path = os.path.join(ROOT, "profiles", name)
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch="):
+            return float(line.split("=", 1)[1])
+    return None
+
+
+def cpu_baseline(f: np.ndarray, slab: int):
+    """The CPU oracle (a C++/OpenMP restatement of the reference path) on a slab
+    of the same volume, all host cores."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib  # noqa: E402  (checker / baseline only)
+
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    oracle_lib.set_threads(cores)
+    sample = np.ascontiguousarray(f[:slab])
+    oracle_lib.steepest_grad(sample[:8])  # warm the thread pool
+    t0 = time.perf_counter()
+    oracle_lib.steepest_grad(sample)
+    dt = time.perf_counter() - t0
+    Z, Y, X = sample.shape
+    return {"value": round(X * Y * Z / dt / 1e6, 3), "unit": "Mvert/s", "cores": cores, "kind": "port",
+            "sample": f"{X}x{Y}x{Z} slab (first {Z} z-planes) of the benchmark volume, oracle "
+                      f"max/min labelling + lower-star pairing + ldir + flags, {dt:.2f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=512)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-slab", type=int, default=96)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
+              "the multi-GPU sharding is described in DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+
+    from msceer_amd import hipmem, native, synth
+
+    n = args.size
+    log(f"generating {n}^3 smoothed noise (sigma 2, seed 202)")
+    f = synth.lstar_large(202, n)
+    Z, Y, X = f.shape
+    cells = (2 * X - 1) * (2 * Y - 1) * (2 * Z - 1)
+    verts = X * Y * Z
+
+    ctx = native.Context(0)
+    dv = hipmem.DeviceBuffer(f.nbytes)
+    dv.upload(f)
+    dg = hipmem.DeviceBuffer(cells)
+    lib = native.lib()
+
+    def step(timing):
+        flags = native.DEVICE_PTRS | (native.TIMING if timing else 0)
+        native.check(lib.msceer_lstar_grad(ctx.handle, ctypes.c_void_p(dv.addr), X, Y, Z,
+                                           ctypes.c_void_p(dg.addr), flags), "msceer_lstar_grad")
+
+    for _ in range(args.warmup):
+        step(False)
+    hipmem.synchronize()
+    lstar_ms, ldir_ms, rounds = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step(True)
+        t = ctx.last_timing()
+        lstar_ms.append(t.lstar_ms)
+        ldir_ms.append(t.ldir_ms)
+        rounds.append(t.ldir_rounds)
+    hipmem.synchronize()
+    elapsed = time.perf_counter() - t0
+    crit = list(ctx.last_timing().critical)
+    ms_per_step = elapsed * 1e3 / args.steps
+
+    algo_bytes = 4.0 * verts + 1.0 * cells  # 4 B read per vertex + 1 B written per cell
+    k_ms = float(np.mean(lstar_ms))
+    achieved = algo_bytes / (k_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": read_traffic(f"lstar_grad_{n}_traffic.txt"),
+                "kernel": "lstar_kernel", "kernel_ms": round(k_ms, 4),
+                "algorithmic_bytes": algo_bytes}
+
+    base = None
+    if not args.no_cpu:
+        log("timing the CPU baseline")
+        base = cpu_baseline(f, min(args.cpu_slab, Z))
+
+    out = {
+        "metric": METRIC,
+        "value": round(verts / (ms_per_step * 1e-3) / 1e6, 2),
+        "unit": "Mvert/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"steepest_lstar .grad, {X}x{Y}x{Z} float32, white noise smoothed "
+                               f"sigma=2 vox, seed 202; values resident in HBM",
+                   "parallelism": "single GPU"},
+        "roofline": roofline,
+        "cpu_baseline": base,
+        "detail": {"lstar_kernel_ms_mean": round(k_ms, 4),
+                   "ldir_ms_mean": round(float(np.mean(ldir_ms)), 4),
+                   "ldir_rounds": int(np.max(rounds)), "critical_cells_by_dim": crit},
+    }
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -6,8 +6,9 @@
 #include <cstring>
 #include <new>
 
-#include "../../include/msceer.h"
+#include "msceer.h"
 #include "internal.hpp"
+#include "mlist.hpp"
 
 using namespace msceer;
 
@@ -34,7 +35,20 @@ struct DevBuf {
 
 constexpr uint32_t kMinListCap = 1u << 16;
 constexpr uint32_t kMaxListCap = 1u << 26;   // entries per critical list
-constexpr uint32_t kMaxFrontCap = 1u << 26;  // entries per frontier buffer
+constexpr uint32_t kMaxFrontCap = 1u << 27;  // entries per frontier buffer
+
+// Control block (uint32 words, device; mirrored to pinned host memory):
+//   sub-list counts of the two critical lists and the three rotating
+//   frontier count arrays, critical statistics, flags.
+enum Ctl : int {
+    kCtlList1 = 0,
+    kCtlList2 = kSubs,
+    kCtlRound = 2 * kSubs,       // three arrays of kSubs
+    kCtlStats = 5 * kSubs,       // kSubs * 4
+    kCtlOverflow = 9 * kSubs,    // a list or frontier ran out of room
+    kCtlTail = 9 * kSubs + 1,    // shadow of the grad array's last partial word
+    kCtlWords = 9 * kSubs + 16
+};
 
 }  // namespace
 
@@ -42,8 +56,8 @@ struct msceer_ctx {
     int device = 0;
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
-    DevBuf values, grad, lists, front0, front1, counters;
-    uint32_t* h_counters = nullptr;  // pinned
+    DevBuf values, grad, lists, front0, front1, ctl;
+    uint32_t* h_ctl = nullptr;  // pinned mirror of the control block
     hipEvent_t ev[6] = {};
     msceer_timing last{};
 };
@@ -69,60 +83,78 @@ int check_dims(int64_t nx, int64_t ny, int64_t nz) {
     return MSCEER_OK;
 }
 
-// Read back counters [0, n) synchronously.
-int read_counters(msceer_ctx* ctx, int n) {
-    CHECK_HIP(hipMemcpyAsync(ctx->h_counters, ctx->counters.ptr, sizeof(uint32_t) * n,
+uint32_t* ctl_dev(msceer_ctx* ctx) { return static_cast<uint32_t*>(ctx->ctl.ptr); }
+
+// Copy control words [first, first+n) to the host mirror and wait.
+int read_ctl(msceer_ctx* ctx, int first, int n) {
+    CHECK_HIP(hipMemcpyAsync(ctx->h_ctl + first, ctl_dev(ctx) + first, sizeof(uint32_t) * n,
                              hipMemcpyDeviceToHost, ctx->stream));
     CHECK_HIP(hipStreamSynchronize(ctx->stream));
     return MSCEER_OK;
 }
 
-int zero_counter(msceer_ctx* ctx, int first, int n) {
-    CHECK_HIP(hipMemsetAsync(static_cast<uint32_t*>(ctx->counters.ptr) + first, 0, sizeof(uint32_t) * n,
-                             ctx->stream));
-    return MSCEER_OK;
+uint64_t list_total(const uint32_t* counts, uint32_t sub_cap) {
+    uint64_t t = 0;
+    for (int i = 0; i < kSubs; i++) t += std::min(counts[i], sub_cap);
+    return t;
 }
 
-// One ldir pass: frontier rounds from `start` (or from a sweep), with a sweep
-// after any overflow, until no item is left.
-int ldir_pass(msceer_ctx* ctx, const Dims& d, uint8_t* grad, uint32_t* tail, uint32_t item_dim,
-              uint32_t value, const uint32_t* start, uint32_t n_start, bool start_valid,
-              uint32_t front_cap, int* rounds) {
-    uint32_t* counters = static_cast<uint32_t*>(ctx->counters.ptr);
-    uint32_t* bufs[2] = {static_cast<uint32_t*>(ctx->front0.ptr), static_cast<uint32_t*>(ctx->front1.ptr)};
-    int cur = 0;
-    const uint32_t* in = start;
-    uint32_t n = n_start;
+bool list_overflowed(const uint32_t* counts, uint32_t sub_cap) {
+    for (int i = 0; i < kSubs; i++)
+        if (counts[i] > sub_cap) return true;
+    return false;
+}
+
+// One ldir pass.  Rounds are enqueued in groups without host synchronisation
+// (each reads its counts on the device); after each group one read-back shows
+// whether the frontier emptied.  A sweep rebuilds the frontier after any
+// overflow of the start list or of a frontier.
+int ldir_pass(msceer_ctx* ctx, const Dims& d, uint8_t* grad, uint32_t item_dim, uint32_t value,
+              const MList& start, bool start_valid, uint32_t front_sub_cap, int* rounds) {
+    uint32_t* ctl = ctl_dev(ctx);
+    uint32_t* tail = ctl + kCtlTail;
+    uint32_t* overflow = ctl + kCtlOverflow;
+    uint32_t* fronts[2] = {static_cast<uint32_t*>(ctx->front0.ptr), static_cast<uint32_t*>(ctx->front1.ptr)};
+    auto counts = [&](int k) { return ctl + kCtlRound + (k % 3) * kSubs; };
+    auto front = [&](int k) { return MList{fronts[k % 2], counts(k), front_sub_cap}; };
+    constexpr int kGroup = 8;
+    MList in = start;
+    int k = 0;
     bool need_sweep = !start_valid;
-    for (int guard = 0; guard < 1000000; guard++) {
+    CHECK_HIP(launch_zero(counts(0), kSubs, ctx->stream));
+    CHECK_HIP(launch_zero(overflow, 1, ctx->stream));
+    for (int guard = 0; guard < 100000; guard++) {
         if (need_sweep) {
-            int rc = zero_counter(ctx, kFrontCount, 2);
+            CHECK_HIP(launch_zero(counts(k), kSubs, ctx->stream));
+            CHECK_HIP(launch_zero(overflow, 1, ctx->stream));
+            CHECK_HIP(launch_ldir_sweep(d, grad, tail, item_dim, value, front(k), overflow, ctx->stream));
+            int rc = read_ctl(ctx, 0, kCtlWords);
             if (rc) return rc;
-            CHECK_HIP(launch_ldir_sweep(d, grad, tail, item_dim, value, bufs[cur], front_cap, counters,
-                                        ctx->stream));
-            rc = read_counters(ctx, kNumCounters);
-            if (rc) return rc;
-            n = std::min(ctx->h_counters[kFrontCount], front_cap);
-            need_sweep = ctx->h_counters[kFrontOverflow] != 0;
-            in = bufs[cur];
-            cur ^= 1;
+            const uint32_t* hc = ctx->h_ctl + kCtlRound + (k % 3) * kSubs;
+            need_sweep = ctx->h_ctl[kCtlOverflow] != 0;
             (*rounds)++;
+            const uint64_t n = list_total(hc, front_sub_cap);
             if (n == 0 && !need_sweep) return MSCEER_OK;
+            CHECK_HIP(launch_zero(overflow, 1, ctx->stream));
+            in = front(k);
+            k++;
+            CHECK_HIP(launch_zero(counts(k), kSubs, ctx->stream));
         }
-        while (n > 0) {
-            int rc = zero_counter(ctx, kFrontCount, 2);
-            if (rc) return rc;
-            CHECK_HIP(launch_ldir_expand(d, grad, tail, in, n, bufs[cur], front_cap, counters, value,
-                                         ctx->stream));
-            rc = read_counters(ctx, kNumCounters);
-            if (rc) return rc;
+        for (int j = 0; j < kGroup; j++) {
+            const MList out = front(k);
+            CHECK_HIP(launch_ldir_round(d, grad, tail, in, out, counts(k + 1), overflow, value, ctx->stream));
             (*rounds)++;
-            if (ctx->h_counters[kFrontOverflow]) need_sweep = true;
-            n = std::min(ctx->h_counters[kFrontCount], front_cap);
-            in = bufs[cur];
-            cur ^= 1;
+            in = out;
+            k++;
         }
-        if (!need_sweep) return MSCEER_OK;
+        int rc = read_ctl(ctx, 0, kCtlWords);
+        if (rc) return rc;
+        if (ctx->h_ctl[kCtlOverflow]) {
+            need_sweep = true;
+            continue;
+        }
+        const uint32_t* hc = ctx->h_ctl + kCtlRound + ((k - 1) % 3) * kSubs;
+        if (list_total(hc, front_sub_cap) == 0) return MSCEER_OK;
     }
     return MSCEER_EHIP;
 }
@@ -156,9 +188,9 @@ int msceer_create(msceer_ctx** out, int device) {
     ctx->device = device;
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->h_counters, sizeof(uint32_t) * kNumCounters, hipHostMallocDefault);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->h_ctl, sizeof(uint32_t) * kCtlWords, hipHostMallocDefault);
     for (int i = 0; i < 6 && e == hipSuccess; i++) e = hipEventCreate(&ctx->ev[i]);
-    if (e == hipSuccess) e = ctx->counters.ensure(sizeof(uint32_t) * kNumCounters);
+    if (e == hipSuccess) e = ctx->ctl.ensure(sizeof(uint32_t) * kCtlWords);
     if (e != hipSuccess) {
         msceer_destroy(ctx);
         return hip_fail(e);
@@ -177,8 +209,8 @@ int msceer_destroy(msceer_ctx* ctx) {
     ctx->lists.release();
     ctx->front0.release();
     ctx->front1.release();
-    ctx->counters.release();
-    if (ctx->h_counters) (void)hipHostFree(ctx->h_counters);
+    ctx->ctl.release();
+    if (ctx->h_ctl) (void)hipHostFree(ctx->h_ctl);
     for (int i = 0; i < 6; i++)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
@@ -219,39 +251,41 @@ int msceer_lstar_grad(msceer_ctx* ctx, const float* values, int64_t nx, int64_t 
         d_grad = static_cast<uint8_t*>(ctx->grad.ptr);
         CHECK_HIP(hipMemcpyAsync(ctx->values.ptr, values, sizeof(float) * d.verts, hipMemcpyHostToDevice, s));
     }
-    // critical-cell lists: sized to the volume, capped; overflow falls back to a sweep
-    const uint32_t list_cap = (uint32_t)std::min<uint64_t>(kMaxListCap, std::max<uint64_t>(kMinListCap, d.verts / 4));
-    const uint32_t front_cap = (uint32_t)std::min<uint64_t>(kMaxFrontCap, std::max<uint64_t>(kMinListCap, d.verts / 2));
-    CHECK_HIP(ctx->lists.ensure(sizeof(uint32_t) * 2 * (size_t)list_cap + 256));
-    CHECK_HIP(ctx->front0.ensure(sizeof(uint32_t) * (size_t)front_cap));
-    CHECK_HIP(ctx->front1.ensure(sizeof(uint32_t) * (size_t)front_cap));
-    uint32_t* counters = static_cast<uint32_t*>(ctx->counters.ptr);
-    uint32_t* list1 = static_cast<uint32_t*>(ctx->lists.ptr);
-    uint32_t* list2 = list1 + list_cap;
-    uint32_t* tail = list2 + list_cap;  // one word after the lists
+    // critical-cell lists and frontiers: sized to the volume, capped; any
+    // overflow falls back to a sweep
+    const uint64_t list_cap = std::min<uint64_t>(kMaxListCap, std::max<uint64_t>(kMinListCap, d.verts / 4));
+    const uint64_t front_cap = std::min<uint64_t>(kMaxFrontCap, std::max<uint64_t>(kMinListCap, d.verts));
+    const uint32_t list_sub = (uint32_t)((list_cap + kSubs - 1) / kSubs);
+    const uint32_t front_sub = (uint32_t)((front_cap + kSubs - 1) / kSubs);
+    CHECK_HIP(ctx->lists.ensure(sizeof(uint32_t) * 2 * (size_t)list_sub * kSubs));
+    CHECK_HIP(ctx->front0.ensure(sizeof(uint32_t) * (size_t)front_sub * kSubs));
+    CHECK_HIP(ctx->front1.ensure(sizeof(uint32_t) * (size_t)front_sub * kSubs));
+    uint32_t* ctl = ctl_dev(ctx);
+    uint32_t* l1 = static_cast<uint32_t*>(ctx->lists.ptr);
+    const MList list1{l1, ctl + kCtlList1, list_sub};
+    const MList list2{l1 + (size_t)list_sub * kSubs, ctl + kCtlList2, list_sub};
 
     if (timing) CHECK_HIP(hipEventRecord(ctx->ev[0], s));
-    rc = zero_counter(ctx, 0, kNumCounters);
-    if (rc) return rc;
-    LstarBuffers b{d_values, d_grad, nullptr, list1, list2, list_cap, counters};
+    CHECK_HIP(launch_zero(ctl, kCtlWords, s));
+    LstarBuffers b{d_values, d_grad, nullptr, &list1, &list2, ctl + kCtlStats, ctl + kCtlOverflow};
     if (timing) CHECK_HIP(hipEventRecord(ctx->ev[1], s));
     CHECK_HIP(launch_lstar(d, b, s));
     if (timing) CHECK_HIP(hipEventRecord(ctx->ev[2], s));
-    rc = read_counters(ctx, kNumCounters);
+    rc = read_ctl(ctx, 0, kCtlWords);
     if (rc) return rc;
     msceer_timing t{};
-    for (int i = 0; i < 4; i++) t.critical[i] = ctx->h_counters[kCritDim0 + i];
-    const bool lists_ok = ctx->h_counters[kCritOverflow] == 0;
-    const uint32_t n1 = std::min(ctx->h_counters[kCritCount1], list_cap);
-    const uint32_t n2 = std::min(ctx->h_counters[kCritCount2], list_cap);
+    for (int i = 0; i < kSubs; i++)
+        for (int k = 0; k < 4; k++) t.critical[k] += ctx->h_ctl[kCtlStats + 4 * i + k];
+    const bool lists_ok = ctx->h_ctl[kCtlOverflow] == 0;
 
     int rounds = 0;
     if (timing) CHECK_HIP(hipEventRecord(ctx->ev[3], s));
     if (!(flags & MSCEER_SKIP_LDIR)) {
+        uint32_t* tail = ctl + kCtlTail;
         CHECK_HIP(launch_ldir_tail(d, d_grad, tail, false, s));
-        rc = ldir_pass(ctx, d, d_grad, tail, 1, 2, list1, n1, lists_ok, front_cap, &rounds);
+        rc = ldir_pass(ctx, d, d_grad, 1, 2, list1, lists_ok, front_sub, &rounds);
         if (rc) return rc;
-        rc = ldir_pass(ctx, d, d_grad, tail, 2, 1, list2, n2, lists_ok, front_cap, &rounds);
+        rc = ldir_pass(ctx, d, d_grad, 2, 1, list2, lists_ok, front_sub, &rounds);
         if (rc) return rc;
         CHECK_HIP(launch_ldir_tail(d, d_grad, tail, true, s));
     }

@@ -22,28 +22,16 @@ inline Dims make_dims(int64_t x, int64_t y, int64_t z) {
     return d;
 }
 
-// Device counters block (uint32 slots).
-enum Counter : int {
-    kCritCount1 = 0,    // critical 1-cells appended to list 1
-    kCritCount2 = 1,    // critical 2-cells appended to list 2
-    kCritOverflow = 2,  // a critical list ran out of room
-    kFrontCount = 3,    // entries written to the outgoing frontier
-    kFrontOverflow = 4, // the outgoing frontier ran out of room
-    kCritDim0 = 8,      // critical cells by dimension (statistics)
-    kCritDim1 = 9,
-    kCritDim2 = 10,
-    kCritDim3 = 11,
-    kNumCounters = 16
-};
+struct MList;
 
 struct LstarBuffers {
-    const float* values;   // device, X*Y*Z
-    uint8_t* grad;         // device, cells
+    const float* values;        // device, X*Y*Z
+    uint8_t* grad;              // device, cells
     const int8_t* restriction;  // device, cells, or null (steepest pass)
-    uint32_t* crit_list1;  // device, crit_cap entries
-    uint32_t* crit_list2;
-    uint32_t crit_cap;
-    uint32_t* counters;    // device, kNumCounters
+    const MList* list1;         // critical 1-cells (ldir pass 1 start)
+    const MList* list2;         // critical 2-cells (ldir pass 2 start)
+    uint32_t* stats;            // device, kSubs*4: critical cells by sub-list and dimension
+    uint32_t* overflow;         // device flag: a critical list ran out of room
 };
 
 // Lower-star pairing of every vertex; writes every cell byte exactly once
@@ -54,13 +42,14 @@ hipError_t launch_lstar(const Dims& d, const LstarBuffers& b, hipStream_t s);
 // partial 32-bit word of the grad array: pack it before the passes
 // (unpack=false) and write it back after them (unpack=true).
 hipError_t launch_ldir_tail(const Dims& d, uint8_t* grad, uint32_t* tail, bool unpack, hipStream_t s);
-// One frontier round (see ldir.hip for the semantics).
-hipError_t launch_ldir_expand(const Dims& d, uint8_t* grad, uint32_t* tail, const uint32_t* in,
-                              uint32_t n_in, uint32_t* out, uint32_t out_cap, uint32_t* counters,
-                              uint32_t value, hipStream_t s);
-// Rebuild a frontier from the grad bytes alone (start, and after an overflow).
+// One frontier round (see ldir.hip): follows the items of `in`, appends what
+// is left to `out` and zeroes `clear_counts` (kSubs words) for the next round.
+hipError_t launch_ldir_round(const Dims& d, uint8_t* grad, uint32_t* tail, const MList& in, const MList& out,
+                             uint32_t* clear_counts, uint32_t* overflow, uint32_t value, hipStream_t s);
+// Rebuild a frontier from the grad bytes alone (after an overflow).
 hipError_t launch_ldir_sweep(const Dims& d, uint8_t* grad, uint32_t* tail, uint32_t item_dim,
-                             uint32_t value, uint32_t* out, uint32_t out_cap, uint32_t* counters,
-                             hipStream_t s);
+                             uint32_t value, const MList& out, uint32_t* overflow, hipStream_t s);
+// Zero n words.
+hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t s);
 
 }  // namespace msceer

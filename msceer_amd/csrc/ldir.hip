@@ -17,14 +17,17 @@
 #include <hip/hip_runtime.h>
 
 #include "internal.hpp"
+#include "mlist.hpp"
 
 namespace msceer {
 
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kStack = 24;        // per-lane depth-first stack (LDS)
-constexpr int kBudget = 1 << 12;  // items per lane before the rest is handed on
+constexpr int kPool = 8192;         // per-block LDS pool of pending items
+constexpr int kStage = 4096;        // per-block LDS staging of outgoing items (sweep)
+constexpr int kMaxSteps = 96;       // block steps per round before the rest is handed on
+constexpr int kRoundBlocks = 2048;  // fixed grid of a round
 
 struct CellGeom {
     uint32_t Mx, My, Mz;
@@ -73,72 +76,135 @@ __device__ __forceinline__ bool claim(const GradView& v, uint32_t c, uint32_t va
     }
 }
 
-__device__ __forceinline__ void push_out(uint32_t* out, uint32_t cap, uint32_t* counters, uint32_t item) {
-    const uint32_t slot = atomicAdd(&counters[kFrontCount], 1u);
-    if (slot < cap) {
-        out[slot] = item;
-    } else {
-        atomicOr(&counters[kFrontOverflow], 1u);
-    }
-}
-
-__global__ void __launch_bounds__(kThreads)
-ldir_expand_kernel(GradView v, CellGeom g, const uint32_t* __restrict__ in,
-                   uint32_t n_in, uint32_t* __restrict__ out, uint32_t cap,
-                   uint32_t* __restrict__ counters, uint32_t value) {
-    __shared__ uint32_t stack_mem[kThreads * kStack];
-    uint32_t* stack = stack_mem + threadIdx.x;  // strided: entry k at stack[k*kThreads]
-    const uint32_t i = blockIdx.x * kThreads + threadIdx.x;
-    if (i >= n_in) return;
-    int sp = 0;
-    stack[0] = in[i];
-    sp = 1;
-    int work = 0;
-    while (sp > 0) {
-        const uint32_t item = stack[(--sp) * kThreads];
-        if (work >= kBudget) {
-            push_out(out, cap, counters, item);
-            continue;
+// Outgoing items are staged per block in LDS and flushed with one
+// reservation; when the stage is full they go straight to the sub-list.
+struct Outbox {
+    uint32_t* stage;  // LDS, kStage
+    uint32_t* n;      // LDS counter
+    MList out;
+    int sub;
+    uint32_t* overflow;
+    __device__ __forceinline__ void push(uint32_t item) const {
+        const uint32_t slot = atomicAdd(n, 1u);
+        if (slot < (uint32_t)kStage) {
+            stage[slot] = item;
+        } else {
+            const uint32_t g = mlist_reserve(out, sub, 1u);
+            if (g < out.sub_cap) out.items[(size_t)sub * out.sub_cap + g] = item;
+            else atomicOr(overflow, 1u);
         }
-        work++;
-        if (!claim(v, item, value)) continue;
-        uint32_t x, y, z;
-        cell_coords(g, item, x, y, z);
-        const uint32_t pos[3] = {x, y, z};
-        const uint32_t ext[3] = {g.Mx, g.My, g.Mz};
-        const int64_t step[3] = {1, (int64_t)g.Mx, (int64_t)g.sxy};
+    }
+};
+
+// One round.  A block takes an even share of the incoming frontier into an
+// LDS pool and works it breadth-first: each step, every lane pops one item,
+// and the items found behind it go back into the pool, so a large manifold
+// reached from one start item is spread over the whole block.  After
+// kMaxSteps steps whatever is left goes to the outgoing frontier, which the
+// next round shares out again.  Counts are read on the device, so rounds are
+// enqueued back to back; `clear` (the counts the next round appends to) is
+// zeroed here.
+//
+// Items need no claim: a non-start item is the partner of exactly one mark,
+// and marks are claimed atomically, so each is pushed once per pass; start
+// items (critical cells) appear once in the start list.
+__global__ void __launch_bounds__(kThreads)
+ldir_round_kernel(GradView v, CellGeom g, MList in, MList out, uint32_t* __restrict__ clear,
+                  uint32_t* __restrict__ overflow, uint32_t value) {
+    __shared__ uint32_t pool[kPool];
+    __shared__ uint32_t prefix[kSubs + 1];
+    __shared__ uint32_t s_n, s_next, s_base;
+    const uint32_t tid = threadIdx.x;
+    if (blockIdx.x == 0 && tid < (uint32_t)kSubs) clear[tid] = 0u;
+    const uint32_t total = mlist_prefix(in, prefix);
+    const uint32_t chunk = (total + gridDim.x - 1) / gridDim.x;
+    const uint32_t lo = min(total, blockIdx.x * chunk), hi = min(total, lo + chunk);
+    const int sub = (int)(blockIdx.x % kSubs);
+    if (tid == 0) { s_n = 0; s_next = lo; }
+    __syncthreads();
+    for (int step = 0; step < kMaxSteps; step++) {
+        uint32_t n = min(s_n, (uint32_t)kPool);
+        const uint32_t nx = s_next;
+        if (n < (uint32_t)kThreads && nx < hi) {
+            const uint32_t take = min(hi - nx, min((uint32_t)kPool - n, 4u * kThreads));
+            for (uint32_t j = tid; j < take; j += kThreads) pool[n + j] = mlist_at(in, prefix, nx + j);
+            __syncthreads();
+            if (tid == 0) { s_n = n + take; s_next = nx + take; }
+            __syncthreads();
+            n += take;
+        }
+        if (n == 0) break;
+        const uint32_t m = min(n, (uint32_t)kThreads);
+        const uint32_t item = tid < m ? pool[n - 1 - tid] : 0xFFFFFFFFu;
+        __syncthreads();
+        if (tid == 0) s_n = n - m;
+        __syncthreads();
+        if (item != 0xFFFFFFFFu) {
+            uint32_t x, y, z;
+            cell_coords(g, item, x, y, z);
+            const uint32_t pos[3] = {x, y, z};
+            const uint32_t ext[3] = {g.Mx, g.My, g.Mz};
+            const int64_t step_off[3] = {1, (int64_t)g.Mx, (int64_t)g.sxy};
+            // the item itself: set its ldir (its other bits never change)
+            if (item >= v.tail_start) {
+                claim(v, item, value);
+            } else {
+                const uint32_t b = v.grad[item];
+                v.grad[item] = (uint8_t)((b & 0x1Fu) | (value << 5));
+            }
 #pragma unroll
-        for (int a = 0; a < 3; a++) {
-            if (pos[a] & 1u) continue;  // cofacets lie along even axes
+            for (int a = 0; a < 3; a++) {
+                if (pos[a] & 1u) continue;  // cofacets lie along even axes
 #pragma unroll
-            for (int sgn = 0; sgn < 2; sgn++) {
-                if (sgn == 0 && pos[a] == ext[a] - 1) continue;
-                if (sgn == 1 && pos[a] == 0) continue;
-                const uint32_t t = (uint32_t)((int64_t)item + (sgn ? -step[a] : step[a]));
-                const uint32_t b = v.grad[t];
-                const uint32_t dir = (b >> 2) & 7u;
-                if (dir == 7u || !(b & 1u)) continue;
-                const int64_t p64 = (int64_t)t + dir_offset(g, dir);
-                if (p64 < 0 || p64 >= (int64_t)g.cells) continue;  // only a corrupt byte points out
-                const uint32_t p = (uint32_t)p64;
-                if (p == item) continue;
-                if (!claim(v, t, value)) continue;
-                if (sp < kStack) {
-                    stack[(sp++) * kThreads] = p;
-                } else {
-                    push_out(out, cap, counters, p);
+                for (int sgn = 0; sgn < 2; sgn++) {
+                    if (sgn == 0 && pos[a] == ext[a] - 1) continue;
+                    if (sgn == 1 && pos[a] == 0) continue;
+                    const uint32_t t = (uint32_t)((int64_t)item + (sgn ? -step_off[a] : step_off[a]));
+                    const uint32_t b = v.grad[t];
+                    const uint32_t dir = (b >> 2) & 7u;
+                    if (dir == 7u || !(b & 1u)) continue;
+                    if (t < v.tail_start && (b >> 5) == value) continue;  // already marked
+                    const int64_t p64 = (int64_t)t + dir_offset(g, dir);
+                    if (p64 < 0 || p64 >= (int64_t)g.cells) continue;  // only a corrupt byte points out
+                    const uint32_t p = (uint32_t)p64;
+                    if (p == item) continue;
+                    if (!claim(v, t, value)) continue;
+                    const uint32_t slot = atomicAdd(&s_n, 1u);
+                    if (slot < (uint32_t)kPool) {
+                        pool[slot] = p;
+                    } else {
+                        const uint32_t gs = mlist_reserve(out, sub, 1u);
+                        if (gs < out.sub_cap) out.items[(size_t)sub * out.sub_cap + gs] = p;
+                        else atomicOr(overflow, 1u);
+                    }
                 }
             }
         }
+        __syncthreads();
+    }
+    // hand on what is left: the pool, then the unread part of the share
+    __syncthreads();
+    const uint32_t n = min(s_n, (uint32_t)kPool);
+    mlist_flush_block(out, sub, pool, n, &s_base, overflow);
+    for (uint32_t nx = s_next; nx < hi; nx += kPool) {
+        const uint32_t take = min(hi - nx, (uint32_t)kPool);
+        __syncthreads();
+        for (uint32_t j = tid; j < take; j += kThreads) pool[j] = mlist_at(in, prefix, nx + j);
+        __syncthreads();
+        mlist_flush_block(out, sub, pool, take, &s_base, overflow);
     }
 }
 
 // Frontier from the bytes alone: unclaimed critical item-dimension cells, and
 // the partner of every claimed mark whose partner is not yet claimed.
 __global__ void __launch_bounds__(kThreads)
-ldir_sweep_kernel(GradView v, CellGeom g, uint64_t cells, uint32_t item_dim,
-                  uint32_t value, uint32_t* __restrict__ out, uint32_t cap,
-                  uint32_t* __restrict__ counters) {
+ldir_sweep_kernel(GradView v, CellGeom g, uint64_t cells, uint32_t item_dim, uint32_t value, MList out,
+                  uint32_t* __restrict__ overflow) {
+    __shared__ uint32_t stage[kStage];
+    __shared__ uint32_t s_n, s_base;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    const Outbox box{stage, &s_n, out, (int)(blockIdx.x % kSubs), overflow};
     const uint32_t mark_dim = item_dim + 1;
     for (uint64_t c = (uint64_t)blockIdx.x * kThreads + threadIdx.x; c < cells;
          c += (uint64_t)gridDim.x * kThreads) {
@@ -149,14 +215,21 @@ ldir_sweep_kernel(GradView v, CellGeom g, uint64_t cells, uint32_t item_dim,
         const uint32_t dir = (b >> 2) & 7u;
         const uint32_t ldir = ldir_of(v, (uint32_t)c);
         if (dim == item_dim && dir == 7u && ldir != value) {
-            push_out(out, cap, counters, (uint32_t)c);
+            box.push((uint32_t)c);
         } else if (dim == mark_dim && dir != 7u && (b & 1u) && ldir == value) {
             const int64_t p64 = (int64_t)c + dir_offset(g, dir);
             if (p64 < 0 || p64 >= (int64_t)g.cells) continue;
             const uint32_t p = (uint32_t)p64;
-            if (ldir_of(v, p) != value) push_out(out, cap, counters, p);
+            if (ldir_of(v, p) != value) box.push(p);
         }
     }
+    __syncthreads();
+    const uint32_t n = s_n < (uint32_t)kStage ? s_n : (uint32_t)kStage;
+    mlist_flush_block(out, box.sub, stage, n, &s_base, overflow);
+}
+
+__global__ void zero_kernel(uint32_t* p, uint32_t n) {
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) p[i] = 0u;
 }
 
 // Shadow of the tail bytes: packed before the passes, unpacked after.
@@ -191,25 +264,29 @@ hipError_t launch_ldir_tail(const Dims& d, uint8_t* grad, uint32_t* tail, bool u
     return hipGetLastError();
 }
 
-hipError_t launch_ldir_expand(const Dims& d, uint8_t* grad, uint32_t* tail, const uint32_t* in,
-                              uint32_t n_in, uint32_t* out, uint32_t out_cap, uint32_t* counters,
-                              uint32_t value, hipStream_t s) {
-    if (n_in == 0) return hipSuccess;
+hipError_t launch_ldir_round(const Dims& d, uint8_t* grad, uint32_t* tail, const MList& in, const MList& out,
+                             uint32_t* clear_counts, uint32_t* overflow, uint32_t value, hipStream_t s) {
     CellGeom g{d.Mx, d.My, d.Mz, d.Mx * d.My, (uint32_t)d.cells};
-    const uint32_t blocks = (n_in + kThreads - 1) / kThreads;
-    hipLaunchKernelGGL(ldir_expand_kernel, dim3(blocks), dim3(kThreads), 0, s, make_view(d, grad, tail),
-                       g, in, n_in, out, out_cap, counters, value);
+    hipLaunchKernelGGL(ldir_round_kernel, dim3(kRoundBlocks), dim3(kThreads), 0, s, make_view(d, grad, tail),
+                       g, in, out, clear_counts, overflow, value);
     return hipGetLastError();
 }
 
 hipError_t launch_ldir_sweep(const Dims& d, uint8_t* grad, uint32_t* tail, uint32_t item_dim,
-                             uint32_t value, uint32_t* out, uint32_t out_cap, uint32_t* counters,
-                             hipStream_t s) {
+                             uint32_t value, const MList& out, uint32_t* overflow, hipStream_t s) {
     CellGeom g{d.Mx, d.My, d.Mz, d.Mx * d.My, (uint32_t)d.cells};
     uint64_t blocks = (d.cells + kThreads - 1) / kThreads;
-    if (blocks > 65536) blocks = 65536;
+    if (blocks > 8192) blocks = 8192;
     hipLaunchKernelGGL(ldir_sweep_kernel, dim3((uint32_t)blocks), dim3(kThreads), 0, s,
-                       make_view(d, grad, tail), g, d.cells, item_dim, value, out, out_cap, counters);
+                       make_view(d, grad, tail), g, d.cells, item_dim, value, out, overflow);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    uint32_t blocks = (n + 255) / 256;
+    if (blocks > 1024) blocks = 1024;
+    hipLaunchKernelGGL(zero_kernel, dim3(blocks), dim3(256), 0, s, p, n);
     return hipGetLastError();
 }
 

@@ -1,0 +1,296 @@
+// Per-vertex lower-star logic shared by the gfx950 kernels and by their
+// host-side emulation in the tests.  Bit conventions are those of star.hpp.
+#pragma once
+#include <stdint.h>
+
+#include "star.hpp"
+
+namespace msceer {
+
+// Order-preserving key of a float: +0 and -0 map to the same key.
+MSC_HD uint32_t float_key(float f) {
+    uint32_t u = __builtin_bit_cast(uint32_t, f);
+    if ((u & 0x7fffffffu) == 0u) u = 0u;
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+
+MSC_HD int tid_x(int t) { return t - 3 * ((t * 11) >> 5) - 1; }
+MSC_HD int tid_y(int t) { return ((t * 11) >> 5) - 3 * ((t * 57) >> 9) - 1; }
+MSC_HD int tid_z(int t) { return ((t * 57) >> 9) - 1; }
+
+// Components of bit t from the constant axis masks (cheaper than the
+// divisions above when t varies per lane).
+MSC_HD int comp_x(int t) { return (int)((kXP >> t) & 1u) - (int)((kXM >> t) & 1u); }
+MSC_HD int comp_y(int t) { return (int)((kYP >> t) & 1u) - (int)((kYM >> t) & 1u); }
+MSC_HD int comp_z(int t) { return (int)((kZP >> t) & 1u) - (int)((kZM >> t) & 1u); }
+
+// Keys of the 27 vertices around one vertex, read through fixed strides (an
+// LDS brick) or run-time strides (a padded host volume).
+template <int SX, int SXY>
+struct FixedKeys {
+    const uint32_t* p;
+    int base;
+    MSC_HD uint32_t at(int t) const { return p[base + tid_x(t) + SX * tid_y(t) + SXY * tid_z(t)]; }
+};
+// One lane's 27 keys laid out t-major with a fixed lane stride (an LDS slice):
+// key t of the lane is p[t * kStride].
+template <int kStride>
+struct SliceKeys {
+    const uint32_t* p;
+    MSC_HD uint32_t at(int t) const { return p[t * kStride]; }
+};
+struct StridedKeys {
+    const uint32_t* p;
+    int64_t base, sx, sxy;
+    MSC_HD uint32_t at(int t) const { return p[base + tid_x(t) + sx * tid_y(t) + sxy * tid_z(t)]; }
+};
+
+// (key, star bit) of a neighbour vertex: comparing these as 64-bit integers is
+// the reference order restricted to one neighbourhood, because star-bit order
+// is global-id order for the vertices of one 3x3x3 block.
+template <class Keys>
+MSC_HD uint64_t vkey(const Keys& k, int t) {
+    return ((uint64_t)k.at(t) << 5) | (uint64_t)t;
+}
+
+// Bits of the cofacets of the cell at bit a (one zero component moved to +-1).
+MSC_HD uint32_t cofacet_mask(int a) {
+    const uint32_t ba = 1u << a;
+    uint32_t m = 0;
+    if (kX0 & ba) m |= (ba << 1) | (ba >> 1);
+    if (kY0 & ba) m |= (ba << 3) | (ba >> 3);
+    if (kZ0 & ba) m |= (ba << 9) | (ba >> 9);
+    return m;
+}
+
+// Lowest vertex of a 2-cell (quad) at bit c of the lower star: its vertices
+// other than the centre are its two edge ends and its far corner.
+template <class Keys>
+MSC_HD uint64_t quad_lowest(const Keys& k, int c) {
+    const int dx = comp_x(c), dy = comp_y(c), dz = comp_z(c);
+    // the two edges: zero one of the two nonzero components
+    const int e1 = dx ? c - dx : c - 3 * dy;           // first nonzero axis zeroed
+    const int e2 = dz ? c - 9 * dz : c - 3 * dy;       // last nonzero axis zeroed
+    uint64_t a = vkey(k, c), b = vkey(k, e1), d = vkey(k, e2);
+    a = b < a ? b : a;
+    return d < a ? d : a;
+}
+
+// Lowest vertex of a 3-cell (hex): all 7 of its vertices below the centre.
+template <class Keys>
+MSC_HD uint64_t hex_lowest(const Keys& k, int c) {
+    const int dx = comp_x(c), dy = comp_y(c), dz = comp_z(c);
+    uint64_t best = vkey(k, c);
+#pragma unroll
+    for (int s = 1; s < 7; s++) {
+        const int q = c - ((s & 1) ? dx : 0) - ((s & 2) ? 3 * dy : 0) - ((s & 4) ? 9 * dz : 0);
+        const uint64_t kq = vkey(k, q);
+        best = kq < best ? kq : best;
+    }
+    return best;
+}
+
+struct StarResult {
+    uint32_t c0 = 0, c1 = 0, c2 = 0;  // bits of each cell's 3-bit pair code
+    uint32_t crit = 0;                // critical cells
+    MSC_HD void set_pair(int a, int b) {
+        const uint32_t ca = star_dir_code(a, b), cb = ca ^ 1u;
+        c0 |= ((ca & 1u) << a) | ((cb & 1u) << b);
+        c1 |= (((ca >> 1) & 1u) << a) | (((cb >> 1) & 1u) << b);
+        c2 |= ((ca >> 2) << a) | ((cb >> 2) << b);
+    }
+    MSC_HD void set_critical(uint32_t m) {
+        c0 |= m; c1 |= m; c2 |= m; crit |= m;
+    }
+};
+
+// First candidate, in cofacet order +x,-x,+y,-y,+z,-z, whose lowest vertex is
+// strictly smallest (gi_fast_robins_noalloc.h:124-137 over the cofacet
+// iteration order of :256-270).
+template <int kCandDim, class Keys>
+MSC_HD int pick_candidate(const Keys& kk, int a, uint32_t cset) {
+    if (!(cset & (cset - 1))) return __builtin_ctz(cset);
+    int best = -1;
+    uint64_t bk = ~0ull;
+#pragma unroll
+    for (int i = 0; i < 6; i++) {
+        const int step = (i < 2 ? 1 : (i < 4 ? 3 : 9)) * ((i & 1) ? -1 : 1);
+        const int c = a + step;
+        if (c < 0 || c > 26 || !((cset >> c) & 1u)) continue;
+        const uint64_t kc = kCandDim == 2 ? quad_lowest(kk, c) : hex_lowest(kk, c);
+        if (kc < bk) { bk = kc; best = c; }
+    }
+    return best;
+}
+
+// Relaxed Robins expansion of one subset S of the lower star
+// (gi_fast_robins_noalloc.h:147-321, gi_modified_robins.h:999-1199):
+//   vertex: pair with the edge whose other vertex comes first, else critical;
+//   then for d = 1, 2: repeatedly take the first unprocessed d-cell (star
+//   order) having a cofacet in S with exactly one unprocessed facet, and pair
+//   it with the first lowest such cofacet; when none has one, the first
+//   unprocessed d-cell with the lowest lowest vertex becomes critical;
+//   leftover 3-cells are critical.
+template <class Keys>
+MSC_HD void expand_subset(uint32_t S, const Keys& k, StarResult& r) {
+    uint32_t P = 0;  // processed (paired or critical)
+    if (S & kCenter) {
+        const uint32_t E = S & kDim1;
+        if (!E) {
+            r.set_critical(kCenter);
+            P |= kCenter;
+        } else {
+            int best = __builtin_ctz(E);
+            if (E & (E - 1)) {
+                uint64_t bk = ~0ull;
+                for (uint32_t m = E; m; m &= m - 1) {
+                    const int t = __builtin_ctz(m);
+                    const uint64_t kt = vkey(k, t);
+                    if (kt < bk) { bk = kt; best = t; }
+                }
+            }
+            r.set_pair(13, best);
+            P |= kCenter | (1u << best);
+        }
+    }
+    // edges
+    while (true) {
+        const uint32_t U = S & ~P;
+        const uint32_t Ud = U & kDim1;
+        if (!Ud) break;
+        // quads have two edge facets: exactly one unprocessed <=> xor
+        const uint32_t cand = S & kDim2 & (star_facet_x(U) ^ star_facet_y(U) ^ star_facet_z(U));
+        const uint32_t ready = Ud & star_has_cofacet_in(cand);
+        if (ready) {
+            const int a = __builtin_ctz(ready);
+            const int c = pick_candidate<2>(k, a, cand & cofacet_mask(a));
+            r.set_pair(a, c);
+            P |= (1u << a) | (1u << c);
+        } else {
+            int best = __builtin_ctz(Ud);
+            if (Ud & (Ud - 1)) {
+                uint64_t bk = ~0ull;
+                for (uint32_t m = Ud; m; m &= m - 1) {
+                    const int t = __builtin_ctz(m);
+                    const uint64_t kt = vkey(k, t);
+                    if (kt < bk) { bk = kt; best = t; }
+                }
+            }
+            r.set_critical(1u << best);
+            P |= 1u << best;
+        }
+    }
+    // quads
+    while (true) {
+        const uint32_t U = S & ~P;
+        const uint32_t Ud = U & kDim2;
+        if (!Ud) break;
+        const uint32_t fx = star_facet_x(U), fy = star_facet_y(U), fz = star_facet_z(U);
+        const uint32_t cand = S & kDim3 & (fx ^ fy ^ fz) & ~(fx & fy & fz);
+        const uint32_t ready = Ud & star_has_cofacet_in(cand);
+        if (ready) {
+            const int a = __builtin_ctz(ready);
+            const int c = pick_candidate<3>(k, a, cand & cofacet_mask(a));
+            r.set_pair(a, c);
+            P |= (1u << a) | (1u << c);
+        } else {
+            int best = __builtin_ctz(Ud);
+            if (Ud & (Ud - 1)) {
+                uint64_t bk = ~0ull;
+                for (uint32_t m = Ud; m; m &= m - 1) {
+                    const int t = __builtin_ctz(m);
+                    const uint64_t kt = quad_lowest(k, t);
+                    if (kt < bk) { bk = kt; best = t; }
+                }
+            }
+            r.set_critical(1u << best);
+            P |= 1u << best;
+        }
+    }
+    // 3-cells left over have no cofacets: all critical
+    r.set_critical(S & kDim3 & ~P);
+}
+
+
+// Lower star of vertex (x,y,z) of an X*Y*Z grid as a star mask: the cells
+// whose every vertex other than the centre comes before it, restricted to
+// the domain (gi_fast_robins_noalloc.h:359-366, gi_modified_robins.h:1293-1297).
+template <class Keys>
+MSC_HD uint32_t lower_star_mask(const Keys& k, int x, int y, int z, int X, int Y, int Z) {
+    const uint32_t kc = k.at(13);
+    uint32_t lower = 0;
+#pragma unroll
+    for (int t = 0; t < 27; t++) {
+        if (t == 13) continue;
+        const uint32_t kq = k.at(t);
+        const bool lo = t < 13 ? (kq <= kc) : (kq < kc);
+        lower |= (uint32_t)lo << t;
+    }
+    const uint32_t invalid = (x == 0 ? kXM : 0u) | (x == X - 1 ? kXP : 0u) | (y == 0 ? kYM : 0u) |
+                             (y == Y - 1 ? kYP : 0u) | (z == 0 ? kZM : 0u) | (z == Z - 1 ? kZP : 0u);
+    return ~star_dilate(((~lower) & kAll & ~kCenter) | invalid) & kAll;
+}
+
+// Boundary strata of the star cells of vertex (x,y,z): bit t of b[a] is set
+// when the cell lies on a domain face normal to axis a
+// (gi_topological_regular_grid.h:1291-1303).
+MSC_HD void star_boundary(int x, int y, int z, int X, int Y, int Z, uint32_t& bx, uint32_t& by, uint32_t& bz) {
+    bx = (x == 0 || x == X - 1) ? kX0 : 0u;
+    by = (y == 0 || y == Y - 1) ? kY0 : 0u;
+    bz = (z == 0 || z == Z - 1) ? kZ0 : 0u;
+}
+
+// Steepest pass: subsets keyed by boundaryValue 0..3 (slots of <5,4>).
+template <class Keys>
+MSC_HD void steepest_expand(const Keys& k, uint32_t L, uint32_t bx, uint32_t by, uint32_t bz, StarResult& r) {
+    const uint32_t onb = bx | by | bz;
+    if (!onb) {
+        expand_subset(L, k, r);
+        return;
+    }
+    const uint32_t odd = bx ^ by ^ bz, two = (bx & by) | (bx & bz) | (by & bz);
+    const uint32_t sub[4] = {~onb, odd & ~two, two & ~odd, odd & two};
+    for (int i = 0; i < 4; i++) {
+        const uint32_t S = L & sub[i];
+        if (S) expand_subset(S, k, r);
+    }
+}
+
+// Constrained pass <4,6>: subsets keyed restriction(c)*3 + boundaryValue(c),
+// clamped to 23 (gi_modified_robins.h:1299-1313); label(t) returns the
+// restriction label of star cell t.  Subsets are independent, so they are
+// expanded in any order.
+template <class Keys, class Labels>
+MSC_HD void constrained_expand(const Keys& k, uint32_t L, uint32_t bx, uint32_t by, uint32_t bz,
+                               const Labels& label, StarResult& r) {
+    uint32_t pending = L;
+    while (pending) {
+        int kmin = 1 << 30;
+        uint32_t S = 0;
+        for (uint32_t m = pending; m; m &= m - 1) {
+            const int t = __builtin_ctz(m);
+            const int bvt = (int)((bx >> t) & 1u) + (int)((by >> t) & 1u) + (int)((bz >> t) & 1u);
+            int key = (int)label(t) * 3 + bvt;
+            key = key > 23 ? 23 : key;
+            if (key < kmin) {
+                kmin = key;
+                S = 1u << t;
+            } else if (key == kmin) {
+                S |= 1u << t;
+            }
+        }
+        pending &= ~S;
+        expand_subset(S, k, r);
+    }
+}
+
+// .grad byte of star cell t after the expansion: assigned, flag, pair code,
+// ldir preset (3, or 0 for a critical 3-cell).
+MSC_HD uint8_t star_cell_byte(const StarResult& r, uint32_t onb, int t) {
+    const uint32_t code = ((r.c0 >> t) & 1u) | (((r.c1 >> t) & 1u) << 1) | (((r.c2 >> t) & 1u) << 2);
+    const uint32_t flag = (onb >> t) & 1u;
+    const uint32_t ldir = ((kDim3 >> t) & (r.crit >> t) & 1u) ? 0u : 3u;
+    return (uint8_t)(1u | (flag << 1) | (code << 2) | (ldir << 5));
+}
+
+}  // namespace msceer

@@ -10,6 +10,13 @@
 #pragma once
 #include <stdint.h>
 
+// Functions usable from kernels and from host-side emulation of the kernels.
+#if defined(__HIPCC__)
+#define MSC_HD __host__ __device__ __forceinline__
+#else
+#define MSC_HD inline
+#endif
+
 namespace msceer {
 
 constexpr uint32_t star_mask_where(int axis, int val) {
@@ -43,7 +50,7 @@ static_assert(__builtin_popcount(kDim1) == 6 && __builtin_popcount(kDim2) == 12 
 
 // Cells whose vertex set meets NL (a mask of "bad" neighbour vertices):
 // dilate from zero components to +-1 along each axis.
-__host__ __device__ inline uint32_t star_dilate(uint32_t nl) {
+MSC_HD uint32_t star_dilate(uint32_t nl) {
     nl |= ((nl & kX0) << 1) | ((nl & kX0) >> 1);
     nl |= ((nl & kY0) << 3) | ((nl & kY0) >> 3);
     nl |= ((nl & kZ0) << 9) | ((nl & kZ0) >> 9);
@@ -52,20 +59,20 @@ __host__ __device__ inline uint32_t star_dilate(uint32_t nl) {
 
 // For every cell at bit t: is its facet along x (the cell with the x
 // component zeroed) set in u?  Only cells with a nonzero x component answer.
-__host__ __device__ inline uint32_t star_facet_x(uint32_t u) { return ((u << 1) & kXP) | ((u >> 1) & kXM); }
-__host__ __device__ inline uint32_t star_facet_y(uint32_t u) { return ((u << 3) & kYP) | ((u >> 3) & kYM); }
-__host__ __device__ inline uint32_t star_facet_z(uint32_t u) { return ((u << 9) & kZP) | ((u >> 9) & kZM); }
+MSC_HD uint32_t star_facet_x(uint32_t u) { return ((u << 1) & kXP) | ((u >> 1) & kXM); }
+MSC_HD uint32_t star_facet_y(uint32_t u) { return ((u << 3) & kYP) | ((u >> 3) & kYM); }
+MSC_HD uint32_t star_facet_z(uint32_t u) { return ((u << 9) & kZP) | ((u >> 9) & kZM); }
 
 // For every cell at bit t: does any cofacet of it (one zero component moved
 // to +-1) lie in c?
-__host__ __device__ inline uint32_t star_has_cofacet_in(uint32_t c) {
+MSC_HD uint32_t star_has_cofacet_in(uint32_t c) {
     return (((c >> 1) | (c << 1)) & kX0) | (((c >> 3) | (c << 3)) & kY0) |
            (((c >> 9) | (c << 9)) & kZ0);
 }
 
 // Pair-direction byte of the step from bit a to bit b (b - a in {+-1,+-3,+-9}):
 // +x 0, -x 1, +y 2, -y 3, +z 4, -z 5 (gi_topological_regular_grid.h:1182-1192).
-__host__ __device__ inline uint32_t star_dir_code(int a, int b) {
+MSC_HD uint32_t star_dir_code(int a, int b) {
     int d = b - a;
     int ad = d < 0 ? -d : d;
     uint32_t base = ad == 1 ? 0u : (ad == 3 ? 2u : 4u);

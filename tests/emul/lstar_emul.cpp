@@ -1,0 +1,52 @@
+// Host emulation of the lower-star kernel's per-lane logic (test
+// infrastructure): the same lstar_core.hpp the gfx950 kernel runs, driven
+// over every vertex on the CPU, so kernel-logic changes are checked against
+// the oracle before they reach a GPU.
+#include <cstdint>
+#include <cstring>
+#include <vector>
+
+#include "lstar_core.hpp"
+
+using namespace msceer;
+
+extern "C" int emul_lstar_prepass(const float* f, long long X, long long Y, long long Z,
+                                  const int8_t* restriction, uint8_t* grad) {
+    const int64_t PX = X + 2, PY = Y + 2, PZ = Z + 2;
+    std::vector<uint32_t> keys((size_t)(PX * PY * PZ));
+    for (int64_t z = 0; z < PZ; z++)
+        for (int64_t y = 0; y < PY; y++)
+            for (int64_t x = 0; x < PX; x++) {
+                int64_t gx = x - 1, gy = y - 1, gz = z - 1;
+                gx = gx < 0 ? 0 : (gx >= X ? X - 1 : gx);
+                gy = gy < 0 ? 0 : (gy >= Y ? Y - 1 : gy);
+                gz = gz < 0 ? 0 : (gz >= Z ? Z - 1 : gz);
+                keys[(size_t)(x + PX * (y + PY * z))] = float_key(f[gx + X * (gy + Y * gz)]);
+            }
+    const int64_t Mx = 2 * X - 1, My = 2 * Y - 1;
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long long z = 0; z < Z; z++)
+        for (long long y = 0; y < Y; y++)
+            for (long long x = 0; x < X; x++) {
+                StridedKeys k{keys.data(), (x + 1) + PX * ((y + 1) + PY * (z + 1)), PX, PX * PY};
+                const uint32_t L = lower_star_mask(k, (int)x, (int)y, (int)z, (int)X, (int)Y, (int)Z);
+                uint32_t bx, by, bz;
+                star_boundary((int)x, (int)y, (int)z, (int)X, (int)Y, (int)Z, bx, by, bz);
+                const int64_t vcell = 2 * x + Mx * (2 * y + My * 2 * z);
+                StarResult r;
+                if (!restriction) {
+                    steepest_expand(k, L, bx, by, bz, r);
+                } else {
+                    const int8_t* rv = restriction + vcell;
+                    auto label = [rv, Mx, My](int t) -> int {
+                        return rv[tid_x(t) + Mx * tid_y(t) + Mx * My * tid_z(t)];
+                    };
+                    constrained_expand(k, L, bx, by, bz, label, r);
+                }
+                for (int t = 0; t < 27; t++) {
+                    if (!((L >> t) & 1u)) continue;
+                    grad[vcell + tid_x(t) + Mx * (tid_y(t) + My * tid_z(t))] = star_cell_byte(r, bx | by | bz, t);
+                }
+            }
+    return 0;
+}

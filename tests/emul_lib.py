@@ -1,0 +1,40 @@
+"""ctypes access to tests/emul/libemul.so: the lower-star kernel's per-lane
+logic (msceer_amd/csrc/lstar_core.hpp) built for the host."""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SO = os.path.join(HERE, "emul", "libemul.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        subprocess.run(["make", "-s", "-C", os.path.join(HERE, "emul")], check=True)
+        h = ctypes.CDLL(SO)
+        P, L = ctypes.c_void_p, ctypes.c_longlong
+        h.emul_lstar_prepass.argtypes = [P, L, L, L, P, P]
+        _lib = h
+    return _lib
+
+
+def lstar_prepass(f, restriction=None):
+    f = np.ascontiguousarray(f, np.float32)
+    Z, Y, X = f.shape
+    g = np.zeros((2 * X - 1) * (2 * Y - 1) * (2 * Z - 1), np.uint8)
+    r = None if restriction is None else np.ascontiguousarray(restriction, np.int8).ctypes.data_as(ctypes.c_void_p)
+    lib().emul_lstar_prepass(f.ctypes.data_as(ctypes.c_void_p), X, Y, Z, r, g.ctypes.data_as(ctypes.c_void_p))
+    return g
+
+
+def expected_prepass(pairs_with_flags, shape_zyx):
+    """Oracle pairs+flags with the kernel's ldir preset (3; 0 for critical 3-cells)."""
+    import gradcheck
+    Z, Y, X = shape_zyx
+    dims, _ = gradcheck.cell_dims(X, Y, Z)
+    crit3 = (((pairs_with_flags >> 2) & 7) == 7) & (dims == 3)
+    return pairs_with_flags | np.where(crit3, 0, 3 << 5).astype(np.uint8)

@@ -1,0 +1,34 @@
+"""The lower-star kernel's per-lane logic, run on the CPU (tests/emul), against
+the oracle: the steepest pass on every golden case and seeded fields, and the
+constrained pass with random restriction labels."""
+import numpy as np
+import pytest
+
+import emul_lib
+import oracle_lib
+from golden import lstar_case_names, load_lstar
+from msceer_amd import synth
+
+
+@pytest.mark.parametrize("name", lstar_case_names())
+def test_emulated_kernel_matches_golden_pairs(name):
+    f, ref = load_lstar(name)
+    expect = emul_lib.expected_prepass(ref & 0x1F, f.shape)
+    got = emul_lib.lstar_prepass(f)
+    assert (got == expect).all(), int((got != expect).sum())
+
+
+@pytest.mark.parametrize("seed", [1, 2, 3])
+def test_emulated_constrained_pass_matches_oracle(seed):
+    s = synth.Stream(seed)
+    shape = (9, 14, 23)
+    f = s.normal(int(np.prod(shape))).reshape(shape).astype(np.float32)
+    if seed == 3:
+        f = np.floor(f * 2).astype(np.float32)  # ties
+    Z, Y, X = shape
+    C = (2 * X - 1) * (2 * Y - 1) * (2 * Z - 1)
+    lab = np.floor(s.uniform(C) * 4).astype(np.int8)  # labels 0..3 (key collisions included)
+    ref = oracle_lib.flags(oracle_lib.lstar_pairs(f, lab), shape)
+    got = emul_lib.lstar_prepass(f, lab)
+    expect = emul_lib.expected_prepass(ref, shape)
+    assert (got == expect).all(), int((got != expect).sum())
