@@ -53,3 +53,16 @@ LSTAR_CASES = {
     "tiny_5x2x3": ((3, 2, 5), lambda: _normal((3, 2, 5), 18)),
     "slab_64x64x2": ((2, 64, 64), lambda: _normal((2, 64, 64), 19)),
 }
+
+
+# ondemandaccurate (ascending 3-manifolds) golden cases:
+# name -> (shape_zyx, generator, err_thr, grad_thr, max_iters, presimp)
+ONDEMAND_CASES = {
+    "labels_48": ((48, 48, 48), lambda: synth.labels_field(303, 48), 1e-3, 1e-6, 4096, 0.0),
+    "gauss_44x36x40": ((40, 36, 44), lambda: synth.gaussian_mixture((40, 36, 44), 7, 20, 3.0, 8.0), 1e-3, 1e-6, 4096, 0.0),
+    "smooth_34x33x32": ((32, 33, 34), lambda: synth.smoothed_noise((32, 33, 34), 8, 2.0), 1e-3, 1e-6, 4096, 0.0),
+    "smooth_presimp_40": ((40, 40, 40), lambda: synth.smoothed_noise((40, 40, 40), 9, 2.5), 1e-3, 1e-6, 4096, 0.05),
+    # a tiny step budget forces OVER_MAX_ITERATIONS stops
+    "budget_24x30x20": ((20, 30, 24), lambda: synth.gaussian_mixture((20, 30, 24), 31, 12, 4.0, 9.0, sine=0.05),
+                        1e-3, 1e-6, 60, 0.0),
+}

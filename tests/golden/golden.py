@@ -23,3 +23,38 @@ def load_lstar(name):
     assert synth.sha256(f) == str(d["input_sha256"]), f"{name}: input generator drifted"
     grad = np.frombuffer(zlib.decompress(d["grad_z"].tobytes()), np.uint8)
     return f, grad
+
+
+def ondemand_case_names():
+    from cases import ONDEMAND_CASES
+    return sorted(ONDEMAND_CASES)
+
+
+def load_ondemand(name):
+    """(values, params(err, gthr, iters, presimp), dict of reference stage outputs)."""
+    from cases import ONDEMAND_CASES
+    d = np.load(os.path.join(HERE, "ondemand", name + ".npz"))
+    shape, gen, err, gthr, iters, presimp = ONDEMAND_CASES[name]
+    f = np.ascontiguousarray(gen(), dtype=np.float32)
+    assert tuple(d["shape"]) == tuple(shape) == f.shape
+    assert synth.sha256(f) == str(d["input_sha256"]), f"{name}: input generator drifted"
+    un = lambda k, dt: np.frombuffer(zlib.decompress(d[k].tobytes()), dt)
+    out = {"certain": un("certain_z", np.int32), "integrated": un("integrated_z", np.int32),
+           "cleaned": un("cleaned_z", np.int32), "restriction": un("restriction_z", np.int8),
+           "grad": un("grad_z", np.uint8)}
+    return f, (err, gthr, iters, presimp), out
+
+
+def canonical(lab, f):
+    """Rename each label by its lowest vertex (value, then vertex id)."""
+    flat = f.ravel()
+    order = np.lexsort((np.arange(flat.size), flat))
+    out = np.full_like(lab, -1)
+    seen = {}
+    for v in order:
+        l = int(lab[v])
+        if l != -1 and l not in seen:
+            seen[l] = int(v)
+    for l, v in seen.items():
+        out[lab == l] = v
+    return out

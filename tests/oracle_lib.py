@@ -79,3 +79,92 @@ def steepest_grad(f):
 
 def set_threads(n):
     return lib().oracle_set_threads(n)
+
+
+# ---- accurate labelling (ondemandaccurate, ascending) ----------------------
+_F = ctypes.c_float
+_I = ctypes.c_int
+
+
+def _integ_sigs():
+    h = lib()
+    if getattr(h, "_integ_ready", False):
+        return h
+    h.oracle_gradient_field.argtypes = [_P, _L, _L, _L, _P]
+    h.oracle_certain_regions.argtypes = [_P, _L, _L, _L, _P, _F, _P, _P]
+    h.oracle_certain_regions.restype = ctypes.c_int
+    h.oracle_integrate.argtypes = [_P, _L, _L, _L, _P, _F, _F, _I, _I, _P, _P]
+    h.oracle_clean.argtypes = [_P, _L, _L, _L, _P]
+    h.oracle_min_boundary.argtypes = [_P, _L, _L, _L, _P]
+    h.oracle_conforming_vertices.argtypes = [_P, _L, _L, _L, _P, _P]
+    h.oracle_conforming_vertices.restype = ctypes.c_longlong
+    h.oracle_ondemand_asc.argtypes = [_P, _L, _L, _L, _F, _F, _I, _F, _I, _P, _P, _P, _P]
+    h._integ_ready = True
+    return h
+
+
+def gradient_field(f):
+    f = np.ascontiguousarray(f, np.float32)
+    Z, Y, X = f.shape
+    g = np.empty((Z, Y, X, 3), np.float64)
+    _integ_sigs().oracle_gradient_field(_p(f), X, Y, Z, _p(g))
+    return g
+
+
+def certain_regions(f, base_grad, presimp=0.0):
+    f = np.ascontiguousarray(f, np.float32)
+    Z, Y, X = f.shape
+    lab = np.empty(X * Y * Z, np.int32)
+    seeds = np.empty(X * Y * Z, np.int8)
+    n = _integ_sigs().oracle_certain_regions(_p(f), X, Y, Z, _p(np.ascontiguousarray(base_grad, np.uint8)),
+                                             presimp, _p(lab), _p(seeds))
+    return lab, seeds, n
+
+
+def integrate(f, certain, seeds, err=1e-3, gthr=1e-6, iters=4096, mode=0):
+    """mode 0: the reference's mixed precision; mode 1: all FP64."""
+    f = np.ascontiguousarray(f, np.float32)
+    Z, Y, X = f.shape
+    lab = np.ascontiguousarray(certain, np.int32).copy()
+    st = np.zeros(8, np.int64)
+    _integ_sigs().oracle_integrate(_p(f), X, Y, Z, _p(np.ascontiguousarray(seeds, np.int8)), err, gthr, iters,
+                                   mode, _p(lab), _p(st))
+    return lab, st
+
+
+def clean(f, labels):
+    f = np.ascontiguousarray(f, np.float32)
+    Z, Y, X = f.shape
+    lab = np.ascontiguousarray(labels, np.int32).copy()
+    _integ_sigs().oracle_clean(_p(f), X, Y, Z, _p(lab))
+    return lab
+
+
+def min_boundary(labels, shape_zyx):
+    Z, Y, X = shape_zyx
+    r = np.empty(ncells(X, Y, Z), np.int8)
+    _integ_sigs().oracle_min_boundary(_p(np.ascontiguousarray(labels, np.int32)), X, Y, Z, _p(r))
+    return r
+
+
+def conforming_vertices(f, restriction):
+    f = np.ascontiguousarray(f, np.float32)
+    Z, Y, X = f.shape
+    r = np.ascontiguousarray(restriction, np.int8)
+    h = _integ_sigs()
+    n = h.oracle_conforming_vertices(_p(f), X, Y, Z, _p(r), None)
+    out = np.empty(n, np.int64)
+    h.oracle_conforming_vertices(_p(f), X, Y, Z, _p(r), _p(out))
+    return out
+
+
+def ondemand_asc(f, err=1e-3, gthr=1e-6, iters=4096, presimp=0.0, mode=0):
+    """(grad, cleaned labels, restriction, stats[traced, over_budget, refixed, conforming])."""
+    f = np.ascontiguousarray(f, np.float32)
+    Z, Y, X = f.shape
+    g = np.empty(ncells(X, Y, Z), np.uint8)
+    lab = np.empty(X * Y * Z, np.int32)
+    r = np.empty(ncells(X, Y, Z), np.int8)
+    st = np.zeros(8, np.int64)
+    _integ_sigs().oracle_ondemand_asc(_p(f), X, Y, Z, err, gthr, iters, presimp, mode, _p(g), _p(lab), _p(r), _p(st))
+    return g, lab, r, st

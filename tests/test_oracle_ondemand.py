@@ -1,0 +1,60 @@
+"""The CPU oracle of accurate ascending labelling, pinned before it is trusted:
+every stage against the reference's own outputs (golden vectors written by
+oracle/refdrv/ref_ondemand.cpp with one thread), plus closed forms."""
+import numpy as np
+import pytest
+
+import gradcheck
+import oracle_lib
+from golden import canonical, load_ondemand, ondemand_case_names
+
+
+@pytest.mark.parametrize("name", ondemand_case_names())
+def test_oracle_stages_match_reference(name):
+    f, (err, gthr, iters, presimp), ref = load_ondemand(name)
+    Z, Y, X = f.shape
+    base = oracle_lib.lstar_pairs(f)
+    cert, seeds, _ = oracle_lib.certain_regions(f, base, presimp)
+    assert (cert == ref["certain"]).all(), int((cert != ref["certain"]).sum())
+    integ, _ = oracle_lib.integrate(f, cert, seeds, err, gthr, iters, mode=0)
+    assert (canonical(integ, f) == ref["integrated"]).all(), int((canonical(integ, f) != ref["integrated"]).sum())
+    cleaned = oracle_lib.clean(f, integ)
+    assert (cleaned == ref["cleaned"]).all()
+    r = oracle_lib.min_boundary(cleaned, f.shape)
+    assert (r == ref["restriction"]).all()
+    g, lab, r2, st = oracle_lib.ondemand_asc(f, err, gthr, iters, presimp, mode=0)
+    assert (g == ref["grad"]).all(), int((g != ref["grad"]).sum())
+    assert (lab == ref["cleaned"]).all() and (r2 == ref["restriction"]).all()
+    gradcheck.check_consistency(g, X, Y, Z)
+
+
+def test_gradient_field_closed_form():
+    f, _, _ = load_ondemand("smooth_34x33x32")
+    g = oracle_lib.gradient_field(f)
+    f64 = f.astype(np.float64)
+    expect = np.zeros(f.shape + (3,))
+    expect[:, :, 1:-1, 0] = 0.5 * (f64[:, :, 2:] - f64[:, :, :-2])
+    expect[:, 1:-1, :, 1] = 0.5 * (f64[:, 2:, :] - f64[:, :-2, :])
+    expect[1:-1, :, :, 2] = 0.5 * (f64[2:] - f64[:-2])
+    assert np.array_equal(g, expect)
+
+
+def test_fp64_mode_close_to_reference_arithmetic():
+    f, (err, gthr, iters, presimp), ref = load_ondemand("labels_48")
+    _, lab64, _, _ = oracle_lib.ondemand_asc(f, err, gthr, iters, presimp, mode=1)
+    mism = float((lab64 != ref["cleaned"]).mean())
+    assert mism < 0.01, mism
+
+
+def test_budget_exhaustion_is_forced():
+    f, (err, gthr, iters, presimp), ref = load_ondemand("budget_24x30x20")
+    base = oracle_lib.lstar_pairs(f)
+    cert, seeds, _ = oracle_lib.certain_regions(f, base, presimp)
+    _, st = oracle_lib.integrate(f, cert, seeds, err, gthr, iters, mode=0)
+    assert st[1] > 0  # some streamlines stopped on OVER_MAX_ITERATIONS
+
+
+def test_conforming_vertices_cover_restricted_lower_stars():
+    f, _, ref = load_ondemand("gauss_44x36x40")
+    vs = oracle_lib.conforming_vertices(f, ref["restriction"])
+    assert vs.size > 0 and (vs % 2 == 0).all()
