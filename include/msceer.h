@@ -39,6 +39,8 @@ extern "C" {
 #define MSCEER_DEVICE_PTRS 0x1u /* buffers are device pointers (else host memory) */
 #define MSCEER_SKIP_LDIR 0x2u   /* leave ldir bits at their pre-pass values */
 #define MSCEER_TIMING 0x4u      /* record per-kernel HIP event timings */
+#define MSCEER_STOP_REGIONS 0x8u     /* msceer_ondemand_asc: return the certain-region labels */
+#define MSCEER_STOP_INTEGRATED 0x10u /* msceer_ondemand_asc: return labels before the cleanup */
 
 typedef struct msceer_ctx msceer_ctx;
 
@@ -51,6 +53,26 @@ typedef struct msceer_timing {
     int32_t ldir_rounds;   /* frontier rounds used by the ldir passes */
     int64_t critical[4];   /* critical cells by dimension */
 } msceer_timing;
+
+/* Statistics of one accurate-labelling call (msceer_ondemand_asc). */
+typedef struct msceer_label_stats {
+    int64_t minima;             /* critical vertices of the base gradient */
+    int64_t saddles;            /* critical 1-cells of the base gradient */
+    int64_t regions;            /* regions after persistence simplification */
+    int64_t traced_fp32;        /* pass-1 streamlines (FP32) */
+    int64_t over_budget_fp32;   /* of those, stopped by the step budget */
+    int64_t reintegrated_fp64;  /* pass-3 re-integrations (FP64) */
+    int64_t over_budget_fp64;   /* of those, stopped by the step budget */
+    int64_t integrated_vertices;/* vertices the learned mask sent to integration (all if none) */
+    int32_t certain_sweeps;     /* sweeps of the certain-region fixed point */
+    int32_t pass3_rounds;       /* rounds of the FP64 re-integration */
+    float ms_lstar;             /* base + constrained lower-star kernels */
+    float ms_regions;           /* extremum graph + certain regions */
+    float ms_integrate;         /* passes 1-3 */
+    float ms_cleanup;           /* isolated-region cleanup + restriction labels */
+    float ms_grad;              /* constrained pass + ldir */
+    float ms_total;
+} msceer_label_stats;
 
 int msceer_version(void);
 const char* msceer_strerror(int code);
@@ -76,6 +98,35 @@ int msceer_last_timing(const msceer_ctx* ctx, msceer_timing* out);
  */
 int msceer_lstar_grad(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz,
                       uint8_t* grad_out, uint32_t flags);
+
+/*
+ * Accurate ascending 3-manifold labelling and its conforming .grad: the
+ * needASC2 branch of ondemandaccurate (ondemandaccurate/ondemandaccurate.cxx:728-1061):
+ *   base gradient (as msceer_lstar_grad, without ldir),
+ *   SimplifiedExtremumGraph::ComputeMinMapFromGradient (gi_extrema_region_builder.h:406-551),
+ *   GridSimplifiedExtremalRegionBuilder::BeginIntegration (gi_extrema_region_builder.h:982-1042),
+ *   NumericIntegratorPathCompressingToTerminal::BeginIntegration
+ *     (gi_numeric_integrator_path_compressing.h:75-433) with
+ *     AdaptiveEulerAdvector3D<F,-1> (gi_adaptive_euler_advector.h:51-305),
+ *   IsolatedCCRegionRemoverNEW::ComputeOutput (gi_isolated_region_remover.h:155-192),
+ *   VertexLabelingToBoundaryLabeling::ComputeMINBoundary (gi_labeling_to_bounary_labeling.h:48-116),
+ *   the constrained MyRobinsNoalloc<...,4,6> pass (gi_modified_robins.h:1281-1343)
+ *   and RecordGrad (ondemandaccurate.cxx:693-699).
+ * err_thr bounds the squared step-doubling error, grad_thr the gradient
+ * magnitude, max_iters the steps (accepted and rejected) per streamline;
+ * presimp is the persistence threshold of the minimum graph.  Pass 1 traces
+ * in FP32; pass 3 re-traces in FP64 every vertex next to a label change and
+ * every pass-1 streamline that exhausted its budget.
+ * labels_out (X*Y*Z int32, may be NULL): region of every vertex, named by the
+ * vertex id of the region's lowest minimum.  grad_out (cells bytes, may be
+ * NULL): the conforming .grad.  integrate_mask (X*Y*Z bytes, may be NULL):
+ * when given, only vertices with a nonzero mask are integrated numerically
+ * and every other vertex takes the label of the minimum its discrete-gradient
+ * V-path descends to.
+ */
+int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz, float err_thr,
+                        float grad_thr, int32_t max_iters, float presimp, const uint8_t* integrate_mask,
+                        uint8_t* grad_out, int32_t* labels_out, msceer_label_stats* stats, uint32_t flags);
 
 #ifdef __cplusplus
 }

@@ -6,74 +6,16 @@
 #include <cstring>
 #include <new>
 
-#include "msceer.h"
-#include "internal.hpp"
-#include "mlist.hpp"
+#include "ctx.hpp"
 
 using namespace msceer;
 
-namespace {
-
-// Grow-only device buffer.
-struct DevBuf {
-    void* ptr = nullptr;
-    size_t bytes = 0;
-    hipError_t ensure(size_t need) {
-        if (need <= bytes) return hipSuccess;
-        if (ptr) { (void)hipFree(ptr); ptr = nullptr; bytes = 0; }
-        need = (need + 255) & ~(size_t)255;
-        hipError_t e = hipMalloc(&ptr, need);
-        if (e == hipSuccess) bytes = need;
-        return e;
-    }
-    void release() {
-        if (ptr) (void)hipFree(ptr);
-        ptr = nullptr;
-        bytes = 0;
-    }
-};
-
-constexpr uint32_t kMinListCap = 1u << 16;
-constexpr uint32_t kMaxListCap = 1u << 26;   // entries per critical list
-constexpr uint32_t kMaxFrontCap = 1u << 27;  // entries per frontier buffer
-
-// Control block (uint32 words, device; mirrored to pinned host memory):
-//   sub-list counts of the two critical lists and the three rotating
-//   frontier count arrays, critical statistics, flags.
-enum Ctl : int {
-    kCtlList1 = 0,
-    kCtlList2 = kSubs,
-    kCtlRound = 2 * kSubs,       // three arrays of kSubs
-    kCtlStats = 5 * kSubs,       // kSubs * 4
-    kCtlOverflow = 9 * kSubs,    // a list or frontier ran out of room
-    kCtlTail = 9 * kSubs + 1,    // shadow of the grad array's last partial word
-    kCtlWords = 9 * kSubs + 16
-};
-
-}  // namespace
-
-struct msceer_ctx {
-    int device = 0;
-    hipStream_t own = nullptr;
-    hipStream_t stream = nullptr;
-    DevBuf values, grad, lists, front0, front1, ctl;
-    uint32_t* h_ctl = nullptr;  // pinned mirror of the control block
-    hipEvent_t ev[6] = {};
-    msceer_timing last{};
-};
-
-namespace {
+namespace msceer {
 
 int hip_fail(hipError_t e) {
     if (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) return MSCEER_ENOMEM;
     return MSCEER_EHIP;
 }
-
-#define CHECK_HIP(expr)                       \
-    do {                                      \
-        hipError_t _e = (expr);               \
-        if (_e != hipSuccess) return hip_fail(_e); \
-    } while (0)
 
 int check_dims(int64_t nx, int64_t ny, int64_t nz) {
     if (nx < 2 || ny < 2 || nz < 2) return MSCEER_EINVAL;
@@ -97,12 +39,6 @@ uint64_t list_total(const uint32_t* counts, uint32_t sub_cap) {
     uint64_t t = 0;
     for (int i = 0; i < kSubs; i++) t += std::min(counts[i], sub_cap);
     return t;
-}
-
-bool list_overflowed(const uint32_t* counts, uint32_t sub_cap) {
-    for (int i = 0; i < kSubs; i++)
-        if (counts[i] > sub_cap) return true;
-    return false;
 }
 
 // One ldir pass.  Rounds are enqueued in groups without host synchronisation
@@ -159,7 +95,30 @@ int ldir_pass(msceer_ctx* ctx, const Dims& d, uint8_t* grad, uint32_t item_dim, 
     return MSCEER_EHIP;
 }
 
-}  // namespace
+int ensure_lists(msceer_ctx* ctx, const Dims& d, uint32_t* list_sub, uint32_t* front_sub) {
+    const uint64_t list_cap = std::min<uint64_t>(kMaxListCap, std::max<uint64_t>(kMinListCap, d.verts / 4));
+    const uint64_t front_cap = std::min<uint64_t>(kMaxFrontCap, std::max<uint64_t>(kMinListCap, d.verts));
+    *list_sub = (uint32_t)((list_cap + kSubs - 1) / kSubs);
+    *front_sub = (uint32_t)((front_cap + kSubs - 1) / kSubs);
+    CHECK_HIP(ctx->lists.ensure(sizeof(uint32_t) * 2 * (size_t)*list_sub * kSubs));
+    CHECK_HIP(ctx->front0.ensure(sizeof(uint32_t) * (size_t)*front_sub * kSubs));
+    CHECK_HIP(ctx->front1.ensure(sizeof(uint32_t) * (size_t)*front_sub * kSubs));
+    return MSCEER_OK;
+}
+
+int run_ldir(msceer_ctx* ctx, const Dims& d, uint8_t* grad, const MList& list1, const MList& list2, bool lists_ok,
+             uint32_t front_sub, int* rounds) {
+    uint32_t* tail = ctl_dev(ctx) + kCtlTail;
+    CHECK_HIP(launch_ldir_tail(d, grad, tail, false, ctx->stream));
+    int rc = ldir_pass(ctx, d, grad, 1, 2, list1, lists_ok, front_sub, rounds);
+    if (rc) return rc;
+    rc = ldir_pass(ctx, d, grad, 2, 1, list2, lists_ok, front_sub, rounds);
+    if (rc) return rc;
+    CHECK_HIP(launch_ldir_tail(d, grad, tail, true, ctx->stream));
+    return MSCEER_OK;
+}
+
+}  // namespace msceer
 
 extern "C" {
 
@@ -189,7 +148,7 @@ int msceer_create(msceer_ctx** out, int device) {
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->h_ctl, sizeof(uint32_t) * kCtlWords, hipHostMallocDefault);
-    for (int i = 0; i < 6 && e == hipSuccess; i++) e = hipEventCreate(&ctx->ev[i]);
+    for (int i = 0; i < 12 && e == hipSuccess; i++) e = hipEventCreate(&ctx->ev[i]);
     if (e == hipSuccess) e = ctx->ctl.ensure(sizeof(uint32_t) * kCtlWords);
     if (e != hipSuccess) {
         msceer_destroy(ctx);
@@ -210,8 +169,11 @@ int msceer_destroy(msceer_ctx* ctx) {
     ctx->front0.release();
     ctx->front1.release();
     ctx->ctl.release();
+    DevBuf* label_bufs[] = {&ctx->gradf, &ctx->label, &ctx->label2, &ctx->dest, &ctx->seedmask, &ctx->redo,
+                            &ctx->stamp, &ctx->rep, &ctx->rep2, &ctx->restriction, &ctx->host_io, &ctx->mask};
+    for (DevBuf* b : label_bufs) b->release();
     if (ctx->h_ctl) (void)hipHostFree(ctx->h_ctl);
-    for (int i = 0; i < 6; i++)
+    for (int i = 0; i < 12; i++)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;
@@ -253,13 +215,9 @@ int msceer_lstar_grad(msceer_ctx* ctx, const float* values, int64_t nx, int64_t 
     }
     // critical-cell lists and frontiers: sized to the volume, capped; any
     // overflow falls back to a sweep
-    const uint64_t list_cap = std::min<uint64_t>(kMaxListCap, std::max<uint64_t>(kMinListCap, d.verts / 4));
-    const uint64_t front_cap = std::min<uint64_t>(kMaxFrontCap, std::max<uint64_t>(kMinListCap, d.verts));
-    const uint32_t list_sub = (uint32_t)((list_cap + kSubs - 1) / kSubs);
-    const uint32_t front_sub = (uint32_t)((front_cap + kSubs - 1) / kSubs);
-    CHECK_HIP(ctx->lists.ensure(sizeof(uint32_t) * 2 * (size_t)list_sub * kSubs));
-    CHECK_HIP(ctx->front0.ensure(sizeof(uint32_t) * (size_t)front_sub * kSubs));
-    CHECK_HIP(ctx->front1.ensure(sizeof(uint32_t) * (size_t)front_sub * kSubs));
+    uint32_t list_sub = 0, front_sub = 0;
+    rc = ensure_lists(ctx, d, &list_sub, &front_sub);
+    if (rc) return rc;
     uint32_t* ctl = ctl_dev(ctx);
     uint32_t* l1 = static_cast<uint32_t*>(ctx->lists.ptr);
     const MList list1{l1, ctl + kCtlList1, list_sub};
@@ -281,13 +239,8 @@ int msceer_lstar_grad(msceer_ctx* ctx, const float* values, int64_t nx, int64_t 
     int rounds = 0;
     if (timing) CHECK_HIP(hipEventRecord(ctx->ev[3], s));
     if (!(flags & MSCEER_SKIP_LDIR)) {
-        uint32_t* tail = ctl + kCtlTail;
-        CHECK_HIP(launch_ldir_tail(d, d_grad, tail, false, s));
-        rc = ldir_pass(ctx, d, d_grad, 1, 2, list1, lists_ok, front_sub, &rounds);
+        rc = run_ldir(ctx, d, d_grad, list1, list2, lists_ok, front_sub, &rounds);
         if (rc) return rc;
-        rc = ldir_pass(ctx, d, d_grad, 2, 1, list2, lists_ok, front_sub, &rounds);
-        if (rc) return rc;
-        CHECK_HIP(launch_ldir_tail(d, d_grad, tail, true, s));
     }
     if (timing) CHECK_HIP(hipEventRecord(ctx->ev[4], s));
     if (!dev) CHECK_HIP(hipMemcpyAsync(grad_out, d_grad, d.cells, hipMemcpyDeviceToHost, s));

@@ -1,0 +1,118 @@
+// Host side of the extremum graph: persistence simplification of the minima
+// joined by 1-saddles (UFMergeGraph MINIMAL + SimplifyToThreshold,
+// gi_extrema_region_builder.h:28-268).  Inputs are small (one node per
+// critical vertex, one arc per 1-saddle) and the cancellation order is a
+// global priority order, so this runs on the host between GPU stages.
+#include <cmath>
+#include <queue>
+#include <unordered_map>
+#include <vector>
+
+#include "extgraph.hpp"
+
+namespace msceer {
+
+namespace {
+
+struct Arc {
+    int64_t saddle;
+    int e1, e2;
+    float pers, sval;
+};
+
+// lowest persistence first, ties by the lower saddle cell id
+struct ArcLater {
+    bool operator()(const Arc& l, const Arc& r) const {
+        if (l.pers < r.pers) return false;
+        if (l.pers > r.pers) return true;
+        return l.saddle > r.saddle;
+    }
+};
+
+struct Graph {
+    const std::vector<MinNode>* nodes;
+    std::vector<int> rep;
+    std::priority_queue<Arc, std::vector<Arc>, ArcLater> arcs;
+    float thr = 0.f;
+
+    // cellValue then cell id (for vertex cells the highest vertex is the cell)
+    bool less(int a, int b) const {
+        const MinNode& x = (*nodes)[a];
+        const MinNode& y = (*nodes)[b];
+        if (x.value < y.value) return true;
+        if (y.value < x.value) return false;
+        return x.cell < y.cell;
+    }
+    int find(int l) {
+        int r = l;
+        while (rep[r] != r) r = rep[r];
+        while (rep[l] != r) { const int n = rep[l]; rep[l] = r; l = n; }
+        return r;
+    }
+    // the later extremum joins the earlier one's set
+    void merge(int a, int b) {
+        if (!less(a, b)) rep[a] = rep[b];
+        else rep[b] = rep[a];
+    }
+    void cancel(Arc m) {
+        if (m.pers > thr) return;
+        const int r1 = find(m.e1), r2 = find(m.e2);
+        if (r1 == r2) return;
+        // current representatives: the reference reads them crossed
+        const float v1 = (*nodes)[r2].value, v2 = (*nodes)[r1].value;
+        if (m.e1 != r1 || m.e2 != r2) {
+            const float d1 = std::fabs(m.sval - v1), d2 = std::fabs(m.sval - v2);
+            if (d1 < d2) {
+                if (d1 > thr) return;
+                m.pers = d1; m.e1 = r2; m.e2 = r1;
+            } else {
+                if (d2 > thr) return;
+                m.pers = d2; m.e1 = r1; m.e2 = r2;
+            }
+            arcs.push(m);
+            return;
+        }
+        merge(r1, r2);
+    }
+};
+
+}  // namespace
+
+std::vector<int> simplify_minima(const std::vector<MinNode>& nodes, const std::vector<SaddleArc>& saddles,
+                                 float threshold) {
+    Graph g;
+    g.nodes = &nodes;
+    g.rep.resize(nodes.size());
+    std::unordered_map<int64_t, int> id;
+    id.reserve(nodes.size() * 2);
+    for (size_t i = 0; i < nodes.size(); i++) {
+        g.rep[i] = (int)i;
+        id[nodes[i].cell] = (int)i;
+    }
+    for (const SaddleArc& s : saddles) {
+        if (s.min1 == s.min2 || s.min1 < 0 || s.min2 < 0) continue;
+        auto i1 = id.find(s.min1), i2 = id.find(s.min2);
+        if (i1 == id.end() || i2 == id.end()) continue;
+        Arc a;
+        a.saddle = s.saddle;
+        a.e1 = i1->second;
+        a.e2 = i2->second;
+        a.sval = s.value;
+        float d1 = nodes[a.e1].value - a.sval, d2 = nodes[a.e2].value - a.sval;
+        if (d1 < 0) d1 *= -1;
+        if (d2 < 0) d2 *= -1;
+        a.pers = d1 < d2 ? d1 : d2;
+        g.arcs.push(a);
+    }
+    g.thr = threshold;
+    while (!g.arcs.empty()) {
+        const Arc a = g.arcs.top();
+        g.arcs.pop();
+        g.cancel(a);
+    }
+    std::vector<int> out(nodes.size());
+    for (size_t i = 0; i < nodes.size(); i++) out[i] = g.find((int)i);
+    return out;
+}
+
+}  // namespace msceer

@@ -1,0 +1,776 @@
+// Accurate ascending-manifold labelling on gfx950: gradient field, certain
+// regions, the adaptive-Euler streamline integrator, region cleanup and the
+// restriction labels.
+//
+// Semantics restated (reference file:line):
+//   gradient:   gi_regular_grid_trilinear_function.h:285-341 (central
+//               differences, zero on the boundary of an axis)
+//   advection:  gi_adaptive_euler_advector.h:51-305 (one voxel per call, step
+//               doubling, step cap 0.5 voxel, x1.8 growth, halving on reject)
+//   checkers:   gi_advection_checker.h:212-262
+//   integrator: gi_numeric_integrator_path_compressing.h:75-433 (pass 1 with
+//               path compression, pass 2 boundary collection, pass 3
+//               re-integration to the original certain regions)
+//   regions:    gi_extrema_region_builder.h:806-1042
+//   cleanup:    gi_isolated_region_remover.h:155-192
+//   boundary:   gi_labeling_to_bounary_labeling.h:48-116
+//
+// Precision: pass 1 traces in FP32 with an FP32 gradient field; pass 3
+// (vertices next to a basin boundary, and streamlines that ran out of step
+// budget in pass 1) re-traces in FP64 with the gradient recomputed in FP64
+// from the values, which is how the reference computes it.
+#include <hip/hip_runtime.h>
+
+#include "integrate.hpp"
+#include "mlist.hpp"
+
+namespace msceer {
+
+namespace {
+
+constexpr int kThreads = 256;
+
+__device__ __forceinline__ bool before(const float* f, int64_t a, int64_t b) {
+    const float fa = f[a], fb = f[b];
+    if (fb > fa) return true;
+    if (fa > fb) return false;
+    return b > a;
+}
+
+// ---- gradient field ---------------------------------------------------------
+
+__global__ void __launch_bounds__(kThreads)
+grad_field_kernel(const float* __restrict__ f, VGrid g, float4* __restrict__ out) {
+    const int64_t n = g.verts();
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        const int x = (int)(i % g.X), y = (int)((i / g.X) % g.Y), z = (int)(i / g.sxy);
+        float4 r;
+        r.x = (x > 0 && x < g.X - 1) ? (float)(0.5 * ((double)f[i + 1] - (double)f[i - 1])) : 0.f;
+        r.y = (y > 0 && y < g.Y - 1) ? (float)(0.5 * ((double)f[i + g.X] - (double)f[i - g.X])) : 0.f;
+        r.z = (z > 0 && z < g.Z - 1) ? (float)(0.5 * ((double)f[i + g.sxy] - (double)f[i - g.sxy])) : 0.f;
+        r.w = 0.f;
+        out[i] = r;
+    }
+}
+
+__global__ void seed_scatter_kernel(const int32_t* __restrict__ seed_vert, const int32_t* __restrict__ seed_label,
+                                    int n, int32_t* __restrict__ label, float4* __restrict__ gradf,
+                                    uint8_t* __restrict__ seedmask) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int32_t v = seed_vert[i];
+    label[v] = seed_label[i];
+    if (gradf) gradf[v] = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (seedmask) seedmask[v] = 1;
+}
+
+__global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void fill_u8_kernel(uint8_t* p, int64_t n, uint8_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+// ---- certain regions --------------------------------------------------------
+
+// One sweep of the region-growing fixed point: an unlabelled vertex takes
+// the label shared by all of its earlier All6 neighbours, if they all carry
+// one.  A label, once set, is final (it depends on earlier vertices only),
+// so sweeps may read each other's writes.
+__global__ void __launch_bounds__(kThreads)
+certain_sweep_kernel(const float* __restrict__ f, VGrid g, int32_t* __restrict__ label, uint32_t* __restrict__ changed) {
+    const int64_t n = g.verts();
+    bool any = false;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        if (label[i] != -1) continue;
+        const int x = (int)(i % g.X), y = (int)((i / g.X) % g.Y), z = (int)(i / g.sxy);
+        int64_t nb[6];
+        int k = 0;
+        if (x < g.X - 1) nb[k++] = i + 1;
+        if (x > 0) nb[k++] = i - 1;
+        if (y < g.Y - 1) nb[k++] = i + g.X;
+        if (y > 0) nb[k++] = i - g.X;
+        if (z < g.Z - 1) nb[k++] = i + g.sxy;
+        if (z > 0) nb[k++] = i - g.sxy;
+        int32_t val = -1;
+        bool has = false, ok = true;
+        for (int j = 0; j < k; j++) {
+            if (!before(f, nb[j], i)) continue;
+            const int32_t l = __hip_atomic_load(&label[nb[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (l == -1) { ok = false; break; }
+            if (!has) { val = l; has = true; }
+            else if (val != l) { ok = false; break; }
+        }
+        if (ok && has) {
+            __hip_atomic_store(&label[i], val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            any = true;
+        }
+    }
+    if (any) atomicOr(changed, 1u);
+}
+
+// ---- streamline integration -------------------------------------------------
+
+enum Dest : uint8_t { kBackground = 0, kUnassigned = 1, kAssigned = 2, kCertainTerminal = 3, kCertainNonterminal = 4 };
+enum Event : int { kNone = 0, kOutOfVoxel = 1, kOverMax = 2, kLowGradient = 3, kHitPreassigned = 5 };
+
+template <typename T>
+struct V3 { T x, y, z; };
+
+// Gradient at a vertex: the stored FP32 field, or (FP64) the central
+// difference recomputed from the values in double, as the reference does.
+template <typename T>
+struct GradSrc;
+template <>
+struct GradSrc<float> {
+    const float4* gradf;
+    const float* f;
+    const uint8_t* seedmask;
+    VGrid g;
+    __device__ __forceinline__ V3<float> at(int64_t v) const {
+        const float4 r = gradf[v];
+        return {r.x, r.y, r.z};
+    }
+};
+template <>
+struct GradSrc<double> {
+    const float4* gradf;
+    const float* f;
+    const uint8_t* seedmask;
+    VGrid g;
+    __device__ __forceinline__ V3<double> at(int64_t v) const {
+        if (seedmask[v]) return {0.0, 0.0, 0.0};
+        const int x = (int)(v % g.X), y = (int)((v / g.X) % g.Y), z = (int)(v / g.sxy);
+        V3<double> r;
+        r.x = (x > 0 && x < g.X - 1) ? (0.0 + (double)(-0.5f * f[v - 1])) + (double)(0.5f * f[v + 1]) : 0.0;
+        r.y = (y > 0 && y < g.Y - 1) ? (0.0 + (double)(-0.5f * f[v - g.X])) + (double)(0.5f * f[v + g.X]) : 0.0;
+        r.z = (z > 0 && z < g.Z - 1) ? (0.0 + (double)(-0.5f * f[v - g.sxy])) + (double)(0.5f * f[v + g.sxy]) : 0.0;
+        return r;
+    }
+};
+
+template <typename T>
+__device__ __forceinline__ V3<T> lerp3(const V3<T>& a, const V3<T>& b, T t) {
+    const T u = (T)1 - t;
+    return {a.x * u + b.x * t, a.y * u + b.y * t, a.z * u + b.z * t};
+}
+
+template <typename T>
+__device__ __forceinline__ V3<T> trilin(const V3<T> c[8], T fx, T fy, T fz) {
+    const V3<T> x0 = lerp3(c[0], c[1], fx), x1 = lerp3(c[2], c[3], fx);
+    const V3<T> x2 = lerp3(c[4], c[5], fx), x3 = lerp3(c[6], c[7], fx);
+    const V3<T> y0 = lerp3(x0, x1, fy), y1 = lerp3(x2, x3, fy);
+    return lerp3(y0, y1, fz);
+}
+
+template <typename T>
+__device__ __forceinline__ T clampc(T p, int n) {
+    if (p < (T)0) p = (T)0;
+    if (p >= (T)n) p = (T)(n - 1);
+    return p;
+}
+
+// Nearest vertex of a position (truncation of p + 0.5, clamped near the
+// boundary as Inbounds does).
+__device__ __forceinline__ int clampi(int v, int n) { return v < 0 ? 0 : (v >= n ? n - 1 : v); }
+
+// (Away from the boundary the position lies inside the grid and the final
+// index clamp changes nothing; it only keeps a corrupt position in bounds.)
+template <typename T>
+__device__ __forceinline__ int64_t nearest(const VGrid& g, const V3<T>& p, bool near) {
+    T qx = p.x + (T)0.5, qy = p.y + (T)0.5, qz = p.z + (T)0.5;
+    if (near) { qx = clampc(qx, g.X); qy = clampc(qy, g.Y); qz = clampc(qz, g.Z); }
+    return (int64_t)clampi((int)qx, g.X) +
+           (int64_t)g.X * ((int64_t)clampi((int)qy, g.Y) + (int64_t)g.Y * (int64_t)clampi((int)qz, g.Z));
+}
+
+// One voxel of adaptive Euler with step doubling in precision T.
+template <typename T>
+__device__ int advect_voxel(const GradSrc<T>& src, const VGrid& g, const uint8_t* __restrict__ dest,
+                            bool stop_on_assigned, T gthr, T ethr, bool near, V3<T>& cur, int& iters) {
+    const T kMaxStep = (T)0.5;
+    V3<T> b0 = cur;
+    if (near) { b0.x = clampc(b0.x, g.X); b0.y = clampc(b0.y, g.Y); b0.z = clampc(b0.z, g.Z); }
+    const int bx = (int)b0.x, by = (int)b0.y, bz = (int)b0.z;
+    int xs[2] = {bx, bx + 1}, ys[2] = {by, by + 1}, zs[2] = {bz, bz + 1};
+    if (near) {
+        if (bx >= g.X - 1) xs[0] = xs[1] = g.X - 1;
+        if (by >= g.Y - 1) ys[0] = ys[1] = g.Y - 1;
+        if (bz >= g.Z - 1) zs[0] = zs[1] = g.Z - 1;
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        xs[k] = clampi(xs[k], g.X);
+        ys[k] = clampi(ys[k], g.Y);
+        zs[k] = clampi(zs[k], g.Z);
+    }
+    V3<T> c[8];
+#pragma unroll
+    for (int k = 0; k < 8; k++)
+        c[k] = src.at((int64_t)xs[k & 1] + (int64_t)g.X * ((int64_t)ys[(k >> 1) & 1] + (int64_t)g.Y * zs[k >> 2]));
+    V3<T> old = cur;
+    V3<T> gr = trilin(c, cur.x - (T)bx, cur.y - (T)by, cur.z - (T)bz);
+    T gm = sqrt(gr.x * gr.x + gr.y * gr.y + gr.z * gr.z);
+    T step = kMaxStep / gm;
+    while (iters-- > 0) {
+        if (!(old.x == cur.x && old.y == cur.y && old.z == cur.z)) {
+            if ((int)cur.x != (int)old.x || (int)cur.y != (int)old.y || (int)cur.z != (int)old.z) return kOutOfVoxel;
+            gr = trilin(c, cur.x - (T)bx, cur.y - (T)by, cur.z - (T)bz);
+            gm = sqrt(gr.x * gr.x + gr.y * gr.y + gr.z * gr.z);
+            if (gm * step > kMaxStep) step = kMaxStep / gm;
+            old = cur;
+        }
+        if (gm < gthr) return kLowGradient;
+        // ascending manifolds: flow downhill (step multiplier -1)
+        const T s = -step, hs = -(T)0.5 * step;
+        const V3<T> full{gr.x * s, gr.y * s, gr.z * s};
+        const V3<T> half{gr.x * hs, gr.y * hs, gr.z * hs};
+        const V3<T> hg = trilin(c, cur.x + half.x - (T)bx, cur.y + half.y - (T)by, cur.z + half.z - (T)bz);
+        const V3<T> two{hg.x * hs + half.x, hg.y * hs + half.y, hg.z * hs + half.z};
+        const T dx = two.x - full.x, dy = two.y - full.y, dz = two.z - full.z;
+        const T err = dx * dx + dy * dy + dz * dz;
+        if (err < ethr) {
+            cur = {cur.x + two.x, cur.y + two.y, cur.z + two.z};
+            if (near) { cur.x = clampc(cur.x, g.X); cur.y = clampc(cur.y, g.Y); cur.z = clampc(cur.z, g.Z); }
+            const T grow = step * (T)1.8, cap = kMaxStep / gm;
+            step = grow < cap ? grow : cap;
+            const uint8_t d = dest[nearest(g, cur, near)];
+            if (d == kCertainTerminal || (stop_on_assigned && d == kAssigned)) return kHitPreassigned;
+        } else {
+            step = step * (T)0.5;
+        }
+    }
+    return kOverMax;
+}
+
+__device__ __forceinline__ int dist_to_boundary(const VGrid& g, int x, int y, int z) {
+    int m = min(x, g.X - 1 - x);
+    m = min(m, min(y, g.Y - 1 - y));
+    return min(m, min(z, g.Z - 1 - z));
+}
+
+// Pass 1 at one level of the coarse-to-fine order: vertices whose coordinates
+// are multiples of `stride` (and not all multiples of 2*stride unless this is
+// the first level).  A streamline stops on a certain or already assigned
+// vertex; the first kPath vertices of its path take the label it reached.
+// Streamlines that run out of step budget are flagged for FP64 re-tracing.
+constexpr int kPath = 32;
+
+__global__ void __launch_bounds__(kThreads)
+pass1_kernel(GradSrc<float> src, VGrid g, LabelBuffers b, int stride, int first_level, float gthr, float ethr,
+             int max_iters) {
+    __shared__ int32_t path_mem[kPath * kThreads];
+    int32_t* path = path_mem + threadIdx.x;
+    const int nx = (g.X + stride - 1) / stride, ny = (g.Y + stride - 1) / stride, nz = (g.Z + stride - 1) / stride;
+    const int64_t n = (int64_t)nx * ny * nz;
+    uint32_t traced = 0, over = 0;
+    for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < n; t += (int64_t)gridDim.x * kThreads) {
+        const int x = (int)(t % nx) * stride, y = (int)((t / nx) % ny) * stride;
+        const int z = (int)(t / ((int64_t)nx * ny)) * stride;
+        if (!first_level && (x % (2 * stride)) == 0 && (y % (2 * stride)) == 0 && (z % (2 * stride)) == 0) continue;
+        const int64_t v0 = (int64_t)x + (int64_t)g.X * ((int64_t)y + (int64_t)g.Y * z);
+        if (b.dest[v0] != kUnassigned) continue;
+        traced++;
+        int np = 0;
+        path[(np++) * kThreads] = (int32_t)v0;
+        V3<float> cur{(float)x, (float)y, (float)z};
+        int cx = x, cy = y, cz = z;
+        int iters = max_iters;
+        int ev;
+        int64_t next;
+        while (true) {
+            const bool near = dist_to_boundary(g, cx, cy, cz) <= 1;
+            ev = advect_voxel<float>(src, g, b.dest, true, gthr, ethr, near, cur, iters);
+            next = nearest(g, cur, near);
+            cx = (int)(next % g.X);
+            cy = (int)((next / g.X) % g.Y);
+            cz = (int)(next / g.sxy);
+            if (np < kPath) path[(np++) * kThreads] = (int32_t)next;
+            if (ev != kOutOfVoxel) break;
+        }
+        const int32_t lab = b.label[next];
+        for (int j = 0; j < np; j++) {
+            const int32_t p = path[j * kThreads];
+            if (b.dest[p] == kUnassigned) {
+                b.label[p] = lab;
+                b.dest[p] = kAssigned;
+            }
+        }
+        if (ev == kOverMax) {
+            over++;
+            b.redo[v0] = 1;
+        }
+    }
+    if (traced) atomicAdd(&b.stats[kStatTraced1], traced);
+    if (over) atomicAdd(&b.stats[kStatOver1], over);
+}
+
+// Block-staged appends to a sub-list (as in the ldir rounds).
+constexpr int kStage = 2048;
+struct Outbox {
+    uint32_t* stage;
+    uint32_t* n;
+    MList out;
+    int sub;
+    uint32_t* overflow;
+    __device__ __forceinline__ void push(uint32_t item) const {
+        const uint32_t slot = atomicAdd(n, 1u);
+        if (slot < (uint32_t)kStage) {
+            stage[slot] = item;
+        } else {
+            const uint32_t gs = mlist_reserve(out, sub, 1u);
+            if (gs < out.sub_cap) out.items[(size_t)sub * out.sub_cap + gs] = item;
+            else atomicOr(overflow, 1u);
+        }
+    }
+};
+
+__device__ __forceinline__ int all6(const VGrid& g, int64_t i, int64_t nb[6]) {
+    const int x = (int)(i % g.X), y = (int)((i / g.X) % g.Y), z = (int)(i / g.sxy);
+    int k = 0;
+    if (x < g.X - 1) nb[k++] = i + 1;
+    if (x > 0) nb[k++] = i - 1;
+    if (y < g.Y - 1) nb[k++] = i + g.X;
+    if (y > 0) nb[k++] = i - g.X;
+    if (z < g.Z - 1) nb[k++] = i + g.sxy;
+    if (z > 0) nb[k++] = i - g.sxy;
+    return k;
+}
+
+__global__ void __launch_bounds__(kThreads)
+pass2_kernel(VGrid g, LabelBuffers b, MList out, uint32_t* __restrict__ overflow) {
+    __shared__ uint32_t stage[kStage];
+    __shared__ uint32_t s_n, s_base;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    const Outbox box{stage, &s_n, out, (int)(blockIdx.x % kSubs), overflow};
+    const int64_t n = g.verts();
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        if (b.dest[i] != kAssigned) continue;
+        bool push = b.redo[i] != 0;
+        if (!push) {
+            int64_t nb[6];
+            const int k = all6(g, i, nb);
+            const int32_t l = b.label[i];
+            for (int j = 0; j < k; j++)
+                if (b.label[nb[j]] != l) { push = true; break; }
+        }
+        if (push) {
+            b.stamp[i] = 1u;
+            box.push((uint32_t)i);
+        }
+    }
+    __syncthreads();
+    mlist_flush_block(out, box.sub, stage, s_n < (uint32_t)kStage ? s_n : (uint32_t)kStage, &s_base, overflow);
+}
+
+// Pass 3 round: FP64 re-integration to the original certain regions only.
+__global__ void __launch_bounds__(kThreads)
+pass3_kernel(GradSrc<double> src, VGrid g, LabelBuffers b, MList in, MList out, uint32_t* __restrict__ clear,
+             uint32_t* __restrict__ overflow, uint32_t round, double gthr, double ethr, int max_iters) {
+    __shared__ uint32_t stage[kStage];
+    __shared__ uint32_t prefix[kSubs + 1];
+    __shared__ uint32_t s_n, s_base;
+    if (blockIdx.x == 0 && threadIdx.x < (uint32_t)kSubs) clear[threadIdx.x] = 0u;
+    if (threadIdx.x == 0) s_n = 0;
+    const uint32_t total = mlist_prefix(in, prefix);
+    const Outbox box{stage, &s_n, out, (int)(blockIdx.x % kSubs), overflow};
+    uint32_t traced = 0, over = 0;
+    for (uint32_t it = blockIdx.x * kThreads + threadIdx.x; it < total; it += gridDim.x * kThreads) {
+        const int64_t v = mlist_at(in, prefix, it);
+        if (b.dest[v] != kAssigned) continue;  // already re-integrated
+        traced++;
+        const int32_t init = b.label[v];
+        const int x = (int)(v % g.X), y = (int)((v / g.X) % g.Y), z = (int)(v / g.sxy);
+        V3<double> cur{(double)x, (double)y, (double)z};
+        int cx = x, cy = y, cz = z;
+        int iters = max_iters;
+        int ev;
+        int64_t next;
+        while (true) {
+            const bool near = dist_to_boundary(g, cx, cy, cz) <= 1;
+            ev = advect_voxel<double>(src, g, b.dest, false, gthr, ethr, near, cur, iters);
+            next = nearest(g, cur, near);
+            cx = (int)(next % g.X);
+            cy = (int)((next / g.X) % g.Y);
+            cz = (int)(next / g.sxy);
+            if (ev != kOutOfVoxel) break;
+        }
+        if (ev == kOverMax) over++;
+        const int32_t nl = b.label[next];
+        b.label[v] = nl;
+        b.dest[v] = kCertainNonterminal;
+        if (nl != init) {
+            int64_t nb[6];
+            const int k = all6(g, v, nb);
+            for (int j = 0; j < k; j++) {
+                if (b.dest[nb[j]] == kAssigned && b.label[nb[j]] != nl &&
+                    atomicExch(&b.stamp[nb[j]], round + 1u) != round + 1u)
+                    box.push((uint32_t)nb[j]);
+            }
+        }
+    }
+    if (traced) atomicAdd(&b.stats[kStatTraced3], traced);
+    if (over) atomicAdd(&b.stats[kStatOver3], over);
+    __syncthreads();
+    mlist_flush_block(out, box.sub, stage, s_n < (uint32_t)kStage ? s_n : (uint32_t)kStage, &s_base, overflow);
+}
+
+// ---- cleanup of isolated components ------------------------------------------
+
+__global__ void cc_init_kernel(VGrid g, LabelBuffers b) {
+    const int64_t n = g.verts();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        b.rep[i] = (int32_t)i;
+        b.label2[i] = b.label[i];
+    }
+}
+
+// Same-label components; a component is represented by its earliest vertex
+// (the root of the reference's union-find, whose merges point the later
+// representative at the earlier one).  Min-propagation plus pointer jumping.
+__global__ void __launch_bounds__(kThreads)
+cc_hook_kernel(const float* __restrict__ f, VGrid g, LabelBuffers b, uint32_t* __restrict__ changed) {
+    const int64_t n = g.verts();
+    bool any = false;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        const int32_t l = b.label2[i];
+        int32_t best = b.rep[i];
+        int64_t nb[6];
+        const int k = all6(g, i, nb);
+        for (int j = 0; j < k; j++) {
+            if (b.label2[nb[j]] != l) continue;
+            const int32_t r = b.rep[nb[j]];
+            if (before(f, r, best)) best = r;
+        }
+        const int32_t jump = b.rep[best];
+        if (before(f, jump, best)) best = jump;
+        if (best != b.rep[i]) {
+            b.rep[i] = best;
+            any = true;
+        }
+    }
+    if (any) atomicOr(changed, 1u);
+}
+
+// For each component root: the earliest All6 neighbour before it in a
+// region with another label; rep2[root] = that neighbour's root (else root).
+__global__ void __launch_bounds__(kThreads)
+cc_roots_kernel(const float* __restrict__ f, VGrid g, LabelBuffers b) {
+    const int64_t n = g.verts();
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        if (b.rep[i] != (int32_t)i) {
+            b.rep2[i] = -1;
+            continue;
+        }
+        int64_t nb[6];
+        const int k = all6(g, i, nb);
+        int64_t earliest = i;
+        const int32_t l = b.label2[i];
+        for (int j = 0; j < k; j++) {
+            if (!before(f, nb[j], i)) continue;
+            if (b.label2[nb[j]] == l) continue;
+            if (before(f, nb[j], earliest)) earliest = nb[j];
+        }
+        b.rep2[i] = earliest == i ? (int32_t)i : b.rep[earliest];
+    }
+}
+
+// Resolve merge chains among roots (targets are strictly earlier: no cycles).
+__global__ void __launch_bounds__(kThreads)
+cc_resolve_kernel(VGrid g, LabelBuffers b, uint32_t* __restrict__ changed) {
+    const int64_t n = g.verts();
+    bool any = false;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        const int32_t t = b.rep2[i];
+        if (t < 0) continue;
+        const int32_t tt = b.rep2[t];
+        if (tt != t) {
+            b.rep2[i] = tt;
+            any = true;
+        }
+    }
+    if (any) atomicOr(changed, 1u);
+}
+
+__global__ void __launch_bounds__(kThreads)
+cc_apply_kernel(VGrid g, LabelBuffers b) {
+    const int64_t n = g.verts();
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        if (b.label2[i] == -1) continue;
+        b.label[i] = b.label2[b.rep2[b.rep[i]]];
+    }
+}
+
+// ---- restriction labels -------------------------------------------------------
+
+__global__ void __launch_bounds__(kThreads)
+min_boundary_kernel(VGrid g, const int32_t* __restrict__ label, int8_t* __restrict__ r) {
+    const int64_t Mx = 2 * (int64_t)g.X - 1, My = 2 * (int64_t)g.Y - 1, Mz = 2 * (int64_t)g.Z - 1;
+    const int64_t C = Mx * My * Mz;
+    for (int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x; c < C; c += (int64_t)gridDim.x * kThreads) {
+        const int64_t cx = c % Mx, cy = (c / Mx) % My, cz = c / (Mx * My);
+        const int ox = (int)(cx & 1), oy = (int)(cy & 1), oz = (int)(cz & 1);
+        int8_t res = 0;
+        if (ox + oy + oz > 0) {
+            // vertices of the cell: corners (cx-ox+2a, ...)/2 for a in {0,1}
+            const int64_t vx = (cx - ox) / 2, vy = (cy - oy) / 2, vz = (cz - oz) / 2;
+            int32_t l[8];
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+                const int a = k & 1, bb = (k >> 1) & 1, d = k >> 2;
+                if ((a && !ox) || (bb && !oy) || (d && !oz)) { l[k] = -2; continue; }
+                l[k] = label[(vx + a) + (int64_t)g.X * ((vy + bb) + (int64_t)g.Y * (vz + d))];
+            }
+            // an edge of the cell joins corners that differ in one odd axis
+#pragma unroll
+            for (int k = 0; k < 8; k++) {
+#pragma unroll
+                for (int ax = 0; ax < 3; ax++) {
+                    const int k2 = k | (1 << ax);
+                    if (k2 == k) continue;
+                    if (l[k] < -1 || l[k2] < -1) continue;
+                    if (l[k] == -1 || l[k2] == -1) continue;
+                    if (l[k] != l[k2]) res = 1;
+                }
+            }
+        }
+        r[c] = res;
+    }
+}
+
+// ---- combinatorial structure --------------------------------------------------
+
+struct CellG {
+    int64_t Mx, My, sxy;
+};
+
+__device__ __forceinline__ int64_t grad_pair(const CellG& cg, const uint8_t* grad, int64_t c) {
+    const uint32_t d = (grad[c] >> 2) & 7u;
+    switch (d) {
+        case 0: return c + 1;
+        case 1: return c - 1;
+        case 2: return c + cg.Mx;
+        case 3: return c - cg.Mx;
+        case 4: return c + cg.sxy;
+        case 5: return c - cg.sxy;
+    }
+    return c;
+}
+
+// From a vertex cell, follow vertex -> paired edge -> other vertex until a
+// critical vertex (rec_td, gi_extrema_region_builder.h:286-301).
+__device__ int64_t trace_down_vertex(const CellG& cg, const uint8_t* grad, int64_t c, int64_t limit, int64_t cells) {
+    for (int64_t s = 0; s < limit; s++) {
+        const uint32_t d = (grad[c] >> 2) & 7u;
+        if (d == 7u) return c;
+        const int64_t e = grad_pair(cg, grad, c);
+        // the other vertex of edge e: e + (e - c)
+        const int64_t nx = 2 * e - c;
+        if (e < 0 || e >= cells || nx < 0 || nx >= cells) return -1;  // only a corrupt byte leads out
+        c = nx;
+    }
+    return -1;
+}
+
+__global__ void __launch_bounds__(kThreads)
+saddle_trace_kernel(VGrid g, const uint8_t* __restrict__ grad, MList list, int64_t* __restrict__ out,
+                    int64_t* __restrict__ saddle_cell, uint32_t n_max, uint32_t* __restrict__ count) {
+    __shared__ uint32_t prefix[kSubs + 1];
+    const uint32_t total = mlist_prefix(list, prefix);
+    if (blockIdx.x == 0 && threadIdx.x == 0) *count = total;
+    const CellG cg{2 * (int64_t)g.X - 1, 2 * (int64_t)g.Y - 1, (2 * (int64_t)g.X - 1) * (2 * (int64_t)g.Y - 1)};
+    const int64_t limit = g.verts() + 1, cells = cg.sxy * (2 * (int64_t)g.Z - 1);
+    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < total && i < n_max; i += gridDim.x * kThreads) {
+        const int64_t s = mlist_at(list, prefix, i);
+        const int64_t cx = s % cg.Mx, cy = (s / cg.Mx) % cg.My;
+        const int64_t step = (cx & 1) ? 1 : ((cy & 1) ? cg.Mx : cg.sxy);
+        // facets of the edge in +axis, -axis order
+        out[2 * (int64_t)i] = trace_down_vertex(cg, grad, s + step, limit, cells);
+        out[2 * (int64_t)i + 1] = trace_down_vertex(cg, grad, s - step, limit, cells);
+        saddle_cell[i] = s;
+    }
+}
+
+__global__ void __launch_bounds__(kThreads)
+collect_minima_kernel(VGrid g, const uint8_t* __restrict__ grad, MList out, uint32_t* __restrict__ overflow) {
+    __shared__ uint32_t stage[kStage];
+    __shared__ uint32_t s_n, s_base;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    const Outbox box{stage, &s_n, out, (int)(blockIdx.x % kSubs), overflow};
+    const int64_t Mx = 2 * (int64_t)g.X - 1, My = 2 * (int64_t)g.Y - 1;
+    const int64_t n = g.verts();
+    for (int64_t v = (int64_t)blockIdx.x * kThreads + threadIdx.x; v < n; v += (int64_t)gridDim.x * kThreads) {
+        const int64_t x = v % g.X, y = (v / g.X) % g.Y, z = v / g.sxy;
+        const int64_t c = 2 * x + Mx * (2 * y + My * 2 * z);
+        if (((grad[c] >> 2) & 7u) == 7u) box.push((uint32_t)v);
+    }
+    __syncthreads();
+    mlist_flush_block(out, box.sub, stage, s_n < (uint32_t)kStage ? s_n : (uint32_t)kStage, &s_base, overflow);
+}
+
+__global__ void __launch_bounds__(kThreads)
+combinatorial_labels_kernel(VGrid g, const uint8_t* __restrict__ grad, const int32_t* __restrict__ min_label,
+                            int32_t* __restrict__ out, const uint8_t* __restrict__ only_where_zero) {
+    const CellG cg{2 * (int64_t)g.X - 1, 2 * (int64_t)g.Y - 1, (2 * (int64_t)g.X - 1) * (2 * (int64_t)g.Y - 1)};
+    const int64_t n = g.verts(), limit = n + 1, cells = cg.sxy * (2 * (int64_t)g.Z - 1);
+    for (int64_t v = (int64_t)blockIdx.x * kThreads + threadIdx.x; v < n; v += (int64_t)gridDim.x * kThreads) {
+        if (only_where_zero && only_where_zero[v]) continue;
+        const int64_t x = v % g.X, y = (v / g.X) % g.Y, z = v / g.sxy;
+        const int64_t c = 2 * x + cg.Mx * (2 * y + cg.My * 2 * z);
+        const int64_t m = trace_down_vertex(cg, grad, c, limit, cells);
+        if (m < 0) { out[v] = -1; continue; }
+        const int64_t mx = (m % cg.Mx) / 2, my = ((m / cg.Mx) % cg.My) / 2, mz = (m / cg.sxy) / 2;
+        out[v] = min_label[mx + (int64_t)g.X * (my + (int64_t)g.Y * mz)];
+    }
+}
+
+__global__ void dest_init_kernel(VGrid g, LabelBuffers b) {
+    const int64_t n = g.verts();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        b.dest[i] = b.label[i] == -1 ? kUnassigned : kCertainTerminal;
+        b.redo[i] = 0;
+        b.stamp[i] = 0;
+    }
+}
+
+__global__ void gather_f32_kernel(const float* __restrict__ f, const int64_t* __restrict__ idx, int64_t n,
+                                  float* __restrict__ out) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        out[i] = idx[i] >= 0 ? f[idx[i]] : 0.f;
+}
+
+__global__ void __launch_bounds__(kThreads)
+apply_mask_kernel(VGrid g, LabelBuffers b, const uint8_t* __restrict__ mask) {
+    const int64_t n = g.verts();
+    uint32_t inside = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        if (b.dest[i] != kUnassigned) continue;
+        if (mask[i]) {
+            inside++;
+        } else {
+            b.label[i] = b.label2[i];
+            b.dest[i] = kAssigned;
+        }
+    }
+    if (inside) atomicAdd(&b.stats[kStatMasked], inside);
+}
+
+__global__ void fill_u32_kernel(uint32_t* p, int64_t n, uint32_t v) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+
+inline uint32_t grid_for(int64_t n, uint32_t cap = 16384) {
+    int64_t b = (n + kThreads - 1) / kThreads;
+    if (b < 1) b = 1;
+    return (uint32_t)(b > cap ? cap : b);
+}
+
+}  // namespace
+
+hipError_t launch_grad_field(const VGrid& g, const float* values, float4* gradf, hipStream_t s) {
+    hipLaunchKernelGGL(grad_field_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, values, g, gradf);
+    return hipGetLastError();
+}
+hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
+    hipLaunchKernelGGL(fill_i32_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, p, n, v);
+    return hipGetLastError();
+}
+hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s) {
+    hipLaunchKernelGGL(fill_u8_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, p, n, v);
+    return hipGetLastError();
+}
+hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, p, n, v);
+    return hipGetLastError();
+}
+hipError_t launch_seed_scatter(const int32_t* seed_vert, const int32_t* seed_label, int n, const LabelBuffers& b,
+                               hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(seed_scatter_kernel, dim3((n + kThreads - 1) / kThreads), dim3(kThreads), 0, s, seed_vert,
+                       seed_label, n, b.label, b.gradf, b.seedmask);
+    return hipGetLastError();
+}
+hipError_t launch_certain_sweep(const VGrid& g, const float* values, int32_t* label, uint32_t* changed, hipStream_t s) {
+    hipLaunchKernelGGL(certain_sweep_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, values, g, label, changed);
+    return hipGetLastError();
+}
+hipError_t launch_dest_init(const VGrid& g, const LabelBuffers& b, hipStream_t s) {
+    hipLaunchKernelGGL(dest_init_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, g, b);
+    return hipGetLastError();
+}
+hipError_t launch_pass1(const VGrid& g, const LabelBuffers& b, int stride, bool first_level, float gthr, float ethr,
+                        int max_iters, hipStream_t s) {
+    GradSrc<float> src{b.gradf, b.values, b.seedmask, g};
+    const int64_t n = (int64_t)((g.X + stride - 1) / stride) * ((g.Y + stride - 1) / stride) * ((g.Z + stride - 1) / stride);
+    hipLaunchKernelGGL(pass1_kernel, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, src, g, b, stride,
+                       first_level ? 1 : 0, gthr, ethr, max_iters);
+    return hipGetLastError();
+}
+hipError_t launch_pass2(const VGrid& g, const LabelBuffers& b, const MList& out, uint32_t* overflow, hipStream_t s) {
+    hipLaunchKernelGGL(pass2_kernel, dim3(grid_for(g.verts(), 8192)), dim3(kThreads), 0, s, g, b, out, overflow);
+    return hipGetLastError();
+}
+hipError_t launch_pass3_round(const VGrid& g, const LabelBuffers& b, const MList& in, const MList& out,
+                              uint32_t* clear_counts, uint32_t* overflow, uint32_t round, float gthr, float ethr,
+                              int max_iters, hipStream_t s) {
+    GradSrc<double> src{b.gradf, b.values, b.seedmask, g};
+    hipLaunchKernelGGL(pass3_kernel, dim3(2048), dim3(kThreads), 0, s, src, g, b, in, out, clear_counts, overflow,
+                       round, (double)gthr, (double)ethr, max_iters);
+    return hipGetLastError();
+}
+hipError_t launch_cc_init(const VGrid& g, const LabelBuffers& b, hipStream_t s) {
+    hipLaunchKernelGGL(cc_init_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, g, b);
+    return hipGetLastError();
+}
+hipError_t launch_cc_hook(const VGrid& g, const LabelBuffers& b, uint32_t* changed, hipStream_t s) {
+    hipLaunchKernelGGL(cc_hook_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, b.values, g, b, changed);
+    return hipGetLastError();
+}
+hipError_t launch_cc_roots(const VGrid& g, const LabelBuffers& b, hipStream_t s) {
+    hipLaunchKernelGGL(cc_roots_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, b.values, g, b);
+    return hipGetLastError();
+}
+hipError_t launch_cc_resolve(const VGrid& g, const LabelBuffers& b, uint32_t* changed, hipStream_t s) {
+    hipLaunchKernelGGL(cc_resolve_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, g, b, changed);
+    return hipGetLastError();
+}
+hipError_t launch_cc_apply(const VGrid& g, const LabelBuffers& b, hipStream_t s) {
+    hipLaunchKernelGGL(cc_apply_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, g, b);
+    return hipGetLastError();
+}
+hipError_t launch_min_boundary(const VGrid& g, const int32_t* label, int8_t* restriction, hipStream_t s) {
+    const int64_t C = (2 * (int64_t)g.X - 1) * (2 * (int64_t)g.Y - 1) * (2 * (int64_t)g.Z - 1);
+    hipLaunchKernelGGL(min_boundary_kernel, dim3(grid_for(C)), dim3(kThreads), 0, s, g, label, restriction);
+    return hipGetLastError();
+}
+hipError_t launch_saddle_trace(const VGrid& g, const uint8_t* grad, const MList& list, int64_t* out,
+                               int64_t* saddle_cell, uint32_t n_max, uint32_t* count, hipStream_t s) {
+    hipLaunchKernelGGL(saddle_trace_kernel, dim3(1024), dim3(kThreads), 0, s, g, grad, list, out, saddle_cell, n_max,
+                       count);
+    return hipGetLastError();
+}
+hipError_t launch_collect_minima(const VGrid& g, const uint8_t* grad, const MList& out, uint32_t* overflow,
+                                 hipStream_t s) {
+    hipLaunchKernelGGL(collect_minima_kernel, dim3(grid_for(g.verts(), 8192)), dim3(kThreads), 0, s, g, grad, out,
+                       overflow);
+    return hipGetLastError();
+}
+hipError_t launch_gather_f32(const float* f, const int64_t* idx, int64_t n, float* out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(gather_f32_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, f, idx, n, out);
+    return hipGetLastError();
+}
+hipError_t launch_apply_mask(const VGrid& g, const LabelBuffers& b, const uint8_t* mask, hipStream_t s) {
+    hipLaunchKernelGGL(apply_mask_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, g, b, mask);
+    return hipGetLastError();
+}
+hipError_t launch_combinatorial_labels(const VGrid& g, const uint8_t* grad, const int32_t* min_label, int32_t* out,
+                                       const uint8_t* only_where_zero, hipStream_t s) {
+    hipLaunchKernelGGL(combinatorial_labels_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, g, grad,
+                       min_label, out, only_where_zero);
+    return hipGetLastError();
+}
+
+}  // namespace msceer

@@ -1,0 +1,98 @@
+// Launchers of the accurate-labelling kernels (integrate.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace msceer {
+
+struct MList;
+
+// Vertex grid geometry for kernels.
+struct VGrid {
+    int X, Y, Z;
+    int64_t sxy;  // X*Y
+    __host__ __device__ int64_t verts() const { return sxy * Z; }
+};
+
+inline VGrid make_vgrid(int64_t x, int64_t y, int64_t z) {
+    VGrid g;
+    g.X = (int)x; g.Y = (int)y; g.Z = (int)z;
+    g.sxy = x * y;
+    return g;
+}
+
+// Device buffers of one labelling run (all sized to the vertex count V unless noted).
+struct LabelBuffers {
+    const float* values;
+    float4* gradf;       // FP32 gradient field
+    int32_t* label;      // region label per vertex (-1 unassigned)
+    int32_t* label2;     // scratch labels
+    uint8_t* dest;       // DestType per vertex
+    uint8_t* seedmask;   // 1 at region seeds (their gradient is zero)
+    uint8_t* redo;       // 1: re-trace in FP64 (pass 1 ran out of budget)
+    uint32_t* stamp;     // work-list dedupe stamps
+    int32_t* rep;        // component representatives (cleanup)
+    int32_t* rep2;
+    uint32_t* stats;     // device counters (see IntegStat)
+};
+
+enum IntegStat : int {
+    kStatTraced1 = 0,    // streamlines traced in pass 1
+    kStatOver1 = 1,      // pass-1 streamlines that ran out of step budget
+    kStatTraced3 = 2,    // FP64 re-integrations in pass 3
+    kStatOver3 = 3,      // pass-3 streamlines that ran out of step budget
+    kStatChanged = 4,    // sweep / round change flag
+    kStatMasked = 5,     // unassigned vertices inside the integration mask
+    kStatNum = 8
+};
+
+hipError_t launch_grad_field(const VGrid& g, const float* values, float4* gradf, hipStream_t s);
+hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
+hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
+// Seeds: label[seed_vert[i]] = seed_label[i]; zero gradient; seedmask = 1.
+hipError_t launch_seed_scatter(const int32_t* seed_vert, const int32_t* seed_label, int n, const LabelBuffers& b,
+                               hipStream_t s);
+// One sweep of the certain-region fixed point; sets *changed when a label was set.
+hipError_t launch_certain_sweep(const VGrid& g, const float* values, int32_t* label, uint32_t* changed, hipStream_t s);
+// dest = UNASSIGNED where label == -1 else CERTAIN_TERMINAL; redo = 0; stamp = 0.
+hipError_t launch_dest_init(const VGrid& g, const LabelBuffers& b, hipStream_t s);
+// Pass 1 at one level of the coarse-to-fine order.
+hipError_t launch_pass1(const VGrid& g, const LabelBuffers& b, int stride, bool first_level, float gthr, float ethr,
+                        int max_iters, hipStream_t s);
+// Pass 2: every ASSIGNED vertex with a differently labelled All6 neighbour
+// (or flagged for re-tracing) goes to `out`.
+hipError_t launch_pass2(const VGrid& g, const LabelBuffers& b, const MList& out, uint32_t* overflow, hipStream_t s);
+// One pass-3 round over `in` (FP64 re-integration to the certain regions);
+// changed labels push their ASSIGNED neighbours to `out`; `clear` counts are zeroed.
+hipError_t launch_pass3_round(const VGrid& g, const LabelBuffers& b, const MList& in, const MList& out,
+                              uint32_t* clear_counts, uint32_t* overflow, uint32_t round, float gthr, float ethr,
+                              int max_iters, hipStream_t s);
+// Cleanup of isolated components.
+hipError_t launch_cc_init(const VGrid& g, const LabelBuffers& b, hipStream_t s);
+hipError_t launch_cc_hook(const VGrid& g, const LabelBuffers& b, uint32_t* changed, hipStream_t s);
+hipError_t launch_cc_roots(const VGrid& g, const LabelBuffers& b, hipStream_t s);
+hipError_t launch_cc_resolve(const VGrid& g, const LabelBuffers& b, uint32_t* changed, hipStream_t s);
+hipError_t launch_cc_apply(const VGrid& g, const LabelBuffers& b, hipStream_t s);
+// Restriction labels from vertex labels (MIN boundary): 1 on every cell
+// with an edge whose two vertices carry valid, different labels.
+hipError_t launch_min_boundary(const VGrid& g, const int32_t* label, int8_t* restriction, hipStream_t s);
+// Follow the V-paths below both vertices of every critical 1-cell of `list`
+// to critical vertices; out[2i], out[2i+1] receive the two minima's cell ids
+// (-1 if a path runs off), saddle_cell[i] the saddle.
+hipError_t launch_saddle_trace(const VGrid& g, const uint8_t* grad, const MList& list, int64_t* out,
+                               int64_t* saddle_cell, uint32_t n_max, uint32_t* count, hipStream_t s);
+// Critical vertices of a .grad (pair code 7 at even cells) appended to `out`.
+hipError_t launch_collect_minima(const VGrid& g, const uint8_t* grad, const MList& out, uint32_t* overflow,
+                                 hipStream_t s);
+// out[i] = f[idx[i]] (0 where idx[i] < 0).
+hipError_t launch_gather_f32(const float* f, const int64_t* idx, int64_t n, float* out, hipStream_t s);
+// Outside the integration mask, unassigned vertices take their
+// combinatorial label (b.label2) and count as assigned; counts the rest.
+hipError_t launch_apply_mask(const VGrid& g, const LabelBuffers& b, const uint8_t* mask, hipStream_t s);
+// Combinatorial labels: every vertex follows its V-path down to a critical
+// vertex and takes that vertex's region label (for vertices not integrated).
+hipError_t launch_combinatorial_labels(const VGrid& g, const uint8_t* grad, const int32_t* min_label, int32_t* out,
+                                       const uint8_t* only_where_zero, hipStream_t s);
+
+}  // namespace msceer

@@ -1,0 +1,330 @@
+// msceer_ondemand_asc: accurate ascending labelling and its conforming .grad
+// (orchestration of the kernels in lstar.hip, ldir.hip and integrate.hip).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <map>
+#include <vector>
+
+#include "ctx.hpp"
+#include "extgraph.hpp"
+#include "integrate.hpp"
+
+using namespace msceer;
+
+namespace msceer {
+namespace {
+
+// Copy the concatenation of a multi-list to the host.
+int mlist_to_host(msceer_ctx* ctx, const MList& l, std::vector<uint32_t>& out) {
+    std::vector<uint32_t> counts(kSubs);
+    CHECK_HIP(hipMemcpyAsync(counts.data(), l.counts, sizeof(uint32_t) * kSubs, hipMemcpyDeviceToHost, ctx->stream));
+    CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    out.clear();
+    for (int s = 0; s < kSubs; s++) {
+        const uint32_t n = std::min(counts[s], l.sub_cap);
+        if (!n) continue;
+        const size_t at = out.size();
+        out.resize(at + n);
+        CHECK_HIP(hipMemcpyAsync(out.data() + at, l.items + (size_t)s * l.sub_cap, sizeof(uint32_t) * n,
+                                 hipMemcpyDeviceToHost, ctx->stream));
+    }
+    CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    return MSCEER_OK;
+}
+
+// Run an iterative kernel in groups of `group` launches until a whole group
+// reports no change (flag word in the control block).
+template <typename Launch>
+int iterate_until_stable(msceer_ctx* ctx, int group, int max_groups, int* count, Launch launch) {
+    uint32_t* flag = ctl_dev(ctx) + kCtlFlag;
+    for (int gi = 0; gi < max_groups; gi++) {
+        CHECK_HIP(launch_zero(flag, 1, ctx->stream));
+        for (int j = 0; j < group; j++) {
+            CHECK_HIP(launch(flag));
+            (*count)++;
+        }
+        int rc = read_ctl(ctx, kCtlFlag, 1);
+        if (rc) return rc;
+        if (ctx->h_ctl[kCtlFlag] == 0) return MSCEER_OK;
+    }
+    return MSCEER_EHIP;
+}
+
+float elapsed(hipEvent_t a, hipEvent_t b) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, a, b);
+    return ms;
+}
+
+}  // namespace
+}  // namespace msceer
+
+extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz,
+                                   float err_thr, float grad_thr, int32_t max_iters, float presimp,
+                                   const uint8_t* integrate_mask, uint8_t* grad_out, int32_t* labels_out,
+                                   msceer_label_stats* stats, uint32_t flags) {
+    if (!ctx || !values) return MSCEER_EINVAL;
+    if (flags & ~(MSCEER_DEVICE_PTRS | MSCEER_TIMING | MSCEER_STOP_REGIONS | MSCEER_STOP_INTEGRATED))
+        return MSCEER_EINVAL;
+    if (!(err_thr > 0.f) || !(grad_thr >= 0.f) || max_iters < 1 || !(presimp >= 0.f)) return MSCEER_EINVAL;
+    int rc = check_dims(nx, ny, nz);
+    if (rc) return rc;
+    if (nx * ny * nz >= (int64_t)1 << 31) return MSCEER_ETOOBIG;
+    CHECK_HIP(hipSetDevice(ctx->device));
+    const Dims d = make_dims(nx, ny, nz);
+    const VGrid g = make_vgrid(nx, ny, nz);
+    const int64_t V = (int64_t)d.verts, C = (int64_t)d.cells;
+    const bool dev = flags & MSCEER_DEVICE_PTRS;
+    hipStream_t s = ctx->stream;
+    hipEvent_t* ev = ctx->ev;
+    msceer_label_stats st{};
+
+    // ---- buffers ----
+    const float* d_values = values;
+    if (!dev) {
+        CHECK_HIP(ctx->values.ensure(sizeof(float) * V));
+        CHECK_HIP(hipMemcpyAsync(ctx->values.ptr, values, sizeof(float) * V, hipMemcpyHostToDevice, s));
+        d_values = ctx->values.as<float>();
+    }
+    uint8_t* d_grad = (dev && grad_out) ? grad_out : nullptr;
+    if (!d_grad) {
+        CHECK_HIP(ctx->grad.ensure(C));
+        d_grad = ctx->grad.as<uint8_t>();
+    }
+    CHECK_HIP(ctx->gradf.ensure(sizeof(float4) * V));
+    CHECK_HIP(ctx->label.ensure(sizeof(int32_t) * V));
+    CHECK_HIP(ctx->label2.ensure(sizeof(int32_t) * V));
+    CHECK_HIP(ctx->rep.ensure(sizeof(int32_t) * V));
+    CHECK_HIP(ctx->rep2.ensure(sizeof(int32_t) * V));
+    CHECK_HIP(ctx->stamp.ensure(sizeof(uint32_t) * V));
+    CHECK_HIP(ctx->dest.ensure(V));
+    CHECK_HIP(ctx->seedmask.ensure(V));
+    CHECK_HIP(ctx->redo.ensure(V));
+    CHECK_HIP(ctx->restriction.ensure(C));
+    uint32_t list_sub = 0, front_sub = 0;
+    rc = ensure_lists(ctx, d, &list_sub, &front_sub);
+    if (rc) return rc;
+    uint32_t* ctl = ctl_dev(ctx);
+    uint32_t* l1 = ctx->lists.as<uint32_t>();
+    const MList list1{l1, ctl + kCtlList1, list_sub};
+    const MList list2{l1 + (size_t)list_sub * kSubs, ctl + kCtlList2, list_sub};
+    LabelBuffers b;
+    b.values = d_values;
+    b.gradf = ctx->gradf.as<float4>();
+    b.label = ctx->label.as<int32_t>();
+    b.label2 = ctx->label2.as<int32_t>();
+    b.dest = ctx->dest.as<uint8_t>();
+    b.seedmask = ctx->seedmask.as<uint8_t>();
+    b.redo = ctx->redo.as<uint8_t>();
+    b.stamp = ctx->stamp.as<uint32_t>();
+    b.rep = ctx->rep.as<int32_t>();
+    b.rep2 = ctx->rep2.as<int32_t>();
+    b.stats = ctl + kCtlIntStats;
+    const uint8_t* d_mask = integrate_mask;
+    if (integrate_mask && !dev) {
+        CHECK_HIP(ctx->mask.ensure(V));
+        CHECK_HIP(hipMemcpyAsync(ctx->mask.ptr, integrate_mask, V, hipMemcpyHostToDevice, s));
+        d_mask = ctx->mask.as<uint8_t>();
+    }
+
+    // ---- base gradient ----
+    CHECK_HIP(hipEventRecord(ev[0], s));
+    CHECK_HIP(launch_zero(ctl, kCtlWords, s));
+    LstarBuffers lb{d_values, d_grad, nullptr, &list1, &list2, ctl + kCtlStats, ctl + kCtlOverflow};
+    CHECK_HIP(launch_lstar(d, lb, s));
+    CHECK_HIP(hipEventRecord(ev[1], s));
+
+    // ---- extremum graph (critical vertices and 1-saddles) ----
+    const uint32_t fs = front_sub;
+    const MList minl{ctx->front0.as<uint32_t>(), ctl + kCtlRound, fs};
+    CHECK_HIP(launch_zero(ctl + kCtlRound, kSubs, s));
+    CHECK_HIP(launch_collect_minima(g, d_grad, minl, ctl + kCtlOverflow, s));
+    rc = read_ctl(ctx, 0, kCtlWords);
+    if (rc) return rc;
+    if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;  // more critical cells than list room
+    std::vector<uint32_t> minima, saddle_list;
+    rc = mlist_to_host(ctx, minl, minima);
+    if (rc) return rc;
+    std::sort(minima.begin(), minima.end());
+    const uint64_t n_sad = list_total(ctx->h_ctl + kCtlList1, list_sub);
+    // V-paths below the saddles (device), then values at minima and saddle vertices
+    CHECK_HIP(ctx->host_io.ensure(sizeof(int64_t) * (3 * n_sad + 16)));
+    int64_t* d_trace = ctx->host_io.as<int64_t>();
+    CHECK_HIP(launch_saddle_trace(g, d_grad, list1, d_trace, d_trace + 2 * n_sad, (uint32_t)n_sad, ctl + kCtlCount, s));
+    std::vector<int64_t> tr(3 * n_sad);
+    if (n_sad) CHECK_HIP(hipMemcpyAsync(tr.data(), d_trace, sizeof(int64_t) * 3 * n_sad, hipMemcpyDeviceToHost, s));
+    CHECK_HIP(hipStreamSynchronize(s));
+    // gather values: minima vertices, then both vertices of every saddle edge
+    const int64_t Mx = 2 * nx - 1, My = 2 * ny - 1;
+    auto vnum = [&](int64_t c) { return (c % Mx) / 2 + nx * (((c / Mx) % My) / 2 + ny * ((c / (Mx * My)) / 2)); };
+    std::vector<int64_t> idx;
+    idx.reserve(minima.size() + 2 * n_sad);
+    for (uint32_t v : minima) idx.push_back(v);
+    std::vector<int64_t> sad_a(n_sad), sad_b(n_sad);
+    for (uint64_t i = 0; i < n_sad; i++) {
+        const int64_t sc = tr[2 * n_sad + i];
+        const int64_t cx = sc % Mx, cy = (sc / Mx) % My;
+        const int64_t step = (cx & 1) ? 1 : ((cy & 1) ? Mx : Mx * My);
+        sad_a[i] = vnum(sc + step);
+        sad_b[i] = vnum(sc - step);
+        idx.push_back(sad_a[i]);
+        idx.push_back(sad_b[i]);
+    }
+    std::vector<float> vals(idx.size());
+    if (!idx.empty()) {
+        CHECK_HIP(ctx->host_io.ensure((sizeof(int64_t) + sizeof(float)) * idx.size() + 64));
+        int64_t* d_idx = ctx->host_io.as<int64_t>();
+        float* d_vals = reinterpret_cast<float*>(d_idx + idx.size());
+        CHECK_HIP(hipMemcpyAsync(d_idx, idx.data(), sizeof(int64_t) * idx.size(), hipMemcpyHostToDevice, s));
+        CHECK_HIP(launch_gather_f32(d_values, d_idx, (int64_t)idx.size(), d_vals, s));
+        CHECK_HIP(hipMemcpyAsync(vals.data(), d_vals, sizeof(float) * idx.size(), hipMemcpyDeviceToHost, s));
+        CHECK_HIP(hipStreamSynchronize(s));
+    }
+    std::vector<MinNode> nodes(minima.size());
+    for (size_t i = 0; i < minima.size(); i++) {
+        const int64_t v = minima[i];
+        const int64_t x = v % nx, y = (v / nx) % ny, z = v / (nx * ny);
+        nodes[i].cell = 2 * x + Mx * (2 * y + My * 2 * z);
+        nodes[i].value = vals[i];
+    }
+    std::vector<SaddleArc> arcs(n_sad);
+    for (uint64_t i = 0; i < n_sad; i++) {
+        const float fa = vals[minima.size() + 2 * i], fb = vals[minima.size() + 2 * i + 1];
+        // value of the saddle's highest vertex (value, then vertex id)
+        const bool a_high = fa > fb || (!(fb > fa) && sad_a[i] > sad_b[i]);
+        arcs[i] = SaddleArc{tr[2 * n_sad + i], tr[2 * i], tr[2 * i + 1], a_high ? fa : fb};
+    }
+    const std::vector<int> rep = simplify_minima(nodes, arcs, presimp);
+    // seeds: every minimum, labelled by its representative's vertex
+    std::vector<int32_t> seed_v(minima.size()), seed_l(minima.size());
+    std::vector<char> is_rep(minima.size(), 0);
+    for (size_t i = 0; i < minima.size(); i++) {
+        seed_v[i] = (int32_t)minima[i];
+        seed_l[i] = (int32_t)minima[rep[i]];
+        is_rep[rep[i]] = 1;
+    }
+    st.minima = (int64_t)minima.size();
+    st.saddles = (int64_t)n_sad;
+    for (char c : is_rep) st.regions += c;
+
+    // ---- certain regions ----
+    CHECK_HIP(launch_fill_i32(b.label, V, -1, s));
+    CHECK_HIP(launch_fill_u8(b.seedmask, V, 0, s));
+    CHECK_HIP(launch_grad_field(g, d_values, b.gradf, s));
+    if (!seed_v.empty()) {
+        CHECK_HIP(ctx->host_io.ensure(2 * sizeof(int32_t) * seed_v.size() + 64));
+        int32_t* d_sv = ctx->host_io.as<int32_t>();
+        int32_t* d_sl = d_sv + seed_v.size();
+        CHECK_HIP(hipMemcpyAsync(d_sv, seed_v.data(), sizeof(int32_t) * seed_v.size(), hipMemcpyHostToDevice, s));
+        CHECK_HIP(hipMemcpyAsync(d_sl, seed_l.data(), sizeof(int32_t) * seed_l.size(), hipMemcpyHostToDevice, s));
+        CHECK_HIP(launch_seed_scatter(d_sv, d_sl, (int)seed_v.size(), b, s));
+    }
+    int sweeps = 0;
+    rc = iterate_until_stable(ctx, 8, 1 << 16, &sweeps,
+                              [&](uint32_t* flag) { return launch_certain_sweep(g, d_values, b.label, flag, s); });
+    if (rc) return rc;
+    st.certain_sweeps = sweeps;
+    CHECK_HIP(hipEventRecord(ev[2], s));
+    auto finish_early = [&]() -> int {
+        if (labels_out)
+            CHECK_HIP(hipMemcpyAsync(labels_out, b.label, sizeof(int32_t) * V,
+                                     dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+        CHECK_HIP(hipStreamSynchronize(s));
+        ctx->last_labels = st;
+        if (stats) *stats = st;
+        return MSCEER_OK;
+    };
+    if (flags & MSCEER_STOP_REGIONS) return finish_early();
+
+    // ---- integration ----
+    CHECK_HIP(launch_dest_init(g, b, s));
+    if (d_mask) {
+        // vertices outside the mask: combinatorial label, treated as assigned
+        CHECK_HIP(launch_combinatorial_labels(g, d_grad, b.label, b.label2, nullptr, s));
+        CHECK_HIP(launch_apply_mask(g, b, d_mask, s));
+    }
+    const int max_dim = (int)std::max(nx, std::max(ny, nz));
+    int top = 1;
+    while (top * 2 <= 8 && top * 2 < max_dim) top *= 2;
+    for (int stride = top; stride >= 1; stride /= 2)
+        CHECK_HIP(launch_pass1(g, b, stride, stride == top, grad_thr, err_thr, max_iters, s));
+    // pass 2 + pass 3 rounds
+    uint32_t* fronts[2] = {ctx->front0.as<uint32_t>(), ctx->front1.as<uint32_t>()};
+    auto counts = [&](int k) { return ctl + kCtlRound + (k % 3) * kSubs; };
+    auto front = [&](int k) { return MList{fronts[k % 2], counts(k), front_sub}; };
+    CHECK_HIP(launch_zero(counts(0), kSubs, s));
+    CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
+    CHECK_HIP(launch_pass2(g, b, front(0), ctl + kCtlOverflow, s));
+    int k = 1;
+    CHECK_HIP(launch_zero(counts(1), kSubs, s));
+    int rounds = 0;
+    for (int guard = 0; guard < 100000; guard++) {
+        constexpr int kGroup = 4;
+        for (int j = 0; j < kGroup; j++) {
+            CHECK_HIP(launch_pass3_round(g, b, front(k - 1), front(k), counts(k + 1), ctl + kCtlOverflow,
+                                         (uint32_t)k, grad_thr, err_thr, max_iters, s));
+            k++;
+            rounds++;
+        }
+        rc = read_ctl(ctx, 0, kCtlWords);
+        if (rc) return rc;
+        if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;
+        if (list_total(ctx->h_ctl + kCtlRound + ((k - 1) % 3) * kSubs, front_sub) == 0) break;
+    }
+    st.pass3_rounds = rounds;
+    st.traced_fp32 = ctx->h_ctl[kCtlIntStats + kStatTraced1];
+    st.over_budget_fp32 = ctx->h_ctl[kCtlIntStats + kStatOver1];
+    st.reintegrated_fp64 = ctx->h_ctl[kCtlIntStats + kStatTraced3];
+    st.over_budget_fp64 = ctx->h_ctl[kCtlIntStats + kStatOver3];
+    st.integrated_vertices = d_mask ? (int64_t)ctx->h_ctl[kCtlIntStats + kStatMasked] : V;
+    CHECK_HIP(hipEventRecord(ev[3], s));
+    if (flags & MSCEER_STOP_INTEGRATED) return finish_early();
+
+    // ---- cleanup of isolated components ----
+    CHECK_HIP(launch_cc_init(g, b, s));
+    int it = 0;
+    rc = iterate_until_stable(ctx, 4, 1 << 16, &it, [&](uint32_t* flag) { return launch_cc_hook(g, b, flag, s); });
+    if (rc) return rc;
+    CHECK_HIP(launch_cc_roots(g, b, s));
+    rc = iterate_until_stable(ctx, 4, 1 << 16, &it, [&](uint32_t* flag) { return launch_cc_resolve(g, b, flag, s); });
+    if (rc) return rc;
+    CHECK_HIP(launch_cc_apply(g, b, s));
+    CHECK_HIP(hipEventRecord(ev[4], s));
+
+    // ---- restriction labels, conforming gradient, ldir ----
+    if (grad_out) {
+        int8_t* restr = ctx->restriction.as<int8_t>();
+        CHECK_HIP(launch_min_boundary(g, b.label, restr, s));
+        CHECK_HIP(hipEventRecord(ev[5], s));
+        CHECK_HIP(launch_zero(ctl + kCtlList1, 2 * kSubs, s));
+        CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
+        LstarBuffers cb{d_values, d_grad, restr, &list1, &list2, ctl + kCtlStats, ctl + kCtlOverflow};
+        CHECK_HIP(launch_lstar(d, cb, s));
+        CHECK_HIP(hipEventRecord(ev[6], s));
+        rc = read_ctl(ctx, kCtlOverflow, 1);
+        if (rc) return rc;
+        int ldir_rounds = 0;
+        rc = run_ldir(ctx, d, d_grad, list1, list2, ctx->h_ctl[kCtlOverflow] == 0, front_sub, &ldir_rounds);
+        if (rc) return rc;
+        if (!dev) CHECK_HIP(hipMemcpyAsync(grad_out, d_grad, C, hipMemcpyDeviceToHost, s));
+    } else {
+        CHECK_HIP(hipEventRecord(ev[5], s));
+        CHECK_HIP(hipEventRecord(ev[6], s));
+    }
+    CHECK_HIP(hipEventRecord(ev[7], s));
+    if (labels_out) {
+        CHECK_HIP(hipMemcpyAsync(labels_out, b.label, sizeof(int32_t) * V,
+                                 dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    }
+    CHECK_HIP(hipStreamSynchronize(s));
+    st.ms_lstar = elapsed(ev[0], ev[1]) + elapsed(ev[5], ev[6]);
+    st.ms_regions = elapsed(ev[1], ev[2]);
+    st.ms_integrate = elapsed(ev[2], ev[3]);
+    st.ms_cleanup = elapsed(ev[3], ev[4]) + elapsed(ev[4], ev[5]);
+    st.ms_grad = elapsed(ev[5], ev[7]);
+    st.ms_total = elapsed(ev[0], ev[7]);
+    ctx->last_labels = st;
+    if (stats) *stats = st;
+    return MSCEER_OK;
+}

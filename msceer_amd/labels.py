@@ -1,0 +1,56 @@
+"""Host-side mirror of the reference's accurate-labelling interface.
+
+``ondemand_asc`` is the needASC2 branch of ondemandaccurate
+(ondemandaccurate/ondemandaccurate.cxx:728-1061): accurate ascending 3-manifold
+labels of every vertex and the conforming ``.grad``.  ``write_labels`` writes
+the int32-per-vertex label file of accurate3dmorselabeling
+(accurate3dmorselabeling.cxx:333-335, ``<file>._int_XxYxZ.raw``).  Work runs on
+the GPU through libmsceer_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+
+from . import native
+from .lstar import num_cells
+
+
+def ondemand_asc(values: np.ndarray, err_thr: float = 1e-3, grad_thr: float = 1e-6, max_iters: int = 4096,
+                 presimp: float = 0.0, mask: np.ndarray | None = None, want_grad: bool = True,
+                 ctx: native.Context | None = None, stop: str | None = None):
+    """(grad or None, labels (Z,Y,X) int32, stats dict) of a (Z,Y,X) float32 volume.
+
+    mask: optional per-vertex uint8; only nonzero vertices are integrated,
+    the others take the label of the minimum their discrete-gradient V-path
+    reaches.  stop: None, "regions" or "integrated" (earlier stage labels).
+    """
+    v = np.ascontiguousarray(values, dtype=np.float32)
+    Z, Y, X = v.shape
+    own = ctx is None
+    ctx = ctx or native.Context()
+    try:
+        labels = np.empty(X * Y * Z, np.int32)
+        grad = np.empty(num_cells(X, Y, Z), np.uint8) if want_grad and stop is None else None
+        m = None if mask is None else np.ascontiguousarray(mask, np.uint8).ravel()
+        st = native.LabelStats()
+        flags = {None: 0, "regions": native.STOP_REGIONS, "integrated": native.STOP_INTEGRATED}[stop]
+        rc = native.lib().msceer_ondemand_asc(
+            ctx.handle, v.ctypes.data_as(ctypes.c_void_p), X, Y, Z, err_thr, grad_thr, max_iters, presimp,
+            None if m is None else m.ctypes.data_as(ctypes.c_void_p),
+            None if grad is None else grad.ctypes.data_as(ctypes.c_void_p),
+            labels.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st), flags)
+        native.check(rc, "msceer_ondemand_asc")
+    finally:
+        if own:
+            ctx.close()
+    return grad, labels.reshape(Z, Y, X), st.as_dict()
+
+
+def label_file_name(base: str, X: int, Y: int, Z: int) -> str:
+    return f"{base}._int_{X}x{Y}x{Z}.raw"
+
+
+def write_labels(path: str, labels: np.ndarray) -> None:
+    np.ascontiguousarray(labels, dtype=np.int32).tofile(path)

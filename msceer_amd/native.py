@@ -14,6 +14,8 @@ LIB_PATH = os.path.join(PKG, "libmsceer_hip.so")
 DEVICE_PTRS = 0x1
 SKIP_LDIR = 0x2
 TIMING = 0x4
+STOP_REGIONS = 0x8
+STOP_INTEGRATED = 0x10
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -30,6 +32,8 @@ SIGNATURES = {
     "msceer_set_stream": (ctypes.c_int, [_P, _P]),
     "msceer_last_timing": (ctypes.c_int, [_P, _P]),
     "msceer_lstar_grad": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _U32]),
+    "msceer_ondemand_asc": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, ctypes.c_float, ctypes.c_float, _I32,
+                                           ctypes.c_float, _P, _P, _P, _P, _U32]),
 }
 
 
@@ -41,6 +45,30 @@ class Timing(ctypes.Structure):
         ("ldir_rounds", ctypes.c_int32),
         ("critical", ctypes.c_int64 * 4),
     ]
+
+
+class LabelStats(ctypes.Structure):
+    _fields_ = [
+        ("minima", ctypes.c_int64),
+        ("saddles", ctypes.c_int64),
+        ("regions", ctypes.c_int64),
+        ("traced_fp32", ctypes.c_int64),
+        ("over_budget_fp32", ctypes.c_int64),
+        ("reintegrated_fp64", ctypes.c_int64),
+        ("over_budget_fp64", ctypes.c_int64),
+        ("integrated_vertices", ctypes.c_int64),
+        ("certain_sweeps", ctypes.c_int32),
+        ("pass3_rounds", ctypes.c_int32),
+        ("ms_lstar", ctypes.c_float),
+        ("ms_regions", ctypes.c_float),
+        ("ms_integrate", ctypes.c_float),
+        ("ms_cleanup", ctypes.c_float),
+        ("ms_grad", ctypes.c_float),
+        ("ms_total", ctypes.c_float),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class MsceerError(RuntimeError):

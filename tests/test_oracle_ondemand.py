@@ -58,3 +58,14 @@ def test_conforming_vertices_cover_restricted_lower_stars():
     f, _, ref = load_ondemand("gauss_44x36x40")
     vs = oracle_lib.conforming_vertices(f, ref["restriction"])
     assert vs.size > 0 and (vs % 2 == 0).all()
+
+
+@pytest.mark.parametrize("name", ["labels_48", "smooth_34x33x32"])
+def test_constrained_pass_over_all_vertices_equals_conforming_recompute(name):
+    # the GPU recomputes every lower star under the restriction instead of
+    # only the affected ones; unaffected stars split exactly as in the
+    # steepest pass, so both give the reference .grad
+    f, _, ref = load_ondemand(name)
+    g = oracle_lib.lstar_pairs(f, ref["restriction"])
+    g = oracle_lib.flags(oracle_lib.ldir(g, f.shape), f.shape)
+    assert (g == ref["grad"]).all()
