@@ -128,6 +128,21 @@ int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t nx, int64_
                         float grad_thr, int32_t max_iters, float presimp, const uint8_t* integrate_mask,
                         uint8_t* grad_out, int32_t* labels_out, msceer_label_stats* stats, uint32_t flags);
 
+/* Stages of the accurate pipeline as separate calls (host or device pointers):
+ * IsolatedCCRegionRemoverNEW::ComputeOutput (gi_isolated_region_remover.h:155-192),
+ * in place on X*Y*Z int32 labels. */
+int msceer_clean_labels(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz,
+                        int32_t* labels_inout, uint32_t flags);
+/* VertexLabelingToBoundaryLabeling::ComputeMINBoundary (gi_labeling_to_bounary_labeling.h:48-116):
+ * int8 restriction label per cell (0 or 1) from X*Y*Z int32 labels. */
+int msceer_restriction_labels(msceer_ctx* ctx, const int32_t* labels, int64_t nx, int64_t ny, int64_t nz,
+                              int8_t* restriction_out, uint32_t flags);
+/* Constrained lower-star pass MyRobinsNoalloc<...,4,6>::ComputeLowerStar over the
+ * vertices of ondemandaccurate.cxx:1014-1052 (recomputing every lower star is
+ * equivalent), then RecordGrad (ldir + flags) unless MSCEER_SKIP_LDIR. */
+int msceer_constrained_lstar(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz,
+                             const int8_t* restriction, uint8_t* grad_inout, uint32_t flags);
+
 #ifdef __cplusplus
 }
 #endif

@@ -328,3 +328,138 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
     if (stats) *stats = st;
     return MSCEER_OK;
 }
+
+namespace {
+
+// Device copy of a host or device input (returns the device pointer).
+template <typename T>
+int stage_in(msceer_ctx* ctx, msceer::DevBuf& buf, const T* src, size_t n, bool dev, const T** out) {
+    if (dev) {
+        *out = src;
+        return MSCEER_OK;
+    }
+    CHECK_HIP(buf.ensure(sizeof(T) * n));
+    CHECK_HIP(hipMemcpyAsync(buf.ptr, src, sizeof(T) * n, hipMemcpyHostToDevice, ctx->stream));
+    *out = buf.as<T>();
+    return MSCEER_OK;
+}
+
+}  // namespace
+
+// IsolatedCCRegionRemoverNEW(func, labels).ComputeOutput()
+// (gi_isolated_region_remover.h:155-192) on a label array, in place.
+extern "C" int msceer_clean_labels(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz,
+                                   int32_t* labels_inout, uint32_t flags) {
+    if (!ctx || !values || !labels_inout) return MSCEER_EINVAL;
+    if (flags & ~MSCEER_DEVICE_PTRS) return MSCEER_EINVAL;
+    int rc = check_dims(nx, ny, nz);
+    if (rc) return rc;
+    if (nx * ny * nz >= (int64_t)1 << 31) return MSCEER_ETOOBIG;
+    CHECK_HIP(hipSetDevice(ctx->device));
+    const bool dev = flags & MSCEER_DEVICE_PTRS;
+    const VGrid g = make_vgrid(nx, ny, nz);
+    const int64_t V = g.verts();
+    hipStream_t s = ctx->stream;
+    const float* d_values = nullptr;
+    rc = stage_in(ctx, ctx->values, values, (size_t)V, dev, &d_values);
+    if (rc) return rc;
+    CHECK_HIP(ctx->label.ensure(sizeof(int32_t) * V));
+    CHECK_HIP(ctx->label2.ensure(sizeof(int32_t) * V));
+    CHECK_HIP(ctx->rep.ensure(sizeof(int32_t) * V));
+    CHECK_HIP(ctx->rep2.ensure(sizeof(int32_t) * V));
+    CHECK_HIP(hipMemcpyAsync(ctx->label.ptr, labels_inout, sizeof(int32_t) * V,
+                             dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    LabelBuffers b{};
+    b.values = d_values;
+    b.label = ctx->label.as<int32_t>();
+    b.label2 = ctx->label2.as<int32_t>();
+    b.rep = ctx->rep.as<int32_t>();
+    b.rep2 = ctx->rep2.as<int32_t>();
+    CHECK_HIP(launch_cc_init(g, b, s));
+    int it = 0;
+    rc = iterate_until_stable(ctx, 4, 1 << 16, &it, [&](uint32_t* flag) { return launch_cc_hook(g, b, flag, s); });
+    if (rc) return rc;
+    CHECK_HIP(launch_cc_roots(g, b, s));
+    rc = iterate_until_stable(ctx, 4, 1 << 16, &it, [&](uint32_t* flag) { return launch_cc_resolve(g, b, flag, s); });
+    if (rc) return rc;
+    CHECK_HIP(launch_cc_apply(g, b, s));
+    CHECK_HIP(hipMemcpyAsync(labels_inout, b.label, sizeof(int32_t) * V,
+                             dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    CHECK_HIP(hipStreamSynchronize(s));
+    return MSCEER_OK;
+}
+
+// VertexLabelingToBoundaryLabeling::ComputeMINBoundary (gi_labeling_to_bounary_labeling.h:48-116)
+// starting from an all-zero restriction (InitializeFirst).
+extern "C" int msceer_restriction_labels(msceer_ctx* ctx, const int32_t* labels, int64_t nx, int64_t ny, int64_t nz,
+                                         int8_t* restriction_out, uint32_t flags) {
+    if (!ctx || !labels || !restriction_out) return MSCEER_EINVAL;
+    if (flags & ~MSCEER_DEVICE_PTRS) return MSCEER_EINVAL;
+    int rc = check_dims(nx, ny, nz);
+    if (rc) return rc;
+    CHECK_HIP(hipSetDevice(ctx->device));
+    const bool dev = flags & MSCEER_DEVICE_PTRS;
+    const Dims d = make_dims(nx, ny, nz);
+    const VGrid g = make_vgrid(nx, ny, nz);
+    hipStream_t s = ctx->stream;
+    const int32_t* d_labels = nullptr;
+    rc = stage_in(ctx, ctx->label, labels, (size_t)d.verts, dev, &d_labels);
+    if (rc) return rc;
+    int8_t* d_r = restriction_out;
+    if (!dev) {
+        CHECK_HIP(ctx->restriction.ensure(d.cells));
+        d_r = ctx->restriction.as<int8_t>();
+    }
+    CHECK_HIP(launch_min_boundary(g, d_labels, d_r, s));
+    if (!dev) CHECK_HIP(hipMemcpyAsync(restriction_out, d_r, d.cells, hipMemcpyDeviceToHost, s));
+    CHECK_HIP(hipStreamSynchronize(s));
+    return MSCEER_OK;
+}
+
+// Lower stars recomputed under a restriction labelling (MyRobinsNoalloc<...,4,6>,
+// gi_modified_robins.h:1281-1343; ondemandaccurate.cxx:1014-1052) followed by
+// RecordGrad (ldir + flags, ondemandaccurate.cxx:693-699) unless MSCEER_SKIP_LDIR.
+// Every lower star is recomputed: a star without restricted cells splits by
+// boundary value alone, exactly as in the steepest pass.
+extern "C" int msceer_constrained_lstar(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz,
+                                        const int8_t* restriction, uint8_t* grad_inout, uint32_t flags) {
+    if (!ctx || !values || !restriction || !grad_inout) return MSCEER_EINVAL;
+    if (flags & ~(MSCEER_DEVICE_PTRS | MSCEER_SKIP_LDIR)) return MSCEER_EINVAL;
+    int rc = check_dims(nx, ny, nz);
+    if (rc) return rc;
+    CHECK_HIP(hipSetDevice(ctx->device));
+    const bool dev = flags & MSCEER_DEVICE_PTRS;
+    const Dims d = make_dims(nx, ny, nz);
+    hipStream_t s = ctx->stream;
+    const float* d_values = nullptr;
+    rc = stage_in(ctx, ctx->values, values, (size_t)d.verts, dev, &d_values);
+    if (rc) return rc;
+    const int8_t* d_r = nullptr;
+    rc = stage_in(ctx, ctx->restriction, restriction, (size_t)d.cells, dev, &d_r);
+    if (rc) return rc;
+    uint8_t* d_grad = grad_inout;
+    if (!dev) {
+        CHECK_HIP(ctx->grad.ensure(d.cells));
+        d_grad = ctx->grad.as<uint8_t>();
+    }
+    uint32_t list_sub = 0, front_sub = 0;
+    rc = ensure_lists(ctx, d, &list_sub, &front_sub);
+    if (rc) return rc;
+    uint32_t* ctl = ctl_dev(ctx);
+    uint32_t* l1 = ctx->lists.as<uint32_t>();
+    const MList list1{l1, ctl + kCtlList1, list_sub};
+    const MList list2{l1 + (size_t)list_sub * kSubs, ctl + kCtlList2, list_sub};
+    CHECK_HIP(launch_zero(ctl, kCtlWords, s));
+    LstarBuffers cb{d_values, d_grad, d_r, &list1, &list2, ctl + kCtlStats, ctl + kCtlOverflow};
+    CHECK_HIP(launch_lstar(d, cb, s));
+    if (!(flags & MSCEER_SKIP_LDIR)) {
+        rc = read_ctl(ctx, kCtlOverflow, 1);
+        if (rc) return rc;
+        int rounds = 0;
+        rc = run_ldir(ctx, d, d_grad, list1, list2, ctx->h_ctl[kCtlOverflow] == 0, front_sub, &rounds);
+        if (rc) return rc;
+    }
+    if (!dev) CHECK_HIP(hipMemcpyAsync(grad_inout, d_grad, d.cells, hipMemcpyDeviceToHost, s));
+    CHECK_HIP(hipStreamSynchronize(s));
+    return MSCEER_OK;
+}

@@ -54,3 +54,58 @@ def label_file_name(base: str, X: int, Y: int, Z: int) -> str:
 
 def write_labels(path: str, labels: np.ndarray) -> None:
     np.ascontiguousarray(labels, dtype=np.int32).tofile(path)
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def clean_labels(values: np.ndarray, labels: np.ndarray, ctx: native.Context | None = None) -> np.ndarray:
+    """IsolatedCCRegionRemoverNEW::ComputeOutput on a copy of `labels`."""
+    v = np.ascontiguousarray(values, np.float32)
+    Z, Y, X = v.shape
+    lab = np.ascontiguousarray(labels, np.int32).copy()
+    own = ctx is None
+    ctx = ctx or native.Context()
+    try:
+        native.check(native.lib().msceer_clean_labels(ctx.handle, _ptr(v), X, Y, Z, _ptr(lab), 0),
+                     "msceer_clean_labels")
+    finally:
+        if own:
+            ctx.close()
+    return lab.reshape(Z, Y, X)
+
+
+def restriction_labels(labels: np.ndarray, ctx: native.Context | None = None) -> np.ndarray:
+    """ComputeMINBoundary: int8 per cell, shape (2Z-1, 2Y-1, 2X-1)."""
+    lab = np.ascontiguousarray(labels, np.int32)
+    Z, Y, X = lab.shape
+    out = np.empty(num_cells(X, Y, Z), np.int8)
+    own = ctx is None
+    ctx = ctx or native.Context()
+    try:
+        native.check(native.lib().msceer_restriction_labels(ctx.handle, _ptr(lab), X, Y, Z, _ptr(out), 0),
+                     "msceer_restriction_labels")
+    finally:
+        if own:
+            ctx.close()
+    return out.reshape(2 * Z - 1, 2 * Y - 1, 2 * X - 1)
+
+
+def constrained_grad(values: np.ndarray, restriction: np.ndarray, ctx: native.Context | None = None,
+                     skip_ldir: bool = False) -> np.ndarray:
+    """Lower stars under a restriction labelling, then ldir and flags."""
+    v = np.ascontiguousarray(values, np.float32)
+    Z, Y, X = v.shape
+    r = np.ascontiguousarray(restriction, np.int8)
+    out = np.empty(num_cells(X, Y, Z), np.uint8)
+    own = ctx is None
+    ctx = ctx or native.Context()
+    try:
+        native.check(native.lib().msceer_constrained_lstar(ctx.handle, _ptr(v), X, Y, Z, _ptr(r), _ptr(out),
+                                                           native.SKIP_LDIR if skip_ldir else 0),
+                     "msceer_constrained_lstar")
+    finally:
+        if own:
+            ctx.close()
+    return out.reshape(2 * Z - 1, 2 * Y - 1, 2 * X - 1)

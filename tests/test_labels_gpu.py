@@ -77,3 +77,15 @@ def test_mask_outside_takes_combinatorial_labels(ctx):
     mism = float((lab.ravel() != ref["cleaned"]).mean())
     _say(f"  combinatorial-only mismatch vs reference {mism:.5f}")
     assert mism < 0.2
+
+
+@pytest.mark.parametrize("name", ["labels_48", "smooth_34x33x32", "smooth_presimp_40"])
+def test_stage_entry_points_match_oracle(ctx, name):
+    f, (err, gthr, iters, presimp), ref = load_ondemand(name)
+    # cleanup as a function of its input: the reference's own integrated labels
+    cleaned = labels.clean_labels(f, ref["integrated"].reshape(f.shape), ctx=ctx).ravel()
+    assert (cleaned == ref["cleaned"]).all(), int((cleaned != ref["cleaned"]).sum())
+    r = labels.restriction_labels(ref["cleaned"].reshape(f.shape), ctx=ctx).ravel()
+    assert (r == ref["restriction"]).all()
+    g = labels.constrained_grad(f, ref["restriction"], ctx=ctx).ravel()
+    assert (g == ref["grad"]).all(), int((g != ref["grad"]).sum())
