@@ -35,10 +35,21 @@ int read_ctl(msceer_ctx* ctx, int first, int n) {
     return MSCEER_OK;
 }
 
-uint64_t list_total(const uint32_t* counts, uint32_t sub_cap) {
-    uint64_t t = 0;
-    for (int i = 0; i < kSubs; i++) t += std::min(counts[i], sub_cap);
+uint64_t list_total(const uint32_t* counts, const MList& l) {
+    uint64_t t = std::min(counts[kSubs], l.spill_cap);
+    for (int i = 0; i < kSubs; i++) t += std::min(counts[i], l.sub_cap);
     return t;
+}
+
+MList crit_list(msceer_ctx* ctx, const ListCaps& c, int which) {
+    const size_t per = (size_t)c.list_sub * kSubs + c.list_spill;
+    return MList{ctx->lists.as<uint32_t>() + which * per, ctl_dev(ctx) + (which ? kCtlList2 : kCtlList1), c.list_sub,
+                 c.list_spill};
+}
+
+MList front_list(msceer_ctx* ctx, const ListCaps& c, int k) {
+    uint32_t* buf = (k % 2 ? ctx->front1 : ctx->front0).as<uint32_t>();
+    return MList{buf, ctl_dev(ctx) + kCtlRound + (k % 3) * kCountWords, c.front_sub, c.front_spill};
 }
 
 // One ldir pass.  Rounds are enqueued in groups without host synchronisation
@@ -46,39 +57,37 @@ uint64_t list_total(const uint32_t* counts, uint32_t sub_cap) {
 // whether the frontier emptied.  A sweep rebuilds the frontier after any
 // overflow of the start list or of a frontier.
 int ldir_pass(msceer_ctx* ctx, const Dims& d, uint8_t* grad, uint32_t item_dim, uint32_t value,
-              const MList& start, bool start_valid, uint32_t front_sub_cap, int* rounds) {
+              const MList& start, bool start_valid, const ListCaps& caps, int* rounds) {
     uint32_t* ctl = ctl_dev(ctx);
     uint32_t* tail = ctl + kCtlTail;
     uint32_t* overflow = ctl + kCtlOverflow;
-    uint32_t* fronts[2] = {static_cast<uint32_t*>(ctx->front0.ptr), static_cast<uint32_t*>(ctx->front1.ptr)};
-    auto counts = [&](int k) { return ctl + kCtlRound + (k % 3) * kSubs; };
-    auto front = [&](int k) { return MList{fronts[k % 2], counts(k), front_sub_cap}; };
+    auto front = [&](int k) { return front_list(ctx, caps, k); };
+    auto host_total = [&](int k) { return list_total(ctx->h_ctl + kCtlRound + (k % 3) * kCountWords, front(k)); };
     constexpr int kGroup = 8;
     MList in = start;
     int k = 0;
     bool need_sweep = !start_valid;
-    CHECK_HIP(launch_zero(counts(0), kSubs, ctx->stream));
+    CHECK_HIP(launch_zero(front(0).counts, kCountWords, ctx->stream));
     CHECK_HIP(launch_zero(overflow, 1, ctx->stream));
     for (int guard = 0; guard < 100000; guard++) {
         if (need_sweep) {
-            CHECK_HIP(launch_zero(counts(k), kSubs, ctx->stream));
+            CHECK_HIP(launch_zero(front(k).counts, kCountWords, ctx->stream));
             CHECK_HIP(launch_zero(overflow, 1, ctx->stream));
             CHECK_HIP(launch_ldir_sweep(d, grad, tail, item_dim, value, front(k), overflow, ctx->stream));
             int rc = read_ctl(ctx, 0, kCtlWords);
             if (rc) return rc;
-            const uint32_t* hc = ctx->h_ctl + kCtlRound + (k % 3) * kSubs;
             need_sweep = ctx->h_ctl[kCtlOverflow] != 0;
             (*rounds)++;
-            const uint64_t n = list_total(hc, front_sub_cap);
+            const uint64_t n = host_total(k);
             if (n == 0 && !need_sweep) return MSCEER_OK;
             CHECK_HIP(launch_zero(overflow, 1, ctx->stream));
             in = front(k);
             k++;
-            CHECK_HIP(launch_zero(counts(k), kSubs, ctx->stream));
+            CHECK_HIP(launch_zero(front(k).counts, kCountWords, ctx->stream));
         }
         for (int j = 0; j < kGroup; j++) {
             const MList out = front(k);
-            CHECK_HIP(launch_ldir_round(d, grad, tail, in, out, counts(k + 1), overflow, value, ctx->stream));
+            CHECK_HIP(launch_ldir_round(d, grad, tail, in, out, front(k + 1).counts, overflow, value, ctx->stream));
             (*rounds)++;
             in = out;
             k++;
@@ -89,30 +98,33 @@ int ldir_pass(msceer_ctx* ctx, const Dims& d, uint8_t* grad, uint32_t item_dim, 
             need_sweep = true;
             continue;
         }
-        const uint32_t* hc = ctx->h_ctl + kCtlRound + ((k - 1) % 3) * kSubs;
-        if (list_total(hc, front_sub_cap) == 0) return MSCEER_OK;
+        if (host_total(k - 1) == 0) return MSCEER_OK;
     }
     return MSCEER_EHIP;
 }
 
-int ensure_lists(msceer_ctx* ctx, const Dims& d, uint32_t* list_sub, uint32_t* front_sub) {
+// Critical lists: a quarter of the vertices per sub-list set, spill for as
+// many again as there are vertices; frontiers: the vertex count in sub-lists
+// and as much again in spill.  Entries are cell ids (< 2^32).
+int ensure_lists(msceer_ctx* ctx, const Dims& d, ListCaps* caps) {
     const uint64_t list_cap = std::min<uint64_t>(kMaxListCap, std::max<uint64_t>(kMinListCap, d.verts / 4));
     const uint64_t front_cap = std::min<uint64_t>(kMaxFrontCap, std::max<uint64_t>(kMinListCap, d.verts));
-    *list_sub = (uint32_t)((list_cap + kSubs - 1) / kSubs);
-    *front_sub = (uint32_t)((front_cap + kSubs - 1) / kSubs);
-    CHECK_HIP(ctx->lists.ensure(sizeof(uint32_t) * 2 * (size_t)*list_sub * kSubs));
-    CHECK_HIP(ctx->front0.ensure(sizeof(uint32_t) * (size_t)*front_sub * kSubs));
-    CHECK_HIP(ctx->front1.ensure(sizeof(uint32_t) * (size_t)*front_sub * kSubs));
+    caps->list_sub = (uint32_t)((list_cap + kSubs - 1) / kSubs);
+    caps->front_sub = (uint32_t)((front_cap + kSubs - 1) / kSubs);
+    caps->list_spill = (uint32_t)std::min<uint64_t>(0xFFFFFFFFull, std::max<uint64_t>(kMinListCap, d.verts));
+    caps->front_spill = caps->list_spill;
+    CHECK_HIP(ctx->lists.ensure(sizeof(uint32_t) * 2 * ((size_t)caps->list_sub * kSubs + caps->list_spill)));
+    CHECK_HIP(ctx->front0.ensure(sizeof(uint32_t) * ((size_t)caps->front_sub * kSubs + caps->front_spill)));
+    CHECK_HIP(ctx->front1.ensure(sizeof(uint32_t) * ((size_t)caps->front_sub * kSubs + caps->front_spill)));
     return MSCEER_OK;
 }
 
-int run_ldir(msceer_ctx* ctx, const Dims& d, uint8_t* grad, const MList& list1, const MList& list2, bool lists_ok,
-             uint32_t front_sub, int* rounds) {
+int run_ldir(msceer_ctx* ctx, const Dims& d, uint8_t* grad, const ListCaps& caps, bool lists_ok, int* rounds) {
     uint32_t* tail = ctl_dev(ctx) + kCtlTail;
     CHECK_HIP(launch_ldir_tail(d, grad, tail, false, ctx->stream));
-    int rc = ldir_pass(ctx, d, grad, 1, 2, list1, lists_ok, front_sub, rounds);
+    int rc = ldir_pass(ctx, d, grad, 1, 2, crit_list(ctx, caps, 0), lists_ok, caps, rounds);
     if (rc) return rc;
-    rc = ldir_pass(ctx, d, grad, 2, 1, list2, lists_ok, front_sub, rounds);
+    rc = ldir_pass(ctx, d, grad, 2, 1, crit_list(ctx, caps, 1), lists_ok, caps, rounds);
     if (rc) return rc;
     CHECK_HIP(launch_ldir_tail(d, grad, tail, true, ctx->stream));
     return MSCEER_OK;
@@ -215,13 +227,11 @@ int msceer_lstar_grad(msceer_ctx* ctx, const float* values, int64_t nx, int64_t 
     }
     // critical-cell lists and frontiers: sized to the volume, capped; any
     // overflow falls back to a sweep
-    uint32_t list_sub = 0, front_sub = 0;
-    rc = ensure_lists(ctx, d, &list_sub, &front_sub);
+    ListCaps caps{};
+    rc = ensure_lists(ctx, d, &caps);
     if (rc) return rc;
     uint32_t* ctl = ctl_dev(ctx);
-    uint32_t* l1 = static_cast<uint32_t*>(ctx->lists.ptr);
-    const MList list1{l1, ctl + kCtlList1, list_sub};
-    const MList list2{l1 + (size_t)list_sub * kSubs, ctl + kCtlList2, list_sub};
+    const MList list1 = crit_list(ctx, caps, 0), list2 = crit_list(ctx, caps, 1);
 
     if (timing) CHECK_HIP(hipEventRecord(ctx->ev[0], s));
     CHECK_HIP(launch_zero(ctl, kCtlWords, s));
@@ -239,7 +249,7 @@ int msceer_lstar_grad(msceer_ctx* ctx, const float* values, int64_t nx, int64_t 
     int rounds = 0;
     if (timing) CHECK_HIP(hipEventRecord(ctx->ev[3], s));
     if (!(flags & MSCEER_SKIP_LDIR)) {
-        rc = run_ldir(ctx, d, d_grad, list1, list2, lists_ok, front_sub, &rounds);
+        rc = run_ldir(ctx, d, d_grad, caps, lists_ok, &rounds);
         if (rc) return rc;
     }
     if (timing) CHECK_HIP(hipEventRecord(ctx->ev[4], s));

@@ -36,19 +36,25 @@ constexpr uint32_t kMaxListCap = 1u << 26;   // entries per critical list
 constexpr uint32_t kMaxFrontCap = 1u << 27;  // entries per frontier buffer
 
 // Control block (uint32 words, device; mirrored to pinned host memory):
-//   sub-list counts of the two critical lists and the three rotating
-//   frontier count arrays, critical statistics, flags.
+//   count arrays (kCountWords each) of the two critical lists and of the
+//   three rotating frontiers, critical statistics, flags.
 enum Ctl : int {
     kCtlList1 = 0,
-    kCtlList2 = kSubs,
-    kCtlRound = 2 * kSubs,       // three arrays of kSubs
-    kCtlStats = 5 * kSubs,       // kSubs * 4
-    kCtlOverflow = 9 * kSubs,    // a list or frontier ran out of room
-    kCtlTail = 9 * kSubs + 1,    // shadow of the grad array's last partial word
-    kCtlFlag = 9 * kSubs + 2,    // change flag of iterative sweeps
-    kCtlCount = 9 * kSubs + 3,   // a kernel-reported count
-    kCtlIntStats = 9 * kSubs + 8,  // eight integrator counters
-    kCtlWords = 9 * kSubs + 16
+    kCtlList2 = kCountWords,
+    kCtlRound = 2 * kCountWords,   // three count arrays
+    kCtlStats = 5 * kCountWords,   // kSubs * 4
+    kCtlOverflow = 5 * kCountWords + 4 * kSubs,  // a list or frontier ran out of room
+    kCtlTail = kCtlOverflow + 1,   // shadow of the grad array's last partial word
+    kCtlFlag = kCtlOverflow + 2,   // change flag of iterative sweeps
+    kCtlCount = kCtlOverflow + 3,  // a kernel-reported count
+    kCtlIntStats = kCtlOverflow + 8,  // eight integrator counters
+    kCtlWords = kCtlOverflow + 16
+};
+
+// Capacities of the critical lists and the frontiers of one volume.
+struct ListCaps {
+    uint32_t list_sub, list_spill;
+    uint32_t front_sub, front_spill;
 };
 
 int hip_fail(hipError_t e);
@@ -73,14 +79,18 @@ namespace msceer {
 
 uint32_t* ctl_dev(msceer_ctx* ctx);
 int read_ctl(msceer_ctx* ctx, int first, int n);
-uint64_t list_total(const uint32_t* counts, uint32_t sub_cap);
+// Entries held by a list whose count array (host mirror) is `counts`.
+uint64_t list_total(const uint32_t* counts, const MList& l);
+// Critical lists and frontiers sized for a volume.
+int ensure_lists(msceer_ctx* ctx, const Dims& d, ListCaps* caps);
+// Critical 1-cell (which = 0) or 2-cell (which = 1) list.
+MList crit_list(msceer_ctx* ctx, const ListCaps& c, int which);
+// Frontier k of a chain of rounds: buffer k % 2, count array k % 3.
+MList front_list(msceer_ctx* ctx, const ListCaps& c, int k);
 int ldir_pass(msceer_ctx* ctx, const Dims& d, uint8_t* grad, uint32_t item_dim, uint32_t value,
-              const MList& start, bool start_valid, uint32_t front_sub_cap, int* rounds);
-// Critical lists and frontiers sized for a volume (sub-list capacities out).
-int ensure_lists(msceer_ctx* ctx, const Dims& d, uint32_t* list_sub, uint32_t* front_sub);
-// ldir passes + tail shadow on a .grad whose lstar pass filled list1/list2.
-int run_ldir(msceer_ctx* ctx, const Dims& d, uint8_t* grad, const MList& list1, const MList& list2, bool lists_ok,
-             uint32_t front_sub, int* rounds);
+              const MList& start, bool start_valid, const ListCaps& caps, int* rounds);
+// ldir passes + tail shadow on a .grad whose lstar pass filled the critical lists.
+int run_ldir(msceer_ctx* ctx, const Dims& d, uint8_t* grad, const ListCaps& caps, bool lists_ok, int* rounds);
 
 #define CHECK_HIP(expr)                                        \
     do {                                                       \

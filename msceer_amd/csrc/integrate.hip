@@ -318,9 +318,7 @@ struct Outbox {
         if (slot < (uint32_t)kStage) {
             stage[slot] = item;
         } else {
-            const uint32_t gs = mlist_reserve(out, sub, 1u);
-            if (gs < out.sub_cap) out.items[(size_t)sub * out.sub_cap + gs] = item;
-            else atomicOr(overflow, 1u);
+            if (!mlist_push(out, sub, item)) atomicOr(overflow, 1u);
         }
     }
 };
@@ -340,7 +338,8 @@ __device__ __forceinline__ int all6(const VGrid& g, int64_t i, int64_t nb[6]) {
 __global__ void __launch_bounds__(kThreads)
 pass2_kernel(VGrid g, LabelBuffers b, MList out, uint32_t* __restrict__ overflow) {
     __shared__ uint32_t stage[kStage];
-    __shared__ uint32_t s_n, s_base;
+    __shared__ uint32_t s_n;
+    __shared__ MSlot s_base;
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
     const Outbox box{stage, &s_n, out, (int)(blockIdx.x % kSubs), overflow};
@@ -369,9 +368,11 @@ __global__ void __launch_bounds__(kThreads)
 pass3_kernel(GradSrc<double> src, VGrid g, LabelBuffers b, MList in, MList out, uint32_t* __restrict__ clear,
              uint32_t* __restrict__ overflow, uint32_t round, double gthr, double ethr, int max_iters) {
     __shared__ uint32_t stage[kStage];
-    __shared__ uint32_t prefix[kSubs + 1];
-    __shared__ uint32_t s_n, s_base;
-    if (blockIdx.x == 0 && threadIdx.x < (uint32_t)kSubs) clear[threadIdx.x] = 0u;
+    __shared__ uint32_t prefix[kSubs + 2];
+    __shared__ uint32_t s_n;
+    __shared__ MSlot s_base;
+    if (blockIdx.x == 0)
+        for (uint32_t j = threadIdx.x; j < (uint32_t)kCountWords; j += kThreads) clear[j] = 0u;
     if (threadIdx.x == 0) s_n = 0;
     const uint32_t total = mlist_prefix(in, prefix);
     const Outbox box{stage, &s_n, out, (int)(blockIdx.x % kSubs), overflow};
@@ -453,44 +454,44 @@ cc_hook_kernel(const float* __restrict__ f, VGrid g, LabelBuffers b, uint32_t* _
     if (any) atomicOr(changed, 1u);
 }
 
-// For each component root: the earliest All6 neighbour before it in a
-// region with another label; rep2[root] = that neighbour's root (else root).
+// Component roots in a compact list (any order): root vertex, its label and
+// the roots of its All6 neighbours that come before it, earliest first (-1
+// padded).  The merge of roots into neighbouring regions is sequential in
+// root order (gi_isolated_region_remover.h:172-186: a root sees the merges of
+// every earlier root), so it runs on the host over this list.
 __global__ void __launch_bounds__(kThreads)
-cc_roots_kernel(const float* __restrict__ f, VGrid g, LabelBuffers b) {
+cc_collect_kernel(const float* __restrict__ f, VGrid g, LabelBuffers b, int32_t* __restrict__ out, uint32_t cap,
+                  uint32_t* __restrict__ count) {
     const int64_t n = g.verts();
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-        if (b.rep[i] != (int32_t)i) {
-            b.rep2[i] = -1;
-            continue;
-        }
+        if (b.rep[i] != (int32_t)i) continue;
         int64_t nb[6];
         const int k = all6(g, i, nb);
-        int64_t earliest = i;
-        const int32_t l = b.label2[i];
+        int64_t lower[6];
+        int m = 0;
         for (int j = 0; j < k; j++) {
             if (!before(f, nb[j], i)) continue;
-            if (b.label2[nb[j]] == l) continue;
-            if (before(f, nb[j], earliest)) earliest = nb[j];
+            int p = m++;
+            while (p > 0 && before(f, nb[j], lower[p - 1])) {
+                lower[p] = lower[p - 1];
+                p--;
+            }
+            lower[p] = nb[j];
         }
-        b.rep2[i] = earliest == i ? (int32_t)i : b.rep[earliest];
+        const uint32_t slot = atomicAdd(count, 1u);
+        if (slot >= cap) continue;
+        int32_t* o = out + (size_t)slot * kCollectStride;
+        o[0] = (int32_t)i;
+        o[1] = b.label2[i];
+        for (int j = 0; j < 6; j++) o[2 + j] = j < m ? b.rep[lower[j]] : -1;
     }
 }
 
-// Resolve merge chains among roots (targets are strictly earlier: no cycles).
+// rep2[root] = the root it finally joins, from the host's merge.
 __global__ void __launch_bounds__(kThreads)
-cc_resolve_kernel(VGrid g, LabelBuffers b, uint32_t* __restrict__ changed) {
-    const int64_t n = g.verts();
-    bool any = false;
-    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-        const int32_t t = b.rep2[i];
-        if (t < 0) continue;
-        const int32_t tt = b.rep2[t];
-        if (tt != t) {
-            b.rep2[i] = tt;
-            any = true;
-        }
-    }
-    if (any) atomicOr(changed, 1u);
+cc_scatter_kernel(const int32_t* __restrict__ pairs, uint32_t n, int32_t* __restrict__ rep2) {
+    for (uint32_t k = blockIdx.x * kThreads + threadIdx.x; k < n; k += gridDim.x * kThreads)
+        rep2[pairs[2 * k]] = pairs[2 * k + 1];
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -576,7 +577,7 @@ __device__ int64_t trace_down_vertex(const CellG& cg, const uint8_t* grad, int64
 __global__ void __launch_bounds__(kThreads)
 saddle_trace_kernel(VGrid g, const uint8_t* __restrict__ grad, MList list, int64_t* __restrict__ out,
                     int64_t* __restrict__ saddle_cell, uint32_t n_max, uint32_t* __restrict__ count) {
-    __shared__ uint32_t prefix[kSubs + 1];
+    __shared__ uint32_t prefix[kSubs + 2];
     const uint32_t total = mlist_prefix(list, prefix);
     if (blockIdx.x == 0 && threadIdx.x == 0) *count = total;
     const CellG cg{2 * (int64_t)g.X - 1, 2 * (int64_t)g.Y - 1, (2 * (int64_t)g.X - 1) * (2 * (int64_t)g.Y - 1)};
@@ -595,7 +596,8 @@ saddle_trace_kernel(VGrid g, const uint8_t* __restrict__ grad, MList list, int64
 __global__ void __launch_bounds__(kThreads)
 collect_minima_kernel(VGrid g, const uint8_t* __restrict__ grad, MList out, uint32_t* __restrict__ overflow) {
     __shared__ uint32_t stage[kStage];
-    __shared__ uint32_t s_n, s_base;
+    __shared__ uint32_t s_n;
+    __shared__ MSlot s_base;
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
     const Outbox box{stage, &s_n, out, (int)(blockIdx.x % kSubs), overflow};
@@ -728,12 +730,15 @@ hipError_t launch_cc_hook(const VGrid& g, const LabelBuffers& b, uint32_t* chang
     hipLaunchKernelGGL(cc_hook_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, b.values, g, b, changed);
     return hipGetLastError();
 }
-hipError_t launch_cc_roots(const VGrid& g, const LabelBuffers& b, hipStream_t s) {
-    hipLaunchKernelGGL(cc_roots_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, b.values, g, b);
+hipError_t launch_cc_collect(const VGrid& g, const LabelBuffers& b, int32_t* out, uint32_t cap, uint32_t* count,
+                             hipStream_t s) {
+    hipLaunchKernelGGL(cc_collect_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, b.values, g, b, out, cap,
+                       count);
     return hipGetLastError();
 }
-hipError_t launch_cc_resolve(const VGrid& g, const LabelBuffers& b, uint32_t* changed, hipStream_t s) {
-    hipLaunchKernelGGL(cc_resolve_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, g, b, changed);
+hipError_t launch_cc_scatter(const int32_t* pairs, uint32_t n, int32_t* rep2, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(cc_scatter_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, pairs, n, rep2);
     return hipGetLastError();
 }
 hipError_t launch_cc_apply(const VGrid& g, const LabelBuffers& b, hipStream_t s) {

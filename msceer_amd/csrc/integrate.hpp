@@ -71,8 +71,11 @@ hipError_t launch_pass3_round(const VGrid& g, const LabelBuffers& b, const MList
 // Cleanup of isolated components.
 hipError_t launch_cc_init(const VGrid& g, const LabelBuffers& b, hipStream_t s);
 hipError_t launch_cc_hook(const VGrid& g, const LabelBuffers& b, uint32_t* changed, hipStream_t s);
-hipError_t launch_cc_roots(const VGrid& g, const LabelBuffers& b, hipStream_t s);
-hipError_t launch_cc_resolve(const VGrid& g, const LabelBuffers& b, uint32_t* changed, hipStream_t s);
+// ints per collected component root: root, label, six earlier neighbour roots
+constexpr int kCollectStride = 8;
+hipError_t launch_cc_collect(const VGrid& g, const LabelBuffers& b, int32_t* out, uint32_t cap, uint32_t* count,
+                             hipStream_t s);
+hipError_t launch_cc_scatter(const int32_t* pairs, uint32_t n, int32_t* rep2, hipStream_t s);
 hipError_t launch_cc_apply(const VGrid& g, const LabelBuffers& b, hipStream_t s);
 // Restriction labels from vertex labels (MIN boundary): 1 on every cell
 // with an edge whose two vertices carry valid, different labels.

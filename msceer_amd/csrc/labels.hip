@@ -17,12 +17,13 @@ namespace {
 
 // Copy the concatenation of a multi-list to the host.
 int mlist_to_host(msceer_ctx* ctx, const MList& l, std::vector<uint32_t>& out) {
-    std::vector<uint32_t> counts(kSubs);
-    CHECK_HIP(hipMemcpyAsync(counts.data(), l.counts, sizeof(uint32_t) * kSubs, hipMemcpyDeviceToHost, ctx->stream));
+    std::vector<uint32_t> counts(kSubs + 1);
+    CHECK_HIP(hipMemcpyAsync(counts.data(), l.counts, sizeof(uint32_t) * (kSubs + 1), hipMemcpyDeviceToHost,
+                             ctx->stream));
     CHECK_HIP(hipStreamSynchronize(ctx->stream));
     out.clear();
-    for (int s = 0; s < kSubs; s++) {
-        const uint32_t n = std::min(counts[s], l.sub_cap);
+    for (int s = 0; s <= kSubs; s++) {  // sub-lists, then the spill list
+        const uint32_t n = std::min(counts[s], s < kSubs ? l.sub_cap : l.spill_cap);
         if (!n) continue;
         const size_t at = out.size();
         out.resize(at + n);
@@ -55,6 +56,78 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, a, b);
     return ms;
+}
+
+// IsolatedCCRegionRemoverNEW::ComputeOutput (gi_isolated_region_remover.h:155-192)
+// on b.label, in place.  Same-label components (root = earliest vertex) on the
+// GPU; the roots then join the region of their earliest lower neighbour with
+// another label, in vertex-id order with every earlier merge visible, which
+// is a sequential scan over the (few) roots on the host.
+int run_cleanup(msceer_ctx* ctx, const VGrid& g, const LabelBuffers& b) {
+    hipStream_t s = ctx->stream;
+    CHECK_HIP(launch_cc_init(g, b, s));
+    int it = 0;
+    int rc = iterate_until_stable(ctx, 4, 1 << 16, &it, [&](uint32_t* flag) { return launch_cc_hook(g, b, flag, s); });
+    if (rc) return rc;
+    uint32_t* count = ctl_dev(ctx) + kCtlCount;
+    const size_t row = sizeof(int32_t) * kCollectStride;
+    uint32_t cap = (uint32_t)std::min<int64_t>(g.verts(), std::max<int64_t>(4096, g.verts() / 64));
+    uint32_t R = 0;
+    for (int attempt = 0; attempt < 2; attempt++) {
+        CHECK_HIP(ctx->host_io.ensure(row * cap));
+        CHECK_HIP(launch_zero(count, 1, s));
+        CHECK_HIP(launch_cc_collect(g, b, ctx->host_io.as<int32_t>(), cap, count, s));
+        rc = read_ctl(ctx, kCtlCount, 1);
+        if (rc) return rc;
+        R = ctx->h_ctl[kCtlCount];
+        if (R <= cap) break;
+        cap = R;
+    }
+    if (R > cap) return MSCEER_EHIP;
+    std::vector<int32_t> h((size_t)R * kCollectStride);
+    if (R) CHECK_HIP(hipMemcpyAsync(h.data(), ctx->host_io.ptr, row * R, hipMemcpyDeviceToHost, s));
+    CHECK_HIP(hipStreamSynchronize(s));
+    std::vector<uint32_t> ord(R);
+    for (uint32_t k = 0; k < R; k++) ord[k] = k;
+    std::sort(ord.begin(), ord.end(),
+              [&](uint32_t a, uint32_t c) { return h[(size_t)a * kCollectStride] < h[(size_t)c * kCollectStride]; });
+    std::vector<int32_t> ids(R), lab(R), parent(R);
+    for (uint32_t k = 0; k < R; k++) {
+        ids[k] = h[(size_t)ord[k] * kCollectStride];
+        lab[k] = h[(size_t)ord[k] * kCollectStride + 1];
+        parent[k] = (int32_t)k;
+    }
+    auto find = [&](int32_t x) {
+        int32_t r = x;
+        while (parent[r] != r) r = parent[r];
+        while (parent[x] != r) { const int32_t n = parent[x]; parent[x] = r; x = n; }
+        return r;
+    };
+    for (uint32_t k = 0; k < R; k++) {
+        const int32_t* e = &h[(size_t)ord[k] * kCollectStride];
+        for (int j = 0; j < 6 && e[2 + j] >= 0; j++) {
+            const int32_t ri = (int32_t)(std::lower_bound(ids.begin(), ids.end(), e[2 + j]) - ids.begin());
+            if (ri >= (int32_t)R || ids[ri] != e[2 + j]) return MSCEER_EHIP;
+            const int32_t fr = find(ri);
+            if (lab[fr] == lab[k]) continue;
+            parent[k] = fr;
+            break;
+        }
+    }
+    std::vector<int32_t> pairs(2 * (size_t)R);
+    for (uint32_t k = 0; k < R; k++) {
+        pairs[2 * k] = ids[k];
+        pairs[2 * k + 1] = ids[find((int32_t)k)];
+    }
+    if (R) {
+        CHECK_HIP(hipMemcpyAsync(ctx->host_io.ptr, pairs.data(), sizeof(int32_t) * pairs.size(),
+                                 hipMemcpyHostToDevice, s));
+        CHECK_HIP(launch_cc_scatter(ctx->host_io.as<int32_t>(), R, b.rep2, s));
+    }
+    CHECK_HIP(launch_cc_apply(g, b, s));
+    // the pairs buffer is host memory: keep it alive until the copy is done
+    CHECK_HIP(hipStreamSynchronize(s));
+    return MSCEER_OK;
 }
 
 }  // namespace
@@ -102,13 +175,11 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
     CHECK_HIP(ctx->seedmask.ensure(V));
     CHECK_HIP(ctx->redo.ensure(V));
     CHECK_HIP(ctx->restriction.ensure(C));
-    uint32_t list_sub = 0, front_sub = 0;
-    rc = ensure_lists(ctx, d, &list_sub, &front_sub);
+    ListCaps caps{};
+    rc = ensure_lists(ctx, d, &caps);
     if (rc) return rc;
     uint32_t* ctl = ctl_dev(ctx);
-    uint32_t* l1 = ctx->lists.as<uint32_t>();
-    const MList list1{l1, ctl + kCtlList1, list_sub};
-    const MList list2{l1 + (size_t)list_sub * kSubs, ctl + kCtlList2, list_sub};
+    const MList list1 = crit_list(ctx, caps, 0), list2 = crit_list(ctx, caps, 1);
     LabelBuffers b;
     b.values = d_values;
     b.gradf = ctx->gradf.as<float4>();
@@ -136,9 +207,8 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
     CHECK_HIP(hipEventRecord(ev[1], s));
 
     // ---- extremum graph (critical vertices and 1-saddles) ----
-    const uint32_t fs = front_sub;
-    const MList minl{ctx->front0.as<uint32_t>(), ctl + kCtlRound, fs};
-    CHECK_HIP(launch_zero(ctl + kCtlRound, kSubs, s));
+    const MList minl = front_list(ctx, caps, 0);
+    CHECK_HIP(launch_zero(minl.counts, kCountWords, s));
     CHECK_HIP(launch_collect_minima(g, d_grad, minl, ctl + kCtlOverflow, s));
     rc = read_ctl(ctx, 0, kCtlWords);
     if (rc) return rc;
@@ -147,7 +217,7 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
     rc = mlist_to_host(ctx, minl, minima);
     if (rc) return rc;
     std::sort(minima.begin(), minima.end());
-    const uint64_t n_sad = list_total(ctx->h_ctl + kCtlList1, list_sub);
+    const uint64_t n_sad = list_total(ctx->h_ctl + kCtlList1, list1);
     // V-paths below the saddles (device), then values at minima and saddle vertices
     CHECK_HIP(ctx->host_io.ensure(sizeof(int64_t) * (3 * n_sad + 16)));
     int64_t* d_trace = ctx->host_io.as<int64_t>();
@@ -250,14 +320,13 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
     for (int stride = top; stride >= 1; stride /= 2)
         CHECK_HIP(launch_pass1(g, b, stride, stride == top, grad_thr, err_thr, max_iters, s));
     // pass 2 + pass 3 rounds
-    uint32_t* fronts[2] = {ctx->front0.as<uint32_t>(), ctx->front1.as<uint32_t>()};
-    auto counts = [&](int k) { return ctl + kCtlRound + (k % 3) * kSubs; };
-    auto front = [&](int k) { return MList{fronts[k % 2], counts(k), front_sub}; };
-    CHECK_HIP(launch_zero(counts(0), kSubs, s));
+    auto front = [&](int k) { return front_list(ctx, caps, k); };
+    auto counts = [&](int k) { return front(k).counts; };
+    CHECK_HIP(launch_zero(counts(0), kCountWords, s));
     CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
     CHECK_HIP(launch_pass2(g, b, front(0), ctl + kCtlOverflow, s));
     int k = 1;
-    CHECK_HIP(launch_zero(counts(1), kSubs, s));
+    CHECK_HIP(launch_zero(counts(1), kCountWords, s));
     int rounds = 0;
     for (int guard = 0; guard < 100000; guard++) {
         constexpr int kGroup = 4;
@@ -270,7 +339,7 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
         rc = read_ctl(ctx, 0, kCtlWords);
         if (rc) return rc;
         if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;
-        if (list_total(ctx->h_ctl + kCtlRound + ((k - 1) % 3) * kSubs, front_sub) == 0) break;
+        if (list_total(ctx->h_ctl + kCtlRound + ((k - 1) % 3) * kCountWords, front(k - 1)) == 0) break;
     }
     st.pass3_rounds = rounds;
     st.traced_fp32 = ctx->h_ctl[kCtlIntStats + kStatTraced1];
@@ -282,14 +351,8 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
     if (flags & MSCEER_STOP_INTEGRATED) return finish_early();
 
     // ---- cleanup of isolated components ----
-    CHECK_HIP(launch_cc_init(g, b, s));
-    int it = 0;
-    rc = iterate_until_stable(ctx, 4, 1 << 16, &it, [&](uint32_t* flag) { return launch_cc_hook(g, b, flag, s); });
+    rc = run_cleanup(ctx, g, b);
     if (rc) return rc;
-    CHECK_HIP(launch_cc_roots(g, b, s));
-    rc = iterate_until_stable(ctx, 4, 1 << 16, &it, [&](uint32_t* flag) { return launch_cc_resolve(g, b, flag, s); });
-    if (rc) return rc;
-    CHECK_HIP(launch_cc_apply(g, b, s));
     CHECK_HIP(hipEventRecord(ev[4], s));
 
     // ---- restriction labels, conforming gradient, ldir ----
@@ -297,7 +360,7 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
         int8_t* restr = ctx->restriction.as<int8_t>();
         CHECK_HIP(launch_min_boundary(g, b.label, restr, s));
         CHECK_HIP(hipEventRecord(ev[5], s));
-        CHECK_HIP(launch_zero(ctl + kCtlList1, 2 * kSubs, s));
+        CHECK_HIP(launch_zero(ctl + kCtlList1, 2 * kCountWords, s));
         CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
         LstarBuffers cb{d_values, d_grad, restr, &list1, &list2, ctl + kCtlStats, ctl + kCtlOverflow};
         CHECK_HIP(launch_lstar(d, cb, s));
@@ -305,7 +368,7 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
         rc = read_ctl(ctx, kCtlOverflow, 1);
         if (rc) return rc;
         int ldir_rounds = 0;
-        rc = run_ldir(ctx, d, d_grad, list1, list2, ctx->h_ctl[kCtlOverflow] == 0, front_sub, &ldir_rounds);
+        rc = run_ldir(ctx, d, d_grad, caps, ctx->h_ctl[kCtlOverflow] == 0, &ldir_rounds);
         if (rc) return rc;
         if (!dev) CHECK_HIP(hipMemcpyAsync(grad_out, d_grad, C, hipMemcpyDeviceToHost, s));
     } else {
@@ -375,14 +438,8 @@ extern "C" int msceer_clean_labels(msceer_ctx* ctx, const float* values, int64_t
     b.label2 = ctx->label2.as<int32_t>();
     b.rep = ctx->rep.as<int32_t>();
     b.rep2 = ctx->rep2.as<int32_t>();
-    CHECK_HIP(launch_cc_init(g, b, s));
-    int it = 0;
-    rc = iterate_until_stable(ctx, 4, 1 << 16, &it, [&](uint32_t* flag) { return launch_cc_hook(g, b, flag, s); });
+    rc = run_cleanup(ctx, g, b);
     if (rc) return rc;
-    CHECK_HIP(launch_cc_roots(g, b, s));
-    rc = iterate_until_stable(ctx, 4, 1 << 16, &it, [&](uint32_t* flag) { return launch_cc_resolve(g, b, flag, s); });
-    if (rc) return rc;
-    CHECK_HIP(launch_cc_apply(g, b, s));
     CHECK_HIP(hipMemcpyAsync(labels_inout, b.label, sizeof(int32_t) * V,
                              dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     CHECK_HIP(hipStreamSynchronize(s));
@@ -442,13 +499,11 @@ extern "C" int msceer_constrained_lstar(msceer_ctx* ctx, const float* values, in
         CHECK_HIP(ctx->grad.ensure(d.cells));
         d_grad = ctx->grad.as<uint8_t>();
     }
-    uint32_t list_sub = 0, front_sub = 0;
-    rc = ensure_lists(ctx, d, &list_sub, &front_sub);
+    ListCaps caps{};
+    rc = ensure_lists(ctx, d, &caps);
     if (rc) return rc;
     uint32_t* ctl = ctl_dev(ctx);
-    uint32_t* l1 = ctx->lists.as<uint32_t>();
-    const MList list1{l1, ctl + kCtlList1, list_sub};
-    const MList list2{l1 + (size_t)list_sub * kSubs, ctl + kCtlList2, list_sub};
+    const MList list1 = crit_list(ctx, caps, 0), list2 = crit_list(ctx, caps, 1);
     CHECK_HIP(launch_zero(ctl, kCtlWords, s));
     LstarBuffers cb{d_values, d_grad, d_r, &list1, &list2, ctl + kCtlStats, ctl + kCtlOverflow};
     CHECK_HIP(launch_lstar(d, cb, s));
@@ -456,7 +511,7 @@ extern "C" int msceer_constrained_lstar(msceer_ctx* ctx, const float* values, in
         rc = read_ctl(ctx, kCtlOverflow, 1);
         if (rc) return rc;
         int rounds = 0;
-        rc = run_ldir(ctx, d, d_grad, list1, list2, ctx->h_ctl[kCtlOverflow] == 0, front_sub, &rounds);
+        rc = run_ldir(ctx, d, d_grad, caps, ctx->h_ctl[kCtlOverflow] == 0, &rounds);
         if (rc) return rc;
     }
     if (!dev) CHECK_HIP(hipMemcpyAsync(grad_inout, d_grad, d.cells, hipMemcpyDeviceToHost, s));

@@ -89,9 +89,7 @@ struct Outbox {
         if (slot < (uint32_t)kStage) {
             stage[slot] = item;
         } else {
-            const uint32_t g = mlist_reserve(out, sub, 1u);
-            if (g < out.sub_cap) out.items[(size_t)sub * out.sub_cap + g] = item;
-            else atomicOr(overflow, 1u);
+            if (!mlist_push(out, sub, item)) atomicOr(overflow, 1u);
         }
     }
 };
@@ -112,10 +110,12 @@ __global__ void __launch_bounds__(kThreads)
 ldir_round_kernel(GradView v, CellGeom g, MList in, MList out, uint32_t* __restrict__ clear,
                   uint32_t* __restrict__ overflow, uint32_t value) {
     __shared__ uint32_t pool[kPool];
-    __shared__ uint32_t prefix[kSubs + 1];
-    __shared__ uint32_t s_n, s_next, s_base;
+    __shared__ uint32_t prefix[kSubs + 2];
+    __shared__ uint32_t s_n, s_next;
+    __shared__ MSlot s_base;
     const uint32_t tid = threadIdx.x;
-    if (blockIdx.x == 0 && tid < (uint32_t)kSubs) clear[tid] = 0u;
+    if (blockIdx.x == 0)
+        for (uint32_t j = tid; j < (uint32_t)kCountWords; j += kThreads) clear[j] = 0u;
     const uint32_t total = mlist_prefix(in, prefix);
     const uint32_t chunk = (total + gridDim.x - 1) / gridDim.x;
     const uint32_t lo = min(total, blockIdx.x * chunk), hi = min(total, lo + chunk);
@@ -173,9 +173,7 @@ ldir_round_kernel(GradView v, CellGeom g, MList in, MList out, uint32_t* __restr
                     if (slot < (uint32_t)kPool) {
                         pool[slot] = p;
                     } else {
-                        const uint32_t gs = mlist_reserve(out, sub, 1u);
-                        if (gs < out.sub_cap) out.items[(size_t)sub * out.sub_cap + gs] = p;
-                        else atomicOr(overflow, 1u);
+                        if (!mlist_push(out, sub, p)) atomicOr(overflow, 1u);
                     }
                 }
             }
@@ -201,7 +199,8 @@ __global__ void __launch_bounds__(kThreads)
 ldir_sweep_kernel(GradView v, CellGeom g, uint64_t cells, uint32_t item_dim, uint32_t value, MList out,
                   uint32_t* __restrict__ overflow) {
     __shared__ uint32_t stage[kStage];
-    __shared__ uint32_t s_n, s_base;
+    __shared__ uint32_t s_n;
+    __shared__ MSlot s_base;
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
     const Outbox box{stage, &s_n, out, (int)(blockIdx.x % kSubs), overflow};

@@ -38,7 +38,7 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
     __shared__ uint32_t tile[kTileN];
     __shared__ uint32_t slice[27 * kThreads];  // each lane's 27 neighbour keys, t-major
     __shared__ uint32_t s_cnt[4];  // per block: critical 1-cells, 2-cells, 0-cells, 3-cells
-    __shared__ uint32_t s_base[2];
+    __shared__ MSlot s_slot[2];
     const int x0 = blockIdx.x * kBX, y0 = blockIdx.y * kBY, z0 = blockIdx.z * kBZ;
     const int flat = threadIdx.x + kBX * (threadIdx.y + kBY * threadIdx.z);
     for (int i = flat; i < kTileN; i += kThreads) {
@@ -105,27 +105,23 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
     const int sub = (int)((blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)) % kSubs);
     if (flat == 0) {
         const uint32_t c1 = s_cnt[0], c2 = s_cnt[1];
-        s_base[0] = c1 ? mlist_reserve(list1, sub, c1) : 0u;
-        s_base[1] = c2 ? mlist_reserve(list2, sub, c2) : 0u;
+        s_slot[0] = c1 ? mlist_reserve(list1, sub, c1) : MSlot{0u, 0u, 0u};
+        s_slot[1] = c2 ? mlist_reserve(list2, sub, c2) : MSlot{0u, 0u, 0u};
         const uint32_t st[4] = {s_cnt[2], c1, c2, s_cnt[3]};
         for (int k = 0; k < 4; k++)
             if (st[k]) atomicAdd(&stats[sub * 4 + k], st[k]);
     }
     __syncthreads();
     if (crit & (kDim1 | kDim2)) {
-        uint32_t slot1 = s_base[0] + o1, slot2 = s_base[1] + o2;
+        const MSlot r1 = s_slot[0], r2 = s_slot[1];
+        uint32_t j1 = o1, j2 = o2;
         bool lost = false;
         for (uint32_t m = crit & (kDim1 | kDim2); m; m &= m - 1) {
             const int t = __builtin_ctz(m);
             const int64_t off = (int64_t)tid_x(t) + (int64_t)Mx * ((int64_t)tid_y(t) + (int64_t)My * tid_z(t));
             const uint32_t cell = (uint32_t)((int64_t)vcell + off);
-            if ((kDim1 >> t) & 1u) {
-                if (slot1 < list1.sub_cap) list1.items[(size_t)sub * list1.sub_cap + slot1] = cell; else lost = true;
-                slot1++;
-            } else {
-                if (slot2 < list2.sub_cap) list2.items[(size_t)sub * list2.sub_cap + slot2] = cell; else lost = true;
-                slot2++;
-            }
+            if ((kDim1 >> t) & 1u) lost |= !mlist_store(list1, sub, r1, j1++, cell);
+            else lost |= !mlist_store(list2, sub, r2, j2++, cell);
         }
         if (lost) atomicOr(overflow, 1u);
     }

@@ -1,0 +1,78 @@
+"""The steepest_lstar / ondemandaccurate command-line drop-ins on the GPU.
+
+They take the reference programs' arguments (steepest_lstar.cxx:52-80,
+ondemandaccurate.cxx:704-760) and write the same files."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from golden import load_lstar, load_ondemand
+from msceer_amd import labels
+
+CLI = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "msceer_amd", "cli")
+
+pytestmark = pytest.mark.gpu
+
+
+def _exe(name):
+    p = os.path.join(CLI, name)
+    if not os.path.exists(p):
+        pytest.fail(f"{p} is not built (python -m msceer_amd.build)")
+    return p
+
+
+def _run(args, tmp_path):
+    r = subprocess.run(args, cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    # the reference programs return 1 on success
+    assert r.returncode == 1, r.stdout + r.stderr
+    return r.stdout
+
+
+def test_steepest_lstar_cli_writes_reference_grad(tmp_path):
+    f, ref = load_lstar("noise_48x97x31")
+    Z, Y, X = f.shape
+    raw = tmp_path / "vol.raw"
+    f.tofile(raw)
+    out = _run([_exe("steepest_lstar"), str(X), str(Y), str(Z), str(raw), "0", "0", "0", "4"], tmp_path)
+    g = np.fromfile(str(raw) + ".grad", np.uint8)
+    assert g.size == ref.size and (g == ref).all()
+    assert "index-0=" in out
+    line = open(str(raw) + ".steep.gtime.txt").read().split()
+    assert line[:3] == ["Total", "Overall", "0"]
+
+
+def test_ondemandaccurate_cli_matches_library(tmp_path):
+    f, (err, gthr, iters, presimp), _ = load_ondemand("smooth_34x33x32")
+    Z, Y, X = f.shape
+    raw = tmp_path / "vol.raw"
+    f.tofile(raw)
+    _run([_exe("ondemandaccurate"), str(X), str(Y), str(Z), str(raw), repr(err), repr(gthr), str(iters),
+          "0", "0", "1", "0", repr(presimp), "8", "1"], tmp_path)
+    g = np.fromfile(str(raw) + ".grad", np.uint8)
+    lab = np.fromfile(f"{raw}._int_{X}x{Y}x{Z}.raw", np.int32)
+    grad, lab_api, _ = labels.ondemand_asc(f, err, gthr, iters, presimp, want_grad=True)
+    assert (g == grad.ravel()).all()
+    assert (lab == lab_api.ravel()).all()
+    tasks = open(str(raw) + ".008.gtime.txt").read().splitlines()
+    assert len(tasks) == 5
+
+
+def test_ondemandaccurate_cli_without_accuracy_writes_lstar_grad(tmp_path):
+    f, ref = load_lstar("noise_48x97x31")
+    Z, Y, X = f.shape
+    raw = tmp_path / "vol.raw"
+    f.tofile(raw)
+    _run([_exe("ondemandaccurate"), str(X), str(Y), str(Z), str(raw), "1e-3", "1e-6", "100",
+          "0", "0", "0", "0", "0"], tmp_path)
+    g = np.fromfile(str(raw) + ".grad", np.uint8)
+    assert (g == ref).all()
+
+
+def test_ondemandaccurate_cli_rejects_unbuilt_branches(tmp_path):
+    raw = tmp_path / "vol.raw"
+    np.zeros(8, np.float32).tofile(raw)
+    r = subprocess.run([_exe("ondemandaccurate"), "2", "2", "2", str(raw), "1e-3", "1e-6", "10",
+                        "1", "0", "0", "0", "0"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "needASC2" in r.stderr
