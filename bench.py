@@ -64,6 +64,18 @@ def cpu_baseline(f: np.ndarray, slab: int):
                       f"max/min labelling + lower-star pairing + ldir + flags, {dt:.2f} s"}
 
 
+def calibrate_copy():
+    import torch
+
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        torch.mul(a, 2.0, out=b)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibration": "torch.mul float32", "bytes_read": 4 * n, "bytes_written": 4 * n}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -72,7 +84,13 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-slab", type=int, default=96)
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run a plain 1 GiB read + 1 GiB write streaming kernel (torch), the "
+                         "reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
               "the multi-GPU sharding is described in DESIGN.md", file=sys.stderr)
