@@ -47,11 +47,7 @@ def cpu_baseline(f: np.ndarray, slab: int):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib  # noqa: E402  (checker / baseline only)
 
-    cores = os.cpu_count() or 1
-    try:
-        cores = len(os.sched_getaffinity(0))
-    except Exception:
-        pass
+    cores = _cores()
     oracle_lib.set_threads(cores)
     sample = np.ascontiguousarray(f[:slab])
     oracle_lib.steepest_grad(sample[:8])  # warm the thread pool
@@ -62,6 +58,94 @@ def cpu_baseline(f: np.ndarray, slab: int):
     return {"value": round(X * Y * Z / dt / 1e6, 3), "unit": "Mvert/s", "cores": cores, "kind": "port",
             "sample": f"{X}x{Y}x{Z} slab (first {Z} z-planes) of the benchmark volume, oracle "
                       f"max/min labelling + lower-star pairing + ldir + flags, {dt:.2f} s"}
+
+
+def labels_cpu_baseline(f: np.ndarray, slab: int, params):
+    """The CPU oracle of the accurate labelling (reference arithmetic, all host
+    cores) on a z-slab of the labelling volume."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib  # noqa: E402  (checker / baseline only)
+
+    cores = _cores()
+    oracle_lib.set_threads(cores)
+    sample = np.ascontiguousarray(f[:slab])
+    t0 = time.perf_counter()
+    oracle_lib.ondemand_asc(sample, *params, mode=0)
+    dt = time.perf_counter() - t0
+    Z, Y, X = sample.shape
+    return {"value": round(X * Y * Z / dt / 1e6, 3), "unit": "Mvert/s", "cores": cores, "kind": "port",
+            "sample": f"{X}x{Y}x{Z} slab (first {Z} z-planes) of the labelling volume, oracle accurate "
+                      f"ascending pipeline in the reference's mixed precision, {dt:.2f} s"}
+
+
+def _cores():
+    cores = os.cpu_count() or 1
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except Exception:
+        pass
+    return cores
+
+
+def bench_labels(args):
+    """Accurate ascending labels + conforming .grad of the 256^3 labelling
+    workload, values resident in HBM; label mismatch against the committed FP64
+    oracle labels."""
+    sys.path.insert(0, os.path.join(ROOT, "tests", "golden"))
+    from golden import C3_PARAMS, load_c3_fp64  # committed fixture (data)
+    from msceer_amd import hipmem, native, synth
+
+    log("generating 256^3 labelling field (200 Gaussians + sine, seed 303)")
+    f = synth.labels_field(303, 256)
+    ref, sha = load_c3_fp64()
+    assert synth.sha256(f) == sha, "labelling field generator drifted from the fixture"
+    Z, Y, X = f.shape
+    V = X * Y * Z
+    cells = (2 * X - 1) * (2 * Y - 1) * (2 * Z - 1)
+    err, gthr, iters, presimp = C3_PARAMS
+    ctx = native.Context(0)
+    lib = native.lib()
+    dv = hipmem.DeviceBuffer(f.nbytes)
+    dv.upload(f)
+    dg = hipmem.DeviceBuffer(cells)
+    dl = hipmem.DeviceBuffer(4 * V)
+    st = native.LabelStats()
+
+    def run(mask_addr=None):
+        native.check(lib.msceer_ondemand_asc(ctx.handle, ctypes.c_void_p(dv.addr), X, Y, Z, err, gthr, iters,
+                                             presimp, ctypes.c_void_p(mask_addr) if mask_addr else None,
+                                             ctypes.c_void_p(dg.addr), ctypes.c_void_p(dl.addr),
+                                             ctypes.byref(st), native.DEVICE_PTRS), "msceer_ondemand_asc")
+
+    for _ in range(args.label_warmup):
+        run()
+    hipmem.synchronize()
+    stages = []
+    t0 = time.perf_counter()
+    for _ in range(args.label_steps):
+        run()
+        stages.append(st.as_dict())
+    hipmem.synchronize()
+    ms = (time.perf_counter() - t0) * 1e3 / args.label_steps
+    lab = dl.download(np.int32, V)
+    mism = float((lab != ref).mean())
+    last = stages[-1]
+    plain = {"value": round(V / (ms * 1e-3) / 1e6, 2), "unit": "Mvert/s", "ms_per_step": round(ms, 3),
+             "steps": args.label_steps, "mismatch_vs_fp64_oracle": mism,
+             "stage_ms": {k[3:]: round(float(np.mean([d[k] for d in stages])), 3)
+                          for k in ("ms_lstar", "ms_regions", "ms_integrate", "ms_cleanup", "ms_grad", "ms_total")},
+             "counts": {k: int(last[k]) for k in ("minima", "regions", "traced_fp32", "over_budget_fp32",
+                                                  "reintegrated_fp64", "over_budget_fp64", "certain_sweeps",
+                                                  "pass3_rounds")}}
+    base = None
+    if not args.no_cpu:
+        log("timing the labelling CPU baseline")
+        base = labels_cpu_baseline(f, args.label_cpu_slab, C3_PARAMS)
+    ctx.close()
+    return {"workload": f"ondemandaccurate needASC2 + conforming .grad, {X}x{Y}x{Z} float32, 200 Gaussians "
+                        f"(sigma 4-16) + 0.05 sin sin sin, seed 303; err 1e-3 (squared error), grad 1e-6, "
+                        f"4096 steps, presimp 0; values resident in HBM",
+            "plain": plain, "learned": None, "cpu_baseline": base}
 
 
 def calibrate_copy():
@@ -84,6 +168,10 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-slab", type=int, default=96)
+    ap.add_argument("--no-labels", action="store_true", help="skip the accurate-labelling measurement")
+    ap.add_argument("--label-steps", type=int, default=3)
+    ap.add_argument("--label-warmup", type=int, default=1)
+    ap.add_argument("--label-cpu-slab", type=int, default=48)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB read + 1 GiB write streaming kernel (torch), the "
                          "reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -168,8 +256,10 @@ def main():
                    "ldir_ms_mean": round(float(np.mean(ldir_ms)), 4),
                    "ldir_rounds": int(np.max(rounds)), "critical_cells_by_dim": crit},
     }
-    print(json.dumps(out), flush=True)
     ctx.close()
+    if not args.no_labels:
+        out["labels"] = bench_labels(args)
+    print(json.dumps(out), flush=True)
 
 
 if __name__ == "__main__":
