@@ -111,11 +111,11 @@ def bench_labels(args):
     dl = hipmem.DeviceBuffer(4 * V)
     st = native.LabelStats()
 
-    def run(mask_addr=None):
+    def run(mask_addr=None, extra=0):
         native.check(lib.msceer_ondemand_asc(ctx.handle, ctypes.c_void_p(dv.addr), X, Y, Z, err, gthr, iters,
                                              presimp, ctypes.c_void_p(mask_addr) if mask_addr else None,
                                              ctypes.c_void_p(dg.addr), ctypes.c_void_p(dl.addr),
-                                             ctypes.byref(st), native.DEVICE_PTRS), "msceer_ondemand_asc")
+                                             ctypes.byref(st), native.DEVICE_PTRS | extra), "msceer_ondemand_asc")
 
     for _ in range(args.label_warmup):
         run()
@@ -136,7 +136,42 @@ def bench_labels(args):
                           for k in ("ms_lstar", "ms_regions", "ms_integrate", "ms_cleanup", "ms_grad", "ms_total")},
              "counts": {k: int(last[k]) for k in ("minima", "regions", "traced_fp32", "over_budget_fp32",
                                                   "reintegrated_fp64", "over_budget_fp64", "certain_sweeps",
-                                                  "pass3_rounds")}}
+                                                  "pass3_rounds", "steps_fp32", "steps_fp64")}}
+    if args.ideal_mask_probe:
+        from scipy import ndimage
+        zero = hipmem.DeviceBuffer(V)
+        zero.upload(np.zeros(V, np.uint8))
+        run(mask_addr=zero.addr)
+        comb = dl.download(np.int32, V).reshape(Z, Y, X)
+        pl = lab.reshape(Z, Y, X)
+        bnd = np.zeros_like(pl, bool)
+        for ax in range(3):
+            d = np.diff(pl, axis=ax) != 0
+            sl0 = [slice(None)] * 3
+            sl1 = [slice(None)] * 3
+            sl0[ax] = slice(0, -1)
+            sl1[ax] = slice(1, None)
+            bnd[tuple(sl0)] |= d
+            bnd[tuple(sl1)] |= d
+        core = bnd | (comb != pl)
+        for margin in (1, 2):
+            m = ndimage.binary_dilation(core, structure=np.ones((3, 3, 3), bool), iterations=margin)
+            dm = hipmem.DeviceBuffer(V)
+            dm.upload(m.astype(np.uint8).ravel())
+            run(mask_addr=dm.addr)
+            hipmem.synchronize()
+            t0 = time.perf_counter()
+            run(mask_addr=dm.addr)
+            hipmem.synchronize()
+            ms_m = (time.perf_counter() - t0) * 1e3
+            labm = dl.download(np.int32, V)
+            plain[f"ideal_mask_probe_{margin}"] = {
+                "ms": round(ms_m, 3), "mask_fraction": float(m.mean()), "core_fraction": float(core.mean()),
+                "comb_mismatch_vs_plain": float((comb != pl).mean()),
+                "mismatch_vs_fp64_oracle": float((labm != ref).mean()),
+                "stage_ms": {k[3:]: round(float(st.as_dict()[k]), 3) for k in
+                             ("ms_lstar", "ms_regions", "ms_integrate", "ms_cleanup", "ms_grad")},
+                "fp64": int(st.reintegrated_fp64)}
     base = None
     if not args.no_cpu:
         log("timing the labelling CPU baseline")
@@ -169,9 +204,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-slab", type=int, default=96)
     ap.add_argument("--no-labels", action="store_true", help="skip the accurate-labelling measurement")
+    ap.add_argument("--labels-only", action="store_true", help="only the accurate-labelling measurement")
     ap.add_argument("--label-steps", type=int, default=3)
     ap.add_argument("--label-warmup", type=int, default=1)
     ap.add_argument("--label-cpu-slab", type=int, default=48)
+    ap.add_argument("--ideal-mask-probe", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB read + 1 GiB write streaming kernel (torch), the "
                          "reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -184,6 +221,9 @@ def main():
               "the multi-GPU sharding is described in DESIGN.md", file=sys.stderr)
         sys.exit(2)
 
+    if args.labels_only:
+        print(json.dumps({"labels": bench_labels(args)}), flush=True)
+        return
     from msceer_amd import hipmem, native, synth
 
     n = args.size

@@ -65,13 +65,15 @@ typedef struct msceer_label_stats {
     int64_t over_budget_fp64;   /* of those, stopped by the step budget */
     int64_t integrated_vertices;/* vertices the learned mask sent to integration (all if none) */
     int32_t certain_sweeps;     /* sweeps of the certain-region fixed point */
-    int32_t pass3_rounds;       /* rounds of the FP64 re-integration */
+    int32_t pass3_rounds;       /* launches of the FP64 re-integration (one persistent launch) */
     float ms_lstar;             /* base + constrained lower-star kernels */
     float ms_regions;           /* extremum graph + certain regions */
     float ms_integrate;         /* passes 1-3 */
     float ms_cleanup;           /* isolated-region cleanup + restriction labels */
     float ms_grad;              /* constrained pass + ldir */
     float ms_total;
+    int64_t steps_fp32;         /* advection steps (accepted and rejected) of pass 1 */
+    int64_t steps_fp64;         /* advection steps of pass 3 */
 } msceer_label_stats;
 
 int msceer_version(void);
@@ -122,7 +124,8 @@ int msceer_lstar_grad(msceer_ctx* ctx, const float* values, int64_t nx, int64_t 
  * NULL): the conforming .grad.  integrate_mask (X*Y*Z bytes, may be NULL):
  * when given, only vertices with a nonzero mask are integrated numerically
  * and every other vertex takes the label of the minimum its discrete-gradient
- * V-path descends to.
+ * V-path descends to; those vertices are trusted, so streamlines of the
+ * integrated vertices (passes 1 and 3) also stop on reaching one.
  */
 int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz, float err_thr,
                         float grad_thr, int32_t max_iters, float presimp, const uint8_t* integrate_mask,

@@ -47,8 +47,9 @@ enum Ctl : int {
     kCtlTail = kCtlOverflow + 1,   // shadow of the grad array's last partial word
     kCtlFlag = kCtlOverflow + 2,   // change flag of iterative sweeps
     kCtlCount = kCtlOverflow + 3,  // a kernel-reported count
-    kCtlIntStats = kCtlOverflow + 8,  // eight integrator counters
-    kCtlWords = kCtlOverflow + 16
+    kCtlCursor = kCtlOverflow + 4,  // head, tail, pending of the re-integration queue
+    kCtlIntStats = kCtlOverflow + 8,  // sixteen integrator counters (8-byte aligned)
+    kCtlWords = kCtlOverflow + 24
 };
 
 // Capacities of the critical lists and the frontiers of one volume.
@@ -68,7 +69,7 @@ struct msceer_ctx {
     hipStream_t stream = nullptr;
     msceer::DevBuf values, grad, lists, front0, front1, ctl;
     // accurate labelling
-    msceer::DevBuf gradf, label, label2, dest, seedmask, redo, stamp, rep, rep2, restriction, host_io, mask;
+    msceer::DevBuf gradf, gradd, label, label2, dest, seedmask, redo, stamp, rep, rep2, restriction, host_io, mask;
     uint32_t* h_ctl = nullptr;  // pinned mirror of the control block
     hipEvent_t ev[12] = {};
     msceer_timing last{};

@@ -134,20 +134,32 @@ struct GradSrc<float> {
 };
 template <>
 struct GradSrc<double> {
-    const float4* gradf;
-    const float* f;
-    const uint8_t* seedmask;
-    VGrid g;
+    const double4* gradd;
     __device__ __forceinline__ V3<double> at(int64_t v) const {
-        if (seedmask[v]) return {0.0, 0.0, 0.0};
-        const int x = (int)(v % g.X), y = (int)((v / g.X) % g.Y), z = (int)(v / g.sxy);
-        V3<double> r;
-        r.x = (x > 0 && x < g.X - 1) ? (0.0 + (double)(-0.5f * f[v - 1])) + (double)(0.5f * f[v + 1]) : 0.0;
-        r.y = (y > 0 && y < g.Y - 1) ? (0.0 + (double)(-0.5f * f[v - g.X])) + (double)(0.5f * f[v + g.X]) : 0.0;
-        r.z = (z > 0 && z < g.Z - 1) ? (0.0 + (double)(-0.5f * f[v - g.sxy])) + (double)(0.5f * f[v + g.sxy]) : 0.0;
-        return r;
+        const double4 r = gradd[v];
+        return {r.x, r.y, r.z};
     }
 };
+
+// FP64 gradient field, as the reference computes it: FP32 coefficients times
+// the values (exact), summed in FP64; zero on boundary axes and at region
+// extrema (seedmask).
+__global__ void __launch_bounds__(kThreads)
+grad_field64_kernel(const float* __restrict__ f, VGrid g, const uint8_t* __restrict__ seedmask,
+                    double4* __restrict__ out) {
+    const int64_t n = g.verts();
+    for (int64_t v = (int64_t)blockIdx.x * kThreads + threadIdx.x; v < n; v += (int64_t)gridDim.x * kThreads) {
+        double4 r = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (!seedmask[v]) {
+            const int x = (int)(v % g.X), y = (int)((v / g.X) % g.Y), z = (int)(v / g.sxy);
+            r.x = (x > 0 && x < g.X - 1) ? (0.0 + (double)(-0.5f * f[v - 1])) + (double)(0.5f * f[v + 1]) : 0.0;
+            r.y = (y > 0 && y < g.Y - 1) ? (0.0 + (double)(-0.5f * f[v - g.X])) + (double)(0.5f * f[v + g.X]) : 0.0;
+            r.z = (z > 0 && z < g.Z - 1) ? (0.0 + (double)(-0.5f * f[v - g.sxy])) + (double)(0.5f * f[v + g.sxy])
+                                         : 0.0;
+        }
+        out[v] = r;
+    }
+}
 
 template <typename T>
 __device__ __forceinline__ V3<T> lerp3(const V3<T>& a, const V3<T>& b, T t) {
@@ -264,6 +276,7 @@ pass1_kernel(GradSrc<float> src, VGrid g, LabelBuffers b, int stride, int first_
     const int nx = (g.X + stride - 1) / stride, ny = (g.Y + stride - 1) / stride, nz = (g.Z + stride - 1) / stride;
     const int64_t n = (int64_t)nx * ny * nz;
     uint32_t traced = 0, over = 0;
+    uint64_t steps = 0;
     for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < n; t += (int64_t)gridDim.x * kThreads) {
         const int x = (int)(t % nx) * stride, y = (int)((t / nx) % ny) * stride;
         const int z = (int)(t / ((int64_t)nx * ny)) * stride;
@@ -288,6 +301,7 @@ pass1_kernel(GradSrc<float> src, VGrid g, LabelBuffers b, int stride, int first_
             if (np < kPath) path[(np++) * kThreads] = (int32_t)next;
             if (ev != kOutOfVoxel) break;
         }
+        steps += (uint64_t)(max_iters - (iters > 0 ? iters : 0));
         const int32_t lab = b.label[next];
         for (int j = 0; j < np; j++) {
             const int32_t p = path[j * kThreads];
@@ -303,6 +317,7 @@ pass1_kernel(GradSrc<float> src, VGrid g, LabelBuffers b, int stride, int first_
     }
     if (traced) atomicAdd(&b.stats[kStatTraced1], traced);
     if (over) atomicAdd(&b.stats[kStatOver1], over);
+    if (steps) atomicAdd(reinterpret_cast<unsigned long long*>(&b.stats[kStatSteps1]), (unsigned long long)steps);
 }
 
 // Block-staged appends to a sub-list (as in the ldir rounds).
@@ -335,16 +350,18 @@ __device__ __forceinline__ int all6(const VGrid& g, int64_t i, int64_t nb[6]) {
     return k;
 }
 
+// Pass 2: every assigned vertex next to a different label, or flagged for
+// re-tracing, is queued (and stamped as queued).  One reservation per block.
 __global__ void __launch_bounds__(kThreads)
-pass2_kernel(VGrid g, LabelBuffers b, MList out, uint32_t* __restrict__ overflow) {
+pass2_kernel(VGrid g, LabelBuffers b, WorkQueue q) {
     __shared__ uint32_t stage[kStage];
-    __shared__ uint32_t s_n;
-    __shared__ MSlot s_base;
+    __shared__ uint32_t s_n, s_base;
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    const Outbox box{stage, &s_n, out, (int)(blockIdx.x % kSubs), overflow};
     const int64_t n = g.verts();
-    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+    // each block covers a contiguous range of at most kStage vertices
+    const int64_t lo = (int64_t)blockIdx.x * kStage, hi = lo + kStage < n ? lo + kStage : n;
+    for (int64_t i = lo + threadIdx.x; i < hi; i += kThreads) {
         if (b.dest[i] != kAssigned) continue;
         bool push = b.redo[i] != 0;
         if (!push) {
@@ -356,47 +373,128 @@ pass2_kernel(VGrid g, LabelBuffers b, MList out, uint32_t* __restrict__ overflow
         }
         if (push) {
             b.stamp[i] = 1u;
-            box.push((uint32_t)i);
+            stage[atomicAdd(&s_n, 1u)] = (uint32_t)i;
         }
     }
     __syncthreads();
-    mlist_flush_block(out, box.sub, stage, s_n < (uint32_t)kStage ? s_n : (uint32_t)kStage, &s_base, overflow);
+    const uint32_t cnt = s_n;
+    if (threadIdx.x == 0 && cnt) {
+        atomicAdd(q.pending, cnt);
+        s_base = atomicAdd(q.tail, cnt);
+    }
+    __syncthreads();
+    uint32_t lost = 0;
+    for (uint32_t j = threadIdx.x; j < cnt; j += kThreads) {
+        const uint32_t slot = s_base + j;
+        if (slot < q.capacity) q.items[slot] = stage[j];
+        else lost++;
+    }
+    if (lost) {
+        atomicSub(q.pending, lost);
+        atomicOr(q.overflow, 1u);
+    }
 }
 
-// Pass 3 round: FP64 re-integration to the original certain regions only.
+// Pass 3: re-integration to the original certain regions only, until the
+// queue drains.  A re-integrated vertex whose label changed queues its
+// assigned All6 neighbours that carry another label (each vertex is queued at
+// most once: after its re-integration it is no longer assigned).  The final
+// labels do not depend on the processing order, since every re-integration
+// ends on a certain terminal, whose label never changes.
+//
+// One persistent launch: lanes claim queue slots (one atomic per wave) and
+// wait for a claimed slot to be written; a lane whose streamline ends takes
+// new work while the others keep advecting.  `pending` counts queued items
+// not yet finished; a producer adds its children before it retires its own
+// item, so pending == 0 means that no slot will be written again.
+constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;
+constexpr uint64_t kIdleTicks = 400000000ull;  // 4 s of the 100 MHz real-time counter
+
+template <typename T>
 __global__ void __launch_bounds__(kThreads)
-pass3_kernel(GradSrc<double> src, VGrid g, LabelBuffers b, MList in, MList out, uint32_t* __restrict__ clear,
-             uint32_t* __restrict__ overflow, uint32_t round, double gthr, double ethr, int max_iters) {
-    __shared__ uint32_t stage[kStage];
-    __shared__ uint32_t prefix[kSubs + 2];
-    __shared__ uint32_t s_n;
-    __shared__ MSlot s_base;
-    if (blockIdx.x == 0)
-        for (uint32_t j = threadIdx.x; j < (uint32_t)kCountWords; j += kThreads) clear[j] = 0u;
-    if (threadIdx.x == 0) s_n = 0;
-    const uint32_t total = mlist_prefix(in, prefix);
-    const Outbox box{stage, &s_n, out, (int)(blockIdx.x % kSubs), overflow};
+pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T gthr, T ethr, int max_iters) {
+    const int lane = (int)(threadIdx.x & 63u);
+    const uint64_t below = (1ull << lane) - 1ull;
     uint32_t traced = 0, over = 0;
-    for (uint32_t it = blockIdx.x * kThreads + threadIdx.x; it < total; it += gridDim.x * kThreads) {
-        const int64_t v = mlist_at(in, prefix, it);
-        if (b.dest[v] != kAssigned) continue;  // already re-integrated
-        traced++;
-        const int32_t init = b.label[v];
-        const int x = (int)(v % g.X), y = (int)((v / g.X) % g.Y), z = (int)(v / g.sxy);
-        V3<double> cur{(double)x, (double)y, (double)z};
-        int cx = x, cy = y, cz = z;
-        int iters = max_iters;
-        int ev;
-        int64_t next;
-        while (true) {
-            const bool near = dist_to_boundary(g, cx, cy, cz) <= 1;
-            ev = advect_voxel<double>(src, g, b.dest, false, gthr, ethr, near, cur, iters);
-            next = nearest(g, cur, near);
-            cx = (int)(next % g.X);
-            cy = (int)((next / g.X) % g.Y);
-            cz = (int)(next / g.sxy);
-            if (ev != kOutOfVoxel) break;
+    uint64_t steps = 0;
+    bool active = false, claimed = false, done = false;
+    uint32_t slot = 0;
+    uint64_t idle_since = 0;
+    int64_t v = 0;
+    int32_t init = 0;
+    V3<T> cur{(T)0, (T)0, (T)0};
+    int cx = 0, cy = 0, cz = 0, iters = 0;
+    while (true) {
+        const bool need = !active && !claimed && !done;
+        const uint64_t want = __ballot(need);
+        if (want) {
+            const int leader = __ffsll((unsigned long long)want) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(q.head, (uint32_t)__popcll(want));
+            base = __shfl(base, leader);
+            if (need) {
+                slot = base + (uint32_t)__popcll(want & below);
+                if (slot < q.capacity) claimed = true;
+                else done = true;
+            }
         }
+        if (claimed) {
+            const uint32_t item = __hip_atomic_load(&q.items[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (item != kEmptySlot) {
+                claimed = false;
+                v = (int64_t)item;
+                if (b.dest[v] == kAssigned) {
+                    active = true;
+                    traced++;
+                    init = b.label[v];
+                    cx = (int)(v % g.X);
+                    cy = (int)((v / g.X) % g.Y);
+                    cz = (int)(v / g.sxy);
+                    cur = {(T)cx, (T)cy, (T)cz};
+                    iters = max_iters;
+                } else {
+                    atomicSub(q.pending, 1u);
+                }
+            }
+        }
+        const uint64_t waiting = __ballot(claimed);
+        if (waiting && !__any(active)) {
+            // nothing to advect in this wave: one lane checks for the end of
+            // the pass, the wave backs off before polling its slots again
+            const int w = __ffsll((unsigned long long)waiting) - 1;
+            uint32_t p = 0;
+            if (lane == w) p = __hip_atomic_load(q.pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            p = __shfl(p, w);
+            if (p == 0u) {
+                if (claimed) done = true;
+                claimed = false;
+            } else {
+                // a wave idle for kIdleTicks gives up and reports it, so the
+                // kernel ends even if the queue's invariant were ever broken
+                const uint64_t now = __builtin_amdgcn_s_memrealtime();
+                if (idle_since == 0) {
+                    idle_since = now;
+                } else if (now - idle_since > kIdleTicks) {
+                    if (lane == w) atomicOr(q.overflow, 2u);
+                    claimed = false;
+                    done = true;
+                }
+                __builtin_amdgcn_s_sleep(2);
+            }
+        } else {
+            idle_since = 0;
+        }
+        if (__all(done)) break;
+        if (!active) continue;
+        const bool near = dist_to_boundary(g, cx, cy, cz) <= 1;
+        const int ev = advect_voxel<T>(src, g, b.dest, false, gthr, ethr, near, cur, iters);
+        const int64_t next = nearest(g, cur, near);
+        cx = (int)(next % g.X);
+        cy = (int)((next / g.X) % g.Y);
+        cz = (int)(next / g.sxy);
+        if (ev == kOutOfVoxel) continue;
+        active = false;
+        steps += (uint64_t)(max_iters - (iters > 0 ? iters : 0));
         if (ev == kOverMax) over++;
         const int32_t nl = b.label[next];
         b.label[v] = nl;
@@ -404,17 +502,34 @@ pass3_kernel(GradSrc<double> src, VGrid g, LabelBuffers b, MList in, MList out, 
         if (nl != init) {
             int64_t nb[6];
             const int k = all6(g, v, nb);
+            uint32_t kids[6];
+            int nk = 0;
             for (int j = 0; j < k; j++) {
-                if (b.dest[nb[j]] == kAssigned && b.label[nb[j]] != nl &&
-                    atomicExch(&b.stamp[nb[j]], round + 1u) != round + 1u)
-                    box.push((uint32_t)nb[j]);
+                if (b.dest[nb[j]] == kAssigned && b.label[nb[j]] != nl && atomicExch(&b.stamp[nb[j]], 1u) == 0u)
+                    kids[nk++] = (uint32_t)nb[j];
+            }
+            if (nk) {
+                atomicAdd(q.pending, (uint32_t)nk);
+                const uint32_t at = atomicAdd(q.tail, (uint32_t)nk);
+                uint32_t lost = 0;
+                for (int j = 0; j < nk; j++) {
+                    if (at + j < q.capacity)
+                        __hip_atomic_store(&q.items[at + j], kids[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                    else
+                        lost++;
+                }
+                if (lost) {
+                    atomicSub(q.pending, lost);
+                    atomicOr(q.overflow, 1u);
+                }
             }
         }
+        __threadfence();
+        atomicSub(q.pending, 1u);
     }
     if (traced) atomicAdd(&b.stats[kStatTraced3], traced);
     if (over) atomicAdd(&b.stats[kStatOver3], over);
-    __syncthreads();
-    mlist_flush_block(out, box.sub, stage, s_n < (uint32_t)kStage ? s_n : (uint32_t)kStage, &s_base, overflow);
+    if (steps) atomicAdd(reinterpret_cast<unsigned long long*>(&b.stats[kStatSteps3]), (unsigned long long)steps);
 }
 
 // ---- cleanup of isolated components ------------------------------------------
@@ -652,8 +767,9 @@ apply_mask_kernel(VGrid g, LabelBuffers b, const uint8_t* __restrict__ mask) {
         if (mask[i]) {
             inside++;
         } else {
+            // trusted combinatorial label: a terminal for every streamline
             b.label[i] = b.label2[i];
-            b.dest[i] = kAssigned;
+            b.dest[i] = kCertainTerminal;
         }
     }
     if (inside) atomicAdd(&b.stats[kStatMasked], inside);
@@ -710,16 +826,33 @@ hipError_t launch_pass1(const VGrid& g, const LabelBuffers& b, int stride, bool 
                        first_level ? 1 : 0, gthr, ethr, max_iters);
     return hipGetLastError();
 }
-hipError_t launch_pass2(const VGrid& g, const LabelBuffers& b, const MList& out, uint32_t* overflow, hipStream_t s) {
-    hipLaunchKernelGGL(pass2_kernel, dim3(grid_for(g.verts(), 8192)), dim3(kThreads), 0, s, g, b, out, overflow);
+hipError_t launch_pass2(const VGrid& g, const LabelBuffers& b, const WorkQueue& q, hipStream_t s) {
+    const int64_t blocks = (g.verts() + kStage - 1) / kStage;
+    hipLaunchKernelGGL(pass2_kernel, dim3((uint32_t)blocks), dim3(kThreads), 0, s, g, b, q);
     return hipGetLastError();
 }
-hipError_t launch_pass3_round(const VGrid& g, const LabelBuffers& b, const MList& in, const MList& out,
-                              uint32_t* clear_counts, uint32_t* overflow, uint32_t round, float gthr, float ethr,
-                              int max_iters, hipStream_t s) {
-    GradSrc<double> src{b.gradf, b.values, b.seedmask, g};
-    hipLaunchKernelGGL(pass3_kernel, dim3(2048), dim3(kThreads), 0, s, src, g, b, in, out, clear_counts, overflow,
-                       round, (double)gthr, (double)ethr, max_iters);
+hipError_t launch_grad_field64(const VGrid& g, const LabelBuffers& b, hipStream_t s) {
+    hipLaunchKernelGGL(grad_field64_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, b.values, g, b.seedmask,
+                       b.gradd);
+    return hipGetLastError();
+}
+namespace {
+// Work-groups of a persistent kernel that are resident at once on the device.
+template <typename K>
+uint32_t resident_blocks(K kernel) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1024;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) return 1024;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kThreads, 0) != hipSuccess || per <= 0) per = 1;
+    return (uint32_t)(cus * per);
+}
+}  // namespace
+hipError_t launch_pass3(const VGrid& g, const LabelBuffers& b, const WorkQueue& q, float gthr, float ethr,
+                        int max_iters, hipStream_t s) {
+    static uint32_t blocks = resident_blocks(pass3_kernel<double>);
+    GradSrc<double> src{b.gradd};
+    hipLaunchKernelGGL(pass3_kernel<double>, dim3(blocks), dim3(kThreads), 0, s, src, g, b, q, (double)gthr,
+                       (double)ethr, max_iters);
     return hipGetLastError();
 }
 hipError_t launch_cc_init(const VGrid& g, const LabelBuffers& b, hipStream_t s) {

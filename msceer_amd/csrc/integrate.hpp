@@ -25,6 +25,7 @@ inline VGrid make_vgrid(int64_t x, int64_t y, int64_t z) {
 struct LabelBuffers {
     const float* values;
     float4* gradf;       // FP32 gradient field
+    double4* gradd;      // FP64 gradient field (pass 3)
     int32_t* label;      // region label per vertex (-1 unassigned)
     int32_t* label2;     // scratch labels
     uint8_t* dest;       // DestType per vertex
@@ -43,7 +44,9 @@ enum IntegStat : int {
     kStatOver3 = 3,      // pass-3 streamlines that ran out of step budget
     kStatChanged = 4,    // sweep / round change flag
     kStatMasked = 5,     // unassigned vertices inside the integration mask
-    kStatNum = 8
+    kStatSteps1 = 8,     // uint64 (two words): pass-1 steps, accepted and rejected
+    kStatSteps3 = 10,    // uint64: pass-3 steps
+    kStatNum = 16
 };
 
 hipError_t launch_grad_field(const VGrid& g, const float* values, float4* gradf, hipStream_t s);
@@ -62,12 +65,23 @@ hipError_t launch_pass1(const VGrid& g, const LabelBuffers& b, int stride, bool 
                         int max_iters, hipStream_t s);
 // Pass 2: every ASSIGNED vertex with a differently labelled All6 neighbour
 // (or flagged for re-tracing) goes to `out`.
-hipError_t launch_pass2(const VGrid& g, const LabelBuffers& b, const MList& out, uint32_t* overflow, hipStream_t s);
+// Re-integration work queue: `items` (capacity entries, 0xFFFFFFFF until
+// written), claim/reserve counters and the count of unfinished items.
+struct WorkQueue {
+    uint32_t* items;
+    uint32_t capacity;
+    uint32_t* head;
+    uint32_t* tail;
+    uint32_t* pending;
+    uint32_t* overflow;
+};
+hipError_t launch_pass2(const VGrid& g, const LabelBuffers& b, const WorkQueue& q, hipStream_t s);
 // One pass-3 round over `in` (FP64 re-integration to the certain regions);
 // changed labels push their ASSIGNED neighbours to `out`; `clear` counts are zeroed.
-hipError_t launch_pass3_round(const VGrid& g, const LabelBuffers& b, const MList& in, const MList& out,
-                              uint32_t* clear_counts, uint32_t* overflow, uint32_t round, float gthr, float ethr,
-                              int max_iters, hipStream_t s);
+hipError_t launch_grad_field64(const VGrid& g, const LabelBuffers& b, hipStream_t s);
+// Persistent FP64 pass 3 over the queue filled by pass 2.
+hipError_t launch_pass3(const VGrid& g, const LabelBuffers& b, const WorkQueue& q, float gthr, float ethr,
+                        int max_iters, hipStream_t s);
 // Cleanup of isolated components.
 hipError_t launch_cc_init(const VGrid& g, const LabelBuffers& b, hipStream_t s);
 hipError_t launch_cc_hook(const VGrid& g, const LabelBuffers& b, uint32_t* changed, hipStream_t s);

@@ -166,6 +166,7 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
         d_grad = ctx->grad.as<uint8_t>();
     }
     CHECK_HIP(ctx->gradf.ensure(sizeof(float4) * V));
+    CHECK_HIP(ctx->gradd.ensure(sizeof(double4) * V));
     CHECK_HIP(ctx->label.ensure(sizeof(int32_t) * V));
     CHECK_HIP(ctx->label2.ensure(sizeof(int32_t) * V));
     CHECK_HIP(ctx->rep.ensure(sizeof(int32_t) * V));
@@ -183,6 +184,7 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
     LabelBuffers b;
     b.values = d_values;
     b.gradf = ctx->gradf.as<float4>();
+    b.gradd = ctx->gradd.as<double4>();
     b.label = ctx->label.as<int32_t>();
     b.label2 = ctx->label2.as<int32_t>();
     b.dest = ctx->dest.as<uint8_t>();
@@ -319,34 +321,31 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
     while (top * 2 <= 8 && top * 2 < max_dim) top *= 2;
     for (int stride = top; stride >= 1; stride /= 2)
         CHECK_HIP(launch_pass1(g, b, stride, stride == top, grad_thr, err_thr, max_iters, s));
-    // pass 2 + pass 3 rounds
-    auto front = [&](int k) { return front_list(ctx, caps, k); };
-    auto counts = [&](int k) { return front(k).counts; };
-    CHECK_HIP(launch_zero(counts(0), kCountWords, s));
+    // pass 2 fills the re-integration queue, pass 3 drains it in one launch
+    const WorkQueue q{ctx->front0.as<uint32_t>(), (uint32_t)V, ctl + kCtlCursor, ctl + kCtlCursor + 1,
+                      ctl + kCtlCursor + 2, ctl + kCtlOverflow};
+    CHECK_HIP(launch_fill_u32(q.items, V, 0xFFFFFFFFu, s));
+    CHECK_HIP(launch_zero(ctl + kCtlCursor, 3, s));
     CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
-    CHECK_HIP(launch_pass2(g, b, front(0), ctl + kCtlOverflow, s));
-    int k = 1;
-    CHECK_HIP(launch_zero(counts(1), kCountWords, s));
-    int rounds = 0;
-    for (int guard = 0; guard < 100000; guard++) {
-        constexpr int kGroup = 4;
-        for (int j = 0; j < kGroup; j++) {
-            CHECK_HIP(launch_pass3_round(g, b, front(k - 1), front(k), counts(k + 1), ctl + kCtlOverflow,
-                                         (uint32_t)k, grad_thr, err_thr, max_iters, s));
-            k++;
-            rounds++;
-        }
-        rc = read_ctl(ctx, 0, kCtlWords);
-        if (rc) return rc;
-        if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;
-        if (list_total(ctx->h_ctl + kCtlRound + ((k - 1) % 3) * kCountWords, front(k - 1)) == 0) break;
-    }
+    CHECK_HIP(launch_pass2(g, b, q, s));
+    CHECK_HIP(launch_grad_field64(g, b, s));
+    CHECK_HIP(launch_pass3(g, b, q, grad_thr, err_thr, max_iters, s));
+    const int rounds = 1;
+    rc = read_ctl(ctx, 0, kCtlWords);
+    if (rc) return rc;
+    if (ctx->h_ctl[kCtlOverflow] & 2u) return MSCEER_EHIP;  // the queue stalled (never expected)
+    if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;
     st.pass3_rounds = rounds;
     st.traced_fp32 = ctx->h_ctl[kCtlIntStats + kStatTraced1];
     st.over_budget_fp32 = ctx->h_ctl[kCtlIntStats + kStatOver1];
     st.reintegrated_fp64 = ctx->h_ctl[kCtlIntStats + kStatTraced3];
     st.over_budget_fp64 = ctx->h_ctl[kCtlIntStats + kStatOver3];
     st.integrated_vertices = d_mask ? (int64_t)ctx->h_ctl[kCtlIntStats + kStatMasked] : V;
+    auto u64 = [&](int k) {
+        return (int64_t)((uint64_t)ctx->h_ctl[kCtlIntStats + k] | ((uint64_t)ctx->h_ctl[kCtlIntStats + k + 1] << 32));
+    };
+    st.steps_fp32 = u64(kStatSteps1);
+    st.steps_fp64 = u64(kStatSteps3);
     CHECK_HIP(hipEventRecord(ev[3], s));
     if (flags & MSCEER_STOP_INTEGRATED) return finish_early();
 

@@ -68,6 +68,8 @@ class LabelStats(ctypes.Structure):
         ("ms_cleanup", ctypes.c_float),
         ("ms_grad", ctypes.c_float),
         ("ms_total", ctypes.c_float),
+        ("steps_fp32", ctypes.c_int64),
+        ("steps_fp64", ctypes.c_int64),
     ]
 
     def as_dict(self):
