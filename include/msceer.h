@@ -64,7 +64,7 @@ typedef struct msceer_label_stats {
     int64_t reintegrated_fp64;  /* pass-3 re-integrations (FP64) */
     int64_t over_budget_fp64;   /* of those, stopped by the step budget */
     int64_t integrated_vertices;/* vertices the learned mask sent to integration (all if none) */
-    int32_t certain_sweeps;     /* sweeps of the certain-region fixed point */
+    int32_t certain_sweeps;     /* rounds of the certain-region resolution (dependency depth) */
     int32_t pass3_rounds;       /* launches of the FP64 re-integration (one persistent launch) */
     float ms_lstar;             /* base + constrained lower-star kernels */
     float ms_regions;           /* extremum graph + certain regions */

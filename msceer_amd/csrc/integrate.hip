@@ -73,41 +73,7 @@ __global__ void fill_u8_kernel(uint8_t* p, int64_t n, uint8_t v) {
 
 // ---- certain regions --------------------------------------------------------
 
-// One sweep of the region-growing fixed point: an unlabelled vertex takes
-// the label shared by all of its earlier All6 neighbours, if they all carry
-// one.  A label, once set, is final (it depends on earlier vertices only),
-// so sweeps may read each other's writes.
-__global__ void __launch_bounds__(kThreads)
-certain_sweep_kernel(const float* __restrict__ f, VGrid g, int32_t* __restrict__ label, uint32_t* __restrict__ changed) {
-    const int64_t n = g.verts();
-    bool any = false;
-    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-        if (label[i] != -1) continue;
-        const int x = (int)(i % g.X), y = (int)((i / g.X) % g.Y), z = (int)(i / g.sxy);
-        int64_t nb[6];
-        int k = 0;
-        if (x < g.X - 1) nb[k++] = i + 1;
-        if (x > 0) nb[k++] = i - 1;
-        if (y < g.Y - 1) nb[k++] = i + g.X;
-        if (y > 0) nb[k++] = i - g.X;
-        if (z < g.Z - 1) nb[k++] = i + g.sxy;
-        if (z > 0) nb[k++] = i - g.sxy;
-        int32_t val = -1;
-        bool has = false, ok = true;
-        for (int j = 0; j < k; j++) {
-            if (!before(f, nb[j], i)) continue;
-            const int32_t l = __hip_atomic_load(&label[nb[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (l == -1) { ok = false; break; }
-            if (!has) { val = l; has = true; }
-            else if (val != l) { ok = false; break; }
-        }
-        if (ok && has) {
-            __hip_atomic_store(&label[i], val, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            any = true;
-        }
-    }
-    if (any) atomicOr(changed, 1u);
-}
+// (The certain-region kernels follow the work-list helpers below.)
 
 // ---- streamline integration -------------------------------------------------
 
@@ -350,6 +316,82 @@ __device__ __forceinline__ int all6(const VGrid& g, int64_t i, int64_t nb[6]) {
     return k;
 }
 
+// ---- certain regions (GridSimplifiedExtremalRegionBuilder::BeginIntegration) ----
+//
+// A vertex takes the label shared by all of its earlier All6 neighbours if
+// they all carry one (gi_extrema_region_builder.h:806-857); the region seeds
+// are labelled from the start.  On the DAG "earlier neighbour -> vertex" this
+// has one solution, computed here in dependency order: a vertex is resolved
+// once its last unresolved earlier neighbour is, so every vertex is visited
+// once instead of once per sweep of a fixed-point iteration.  cnt[v] counts
+// the unresolved earlier neighbours of a non-seed vertex.
+
+__global__ void __launch_bounds__(kThreads)
+certain_init_kernel(const float* __restrict__ f, VGrid g, const uint8_t* __restrict__ seedmask,
+                    uint32_t* __restrict__ cnt, MList out, uint32_t* __restrict__ overflow) {
+    __shared__ uint32_t stage[kStage];
+    __shared__ uint32_t s_n;
+    __shared__ MSlot s_base;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    const Outbox box{stage, &s_n, out, (int)(blockIdx.x % kSubs), overflow};
+    const int64_t n = g.verts();
+    for (int64_t v = (int64_t)blockIdx.x * kThreads + threadIdx.x; v < n; v += (int64_t)gridDim.x * kThreads) {
+        if (seedmask[v]) {
+            cnt[v] = 0u;
+            continue;
+        }
+        int64_t nb[6];
+        const int k = all6(g, v, nb);
+        uint32_t c = 0;
+        for (int j = 0; j < k; j++)
+            if (before(f, nb[j], v) && !seedmask[nb[j]]) c++;
+        cnt[v] = c;
+        if (c == 0) box.push((uint32_t)v);
+    }
+    __syncthreads();
+    mlist_flush_block(out, box.sub, stage, s_n < (uint32_t)kStage ? s_n : (uint32_t)kStage, &s_base, overflow);
+}
+
+// One round: resolve the ready vertices, release their later neighbours.
+// Block 0 clears the counts of the list the next round appends to.
+__global__ void __launch_bounds__(kThreads)
+certain_round_kernel(const float* __restrict__ f, VGrid g, const uint8_t* __restrict__ seedmask,
+                     int32_t* __restrict__ label, uint32_t* __restrict__ cnt, MList in, MList out,
+                     uint32_t* __restrict__ clear, uint32_t* __restrict__ overflow) {
+    __shared__ uint32_t stage[kStage];
+    __shared__ uint32_t prefix[kSubs + 2];
+    __shared__ uint32_t s_n;
+    __shared__ MSlot s_base;
+    if (blockIdx.x == 0)
+        for (uint32_t j = threadIdx.x; j < (uint32_t)kCountWords; j += kThreads) clear[j] = 0u;
+    if (threadIdx.x == 0) s_n = 0;
+    const uint32_t total = mlist_prefix(in, prefix);
+    const Outbox box{stage, &s_n, out, (int)(blockIdx.x % kSubs), overflow};
+    for (uint32_t it = blockIdx.x * kThreads + threadIdx.x; it < total; it += gridDim.x * kThreads) {
+        const int64_t v = mlist_at(in, prefix, it);
+        int64_t nb[6];
+        const int k = all6(g, v, nb);
+        int32_t val = -1;
+        bool has = false, ok = true;
+        for (int j = 0; j < k; j++) {
+            if (!before(f, nb[j], v)) continue;
+            const int32_t l = label[nb[j]];
+            if (l == -1) { ok = false; break; }
+            if (!has) { val = l; has = true; }
+            else if (val != l) { ok = false; break; }
+        }
+        if (ok && has) label[v] = val;
+        for (int j = 0; j < k; j++) {
+            const int64_t u = nb[j];
+            if (seedmask[u] || !before(f, v, u)) continue;
+            if (atomicSub(&cnt[u], 1u) == 1u) box.push((uint32_t)u);
+        }
+    }
+    __syncthreads();
+    mlist_flush_block(out, box.sub, stage, s_n < (uint32_t)kStage ? s_n : (uint32_t)kStage, &s_base, overflow);
+}
+
 // Pass 2: every assigned vertex next to a different label, or flagged for
 // re-tracing, is queued (and stamped as queued).  One reservation per block.
 __global__ void __launch_bounds__(kThreads)
@@ -542,31 +584,70 @@ __global__ void cc_init_kernel(VGrid g, LabelBuffers b) {
     }
 }
 
-// Same-label components; a component is represented by its earliest vertex
-// (the root of the reference's union-find, whose merges point the later
-// representative at the earlier one).  Min-propagation plus pointer jumping.
+// Same-label All6 components by concurrent union-find.  Roots are always
+// hooked under the earlier of the two roots, so a component's root is its
+// earliest vertex, as in the reference's union-find (whose merges point the
+// later representative at the earlier one).  Finds halve paths with plain
+// stores: a racing store only ever points a vertex at one of its ancestors.
+__device__ __forceinline__ int32_t uf_parent(int32_t* rep, int32_t x) {
+    return __hip_atomic_load(&rep[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ int32_t uf_find(int32_t* rep, int32_t x) {
+    int32_t p = uf_parent(rep, x);
+    while (p != x) {
+        const int32_t gp = uf_parent(rep, p);
+        if (gp != p) __hip_atomic_store(&rep[x], gp, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        x = p;
+        p = gp;
+    }
+    return x;
+}
+
+__device__ void uf_union(const float* f, int32_t* rep, int32_t a, int32_t b) {
+    while (true) {
+        a = uf_find(rep, a);
+        b = uf_find(rep, b);
+        if (a == b) return;
+        if (before(f, b, a)) {  // hook the later root a under b
+            const int32_t t = a; a = b; b = t;
+        }
+        // b is the later root now
+        int32_t expected = b;
+        if (__hip_atomic_compare_exchange_strong(&rep[b], &expected, a, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT))
+            return;
+    }
+}
+
+// Each edge once: +x, +y, +z neighbours with the same label.
 __global__ void __launch_bounds__(kThreads)
-cc_hook_kernel(const float* __restrict__ f, VGrid g, LabelBuffers b, uint32_t* __restrict__ changed) {
+cc_union_kernel(const float* __restrict__ f, VGrid g, LabelBuffers b) {
     const int64_t n = g.verts();
-    bool any = false;
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
         const int32_t l = b.label2[i];
-        int32_t best = b.rep[i];
-        int64_t nb[6];
-        const int k = all6(g, i, nb);
-        for (int j = 0; j < k; j++) {
-            if (b.label2[nb[j]] != l) continue;
-            const int32_t r = b.rep[nb[j]];
-            if (before(f, r, best)) best = r;
-        }
-        const int32_t jump = b.rep[best];
-        if (before(f, jump, best)) best = jump;
-        if (best != b.rep[i]) {
-            b.rep[i] = best;
-            any = true;
-        }
+        const int x = (int)(i % g.X), y = (int)((i / g.X) % g.Y), z = (int)(i / g.sxy);
+        if (x < g.X - 1 && b.label2[i + 1] == l) uf_union(f, b.rep, (int32_t)i, (int32_t)(i + 1));
+        if (y < g.Y - 1 && b.label2[i + g.X] == l) uf_union(f, b.rep, (int32_t)i, (int32_t)(i + g.X));
+        if (z < g.Z - 1 && b.label2[i + g.sxy] == l) uf_union(f, b.rep, (int32_t)i, (int32_t)(i + g.sxy));
     }
-    if (any) atomicOr(changed, 1u);
+}
+
+// After the unions: rep[i] = root.  No path halving here: a halving store
+// racing with thread i's own store could leave rep[i] at a non-root ancestor.
+// Only thread i writes rep[i]; other threads' walks see either the old
+// ancestor or the root, both on i's path.
+__global__ void __launch_bounds__(kThreads)
+cc_flatten_kernel(VGrid g, LabelBuffers b) {
+    const int64_t n = g.verts();
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        int32_t x = (int32_t)i, p = uf_parent(b.rep, x);
+        while (p != x) {
+            x = p;
+            p = uf_parent(b.rep, x);
+        }
+        __hip_atomic_store(&b.rep[i], x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
 }
 
 // Component roots in a compact list (any order): root vertex, its label and
@@ -810,8 +891,17 @@ hipError_t launch_seed_scatter(const int32_t* seed_vert, const int32_t* seed_lab
                        seed_label, n, b.label, b.gradf, b.seedmask);
     return hipGetLastError();
 }
-hipError_t launch_certain_sweep(const VGrid& g, const float* values, int32_t* label, uint32_t* changed, hipStream_t s) {
-    hipLaunchKernelGGL(certain_sweep_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, values, g, label, changed);
+hipError_t launch_certain_init(const VGrid& g, const float* values, const uint8_t* seedmask, uint32_t* cnt,
+                               const MList& out, uint32_t* overflow, hipStream_t s) {
+    hipLaunchKernelGGL(certain_init_kernel, dim3(grid_for(g.verts(), 8192)), dim3(kThreads), 0, s, values, g,
+                       seedmask, cnt, out, overflow);
+    return hipGetLastError();
+}
+hipError_t launch_certain_round(const VGrid& g, const float* values, const uint8_t* seedmask, int32_t* label,
+                                uint32_t* cnt, const MList& in, const MList& out, uint32_t* clear_counts,
+                                uint32_t* overflow, hipStream_t s) {
+    hipLaunchKernelGGL(certain_round_kernel, dim3(1024), dim3(kThreads), 0, s, values, g, seedmask, label, cnt, in,
+                       out, clear_counts, overflow);
     return hipGetLastError();
 }
 hipError_t launch_dest_init(const VGrid& g, const LabelBuffers& b, hipStream_t s) {
@@ -859,8 +949,10 @@ hipError_t launch_cc_init(const VGrid& g, const LabelBuffers& b, hipStream_t s) 
     hipLaunchKernelGGL(cc_init_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, g, b);
     return hipGetLastError();
 }
-hipError_t launch_cc_hook(const VGrid& g, const LabelBuffers& b, uint32_t* changed, hipStream_t s) {
-    hipLaunchKernelGGL(cc_hook_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, b.values, g, b, changed);
+hipError_t launch_cc_union(const VGrid& g, const LabelBuffers& b, hipStream_t s) {
+    hipLaunchKernelGGL(cc_union_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, b.values, g, b);
+    if (hipGetLastError() != hipSuccess) return hipGetLastError();
+    hipLaunchKernelGGL(cc_flatten_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, g, b);
     return hipGetLastError();
 }
 hipError_t launch_cc_collect(const VGrid& g, const LabelBuffers& b, int32_t* out, uint32_t cap, uint32_t* count,

@@ -57,7 +57,14 @@ hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 hipError_t launch_seed_scatter(const int32_t* seed_vert, const int32_t* seed_label, int n, const LabelBuffers& b,
                                hipStream_t s);
 // One sweep of the certain-region fixed point; sets *changed when a label was set.
-hipError_t launch_certain_sweep(const VGrid& g, const float* values, int32_t* label, uint32_t* changed, hipStream_t s);
+// Certain regions in dependency order: init queues the vertices with no
+// unresolved earlier neighbour, each round resolves its list and queues the
+// vertices it releases (cnt: one uint32 per vertex).
+hipError_t launch_certain_init(const VGrid& g, const float* values, const uint8_t* seedmask, uint32_t* cnt,
+                               const MList& out, uint32_t* overflow, hipStream_t s);
+hipError_t launch_certain_round(const VGrid& g, const float* values, const uint8_t* seedmask, int32_t* label,
+                                uint32_t* cnt, const MList& in, const MList& out, uint32_t* clear_counts,
+                                uint32_t* overflow, hipStream_t s);
 // dest = UNASSIGNED where label == -1 else CERTAIN_TERMINAL; redo = 0; stamp = 0.
 hipError_t launch_dest_init(const VGrid& g, const LabelBuffers& b, hipStream_t s);
 // Pass 1 at one level of the coarse-to-fine order.
@@ -84,7 +91,8 @@ hipError_t launch_pass3(const VGrid& g, const LabelBuffers& b, const WorkQueue& 
                         int max_iters, hipStream_t s);
 // Cleanup of isolated components.
 hipError_t launch_cc_init(const VGrid& g, const LabelBuffers& b, hipStream_t s);
-hipError_t launch_cc_hook(const VGrid& g, const LabelBuffers& b, uint32_t* changed, hipStream_t s);
+// Same-label components (rep[] = earliest vertex of each component).
+hipError_t launch_cc_union(const VGrid& g, const LabelBuffers& b, hipStream_t s);
 // ints per collected component root: root, label, six earlier neighbour roots
 constexpr int kCollectStride = 8;
 hipError_t launch_cc_collect(const VGrid& g, const LabelBuffers& b, int32_t* out, uint32_t cap, uint32_t* count,

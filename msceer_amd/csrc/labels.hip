@@ -66,9 +66,8 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 int run_cleanup(msceer_ctx* ctx, const VGrid& g, const LabelBuffers& b) {
     hipStream_t s = ctx->stream;
     CHECK_HIP(launch_cc_init(g, b, s));
-    int it = 0;
-    int rc = iterate_until_stable(ctx, 4, 1 << 16, &it, [&](uint32_t* flag) { return launch_cc_hook(g, b, flag, s); });
-    if (rc) return rc;
+    CHECK_HIP(launch_cc_union(g, b, s));
+    int rc = MSCEER_OK;
     uint32_t* count = ctl_dev(ctx) + kCtlCount;
     const size_t row = sizeof(int32_t) * kCollectStride;
     uint32_t cap = (uint32_t)std::min<int64_t>(g.verts(), std::max<int64_t>(4096, g.verts() / 64));
@@ -292,10 +291,30 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
         CHECK_HIP(hipMemcpyAsync(d_sl, seed_l.data(), sizeof(int32_t) * seed_l.size(), hipMemcpyHostToDevice, s));
         CHECK_HIP(launch_seed_scatter(d_sv, d_sl, (int)seed_v.size(), b, s));
     }
+    // rounds in dependency order, enqueued in groups with device-side counts
     int sweeps = 0;
-    rc = iterate_until_stable(ctx, 8, 1 << 16, &sweeps,
-                              [&](uint32_t* flag) { return launch_certain_sweep(g, d_values, b.label, flag, s); });
-    if (rc) return rc;
+    {
+        auto front = [&](int k) { return front_list(ctx, caps, k); };
+        CHECK_HIP(launch_zero(front(0).counts, kCountWords, s));
+        CHECK_HIP(launch_zero(front(1).counts, kCountWords, s));
+        CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
+        CHECK_HIP(launch_certain_init(g, d_values, b.seedmask, b.stamp, front(0), ctl + kCtlOverflow, s));
+        int k = 0;
+        for (int guard = 0;; guard++) {
+            if (guard > (int)(V / 16 + 16)) return MSCEER_EHIP;  // a round resolves at least one vertex
+            constexpr int kGroup = 16;
+            for (int j = 0; j < kGroup; j++) {
+                CHECK_HIP(launch_certain_round(g, d_values, b.seedmask, b.label, b.stamp, front(k), front(k + 1),
+                                               front(k + 2).counts, ctl + kCtlOverflow, s));
+                k++;
+            }
+            rc = read_ctl(ctx, 0, kCtlWords);
+            if (rc) return rc;
+            if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;
+            if (list_total(ctx->h_ctl + kCtlRound + (k % 3) * kCountWords, front(k)) == 0) break;
+        }
+        sweeps = k;
+    }
     st.certain_sweeps = sweeps;
     CHECK_HIP(hipEventRecord(ev[2], s));
     auto finish_early = [&]() -> int {
