@@ -146,6 +146,39 @@ int msceer_restriction_labels(msceer_ctx* ctx, const int32_t* labels, int64_t nx
 int msceer_constrained_lstar(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz,
                              const int8_t* restriction, uint8_t* grad_inout, uint32_t flags);
 
+/*
+ * Near-separatrix classifier (learned integration mask).  No counterpart in
+ * the reference: a small 3-D CNN flags the vertices whose accurate label may
+ * differ from the combinatorial one, and msceer_ondemand_asc integrates only
+ * those (integrate_mask).  Network: the values normalised over each vertex's
+ * 9x9x9 window, three 3x3x3 convolutions (1->16->32->16, ReLU; BF16 weights
+ * and activations, FP32 accumulation), a 16->1 sigmoid head.  Parameters are
+ * a flat FP32 vector of msceer_cnn_num_params() entries.
+ */
+typedef struct msceer_cnn msceer_cnn;
+int msceer_cnn_num_params(void);
+/* He-initialised network from a seed (SplitMix64 + Box-Muller). */
+int msceer_cnn_create(msceer_ctx* ctx, uint64_t seed, msceer_cnn** net);
+int msceer_cnn_destroy(msceer_cnn* net);
+int msceer_cnn_get_params(msceer_ctx* ctx, msceer_cnn* net, float* params_out);
+/* Replaces the parameters and resets the optimiser state. */
+int msceer_cnn_set_params(msceer_ctx* ctx, msceer_cnn* net, const float* params);
+/* Per-vertex probability of lying near a separatrix (X*Y*Z floats). */
+int msceer_cnn_predict(msceer_ctx* ctx, msceer_cnn* net, const float* values, int64_t nx, int64_t ny, int64_t nz,
+                       float* prob_out, uint32_t flags);
+/* Mean weighted binary cross-entropy against target (X*Y*Z bytes, nonzero =
+ * near a separatrix; positives weighted pos_weight) and its gradient with
+ * respect to the parameters (grad_out may be NULL; the gradient is kept for
+ * msceer_cnn_adam_step). */
+int msceer_cnn_gradient(msceer_ctx* ctx, msceer_cnn* net, const float* values, int64_t nx, int64_t ny, int64_t nz,
+                        const uint8_t* target, float pos_weight, float* loss_out, float* grad_out, uint32_t flags);
+/* One Adam step (beta 0.9 / 0.999, eps 1e-8) with the last gradient. */
+int msceer_cnn_adam_step(msceer_ctx* ctx, msceer_cnn* net, float lr);
+/* Integration mask: probability > threshold, dilated by `dilation` vertices
+ * (Chebyshev distance); flagged (may be NULL) receives the mask's count. */
+int msceer_cnn_mask(msceer_ctx* ctx, msceer_cnn* net, const float* values, int64_t nx, int64_t ny, int64_t nz,
+                    float threshold, int32_t dilation, uint8_t* mask_out, int64_t* flagged, uint32_t flags);
+
 #ifdef __cplusplus
 }
 #endif

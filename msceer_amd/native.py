@@ -37,6 +37,15 @@ SIGNATURES = {
     "msceer_clean_labels": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _U32]),
     "msceer_restriction_labels": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _U32]),
     "msceer_constrained_lstar": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _P, _U32]),
+    "msceer_cnn_num_params": (ctypes.c_int, []),
+    "msceer_cnn_create": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(_P)]),
+    "msceer_cnn_destroy": (ctypes.c_int, [_P]),
+    "msceer_cnn_get_params": (ctypes.c_int, [_P, _P, _P]),
+    "msceer_cnn_set_params": (ctypes.c_int, [_P, _P, _P]),
+    "msceer_cnn_predict": (ctypes.c_int, [_P, _P, _P, _I64, _I64, _I64, _P, _U32]),
+    "msceer_cnn_gradient": (ctypes.c_int, [_P, _P, _P, _I64, _I64, _I64, _P, ctypes.c_float, _P, _P, _U32]),
+    "msceer_cnn_adam_step": (ctypes.c_int, [_P, _P, ctypes.c_float]),
+    "msceer_cnn_mask": (ctypes.c_int, [_P, _P, _P, _I64, _I64, _I64, ctypes.c_float, _I32, _P, _P, _U32]),
 }
 
 

@@ -168,3 +168,53 @@ def ondemand_asc(f, err=1e-3, gthr=1e-6, iters=4096, presimp=0.0, mode=0):
     st = np.zeros(8, np.int64)
     _integ_sigs().oracle_ondemand_asc(_p(f), X, Y, Z, err, gthr, iters, presimp, mode, _p(g), _p(lab), _p(r), _p(st))
     return g, lab, r, st
+
+
+# ---- near-separatrix classifier ----------------------------------------------
+def _cnn_sigs():
+    h = lib()
+    if getattr(h, "_cnn_ready", False):
+        return h
+    h.oracle_cnn_num_params.restype = ctypes.c_int
+    h.oracle_cnn_normalize.argtypes = [ctypes.c_void_p, ctypes.c_longlong, ctypes.c_longlong, ctypes.c_longlong,
+                                       ctypes.c_void_p]
+    h.oracle_cnn_predict.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_longlong,
+                                     ctypes.c_longlong, ctypes.c_int, ctypes.c_void_p]
+    h.oracle_cnn_gradient.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_longlong,
+                                      ctypes.c_longlong, ctypes.c_void_p, ctypes.c_double, ctypes.c_int,
+                                      ctypes.c_void_p, ctypes.c_void_p]
+    h._cnn_ready = True
+    return h
+
+
+def cnn_num_params():
+    return _cnn_sigs().oracle_cnn_num_params()
+
+
+def cnn_normalize(f):
+    f = np.ascontiguousarray(f, np.float32)
+    Z, Y, X = f.shape
+    out = np.empty_like(f)
+    _cnn_sigs().oracle_cnn_normalize(_p(f), X, Y, Z, _p(out))
+    return out
+
+
+def cnn_predict(params, f, quant=1):
+    f = np.ascontiguousarray(f, np.float32)
+    p = np.ascontiguousarray(params, np.float32)
+    Z, Y, X = f.shape
+    out = np.empty(f.shape, np.float64)
+    _cnn_sigs().oracle_cnn_predict(_p(p), _p(f), X, Y, Z, quant, _p(out))
+    return out
+
+
+def cnn_gradient(params, f, target, pos_w=1.0, quant=1):
+    """(mean weighted BCE, gradient (float64))."""
+    f = np.ascontiguousarray(f, np.float32)
+    p = np.ascontiguousarray(params, np.float32)
+    t = np.ascontiguousarray(target, np.uint8)
+    Z, Y, X = f.shape
+    loss = np.zeros(1, np.float64)
+    grad = np.zeros(cnn_num_params(), np.float64)
+    _cnn_sigs().oracle_cnn_gradient(_p(p), _p(f), X, Y, Z, _p(t), float(pos_w), quant, _p(loss), _p(grad))
+    return float(loss[0]), grad

@@ -1,0 +1,101 @@
+"""GPU kernels of the near-separatrix classifier against the CPU oracle
+(FP64 with BF16 rounding where the kernels store BF16), then training and the
+learned labelling.
+
+Tolerances: probabilities 3e-3 absolute (FP32 accumulation order over 432 to
+864 products per output, the odd BF16 rounding flip); gradients per
+parameter block: relative L2 error below 3e-2 and cosine above 0.999
+(gradients pass through two more BF16 roundings)."""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib
+from msceer_amd import learned, native, synth
+from test_oracle_cnn import BLOCKS, _params
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = native.Context(0)
+    yield c
+    c.close()
+
+
+@pytest.fixture(scope="module")
+def net(ctx):
+    n = learned.SeparatrixNet(ctx, seed=3)
+    yield n
+    n.close()
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+# x extents that leave partial 32- and 64-voxel tiles; a one-vertex-thick axis
+SHAPES = [(9, 11, 13), (5, 7, 70), (3, 40, 33), (12, 1, 17)]
+
+
+@pytest.mark.parametrize("shape", SHAPES)
+def test_predict_matches_oracle(net, shape):
+    _say(f"cnn predict {shape}")
+    f = synth.smoothed_noise(shape, 21, 1.5).astype(np.float32)
+    p = _params(31)
+    net.set_params(p)
+    got = net.predict(f)
+    ref = oracle_lib.cnn_predict(p, f, quant=1)
+    err = float(np.abs(got - ref).max())
+    _say(f"  max |p_gpu - p_oracle| = {err:.2e}")
+    assert err < 3e-3
+
+
+@pytest.mark.parametrize("shape", SHAPES[:3])
+def test_gradient_matches_oracle(net, shape):
+    _say(f"cnn gradient {shape}")
+    f = synth.smoothed_noise(shape, 22, 2.0).astype(np.float32)
+    t = (np.random.default_rng(4).random(shape) < 0.25).astype(np.uint8)
+    p = _params(32)
+    net.set_params(p)
+    loss, g = net.gradient(f, t, pos_weight=3.0)
+    lref, gref = oracle_lib.cnn_gradient(p, f, t, 3.0, quant=1)
+    assert abs(loss - lref) < 1e-4 * max(1.0, lref)
+    for name, (o, n) in BLOCKS.items():
+        a, b = g[o:o + n].astype(np.float64), gref[o:o + n]
+        nb = np.linalg.norm(b)
+        if nb == 0:
+            assert np.linalg.norm(a) < 1e-12
+            continue
+        rel = np.linalg.norm(a - b) / nb
+        cos = float(np.dot(a, b) / (np.linalg.norm(a) * nb))
+        _say(f"  {name}: rel {rel:.2e} cos {cos:.6f}")
+        assert rel < 3e-2 and cos > 0.999, name
+
+
+def test_training_reduces_loss(ctx):
+    n = learned.SeparatrixNet(ctx, seed=5)
+    f = learned.training_volume(1, 48)
+    acc, comb = learned.label_pair(f, ctx)
+    t = learned.separatrix_target(acc, comb)
+    l0, _ = n.gradient(f, t, 4.0, want_grad=False)
+    hist = learned.train(n, [f], [t], epochs=30, lr=3e-3, pos_weight=4.0)
+    l1, _ = n.gradient(f, t, 4.0, want_grad=False)
+    _say(f"  loss {l0:.4f} -> {l1:.4f} (target fraction {t.mean():.3f})")
+    assert l1 < 0.8 * l0
+    n.close()
+
+
+def test_mask_is_thresholded_and_dilated(net):
+    f = synth.smoothed_noise((10, 12, 14), 23, 1.5).astype(np.float32)
+    net.set_params(_params(33))
+    prob = net.predict(f)
+    thr = float(np.quantile(prob, 0.9))
+    m, cnt = net.mask(f, threshold=thr, dilation=1)
+    core = prob > thr
+    from scipy import ndimage
+    expect = ndimage.binary_dilation(core, structure=np.ones((3, 3, 3), bool))
+    assert (m.astype(bool) == expect).all() and cnt == int(expect.sum())
