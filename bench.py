@@ -172,6 +172,9 @@ def bench_labels(args):
                 "stage_ms": {k[3:]: round(float(st.as_dict()[k]), 3) for k in
                              ("ms_lstar", "ms_regions", "ms_integrate", "ms_cleanup", "ms_grad")},
                 "fp64": int(st.reintegrated_fp64)}
+    learned_out = None
+    if not args.no_learned:
+        learned_out = bench_learned(args, ctx, lib, dv, dg, dl, st, X, Y, Z, ref, plain, run)
     base = None
     if not args.no_cpu:
         log("timing the labelling CPU baseline")
@@ -180,7 +183,59 @@ def bench_labels(args):
     return {"workload": f"ondemandaccurate needASC2 + conforming .grad, {X}x{Y}x{Z} float32, 200 Gaussians "
                         f"(sigma 4-16) + 0.05 sin sin sin, seed 303; err 1e-3 (squared error), grad 1e-6, "
                         f"4096 steps, presimp 0; values resident in HBM",
-            "plain": plain, "learned": None, "cpu_baseline": base}
+            "plain": plain, "learned": learned_out, "cpu_baseline": base}
+
+
+CNN_FLOP_PER_VOXEL = 2 * (27 * 16 + 27 * 16 * 32 + 27 * 32 * 16 + 16)  # 56 288
+
+
+def bench_learned(args, ctx, lib, dv, dg, dl, st, X, Y, Z, ref, plain, run):
+    """Learned mask (trained weights, msceer_amd/weights) + masked labelling."""
+    from msceer_amd import hipmem, learned, native
+
+    if not os.path.exists(learned.WEIGHTS):
+        return {"error": f"{learned.WEIGHTS} missing (python -m msceer_amd.train_sepnet)"}
+    V = X * Y * Z
+    net = learned.SeparatrixNet(ctx)
+    meta = net.load()
+    dm = hipmem.DeviceBuffer(V)
+    cnt = ctypes.c_int64()
+
+    def mask():
+        native.check(lib.msceer_cnn_mask(ctx.handle, net._h, ctypes.c_void_p(dv.addr), X, Y, Z,
+                                         float(net.threshold), int(net.dilation), ctypes.c_void_p(dm.addr),
+                                         ctypes.byref(cnt), native.DEVICE_PTRS), "msceer_cnn_mask")
+
+    mask()
+    run(mask_addr=dm.addr)
+    hipmem.synchronize()
+    cnn_ms, tot_ms = [], []
+    for _ in range(args.label_steps):
+        t0 = time.perf_counter()
+        mask()
+        t1 = time.perf_counter()
+        run(mask_addr=dm.addr)
+        hipmem.synchronize()
+        t2 = time.perf_counter()
+        cnn_ms.append((t1 - t0) * 1e3)
+        tot_ms.append((t2 - t0) * 1e3)
+    lab = dl.download(np.int32, V)
+    ms = float(np.mean(tot_ms))
+    c_ms = float(np.mean(cnn_ms))
+    tflops = CNN_FLOP_PER_VOXEL * V / (c_ms * 1e-3) / 1e12
+    net.close()
+    return {"value": round(V / (ms * 1e-3) / 1e6, 2), "unit": "Mvert/s", "ms_per_step": round(ms, 3),
+            "speedup_vs_plain": round(plain["ms_per_step"] / ms, 3),
+            "mismatch_vs_fp64_oracle": float((lab != ref).mean()),
+            "mask_fraction": cnt.value / V, "threshold": net.threshold, "dilation": net.dilation,
+            "cnn_ms": round(c_ms, 3),
+            "cnn_roofline": {"bound": "mfma", "achieved": round(tflops, 2), "peak": 2500.0, "unit": "TFLOP/s",
+                             "frac": round(tflops / 2500.0, 5),
+                             "note": "normalisation + 3 convolutions + head + threshold + dilation, "
+                                     f"{CNN_FLOP_PER_VOXEL} flop/voxel"},
+            "integrated_vertices": int(st.integrated_vertices),
+            "training": {"volumes": int(meta["volumes"]), "size": int(meta["size"]), "epochs": int(meta["epochs"]),
+                         "seconds_on_gpu": round(float(meta["train_seconds"]), 2)}}
 
 
 def calibrate_copy():
@@ -209,6 +264,7 @@ def main():
     ap.add_argument("--label-warmup", type=int, default=1)
     ap.add_argument("--label-cpu-slab", type=int, default=48)
     ap.add_argument("--ideal-mask-probe", action="store_true")
+    ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB read + 1 GiB write streaming kernel (torch), the "
                          "reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE")

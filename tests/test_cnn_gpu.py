@@ -99,3 +99,26 @@ def test_mask_is_thresholded_and_dilated(net):
     from scipy import ndimage
     expect = ndimage.binary_dilation(core, structure=np.ones((3, 3, 3), bool))
     assert (m.astype(bool) == expect).all() and cnt == int(expect.sum())
+
+
+def test_trained_network_beats_a_random_mask_of_equal_size(ctx):
+    """What the trained network gains over the method without it: at equal
+    integration budget (mask size), its mask leaves fewer wrong labels than an
+    uninformed mask; and at the calibrated threshold its labels stay within
+    0.1% of the full integrator's on a held-out volume."""
+    f = learned.training_volume(69, 96).astype(np.float32)  # held out (training used 0..63)
+    net = learned.SeparatrixNet(ctx)
+    net.load()
+    _, plain, _ = learned.labels.ondemand_asc(f, want_grad=False, ctx=ctx)
+    m, cnt = net.mask(f, threshold=0.4)
+    _, lab, _ = learned.labels.ondemand_asc(f, mask=m, want_grad=False, ctx=ctx)
+    rnd = (np.random.default_rng(0).random(f.shape) < cnt / f.size).astype(np.uint8)
+    _, lab_r, _ = learned.labels.ondemand_asc(f, mask=rnd, want_grad=False, ctx=ctx)
+    e_net, e_rnd = float((lab != plain).mean()), float((lab_r != plain).mean())
+    _say(f"  mask {cnt / f.size:.3f}: mismatch learned {e_net:.4f} vs random {e_rnd:.4f}")
+    assert e_net < 0.8 * e_rnd
+    _, lab_c, _, frac = learned.learned_labels(f, net, ctx, want_grad=False)
+    e_cal = float((lab_c != plain).mean())
+    _say(f"  calibrated threshold {net.threshold}: mask {frac:.3f}, mismatch {e_cal:.5f}")
+    assert e_cal <= 1e-3
+    net.close()
