@@ -86,6 +86,14 @@ int msceer_destroy(msceer_ctx* ctx);
  * context's own stream. */
 int msceer_set_stream(msceer_ctx* ctx, void* hip_stream);
 int msceer_last_timing(const msceer_ctx* ctx, msceer_timing* out);
+/* Lower-star brick autotuning (no counterpart in the reference, which has no
+ * GPU launch shape): mode 1 (default) times the four brick shapes 64x2x2,
+ * 32x4x2, 16x4x4, 8x8x4 on the first volume of each x extent and keeps the
+ * fastest; mode 0 always uses 64x2x2, mode 2+k always shape k.  All shapes
+ * give identical bytes. */
+int msceer_set_autotune(msceer_ctx* ctx, int32_t mode);
+/* Brick shape (0..3 as above) used for x extent nx; -1 while not yet tuned. */
+int msceer_lstar_brick(msceer_ctx* ctx, int64_t nx);
 
 /*
  * Full steepest_lstar .grad of a volume: one byte per cell, LSB first
@@ -130,6 +138,27 @@ int msceer_lstar_grad(msceer_ctx* ctx, const float* values, int64_t nx, int64_t 
 int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz, float err_thr,
                         float grad_thr, int32_t max_iters, float presimp, const uint8_t* integrate_mask,
                         uint8_t* grad_out, int32_t* labels_out, msceer_label_stats* stats, uint32_t flags);
+
+/*
+ * Batch of independent volumes (no counterpart in the reference, which
+ * processes one volume per program run): for each volume, the lower-star
+ * .grad (msceer_lstar_grad) when labels_out is NULL, else the accurate
+ * labelling and conforming .grad (msceer_ondemand_asc with the shared
+ * parameters; grad_out may then be NULL).  `streams` worker threads (0: 8)
+ * each drive a context of their own on the same GPU and take the volumes
+ * largest first; the lower-star brick tuning is shared.  Host buffers only.
+ * Per volume, ms receives the wall time of its pipeline and status its code.
+ */
+typedef struct msceer_volume {
+    const float* values;
+    int64_t nx, ny, nz;
+    uint8_t* grad_out;
+    int32_t* labels_out;
+    float ms;
+    int32_t status;
+} msceer_volume;
+int msceer_batch(msceer_ctx* ctx, msceer_volume* vols, int32_t n, int32_t streams, float err_thr, float grad_thr,
+                 int32_t max_iters, float presimp, uint32_t flags);
 
 /* Stages of the accurate pipeline as separate calls (host or device pointers):
  * IsolatedCCRegionRemoverNEW::ComputeOutput (gi_isolated_region_remover.h:155-192),

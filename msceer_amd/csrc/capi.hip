@@ -106,6 +106,46 @@ int ldir_pass(msceer_ctx* ctx, const Dims& d, uint8_t* grad, uint32_t item_dim, 
 // Critical lists: a quarter of the vertices per sub-list set, spill for as
 // many again as there are vertices; frontiers: the vertex count in sub-lists
 // and as much again in spill.  Entries are cell ids (< 2^32).
+int lstar_shape(msceer_ctx* ctx, const Dims& d, const LstarBuffers& b, int* shape) {
+    Tuning& t = *ctx->tuning;
+    {
+        std::lock_guard<std::mutex> lk(t.mu);
+        if (t.mode != 1) {
+            *shape = t.mode == 0 ? 0 : t.mode - 2;
+            return MSCEER_OK;
+        }
+        auto it = t.brick_by_x.find(d.X);
+        if (it != t.brick_by_x.end()) {
+            *shape = it->second;
+            return MSCEER_OK;
+        }
+    }
+    // time every shape twice on this volume, keep the fastest second run
+    float best = 1e30f;
+    int pick = 0;
+    for (int sh = 0; sh < kLstarBricks; sh++) {
+        float ms = 0.f;
+        for (int rep = 0; rep < 2; rep++) {
+            CHECK_HIP(launch_zero(ctl_dev(ctx), kCtlWords, ctx->stream));
+            CHECK_HIP(hipEventRecord(ctx->ev[10], ctx->stream));
+            CHECK_HIP(launch_lstar(d, b, ctx->stream, sh));
+            CHECK_HIP(hipEventRecord(ctx->ev[11], ctx->stream));
+            CHECK_HIP(hipEventSynchronize(ctx->ev[11]));
+            CHECK_HIP(hipEventElapsedTime(&ms, ctx->ev[10], ctx->ev[11]));
+        }
+        if (ms < best) {
+            best = ms;
+            pick = sh;
+        }
+    }
+    {
+        std::lock_guard<std::mutex> lk(t.mu);
+        t.brick_by_x[d.X] = pick;
+    }
+    *shape = pick;
+    return MSCEER_OK;
+}
+
 int ensure_lists(msceer_ctx* ctx, const Dims& d, ListCaps* caps) {
     const uint64_t list_cap = std::min<uint64_t>(kMaxListCap, std::max<uint64_t>(kMinListCap, d.verts / 4));
     const uint64_t front_cap = std::min<uint64_t>(kMaxFrontCap, std::max<uint64_t>(kMinListCap, d.verts));
@@ -135,6 +175,21 @@ int run_ldir(msceer_ctx* ctx, const Dims& d, uint8_t* grad, const ListCaps& caps
 extern "C" {
 
 int msceer_version(void) { return MSCEER_VERSION; }
+
+int msceer_set_autotune(msceer_ctx* ctx, int32_t mode) {
+    if (!ctx || mode < 0 || mode > 1 + kLstarBricks) return MSCEER_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->tuning->mu);
+    ctx->tuning->mode = mode;
+    return MSCEER_OK;
+}
+
+int msceer_lstar_brick(msceer_ctx* ctx, int64_t nx) {
+    if (!ctx) return MSCEER_EINVAL;
+    std::lock_guard<std::mutex> lk(ctx->tuning->mu);
+    if (ctx->tuning->mode != 1) return ctx->tuning->mode == 0 ? 0 : ctx->tuning->mode - 2;
+    auto it = ctx->tuning->brick_by_x.find((int)nx);
+    return it == ctx->tuning->brick_by_x.end() ? -1 : it->second;
+}
 
 const char* msceer_strerror(int code) {
     switch (code) {
@@ -173,6 +228,8 @@ int msceer_create(msceer_ctx** out, int device) {
 
 int msceer_destroy(msceer_ctx* ctx) {
     if (!ctx) return MSCEER_EINVAL;
+    for (msceer_ctx* w : ctx->workers) msceer_destroy(w);
+    ctx->workers.clear();
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     ctx->values.release();
@@ -233,11 +290,14 @@ int msceer_lstar_grad(msceer_ctx* ctx, const float* values, int64_t nx, int64_t 
     uint32_t* ctl = ctl_dev(ctx);
     const MList list1 = crit_list(ctx, caps, 0), list2 = crit_list(ctx, caps, 1);
 
+    LstarBuffers b{d_values, d_grad, nullptr, &list1, &list2, ctl + kCtlStats, ctl + kCtlOverflow};
+    int shape = 0;
+    rc = lstar_shape(ctx, d, b, &shape);
+    if (rc) return rc;
     if (timing) CHECK_HIP(hipEventRecord(ctx->ev[0], s));
     CHECK_HIP(launch_zero(ctl, kCtlWords, s));
-    LstarBuffers b{d_values, d_grad, nullptr, &list1, &list2, ctl + kCtlStats, ctl + kCtlOverflow};
     if (timing) CHECK_HIP(hipEventRecord(ctx->ev[1], s));
-    CHECK_HIP(launch_lstar(d, b, s));
+    CHECK_HIP(launch_lstar(d, b, s, shape));
     if (timing) CHECK_HIP(hipEventRecord(ctx->ev[2], s));
     rc = read_ctl(ctx, 0, kCtlWords);
     if (rc) return rc;

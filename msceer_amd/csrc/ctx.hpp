@@ -4,6 +4,11 @@
 
 #include <stdint.h>
 
+#include <map>
+#include <memory>
+#include <mutex>
+#include <vector>
+
 #include "msceer.h"
 #include "internal.hpp"
 #include "mlist.hpp"
@@ -58,6 +63,14 @@ struct ListCaps {
     uint32_t front_sub, front_spill;
 };
 
+// Lower-star brick shapes chosen per x extent (shared by a context and the
+// worker contexts of its batches).
+struct Tuning {
+    std::mutex mu;
+    int mode = 1;                       // 0: fixed 64x2x2; 1: measured per x extent; 2+k: shape k
+    std::map<int, int> brick_by_x;      // x extent -> brick shape
+};
+
 int hip_fail(hipError_t e);
 int check_dims(int64_t nx, int64_t ny, int64_t nz);
 
@@ -74,6 +87,8 @@ struct msceer_ctx {
     hipEvent_t ev[12] = {};
     msceer_timing last{};
     msceer_label_stats last_labels{};
+    std::shared_ptr<msceer::Tuning> tuning = std::make_shared<msceer::Tuning>();
+    std::vector<msceer_ctx*> workers;  // contexts of msceer_batch's worker threads
 };
 
 namespace msceer {
@@ -90,6 +105,10 @@ MList crit_list(msceer_ctx* ctx, const ListCaps& c, int which);
 MList front_list(msceer_ctx* ctx, const ListCaps& c, int k);
 int ldir_pass(msceer_ctx* ctx, const Dims& d, uint8_t* grad, uint32_t item_dim, uint32_t value,
               const MList& start, bool start_valid, const ListCaps& caps, int* rounds);
+// Brick shape for the lower-star kernel on this volume: tuned on first use of
+// an x extent (times every shape on the given buffers; call before the
+// control block is cleared for the real launch).
+int lstar_shape(msceer_ctx* ctx, const Dims& d, const LstarBuffers& b, int* shape);
 // ldir passes + tail shadow on a .grad whose lstar pass filled the critical lists.
 int run_ldir(msceer_ctx* ctx, const Dims& d, uint8_t* grad, const ListCaps& caps, bool lists_ok, int* rounds);
 

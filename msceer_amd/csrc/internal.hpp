@@ -36,7 +36,11 @@ struct LstarBuffers {
 
 // Lower-star pairing of every vertex; writes every cell byte exactly once
 // with assigned/flag/pair set and ldir preset to 3 (0 for critical 3-cells).
-hipError_t launch_lstar(const Dims& d, const LstarBuffers& b, hipStream_t s);
+// Lower-star kernel with brick shape `shape` (0: 64x2x2, 1: 32x4x2,
+// 2: 16x4x4, 3: 8x8x4 vertices); every shape gives the same bytes.
+constexpr int kLstarBricks = 4;
+const char* lstar_brick_name(int shape);
+hipError_t launch_lstar(const Dims& d, const LstarBuffers& b, hipStream_t s, int shape = 0);
 
 // ldir passes.  `tail` is one device uint32 shadowing the bytes of the last,
 // partial 32-bit word of the grad array: pack it before the passes

@@ -201,10 +201,13 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
     }
 
     // ---- base gradient ----
+    LstarBuffers lb{d_values, d_grad, nullptr, &list1, &list2, ctl + kCtlStats, ctl + kCtlOverflow};
+    int shape = 0;
+    rc = lstar_shape(ctx, d, lb, &shape);
+    if (rc) return rc;
     CHECK_HIP(hipEventRecord(ev[0], s));
     CHECK_HIP(launch_zero(ctl, kCtlWords, s));
-    LstarBuffers lb{d_values, d_grad, nullptr, &list1, &list2, ctl + kCtlStats, ctl + kCtlOverflow};
-    CHECK_HIP(launch_lstar(d, lb, s));
+    CHECK_HIP(launch_lstar(d, lb, s, shape));
     CHECK_HIP(hipEventRecord(ev[1], s));
 
     // ---- extremum graph (critical vertices and 1-saddles) ----
@@ -381,7 +384,7 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
         CHECK_HIP(launch_zero(ctl + kCtlList1, 2 * kCountWords, s));
         CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
         LstarBuffers cb{d_values, d_grad, restr, &list1, &list2, ctl + kCtlStats, ctl + kCtlOverflow};
-        CHECK_HIP(launch_lstar(d, cb, s));
+        CHECK_HIP(launch_lstar(d, cb, s, shape));
         CHECK_HIP(hipEventRecord(ev[6], s));
         rc = read_ctl(ctx, kCtlOverflow, 1);
         if (rc) return rc;
@@ -522,9 +525,16 @@ extern "C" int msceer_constrained_lstar(msceer_ctx* ctx, const float* values, in
     if (rc) return rc;
     uint32_t* ctl = ctl_dev(ctx);
     const MList list1 = crit_list(ctx, caps, 0), list2 = crit_list(ctx, caps, 1);
-    CHECK_HIP(launch_zero(ctl, kCtlWords, s));
     LstarBuffers cb{d_values, d_grad, d_r, &list1, &list2, ctl + kCtlStats, ctl + kCtlOverflow};
-    CHECK_HIP(launch_lstar(d, cb, s));
+    int shape = 0;
+    {
+        std::lock_guard<std::mutex> lk(ctx->tuning->mu);
+        auto it = ctx->tuning->brick_by_x.find(d.X);
+        if (ctx->tuning->mode == 1 && it != ctx->tuning->brick_by_x.end()) shape = it->second;
+        else if (ctx->tuning->mode > 1) shape = ctx->tuning->mode - 2;
+    }
+    CHECK_HIP(launch_zero(ctl, kCtlWords, s));
+    CHECK_HIP(launch_lstar(d, cb, s, shape));
     if (!(flags & MSCEER_SKIP_LDIR)) {
         rc = read_ctl(ctx, kCtlOverflow, 1);
         if (rc) return rc;

@@ -25,16 +25,25 @@ namespace msceer {
 
 namespace {
 
-constexpr int kBX = 64, kBY = 2, kBZ = 2;           // vertices per brick
-constexpr int kTX = kBX + 2, kTY = kBY + 2, kTZ = kBZ + 2;
-constexpr int kTileN = kTX * kTY * kTZ;
-constexpr int kThreads = kBX * kBY * kBZ;
+constexpr int kThreads = 256;  // vertices per brick, any of the shapes below
 
-template <bool kConstrained>
+// Brick shapes (vertices along x, y, z); the autotuner picks one per volume
+// shape: wide bricks coalesce best, narrow ones waste fewer lanes when the
+// x extent is small or not a multiple of the brick width.
+template <int kBX, int kBY, int kBZ>
+struct Brick {
+    static_assert(kBX * kBY * kBZ == kThreads, "256 lanes");
+    static constexpr int kTX = kBX + 2, kTY = kBY + 2, kTZ = kBZ + 2;
+    static constexpr int kTileN = kTX * kTY * kTZ;
+};
+
+template <bool kConstrained, int kBX, int kBY, int kBZ>
 __global__ void __launch_bounds__(kThreads)
 lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
              const int8_t* __restrict__ restriction, int X, int Y, int Z, uint32_t Mx, uint32_t My,
              MList list1, MList list2, uint32_t* __restrict__ stats, uint32_t* __restrict__ overflow) {
+    using B = Brick<kBX, kBY, kBZ>;
+    constexpr int kTX = B::kTX, kTY = B::kTY, kTZ = B::kTZ, kTileN = B::kTileN;
     __shared__ uint32_t tile[kTileN];
     __shared__ uint32_t slice[27 * kThreads];  // each lane's 27 neighbour keys, t-major
     __shared__ uint32_t s_cnt[4];  // per block: critical 1-cells, 2-cells, 0-cells, 3-cells
@@ -129,15 +138,33 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
 
 }  // namespace
 
-hipError_t launch_lstar(const Dims& d, const LstarBuffers& b, hipStream_t s) {
-    dim3 block(kBX, kBY, kBZ);
-    dim3 grid((d.X + kBX - 1) / kBX, (d.Y + kBY - 1) / kBY, (d.Z + kBZ - 1) / kBZ);
-    if (b.restriction) {
-        hipLaunchKernelGGL(lstar_kernel<true>, grid, block, 0, s, b.values, b.grad, b.restriction, d.X,
-                           d.Y, d.Z, d.Mx, d.My, *b.list1, *b.list2, b.stats, b.overflow);
-    } else {
-        hipLaunchKernelGGL(lstar_kernel<false>, grid, block, 0, s, b.values, b.grad, b.restriction,
+namespace {
+
+template <int kBX, int kBY, int kBZ>
+void launch_shape(const Dims& d, const LstarBuffers& b, hipStream_t s) {
+    const dim3 block(kBX, kBY, kBZ);
+    const dim3 grid((d.X + kBX - 1) / kBX, (d.Y + kBY - 1) / kBY, (d.Z + kBZ - 1) / kBZ);
+    if (b.restriction)
+        hipLaunchKernelGGL((lstar_kernel<true, kBX, kBY, kBZ>), grid, block, 0, s, b.values, b.grad, b.restriction,
                            d.X, d.Y, d.Z, d.Mx, d.My, *b.list1, *b.list2, b.stats, b.overflow);
+    else
+        hipLaunchKernelGGL((lstar_kernel<false, kBX, kBY, kBZ>), grid, block, 0, s, b.values, b.grad, b.restriction,
+                           d.X, d.Y, d.Z, d.Mx, d.My, *b.list1, *b.list2, b.stats, b.overflow);
+}
+
+}  // namespace
+
+const char* lstar_brick_name(int shape) {
+    static const char* names[kLstarBricks] = {"64x2x2", "32x4x2", "16x4x4", "8x8x4"};
+    return shape >= 0 && shape < kLstarBricks ? names[shape] : "?";
+}
+
+hipError_t launch_lstar(const Dims& d, const LstarBuffers& b, hipStream_t s, int shape) {
+    switch (shape) {
+        case 1: launch_shape<32, 4, 2>(d, b, s); break;
+        case 2: launch_shape<16, 4, 4>(d, b, s); break;
+        case 3: launch_shape<8, 8, 4>(d, b, s); break;
+        default: launch_shape<64, 2, 2>(d, b, s); break;
     }
     return hipGetLastError();
 }

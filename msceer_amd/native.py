@@ -31,12 +31,15 @@ SIGNATURES = {
     "msceer_destroy": (ctypes.c_int, [_P]),
     "msceer_set_stream": (ctypes.c_int, [_P, _P]),
     "msceer_last_timing": (ctypes.c_int, [_P, _P]),
+    "msceer_set_autotune": (ctypes.c_int, [_P, _I32]),
+    "msceer_lstar_brick": (ctypes.c_int, [_P, _I64]),
     "msceer_lstar_grad": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _U32]),
     "msceer_ondemand_asc": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, ctypes.c_float, ctypes.c_float, _I32,
                                            ctypes.c_float, _P, _P, _P, _P, _U32]),
     "msceer_clean_labels": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _U32]),
     "msceer_restriction_labels": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _U32]),
     "msceer_constrained_lstar": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _P, _U32]),
+    "msceer_batch": (ctypes.c_int, [_P, _P, _I32, _I32, ctypes.c_float, ctypes.c_float, _I32, ctypes.c_float, _U32]),
     "msceer_cnn_num_params": (ctypes.c_int, []),
     "msceer_cnn_create": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(_P)]),
     "msceer_cnn_destroy": (ctypes.c_int, [_P]),
@@ -83,6 +86,19 @@ class LabelStats(ctypes.Structure):
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class Volume(ctypes.Structure):
+    _fields_ = [
+        ("values", ctypes.c_void_p),
+        ("nx", ctypes.c_int64),
+        ("ny", ctypes.c_int64),
+        ("nz", ctypes.c_int64),
+        ("grad_out", ctypes.c_void_p),
+        ("labels_out", ctypes.c_void_p),
+        ("ms", ctypes.c_float),
+        ("status", ctypes.c_int32),
+    ]
 
 
 class MsceerError(RuntimeError):

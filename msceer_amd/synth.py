@@ -112,3 +112,16 @@ def labels_field(seed: int = 303, n: int = 256) -> np.ndarray:
 
 def sha256(a: np.ndarray) -> str:
     return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def batch_volume(i: int) -> np.ndarray:
+    """Volume i of the batched workload (seed 4000 + i): sides uniform in
+    32..160 per axis; smoothed noise (sigma 1.5-3) or a Gaussian mixture
+    (200 Gaussians per 256^3, sigma 4-16, with the sine term), 50/50."""
+    s = Stream(4000 + i)
+    u = s.uniform(5)
+    shape = tuple(int(32 + min(128, int(129 * float(u[k])))) for k in range(3))  # (z, y, x)
+    if u[3] < 0.5:
+        return smoothed_noise(shape, 4000 + i, 1.5 + 1.5 * float(u[4]))
+    n = max(4, int(round(200 * shape[0] * shape[1] * shape[2] / 256 ** 3)))
+    return gaussian_mixture(shape, 4000 + i, n, 4.0, 16.0, -1.0, 1.0, sine=0.05)
