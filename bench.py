@@ -238,6 +238,60 @@ def bench_learned(args, ctx, lib, dv, dg, dl, st, X, Y, Z, ref, plain, run):
                          "seconds_on_gpu": round(float(meta["train_seconds"]), 2)}}
 
 
+def _batch_volume(i):
+    from msceer_amd import synth
+    return synth.batch_volume(i).astype(np.float32)
+
+
+def bench_batch(args):
+    """512 independent volumes (sides 32..160, smoothed noise / Gaussian
+    mixtures): lower-star .grad for all, then accurate labels + conforming
+    .grad for all, through msceer_batch; aggregate vertices/s and per-volume
+    time percentiles."""
+    from concurrent.futures import ProcessPoolExecutor
+
+    from msceer_amd import batch, native
+
+    n = args.batch_volumes
+    log(f"generating {n} batch volumes")
+    with ProcessPoolExecutor(max_workers=min(16, _cores())) as ex:
+        vols = list(ex.map(_batch_volume, range(n), chunksize=8))
+    V = int(sum(v.size for v in vols))
+    ctx = native.Context(0)
+    err, gthr, iters, presimp = 1e-3, 1e-6, 4096, 0.0
+    out = {"workload": f"{n} volumes, sides uniform 32..160 per axis (seed 4000+i), 50/50 smoothed noise "
+                       f"(sigma 1.5-3) / Gaussian mixtures (sigma 4-16, sine term); host buffers in and out",
+           "vertices": V}
+
+    def timed(labels, streams):
+        t0 = time.perf_counter()
+        _, _, ms = batch.run_batch(vols, labels=labels, streams=streams, err=err, gthr=gthr, iters=iters,
+                                   presimp=presimp, ctx=ctx)
+        return time.perf_counter() - t0, ms
+
+    def summary(wall, ms, streams):
+        return {"value": round(V / wall / 1e6, 2), "unit": "Mvert/s", "seconds": round(wall, 3), "streams": streams,
+                "per_volume_ms": {"p50": round(float(np.percentile(ms, 50)), 3),
+                                  "p90": round(float(np.percentile(ms, 90)), 3),
+                                  "p99": round(float(np.percentile(ms, 99)), 3),
+                                  "max": round(float(ms.max()), 3)}}
+
+    timed(False, 8)  # brick tuning for every x extent, buffers grown
+    out["grad"] = summary(*timed(False, 8), 8)
+    out["grad_one_stream"] = summary(*timed(False, 1), 1)
+    timed(True, 8)
+    out["labels"] = summary(*timed(True, 8), 8)
+    lib = native.lib()
+    tuned = {}
+    for x in sorted({int(v.shape[2]) for v in vols}):
+        sh = lib.msceer_lstar_brick(ctx.handle, x)
+        tuned[native.BRICKS[sh] if 0 <= sh < len(native.BRICKS) else "untuned"] = \
+            tuned.get(native.BRICKS[sh] if 0 <= sh < len(native.BRICKS) else "untuned", 0) + 1
+    out["brick_choice_by_x_extent"] = tuned
+    ctx.close()
+    return out
+
+
 def calibrate_copy():
     import torch
 
@@ -265,6 +319,9 @@ def main():
     ap.add_argument("--label-cpu-slab", type=int, default=48)
     ap.add_argument("--ideal-mask-probe", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-only", action="store_true")
+    ap.add_argument("--batch-volumes", type=int, default=512)
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB read + 1 GiB write streaming kernel (torch), the "
                          "reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -279,6 +336,9 @@ def main():
 
     if args.labels_only:
         print(json.dumps({"labels": bench_labels(args)}), flush=True)
+        return
+    if args.batch_only:
+        print(json.dumps({"batch": bench_batch(args)}), flush=True)
         return
     from msceer_amd import hipmem, native, synth
 
@@ -355,6 +415,8 @@ def main():
     ctx.close()
     if not args.no_labels:
         out["labels"] = bench_labels(args)
+    if not args.no_batch:
+        out["batch"] = bench_batch(args)
     print(json.dumps(out), flush=True)
 
 

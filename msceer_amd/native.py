@@ -16,6 +16,7 @@ SKIP_LDIR = 0x2
 TIMING = 0x4
 STOP_REGIONS = 0x8
 STOP_INTEGRATED = 0x10
+BRICKS = ("64x2x2", "32x4x2", "16x4x4", "8x8x4")  # lower-star brick shapes by index
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
