@@ -292,6 +292,69 @@ def bench_batch(args):
     return out
 
 
+def bench_learned_large(args):
+    """The learned mask on a 1024x768x512 smoothed-noise volume (sigma 2.5,
+    seed 5999) against the plain full GPU integrator: speed and label
+    mismatch (learned vs plain)."""
+    from msceer_amd import hipmem, learned, native, synth
+
+    log("generating 1024x768x512 smoothed noise (sigma 2.5, seed 5999)")
+    f = synth.smoothed_noise((512, 768, 1024), 5999, 2.5)
+    Z, Y, X = f.shape
+    V = X * Y * Z
+    cells = (2 * X - 1) * (2 * Y - 1) * (2 * Z - 1)
+    ctx = native.Context(0)
+    lib = native.lib()
+    dv = hipmem.DeviceBuffer(f.nbytes)
+    dv.upload(f)
+    del f
+    dg = hipmem.DeviceBuffer(cells)
+    dl = hipmem.DeviceBuffer(4 * V)
+    dm = hipmem.DeviceBuffer(V)
+    st = native.LabelStats()
+    err, gthr, iters, presimp = 1e-3, 1e-6, 4096, 0.0
+
+    def run(mask=None):
+        native.check(lib.msceer_ondemand_asc(ctx.handle, ctypes.c_void_p(dv.addr), X, Y, Z, err, gthr, iters, presimp,
+                                             ctypes.c_void_p(mask) if mask else None, ctypes.c_void_p(dg.addr),
+                                             ctypes.c_void_p(dl.addr), ctypes.byref(st), native.DEVICE_PTRS),
+                     "msceer_ondemand_asc")
+        hipmem.synchronize()
+
+    log("plain integrator (warm-up)")
+    run()
+    log("plain integrator")
+    t0 = time.perf_counter()
+    run()
+    plain_s = time.perf_counter() - t0
+    plain_stats = st.as_dict()
+    plain = dl.download(np.int32, V)
+    net = learned.SeparatrixNet(ctx)
+    net.load()
+    cnt = ctypes.c_int64()
+    log("learned mask + masked integration")
+    t0 = time.perf_counter()
+    native.check(lib.msceer_cnn_mask(ctx.handle, net._h, ctypes.c_void_p(dv.addr), X, Y, Z, float(net.threshold),
+                                     int(net.dilation), ctypes.c_void_p(dm.addr), ctypes.byref(cnt),
+                                     native.DEVICE_PTRS), "msceer_cnn_mask")
+    hipmem.synchronize()
+    cnn_s = time.perf_counter() - t0
+    run(dm.addr)
+    learned_s = time.perf_counter() - t0
+    lab = dl.download(np.int32, V)
+    net.close()
+    ctx.close()
+    return {"workload": f"{X}x{Y}x{Z} smoothed noise sigma 2.5 seed 5999; accurate labels + conforming .grad",
+            "plain": {"seconds": round(plain_s, 3), "value": round(V / plain_s / 1e6, 2), "unit": "Mvert/s",
+                      "stage_ms": {k[3:]: round(float(plain_stats[k]), 1) for k in
+                                   ("ms_lstar", "ms_regions", "ms_integrate", "ms_cleanup", "ms_grad")}},
+            "learned": {"seconds": round(learned_s, 3), "value": round(V / learned_s / 1e6, 2), "unit": "Mvert/s",
+                        "cnn_seconds": round(cnn_s, 3), "mask_fraction": cnt.value / V,
+                        "threshold": net.threshold, "dilation": net.dilation,
+                        "mismatch_vs_plain": float((lab != plain).mean()),
+                        "speedup_vs_plain": round(plain_s / learned_s, 3)}}
+
+
 def calibrate_copy():
     import torch
 
@@ -322,6 +385,8 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--batch-volumes", type=int, default=512)
+    ap.add_argument("--learned-large", action="store_true",
+                    help="only the learned-mask comparison on the 1024x768x512 volume")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB read + 1 GiB write streaming kernel (torch), the "
                          "reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -336,6 +401,9 @@ def main():
 
     if args.labels_only:
         print(json.dumps({"labels": bench_labels(args)}), flush=True)
+        return
+    if args.learned_large:
+        print(json.dumps({"learned_large": bench_learned_large(args)}), flush=True)
         return
     if args.batch_only:
         print(json.dumps({"batch": bench_batch(args)}), flush=True)
