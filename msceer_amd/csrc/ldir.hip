@@ -61,11 +61,18 @@ __device__ __forceinline__ int64_t dir_offset(const CellGeom& g, uint32_t dir) {
     return (dir & 1u) ? -a : a;
 }
 
-// Set the ldir field of byte c to value unless it already holds it.
+// Set the ldir field of byte c to value unless it already holds it; true if
+// this call set it.  Pass 1 (value 2) only ever moves ldir 3 -> 2, one
+// cleared bit: a single atomic AND decides.  Pass 2 (value 1) moves 3 or 2
+// -> 1 and takes a compare-and-swap.
 __device__ __forceinline__ bool claim(const GradView& v, uint32_t c, uint32_t value) {
     const bool in_tail = c >= v.tail_start;
     uint32_t* w = in_tail ? v.tail : reinterpret_cast<uint32_t*>(v.grad) + (c >> 2);
     const uint32_t sh = (in_tail ? (c - v.tail_start) : (c & 3u)) * 8u + 5u;
+    if (value == 2u) {
+        const uint32_t old = atomicAnd(w, ~(1u << sh));
+        return ((old >> sh) & 7u) == 3u;
+    }
     uint32_t old = __hip_atomic_load(w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     while (true) {
         if (((old >> sh) & 7u) == value) return false;
@@ -73,6 +80,20 @@ __device__ __forceinline__ bool claim(const GradView& v, uint32_t c, uint32_t va
         const uint32_t prev = atomicCAS(w, old, nw);
         if (prev == old) return true;
         old = prev;
+    }
+}
+
+// Set the ldir of an item (3 -> 2 in pass 1; 3 or 2 -> 1 in pass 2) without
+// waiting for the result.
+__device__ __forceinline__ void set_ldir(const GradView& v, uint32_t c, uint32_t value) {
+    const bool in_tail = c >= v.tail_start;
+    uint32_t* w = in_tail ? v.tail : reinterpret_cast<uint32_t*>(v.grad) + (c >> 2);
+    const uint32_t sh = (in_tail ? (c - v.tail_start) : (c & 3u)) * 8u + 5u;
+    if (value == 2u) {
+        atomicAnd(w, ~(1u << sh));
+    } else {
+        atomicAnd(w, ~(2u << sh));
+        atomicOr(w, 1u << sh);
     }
 }
 
@@ -145,13 +166,13 @@ ldir_round_kernel(GradView v, CellGeom g, MList in, MList out, uint32_t* __restr
             const uint32_t pos[3] = {x, y, z};
             const uint32_t ext[3] = {g.Mx, g.My, g.Mz};
             const int64_t step_off[3] = {1, (int64_t)g.Mx, (int64_t)g.sxy};
-            // the item itself: set its ldir (its other bits never change)
-            if (item >= v.tail_start) {
-                claim(v, item, value);
-            } else {
-                const uint32_t b = v.grad[item];
-                v.grad[item] = (uint8_t)((b & 0x1Fu) | (value << 5));
-            }
+            // the item itself: set its ldir with non-returning atomics on its
+            // word (its other bits never change; the passes never read an
+            // item's own ldir, so the intermediate 0 of 2 -> 1 is harmless)
+            set_ldir(v, item, value);
+            // its cofacets: issue the (independent) byte loads first
+            uint32_t tc[4], tb[4];
+            int nt = 0;
 #pragma unroll
             for (int a = 0; a < 3; a++) {
                 if (pos[a] & 1u) continue;  // cofacets lie along even axes
@@ -159,8 +180,18 @@ ldir_round_kernel(GradView v, CellGeom g, MList in, MList out, uint32_t* __restr
                 for (int sgn = 0; sgn < 2; sgn++) {
                     if (sgn == 0 && pos[a] == ext[a] - 1) continue;
                     if (sgn == 1 && pos[a] == 0) continue;
-                    const uint32_t t = (uint32_t)((int64_t)item + (sgn ? -step_off[a] : step_off[a]));
-                    const uint32_t b = v.grad[t];
+                    if (nt < 4) tc[nt++] = (uint32_t)((int64_t)item + (sgn ? -step_off[a] : step_off[a]));
+                }
+            }
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                if (j < nt) tb[j] = v.grad[tc[j]];
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (j >= nt) break;
+                {
+                    const uint32_t t = tc[j];
+                    const uint32_t b = tb[j];
                     const uint32_t dir = (b >> 2) & 7u;
                     if (dir == 7u || !(b & 1u)) continue;
                     if (t < v.tail_start && (b >> 5) == value) continue;  // already marked

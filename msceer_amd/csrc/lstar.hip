@@ -43,7 +43,7 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
              const int8_t* __restrict__ restriction, int X, int Y, int Z, uint32_t Mx, uint32_t My,
              MList list1, MList list2, uint32_t* __restrict__ stats, uint32_t* __restrict__ overflow) {
     using B = Brick<kBX, kBY, kBZ>;
-    constexpr int kTX = B::kTX, kTY = B::kTY, kTZ = B::kTZ, kTileN = B::kTileN;
+    constexpr int kTX = B::kTX, kTY = B::kTY, kTileN = B::kTileN;
     __shared__ uint32_t tile[kTileN];
     __shared__ uint32_t slice[27 * kThreads];  // each lane's 27 neighbour keys, t-major
     __shared__ uint32_t s_cnt[4];  // per block: critical 1-cells, 2-cells, 0-cells, 3-cells
