@@ -24,7 +24,7 @@ namespace msceer {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr int kPool = 8192;         // per-block LDS pool of pending items
+constexpr int kPool = 4096;         // per-block LDS pool of pending items
 constexpr int kStage = 4096;        // per-block LDS staging of outgoing items (sweep)
 constexpr int kMaxSteps = 96;       // block steps per round before the rest is handed on
 constexpr int kRoundBlocks = 2048;  // fixed grid of a round
