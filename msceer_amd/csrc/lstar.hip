@@ -45,7 +45,7 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
     using B = Brick<kBX, kBY, kBZ>;
     constexpr int kTX = B::kTX, kTY = B::kTY, kTileN = B::kTileN;
     __shared__ uint32_t tile[kTileN];
-    __shared__ uint32_t slice[27 * kThreads];  // each lane's 27 neighbour keys, t-major
+    __shared__ int s_off[27];  // tile offset of star vertex t
     __shared__ uint32_t s_cnt[4];  // per block: critical 1-cells, 2-cells, 0-cells, 3-cells
     __shared__ MSlot s_slot[2];
     const int x0 = blockIdx.x * kBX, y0 = blockIdx.y * kBY, z0 = blockIdx.z * kBZ;
@@ -59,6 +59,7 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
         tile[i] = float_key(values[(size_t)gx + (size_t)X * ((size_t)gy + (size_t)Y * gz)]);
     }
     if (flat < 4) s_cnt[flat] = 0;
+    if (flat < 27) s_off[flat] = tid_x(flat) + kTX * (tid_y(flat) + kTY * tid_z(flat));
     __syncthreads();
 
     const int x = x0 + threadIdx.x, y = y0 + threadIdx.y, z = z0 + threadIdx.z;
@@ -69,9 +70,7 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
     const int base = (threadIdx.x + 1) + kTX * ((threadIdx.y + 1) + kTY * (threadIdx.z + 1));
     const FixedKeys<kTX, kTX * kTY> tkeys{tile, base};
     const uint32_t L = lower_star_mask(tkeys, x, y, z, X, Y, Z);
-#pragma unroll
-    for (int t = 0; t < 27; t++) slice[t * kThreads + flat] = tkeys.at(t);
-    const SliceKeys<kThreads> keys{slice + flat};
+    const TableKeys keys{tile + base, s_off};
     uint32_t bx, by, bz;
     star_boundary(x, y, z, X, Y, Z, bx, by, bz);
     const uint32_t onb = bx | by | bz;

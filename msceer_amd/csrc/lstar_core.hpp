@@ -39,6 +39,13 @@ struct SliceKeys {
     const uint32_t* p;
     MSC_HD uint32_t at(int t) const { return p[t * kStride]; }
 };
+// Keys read through a per-star offset table (tile offsets of the 27 star
+// vertices, shared by the work-group) for star indices known only at run time.
+struct TableKeys {
+    const uint32_t* p;   // centre of the lane's star in the tile
+    const int* off;
+    MSC_HD uint32_t at(int t) const { return p[off[t]]; }
+};
 struct StridedKeys {
     const uint32_t* p;
     int64_t base, sx, sxy;
