@@ -27,6 +27,7 @@ constexpr int kThreads = 256;
 constexpr int kPool = 4096;         // per-block LDS pool of pending items
 constexpr int kStage = 4096;        // per-block LDS staging of outgoing items (sweep)
 constexpr int kMaxSteps = 96;       // block steps per round before the rest is handed on
+constexpr int kPerLane = 2;         // items a lane works on per step
 constexpr int kRoundBlocks = 2048;  // fixed grid of a round
 
 struct CellGeom {
@@ -146,7 +147,7 @@ ldir_round_kernel(GradView v, CellGeom g, MList in, MList out, uint32_t* __restr
     for (int step = 0; step < kMaxSteps; step++) {
         uint32_t n = min(s_n, (uint32_t)kPool);
         const uint32_t nx = s_next;
-        if (n < (uint32_t)kThreads && nx < hi) {
+        if (n < (uint32_t)(kPerLane * kThreads) && nx < hi) {
             const uint32_t take = min(hi - nx, min((uint32_t)kPool - n, 4u * kThreads));
             for (uint32_t j = tid; j < take; j += kThreads) pool[n + j] = mlist_at(in, prefix, nx + j);
             __syncthreads();
@@ -155,12 +156,25 @@ ldir_round_kernel(GradView v, CellGeom g, MList in, MList out, uint32_t* __restr
             n += take;
         }
         if (n == 0) break;
-        const uint32_t m = min(n, (uint32_t)kThreads);
-        const uint32_t item = tid < m ? pool[n - 1 - tid] : 0xFFFFFFFFu;
+        // every lane takes up to kPerLane items and keeps their cofacet
+        // loads in flight together
+        const uint32_t m = min(n, (uint32_t)(kPerLane * kThreads));
+        uint32_t items[kPerLane];
+#pragma unroll
+        for (int k = 0; k < kPerLane; k++) {
+            const uint32_t j = tid + (uint32_t)k * kThreads;
+            items[k] = j < m ? pool[n - 1 - j] : 0xFFFFFFFFu;
+        }
         __syncthreads();
         if (tid == 0) s_n = n - m;
         __syncthreads();
-        if (item != 0xFFFFFFFFu) {
+        uint32_t tc[kPerLane][4], tb[kPerLane][4];
+        int nt[kPerLane];
+#pragma unroll
+        for (int k = 0; k < kPerLane; k++) {
+            nt[k] = 0;
+            const uint32_t item = items[k];
+            if (item == 0xFFFFFFFFu) continue;
             uint32_t x, y, z;
             cell_coords(g, item, x, y, z);
             const uint32_t pos[3] = {x, y, z};
@@ -170,9 +184,6 @@ ldir_round_kernel(GradView v, CellGeom g, MList in, MList out, uint32_t* __restr
             // word (its other bits never change; the passes never read an
             // item's own ldir, so the intermediate 0 of 2 -> 1 is harmless)
             set_ldir(v, item, value);
-            // its cofacets: issue the (independent) byte loads first
-            uint32_t tc[4], tb[4];
-            int nt = 0;
 #pragma unroll
             for (int a = 0; a < 3; a++) {
                 if (pos[a] & 1u) continue;  // cofacets lie along even axes
@@ -180,32 +191,36 @@ ldir_round_kernel(GradView v, CellGeom g, MList in, MList out, uint32_t* __restr
                 for (int sgn = 0; sgn < 2; sgn++) {
                     if (sgn == 0 && pos[a] == ext[a] - 1) continue;
                     if (sgn == 1 && pos[a] == 0) continue;
-                    if (nt < 4) tc[nt++] = (uint32_t)((int64_t)item + (sgn ? -step_off[a] : step_off[a]));
+                    if (nt[k] < 4) tc[k][nt[k]++] = (uint32_t)((int64_t)item + (sgn ? -step_off[a] : step_off[a]));
                 }
             }
+        }
+#pragma unroll
+        for (int k = 0; k < kPerLane; k++)
 #pragma unroll
             for (int j = 0; j < 4; j++)
-                if (j < nt) tb[j] = v.grad[tc[j]];
+                if (j < nt[k]) tb[k][j] = v.grad[tc[k][j]];
+#pragma unroll
+        for (int k = 0; k < kPerLane; k++) {
+            const uint32_t item = items[k];
 #pragma unroll
             for (int j = 0; j < 4; j++) {
-                if (j >= nt) break;
-                {
-                    const uint32_t t = tc[j];
-                    const uint32_t b = tb[j];
-                    const uint32_t dir = (b >> 2) & 7u;
-                    if (dir == 7u || !(b & 1u)) continue;
-                    if (t < v.tail_start && (b >> 5) == value) continue;  // already marked
-                    const int64_t p64 = (int64_t)t + dir_offset(g, dir);
-                    if (p64 < 0 || p64 >= (int64_t)g.cells) continue;  // only a corrupt byte points out
-                    const uint32_t p = (uint32_t)p64;
-                    if (p == item) continue;
-                    if (!claim(v, t, value)) continue;
-                    const uint32_t slot = atomicAdd(&s_n, 1u);
-                    if (slot < (uint32_t)kPool) {
-                        pool[slot] = p;
-                    } else {
-                        if (!mlist_push(out, sub, p)) atomicOr(overflow, 1u);
-                    }
+                if (j >= nt[k]) break;
+                const uint32_t t = tc[k][j];
+                const uint32_t b = tb[k][j];
+                const uint32_t dir = (b >> 2) & 7u;
+                if (dir == 7u || !(b & 1u)) continue;
+                if (t < v.tail_start && (b >> 5) == value) continue;  // already marked
+                const int64_t p64 = (int64_t)t + dir_offset(g, dir);
+                if (p64 < 0 || p64 >= (int64_t)g.cells) continue;  // only a corrupt byte points out
+                const uint32_t p = (uint32_t)p64;
+                if (p == item) continue;
+                if (!claim(v, t, value)) continue;
+                const uint32_t slot = atomicAdd(&s_n, 1u);
+                if (slot < (uint32_t)kPool) {
+                    pool[slot] = p;
+                } else {
+                    if (!mlist_push(out, sub, p)) atomicOr(overflow, 1u);
                 }
             }
         }
