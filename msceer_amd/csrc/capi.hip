@@ -238,7 +238,7 @@ int msceer_destroy(msceer_ctx* ctx) {
     ctx->front0.release();
     ctx->front1.release();
     ctx->ctl.release();
-    DevBuf* label_bufs[] = {&ctx->gradf, &ctx->gradd, &ctx->label, &ctx->label2, &ctx->dest, &ctx->seedmask, &ctx->redo,
+    DevBuf* label_bufs[] = {&ctx->gradf, &ctx->gradd, &ctx->scan, &ctx->label, &ctx->label2, &ctx->dest, &ctx->seedmask, &ctx->redo,
                             &ctx->stamp, &ctx->rep, &ctx->rep2, &ctx->restriction, &ctx->host_io, &ctx->mask};
     for (DevBuf* b : label_bufs) b->release();
     if (ctx->h_ctl) (void)hipHostFree(ctx->h_ctl);

@@ -82,7 +82,7 @@ struct msceer_ctx {
     hipStream_t stream = nullptr;
     msceer::DevBuf values, grad, lists, front0, front1, ctl;
     // accurate labelling
-    msceer::DevBuf gradf, gradd, label, label2, dest, seedmask, redo, stamp, rep, rep2, restriction, host_io, mask;
+    msceer::DevBuf gradf, gradd, scan, label, label2, dest, seedmask, redo, stamp, rep, rep2, restriction, host_io, mask;
     uint32_t* h_ctl = nullptr;  // pinned mirror of the control block
     hipEvent_t ev[12] = {};
     msceer_timing last{};

@@ -3,6 +3,7 @@
 // gi_extrema_region_builder.h:28-268).  Inputs are small (one node per
 // critical vertex, one arc per 1-saddle) and the cancellation order is a
 // global priority order, so this runs on the host between GPU stages.
+#include <algorithm>
 #include <cmath>
 #include <queue>
 #include <unordered_map>
@@ -83,27 +84,44 @@ std::vector<int> simplify_minima(const std::vector<MinNode>& nodes, const std::v
     Graph g;
     g.nodes = &nodes;
     g.rep.resize(nodes.size());
+    for (size_t i = 0; i < nodes.size(); i++) g.rep[i] = (int)i;
+    // node index of a cell: nodes come in ascending cell order
+    const bool sorted = std::is_sorted(nodes.begin(), nodes.end(),
+                                       [](const MinNode& a, const MinNode& b) { return a.cell < b.cell; });
     std::unordered_map<int64_t, int> id;
-    id.reserve(nodes.size() * 2);
-    for (size_t i = 0; i < nodes.size(); i++) {
-        g.rep[i] = (int)i;
-        id[nodes[i].cell] = (int)i;
+    if (!sorted) {
+        id.reserve(nodes.size() * 2);
+        for (size_t i = 0; i < nodes.size(); i++) id[nodes[i].cell] = (int)i;
     }
+    auto index_of = [&](int64_t cell) -> int {
+        if (!sorted) {
+            auto it = id.find(cell);
+            return it == id.end() ? -1 : it->second;
+        }
+        auto it = std::lower_bound(nodes.begin(), nodes.end(), cell,
+                                   [](const MinNode& a, int64_t c) { return a.cell < c; });
+        return (it != nodes.end() && it->cell == cell) ? (int)(it - nodes.begin()) : -1;
+    };
+    std::vector<Arc> start;
     for (const SaddleArc& s : saddles) {
         if (s.min1 == s.min2 || s.min1 < 0 || s.min2 < 0) continue;
-        auto i1 = id.find(s.min1), i2 = id.find(s.min2);
-        if (i1 == id.end() || i2 == id.end()) continue;
+        const int i1 = index_of(s.min1), i2 = index_of(s.min2);
+        if (i1 < 0 || i2 < 0) continue;
         Arc a;
         a.saddle = s.saddle;
-        a.e1 = i1->second;
-        a.e2 = i2->second;
+        a.e1 = i1;
+        a.e2 = i2;
         a.sval = s.value;
         float d1 = nodes[a.e1].value - a.sval, d2 = nodes[a.e2].value - a.sval;
         if (d1 < 0) d1 *= -1;
         if (d2 < 0) d2 *= -1;
         a.pers = d1 < d2 ? d1 : d2;
-        g.arcs.push(a);
+        // an arc above the threshold is popped after every cancellable one and
+        // then dropped unprocessed, so it never needs to enter the queue
+        if (a.pers > threshold) continue;
+        start.push_back(a);
     }
+    g.arcs = std::priority_queue<Arc, std::vector<Arc>, ArcLater>(ArcLater(), std::move(start));
     g.thr = threshold;
     while (!g.arcs.empty()) {
         const Arc a = g.arcs.top();

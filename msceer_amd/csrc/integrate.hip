@@ -20,6 +20,7 @@
 // budget in pass 1) re-traces in FP64 with the gradient recomputed in FP64
 // from the values, which is how the reference computes it.
 #include <hip/hip_runtime.h>
+#include <hipcub/hipcub.hpp>
 
 #include "integrate.hpp"
 #include "mlist.hpp"
@@ -650,17 +651,45 @@ cc_flatten_kernel(VGrid g, LabelBuffers b) {
     }
 }
 
-// Component roots in a compact list (any order): root vertex, its label and
-// the roots of its All6 neighbours that come before it, earliest first (-1
-// padded).  The merge of roots into neighbouring regions is sequential in
-// root order (gi_isolated_region_remover.h:172-186: a root sees the merges of
-// every earlier root), so it runs on the host over this list.
+// Component roots that may merge: roots with an All6 neighbour before them
+// carrying another label (gi_isolated_region_remover.h:172-186).  Any other
+// root keeps its region: its earlier neighbours lie in its own component.
+// Every root starts mapped to itself (rep2); candidates are flagged in cand.
 __global__ void __launch_bounds__(kThreads)
-cc_collect_kernel(const float* __restrict__ f, VGrid g, LabelBuffers b, int32_t* __restrict__ out, uint32_t cap,
-                  uint32_t* __restrict__ count) {
+cc_candidates_kernel(const float* __restrict__ f, VGrid g, LabelBuffers b, uint32_t* __restrict__ cand,
+                     uint32_t* __restrict__ count) {
+    const int64_t n = g.verts();
+    uint32_t mine = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        cand[i] = 0u;
+        if (b.rep[i] != (int32_t)i) continue;
+        b.rep2[i] = (int32_t)i;
+        int64_t nb[6];
+        const int k = all6(g, i, nb);
+        const int32_t l = b.label2[i];
+        bool c = false;
+        for (int j = 0; j < k; j++)
+            if (before(f, nb[j], i) && b.label2[nb[j]] != l) c = true;
+        if (c) {
+            cand[i] = 1u;
+            mine++;
+        }
+    }
+    if (mine) atomicAdd(count, mine);
+}
+
+// Candidate roots in vertex-id order (slot = exclusive prefix count of the
+// candidate flags): root, its label, then for its All6 neighbours before it,
+// earliest first, their component roots, those roots' candidate slots (-1
+// for a root that is no candidate) and their labels (-1 padded).  The merge
+// itself is sequential in root order (a root sees the merges of every
+// earlier root), so it runs on the host over this list.
+__global__ void __launch_bounds__(kThreads)
+cc_collect_kernel(const float* __restrict__ f, VGrid g, LabelBuffers b, const uint32_t* __restrict__ cand,
+                  const uint32_t* __restrict__ slot_of, int32_t* __restrict__ out) {
     const int64_t n = g.verts();
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-        if (b.rep[i] != (int32_t)i) continue;
+        if (!cand[i]) continue;
         int64_t nb[6];
         const int k = all6(g, i, nb);
         int64_t lower[6];
@@ -674,12 +703,15 @@ cc_collect_kernel(const float* __restrict__ f, VGrid g, LabelBuffers b, int32_t*
             }
             lower[p] = nb[j];
         }
-        const uint32_t slot = atomicAdd(count, 1u);
-        if (slot >= cap) continue;
-        int32_t* o = out + (size_t)slot * kCollectStride;
+        int32_t* o = out + (size_t)slot_of[i] * kCollectStride;
         o[0] = (int32_t)i;
         o[1] = b.label2[i];
-        for (int j = 0; j < 6; j++) o[2 + j] = j < m ? b.rep[lower[j]] : -1;
+        for (int j = 0; j < 6; j++) {
+            const int32_t r = j < m ? b.rep[lower[j]] : -1;
+            o[2 + j] = r;
+            o[8 + j] = r >= 0 && cand[r] ? (int32_t)slot_of[r] : -1;
+            o[14 + j] = j < m ? b.label2[lower[j]] : -1;
+        }
     }
 }
 
@@ -955,11 +987,21 @@ hipError_t launch_cc_union(const VGrid& g, const LabelBuffers& b, hipStream_t s)
     hipLaunchKernelGGL(cc_flatten_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, g, b);
     return hipGetLastError();
 }
-hipError_t launch_cc_collect(const VGrid& g, const LabelBuffers& b, int32_t* out, uint32_t cap, uint32_t* count,
-                             hipStream_t s) {
-    hipLaunchKernelGGL(cc_collect_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, b.values, g, b, out, cap,
+hipError_t launch_cc_candidates(const VGrid& g, const LabelBuffers& b, uint32_t* cand, uint32_t* count,
+                                hipStream_t s) {
+    hipLaunchKernelGGL(cc_candidates_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, b.values, g, b, cand,
                        count);
     return hipGetLastError();
+}
+hipError_t launch_cc_collect(const VGrid& g, const LabelBuffers& b, const uint32_t* cand, const uint32_t* slot_of,
+                             int32_t* out, hipStream_t s) {
+    hipLaunchKernelGGL(cc_collect_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, b.values, g, b, cand,
+                       slot_of, out);
+    return hipGetLastError();
+}
+hipError_t launch_exclusive_sum(const uint32_t* in, uint32_t* out, int64_t n, void* temp, size_t* temp_bytes,
+                                hipStream_t s) {
+    return hipcub::DeviceScan::ExclusiveSum(temp, *temp_bytes, in, out, (int)n, s);
 }
 hipError_t launch_cc_scatter(const int32_t* pairs, uint32_t n, int32_t* rep2, hipStream_t s) {
     if (!n) return hipSuccess;

@@ -93,10 +93,18 @@ hipError_t launch_pass3(const VGrid& g, const LabelBuffers& b, const WorkQueue& 
 hipError_t launch_cc_init(const VGrid& g, const LabelBuffers& b, hipStream_t s);
 // Same-label components (rep[] = earliest vertex of each component).
 hipError_t launch_cc_union(const VGrid& g, const LabelBuffers& b, hipStream_t s);
-// ints per collected component root: root, label, six earlier neighbour roots
-constexpr int kCollectStride = 8;
-hipError_t launch_cc_collect(const VGrid& g, const LabelBuffers& b, int32_t* out, uint32_t cap, uint32_t* count,
-                             hipStream_t s);
+// ints per collected candidate root: root, label, then for six earlier
+// neighbours their component roots, those roots' candidate slots, their labels
+constexpr int kCollectStride = 20;
+// Flags the component roots that may merge (cand, one word per vertex) and
+// counts them; maps every root to itself in rep2.
+hipError_t launch_cc_candidates(const VGrid& g, const LabelBuffers& b, uint32_t* cand, uint32_t* count,
+                                hipStream_t s);
+hipError_t launch_cc_collect(const VGrid& g, const LabelBuffers& b, const uint32_t* cand, const uint32_t* slot_of,
+                             int32_t* out, hipStream_t s);
+// Exclusive prefix sum of n words (hipCUB); temp == nullptr queries *temp_bytes.
+hipError_t launch_exclusive_sum(const uint32_t* in, uint32_t* out, int64_t n, void* temp, size_t* temp_bytes,
+                                hipStream_t s);
 hipError_t launch_cc_scatter(const int32_t* pairs, uint32_t n, int32_t* rep2, hipStream_t s);
 hipError_t launch_cc_apply(const VGrid& g, const LabelBuffers& b, hipStream_t s);
 // Restriction labels from vertex labels (MIN boundary): 1 on every cell

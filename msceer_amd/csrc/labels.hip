@@ -65,58 +65,65 @@ float elapsed(hipEvent_t a, hipEvent_t b) {
 // is a sequential scan over the (few) roots on the host.
 int run_cleanup(msceer_ctx* ctx, const VGrid& g, const LabelBuffers& b) {
     hipStream_t s = ctx->stream;
+    const int64_t V = g.verts();
     CHECK_HIP(launch_cc_init(g, b, s));
     CHECK_HIP(launch_cc_union(g, b, s));
-    int rc = MSCEER_OK;
     uint32_t* count = ctl_dev(ctx) + kCtlCount;
+    // candidate roots (all others keep their region) and their slots in
+    // vertex order; b.stamp is free here
+    uint32_t* cand = b.stamp;
+    size_t temp_bytes = 0;
+    CHECK_HIP(launch_exclusive_sum(cand, nullptr, V, nullptr, &temp_bytes, s));
+    CHECK_HIP(ctx->scan.ensure(sizeof(uint32_t) * V + temp_bytes + 256));
+    uint32_t* slot_of = ctx->scan.as<uint32_t>();
+    void* temp = reinterpret_cast<uint8_t*>(ctx->scan.ptr) + ((sizeof(uint32_t) * V + 255) & ~(size_t)255);
+    CHECK_HIP(launch_zero(count, 1, s));
+    CHECK_HIP(launch_cc_candidates(g, b, cand, count, s));
+    CHECK_HIP(launch_exclusive_sum(cand, slot_of, V, temp, &temp_bytes, s));
+    int rc = read_ctl(ctx, kCtlCount, 1);
+    if (rc) return rc;
+    const uint32_t R = ctx->h_ctl[kCtlCount];
     const size_t row = sizeof(int32_t) * kCollectStride;
-    uint32_t cap = (uint32_t)std::min<int64_t>(g.verts(), std::max<int64_t>(4096, g.verts() / 64));
-    uint32_t R = 0;
-    for (int attempt = 0; attempt < 2; attempt++) {
-        CHECK_HIP(ctx->host_io.ensure(row * cap));
-        CHECK_HIP(launch_zero(count, 1, s));
-        CHECK_HIP(launch_cc_collect(g, b, ctx->host_io.as<int32_t>(), cap, count, s));
-        rc = read_ctl(ctx, kCtlCount, 1);
-        if (rc) return rc;
-        R = ctx->h_ctl[kCtlCount];
-        if (R <= cap) break;
-        cap = R;
-    }
-    if (R > cap) return MSCEER_EHIP;
     std::vector<int32_t> h((size_t)R * kCollectStride);
-    if (R) CHECK_HIP(hipMemcpyAsync(h.data(), ctx->host_io.ptr, row * R, hipMemcpyDeviceToHost, s));
-    CHECK_HIP(hipStreamSynchronize(s));
-    std::vector<uint32_t> ord(R);
-    for (uint32_t k = 0; k < R; k++) ord[k] = k;
-    std::sort(ord.begin(), ord.end(),
-              [&](uint32_t a, uint32_t c) { return h[(size_t)a * kCollectStride] < h[(size_t)c * kCollectStride]; });
-    std::vector<int32_t> ids(R), lab(R), parent(R);
-    for (uint32_t k = 0; k < R; k++) {
-        ids[k] = h[(size_t)ord[k] * kCollectStride];
-        lab[k] = h[(size_t)ord[k] * kCollectStride + 1];
-        parent[k] = (int32_t)k;
+    if (R) {
+        CHECK_HIP(ctx->host_io.ensure(std::max(row * R, sizeof(int32_t) * 2 * (size_t)R)));
+        CHECK_HIP(launch_cc_collect(g, b, cand, slot_of, ctx->host_io.as<int32_t>(), s));
+        CHECK_HIP(hipMemcpyAsync(h.data(), ctx->host_io.ptr, row * R, hipMemcpyDeviceToHost, s));
+        CHECK_HIP(hipStreamSynchronize(s));
     }
+    // candidates in vertex-id order; a set of candidates merged together has
+    // a representative q, and term[q] >= 0 once the set joined a region that
+    // no candidate represents (that root, with label termlab[q])
+    std::vector<int32_t> parent(R), term(R, -1), termlab(R, -1);
+    for (uint32_t k = 0; k < R; k++) parent[k] = (int32_t)k;
     auto find = [&](int32_t x) {
         int32_t r = x;
         while (parent[r] != r) r = parent[r];
         while (parent[x] != r) { const int32_t n = parent[x]; parent[x] = r; x = n; }
         return r;
     };
+    auto region_label = [&](int32_t q) { return term[q] >= 0 ? termlab[q] : h[(size_t)q * kCollectStride + 1]; };
     for (uint32_t k = 0; k < R; k++) {
-        const int32_t* e = &h[(size_t)ord[k] * kCollectStride];
+        const int32_t* e = &h[(size_t)k * kCollectStride];
+        const int32_t lab = e[1];
         for (int j = 0; j < 6 && e[2 + j] >= 0; j++) {
-            const int32_t ri = (int32_t)(std::lower_bound(ids.begin(), ids.end(), e[2 + j]) - ids.begin());
-            if (ri >= (int32_t)R || ids[ri] != e[2 + j]) return MSCEER_EHIP;
-            const int32_t fr = find(ri);
-            if (lab[fr] == lab[k]) continue;
-            parent[k] = fr;
+            if (e[8 + j] >= 0) {
+                const int32_t fr = find(e[8 + j]);
+                if (region_label(fr) == lab) continue;
+                parent[k] = fr;  // k is still its set's representative
+            } else {
+                if (e[14 + j] == lab) continue;  // a root that never moves: its own label
+                term[k] = e[2 + j];
+                termlab[k] = e[14 + j];
+            }
             break;
         }
     }
     std::vector<int32_t> pairs(2 * (size_t)R);
     for (uint32_t k = 0; k < R; k++) {
-        pairs[2 * k] = ids[k];
-        pairs[2 * k + 1] = ids[find((int32_t)k)];
+        const int32_t q = find((int32_t)k);
+        pairs[2 * k] = h[(size_t)k * kCollectStride];
+        pairs[2 * k + 1] = term[q] >= 0 ? term[q] : h[(size_t)q * kCollectStride];
     }
     if (R) {
         CHECK_HIP(hipMemcpyAsync(ctx->host_io.ptr, pairs.data(), sizeof(int32_t) * pairs.size(),
@@ -451,6 +458,7 @@ extern "C" int msceer_clean_labels(msceer_ctx* ctx, const float* values, int64_t
     CHECK_HIP(ctx->label2.ensure(sizeof(int32_t) * V));
     CHECK_HIP(ctx->rep.ensure(sizeof(int32_t) * V));
     CHECK_HIP(ctx->rep2.ensure(sizeof(int32_t) * V));
+    CHECK_HIP(ctx->stamp.ensure(sizeof(uint32_t) * V));
     CHECK_HIP(hipMemcpyAsync(ctx->label.ptr, labels_inout, sizeof(int32_t) * V,
                              dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
     LabelBuffers b{};
@@ -459,6 +467,7 @@ extern "C" int msceer_clean_labels(msceer_ctx* ctx, const float* values, int64_t
     b.label2 = ctx->label2.as<int32_t>();
     b.rep = ctx->rep.as<int32_t>();
     b.rep2 = ctx->rep2.as<int32_t>();
+    b.stamp = ctx->stamp.as<uint32_t>();
     rc = run_cleanup(ctx, g, b);
     if (rc) return rc;
     CHECK_HIP(hipMemcpyAsync(labels_inout, b.label, sizeof(int32_t) * V,
