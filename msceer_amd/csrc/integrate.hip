@@ -802,22 +802,56 @@ __device__ int64_t trace_down_vertex(const CellG& cg, const uint8_t* grad, int64
     return -1;
 }
 
+__device__ inline int64_t vertex_of_cell(const CellG& cg, int64_t c) {
+    const int64_t x = (c % cg.Mx) / 2, y = ((c / cg.Mx) % cg.My) / 2, z = (c / cg.sxy) / 2;
+    return x + ((cg.Mx + 1) / 2) * (y + ((cg.My + 1) / 2) * z);
+}
+
+// Arcs of the extremum graph below the simplification threshold.  The
+// persistence is computed as the host simplification computes it (float
+// differences, then the smaller magnitude); an arc above the threshold is
+// never cancelled, so only the others leave the device.
 __global__ void __launch_bounds__(kThreads)
-saddle_trace_kernel(VGrid g, const uint8_t* __restrict__ grad, MList list, int64_t* __restrict__ out,
-                    int64_t* __restrict__ saddle_cell, uint32_t n_max, uint32_t* __restrict__ count) {
+saddle_arcs_kernel(VGrid g, const float* __restrict__ f, const uint8_t* __restrict__ grad, MList list, float thr,
+                   ArcRecord* __restrict__ out, uint32_t cap, uint32_t* __restrict__ count) {
     __shared__ uint32_t prefix[kSubs + 2];
     const uint32_t total = mlist_prefix(list, prefix);
-    if (blockIdx.x == 0 && threadIdx.x == 0) *count = total;
     const CellG cg{2 * (int64_t)g.X - 1, 2 * (int64_t)g.Y - 1, (2 * (int64_t)g.X - 1) * (2 * (int64_t)g.Y - 1)};
     const int64_t limit = g.verts() + 1, cells = cg.sxy * (2 * (int64_t)g.Z - 1);
-    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < total && i < n_max; i += gridDim.x * kThreads) {
+    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < total; i += gridDim.x * kThreads) {
         const int64_t s = mlist_at(list, prefix, i);
         const int64_t cx = s % cg.Mx, cy = (s / cg.Mx) % cg.My;
         const int64_t step = (cx & 1) ? 1 : ((cy & 1) ? cg.Mx : cg.sxy);
         // facets of the edge in +axis, -axis order
-        out[2 * (int64_t)i] = trace_down_vertex(cg, grad, s + step, limit, cells);
-        out[2 * (int64_t)i + 1] = trace_down_vertex(cg, grad, s - step, limit, cells);
-        saddle_cell[i] = s;
+        const int64_t m1 = trace_down_vertex(cg, grad, s + step, limit, cells);
+        const int64_t m2 = trace_down_vertex(cg, grad, s - step, limit, cells);
+        if (m1 == m2 || m1 < 0 || m2 < 0) continue;
+        // value of the saddle's highest vertex (value, then vertex id)
+        const int64_t va = vertex_of_cell(cg, s + step), vb = vertex_of_cell(cg, s - step);
+        const float fa = f[va], fb = f[vb];
+        const bool a_high = fa > fb || (!(fb > fa) && va > vb);
+        const float sval = a_high ? fa : fb;
+        const float v1 = f[vertex_of_cell(cg, m1)], v2 = f[vertex_of_cell(cg, m2)];
+        float d1 = v1 - sval, d2 = v2 - sval;
+        if (d1 < 0) d1 *= -1;
+        if (d2 < 0) d2 *= -1;
+        const float pers = d1 < d2 ? d1 : d2;
+        if (pers > thr) continue;
+        const uint32_t slot = atomicAdd(count, 1u);
+        if (slot < cap) out[slot] = ArcRecord{s, m1, m2, sval, v1, v2, 0.f};
+    }
+}
+
+__global__ void __launch_bounds__(kThreads)
+seed_minima_kernel(MList minima, int32_t* __restrict__ label, float4* __restrict__ gradf,
+                   uint8_t* __restrict__ seedmask) {
+    __shared__ uint32_t prefix[kSubs + 2];
+    const uint32_t total = mlist_prefix(minima, prefix);
+    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < total; i += gridDim.x * kThreads) {
+        const int64_t v = mlist_at(minima, prefix, i);
+        label[v] = (int32_t)v;
+        gradf[v] = make_float4(0.f, 0.f, 0.f, 0.f);
+        seedmask[v] = 1;
     }
 }
 
@@ -863,12 +897,6 @@ __global__ void dest_init_kernel(VGrid g, LabelBuffers b) {
         b.redo[i] = 0;
         b.stamp[i] = 0;
     }
-}
-
-__global__ void gather_f32_kernel(const float* __restrict__ f, const int64_t* __restrict__ idx, int64_t n,
-                                  float* __restrict__ out) {
-    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
-        out[i] = idx[i] >= 0 ? f[idx[i]] : 0.f;
 }
 
 __global__ void __launch_bounds__(kThreads)
@@ -1017,21 +1045,20 @@ hipError_t launch_min_boundary(const VGrid& g, const int32_t* label, int8_t* res
     hipLaunchKernelGGL(min_boundary_kernel, dim3(grid_for(C)), dim3(kThreads), 0, s, g, label, restriction);
     return hipGetLastError();
 }
-hipError_t launch_saddle_trace(const VGrid& g, const uint8_t* grad, const MList& list, int64_t* out,
-                               int64_t* saddle_cell, uint32_t n_max, uint32_t* count, hipStream_t s) {
-    hipLaunchKernelGGL(saddle_trace_kernel, dim3(1024), dim3(kThreads), 0, s, g, grad, list, out, saddle_cell, n_max,
-                       count);
+hipError_t launch_saddle_arcs(const VGrid& g, const float* values, const uint8_t* grad, const MList& list,
+                              float threshold, ArcRecord* out, uint32_t cap, uint32_t* count, hipStream_t s) {
+    hipLaunchKernelGGL(saddle_arcs_kernel, dim3(1024), dim3(kThreads), 0, s, g, values, grad, list, threshold, out,
+                       cap, count);
+    return hipGetLastError();
+}
+hipError_t launch_seed_minima(const MList& minima, const LabelBuffers& b, hipStream_t s) {
+    hipLaunchKernelGGL(seed_minima_kernel, dim3(256), dim3(kThreads), 0, s, minima, b.label, b.gradf, b.seedmask);
     return hipGetLastError();
 }
 hipError_t launch_collect_minima(const VGrid& g, const uint8_t* grad, const MList& out, uint32_t* overflow,
                                  hipStream_t s) {
     hipLaunchKernelGGL(collect_minima_kernel, dim3(grid_for(g.verts(), 8192)), dim3(kThreads), 0, s, g, grad, out,
                        overflow);
-    return hipGetLastError();
-}
-hipError_t launch_gather_f32(const float* f, const int64_t* idx, int64_t n, float* out, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(gather_f32_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, f, idx, n, out);
     return hipGetLastError();
 }
 hipError_t launch_apply_mask(const VGrid& g, const LabelBuffers& b, const uint8_t* mask, hipStream_t s) {

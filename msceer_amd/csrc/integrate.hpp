@@ -110,16 +110,25 @@ hipError_t launch_cc_apply(const VGrid& g, const LabelBuffers& b, hipStream_t s)
 // Restriction labels from vertex labels (MIN boundary): 1 on every cell
 // with an edge whose two vertices carry valid, different labels.
 hipError_t launch_min_boundary(const VGrid& g, const int32_t* label, int8_t* restriction, hipStream_t s);
+// One arc of the extremum graph: a critical 1-cell (saddle), the critical
+// vertices its two V-paths end at, the value of its highest vertex and the
+// values at the two minima.
+struct ArcRecord {
+    int64_t saddle, min1, min2;
+    float sval, v1, v2, pad;
+};
 // Follow the V-paths below both vertices of every critical 1-cell of `list`
-// to critical vertices; out[2i], out[2i+1] receive the two minima's cell ids
-// (-1 if a path runs off), saddle_cell[i] the saddle.
-hipError_t launch_saddle_trace(const VGrid& g, const uint8_t* grad, const MList& list, int64_t* out,
-                               int64_t* saddle_cell, uint32_t n_max, uint32_t* count, hipStream_t s);
+// to critical vertices and append the arcs that join two different minima
+// with persistence min(|v1 - sval|, |v2 - sval|) not above `threshold` to
+// out[0..cap) (count: appended arcs, also those beyond cap; order arbitrary).
+hipError_t launch_saddle_arcs(const VGrid& g, const float* values, const uint8_t* grad, const MList& list,
+                              float threshold, ArcRecord* out, uint32_t cap, uint32_t* count, hipStream_t s);
 // Critical vertices of a .grad (pair code 7 at even cells) appended to `out`.
 hipError_t launch_collect_minima(const VGrid& g, const uint8_t* grad, const MList& out, uint32_t* overflow,
                                  hipStream_t s);
-// out[i] = f[idx[i]] (0 where idx[i] < 0).
-hipError_t launch_gather_f32(const float* f, const int64_t* idx, int64_t n, float* out, hipStream_t s);
+// Every vertex of `minima` becomes a seed labelled by itself (zero gradient,
+// seed mask set).
+hipError_t launch_seed_minima(const MList& minima, const LabelBuffers& b, hipStream_t s);
 // Outside the integration mask, unassigned vertices take their
 // combinatorial label (b.label2) and count as assigned; counts the rest.
 hipError_t launch_apply_mask(const VGrid& g, const LabelBuffers& b, const uint8_t* mask, hipStream_t s);

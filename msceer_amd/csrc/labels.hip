@@ -3,7 +3,6 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <map>
 #include <vector>
 
 #include "ctx.hpp"
@@ -14,25 +13,6 @@ using namespace msceer;
 
 namespace msceer {
 namespace {
-
-// Copy the concatenation of a multi-list to the host.
-int mlist_to_host(msceer_ctx* ctx, const MList& l, std::vector<uint32_t>& out) {
-    std::vector<uint32_t> counts(kSubs + 1);
-    CHECK_HIP(hipMemcpyAsync(counts.data(), l.counts, sizeof(uint32_t) * (kSubs + 1), hipMemcpyDeviceToHost,
-                             ctx->stream));
-    CHECK_HIP(hipStreamSynchronize(ctx->stream));
-    out.clear();
-    for (int s = 0; s <= kSubs; s++) {  // sub-lists, then the spill list
-        const uint32_t n = std::min(counts[s], s < kSubs ? l.sub_cap : l.spill_cap);
-        if (!n) continue;
-        const size_t at = out.size();
-        out.resize(at + n);
-        CHECK_HIP(hipMemcpyAsync(out.data() + at, l.items + (size_t)s * l.sub_cap, sizeof(uint32_t) * n,
-                                 hipMemcpyDeviceToHost, ctx->stream));
-    }
-    CHECK_HIP(hipStreamSynchronize(ctx->stream));
-    return MSCEER_OK;
-}
 
 // Run an iterative kernel in groups of `group` launches until a whole group
 // reports no change (flag word in the control block).
@@ -218,81 +198,63 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
     CHECK_HIP(hipEventRecord(ev[1], s));
 
     // ---- extremum graph (critical vertices and 1-saddles) ----
+    // The minima stay on the device; only the arcs that the simplification can
+    // cancel (persistence at most presimp) come to the host.
     const MList minl = front_list(ctx, caps, 0);
     CHECK_HIP(launch_zero(minl.counts, kCountWords, s));
     CHECK_HIP(launch_collect_minima(g, d_grad, minl, ctl + kCtlOverflow, s));
     rc = read_ctl(ctx, 0, kCtlWords);
     if (rc) return rc;
     if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;  // more critical cells than list room
-    std::vector<uint32_t> minima, saddle_list;
-    rc = mlist_to_host(ctx, minl, minima);
-    if (rc) return rc;
-    std::sort(minima.begin(), minima.end());
+    const uint64_t n_min = list_total(ctx->h_ctl + kCtlRound, minl);
     const uint64_t n_sad = list_total(ctx->h_ctl + kCtlList1, list1);
-    // V-paths below the saddles (device), then values at minima and saddle vertices
-    CHECK_HIP(ctx->host_io.ensure(sizeof(int64_t) * (3 * n_sad + 16)));
-    int64_t* d_trace = ctx->host_io.as<int64_t>();
-    CHECK_HIP(launch_saddle_trace(g, d_grad, list1, d_trace, d_trace + 2 * n_sad, (uint32_t)n_sad, ctl + kCtlCount, s));
-    std::vector<int64_t> tr(3 * n_sad);
-    if (n_sad) CHECK_HIP(hipMemcpyAsync(tr.data(), d_trace, sizeof(int64_t) * 3 * n_sad, hipMemcpyDeviceToHost, s));
-    CHECK_HIP(hipStreamSynchronize(s));
-    // gather values: minima vertices, then both vertices of every saddle edge
+    std::vector<ArcRecord> rec;
+    if (n_sad) {
+        CHECK_HIP(ctx->host_io.ensure(sizeof(ArcRecord) * n_sad + 64));
+        ArcRecord* d_arcs = ctx->host_io.as<ArcRecord>();
+        CHECK_HIP(launch_zero(ctl + kCtlCount, 1, s));
+        CHECK_HIP(launch_saddle_arcs(g, d_values, d_grad, list1, presimp, d_arcs, (uint32_t)n_sad, ctl + kCtlCount, s));
+        rc = read_ctl(ctx, kCtlCount, 1);
+        if (rc) return rc;
+        rec.resize(std::min<uint64_t>(ctx->h_ctl[kCtlCount], n_sad));
+        if (!rec.empty()) {
+            CHECK_HIP(hipMemcpyAsync(rec.data(), d_arcs, sizeof(ArcRecord) * rec.size(), hipMemcpyDeviceToHost, s));
+            CHECK_HIP(hipStreamSynchronize(s));
+        }
+    }
+    // nodes: the minima the cancellable arcs touch, in ascending cell order
+    std::vector<MinNode> nodes;
+    nodes.reserve(2 * rec.size());
+    std::vector<SaddleArc> arcs(rec.size());
+    for (size_t i = 0; i < rec.size(); i++) {
+        nodes.push_back(MinNode{rec[i].min1, rec[i].v1});
+        nodes.push_back(MinNode{rec[i].min2, rec[i].v2});
+        arcs[i] = SaddleArc{rec[i].saddle, rec[i].min1, rec[i].min2, rec[i].sval};
+    }
+    std::sort(nodes.begin(), nodes.end(), [](const MinNode& x, const MinNode& y) { return x.cell < y.cell; });
+    nodes.erase(std::unique(nodes.begin(), nodes.end(),
+                            [](const MinNode& x, const MinNode& y) { return x.cell == y.cell; }),
+                nodes.end());
+    const std::vector<int> rep = simplify_minima(nodes, arcs, presimp);
+    // seeds: every minimum labelled by itself, then the cancelled ones by
+    // their representative's vertex
     const int64_t Mx = 2 * nx - 1, My = 2 * ny - 1;
     auto vnum = [&](int64_t c) { return (c % Mx) / 2 + nx * (((c / Mx) % My) / 2 + ny * ((c / (Mx * My)) / 2)); };
-    std::vector<int64_t> idx;
-    idx.reserve(minima.size() + 2 * n_sad);
-    for (uint32_t v : minima) idx.push_back(v);
-    std::vector<int64_t> sad_a(n_sad), sad_b(n_sad);
-    for (uint64_t i = 0; i < n_sad; i++) {
-        const int64_t sc = tr[2 * n_sad + i];
-        const int64_t cx = sc % Mx, cy = (sc / Mx) % My;
-        const int64_t step = (cx & 1) ? 1 : ((cy & 1) ? Mx : Mx * My);
-        sad_a[i] = vnum(sc + step);
-        sad_b[i] = vnum(sc - step);
-        idx.push_back(sad_a[i]);
-        idx.push_back(sad_b[i]);
+    std::vector<int32_t> seed_v, seed_l;
+    for (size_t i = 0; i < nodes.size(); i++) {
+        if (rep[i] == (int)i) continue;
+        seed_v.push_back((int32_t)vnum(nodes[i].cell));
+        seed_l.push_back((int32_t)vnum(nodes[rep[i]].cell));
     }
-    std::vector<float> vals(idx.size());
-    if (!idx.empty()) {
-        CHECK_HIP(ctx->host_io.ensure((sizeof(int64_t) + sizeof(float)) * idx.size() + 64));
-        int64_t* d_idx = ctx->host_io.as<int64_t>();
-        float* d_vals = reinterpret_cast<float*>(d_idx + idx.size());
-        CHECK_HIP(hipMemcpyAsync(d_idx, idx.data(), sizeof(int64_t) * idx.size(), hipMemcpyHostToDevice, s));
-        CHECK_HIP(launch_gather_f32(d_values, d_idx, (int64_t)idx.size(), d_vals, s));
-        CHECK_HIP(hipMemcpyAsync(vals.data(), d_vals, sizeof(float) * idx.size(), hipMemcpyDeviceToHost, s));
-        CHECK_HIP(hipStreamSynchronize(s));
-    }
-    std::vector<MinNode> nodes(minima.size());
-    for (size_t i = 0; i < minima.size(); i++) {
-        const int64_t v = minima[i];
-        const int64_t x = v % nx, y = (v / nx) % ny, z = v / (nx * ny);
-        nodes[i].cell = 2 * x + Mx * (2 * y + My * 2 * z);
-        nodes[i].value = vals[i];
-    }
-    std::vector<SaddleArc> arcs(n_sad);
-    for (uint64_t i = 0; i < n_sad; i++) {
-        const float fa = vals[minima.size() + 2 * i], fb = vals[minima.size() + 2 * i + 1];
-        // value of the saddle's highest vertex (value, then vertex id)
-        const bool a_high = fa > fb || (!(fb > fa) && sad_a[i] > sad_b[i]);
-        arcs[i] = SaddleArc{tr[2 * n_sad + i], tr[2 * i], tr[2 * i + 1], a_high ? fa : fb};
-    }
-    const std::vector<int> rep = simplify_minima(nodes, arcs, presimp);
-    // seeds: every minimum, labelled by its representative's vertex
-    std::vector<int32_t> seed_v(minima.size()), seed_l(minima.size());
-    std::vector<char> is_rep(minima.size(), 0);
-    for (size_t i = 0; i < minima.size(); i++) {
-        seed_v[i] = (int32_t)minima[i];
-        seed_l[i] = (int32_t)minima[rep[i]];
-        is_rep[rep[i]] = 1;
-    }
-    st.minima = (int64_t)minima.size();
+    st.minima = (int64_t)n_min;
     st.saddles = (int64_t)n_sad;
-    for (char c : is_rep) st.regions += c;
+    st.regions = (int64_t)n_min - (int64_t)seed_v.size();
 
     // ---- certain regions ----
     CHECK_HIP(launch_fill_i32(b.label, V, -1, s));
     CHECK_HIP(launch_fill_u8(b.seedmask, V, 0, s));
     CHECK_HIP(launch_grad_field(g, d_values, b.gradf, s));
+    CHECK_HIP(launch_seed_minima(minl, b, s));
     if (!seed_v.empty()) {
         CHECK_HIP(ctx->host_io.ensure(2 * sizeof(int32_t) * seed_v.size() + 64));
         int32_t* d_sv = ctx->host_io.as<int32_t>();
