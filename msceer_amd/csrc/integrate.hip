@@ -18,12 +18,17 @@
 // Precision: pass 1 traces in FP32 with an FP32 gradient field; pass 3
 // (vertices next to a basin boundary, and streamlines that ran out of step
 // budget in pass 1) re-traces in FP64 with the gradient recomputed in FP64
-// from the values, which is how the reference computes it.
+// from the values, which is how the reference computes it.  Products and sums
+// are not fused into FMAs here: each rounds as in the reference's scalar
+// arithmetic, so a streamline's positions follow the reference's operation by
+// operation.
 #include <hip/hip_runtime.h>
 #include <hipcub/hipcub.hpp>
 
 #include "integrate.hpp"
 #include "mlist.hpp"
+
+#pragma clang fp contract(off)
 
 namespace msceer {
 
@@ -163,19 +168,51 @@ __device__ __forceinline__ int64_t nearest(const VGrid& g, const V3<T>& p, bool 
            (int64_t)g.X * ((int64_t)clampi((int)qy, g.Y) + (int64_t)g.Y * (int64_t)clampi((int)qz, g.Z));
 }
 
-// One voxel of adaptive Euler with step doubling in precision T.
+__device__ __forceinline__ int dist_to_boundary(const VGrid& g, int x, int y, int z) {
+    int m = min(x, g.X - 1 - x);
+    m = min(m, min(y, g.Y - 1 - y));
+    return min(m, min(z, g.Z - 1 - z));
+}
+
+// Adaptive Euler with step doubling (gi_adaptive_euler_advector.h:51-305) in
+// precision T, as one lane's state advanced one iteration of the step loop at
+// a time.  The step loop of the reference runs voxel by voxel: on entering a
+// voxel it gathers the 8 corner gradients and restarts the step length; an
+// iteration recomputes the gradient when the position moved, then takes or
+// rejects one step.  Here a step that leaves the voxel enters the next one in
+// the same iteration (its corner loads are issued beside the destination
+// check of that step), so the lanes of a wave stay in step instead of each
+// voxel's loop running until its slowest lane leaves.  Per lane, the
+// arithmetic and the iteration count are the reference's: leaving a voxel
+// costs one iteration without a step.
+constexpr double kMaxStep = 0.5;
+
 template <typename T>
-__device__ int advect_voxel(const GradSrc<T>& src, const VGrid& g, const uint8_t* __restrict__ dest,
-                            bool stop_on_assigned, T gthr, T ethr, bool near, V3<T>& cur, int& iters) {
-    const T kMaxStep = (T)0.5;
-    V3<T> b0 = cur;
-    if (near) { b0.x = clampc(b0.x, g.X); b0.y = clampc(b0.y, g.Y); b0.z = clampc(b0.z, g.Z); }
-    const int bx = (int)b0.x, by = (int)b0.y, bz = (int)b0.z;
-    int xs[2] = {bx, bx + 1}, ys[2] = {by, by + 1}, zs[2] = {bz, bz + 1};
-    if (near) {
-        if (bx >= g.X - 1) xs[0] = xs[1] = g.X - 1;
-        if (by >= g.Y - 1) ys[0] = ys[1] = g.Y - 1;
-        if (bz >= g.Z - 1) zs[0] = zs[1] = g.Z - 1;
+struct Tracer {
+    V3<T> cur, old, gr;
+    V3<T> c[8];
+    T gm, step;
+    int bx, by, bz;
+    bool near;
+    int iters;
+};
+
+// Corner gradients of the voxel of t.cur; (cx, cy, cz) is the vertex nearest
+// to the position on entry, which decides whether the boundary clamps apply.
+template <typename T>
+__device__ __forceinline__ void tracer_load(const GradSrc<T>& src, const VGrid& g, Tracer<T>& t, int cx, int cy,
+                                            int cz) {
+    t.near = dist_to_boundary(g, cx, cy, cz) <= 1;
+    V3<T> b0 = t.cur;
+    if (t.near) { b0.x = clampc(b0.x, g.X); b0.y = clampc(b0.y, g.Y); b0.z = clampc(b0.z, g.Z); }
+    t.bx = (int)b0.x;
+    t.by = (int)b0.y;
+    t.bz = (int)b0.z;
+    int xs[2] = {t.bx, t.bx + 1}, ys[2] = {t.by, t.by + 1}, zs[2] = {t.bz, t.bz + 1};
+    if (t.near) {
+        if (t.bx >= g.X - 1) xs[0] = xs[1] = g.X - 1;
+        if (t.by >= g.Y - 1) ys[0] = ys[1] = g.Y - 1;
+        if (t.bz >= g.Z - 1) zs[0] = zs[1] = g.Z - 1;
     }
 #pragma unroll
     for (int k = 0; k < 2; k++) {
@@ -183,50 +220,93 @@ __device__ int advect_voxel(const GradSrc<T>& src, const VGrid& g, const uint8_t
         ys[k] = clampi(ys[k], g.Y);
         zs[k] = clampi(zs[k], g.Z);
     }
-    V3<T> c[8];
 #pragma unroll
     for (int k = 0; k < 8; k++)
-        c[k] = src.at((int64_t)xs[k & 1] + (int64_t)g.X * ((int64_t)ys[(k >> 1) & 1] + (int64_t)g.Y * zs[k >> 2]));
-    V3<T> old = cur;
-    V3<T> gr = trilin(c, cur.x - (T)bx, cur.y - (T)by, cur.z - (T)bz);
-    T gm = sqrt(gr.x * gr.x + gr.y * gr.y + gr.z * gr.z);
-    T step = kMaxStep / gm;
-    while (iters-- > 0) {
-        if (!(old.x == cur.x && old.y == cur.y && old.z == cur.z)) {
-            if ((int)cur.x != (int)old.x || (int)cur.y != (int)old.y || (int)cur.z != (int)old.z) return kOutOfVoxel;
-            gr = trilin(c, cur.x - (T)bx, cur.y - (T)by, cur.z - (T)bz);
-            gm = sqrt(gr.x * gr.x + gr.y * gr.y + gr.z * gr.z);
-            if (gm * step > kMaxStep) step = kMaxStep / gm;
-            old = cur;
-        }
-        if (gm < gthr) return kLowGradient;
-        // ascending manifolds: flow downhill (step multiplier -1)
-        const T s = -step, hs = -(T)0.5 * step;
-        const V3<T> full{gr.x * s, gr.y * s, gr.z * s};
-        const V3<T> half{gr.x * hs, gr.y * hs, gr.z * hs};
-        const V3<T> hg = trilin(c, cur.x + half.x - (T)bx, cur.y + half.y - (T)by, cur.z + half.z - (T)bz);
-        const V3<T> two{hg.x * hs + half.x, hg.y * hs + half.y, hg.z * hs + half.z};
-        const T dx = two.x - full.x, dy = two.y - full.y, dz = two.z - full.z;
-        const T err = dx * dx + dy * dy + dz * dz;
-        if (err < ethr) {
-            cur = {cur.x + two.x, cur.y + two.y, cur.z + two.z};
-            if (near) { cur.x = clampc(cur.x, g.X); cur.y = clampc(cur.y, g.Y); cur.z = clampc(cur.z, g.Z); }
-            const T grow = step * (T)1.8, cap = kMaxStep / gm;
-            step = grow < cap ? grow : cap;
-            const uint8_t d = dest[nearest(g, cur, near)];
-            if (d == kCertainTerminal || (stop_on_assigned && d == kAssigned)) return kHitPreassigned;
-        } else {
-            step = step * (T)0.5;
-        }
-    }
-    return kOverMax;
+        t.c[k] = src.at((int64_t)xs[k & 1] + (int64_t)g.X * ((int64_t)ys[(k >> 1) & 1] + (int64_t)g.Y * zs[k >> 2]));
 }
 
-__device__ __forceinline__ int dist_to_boundary(const VGrid& g, int x, int y, int z) {
-    int m = min(x, g.X - 1 - x);
-    m = min(m, min(y, g.Y - 1 - y));
-    return min(m, min(z, g.Z - 1 - z));
+// Voxel entry: gradient at the position and a fresh step length.
+template <typename T>
+__device__ __forceinline__ void tracer_begin(Tracer<T>& t) {
+    t.old = t.cur;
+    t.gr = trilin(t.c, t.cur.x - (T)t.bx, t.cur.y - (T)t.by, t.cur.z - (T)t.bz);
+    t.gm = sqrt(t.gr.x * t.gr.x + t.gr.y * t.gr.y + t.gr.z * t.gr.z);
+    t.step = (T)kMaxStep / t.gm;
 }
+
+// One iteration.  kNone: the streamline goes on (moved: it entered the voxel
+// of vertex `next`); kHitPreassigned (next: the vertex reached), kLowGradient
+// and kOverMax end it; kOutOfVoxel asks the caller to enter the voxel of the
+// position (only a position changed outside a step could lead there).
+template <typename T>
+__device__ __forceinline__ int tracer_iterate(const GradSrc<T>& src, const VGrid& g,
+                                              const uint8_t* __restrict__ dest, bool stop_on_assigned, T gthr,
+                                              T ethr, Tracer<T>& t, int64_t& next, bool& moved) {
+    moved = false;
+    if (!(t.iters-- > 0)) return kOverMax;
+    if (!(t.old.x == t.cur.x && t.old.y == t.cur.y && t.old.z == t.cur.z)) {
+        if ((int)t.cur.x != (int)t.old.x || (int)t.cur.y != (int)t.old.y || (int)t.cur.z != (int)t.old.z)
+            return kOutOfVoxel;
+        t.gr = trilin(t.c, t.cur.x - (T)t.bx, t.cur.y - (T)t.by, t.cur.z - (T)t.bz);
+        t.gm = sqrt(t.gr.x * t.gr.x + t.gr.y * t.gr.y + t.gr.z * t.gr.z);
+        if (t.gm * t.step > (T)kMaxStep) t.step = (T)kMaxStep / t.gm;
+        t.old = t.cur;
+    }
+    if (t.gm < gthr) return kLowGradient;
+    // ascending manifolds: flow downhill (step multiplier -1)
+    const T s = -t.step, hs = -(T)0.5 * t.step;
+    const V3<T> full{t.gr.x * s, t.gr.y * s, t.gr.z * s};
+    const V3<T> half{t.gr.x * hs, t.gr.y * hs, t.gr.z * hs};
+    const V3<T> hg = trilin(t.c, t.cur.x + half.x - (T)t.bx, t.cur.y + half.y - (T)t.by, t.cur.z + half.z - (T)t.bz);
+    const V3<T> two{hg.x * hs + half.x, hg.y * hs + half.y, hg.z * hs + half.z};
+    const T dx = two.x - full.x, dy = two.y - full.y, dz = two.z - full.z;
+    const T err = dx * dx + dy * dy + dz * dz;
+    if (!(err < ethr)) {
+        t.step = t.step * (T)0.5;
+        return kNone;
+    }
+    t.cur = {t.cur.x + two.x, t.cur.y + two.y, t.cur.z + two.z};
+    if (t.near) { t.cur.x = clampc(t.cur.x, g.X); t.cur.y = clampc(t.cur.y, g.Y); t.cur.z = clampc(t.cur.z, g.Z); }
+    const T grow = t.step * (T)1.8, cap = (T)kMaxStep / t.gm;
+    t.step = grow < cap ? grow : cap;
+    next = nearest(g, t.cur, t.near);
+    const uint8_t d = dest[next];
+    // the next iteration would leave the voxel: gather the next voxel's
+    // corners while the destination byte is in flight
+    const bool leave = t.iters > 0 &&
+                       ((int)t.cur.x != (int)t.old.x || (int)t.cur.y != (int)t.old.y || (int)t.cur.z != (int)t.old.z);
+    if (leave) {
+        const int cx = (int)(next % g.X), cy = (int)((next / g.X) % g.Y), cz = (int)(next / g.sxy);
+        tracer_load(src, g, t, cx, cy, cz);
+    }
+    if (d == kCertainTerminal || (stop_on_assigned && d == kAssigned)) return kHitPreassigned;
+    if (leave) {
+        t.iters--;  // the iteration that leaves the voxel
+        tracer_begin(t);
+        moved = true;
+    }
+    return kNone;
+}
+
+// Enter the voxel of the position after kOutOfVoxel; returns the vertex nearest to it.
+template <typename T>
+__device__ __forceinline__ int64_t tracer_reenter(const GradSrc<T>& src, const VGrid& g, Tracer<T>& t) {
+    const int64_t next = nearest(g, t.cur, t.near);
+    tracer_load(src, g, t, (int)(next % g.X), (int)((next / g.X) % g.Y), (int)(next / g.sxy));
+    tracer_begin(t);
+    return next;
+}
+
+// Start a streamline at vertex (x, y, z).
+template <typename T>
+__device__ __forceinline__ void tracer_start(const GradSrc<T>& src, const VGrid& g, Tracer<T>& t, int x, int y, int z,
+                                             int max_iters) {
+    t.cur = {(T)x, (T)y, (T)z};
+    t.iters = max_iters;
+    tracer_load(src, g, t, x, y, z);
+    tracer_begin(t);
+}
+
 
 // Pass 1 at one level of the coarse-to-fine order: vertices whose coordinates
 // are multiples of `stride` (and not all multiples of 2*stride unless this is
@@ -244,39 +324,77 @@ pass1_kernel(GradSrc<float> src, VGrid g, LabelBuffers b, int stride, int first_
     const int64_t n = (int64_t)nx * ny * nz;
     uint32_t traced = 0, over = 0;
     uint64_t steps = 0;
-    for (int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x; t < n; t += (int64_t)gridDim.x * kThreads) {
-        const int x = (int)(t % nx) * stride, y = (int)((t / nx) % ny) * stride;
-        const int z = (int)(t / ((int64_t)nx * ny)) * stride;
-        if (!first_level && (x % (2 * stride)) == 0 && (y % (2 * stride)) == 0 && (z % (2 * stride)) == 0) continue;
-        const int64_t v0 = (int64_t)x + (int64_t)g.X * ((int64_t)y + (int64_t)g.Y * z);
-        if (b.dest[v0] != kUnassigned) continue;
-        traced++;
-        int np = 0;
-        path[(np++) * kThreads] = (int32_t)v0;
-        V3<float> cur{(float)x, (float)y, (float)z};
-        int cx = x, cy = y, cz = z;
-        int iters = max_iters;
-        int ev;
-        int64_t next;
-        while (true) {
-            const bool near = dist_to_boundary(g, cx, cy, cz) <= 1;
-            ev = advect_voxel<float>(src, g, b.dest, true, gthr, ethr, near, cur, iters);
-            next = nearest(g, cur, near);
-            cx = (int)(next % g.X);
-            cy = (int)((next / g.X) % g.Y);
-            cz = (int)(next / g.sxy);
-            if (np < kPath) path[(np++) * kThreads] = (int32_t)next;
-            if (ev != kOutOfVoxel) break;
-        }
-        steps += (uint64_t)(max_iters - (iters > 0 ? iters : 0));
-        const int32_t lab = b.label[next];
-        for (int j = 0; j < np; j++) {
-            const int32_t p = path[j * kThreads];
-            if (b.dest[p] == kUnassigned) {
-                b.label[p] = lab;
-                b.dest[p] = kAssigned;
+    int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int64_t t_step = (int64_t)gridDim.x * kThreads;
+    Tracer<float> tr;
+    bool active = false;
+    int np = 0;
+    int64_t v0 = 0;
+    bool parked = false, ended = false;
+    int64_t end_vertex = 0;
+    // The lanes of a wave trace a batch of vertices (the next vertex of each
+    // lane's grid-stride sequence; a lane whose vertex needs no tracing idles)
+    // voxel by voxel: a lane that enters another voxel waits until every
+    // streamline of the wave has left its voxel or ended.  When all have
+    // ended, every path of the batch takes the label its end vertex carries
+    // at that moment, and the next batch starts.  Streamlines of one batch
+    // thus never stop on each other's paths, which leaves fewer wrong path
+    // labels for pass 3 to correct than labelling each path as it ends.
+    while (true) {
+        if (!__any(active)) {
+            if (ended) {
+                ended = false;
+                const int32_t lab = b.label[end_vertex];
+                for (int j = 0; j < np; j++) {
+                    const int32_t p = path[j * kThreads];
+                    if (b.dest[p] == kUnassigned) {
+                        b.label[p] = lab;
+                        b.dest[p] = kAssigned;
+                    }
+                }
+            }
+            while (!__any(active) && __any(t < n)) {
+                if (t < n) {
+                    const int x = (int)(t % nx) * stride, y = (int)((t / nx) % ny) * stride;
+                    const int z = (int)(t / ((int64_t)nx * ny)) * stride;
+                    t += t_step;
+                    v0 = (int64_t)x + (int64_t)g.X * ((int64_t)y + (int64_t)g.Y * z);
+                    const bool skip =
+                        !first_level && (x % (2 * stride)) == 0 && (y % (2 * stride)) == 0 && (z % (2 * stride)) == 0;
+                    if (!skip && b.dest[v0] == kUnassigned) {
+                        traced++;
+                        np = 0;
+                        path[(np++) * kThreads] = (int32_t)v0;
+                        tracer_start(src, g, tr, x, y, z, max_iters);
+                        active = true;
+                    }
+                }
             }
         }
+        if (!__any(active)) break;
+        if (!__any(active && !parked)) parked = false;
+        if (!active || parked) continue;
+        int64_t next = 0;
+        bool moved = false;
+        int ev = tracer_iterate(src, g, b.dest, true, gthr, ethr, tr, next, moved);
+        if (ev == kOutOfVoxel) {
+            next = tracer_reenter(src, g, tr);
+            moved = true;
+            ev = kNone;
+        }
+        if (moved) {
+            parked = true;
+            if (np < kPath) path[(np++) * kThreads] = (int32_t)next;
+        }
+        if (ev == kNone) continue;
+        // the streamline ended; its path is labelled when the batch ends
+        active = false;
+        parked = false;
+        ended = true;
+        if (ev != kHitPreassigned) next = nearest(g, tr.cur, tr.near);
+        if (np < kPath) path[(np++) * kThreads] = (int32_t)next;
+        end_vertex = next;
+        steps += (uint64_t)(max_iters - (tr.iters > 0 ? tr.iters : 0));
         if (ev == kOverMax) {
             over++;
             b.redo[v0] = 1;
@@ -445,130 +563,234 @@ pass2_kernel(VGrid g, LabelBuffers b, WorkQueue q) {
 // labels do not depend on the processing order, since every re-integration
 // ends on a certain terminal, whose label never changes.
 //
-// One persistent launch: lanes claim queue slots (one atomic per wave) and
-// wait for a claimed slot to be written; a lane whose streamline ends takes
-// new work while the others keep advecting.  `pending` counts queued items
-// not yet finished; a producer adds its children before it retires its own
-// item, so pending == 0 means that no slot will be written again.
+// One persistent launch.  Each wave keeps a ring of queued vertices in LDS:
+// it refills an empty ring with a chunk of the items pass 2 queued (one
+// atomic per chunk) and keeps the children it queues there while the ring
+// has room.  Children beyond that go to the global part of the queue, whose
+// slots lanes claim one at a time once the pass-2 chunks are gone (a lane
+// holding a claimed slot still takes ring items, and polls the slot when it
+// has nothing else to do).  A lane
+// whose streamline ends takes new work while the others keep advecting, one
+// step per iteration.  `pending` counts the items of the global queue (pass
+// 2's included) not yet finished: children are added before their producer
+// is retired, and a wave subtracts the items it finished only when it has no
+// streamline left and an empty ring, so an idle wave that reads pending == 0
+// knows that every item is done and that no slot will be written again.
 constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;
 constexpr uint64_t kIdleTicks = 400000000ull;  // 4 s of the 100 MHz real-time counter
+constexpr uint32_t kRing = 256;                // per-wave ring (a power of two)
+constexpr uint32_t kRingKeep = 192;            // children stay in the ring up to this fill
+constexpr uint32_t kChunk = 64;                // pass-2 items claimed at a time
+constexpr uint32_t kCounted = 0x80000000u;     // ring item flag: counted in pending (vertex ids < 2^31)
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+__device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t x, int lane) {
+    uint32_t incl = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    return incl - x;
+}
+
+// Between pass 2 and pass 3: the pass-2 items are [0, tail); single slots
+// are claimed from tail on.
+__global__ void pass3_prepare_kernel(WorkQueue q) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        const uint32_t n = *q.tail;
+        *q.first_count = n;
+        *q.head = n;
+        *q.chunk_head = 0u;
+    }
+}
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T gthr, T ethr, int max_iters) {
+    __shared__ uint32_t ring_mem[kThreads / 64][kRing];
+    uint32_t* ring = ring_mem[threadIdx.x >> 6];
     const int lane = (int)(threadIdx.x & 63u);
     const uint64_t below = (1ull << lane) - 1ull;
-    uint32_t traced = 0, over = 0;
+    const uint32_t first = *q.first_count;
+    uint32_t traced = 0, over = 0, retired = 0;
     uint64_t steps = 0;
-    bool active = false, claimed = false, done = false;
+    uint32_t rhead = 0, rtail = 0;  // ring cursors (the same in every lane)
+    bool chunks = true;             // pass-2 chunks may remain (the same in every lane)
+    bool active = false, claimed = false, done = false, counted = false;
     uint32_t slot = 0;
     uint64_t idle_since = 0;
     int64_t v = 0;
     int32_t init = 0;
-    V3<T> cur{(T)0, (T)0, (T)0};
-    int cx = 0, cy = 0, cz = 0, iters = 0;
+    Tracer<T> tr;
     while (true) {
-        const bool need = !active && !claimed && !done;
-        const uint64_t want = __ballot(need);
-        if (want) {
+        // a lane without a streamline takes ring items first, also while it
+        // holds a claimed global slot (to which it returns when idle)
+        bool need = !active && !done;
+        uint64_t want = __ballot(need);
+        if (want && rhead == rtail && chunks) {
+            const int leader = __ffsll((unsigned long long)want) - 1;
+            uint32_t base = 0;
+            if (lane == leader) base = atomicAdd(q.chunk_head, kChunk);
+            base = __shfl(base, leader);
+            if (base >= first) {
+                chunks = false;
+            } else {
+                const uint32_t n = first - base < kChunk ? first - base : kChunk;
+                if ((uint32_t)lane < n) ring[(rtail + (uint32_t)lane) & (kRing - 1)] = q.items[base + lane] | kCounted;
+                rtail += n;
+                __builtin_amdgcn_wave_barrier();
+            }
+        }
+        uint32_t item = kEmptySlot;
+        if (want && rhead != rtail) {
+            const uint32_t avail = rtail - rhead, nw = (uint32_t)__popcll(want);
+            const uint32_t rank = (uint32_t)__popcll(want & below);
+            if (need && rank < avail) {
+                item = ring[(rhead + rank) & (kRing - 1)];
+                need = false;
+            }
+            rhead += avail < nw ? avail : nw;
+        }
+        want = __ballot(need && !claimed);
+        if (want && !chunks) {
             const int leader = __ffsll((unsigned long long)want) - 1;
             uint32_t base = 0;
             if (lane == leader) base = atomicAdd(q.head, (uint32_t)__popcll(want));
             base = __shfl(base, leader);
-            if (need) {
+            if (need && !claimed) {
                 slot = base + (uint32_t)__popcll(want & below);
                 if (slot < q.capacity) claimed = true;
                 else done = true;
             }
         }
-        if (claimed) {
-            const uint32_t item = __hip_atomic_load(&q.items[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            if (item != kEmptySlot) {
+        if (need && claimed) {
+            const uint32_t it = __hip_atomic_load(&q.items[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (it != kEmptySlot) {
                 claimed = false;
-                v = (int64_t)item;
-                if (b.dest[v] == kAssigned) {
-                    active = true;
-                    traced++;
-                    init = b.label[v];
-                    cx = (int)(v % g.X);
-                    cy = (int)((v / g.X) % g.Y);
-                    cz = (int)(v / g.sxy);
-                    cur = {(T)cx, (T)cy, (T)cz};
-                    iters = max_iters;
-                } else {
-                    atomicSub(q.pending, 1u);
-                }
+                item = it | kCounted;
             }
         }
-        const uint64_t waiting = __ballot(claimed);
-        if (waiting && !__any(active)) {
-            // nothing to advect in this wave: one lane checks for the end of
-            // the pass, the wave backs off before polling its slots again
-            const int w = __ffsll((unsigned long long)waiting) - 1;
-            uint32_t p = 0;
-            if (lane == w) p = __hip_atomic_load(q.pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            p = __shfl(p, w);
-            if (p == 0u) {
-                if (claimed) done = true;
-                claimed = false;
-            } else {
-                // a wave idle for kIdleTicks gives up and reports it, so the
-                // kernel ends even if the queue's invariant were ever broken
-                const uint64_t now = __builtin_amdgcn_s_memrealtime();
-                if (idle_since == 0) {
-                    idle_since = now;
-                } else if (now - idle_since > kIdleTicks) {
-                    if (lane == w) atomicOr(q.overflow, 2u);
+        if (item != kEmptySlot) {
+            counted = (item & kCounted) != 0u;
+            v = (int64_t)(item & ~kCounted);
+            if (b.dest[v] == kAssigned) {
+                active = true;
+                traced++;
+                init = b.label[v];
+                tracer_start(src, g, tr, (int)(v % g.X), (int)((v / g.X) % g.Y), (int)(v / g.sxy), max_iters);
+            } else if (counted) {
+                retired++;
+            }
+        }
+        if (!__any(active) && rhead == rtail) {
+            // out of work: retire what this wave finished, then wait for a
+            // claimed slot to be written or for the end of the pass
+            const uint32_t r = wave_sum(retired);
+            retired = 0;
+            if (r) {
+                __threadfence();
+                if (lane == 0) atomicSub(q.pending, r);
+            }
+            const uint64_t waiting = __ballot(claimed);
+            if (waiting) {
+                const int w = __ffsll((unsigned long long)waiting) - 1;
+                uint32_t p = 0;
+                if (lane == w) p = __hip_atomic_load(q.pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                p = __shfl(p, w);
+                if (p == 0u) {
+                    if (claimed) done = true;
                     claimed = false;
-                    done = true;
+                } else {
+                    // a wave idle for kIdleTicks gives up and reports it, so the
+                    // kernel ends even if the queue's invariant were ever broken
+                    const uint64_t now = __builtin_amdgcn_s_memrealtime();
+                    if (idle_since == 0) {
+                        idle_since = now;
+                    } else if (now - idle_since > kIdleTicks) {
+                        if (lane == w) atomicOr(q.overflow, 2u);
+                        claimed = false;
+                        done = true;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
                 }
-                __builtin_amdgcn_s_sleep(2);
+            } else {
+                idle_since = 0;
             }
         } else {
             idle_since = 0;
         }
         if (__all(done)) break;
-        if (!active) continue;
-        const bool near = dist_to_boundary(g, cx, cy, cz) <= 1;
-        const int ev = advect_voxel<T>(src, g, b.dest, false, gthr, ethr, near, cur, iters);
-        const int64_t next = nearest(g, cur, near);
-        cx = (int)(next % g.X);
-        cy = (int)((next / g.X) % g.Y);
-        cz = (int)(next / g.sxy);
-        if (ev == kOutOfVoxel) continue;
-        active = false;
-        steps += (uint64_t)(max_iters - (iters > 0 ? iters : 0));
-        if (ev == kOverMax) over++;
-        const int32_t nl = b.label[next];
-        b.label[v] = nl;
-        b.dest[v] = kCertainNonterminal;
-        if (nl != init) {
-            int64_t nb[6];
-            const int k = all6(g, v, nb);
-            uint32_t kids[6];
-            int nk = 0;
-            for (int j = 0; j < k; j++) {
-                if (b.dest[nb[j]] == kAssigned && b.label[nb[j]] != nl && atomicExch(&b.stamp[nb[j]], 1u) == 0u)
-                    kids[nk++] = (uint32_t)nb[j];
+        uint32_t kids[6];
+        int nk = 0;
+        if (active) {
+            int64_t next = 0;
+            bool moved = false;
+            const int ev = tracer_iterate(src, g, b.dest, false, gthr, ethr, tr, next, moved);
+            if (ev == kOutOfVoxel) {
+                tracer_reenter(src, g, tr);
+            } else if (ev != kNone) {
+                active = false;
+                if (ev != kHitPreassigned) next = nearest(g, tr.cur, tr.near);
+                steps += (uint64_t)(max_iters - (tr.iters > 0 ? tr.iters : 0));
+                if (ev == kOverMax) over++;
+                const int32_t nl = b.label[next];
+                b.label[v] = nl;
+                b.dest[v] = kCertainNonterminal;
+                if (nl != init) {
+                    int64_t nb[6];
+                    const int k = all6(g, v, nb);
+                    for (int j = 0; j < k; j++) {
+                        if (b.dest[nb[j]] == kAssigned && b.label[nb[j]] != nl &&
+                            atomicExch(&b.stamp[nb[j]], 1u) == 0u)
+                            kids[nk++] = (uint32_t)nb[j];
+                    }
+                }
+                if (counted) retired++;
             }
-            if (nk) {
-                atomicAdd(q.pending, (uint32_t)nk);
-                const uint32_t at = atomicAdd(q.tail, (uint32_t)nk);
-                uint32_t lost = 0;
-                for (int j = 0; j < nk; j++) {
-                    if (at + j < q.capacity)
-                        __hip_atomic_store(&q.items[at + j], kids[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        // queue the children: in the ring while it has room, globally beyond
+        if (__any(nk > 0)) {
+            const uint32_t off = wave_exclusive_scan((uint32_t)nk, lane);
+            const uint32_t total = __shfl(off + (uint32_t)nk, 63);
+            const uint32_t fill = rtail - rhead;
+            const uint32_t room = fill < kRingKeep ? kRingKeep - fill : 0u;
+            const uint32_t nloc = total < room ? total : room;
+            const uint32_t nglob = total - nloc;
+            uint32_t gbase = 0;
+            if (nglob) {
+                if (lane == 0) {
+                    atomicAdd(q.pending, nglob);
+                    gbase = atomicAdd(q.tail, nglob);
+                }
+                gbase = __shfl(gbase, 0);
+            }
+            uint32_t lost = 0;
+            for (int j = 0; j < nk; j++) {
+                const uint32_t k = off + (uint32_t)j;
+                if (k < nloc) {
+                    ring[(rtail + k) & (kRing - 1)] = kids[j];
+                } else {
+                    const uint32_t at = gbase + (k - nloc);
+                    if (at < q.capacity)
+                        __hip_atomic_store(&q.items[at], kids[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     else
                         lost++;
                 }
-                if (lost) {
-                    atomicSub(q.pending, lost);
-                    atomicOr(q.overflow, 1u);
-                }
+            }
+            rtail += nloc;
+            __builtin_amdgcn_wave_barrier();
+            lost = wave_sum(lost);
+            if (lost && lane == 0) {
+                atomicSub(q.pending, lost);
+                atomicOr(q.overflow, 1u);
             }
         }
-        __threadfence();
-        atomicSub(q.pending, 1u);
     }
     if (traced) atomicAdd(&b.stats[kStatTraced3], traced);
     if (over) atomicAdd(&b.stats[kStatOver3], over);
@@ -1001,6 +1223,7 @@ hipError_t launch_pass3(const VGrid& g, const LabelBuffers& b, const WorkQueue& 
                         int max_iters, hipStream_t s) {
     static uint32_t blocks = resident_blocks(pass3_kernel<double>);
     GradSrc<double> src{b.gradd};
+    hipLaunchKernelGGL(pass3_prepare_kernel, dim3(1), dim3(64), 0, s, q);
     hipLaunchKernelGGL(pass3_kernel<double>, dim3(blocks), dim3(kThreads), 0, s, src, g, b, q, (double)gthr,
                        (double)ethr, max_iters);
     return hipGetLastError();

@@ -77,16 +77,19 @@ hipError_t launch_pass1(const VGrid& g, const LabelBuffers& b, int stride, bool 
 struct WorkQueue {
     uint32_t* items;
     uint32_t capacity;
-    uint32_t* head;
-    uint32_t* tail;
-    uint32_t* pending;
+    uint32_t* head;         // next slot of the items queued during pass 3
+    uint32_t* tail;         // slots reserved
+    uint32_t* pending;      // queued items not yet finished
     uint32_t* overflow;
+    uint32_t* chunk_head;   // next chunk of the items pass 2 queued
+    uint32_t* first_count;  // items pass 2 queued
 };
 hipError_t launch_pass2(const VGrid& g, const LabelBuffers& b, const WorkQueue& q, hipStream_t s);
 // One pass-3 round over `in` (FP64 re-integration to the certain regions);
 // changed labels push their ASSIGNED neighbours to `out`; `clear` counts are zeroed.
 hipError_t launch_grad_field64(const VGrid& g, const LabelBuffers& b, hipStream_t s);
-// Persistent FP64 pass 3 over the queue filled by pass 2.
+// Persistent FP64 pass 3 over the queue filled by pass 2 (zero head, tail,
+// pending and chunk_head before pass 2).
 hipError_t launch_pass3(const VGrid& g, const LabelBuffers& b, const WorkQueue& q, float gthr, float ethr,
                         int max_iters, hipStream_t s);
 // Cleanup of isolated components.
