@@ -330,16 +330,15 @@ pass1_kernel(GradSrc<float> src, VGrid g, LabelBuffers b, int stride, int first_
     bool active = false;
     int np = 0;
     int64_t v0 = 0;
-    bool parked = false, ended = false;
+    bool ended = false;
     int64_t end_vertex = 0;
     // The lanes of a wave trace a batch of vertices (the next vertex of each
-    // lane's grid-stride sequence; a lane whose vertex needs no tracing idles)
-    // voxel by voxel: a lane that enters another voxel waits until every
-    // streamline of the wave has left its voxel or ended.  When all have
-    // ended, every path of the batch takes the label its end vertex carries
-    // at that moment, and the next batch starts.  Streamlines of one batch
-    // thus never stop on each other's paths, which leaves fewer wrong path
-    // labels for pass 3 to correct than labelling each path as it ends.
+    // lane's grid-stride sequence; a lane whose vertex needs no tracing idles),
+    // one step per iteration.  When all have ended, every path of the batch
+    // takes the label its end vertex carries at that moment, and the next
+    // batch starts.  Streamlines of one batch thus never stop on each other's
+    // paths, which leaves fewer wrong path labels for pass 3 to correct than
+    // labelling each path as it ends.
     while (true) {
         if (!__any(active)) {
             if (ended) {
@@ -372,8 +371,7 @@ pass1_kernel(GradSrc<float> src, VGrid g, LabelBuffers b, int stride, int first_
             }
         }
         if (!__any(active)) break;
-        if (!__any(active && !parked)) parked = false;
-        if (!active || parked) continue;
+        if (!active) continue;
         int64_t next = 0;
         bool moved = false;
         int ev = tracer_iterate(src, g, b.dest, true, gthr, ethr, tr, next, moved);
@@ -382,14 +380,10 @@ pass1_kernel(GradSrc<float> src, VGrid g, LabelBuffers b, int stride, int first_
             moved = true;
             ev = kNone;
         }
-        if (moved) {
-            parked = true;
-            if (np < kPath) path[(np++) * kThreads] = (int32_t)next;
-        }
+        if (moved && np < kPath) path[(np++) * kThreads] = (int32_t)next;
         if (ev == kNone) continue;
         // the streamline ended; its path is labelled when the batch ends
         active = false;
-        parked = false;
         ended = true;
         if (ev != kHitPreassigned) next = nearest(g, tr.cur, tr.near);
         if (np < kPath) path[(np++) * kThreads] = (int32_t)next;
@@ -435,6 +429,25 @@ __device__ __forceinline__ int all6(const VGrid& g, int64_t i, int64_t nb[6]) {
     return k;
 }
 
+constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;             // queue slot not written yet
+constexpr uint64_t kIdleTicks = 400000000ull;  // 4 s of the 100 MHz real-time counter
+
+__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
+    return x;
+}
+
+__device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t x, int lane) {
+    uint32_t incl = x;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t y = __shfl_up(incl, o);
+        if (lane >= o) incl += y;
+    }
+    return incl - x;
+}
+
 // ---- certain regions (GridSimplifiedExtremalRegionBuilder::BeginIntegration) ----
 //
 // A vertex takes the label shared by all of its earlier All6 neighbours if
@@ -444,18 +457,25 @@ __device__ __forceinline__ int all6(const VGrid& g, int64_t i, int64_t nb[6]) {
 // once its last unresolved earlier neighbour is, so every vertex is visited
 // once instead of once per sweep of a fixed-point iteration.  cnt[v] counts
 // the unresolved earlier neighbours of a non-seed vertex.
+//
+// Rounds in dependency order: round r resolves the vertices released in
+// round r - 1 and appends those it releases to a flat list for round r + 1
+// (one atomic per wave).  Lists alternate between two buffers; three count
+// words rotate, and each round clears the one round r + 2 will append to.
 
+// The vertices with no earlier non-seed neighbour start the first list.
 __global__ void __launch_bounds__(kThreads)
 certain_init_kernel(const float* __restrict__ f, VGrid g, const uint8_t* __restrict__ seedmask,
-                    uint32_t* __restrict__ cnt, MList out, uint32_t* __restrict__ overflow) {
+                    uint32_t* __restrict__ cnt, uint32_t* __restrict__ out, uint32_t* __restrict__ out_count,
+                    uint32_t capacity, uint32_t* __restrict__ overflow) {
     __shared__ uint32_t stage[kStage];
-    __shared__ uint32_t s_n;
-    __shared__ MSlot s_base;
+    __shared__ uint32_t s_n, s_base;
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
-    const Outbox box{stage, &s_n, out, (int)(blockIdx.x % kSubs), overflow};
     const int64_t n = g.verts();
-    for (int64_t v = (int64_t)blockIdx.x * kThreads + threadIdx.x; v < n; v += (int64_t)gridDim.x * kThreads) {
+    // each block covers a contiguous range of at most kStage vertices
+    const int64_t lo = (int64_t)blockIdx.x * kStage, hi = lo + kStage < n ? lo + kStage : n;
+    for (int64_t v = lo + threadIdx.x; v < hi; v += kThreads) {
         if (seedmask[v]) {
             cnt[v] = 0u;
             continue;
@@ -466,49 +486,69 @@ certain_init_kernel(const float* __restrict__ f, VGrid g, const uint8_t* __restr
         for (int j = 0; j < k; j++)
             if (before(f, nb[j], v) && !seedmask[nb[j]]) c++;
         cnt[v] = c;
-        if (c == 0) box.push((uint32_t)v);
+        if (c == 0) stage[atomicAdd(&s_n, 1u)] = (uint32_t)v;
     }
     __syncthreads();
-    mlist_flush_block(out, box.sub, stage, s_n < (uint32_t)kStage ? s_n : (uint32_t)kStage, &s_base, overflow);
+    const uint32_t m = s_n;
+    if (threadIdx.x == 0 && m) s_base = atomicAdd(out_count, m);
+    __syncthreads();
+    for (uint32_t j = threadIdx.x; j < m; j += kThreads) {
+        const uint32_t at = s_base + j;
+        if (at < capacity) out[at] = stage[j];
+        else atomicOr(overflow, 1u);
+    }
 }
 
-// One round: resolve the ready vertices, release their later neighbours.
-// Block 0 clears the counts of the list the next round appends to.
+// One round: resolve the listed vertices, release their later neighbours.
 __global__ void __launch_bounds__(kThreads)
 certain_round_kernel(const float* __restrict__ f, VGrid g, const uint8_t* __restrict__ seedmask,
-                     int32_t* __restrict__ label, uint32_t* __restrict__ cnt, MList in, MList out,
-                     uint32_t* __restrict__ clear, uint32_t* __restrict__ overflow) {
-    __shared__ uint32_t stage[kStage];
-    __shared__ uint32_t prefix[kSubs + 2];
-    __shared__ uint32_t s_n;
-    __shared__ MSlot s_base;
-    if (blockIdx.x == 0)
-        for (uint32_t j = threadIdx.x; j < (uint32_t)kCountWords; j += kThreads) clear[j] = 0u;
-    if (threadIdx.x == 0) s_n = 0;
-    const uint32_t total = mlist_prefix(in, prefix);
-    const Outbox box{stage, &s_n, out, (int)(blockIdx.x % kSubs), overflow};
-    for (uint32_t it = blockIdx.x * kThreads + threadIdx.x; it < total; it += gridDim.x * kThreads) {
-        const int64_t v = mlist_at(in, prefix, it);
-        int64_t nb[6];
-        const int k = all6(g, v, nb);
-        int32_t val = -1;
-        bool has = false, ok = true;
-        for (int j = 0; j < k; j++) {
-            if (!before(f, nb[j], v)) continue;
-            const int32_t l = label[nb[j]];
-            if (l == -1) { ok = false; break; }
-            if (!has) { val = l; has = true; }
-            else if (val != l) { ok = false; break; }
+                     int32_t* __restrict__ label, uint32_t* __restrict__ cnt, const uint32_t* __restrict__ in,
+                     const uint32_t* __restrict__ in_count, uint32_t* __restrict__ out,
+                     uint32_t* __restrict__ out_count, uint32_t* __restrict__ clear_count, uint32_t capacity,
+                     uint32_t* __restrict__ overflow) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) *clear_count = 0u;
+    const int lane = (int)(threadIdx.x & 63u);
+    const uint32_t total = *in_count < capacity ? *in_count : capacity;
+    const uint32_t stride = gridDim.x * kThreads;
+    // every lane of a wave runs the same number of iterations (wave-wide appends)
+    const uint32_t first = blockIdx.x * kThreads + (threadIdx.x & ~63u);
+    for (uint32_t base = first; base < total; base += stride) {
+        const uint32_t it = base + (uint32_t)lane;
+        uint32_t kids[6];
+        int nk = 0;
+        if (it < total) {
+            const int64_t v = (int64_t)in[it];
+            int64_t nb[6];
+            const int k = all6(g, v, nb);
+            int32_t val = -1;
+            bool has = false, ok = true;
+            for (int j = 0; j < k; j++) {
+                if (!before(f, nb[j], v)) continue;
+                const int32_t l = label[nb[j]];
+                if (l == -1) { ok = false; break; }
+                if (!has) { val = l; has = true; }
+                else if (val != l) { ok = false; break; }
+            }
+            if (ok && has) label[v] = val;
+            for (int j = 0; j < k; j++) {
+                const int64_t u = nb[j];
+                if (seedmask[u] || !before(f, v, u)) continue;
+                if (atomicSub(&cnt[u], 1u) == 1u) kids[nk++] = (uint32_t)u;
+            }
         }
-        if (ok && has) label[v] = val;
-        for (int j = 0; j < k; j++) {
-            const int64_t u = nb[j];
-            if (seedmask[u] || !before(f, v, u)) continue;
-            if (atomicSub(&cnt[u], 1u) == 1u) box.push((uint32_t)u);
+        if (__any(nk > 0)) {
+            const uint32_t off = wave_exclusive_scan((uint32_t)nk, lane);
+            const uint32_t n_all = __shfl(off + (uint32_t)nk, 63);
+            uint32_t at0 = 0;
+            if (lane == 0) at0 = atomicAdd(out_count, n_all);
+            at0 = __shfl(at0, 0);
+            for (int j = 0; j < nk; j++) {
+                const uint32_t at = at0 + off + (uint32_t)j;
+                if (at < capacity) out[at] = kids[j];
+                else atomicOr(overflow, 1u);
+            }
         }
     }
-    __syncthreads();
-    mlist_flush_block(out, box.sub, stage, s_n < (uint32_t)kStage ? s_n : (uint32_t)kStage, &s_base, overflow);
 }
 
 // Pass 2: every assigned vertex next to a different label, or flagged for
@@ -576,28 +616,10 @@ pass2_kernel(VGrid g, LabelBuffers b, WorkQueue q) {
 // is retired, and a wave subtracts the items it finished only when it has no
 // streamline left and an empty ring, so an idle wave that reads pending == 0
 // knows that every item is done and that no slot will be written again.
-constexpr uint32_t kEmptySlot = 0xFFFFFFFFu;
-constexpr uint64_t kIdleTicks = 400000000ull;  // 4 s of the 100 MHz real-time counter
 constexpr uint32_t kRing = 256;                // per-wave ring (a power of two)
 constexpr uint32_t kRingKeep = 192;            // children stay in the ring up to this fill
 constexpr uint32_t kChunk = 64;                // pass-2 items claimed at a time
 constexpr uint32_t kCounted = 0x80000000u;     // ring item flag: counted in pending (vertex ids < 2^31)
-
-__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) x += __shfl_xor(x, o);
-    return x;
-}
-
-__device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t x, int lane) {
-    uint32_t incl = x;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t y = __shfl_up(incl, o);
-        if (lane >= o) incl += y;
-    }
-    return incl - x;
-}
 
 // Between pass 2 and pass 3: the pass-2 items are [0, tail); single slots
 // are claimed from tail on.
@@ -717,7 +739,7 @@ pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T gthr, T eth
                         claimed = false;
                         done = true;
                     }
-                    __builtin_amdgcn_s_sleep(2);
+                    __builtin_amdgcn_s_sleep(8);
                 }
             } else {
                 idle_since = 0;
@@ -1173,17 +1195,38 @@ hipError_t launch_seed_scatter(const int32_t* seed_vert, const int32_t* seed_lab
                        seed_label, n, b.label, b.gradf, b.seedmask);
     return hipGetLastError();
 }
+namespace {
+uint32_t compute_units() {
+    int dev = 0, cus = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) return 256;
+    return (uint32_t)cus;
+}
+// Work-groups of a persistent kernel that are resident at once on the device.
+template <typename K>
+uint32_t resident_blocks(K kernel) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 1024;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) return 1024;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kThreads, 0) != hipSuccess || per <= 0) per = 1;
+    return (uint32_t)(cus * per);
+}
+}  // namespace
 hipError_t launch_certain_init(const VGrid& g, const float* values, const uint8_t* seedmask, uint32_t* cnt,
-                               const MList& out, uint32_t* overflow, hipStream_t s) {
-    hipLaunchKernelGGL(certain_init_kernel, dim3(grid_for(g.verts(), 8192)), dim3(kThreads), 0, s, values, g,
-                       seedmask, cnt, out, overflow);
+                               uint32_t* out, uint32_t* out_count, uint32_t capacity, uint32_t* overflow,
+                               hipStream_t s) {
+    const int64_t blocks = (g.verts() + kStage - 1) / kStage;
+    hipLaunchKernelGGL(certain_init_kernel, dim3((uint32_t)blocks), dim3(kThreads), 0, s, values, g, seedmask, cnt,
+                       out, out_count, capacity, overflow);
     return hipGetLastError();
 }
 hipError_t launch_certain_round(const VGrid& g, const float* values, const uint8_t* seedmask, int32_t* label,
-                                uint32_t* cnt, const MList& in, const MList& out, uint32_t* clear_counts,
-                                uint32_t* overflow, hipStream_t s) {
-    hipLaunchKernelGGL(certain_round_kernel, dim3(1024), dim3(kThreads), 0, s, values, g, seedmask, label, cnt, in,
-                       out, clear_counts, overflow);
+                                uint32_t* cnt, const uint32_t* in, const uint32_t* in_count, uint32_t* out,
+                                uint32_t* out_count, uint32_t* clear_count, uint32_t capacity, uint32_t* overflow,
+                                hipStream_t s) {
+    static const uint32_t blocks = compute_units();  // a level holds thousands of vertices, not millions
+    hipLaunchKernelGGL(certain_round_kernel, dim3(blocks), dim3(kThreads), 0, s, values, g, seedmask, label,
+                       cnt, in, in_count, out, out_count, clear_count, capacity, overflow);
     return hipGetLastError();
 }
 hipError_t launch_dest_init(const VGrid& g, const LabelBuffers& b, hipStream_t s) {
@@ -1208,17 +1251,6 @@ hipError_t launch_grad_field64(const VGrid& g, const LabelBuffers& b, hipStream_
                        b.gradd);
     return hipGetLastError();
 }
-namespace {
-// Work-groups of a persistent kernel that are resident at once on the device.
-template <typename K>
-uint32_t resident_blocks(K kernel) {
-    int dev = 0, cus = 0, per = 0;
-    if (hipGetDevice(&dev) != hipSuccess) return 1024;
-    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || cus <= 0) return 1024;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, kThreads, 0) != hipSuccess || per <= 0) per = 1;
-    return (uint32_t)(cus * per);
-}
-}  // namespace
 hipError_t launch_pass3(const VGrid& g, const LabelBuffers& b, const WorkQueue& q, float gthr, float ethr,
                         int max_iters, hipStream_t s) {
     static uint32_t blocks = resident_blocks(pass3_kernel<double>);

@@ -56,15 +56,6 @@ hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 // Seeds: label[seed_vert[i]] = seed_label[i]; zero gradient; seedmask = 1.
 hipError_t launch_seed_scatter(const int32_t* seed_vert, const int32_t* seed_label, int n, const LabelBuffers& b,
                                hipStream_t s);
-// One sweep of the certain-region fixed point; sets *changed when a label was set.
-// Certain regions in dependency order: init queues the vertices with no
-// unresolved earlier neighbour, each round resolves its list and queues the
-// vertices it releases (cnt: one uint32 per vertex).
-hipError_t launch_certain_init(const VGrid& g, const float* values, const uint8_t* seedmask, uint32_t* cnt,
-                               const MList& out, uint32_t* overflow, hipStream_t s);
-hipError_t launch_certain_round(const VGrid& g, const float* values, const uint8_t* seedmask, int32_t* label,
-                                uint32_t* cnt, const MList& in, const MList& out, uint32_t* clear_counts,
-                                uint32_t* overflow, hipStream_t s);
 // dest = UNASSIGNED where label == -1 else CERTAIN_TERMINAL; redo = 0; stamp = 0.
 hipError_t launch_dest_init(const VGrid& g, const LabelBuffers& b, hipStream_t s);
 // Pass 1 at one level of the coarse-to-fine order.
@@ -85,6 +76,18 @@ struct WorkQueue {
     uint32_t* first_count;  // items pass 2 queued
 };
 hipError_t launch_pass2(const VGrid& g, const LabelBuffers& b, const WorkQueue& q, hipStream_t s);
+// Certain regions in dependency order (label[v] becomes the label shared by
+// all earlier All6 neighbours of v when they all carry one; seeds are
+// labelled beforehand; cnt: one uint32 per vertex).  init lists the vertices
+// with no earlier non-seed neighbour; each round resolves the list `in` and
+// appends the vertices it releases to `out`, and zeroes *clear_count.
+hipError_t launch_certain_init(const VGrid& g, const float* values, const uint8_t* seedmask, uint32_t* cnt,
+                               uint32_t* out, uint32_t* out_count, uint32_t capacity, uint32_t* overflow,
+                               hipStream_t s);
+hipError_t launch_certain_round(const VGrid& g, const float* values, const uint8_t* seedmask, int32_t* label,
+                                uint32_t* cnt, const uint32_t* in, const uint32_t* in_count, uint32_t* out,
+                                uint32_t* out_count, uint32_t* clear_count, uint32_t capacity, uint32_t* overflow,
+                                hipStream_t s);
 // One pass-3 round over `in` (FP64 re-integration to the certain regions);
 // changed labels push their ASSIGNED neighbours to `out`; `clear` counts are zeroed.
 hipError_t launch_grad_field64(const VGrid& g, const LabelBuffers& b, hipStream_t s);

@@ -263,27 +263,28 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
         CHECK_HIP(hipMemcpyAsync(d_sl, seed_l.data(), sizeof(int32_t) * seed_l.size(), hipMemcpyHostToDevice, s));
         CHECK_HIP(launch_seed_scatter(d_sv, d_sl, (int)seed_v.size(), b, s));
     }
-    // rounds in dependency order, enqueued in groups with device-side counts
+    // rounds in dependency order, enqueued in groups; one read-back per group
+    // shows whether the next round's list is empty
     int sweeps = 0;
     {
-        auto front = [&](int k) { return front_list(ctx, caps, k); };
-        CHECK_HIP(launch_zero(front(0).counts, kCountWords, s));
-        CHECK_HIP(launch_zero(front(1).counts, kCountWords, s));
+        uint32_t* lists[2] = {ctx->front0.as<uint32_t>(), ctx->front1.as<uint32_t>()};
+        uint32_t* counts = ctl + kCtlRound;  // three rotating count words
+        CHECK_HIP(launch_zero(counts, 3, s));
         CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
-        CHECK_HIP(launch_certain_init(g, d_values, b.seedmask, b.stamp, front(0), ctl + kCtlOverflow, s));
+        CHECK_HIP(launch_certain_init(g, d_values, b.seedmask, b.stamp, lists[0], counts, (uint32_t)V,
+                                      ctl + kCtlOverflow, s));
         int k = 0;
         for (int guard = 0;; guard++) {
             if (guard > (int)(V / 16 + 16)) return MSCEER_EHIP;  // a round resolves at least one vertex
             constexpr int kGroup = 16;
-            for (int j = 0; j < kGroup; j++) {
-                CHECK_HIP(launch_certain_round(g, d_values, b.seedmask, b.label, b.stamp, front(k), front(k + 1),
-                                               front(k + 2).counts, ctl + kCtlOverflow, s));
-                k++;
-            }
+            for (int j = 0; j < kGroup; j++, k++)
+                CHECK_HIP(launch_certain_round(g, d_values, b.seedmask, b.label, b.stamp, lists[k % 2],
+                                               counts + k % 3, lists[(k + 1) % 2], counts + (k + 1) % 3,
+                                               counts + (k + 2) % 3, (uint32_t)V, ctl + kCtlOverflow, s));
             rc = read_ctl(ctx, 0, kCtlWords);
             if (rc) return rc;
             if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;
-            if (list_total(ctx->h_ctl + kCtlRound + (k % 3) * kCountWords, front(k)) == 0) break;
+            if (ctx->h_ctl[kCtlRound + k % 3] == 0) break;
         }
         sweeps = k;
     }
