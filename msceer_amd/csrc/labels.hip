@@ -167,7 +167,7 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
     if (rc) return rc;
     uint32_t* ctl = ctl_dev(ctx);
     const MList list1 = crit_list(ctx, caps, 0), list2 = crit_list(ctx, caps, 1);
-    LabelBuffers b;
+    LabelBuffers b{};
     b.values = d_values;
     b.gradf = ctx->gradf.as<float4>();
     b.gradd = ctx->gradd.as<double4>();
