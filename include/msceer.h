@@ -140,6 +140,25 @@ int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t nx, int64_
                         uint8_t* grad_out, int32_t* labels_out, msceer_label_stats* stats, uint32_t flags);
 
 /*
+ * The numeric integrator alone on given terminals: replaces
+ * NumericIntegratorPathCompressingToTerminal::BeginIntegration(remap, terminals)
+ * (gi_numeric_integrator_path_compressing.h:75-433) with
+ * AdaptiveEulerAdvector3D<F,sign> (gi_adaptive_euler_advector.h:51-305).
+ * terminals (X*Y*Z int32): -1 = integrate this vertex, any other value = a
+ * certain terminal carrying that label.  zero_grad (X*Y*Z bytes, may be NULL):
+ * vertices whose gradient is zero (the seeds of the remap argument).
+ * sign -1 flows downhill (ascending manifolds), +1 uphill (descending).
+ * labels_out (X*Y*Z int32): the label every vertex reaches; pass 1 in FP32,
+ * pass 3 in FP64 as in msceer_ondemand_asc; no component cleanup.  Flags:
+ * MSCEER_DEVICE_PTRS (all arrays in device memory), MSCEER_TIMING.
+ * stats (may be NULL): the integrator fields and ms_integrate.
+ */
+int msceer_integrate_labels(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz,
+                            const int32_t* terminals, const uint8_t* zero_grad, float err_thr, float grad_thr,
+                            int32_t max_iters, int32_t sign, int32_t* labels_out, msceer_label_stats* stats,
+                            uint32_t flags);
+
+/*
  * Batch of independent volumes (no counterpart in the reference, which
  * processes one volume per program run): for each volume, the lower-star
  * .grad (msceer_lstar_grad) when labels_out is NULL, else the accurate

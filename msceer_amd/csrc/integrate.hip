@@ -71,6 +71,12 @@ __global__ void seed_scatter_kernel(const int32_t* __restrict__ seed_vert, const
     if (seedmask) seedmask[v] = 1;
 }
 
+__global__ void zero_seed_gradient_kernel(VGrid g, const uint8_t* __restrict__ seedmask, float4* __restrict__ gradf) {
+    const int64_t n = g.verts();
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        if (seedmask[i]) gradf[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
 __global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
 }
@@ -112,8 +118,8 @@ grad_field64_kernel(const float* __restrict__ f, VGrid g, const uint8_t* __restr
 constexpr int kPath = 32;
 
 __global__ void __launch_bounds__(kThreads)
-pass1_kernel(GradSrc<float> src, VGrid g, LabelBuffers b, int stride, int first_level, float gthr, float ethr,
-             int max_iters) {
+pass1_kernel(GradSrc<float> src, VGrid g, LabelBuffers b, int stride, int first_level, float dir, float gthr,
+             float ethr, int max_iters) {
     __shared__ int32_t path_mem[kPath * kThreads];
     int32_t* path = path_mem + threadIdx.x;
     const int nx = (g.X + stride - 1) / stride, ny = (g.Y + stride - 1) / stride, nz = (g.Z + stride - 1) / stride;
@@ -170,7 +176,7 @@ pass1_kernel(GradSrc<float> src, VGrid g, LabelBuffers b, int stride, int first_
         if (!active) continue;
         int64_t next = 0;
         bool moved = false;
-        int ev = tracer_iterate(src, g, b.dest, true, gthr, ethr, tr, next, moved);
+        int ev = tracer_iterate(src, g, b.dest, true, dir, gthr, ethr, tr, next, moved);
         if (ev == kOutOfVoxel) {
             next = tracer_reenter(src, g, tr);
             moved = true;
@@ -430,7 +436,7 @@ __global__ void pass3_prepare_kernel(WorkQueue q) {
 
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
-pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T gthr, T ethr, int max_iters) {
+pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr, T ethr, int max_iters) {
     __shared__ uint32_t ring_mem[kThreads / 64][kRing];
     uint32_t* ring = ring_mem[threadIdx.x >> 6];
     const int lane = (int)(threadIdx.x & 63u);
@@ -549,7 +555,7 @@ pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T gthr, T eth
         if (active) {
             int64_t next = 0;
             bool moved = false;
-            const int ev = tracer_iterate(src, g, b.dest, false, gthr, ethr, tr, next, moved);
+            const int ev = tracer_iterate(src, g, b.dest, false, dir, gthr, ethr, tr, next, moved);
             if (ev == kOutOfVoxel) {
                 tracer_reenter(src, g, tr);
             } else if (ev != kNone) {
@@ -972,6 +978,11 @@ hipError_t launch_grad_field(const VGrid& g, const float* values, float4* gradf,
     hipLaunchKernelGGL(grad_field_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, values, g, gradf);
     return hipGetLastError();
 }
+hipError_t launch_zero_seed_gradient(const VGrid& g, const LabelBuffers& b, hipStream_t s) {
+    hipLaunchKernelGGL(zero_seed_gradient_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, g, b.seedmask,
+                       b.gradf);
+    return hipGetLastError();
+}
 hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s) {
     hipLaunchKernelGGL(fill_i32_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, p, n, v);
     return hipGetLastError();
@@ -1030,11 +1041,11 @@ hipError_t launch_dest_init(const VGrid& g, const LabelBuffers& b, hipStream_t s
     return hipGetLastError();
 }
 hipError_t launch_pass1(const VGrid& g, const LabelBuffers& b, int stride, bool first_level, float gthr, float ethr,
-                        int max_iters, hipStream_t s) {
+                        int max_iters, int sign, hipStream_t s) {
     GradSrc<float> src{b.gradf, b.values, b.seedmask, g};
     const int64_t n = (int64_t)((g.X + stride - 1) / stride) * ((g.Y + stride - 1) / stride) * ((g.Z + stride - 1) / stride);
     hipLaunchKernelGGL(pass1_kernel, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, src, g, b, stride,
-                       first_level ? 1 : 0, gthr, ethr, max_iters);
+                       first_level ? 1 : 0, (float)sign, gthr, ethr, max_iters);
     return hipGetLastError();
 }
 hipError_t launch_pass2(const VGrid& g, const LabelBuffers& b, const WorkQueue& q, hipStream_t s) {
@@ -1048,12 +1059,12 @@ hipError_t launch_grad_field64(const VGrid& g, const LabelBuffers& b, hipStream_
     return hipGetLastError();
 }
 hipError_t launch_pass3(const VGrid& g, const LabelBuffers& b, const WorkQueue& q, float gthr, float ethr,
-                        int max_iters, hipStream_t s) {
+                        int max_iters, int sign, hipStream_t s) {
     static uint32_t blocks = resident_blocks(pass3_kernel<double>);
     GradSrc<double> src{b.gradd};
     hipLaunchKernelGGL(pass3_prepare_kernel, dim3(1), dim3(64), 0, s, q);
-    hipLaunchKernelGGL(pass3_kernel<double>, dim3(blocks), dim3(kThreads), 0, s, src, g, b, q, (double)gthr,
-                       (double)ethr, max_iters);
+    hipLaunchKernelGGL(pass3_kernel<double>, dim3(blocks), dim3(kThreads), 0, s, src, g, b, q, (double)sign,
+                       (double)gthr, (double)ethr, max_iters);
     return hipGetLastError();
 }
 hipError_t launch_cc_init(const VGrid& g, const LabelBuffers& b, hipStream_t s) {

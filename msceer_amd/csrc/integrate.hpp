@@ -51,6 +51,8 @@ enum IntegStat : int {
 
 hipError_t launch_grad_field(const VGrid& g, const float* values, float4* gradf, hipStream_t s);
 hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
+// b.gradf = 0 where b.seedmask is set (the region extrema).
+hipError_t launch_zero_seed_gradient(const VGrid& g, const LabelBuffers& b, hipStream_t s);
 hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 // Seeds: label[seed_vert[i]] = seed_label[i]; zero gradient; seedmask = 1.
@@ -58,9 +60,10 @@ hipError_t launch_seed_scatter(const int32_t* seed_vert, const int32_t* seed_lab
                                hipStream_t s);
 // dest = UNASSIGNED where label == -1 else CERTAIN_TERMINAL; redo = 0; stamp = 0.
 hipError_t launch_dest_init(const VGrid& g, const LabelBuffers& b, hipStream_t s);
-// Pass 1 at one level of the coarse-to-fine order.
+// Pass 1 at one level of the coarse-to-fine order (sign -1: flow downhill to
+// minima, +1: uphill to maxima; the same for pass 3).
 hipError_t launch_pass1(const VGrid& g, const LabelBuffers& b, int stride, bool first_level, float gthr, float ethr,
-                        int max_iters, hipStream_t s);
+                        int max_iters, int sign, hipStream_t s);
 // Pass 2: every ASSIGNED vertex with a differently labelled All6 neighbour
 // (or flagged for re-tracing) goes to `out`.
 // Re-integration work queue: `items` (capacity entries, 0xFFFFFFFF until
@@ -94,7 +97,7 @@ hipError_t launch_grad_field64(const VGrid& g, const LabelBuffers& b, hipStream_
 // Persistent FP64 pass 3 over the queue filled by pass 2 (zero head, tail,
 // pending and chunk_head before pass 2).
 hipError_t launch_pass3(const VGrid& g, const LabelBuffers& b, const WorkQueue& q, float gthr, float ethr,
-                        int max_iters, hipStream_t s);
+                        int max_iters, int sign, hipStream_t s);
 // Cleanup of isolated components.
 hipError_t launch_cc_init(const VGrid& g, const LabelBuffers& b, hipStream_t s);
 // Same-label components (rep[] = earliest vertex of each component).

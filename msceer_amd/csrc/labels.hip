@@ -116,6 +116,56 @@ int run_cleanup(msceer_ctx* ctx, const VGrid& g, const LabelBuffers& b) {
     return MSCEER_OK;
 }
 
+// Passes 1-3 of the integrator (NumericIntegratorPathCompressingToTerminal::
+// BeginIntegration, gi_numeric_integrator_path_compressing.h:75-433): labels
+// >= 0 in b.label on entry are certain terminals, b.gradf and b.seedmask are
+// set, the integrator counters are zero.  With d_mask, vertices outside the
+// mask take their combinatorial label from d_grad and count as assigned.
+// sign -1 integrates ascending manifolds (downhill), +1 descending (uphill).
+int run_integration(msceer_ctx* ctx, const VGrid& g, const LabelBuffers& b, const uint8_t* d_mask,
+                    const uint8_t* d_grad, float grad_thr, float err_thr, int32_t max_iters, int sign,
+                    msceer_label_stats& st) {
+    hipStream_t s = ctx->stream;
+    uint32_t* ctl = ctl_dev(ctx);
+    const int64_t V = g.verts();
+    CHECK_HIP(launch_dest_init(g, b, s));
+    if (d_mask) {
+        // vertices outside the mask: combinatorial label, treated as assigned
+        CHECK_HIP(launch_combinatorial_labels(g, d_grad, b.label, b.label2, nullptr, s));
+        CHECK_HIP(launch_apply_mask(g, b, d_mask, s));
+    }
+    const int max_dim = std::max(g.X, std::max(g.Y, g.Z));
+    int top = 1;
+    while (top * 2 <= 8 && top * 2 < max_dim) top *= 2;
+    for (int stride = top; stride >= 1; stride /= 2)
+        CHECK_HIP(launch_pass1(g, b, stride, stride == top, grad_thr, err_thr, max_iters, sign, s));
+    // pass 2 fills the re-integration queue, pass 3 drains it in one launch
+    const WorkQueue q{ctx->front0.as<uint32_t>(), (uint32_t)V, ctl + kCtlCursor, ctl + kCtlCursor + 1,
+                      ctl + kCtlCursor + 2, ctl + kCtlOverflow, ctl + kCtlCursor + 3, ctl + kCtlCursor + 4};
+    CHECK_HIP(launch_fill_u32(q.items, V, 0xFFFFFFFFu, s));
+    CHECK_HIP(launch_zero(ctl + kCtlCursor, 5, s));
+    CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
+    CHECK_HIP(launch_pass2(g, b, q, s));
+    CHECK_HIP(launch_grad_field64(g, b, s));
+    CHECK_HIP(launch_pass3(g, b, q, grad_thr, err_thr, max_iters, sign, s));
+    int rc = read_ctl(ctx, 0, kCtlWords);
+    if (rc) return rc;
+    if (ctx->h_ctl[kCtlOverflow] & 2u) return MSCEER_EHIP;  // the queue stalled (never expected)
+    if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;
+    st.pass3_rounds = 1;
+    st.traced_fp32 = ctx->h_ctl[kCtlIntStats + kStatTraced1];
+    st.over_budget_fp32 = ctx->h_ctl[kCtlIntStats + kStatOver1];
+    st.reintegrated_fp64 = ctx->h_ctl[kCtlIntStats + kStatTraced3];
+    st.over_budget_fp64 = ctx->h_ctl[kCtlIntStats + kStatOver3];
+    st.integrated_vertices = d_mask ? (int64_t)ctx->h_ctl[kCtlIntStats + kStatMasked] : V;
+    auto u64 = [&](int k) {
+        return (int64_t)((uint64_t)ctx->h_ctl[kCtlIntStats + k] | ((uint64_t)ctx->h_ctl[kCtlIntStats + k + 1] << 32));
+    };
+    st.steps_fp32 = u64(kStatSteps1);
+    st.steps_fp64 = u64(kStatSteps3);
+    return MSCEER_OK;
+}
+
 }  // namespace
 }  // namespace msceer
 
@@ -302,42 +352,8 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
     if (flags & MSCEER_STOP_REGIONS) return finish_early();
 
     // ---- integration ----
-    CHECK_HIP(launch_dest_init(g, b, s));
-    if (d_mask) {
-        // vertices outside the mask: combinatorial label, treated as assigned
-        CHECK_HIP(launch_combinatorial_labels(g, d_grad, b.label, b.label2, nullptr, s));
-        CHECK_HIP(launch_apply_mask(g, b, d_mask, s));
-    }
-    const int max_dim = (int)std::max(nx, std::max(ny, nz));
-    int top = 1;
-    while (top * 2 <= 8 && top * 2 < max_dim) top *= 2;
-    for (int stride = top; stride >= 1; stride /= 2)
-        CHECK_HIP(launch_pass1(g, b, stride, stride == top, grad_thr, err_thr, max_iters, s));
-    // pass 2 fills the re-integration queue, pass 3 drains it in one launch
-    const WorkQueue q{ctx->front0.as<uint32_t>(), (uint32_t)V, ctl + kCtlCursor, ctl + kCtlCursor + 1,
-                      ctl + kCtlCursor + 2, ctl + kCtlOverflow, ctl + kCtlCursor + 3, ctl + kCtlCursor + 4};
-    CHECK_HIP(launch_fill_u32(q.items, V, 0xFFFFFFFFu, s));
-    CHECK_HIP(launch_zero(ctl + kCtlCursor, 5, s));
-    CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
-    CHECK_HIP(launch_pass2(g, b, q, s));
-    CHECK_HIP(launch_grad_field64(g, b, s));
-    CHECK_HIP(launch_pass3(g, b, q, grad_thr, err_thr, max_iters, s));
-    const int rounds = 1;
-    rc = read_ctl(ctx, 0, kCtlWords);
+    rc = run_integration(ctx, g, b, d_mask, d_grad, grad_thr, err_thr, max_iters, -1, st);
     if (rc) return rc;
-    if (ctx->h_ctl[kCtlOverflow] & 2u) return MSCEER_EHIP;  // the queue stalled (never expected)
-    if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;
-    st.pass3_rounds = rounds;
-    st.traced_fp32 = ctx->h_ctl[kCtlIntStats + kStatTraced1];
-    st.over_budget_fp32 = ctx->h_ctl[kCtlIntStats + kStatOver1];
-    st.reintegrated_fp64 = ctx->h_ctl[kCtlIntStats + kStatTraced3];
-    st.over_budget_fp64 = ctx->h_ctl[kCtlIntStats + kStatOver3];
-    st.integrated_vertices = d_mask ? (int64_t)ctx->h_ctl[kCtlIntStats + kStatMasked] : V;
-    auto u64 = [&](int k) {
-        return (int64_t)((uint64_t)ctx->h_ctl[kCtlIntStats + k] | ((uint64_t)ctx->h_ctl[kCtlIntStats + k + 1] << 32));
-    };
-    st.steps_fp32 = u64(kStatSteps1);
-    st.steps_fp64 = u64(kStatSteps3);
     CHECK_HIP(hipEventRecord(ev[3], s));
     if (flags & MSCEER_STOP_INTEGRATED) return finish_early();
 
@@ -516,5 +532,78 @@ extern "C" int msceer_constrained_lstar(msceer_ctx* ctx, const float* values, in
     }
     if (!dev) CHECK_HIP(hipMemcpyAsync(grad_inout, d_grad, d.cells, hipMemcpyDeviceToHost, s));
     CHECK_HIP(hipStreamSynchronize(s));
+    return MSCEER_OK;
+}
+
+// NumericIntegratorPathCompressingToTerminal::BeginIntegration on given
+// terminals (gi_numeric_integrator_path_compressing.h:75-433): labels >= 0 are
+// certain terminals, -1 vertices are integrated; the gradient is zero where
+// zero_grad is set (the seeds of BeginIntegration's remap).  sign -1 flows
+// downhill (ascending manifolds, AdaptiveEulerAdvector3D<F,-1>), +1 uphill.
+// Passes 1-3 as in msceer_ondemand_asc; no cleanup.
+extern "C" int msceer_integrate_labels(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz,
+                                       const int32_t* terminals, const uint8_t* zero_grad, float err_thr,
+                                       float grad_thr, int32_t max_iters, int32_t sign, int32_t* labels_out,
+                                       msceer_label_stats* stats, uint32_t flags) {
+    if (!ctx || !values || !terminals || !labels_out) return MSCEER_EINVAL;
+    if (flags & ~(MSCEER_DEVICE_PTRS | MSCEER_TIMING)) return MSCEER_EINVAL;
+    if (sign != -1 && sign != 1) return MSCEER_EINVAL;
+    if (!(err_thr > 0.f) || !(grad_thr >= 0.f) || max_iters < 1) return MSCEER_EINVAL;
+    int rc = check_dims(nx, ny, nz);
+    if (rc) return rc;
+    if (nx * ny * nz >= (int64_t)1 << 31) return MSCEER_ETOOBIG;
+    CHECK_HIP(hipSetDevice(ctx->device));
+    const bool dev = flags & MSCEER_DEVICE_PTRS;
+    const Dims d = make_dims(nx, ny, nz);
+    const VGrid g = make_vgrid(nx, ny, nz);
+    const int64_t V = (int64_t)d.verts;
+    hipStream_t s = ctx->stream;
+    hipEvent_t* ev = ctx->ev;
+    msceer_label_stats st{};
+    const float* d_values = nullptr;
+    rc = stage_in(ctx, ctx->values, values, (size_t)V, dev, &d_values);
+    if (rc) return rc;
+    CHECK_HIP(ctx->gradf.ensure(sizeof(float4) * V));
+    CHECK_HIP(ctx->gradd.ensure(sizeof(double4) * V));
+    CHECK_HIP(ctx->label.ensure(sizeof(int32_t) * V));
+    CHECK_HIP(ctx->label2.ensure(sizeof(int32_t) * V));
+    CHECK_HIP(ctx->stamp.ensure(sizeof(uint32_t) * V));
+    CHECK_HIP(ctx->dest.ensure(V));
+    CHECK_HIP(ctx->seedmask.ensure(V));
+    CHECK_HIP(ctx->redo.ensure(V));
+    ListCaps caps{};
+    rc = ensure_lists(ctx, d, &caps);  // the re-integration queue lives in the frontier buffer
+    if (rc) return rc;
+    uint32_t* ctl = ctl_dev(ctx);
+    LabelBuffers b{};
+    b.values = d_values;
+    b.gradf = ctx->gradf.as<float4>();
+    b.gradd = ctx->gradd.as<double4>();
+    b.label = ctx->label.as<int32_t>();
+    b.label2 = ctx->label2.as<int32_t>();
+    b.dest = ctx->dest.as<uint8_t>();
+    b.seedmask = ctx->seedmask.as<uint8_t>();
+    b.redo = ctx->redo.as<uint8_t>();
+    b.stamp = ctx->stamp.as<uint32_t>();
+    b.stats = ctl + kCtlIntStats;
+    CHECK_HIP(hipEventRecord(ev[0], s));
+    CHECK_HIP(hipMemcpyAsync(b.label, terminals, sizeof(int32_t) * V,
+                             dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    if (zero_grad)
+        CHECK_HIP(hipMemcpyAsync(b.seedmask, zero_grad, V, dev ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, s));
+    else
+        CHECK_HIP(launch_fill_u8(b.seedmask, V, 0, s));
+    CHECK_HIP(launch_grad_field(g, d_values, b.gradf, s));
+    CHECK_HIP(launch_zero_seed_gradient(g, b, s));
+    CHECK_HIP(launch_zero(ctl, kCtlWords, s));
+    rc = run_integration(ctx, g, b, nullptr, nullptr, grad_thr, err_thr, max_iters, sign, st);
+    if (rc) return rc;
+    CHECK_HIP(hipMemcpyAsync(labels_out, b.label, sizeof(int32_t) * V,
+                             dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    CHECK_HIP(hipEventRecord(ev[1], s));
+    CHECK_HIP(hipStreamSynchronize(s));
+    st.ms_integrate = elapsed(ev[0], ev[1]);
+    st.ms_total = st.ms_integrate;
+    if (stats) *stats = st;
     return MSCEER_OK;
 }

@@ -152,8 +152,8 @@ __device__ __forceinline__ void tracer_begin(Tracer<T>& t) {
 // position (only a position changed outside a step could lead there).
 template <typename T>
 __device__ __forceinline__ int tracer_iterate(const GradSrc<T>& src, const VGrid& g,
-                                              const uint8_t* __restrict__ dest, bool stop_on_assigned, T gthr,
-                                              T ethr, Tracer<T>& t, int64_t& next, bool& moved) {
+                                              const uint8_t* __restrict__ dest, bool stop_on_assigned, T dir,
+                                              T gthr, T ethr, Tracer<T>& t, int64_t& next, bool& moved) {
     moved = false;
     if (!(t.iters-- > 0)) return kOverMax;
     if (!(t.old.x == t.cur.x && t.old.y == t.cur.y && t.old.z == t.cur.z)) {
@@ -165,8 +165,8 @@ __device__ __forceinline__ int tracer_iterate(const GradSrc<T>& src, const VGrid
         t.old = t.cur;
     }
     if (t.gm < gthr) return kLowGradient;
-    // ascending manifolds: flow downhill (step multiplier -1)
-    const T s = -t.step, hs = -(T)0.5 * t.step;
+    // dir = -1: ascending manifolds, flow downhill; +1: descending, uphill
+    const T s = dir * t.step, hs = (dir * (T)0.5) * t.step;
     const V3<T> full{t.gr.x * s, t.gr.y * s, t.gr.z * s};
     const V3<T> half{t.gr.x * hs, t.gr.y * hs, t.gr.z * hs};
     const V3<T> hg = trilin(t.c, t.cur.x + half.x - (T)t.bx, t.cur.y + half.y - (T)t.by, t.cur.z + half.z - (T)t.bz);

@@ -60,6 +60,36 @@ def _ptr(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
 
+def integrate_labels(values: np.ndarray, terminals: np.ndarray, zero_grad: np.ndarray | None = None,
+                     err_thr: float = 1e-3, grad_thr: float = 1e-6, max_iters: int = 4096, sign: int = -1,
+                     ctx: native.Context | None = None):
+    """NumericIntegratorPathCompressingToTerminal::BeginIntegration on given terminals.
+
+    terminals: int32 per vertex, -1 = integrate, else a certain terminal with that
+    label; zero_grad: optional uint8 per vertex, gradient set to zero there;
+    sign -1 flows downhill (ascending manifolds), +1 uphill.  Returns
+    (labels (Z,Y,X) int32, stats dict).
+    """
+    v = np.ascontiguousarray(values, np.float32)
+    Z, Y, X = v.shape
+    t = np.ascontiguousarray(terminals, np.int32)
+    assert t.shape == v.shape
+    zg = None if zero_grad is None else np.ascontiguousarray(zero_grad, np.uint8)
+    out = np.empty(v.shape, np.int32)
+    st = native.LabelStats()
+    own = ctx is None
+    ctx = ctx or native.Context()
+    try:
+        native.check(native.lib().msceer_integrate_labels(ctx.handle, _ptr(v), X, Y, Z, _ptr(t),
+                                                          _ptr(zg) if zg is not None else None, float(err_thr),
+                                                          float(grad_thr), int(max_iters), int(sign), _ptr(out),
+                                                          ctypes.byref(st), 0), "msceer_integrate_labels")
+    finally:
+        if own:
+            ctx.close()
+    return out, st.as_dict()
+
+
 def clean_labels(values: np.ndarray, labels: np.ndarray, ctx: native.Context | None = None) -> np.ndarray:
     """IsolatedCCRegionRemoverNEW::ComputeOutput on a copy of `labels`."""
     v = np.ascontiguousarray(values, np.float32)

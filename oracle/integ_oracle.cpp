@@ -183,10 +183,10 @@ inline double mag(const V3d& a) { return std::sqrt(a.v[0] * a.v[0] + a.v[1] * a.
 // (gi_adaptive_euler_advector.h:51-305).  kFp64 selects all-FP64 arithmetic.
 template <bool kFp64>
 Event advect_voxel(const Grid& g, const std::vector<V3d>& grad, const Checker& chk, float gthr, float ethr,
-                   bool near_boundary, V3d& cur, int& iters) {
+                   bool near_boundary, V3d& cur, int& iters, int kMul) {
     typedef typename std::conditional<kFp64, double, float>::type F;
     const double kMaxStep = 0.5;  // MAX_STEP_IN_GRID_CELLS
-    const int kMul = -1;          // ascending manifolds flow downhill
+    // kMul: -1 for ascending manifolds (flow downhill), +1 for descending
     V3d vals[8];
     V3d old = cur;
     const V3l base = near_boundary ? trunc_l(g.inbounds(cur)) : trunc_l(cur);
@@ -249,17 +249,17 @@ Event advect_voxel(const Grid& g, const std::vector<V3d>& grad, const Checker& c
 // every voxel into `path` when given.
 template <bool kFp64>
 idx_t trace(const Grid& g, const std::vector<V3d>& grad, const Checker& chk, float gthr, float ethr, int max_iters,
-            idx_t start, std::vector<idx_t>* path, Event* ev) {
+            idx_t start, std::vector<idx_t>* path, Event* ev, int mul = -1) {
     V3l c = g.coords(start);
     V3d cur{{(double)c.v[0], (double)c.v[1], (double)c.v[2]}};
     int iters = max_iters;
     while (true) {
         Event e;
         if (g.dist_to_boundary(c) <= 1) {
-            e = advect_voxel<kFp64>(g, grad, chk, gthr, ethr, true, cur, iters);
+            e = advect_voxel<kFp64>(g, grad, chk, gthr, ethr, true, cur, iters, mul);
             c = trunc_l(g.inbounds(V3d{{cur.v[0] + 0.5, cur.v[1] + 0.5, cur.v[2] + 0.5}}));
         } else {
-            e = advect_voxel<kFp64>(g, grad, chk, gthr, ethr, false, cur, iters);
+            e = advect_voxel<kFp64>(g, grad, chk, gthr, ethr, false, cur, iters, mul);
             c = trunc_l(V3d{{cur.v[0] + 0.5, cur.v[1] + 0.5, cur.v[2] + 0.5}});
         }
         const idx_t next = g.index(c);
@@ -276,7 +276,7 @@ idx_t trace(const Grid& g, const std::vector<V3d>& grad, const Checker& chk, flo
 // (gi_numeric_integrator_path_compressing.h:75-433).
 template <bool kFp64>
 void integrate(const Grid& g, const float* f, std::vector<V3d> grad, const std::vector<std::vector<idx_t>>& seeds,
-               float ethr, float gthr, int max_iters, int* label, long long* stats) {
+               float ethr, float gthr, int max_iters, int* label, long long* stats, int mul = -1) {
     (void)f;
     const idx_t n = g.X * g.Y * g.Z;
     std::vector<Dest> dest((size_t)n);
@@ -296,7 +296,7 @@ void integrate(const Grid& g, const float* f, std::vector<V3d> grad, const std::
             path.clear();
             path.push_back(i);
             Event ev;
-            const idx_t end = trace<kFp64>(g, grad, chk1, gthr, ethr, max_iters, i, &path, &ev);
+            const idx_t end = trace<kFp64>(g, grad, chk1, gthr, ethr, max_iters, i, &path, &ev, mul);
             traced++;
             if (ev == OVER_MAX_ITERATIONS) steps_over++;
             const int lab = label[end];
@@ -329,7 +329,7 @@ void integrate(const Grid& g, const float* f, std::vector<V3d> grad, const std::
         for (idx_t v : work) {
             const int init = label[v];
             Event ev;
-            const idx_t end = trace<kFp64>(g, grad, chk3, gthr, ethr, max_iters, v, nullptr, &ev);
+            const idx_t end = trace<kFp64>(g, grad, chk3, gthr, ethr, max_iters, v, nullptr, &ev, mul);
             traced++;
             if (ev == OVER_MAX_ITERATIONS) steps_over++;
             const int nl = label[end];
@@ -647,16 +647,23 @@ int oracle_certain_regions(const float* f, long long X, long long Y, long long Z
 
 // Path-compressing integration from certain regions (labels in/out).
 // seeds: per-vertex flag of region seeds (their gradient is zeroed).
-int oracle_integrate(const float* f, long long X, long long Y, long long Z, const signed char* seeds, float ethr,
-                     float gthr, int max_iters, int mode, int* labels, long long* stats) {
+// sign -1 integrates ascending manifolds (downhill), +1 descending (uphill).
+int oracle_integrate_signed(const float* f, long long X, long long Y, long long Z, const signed char* seeds,
+                            float ethr, float gthr, int max_iters, int sign, int mode, int* labels, long long* stats) {
+    if (sign != -1 && sign != 1) return -1;
     Grid g(X, Y, Z);
     std::vector<V3d> gr;
     gradient_field(g, f, gr);
     std::vector<std::vector<idx_t>> sd(1);
     for (idx_t i = 0; i < X * Y * Z; i++) if (seeds && seeds[i]) sd[0].push_back(i);
-    if (mode == 0) integrate<false>(g, f, gr, sd, ethr, gthr, max_iters, labels, stats);
-    else integrate<true>(g, f, gr, sd, ethr, gthr, max_iters, labels, stats);
+    if (mode == 0) integrate<false>(g, f, gr, sd, ethr, gthr, max_iters, labels, stats, sign);
+    else integrate<true>(g, f, gr, sd, ethr, gthr, max_iters, labels, stats, sign);
     return 0;
+}
+
+int oracle_integrate(const float* f, long long X, long long Y, long long Z, const signed char* seeds, float ethr,
+                     float gthr, int max_iters, int mode, int* labels, long long* stats) {
+    return oracle_integrate_signed(f, X, Y, Z, seeds, ethr, gthr, max_iters, -1, mode, labels, stats);
 }
 
 int oracle_clean(const float* f, long long X, long long Y, long long Z, int* labels) {

@@ -94,6 +94,7 @@ def _integ_sigs():
     h.oracle_certain_regions.argtypes = [_P, _L, _L, _L, _P, _F, _P, _P]
     h.oracle_certain_regions.restype = ctypes.c_int
     h.oracle_integrate.argtypes = [_P, _L, _L, _L, _P, _F, _F, _I, _I, _P, _P]
+    h.oracle_integrate_signed.argtypes = [_P, _L, _L, _L, _P, _F, _F, _I, _I, _I, _P, _P]
     h.oracle_clean.argtypes = [_P, _L, _L, _L, _P]
     h.oracle_min_boundary.argtypes = [_P, _L, _L, _L, _P]
     h.oracle_conforming_vertices.argtypes = [_P, _L, _L, _L, _P, _P]
@@ -129,6 +130,19 @@ def integrate(f, certain, seeds, err=1e-3, gthr=1e-6, iters=4096, mode=0):
     st = np.zeros(8, np.int64)
     _integ_sigs().oracle_integrate(_p(f), X, Y, Z, _p(np.ascontiguousarray(seeds, np.int8)), err, gthr, iters,
                                    mode, _p(lab), _p(st))
+    return lab, st
+
+
+def integrate_signed(f, terminals, seeds, err=1e-3, gthr=1e-6, iters=4096, sign=-1, mode=0):
+    """As integrate, flowing downhill (sign -1) or uphill (+1)."""
+    f = np.ascontiguousarray(f, np.float32)
+    Z, Y, X = f.shape
+    lab = np.ascontiguousarray(terminals, np.int32).ravel().copy()
+    st = np.zeros(8, np.int64)
+    sd = None if seeds is None else np.ascontiguousarray(seeds, np.int8).ravel()
+    rc = _integ_sigs().oracle_integrate_signed(_p(f), X, Y, Z, _p(sd) if sd is not None else None, err, gthr, iters,
+                                               sign, mode, _p(lab), _p(st))
+    assert rc == 0
     return lab, st
 
 

@@ -89,3 +89,43 @@ def test_stage_entry_points_match_oracle(ctx, name):
     assert (r == ref["restriction"]).all()
     g = labels.constrained_grad(f, ref["restriction"], ctx=ctx).ravel()
     assert (g == ref["grad"]).all(), int((g != ref["grad"]).sum())
+
+
+@pytest.mark.parametrize("sign", [-1, 1])
+@pytest.mark.parametrize("name", ["labels_48", "smooth_34x33x32"])
+def test_integrate_labels_matches_oracle(ctx, name, sign):
+    # msceer_integrate_labels on the certain regions of f (downhill) or of -f
+    # (uphill) against the FP64 oracle integrator; pass 1 depends on the
+    # visiting order, so a small mismatch is allowed.
+    f, (err, gthr, iters, _), _ = load_ondemand(name)
+    src = f if sign < 0 else (-f).astype(np.float32)
+    term, seeds, _ = oracle_lib.certain_regions(src, oracle_lib.lstar_pairs(src), 0.0)
+    _say(f"integrate_labels {name} sign {sign} {f.shape}")
+    lab, st = labels.integrate_labels(f, term.reshape(f.shape), seeds.reshape(f.shape).astype(np.uint8), err, gthr,
+                                      iters, sign, ctx=ctx)
+    ref, _ = oracle_lib.integrate_signed(f, term, seeds, err, gthr, iters, sign=sign, mode=1)
+    mism = float((lab.ravel() != ref).mean())
+    _say(f"  mismatch vs FP64 oracle {mism:.5f}, {st}")
+    assert mism < 0.01
+    t = term.ravel()
+    assert (lab.ravel()[t != -1] == t[t != -1]).all()  # terminals keep their labels
+    assert float((lab == -1).mean()) < 0.01
+
+
+def test_integrate_labels_uphill_is_downhill_of_the_negated_field(ctx):
+    f, (err, gthr, iters, _), _ = load_ondemand("smooth_34x33x32")
+    neg = (-f).astype(np.float32)
+    term, seeds, _ = oracle_lib.certain_regions(neg, oracle_lib.lstar_pairs(neg), 0.0)
+    t, sd = term.reshape(f.shape), seeds.reshape(f.shape).astype(np.uint8)
+    up, _ = labels.integrate_labels(f, t, sd, err, gthr, iters, 1, ctx=ctx)
+    down, _ = labels.integrate_labels(neg, t, sd, err, gthr, iters, -1, ctx=ctx)
+    # identical arithmetic; only pass 1's visiting order may differ between runs
+    assert float((up != down).mean()) < 0.005
+
+
+def test_integrate_labels_rejects_bad_arguments(ctx):
+    f = np.zeros((4, 5, 6), np.float32)
+    t = np.full(f.shape, -1, np.int32)
+    for kw in ({"sign": 0}, {"sign": 2}, {"err_thr": 0.0}, {"max_iters": 0}):
+        with pytest.raises(native.MsceerError):
+            labels.integrate_labels(f, t, ctx=ctx, **kw)

@@ -69,3 +69,21 @@ def test_constrained_pass_over_all_vertices_equals_conforming_recompute(name):
     g = oracle_lib.lstar_pairs(f, ref["restriction"])
     g = oracle_lib.flags(oracle_lib.ldir(g, f.shape), f.shape)
     assert (g == ref["grad"]).all()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_oracle_uphill_is_downhill_of_the_negated_field(mode):
+    # AdaptiveEulerAdvector3D<F,+1> flows up the gradient of f, which is down
+    # the gradient of -f; every product and sum only changes sign, so the
+    # labels agree exactly (sign -1 is pinned to the reference above).
+    f, (err, gthr, iters, _), _ = load_ondemand("smooth_34x33x32")
+    neg = (-f).astype(np.float32)
+    term, seeds, _ = oracle_lib.certain_regions(neg, oracle_lib.lstar_pairs(neg), 0.0)
+    up, st_up = oracle_lib.integrate_signed(f, term, seeds, err, gthr, iters, sign=1, mode=mode)
+    down, st_down = oracle_lib.integrate_signed(neg, term, seeds, err, gthr, iters, sign=-1, mode=mode)
+    assert (up == down).all(), int((up != down).sum())
+    assert (st_up == st_down).all()
+    assert float((up == -1).mean()) < 0.01  # streamlines reach the maxima's regions
+    # flowing the other way ends elsewhere
+    other, _ = oracle_lib.integrate_signed(f, term, seeds, err, gthr, iters, sign=-1, mode=mode)
+    assert float((other != up).mean()) > 0.1

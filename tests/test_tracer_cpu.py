@@ -4,10 +4,10 @@ msceer_amd/csrc/tracer.hpp holds the device code that pass 3 runs in FP64
 (one iteration of the adaptive Euler loop per call, voxel entries folded into
 the step that leaves a voxel).  tests/tracer_harness.cpp compiles it with g++
 next to oracle/integ_oracle.cpp (the reference's voxel-by-voxel loop restated
-in FP64, gi_adaptive_euler_advector.h:51-305) and traces 165 888 streamlines
+in FP64, gi_adaptive_euler_advector.h:51-305) and traces 331 776 streamlines
 on a Gaussian-mixture field with random certain/assigned vertices, both stop
-rules and step budgets 7, 60 and 4096: every path, end vertex and event must
-be identical.  Neither side fuses products and sums (-ffp-contract=off here,
+rules, step budgets 7, 60 and 4096 and both flow directions (sign -1 down,
++1 up): every path, end vertex and event must be identical.  Neither side fuses products and sums (-ffp-contract=off here,
 `#pragma clang fp contract(off)` in the HIP build).
 """
 import os
@@ -28,5 +28,5 @@ def test_tracer_matches_oracle_fp64_streamlines(tmp_path):
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     words = r.stdout.split()
-    assert words[words.index("streamlines") + 1] == "165888"
+    assert words[words.index("streamlines") + 1] == "331776"
     assert words[words.index("differing") + 1] == "0"

@@ -1,8 +1,9 @@
 // Host build of the GPU streamline tracer (msceer_amd/csrc/tracer.hpp, the
 // code pass 3 runs in FP64) against the oracle's FP64 advector
 // (oracle/integ_oracle.cpp, included so that its internal functions are
-// visible): from every start vertex both must visit the same vertices and end
-// at the same vertex with the same event.  Test infrastructure only.
+// visible): from every start vertex, flowing down (sign -1) and up (+1), both
+// must visit the same vertices and end at the same vertex with the same event.
+// Test infrastructure only.
 //
 //   usage: tracer_harness  -> prints "streamlines N differing D", exit 0 iff D == 0
 #include <cmath>
@@ -39,15 +40,15 @@ uint64_t lcg(uint64_t& s) {
 double unit(uint64_t& s) { return (double)lcg(s) * (1.0 / 9007199254740992.0); }
 
 // tracer streamline: visited vertices (after each voxel and at the end), event
-int gpu_trace(const msceer::GradSrc<double>& src, const msceer::VGrid& g, const uint8_t* dest, bool stop, double gthr,
-              double ethr, int x, int y, int z, int max_iters, std::vector<long long>& path) {
+int gpu_trace(const msceer::GradSrc<double>& src, const msceer::VGrid& g, const uint8_t* dest, bool stop, double dir,
+              double gthr, double ethr, int x, int y, int z, int max_iters, std::vector<long long>& path) {
     using namespace msceer;
     Tracer<double> t;
     tracer_start(src, g, t, x, y, z, max_iters);
     while (true) {
         int64_t next = 0;
         bool moved = false;
-        int ev = tracer_iterate(src, g, dest, stop, gthr, ethr, t, next, moved);
+        int ev = tracer_iterate(src, g, dest, stop, dir, gthr, ethr, t, next, moved);
         if (ev == kOutOfVoxel) {
             next = tracer_reenter(src, g, t);
             moved = true;
@@ -112,19 +113,21 @@ int main() {
         }
         for (int stop = 0; stop < 2; stop++) {
             const Checker chk{reinterpret_cast<const Dest*>(dest.data()), stop != 0, &og};
-            for (int mi : {4096, 60, 7}) {
+            for (int mi : {4096, 60, 7})
+            for (int sign : {-1, 1}) {
                 for (long long v = trial; v < V; v += 5) {
                     const int x = (int)(v % X), y = (int)((v / X) % Y), z = (int)(v / ((long long)X * Y));
                     std::vector<long long> po, pg;
                     Event eo;
-                    const long long end = trace<true>(og, grad, chk, gthr, ethr, mi, v, &po, &eo);
-                    const int eg = gpu_trace(src, g, dest.data(), stop != 0, (double)gthr, (double)ethr, x, y, z, mi, pg);
+                    const long long end = trace<true>(og, grad, chk, gthr, ethr, mi, v, &po, &eo, sign);
+                    const int eg = gpu_trace(src, g, dest.data(), stop != 0, (double)sign, (double)gthr, (double)ethr,
+                                             x, y, z, mi, pg);
                     total++;
                     steps_path += (long long)po.size();
                     if (oracle_event(eg) != (int)eo || pg != po || pg.back() != end) {
                         if (differing < 5)
-                            printf("differ: start %lld stop %d budget %d: event %d/%d, path %zu/%zu vertices\n", v, stop,
-                                   mi, (int)eo, eg, po.size(), pg.size());
+                            printf("differ: start %lld stop %d budget %d sign %d: event %d/%d, path %zu/%zu vertices\n",
+                                   v, stop, mi, sign, (int)eo, eg, po.size(), pg.size());
                         differing++;
                     }
                 }
