@@ -188,6 +188,18 @@ int msceer_clean_labels(msceer_ctx* ctx, const float* values, int64_t nx, int64_
  * int8 restriction label per cell (0 or 1) from X*Y*Z int32 labels. */
 int msceer_restriction_labels(msceer_ctx* ctx, const int32_t* labels, int64_t nx, int64_t ny, int64_t nz,
                               int8_t* restriction_out, uint32_t flags);
+/*
+ * VertexLabelingToBoundaryLabeling::ComputeMAXBoundary
+ * (gi_labeling_to_bounary_labeling.h:118-195) added onto restriction_inout
+ * (cells bytes; e.g. the output of msceer_restriction_labels): +2 on every
+ * interior quad whose two hexes have different highest vertices with
+ * different valid labels, then on interior edges with such a quad among
+ * their cofacets, then on interior vertices with such an edge.  labels: the
+ * descending-manifold label of every vertex (-1 = none).
+ */
+int msceer_restriction_add_max(msceer_ctx* ctx, const float* values, const int32_t* labels, int64_t nx, int64_t ny,
+                               int64_t nz, int8_t* restriction_inout, uint32_t flags);
+
 /* Constrained lower-star pass MyRobinsNoalloc<...,4,6>::ComputeLowerStar over the
  * vertices of ondemandaccurate.cxx:1014-1052 (recomputing every lower star is
  * equivalent), then RecordGrad (ldir + flags) unless MSCEER_SKIP_LDIR. */

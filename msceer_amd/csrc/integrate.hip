@@ -779,6 +779,61 @@ cc_apply_kernel(VGrid g, LabelBuffers b) {
 
 // ---- restriction labels -------------------------------------------------------
 
+// VertexLabelingToBoundaryLabeling::ComputeMAXBoundary
+// (gi_labeling_to_bounary_labeling.h:118-195), three passes that add 2:
+// (stage 0) every interior quad whose two hexes have different highest
+// vertices with different valid labels; (stage 1) every interior edge with
+// such a quad (label > 1) among its cofacets; (stage 2) every interior vertex
+// with such an edge.  "Interior" is boundaryValue 0: no coordinate on a face.
+__device__ __forceinline__ int64_t hex_highest_vertex(const VGrid& g, const float* __restrict__ f, int64_t hx,
+                                                      int64_t hy, int64_t hz) {
+    // vertices of the hex at odd cell coordinates (hx, hy, hz)
+    const int64_t x0 = (hx - 1) / 2, y0 = (hy - 1) / 2, z0 = (hz - 1) / 2;
+    int64_t best = x0 + (int64_t)g.X * (y0 + (int64_t)g.Y * z0);
+#pragma unroll
+    for (int k = 1; k < 8; k++) {
+        const int64_t v = (x0 + (k & 1)) + (int64_t)g.X * ((y0 + ((k >> 1) & 1)) + (int64_t)g.Y * (z0 + (k >> 2)));
+        if (before(f, best, v)) best = v;  // highest by (value, vertex id)
+    }
+    return best;
+}
+
+template <int kStage>
+__global__ void __launch_bounds__(kThreads)
+max_boundary_kernel(VGrid g, const float* __restrict__ f, const int32_t* __restrict__ label, int8_t* __restrict__ r) {
+    const int64_t Mx = 2 * (int64_t)g.X - 1, My = 2 * (int64_t)g.Y - 1, Mz = 2 * (int64_t)g.Z - 1;
+    const int64_t sxy = Mx * My, C = sxy * Mz;
+    for (int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x; c < C; c += (int64_t)gridDim.x * kThreads) {
+        const int64_t cx = c % Mx, cy = (c / Mx) % My, cz = c / sxy;
+        if (cx == 0 || cx == Mx - 1 || cy == 0 || cy == My - 1 || cz == 0 || cz == Mz - 1) continue;
+        const int ox = (int)(cx & 1), oy = (int)(cy & 1), oz = (int)(cz & 1);
+        const int dim = ox + oy + oz;
+        if (kStage == 0) {
+            if (dim != 2) continue;
+            // the even axis carries the two hexes
+            const int64_t dx = ox ? 0 : 1, dy = oy ? 0 : 1, dz = oz ? 0 : 1;
+            const int64_t v1 = hex_highest_vertex(g, f, cx + dx, cy + dy, cz + dz);
+            const int64_t v2 = hex_highest_vertex(g, f, cx - dx, cy - dy, cz - dz);
+            if (v1 == v2) continue;  // both hexes in one lower star
+            const int32_t l1 = label[v1], l2 = label[v2];
+            if (l1 == -1 || l2 == -1 || l1 == l2) continue;
+            r[c] = (int8_t)(r[c] + 2);
+        } else {
+            // edges (stage 1) look at their quads, vertices (stage 2) at their edges
+            if (dim != 2 - kStage) continue;
+            const int64_t off[3] = {1, Mx, sxy};
+            const int odd[3] = {ox, oy, oz};
+            bool mark = false;
+#pragma unroll
+            for (int a = 0; a < 3; a++) {
+                if (odd[a]) continue;
+                if (r[c + off[a]] > 1 || r[c - off[a]] > 1) mark = true;
+            }
+            if (mark) r[c] = (int8_t)(r[c] + 2);
+        }
+    }
+}
+
 __global__ void __launch_bounds__(kThreads)
 min_boundary_kernel(VGrid g, const int32_t* __restrict__ label, int8_t* __restrict__ r) {
     const int64_t Mx = 2 * (int64_t)g.X - 1, My = 2 * (int64_t)g.Y - 1, Mz = 2 * (int64_t)g.Z - 1;
@@ -1105,6 +1160,14 @@ hipError_t launch_cc_apply(const VGrid& g, const LabelBuffers& b, hipStream_t s)
 hipError_t launch_min_boundary(const VGrid& g, const int32_t* label, int8_t* restriction, hipStream_t s) {
     const int64_t C = (2 * (int64_t)g.X - 1) * (2 * (int64_t)g.Y - 1) * (2 * (int64_t)g.Z - 1);
     hipLaunchKernelGGL(min_boundary_kernel, dim3(grid_for(C)), dim3(kThreads), 0, s, g, label, restriction);
+    return hipGetLastError();
+}
+hipError_t launch_max_boundary(const VGrid& g, const float* values, const int32_t* label, int8_t* restriction,
+                               hipStream_t s) {
+    const int64_t C = (2 * (int64_t)g.X - 1) * (2 * (int64_t)g.Y - 1) * (2 * (int64_t)g.Z - 1);
+    hipLaunchKernelGGL(max_boundary_kernel<0>, dim3(grid_for(C)), dim3(kThreads), 0, s, g, values, label, restriction);
+    hipLaunchKernelGGL(max_boundary_kernel<1>, dim3(grid_for(C)), dim3(kThreads), 0, s, g, values, label, restriction);
+    hipLaunchKernelGGL(max_boundary_kernel<2>, dim3(grid_for(C)), dim3(kThreads), 0, s, g, values, label, restriction);
     return hipGetLastError();
 }
 hipError_t launch_saddle_arcs(const VGrid& g, const float* values, const uint8_t* grad, const MList& list,

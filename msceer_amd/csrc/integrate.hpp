@@ -119,6 +119,12 @@ hipError_t launch_cc_apply(const VGrid& g, const LabelBuffers& b, hipStream_t s)
 // Restriction labels from vertex labels (MIN boundary): 1 on every cell
 // with an edge whose two vertices carry valid, different labels.
 hipError_t launch_min_boundary(const VGrid& g, const int32_t* label, int8_t* restriction, hipStream_t s);
+// MAX boundary (descending manifolds) added onto `restriction`: +2 on interior
+// quads between hexes whose highest vertices carry different valid labels,
+// then on interior edges under such a quad, then on interior vertices under
+// such an edge.
+hipError_t launch_max_boundary(const VGrid& g, const float* values, const int32_t* label, int8_t* restriction,
+                               hipStream_t s);
 // One arc of the extremum graph: a critical 1-cell (saddle), the critical
 // vertices its two V-paths end at, the value of its highest vertex and the
 // values at the two minima.

@@ -482,6 +482,38 @@ extern "C" int msceer_restriction_labels(msceer_ctx* ctx, const int32_t* labels,
     return MSCEER_OK;
 }
 
+// VertexLabelingToBoundaryLabeling::ComputeMAXBoundary (gi_labeling_to_bounary_labeling.h:118-195)
+// added onto an existing restriction labelling (in/out), as ondemandaccurate
+// does after the MIN boundary of the ascending labels (ondemandaccurate.cxx:988).
+extern "C" int msceer_restriction_add_max(msceer_ctx* ctx, const float* values, const int32_t* labels, int64_t nx,
+                                          int64_t ny, int64_t nz, int8_t* restriction_inout, uint32_t flags) {
+    if (!ctx || !values || !labels || !restriction_inout) return MSCEER_EINVAL;
+    if (flags & ~MSCEER_DEVICE_PTRS) return MSCEER_EINVAL;
+    int rc = check_dims(nx, ny, nz);
+    if (rc) return rc;
+    CHECK_HIP(hipSetDevice(ctx->device));
+    const bool dev = flags & MSCEER_DEVICE_PTRS;
+    const Dims d = make_dims(nx, ny, nz);
+    const VGrid g = make_vgrid(nx, ny, nz);
+    hipStream_t s = ctx->stream;
+    const float* d_values = nullptr;
+    rc = stage_in(ctx, ctx->values, values, (size_t)d.verts, dev, &d_values);
+    if (rc) return rc;
+    const int32_t* d_labels = nullptr;
+    rc = stage_in(ctx, ctx->label, labels, (size_t)d.verts, dev, &d_labels);
+    if (rc) return rc;
+    int8_t* d_r = restriction_inout;
+    if (!dev) {
+        CHECK_HIP(ctx->restriction.ensure(d.cells));
+        d_r = ctx->restriction.as<int8_t>();
+        CHECK_HIP(hipMemcpyAsync(d_r, restriction_inout, d.cells, hipMemcpyHostToDevice, s));
+    }
+    CHECK_HIP(launch_max_boundary(g, d_values, d_labels, d_r, s));
+    if (!dev) CHECK_HIP(hipMemcpyAsync(restriction_inout, d_r, d.cells, hipMemcpyDeviceToHost, s));
+    CHECK_HIP(hipStreamSynchronize(s));
+    return MSCEER_OK;
+}
+
 // Lower stars recomputed under a restriction labelling (MyRobinsNoalloc<...,4,6>,
 // gi_modified_robins.h:1281-1343; ondemandaccurate.cxx:1014-1052) followed by
 // RecordGrad (ldir + flags, ondemandaccurate.cxx:693-699) unless MSCEER_SKIP_LDIR.

@@ -122,6 +122,26 @@ def restriction_labels(labels: np.ndarray, ctx: native.Context | None = None) ->
     return out.reshape(2 * Z - 1, 2 * Y - 1, 2 * X - 1)
 
 
+def restriction_add_max(values: np.ndarray, labels: np.ndarray, restriction: np.ndarray,
+                        ctx: native.Context | None = None) -> np.ndarray:
+    """ComputeMAXBoundary of `labels` added onto a copy of `restriction` (int8 per cell)."""
+    v = np.ascontiguousarray(values, np.float32)
+    Z, Y, X = v.shape
+    lab = np.ascontiguousarray(labels, np.int32)
+    assert lab.size == v.size
+    r = np.ascontiguousarray(restriction, np.int8).ravel().copy()
+    assert r.size == num_cells(X, Y, Z)
+    own = ctx is None
+    ctx = ctx or native.Context()
+    try:
+        native.check(native.lib().msceer_restriction_add_max(ctx.handle, _ptr(v), _ptr(lab), X, Y, Z, _ptr(r), 0),
+                     "msceer_restriction_add_max")
+    finally:
+        if own:
+            ctx.close()
+    return r
+
+
 def constrained_grad(values: np.ndarray, restriction: np.ndarray, ctx: native.Context | None = None,
                      skip_ldir: bool = False) -> np.ndarray:
     """Lower stars under a restriction labelling, then ldir and flags."""

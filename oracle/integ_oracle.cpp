@@ -672,6 +672,51 @@ int oracle_clean(const float* f, long long X, long long Y, long long Z, int* lab
     return 0;
 }
 
+// VertexLabelingToBoundaryLabeling::ComputeMAXBoundary (gi_labeling_to_bounary_labeling.h:118-195),
+// added onto restriction (in/out): the highest vertex of each hex comes from
+// the max/min vertex labelling (oracle_maxmin, gi_max_vertex_labeling.h).
+int oracle_max_boundary(const float* f, const int* labels, long long X, long long Y, long long Z,
+                        signed char* restriction) {
+    CellMesh m(X, Y, Z);
+    const idx_t C = m.M[0] * m.M[1] * m.M[2];
+    std::vector<unsigned char> maxv((size_t)C), minv((size_t)C);
+    oracle_maxmin(f, X, Y, Z, maxv.data(), minv.data());
+    auto highest = [&](idx_t c) {
+        const int t = maxv[(size_t)c];
+        const idx_t d = (idx_t)(t % 3 - 1) + m.M[0] * ((idx_t)((t / 3) % 3 - 1) + m.M[1] * (idx_t)(t / 9 - 1));
+        return c + d;  // vertex cell
+    };
+    auto interior = [&](idx_t c) { return m.bv(c) == 0; };
+    // quads between hexes whose highest vertices carry different valid labels
+    for (idx_t c = 0; c < C; c++) {
+        if (m.dim(c) != 2 || !interior(c)) continue;
+        idx_t p[3];
+        m.coords(c, p);
+        int a = 0;
+        while (p[a] & 1) a++;  // the even axis
+        const idx_t v1 = highest(c + m.off(a)), v2 = highest(c - m.off(a));  // cofacets +axis, -axis
+        if (v1 == v2) continue;
+        const int l1 = labels[m.vertex_number(v1)], l2 = labels[m.vertex_number(v2)];
+        if (l1 == -1 || l2 == -1) continue;
+        if (l1 != l2) restriction[c] = (signed char)(restriction[c] + 2);
+    }
+    // then edges under such a quad, then vertices under such an edge
+    for (int d = 1; d >= 0; d--)
+        for (idx_t c = 0; c < C; c++) {
+            if (m.dim(c) != d || !interior(c)) continue;
+            idx_t p[3];
+            m.coords(c, p);
+            for (int a = 0; a < 3; a++) {
+                if (p[a] & 1) continue;  // cofacets lie along the even axes
+                if (restriction[c + m.off(a)] > 1 || restriction[c - m.off(a)] > 1) {
+                    restriction[c] = (signed char)(restriction[c] + 2);
+                    break;
+                }
+            }
+        }
+    return 0;
+}
+
 int oracle_min_boundary(const int* labels, long long X, long long Y, long long Z, signed char* restriction) {
     CellMesh m(X, Y, Z);
     const idx_t C = m.M[0] * m.M[1] * m.M[2];

@@ -97,6 +97,7 @@ def _integ_sigs():
     h.oracle_integrate_signed.argtypes = [_P, _L, _L, _L, _P, _F, _F, _I, _I, _I, _P, _P]
     h.oracle_clean.argtypes = [_P, _L, _L, _L, _P]
     h.oracle_min_boundary.argtypes = [_P, _L, _L, _L, _P]
+    h.oracle_max_boundary.argtypes = [_P, _P, _L, _L, _L, _P]
     h.oracle_conforming_vertices.argtypes = [_P, _L, _L, _L, _P, _P]
     h.oracle_conforming_vertices.restype = ctypes.c_longlong
     h.oracle_ondemand_asc.argtypes = [_P, _L, _L, _L, _F, _F, _I, _F, _I, _P, _P, _P, _P]
@@ -158,6 +159,15 @@ def min_boundary(labels, shape_zyx):
     Z, Y, X = shape_zyx
     r = np.empty(ncells(X, Y, Z), np.int8)
     _integ_sigs().oracle_min_boundary(_p(np.ascontiguousarray(labels, np.int32)), X, Y, Z, _p(r))
+    return r
+
+
+def max_boundary(f, labels, restriction):
+    """ComputeMAXBoundary of `labels` added onto a copy of `restriction`."""
+    f = np.ascontiguousarray(f, np.float32)
+    Z, Y, X = f.shape
+    r = np.ascontiguousarray(restriction, np.int8).ravel().copy()
+    _integ_sigs().oracle_max_boundary(_p(f), _p(np.ascontiguousarray(labels, np.int32)), X, Y, Z, _p(r))
     return r
 
 

@@ -129,3 +129,19 @@ def test_integrate_labels_rejects_bad_arguments(ctx):
     for kw in ({"sign": 0}, {"sign": 2}, {"err_thr": 0.0}, {"max_iters": 0}):
         with pytest.raises(native.MsceerError):
             labels.integrate_labels(f, t, ctx=ctx, **kw)
+
+
+@pytest.mark.parametrize("name", ["labels_48", "smooth_34x33x32", "gauss_44x36x40"])
+def test_max_boundary_bit_exact(ctx, name):
+    # onto the reference's MIN restriction: the reference's ascending labels,
+    # then descending labels from the oracle integrator (uphill, terminals from -f)
+    f, (err, gthr, iters, _), ref = load_ondemand(name)
+    neg = (-f).astype(np.float32)
+    term, seeds, _ = oracle_lib.certain_regions(neg, oracle_lib.lstar_pairs(neg), 0.0)
+    dsc, _ = oracle_lib.integrate_signed(f, term, seeds, err, gthr, iters, sign=1, mode=1)
+    for lab in (ref["cleaned"], dsc):
+        _say(f"max boundary {name} {f.shape}")
+        got = labels.restriction_add_max(f, lab.reshape(f.shape), ref["restriction"], ctx=ctx)
+        expect = oracle_lib.max_boundary(f, lab, ref["restriction"])
+        assert (got == expect).all(), int((got != expect).sum())
+        assert int((expect > 1).sum()) > 0

@@ -87,3 +87,28 @@ def test_oracle_uphill_is_downhill_of_the_negated_field(mode):
     # flowing the other way ends elsewhere
     other, _ = oracle_lib.integrate_signed(f, term, seeds, err, gthr, iters, sign=-1, mode=mode)
     assert float((other != up).mean()) > 0.1
+
+
+@pytest.mark.parametrize("field", ["ramp", "constant"])
+def test_oracle_max_boundary_closed_form(field):
+    # f increasing with the vertex id (or constant: ties fall back to id order),
+    # so every hex's highest vertex is its (+x,+y,+z) corner; labels split at
+    # x = k.  Only quads normal to x in the plane x = k - 1 separate hexes
+    # whose highest vertices differ in label, so exactly the interior cells
+    # with cell x-coordinate 2(k - 1) get 2 (quads, then the edges and vertices
+    # under them), and nothing else changes.
+    X, Y, Z, k = 9, 7, 6, 4
+    f = (np.arange(X * Y * Z, dtype=np.float32) if field == "ramp" else np.zeros(X * Y * Z, np.float32))
+    f = f.reshape(Z, Y, X)
+    xs = np.arange(X)[None, None, :].repeat(Y, 1).repeat(Z, 0)
+    lab = (xs >= k).astype(np.int32)
+    r = oracle_lib.max_boundary(f, lab, np.zeros((2 * Z - 1) * (2 * Y - 1) * (2 * X - 1), np.int8))
+    cz, cy, cx = np.meshgrid(np.arange(2 * Z - 1), np.arange(2 * Y - 1), np.arange(2 * X - 1), indexing="ij")
+    interior = (cx > 0) & (cx < 2 * X - 2) & (cy > 0) & (cy < 2 * Y - 2) & (cz > 0) & (cz < 2 * Z - 2)
+    expect = np.where(interior & (cx == 2 * (k - 1)), 2, 0).astype(np.int8).ravel()
+    assert (r == expect).all(), int((r != expect).sum())
+    # accumulates onto an existing labelling; constant or unassigned labels add nothing
+    ones = np.ones_like(expect)
+    assert (oracle_lib.max_boundary(f, lab, ones) == ones + expect).all()
+    assert (oracle_lib.max_boundary(f, np.zeros_like(lab), ones) == ones).all()
+    assert (oracle_lib.max_boundary(f, np.full_like(lab, -1), ones) == ones).all()
