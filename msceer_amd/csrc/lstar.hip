@@ -44,10 +44,13 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
              MList list1, MList list2, uint32_t* __restrict__ stats, uint32_t* __restrict__ overflow) {
     using B = Brick<kBX, kBY, kBZ>;
     constexpr int kTX = B::kTX, kTY = B::kTY, kTileN = B::kTileN;
-    __shared__ uint32_t tile[kTileN];
+    __shared__ uint64_t tile[kTileN];  // (order key << 32 | vertex id) of the halo'd brick
     __shared__ int s_off[27];  // tile offset of star vertex t
     __shared__ uint32_t s_cnt[4];  // per block: critical 1-cells, 2-cells, 0-cells, 3-cells
     __shared__ MSlot s_slot[2];
+    __shared__ uint32_t s_mask[kThreads];     // lower star of each vertex of the brick
+    __shared__ uint16_t s_order[kThreads];    // brick vertices by lower-star size
+    __shared__ uint32_t s_bucket[32];
     const int x0 = blockIdx.x * kBX, y0 = blockIdx.y * kBY, z0 = blockIdx.z * kBZ;
     const int flat = threadIdx.x + kBX * (threadIdx.y + kBY * threadIdx.z);
     for (int i = flat; i < kTileN; i += kThreads) {
@@ -56,21 +59,55 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
         gx = gx < 0 ? 0 : (gx >= X ? X - 1 : gx);
         gy = gy < 0 ? 0 : (gy >= Y ? Y - 1 : gy);
         gz = gz < 0 ? 0 : (gz >= Z ? Z - 1 : gz);
-        tile[i] = float_key(values[(size_t)gx + (size_t)X * ((size_t)gy + (size_t)Y * gz)]);
+        const uint32_t id = (uint32_t)gx + (uint32_t)X * ((uint32_t)gy + (uint32_t)Y * (uint32_t)gz);
+        tile[i] = ((uint64_t)float_key(values[id]) << 32) | id;
     }
     if (flat < 4) s_cnt[flat] = 0;
     if (flat < 27) s_off[flat] = tid_x(flat) + kTX * (tid_y(flat) + kTY * tid_z(flat));
+    if (flat < 32) s_bucket[flat] = 0;
     __syncthreads();
 
-    const int x = x0 + threadIdx.x, y = y0 + threadIdx.y, z = z0 + threadIdx.z;
-    const bool active = x < X && y < Y && z < Z;
+    // Lower star of the lane's own vertex (0: outside the volume; a lower star
+    // always holds its vertex).  The expansion loops of a wave run as long as
+    // its largest star needs, so the brick's vertices are then dealt to the
+    // lanes in order of lower-star size (counting sort in LDS): a wave gets
+    // stars of similar size.  Any lane can expand any vertex of the brick,
+    // since all keys are in the tile.
+    {
+        const int ox = x0 + threadIdx.x, oy = y0 + threadIdx.y, oz = z0 + threadIdx.z;
+        uint32_t own = 0;
+        if (ox < X && oy < Y && oz < Z) {
+            const int obase = (threadIdx.x + 1) + kTX * ((threadIdx.y + 1) + kTY * (threadIdx.z + 1));
+            own = lower_star_mask(FixedKeys64<kTX, kTX * kTY>{tile, obase}, ox, oy, oz, X, Y, Z);
+        }
+        s_mask[flat] = own;
+        const uint32_t w = (uint32_t)__builtin_popcount(own);
+        const uint32_t rank = atomicAdd(&s_bucket[w], 1u);
+        __syncthreads();
+        if (flat < 64) {  // exclusive prefix over the 28 sizes (one wave)
+            const uint32_t c = flat < 32 ? s_bucket[flat] : 0u;
+            uint32_t incl = c;
+#pragma unroll
+            for (int o = 1; o < 32; o <<= 1) {
+                const uint32_t t = __shfl_up(incl, o);
+                if ((flat & 63) >= o) incl += t;
+            }
+            if (flat < 32) s_bucket[flat] = incl - c;
+        }
+        __syncthreads();
+        s_order[s_bucket[w] + rank] = (uint16_t)flat;
+        __syncthreads();
+    }
+    const int me = s_order[flat];
+    const int mx = me % kBX, my = (me / kBX) % kBY, mz = me / (kBX * kBY);
+    const int x = x0 + mx, y = y0 + my, z = z0 + mz;
+    const uint32_t L = s_mask[me];
+    const bool active = L != 0u;
     uint32_t crit = 0;
     uint64_t vcell = 0;
     if (active) {
-    const int base = (threadIdx.x + 1) + kTX * ((threadIdx.y + 1) + kTY * (threadIdx.z + 1));
-    const FixedKeys<kTX, kTX * kTY> tkeys{tile, base};
-    const uint32_t L = lower_star_mask(tkeys, x, y, z, X, Y, Z);
-    const TableKeys keys{tile + base, s_off};
+    const int base = (mx + 1) + kTX * ((my + 1) + kTY * (mz + 1));
+    const TableKeys64 keys{tile + base, s_off};
     uint32_t bx, by, bz;
     star_boundary(x, y, z, X, Y, Z, bx, by, bz);
     const uint32_t onb = bx | by | bz;

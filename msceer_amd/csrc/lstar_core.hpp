@@ -26,11 +26,28 @@ MSC_HD int comp_z(int t) { return (int)((kZP >> t) & 1u) - (int)((kZM >> t) & 1u
 
 // Keys of the 27 vertices around one vertex, read through fixed strides (an
 // LDS brick) or run-time strides (a padded host volume).
+// Every key source gives at64(t): the order key of star vertex t as a 64-bit
+// integer, comparing as the reference order restricted to one neighbourhood.
+// 32-bit sources append the star bit (star-bit order is global-id order for
+// the vertices of one 3x3x3 block); 64-bit sources hold (key << 32 | id).
 template <int SX, int SXY>
 struct FixedKeys {
     const uint32_t* p;
     int base;
     MSC_HD uint32_t at(int t) const { return p[base + tid_x(t) + SX * tid_y(t) + SXY * tid_z(t)]; }
+    MSC_HD uint64_t at64(int t) const { return ((uint64_t)at(t) << 5) | (uint64_t)t; }
+};
+// 64-bit (key << 32 | vertex id) entries through fixed strides or a table.
+template <int SX, int SXY>
+struct FixedKeys64 {
+    const uint64_t* p;
+    int base;
+    MSC_HD uint64_t at64(int t) const { return p[base + tid_x(t) + SX * tid_y(t) + SXY * tid_z(t)]; }
+};
+struct TableKeys64 {
+    const uint64_t* p;   // centre of the lane's star in the tile
+    const int* off;
+    MSC_HD uint64_t at64(int t) const { return p[off[t]]; }
 };
 // One lane's 27 keys laid out t-major with a fixed lane stride (an LDS slice):
 // key t of the lane is p[t * kStride].
@@ -38,6 +55,7 @@ template <int kStride>
 struct SliceKeys {
     const uint32_t* p;
     MSC_HD uint32_t at(int t) const { return p[t * kStride]; }
+    MSC_HD uint64_t at64(int t) const { return ((uint64_t)at(t) << 5) | (uint64_t)t; }
 };
 // Keys read through a per-star offset table (tile offsets of the 27 star
 // vertices, shared by the work-group) for star indices known only at run time.
@@ -45,19 +63,19 @@ struct TableKeys {
     const uint32_t* p;   // centre of the lane's star in the tile
     const int* off;
     MSC_HD uint32_t at(int t) const { return p[off[t]]; }
+    MSC_HD uint64_t at64(int t) const { return ((uint64_t)at(t) << 5) | (uint64_t)t; }
 };
 struct StridedKeys {
     const uint32_t* p;
     int64_t base, sx, sxy;
     MSC_HD uint32_t at(int t) const { return p[base + tid_x(t) + sx * tid_y(t) + sxy * tid_z(t)]; }
+    MSC_HD uint64_t at64(int t) const { return ((uint64_t)at(t) << 5) | (uint64_t)t; }
 };
 
-// (key, star bit) of a neighbour vertex: comparing these as 64-bit integers is
-// the reference order restricted to one neighbourhood, because star-bit order
-// is global-id order for the vertices of one 3x3x3 block.
+// Order key of star vertex t (see at64 above).
 template <class Keys>
 MSC_HD uint64_t vkey(const Keys& k, int t) {
-    return ((uint64_t)k.at(t) << 5) | (uint64_t)t;
+    return k.at64(t);
 }
 
 // Bits of the cofacets of the cell at bit a (one zero component moved to +-1).
@@ -224,14 +242,12 @@ MSC_HD void expand_subset(uint32_t S, const Keys& k, StarResult& r) {
 // the domain (gi_fast_robins_noalloc.h:359-366, gi_modified_robins.h:1293-1297).
 template <class Keys>
 MSC_HD uint32_t lower_star_mask(const Keys& k, int x, int y, int z, int X, int Y, int Z) {
-    const uint32_t kc = k.at(13);
+    const uint64_t kc = k.at64(13);
     uint32_t lower = 0;
 #pragma unroll
     for (int t = 0; t < 27; t++) {
         if (t == 13) continue;
-        const uint32_t kq = k.at(t);
-        const bool lo = t < 13 ? (kq <= kc) : (kq < kc);
-        lower |= (uint32_t)lo << t;
+        lower |= (uint32_t)(k.at64(t) < kc) << t;
     }
     const uint32_t invalid = (x == 0 ? kXM : 0u) | (x == X - 1 ? kXP : 0u) | (y == 0 ? kYM : 0u) |
                              (y == Y - 1 ? kYP : 0u) | (z == 0 ? kZM : 0u) | (z == Z - 1 ? kZP : 0u);
