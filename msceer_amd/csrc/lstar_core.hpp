@@ -135,15 +135,16 @@ struct StarResult {
 template <int kCandDim, class Keys>
 MSC_HD int pick_candidate(const Keys& kk, int a, uint32_t cset) {
     if (!(cset & (cset - 1))) return __builtin_ctz(cset);
-    int best = -1;
+    // only the candidates are visited; comparing (lowest key, position in the
+    // cofacet order) makes the earliest of equal keys win, as the ordered scan
+    int best = -1, brank = 6;
     uint64_t bk = ~0ull;
-#pragma unroll
-    for (int i = 0; i < 6; i++) {
-        const int step = (i < 2 ? 1 : (i < 4 ? 3 : 9)) * ((i & 1) ? -1 : 1);
-        const int c = a + step;
-        if (c < 0 || c > 26 || !((cset >> c) & 1u)) continue;
+    for (uint32_t m = cset; m; m &= m - 1) {
+        const int c = __builtin_ctz(m);
+        const int d = c - a;
+        const int rank = d == 1 ? 0 : (d == -1 ? 1 : (d == 3 ? 2 : (d == -3 ? 3 : (d == 9 ? 4 : 5))));
         const uint64_t kc = kCandDim == 2 ? quad_lowest(kk, c) : hex_lowest(kk, c);
-        if (kc < bk) { bk = kc; best = c; }
+        if (kc < bk || (kc == bk && rank < brank)) { bk = kc; best = c; brank = rank; }
     }
     return best;
 }
