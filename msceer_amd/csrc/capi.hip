@@ -241,6 +241,7 @@ int msceer_destroy(msceer_ctx* ctx) {
     DevBuf* label_bufs[] = {&ctx->gradf, &ctx->gradd, &ctx->scan, &ctx->label, &ctx->label2, &ctx->dest, &ctx->seedmask, &ctx->redo,
                             &ctx->stamp, &ctx->rep, &ctx->rep2, &ctx->restriction, &ctx->host_io, &ctx->mask};
     for (DevBuf* b : label_bufs) b->release();
+    ctx->host_stage.release();
     if (ctx->h_ctl) (void)hipHostFree(ctx->h_ctl);
     for (int i = 0; i < 12; i++)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);

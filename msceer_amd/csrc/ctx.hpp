@@ -7,6 +7,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <vector>
 
 #include "msceer.h"
@@ -31,6 +32,30 @@ struct DevBuf {
     T* as() const { return static_cast<T*>(ptr); }
     void release() {
         if (ptr) (void)hipFree(ptr);
+        ptr = nullptr;
+        bytes = 0;
+    }
+};
+
+// Pinned host staging memory (hipHostMalloc): copies to and from it run at
+// full link speed.  Grown with 25% headroom, because the sizes it holds vary
+// a little from call to call and pinning a fresh buffer costs more than the
+// copy; never cleared.
+struct HostBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+    hipError_t ensure(size_t need) {
+        if (need <= bytes) return hipSuccess;
+        if (ptr) { (void)hipHostFree(ptr); ptr = nullptr; bytes = 0; }
+        need = ((need + need / 4) + 4095) & ~(size_t)4095;
+        hipError_t e = hipHostMalloc(&ptr, need, hipHostMallocDefault);
+        if (e == hipSuccess) bytes = need;
+        return e;
+    }
+    template <typename T>
+    T* as() const { return static_cast<T*>(ptr); }
+    void release() {
+        if (ptr) (void)hipHostFree(ptr);
         ptr = nullptr;
         bytes = 0;
     }
@@ -83,6 +108,7 @@ struct msceer_ctx {
     msceer::DevBuf values, grad, lists, front0, front1, ctl;
     // accurate labelling
     msceer::DevBuf gradf, gradd, scan, label, label2, dest, seedmask, redo, stamp, rep, rep2, restriction, host_io, mask;
+    msceer::HostBuf host_stage;  // pinned staging of host-side merges (cleanup)
     uint32_t* h_ctl = nullptr;  // pinned mirror of the control block
     hipEvent_t ev[12] = {};
     msceer_timing last{};

@@ -64,11 +64,15 @@ int run_cleanup(msceer_ctx* ctx, const VGrid& g, const LabelBuffers& b) {
     if (rc) return rc;
     const uint32_t R = ctx->h_ctl[kCtlCount];
     const size_t row = sizeof(int32_t) * kCollectStride;
-    std::vector<int32_t> h((size_t)R * kCollectStride);
+    // rows of the candidates, then the (vertex, new label) pairs (pinned
+    // staging, not cleared: every entry read is written first)
+    CHECK_HIP(ctx->host_stage.ensure(row * R + sizeof(int32_t) * 2 * (size_t)R + 64));
+    const int32_t* h = ctx->host_stage.as<int32_t>();
+    int32_t* pairs = ctx->host_stage.as<int32_t>() + (size_t)R * kCollectStride;
     if (R) {
         CHECK_HIP(ctx->host_io.ensure(std::max(row * R, sizeof(int32_t) * 2 * (size_t)R)));
         CHECK_HIP(launch_cc_collect(g, b, cand, slot_of, ctx->host_io.as<int32_t>(), s));
-        CHECK_HIP(hipMemcpyAsync(h.data(), ctx->host_io.ptr, row * R, hipMemcpyDeviceToHost, s));
+        CHECK_HIP(hipMemcpyAsync(ctx->host_stage.as<void>(), ctx->host_io.ptr, row * R, hipMemcpyDeviceToHost, s));
         CHECK_HIP(hipStreamSynchronize(s));
     }
     // candidates in vertex-id order; a set of candidates merged together has
@@ -99,19 +103,17 @@ int run_cleanup(msceer_ctx* ctx, const VGrid& g, const LabelBuffers& b) {
             break;
         }
     }
-    std::vector<int32_t> pairs(2 * (size_t)R);
     for (uint32_t k = 0; k < R; k++) {
         const int32_t q = find((int32_t)k);
-        pairs[2 * k] = h[(size_t)k * kCollectStride];
-        pairs[2 * k + 1] = term[q] >= 0 ? term[q] : h[(size_t)q * kCollectStride];
+        pairs[2 * (size_t)k] = h[(size_t)k * kCollectStride];
+        pairs[2 * (size_t)k + 1] = term[q] >= 0 ? term[q] : h[(size_t)q * kCollectStride];
     }
     if (R) {
-        CHECK_HIP(hipMemcpyAsync(ctx->host_io.ptr, pairs.data(), sizeof(int32_t) * pairs.size(),
-                                 hipMemcpyHostToDevice, s));
+        CHECK_HIP(hipMemcpyAsync(ctx->host_io.ptr, pairs, sizeof(int32_t) * 2 * (size_t)R, hipMemcpyHostToDevice, s));
         CHECK_HIP(launch_cc_scatter(ctx->host_io.as<int32_t>(), R, b.rep2, s));
     }
     CHECK_HIP(launch_cc_apply(g, b, s));
-    // the pairs buffer is host memory: keep it alive until the copy is done
+    // the staging buffer is reused by the next call: wait for the copy
     CHECK_HIP(hipStreamSynchronize(s));
     return MSCEER_OK;
 }
