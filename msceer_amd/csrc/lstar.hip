@@ -51,6 +51,7 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
     __shared__ uint32_t s_mask[kThreads];     // lower star of each vertex of the brick
     __shared__ uint16_t s_order[kThreads];    // brick vertices by lower-star size
     __shared__ uint32_t s_bucket[32];
+    __shared__ uint32_t s_res[4][kThreads];   // expansion result by brick vertex: pair-code bits, critical
     const int x0 = blockIdx.x * kBX, y0 = blockIdx.y * kBY, z0 = blockIdx.z * kBZ;
     const int flat = threadIdx.x + kBX * (threadIdx.y + kBY * threadIdx.z);
     for (int i = flat; i < kTileN; i += kThreads) {
@@ -110,7 +111,6 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
     const TableKeys64 keys{tile + base, s_off};
     uint32_t bx, by, bz;
     star_boundary(x, y, z, X, Y, Z, bx, by, bz);
-    const uint32_t onb = bx | by | bz;
 
     vcell = 2ull * x + (uint64_t)Mx * (2ull * y + (uint64_t)My * 2ull * z);
     StarResult r;
@@ -125,16 +125,36 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
         constrained_expand(keys, L, bx, by, bz, label, r);
     }
 
-    // write the owned cells
-#pragma unroll
-    for (int t = 0; t < 27; t++) {
-        if (!((L >> t) & 1u)) continue;
-        const int64_t off = (int64_t)tid_x(t) + (int64_t)Mx * ((int64_t)tid_y(t) + (int64_t)My * tid_z(t));
-        grad[(int64_t)vcell + off] = star_cell_byte(r, onb, t);
-    }
-
-    crit = r.crit;
+    s_res[0][me] = r.c0;
+    s_res[1][me] = r.c1;
+    s_res[2][me] = r.c2;
+    s_res[3][me] = r.crit;
     }  // active
+    __syncthreads();
+    // each lane writes the owned cells of its own vertex: the lanes of a wave
+    // hold neighbouring vertices, so their byte stores share cache lines
+    {
+        const uint32_t Lh = s_mask[flat];
+        const int hx = x0 + threadIdx.x, hy = y0 + threadIdx.y, hz = z0 + threadIdx.z;
+        vcell = 2ull * hx + (uint64_t)Mx * (2ull * hy + (uint64_t)My * 2ull * hz);
+        if (Lh) {
+            StarResult rh;
+            rh.c0 = s_res[0][flat];
+            rh.c1 = s_res[1][flat];
+            rh.c2 = s_res[2][flat];
+            rh.crit = s_res[3][flat];
+            uint32_t hbx, hby, hbz;
+            star_boundary(hx, hy, hz, X, Y, Z, hbx, hby, hbz);
+            const uint32_t honb = hbx | hby | hbz;
+#pragma unroll
+            for (int t = 0; t < 27; t++) {
+                if (!((Lh >> t) & 1u)) continue;
+                const int64_t off = (int64_t)tid_x(t) + (int64_t)Mx * ((int64_t)tid_y(t) + (int64_t)My * tid_z(t));
+                grad[(int64_t)vcell + off] = star_cell_byte(rh, honb, t);
+            }
+            crit = rh.crit;
+        }
+    }
 
     // critical cells: statistics and the ldir start lists, reserved once per
     // block in sub-list blockIdx % kSubs
