@@ -136,7 +136,7 @@ def bench_labels(args):
                           for k in ("ms_lstar", "ms_regions", "ms_integrate", "ms_cleanup", "ms_grad", "ms_total")},
              "counts": {k: int(last[k]) for k in ("minima", "regions", "traced_fp32", "over_budget_fp32",
                                                   "reintegrated_fp64", "over_budget_fp64", "certain_sweeps",
-                                                  "pass3_rounds", "steps_fp32", "steps_fp64")}}
+                                                  "pass3_rounds", "steps_fp32", "steps_fp64", "ahead_fp64")}}
     if args.ideal_mask_probe:
         from scipy import ndimage
         zero = hipmem.DeviceBuffer(V)

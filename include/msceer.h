@@ -73,7 +73,8 @@ typedef struct msceer_label_stats {
     float ms_grad;              /* constrained pass + ldir */
     float ms_total;
     int64_t steps_fp32;         /* advection steps (accepted and rejected) of pass 1 */
-    int64_t steps_fp64;         /* advection steps of pass 3 */
+    int64_t steps_fp64;         /* advection steps of pass 3 (streamlines traced ahead included) */
+    int64_t ahead_fp64;         /* pass-3 streamlines traced ahead of a chain of long re-integrations */
 } msceer_label_stats;
 
 int msceer_version(void);

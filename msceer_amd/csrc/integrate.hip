@@ -381,10 +381,7 @@ pass2_kernel(VGrid g, LabelBuffers b, WorkQueue q) {
     }
     __syncthreads();
     const uint32_t cnt = s_n;
-    if (threadIdx.x == 0 && cnt) {
-        atomicAdd(q.pending, cnt);
-        s_base = atomicAdd(q.tail, cnt);
-    }
+    if (threadIdx.x == 0 && cnt) s_base = atomicAdd(q.tail, cnt);
     __syncthreads();
     uint32_t lost = 0;
     for (uint32_t j = threadIdx.x; j < cnt; j += kThreads) {
@@ -392,10 +389,7 @@ pass2_kernel(VGrid g, LabelBuffers b, WorkQueue q) {
         if (slot < q.capacity) q.items[slot] = stage[j];
         else lost++;
     }
-    if (lost) {
-        atomicSub(q.pending, lost);
-        atomicOr(q.overflow, 1u);
-    }
+    if (lost) atomicOr(q.overflow, 1u);
 }
 
 // Pass 3: re-integration to the original certain regions only, until the
@@ -405,25 +399,27 @@ pass2_kernel(VGrid g, LabelBuffers b, WorkQueue q) {
 // labels do not depend on the processing order, since every re-integration
 // ends on a certain terminal, whose label never changes.
 //
-// One persistent launch.  Each wave keeps a ring of queued vertices in LDS:
-// it refills an empty ring with a chunk of the items pass 2 queued (one
-// atomic per chunk) and keeps the children it queues there while the ring
-// has room.  Children beyond that go to the global part of the queue, whose
-// slots lanes claim one at a time once the pass-2 chunks are gone (a lane
-// holding a claimed slot still takes ring items, and polls the slot when it
-// has nothing else to do).  A lane
-// whose streamline ends takes new work while the others keep advecting, one
-// step per iteration.  `pending` counts the items of the global queue (pass
-// 2's included) not yet finished: children are added before their producer
-// is retired, and a wave subtracts the items it finished only when it has no
-// streamline left and an empty ring, so an idle wave that reads pending == 0
-// knows that every item is done and that no slot will be written again.
+// One launch of resident waves.  Each wave keeps a ring of queued vertices in
+// LDS: it refills an empty ring with a chunk of the items pass 2 queued (one
+// atomic per chunk) and keeps the children it queues there while the ring has
+// room.  Children beyond that go to the global part of the queue: a producer
+// reserves slots (tail) and writes them at once; idle lanes claim only
+// reserved slots (head < tail, by compare-and-swap), so a claimed slot is
+// always about to be written.  A lane whose streamline ends takes new work
+// while the others keep advecting; the wave advects in bursts of up to kBurst
+// steps that end with the first streamline to end, so a wave left with one
+// long streamline does not visit the queue at every step.  A wave that holds
+// no streamline, no ring item and no claimed slot leaves as soon as no
+// reserved slot is unclaimed: every item queued later is queued by a wave
+// still running, which drains it before it leaves itself.  Waves thus do not
+// wait through the tail of a few long dependent streamlines, and kernels of
+// other streams get their compute units.
 constexpr uint32_t kRing = 256;                // per-wave ring (a power of two)
 constexpr uint32_t kRingKeep = 192;            // children stay in the ring up to this fill
 constexpr uint32_t kChunk = 64;                // pass-2 items claimed at a time
-constexpr uint32_t kCounted = 0x80000000u;     // ring item flag: counted in pending (vertex ids < 2^31)
+constexpr int kBurst = 32;                     // advection steps between visits to the queue
 
-// Between pass 2 and pass 3: the pass-2 items are [0, tail); single slots
+// Between pass 2 and pass 3: the pass-2 items are [0, tail); global slots
 // are claimed from tail on.
 __global__ void pass3_prepare_kernel(WorkQueue q) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -434,28 +430,82 @@ __global__ void pass3_prepare_kernel(WorkQueue q) {
     }
 }
 
+__device__ __forceinline__ uint32_t load_agent(const uint32_t* p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Tracing ahead.  The tail of the pass is a chain of long streamlines run
+// one after another: a re-integration whose label changed queues its
+// neighbours, which re-integrate and queue theirs.  A streamline depends on no
+// label (it stops only at certain terminals), so where such a chain runs, idle
+// lanes trace the vertices two and three All6 steps further ahead and keep
+// only the vertex each streamline ends at.  When the chain reaches such a
+// vertex, its re-integration takes that end vertex's label at that time, as a
+// trace of its own would.  spec[v] (the scratch labels): kSpecNone,
+// kSpecBusy (a trace ahead runs), kSpecWanted (v was reached meanwhile: the
+// lane tracing ahead finishes it), else the end vertex.
+constexpr int32_t kSpecNone = -1, kSpecBusy = -2, kSpecWanted = -3;
+constexpr uint32_t kSpecItem = 0x80000000u;  // queue item flag: trace ahead (vertex ids < 2^31)
+constexpr int kSpecSteps = 512;              // only chains of streamlines this long are traced ahead
+constexpr int kShell = 56;                   // vertices at All6 distance 2 and 3
+// (dx + 3) | (dy + 3) << 3 | (dz + 3) << 6, distance 2 first
+__constant__ uint16_t kShellOffsets[kShell] = {
+    91,  147, 154, 156, 163, 203, 210, 212, 217, 221, 226, 228, 235, 275, 282, 284, 291, 347, 27,
+    83,  90,  92,  99,  139, 146, 148, 153, 157, 162, 164, 171, 195, 202, 204, 209, 213, 216, 222,
+    225, 229, 234, 236, 243, 267, 274, 276, 281, 285, 290, 292, 299, 339, 346, 348, 355, 411};
+
 template <typename T>
 __global__ void __launch_bounds__(kThreads)
 pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr, T ethr, int max_iters) {
     __shared__ uint32_t ring_mem[kThreads / 64][kRing];
     uint32_t* ring = ring_mem[threadIdx.x >> 6];
+    int32_t* const spec = b.label2;
     const int lane = (int)(threadIdx.x & 63u);
     const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t first = *q.first_count;
-    uint32_t traced = 0, over = 0, retired = 0;
+    uint32_t traced = 0, over = 0, ahead = 0;
     uint64_t steps = 0;
     uint32_t rhead = 0, rtail = 0;  // ring cursors (the same in every lane)
     bool chunks = true;             // pass-2 chunks may remain (the same in every lane)
-    bool active = false, claimed = false, done = false, counted = false;
+    bool active = false, claimed = false, speculative = false;
     uint32_t slot = 0;
-    uint64_t idle_since = 0;
+    uint64_t wait_since = 0;
     int64_t v = 0;
-    int32_t init = 0;
     Tracer<T> tr;
+    // queue n items of every lane: in the ring while it has room, globally beyond
+    auto push = [&](const uint32_t* items, int n) {
+        if (!__any(n > 0)) return;
+        const uint32_t off = wave_exclusive_scan((uint32_t)n, lane);
+        const uint32_t total = __shfl(off + (uint32_t)n, 63);
+        const uint32_t fill = rtail - rhead;
+        const uint32_t room = fill < kRingKeep ? kRingKeep - fill : 0u;
+        const uint32_t nloc = total < room ? total : room;
+        const uint32_t nglob = total - nloc;
+        uint32_t gbase = 0;
+        if (nglob) {
+            if (lane == 0) gbase = atomicAdd(q.tail, nglob);
+            gbase = __shfl(gbase, 0);
+        }
+        uint32_t lost = 0;
+        for (int j = 0; j < n; j++) {
+            const uint32_t k = off + (uint32_t)j;
+            if (k < nloc) {
+                ring[(rtail + k) & (kRing - 1)] = items[j];
+            } else {
+                const uint32_t at = gbase + (k - nloc);
+                if (at < q.capacity)
+                    __hip_atomic_store(&q.items[at], items[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                else
+                    lost++;
+            }
+        }
+        rtail += nloc;
+        __builtin_amdgcn_wave_barrier();
+        if (__any(lost != 0u) && lane == 0) atomicOr(q.overflow, 1u);
+    };
     while (true) {
-        // a lane without a streamline takes ring items first, also while it
-        // holds a claimed global slot (to which it returns when idle)
-        bool need = !active && !done;
+        // a lane without a streamline or a claimed slot takes ring items first
+        bool need = !active && !claimed;
         uint64_t want = __ballot(need);
         if (want && rhead == rtail && chunks) {
             const int leader = __ffsll((unsigned long long)want) - 1;
@@ -466,7 +516,7 @@ pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr
                 chunks = false;
             } else {
                 const uint32_t n = first - base < kChunk ? first - base : kChunk;
-                if ((uint32_t)lane < n) ring[(rtail + (uint32_t)lane) & (kRing - 1)] = q.items[base + lane] | kCounted;
+                if ((uint32_t)lane < n) ring[(rtail + (uint32_t)lane) & (kRing - 1)] = q.items[base + lane];
                 rtail += n;
                 __builtin_amdgcn_wave_barrier();
             }
@@ -481,143 +531,167 @@ pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr
             }
             rhead += avail < nw ? avail : nw;
         }
-        want = __ballot(need && !claimed);
-        if (want && !chunks) {
+        // then reserved global slots, one per lane
+        want = __ballot(need);
+        if (want && rhead == rtail && !chunks) {
             const int leader = __ffsll((unsigned long long)want) - 1;
-            uint32_t base = 0;
-            if (lane == leader) base = atomicAdd(q.head, (uint32_t)__popcll(want));
+            uint32_t base = 0, n = 0;
+            if (lane == leader) {
+                uint32_t h = load_agent(q.head);
+                uint32_t t = load_agent(q.tail);
+                t = t < q.capacity ? t : q.capacity;
+                if (h < t) {
+                    const uint32_t nw = (uint32_t)__popcll(want), k = t - h < nw ? t - h : nw;
+                    if (__hip_atomic_compare_exchange_strong(q.head, &h, h + k, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                                             __HIP_MEMORY_SCOPE_AGENT)) {
+                        base = h;
+                        n = k;
+                    }
+                }
+            }
             base = __shfl(base, leader);
-            if (need && !claimed) {
-                slot = base + (uint32_t)__popcll(want & below);
-                if (slot < q.capacity) claimed = true;
-                else done = true;
+            n = __shfl(n, leader);
+            const uint32_t rank = (uint32_t)__popcll(want & below);
+            if (need && rank < n) {
+                slot = base + rank;
+                claimed = true;
             }
         }
-        if (need && claimed) {
-            const uint32_t it = __hip_atomic_load(&q.items[slot], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (claimed) {
+            const uint32_t it = load_agent(&q.items[slot]);
             if (it != kEmptySlot) {
                 claimed = false;
-                item = it | kCounted;
+                item = it;
             }
         }
+        // fin: re-integration of v done, ending at vertex `end`
+        bool fin = false, chain = false;
+        int64_t end = 0;
         if (item != kEmptySlot) {
-            counted = (item & kCounted) != 0u;
-            v = (int64_t)(item & ~kCounted);
+            const bool ahead_item = (item & kSpecItem) != 0u;
+            v = (int64_t)(item & ~kSpecItem);
             if (b.dest[v] == kAssigned) {
-                active = true;
-                traced++;
-                init = b.label[v];
-                tracer_start(src, g, tr, (int)(v % g.X), (int)((v / g.X) % g.Y), (int)(v / g.sxy), max_iters);
-            } else if (counted) {
-                retired++;
-            }
-        }
-        if (!__any(active) && rhead == rtail) {
-            // out of work: retire what this wave finished, then wait for a
-            // claimed slot to be written or for the end of the pass
-            const uint32_t r = wave_sum(retired);
-            retired = 0;
-            if (r) {
-                __threadfence();
-                if (lane == 0) atomicSub(q.pending, r);
-            }
-            const uint64_t waiting = __ballot(claimed);
-            if (waiting) {
-                const int w = __ffsll((unsigned long long)waiting) - 1;
-                uint32_t p = 0;
-                if (lane == w) p = __hip_atomic_load(q.pending, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                p = __shfl(p, w);
-                if (p == 0u) {
-                    if (claimed) done = true;
-                    claimed = false;
-                } else {
-                    // a wave idle for kIdleTicks gives up and reports it, so the
-                    // kernel ends even if the queue's invariant were ever broken
-                    const uint64_t now = __builtin_amdgcn_s_memrealtime();
-                    if (idle_since == 0) {
-                        idle_since = now;
-                    } else if (now - idle_since > kIdleTicks) {
-                        if (lane == w) atomicOr(q.overflow, 2u);
-                        claimed = false;
-                        done = true;
+                bool trace = true;
+                if (!ahead_item) {
+                    traced++;
+                    const int32_t sp = atomicCAS(&spec[v], kSpecBusy, kSpecWanted);
+                    if (sp == kSpecBusy) {
+                        trace = false;  // the lane tracing v ahead finishes it
+                    } else if (sp >= 0) {
+                        trace = false;  // traced ahead
+                        fin = chain = true;
+                        end = sp;
                     }
-                    __builtin_amdgcn_s_sleep(8);
                 }
-            } else {
-                idle_since = 0;
+                if (trace) {
+                    active = true;
+                    speculative = ahead_item;
+                    if (ahead_item) ahead++;
+                    tracer_start(src, g, tr, (int)(v % g.X), (int)((v / g.X) % g.Y), (int)(v / g.sxy), max_iters);
+                }
             }
-        } else {
-            idle_since = 0;
         }
-        if (__all(done)) break;
-        uint32_t kids[6];
-        int nk = 0;
+        if (!__any(active || fin)) {
+            if (rhead == rtail && !chunks) {
+                const uint64_t waiting = __ballot(claimed);
+                if (!waiting) {
+                    // nothing held: leave unless reserved slots are unclaimed
+                    uint32_t left = 0;
+                    if (lane == 0) {
+                        const uint32_t t = load_agent(q.tail);
+                        left = load_agent(q.head) < (t < q.capacity ? t : q.capacity);
+                    }
+                    if (!__shfl(left, 0)) break;
+                } else {
+                    // a claimed slot is written right after its reservation; a
+                    // wave that waited kIdleTicks reports it and leaves, so the
+                    // kernel ends even if that were ever broken
+                    const uint64_t now = __builtin_amdgcn_s_memrealtime();
+                    if (wait_since == 0) {
+                        wait_since = now;
+                    } else if (now - wait_since > kIdleTicks) {
+                        if (lane == __ffsll((unsigned long long)waiting) - 1) atomicOr(q.overflow, 2u);
+                        break;
+                    }
+                    __builtin_amdgcn_s_sleep(2);
+                }
+            }
+            continue;
+        }
+        wait_since = 0;
+        // advect in bursts of up to kBurst steps (none while a vertex traced
+        // ahead waits to be finished); a burst ends when a streamline of the
+        // wave ends (the lanes that ran the burst agree)
+        const int burst = __any(fin) ? 0 : kBurst;
         if (active) {
             int64_t next = 0;
-            bool moved = false;
-            const int ev = tracer_iterate(src, g, b.dest, false, dir, gthr, ethr, tr, next, moved);
-            if (ev == kOutOfVoxel) {
-                tracer_reenter(src, g, tr);
-            } else if (ev != kNone) {
+            int ev = kNone;
+            for (int k = 0; k < burst; k++) {
+                bool moved = false;
+                ev = tracer_iterate(src, g, b.dest, false, dir, gthr, ethr, tr, next, moved);
+                if (ev == kOutOfVoxel) {
+                    tracer_reenter(src, g, tr);
+                    ev = kNone;
+                }
+                if (__any(ev != kNone)) break;
+            }
+            if (ev != kNone) {
                 active = false;
                 if (ev != kHitPreassigned) next = nearest(g, tr.cur, tr.near);
-                steps += (uint64_t)(max_iters - (tr.iters > 0 ? tr.iters : 0));
-                if (ev == kOverMax) over++;
-                const int32_t nl = b.label[next];
-                b.label[v] = nl;
-                b.dest[v] = kCertainNonterminal;
-                if (nl != init) {
-                    int64_t nb[6];
-                    const int k = all6(g, v, nb);
-                    for (int j = 0; j < k; j++) {
-                        if (b.dest[nb[j]] == kAssigned && b.label[nb[j]] != nl &&
-                            atomicExch(&b.stamp[nb[j]], 1u) == 0u)
-                            kids[nk++] = (uint32_t)nb[j];
-                    }
-                }
-                if (counted) retired++;
+                const int n_steps = max_iters - (tr.iters > 0 ? tr.iters : 0);
+                steps += (uint64_t)n_steps;
+                end = next;
+                // a vertex traced ahead is finished here only if it was reached meanwhile
+                fin = !speculative || atomicExch(&spec[v], (int32_t)next) == kSpecWanted;
+                chain = n_steps >= kSpecSteps;
+                if (fin && ev == kOverMax) over++;
             }
         }
-        // queue the children: in the ring while it has room, globally beyond
-        if (__any(nk > 0)) {
-            const uint32_t off = wave_exclusive_scan((uint32_t)nk, lane);
-            const uint32_t total = __shfl(off + (uint32_t)nk, 63);
-            const uint32_t fill = rtail - rhead;
-            const uint32_t room = fill < kRingKeep ? kRingKeep - fill : 0u;
-            const uint32_t nloc = total < room ? total : room;
-            const uint32_t nglob = total - nloc;
-            uint32_t gbase = 0;
-            if (nglob) {
-                if (lane == 0) {
-                    atomicAdd(q.pending, nglob);
-                    gbase = atomicAdd(q.tail, nglob);
+        uint32_t kids[6];
+        int nk = 0;
+        bool spawn = false;
+        if (fin) {
+            const int32_t init = b.label[v];
+            const int32_t nl = b.label[end];
+            b.label[v] = nl;
+            b.dest[v] = kCertainNonterminal;
+            if (nl != init) {
+                int64_t nb[6];
+                const int k = all6(g, v, nb);
+                for (int j = 0; j < k; j++) {
+                    if (b.dest[nb[j]] == kAssigned && b.label[nb[j]] != nl &&
+                        atomicExch(&b.stamp[nb[j]], 1u) == 0u)
+                        kids[nk++] = (uint32_t)nb[j];
                 }
-                gbase = __shfl(gbase, 0);
+                spawn = chain && !chunks;
             }
-            uint32_t lost = 0;
-            for (int j = 0; j < nk; j++) {
-                const uint32_t k = off + (uint32_t)j;
-                if (k < nloc) {
-                    ring[(rtail + k) & (kRing - 1)] = kids[j];
-                } else {
-                    const uint32_t at = gbase + (k - nloc);
-                    if (at < q.capacity)
-                        __hip_atomic_store(&q.items[at], kids[j], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                    else
-                        lost++;
+        }
+        push(kids, nk);
+        // trace ahead around each spawning lane's vertex, one shell vertex per lane
+        for (uint64_t sp = __ballot(spawn); sp; sp &= sp - 1) {
+            const int from = __ffsll((unsigned long long)sp) - 1;
+            const int64_t c = (int64_t)__shfl((uint32_t)v, from);
+            uint32_t it = 0;
+            int n = 0;
+            if (lane < kShell) {
+                const uint32_t o = kShellOffsets[lane];
+                const int ux = (int)(c % g.X) + (int)(o & 7u) - 3, uy = (int)((c / g.X) % g.Y) + (int)((o >> 3) & 7u) - 3;
+                const int uz = (int)(c / g.sxy) + (int)(o >> 6) - 3;
+                if (ux >= 0 && uy >= 0 && uz >= 0 && ux < g.X && uy < g.Y && uz < g.Z) {
+                    const int64_t u = (int64_t)ux + (int64_t)g.X * ((int64_t)uy + (int64_t)g.Y * uz);
+                    if (b.dest[u] == kAssigned && b.stamp[u] == 0u && spec[u] == kSpecNone &&
+                        atomicCAS(&spec[u], kSpecNone, kSpecBusy) == kSpecNone) {
+                        it = (uint32_t)u | kSpecItem;
+                        n = 1;
+                    }
                 }
             }
-            rtail += nloc;
-            __builtin_amdgcn_wave_barrier();
-            lost = wave_sum(lost);
-            if (lost && lane == 0) {
-                atomicSub(q.pending, lost);
-                atomicOr(q.overflow, 1u);
-            }
+            push(&it, n);
         }
     }
     if (traced) atomicAdd(&b.stats[kStatTraced3], traced);
     if (over) atomicAdd(&b.stats[kStatOver3], over);
+    if (ahead) atomicAdd(&b.stats[kStatAhead3], ahead);
     if (steps) atomicAdd(reinterpret_cast<unsigned long long*>(&b.stats[kStatSteps3]), (unsigned long long)steps);
 }
 

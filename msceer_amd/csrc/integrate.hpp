@@ -27,7 +27,7 @@ struct LabelBuffers {
     float4* gradf;       // FP32 gradient field
     double4* gradd;      // FP64 gradient field (pass 3)
     int32_t* label;      // region label per vertex (-1 unassigned)
-    int32_t* label2;     // scratch labels
+    int32_t* label2;     // scratch labels (pass 3: end vertices of streamlines traced ahead)
     uint8_t* dest;       // DestType per vertex
     uint8_t* seedmask;   // 1 at region seeds (their gradient is zero)
     uint8_t* redo;       // 1: re-trace in FP64 (pass 1 ran out of budget)
@@ -44,6 +44,7 @@ enum IntegStat : int {
     kStatOver3 = 3,      // pass-3 streamlines that ran out of step budget
     kStatChanged = 4,    // sweep / round change flag
     kStatMasked = 5,     // unassigned vertices inside the integration mask
+    kStatAhead3 = 6,     // pass-3 streamlines traced ahead of a chain
     kStatSteps1 = 8,     // uint64 (two words): pass-1 steps, accepted and rejected
     kStatSteps3 = 10,    // uint64: pass-3 steps
     kStatNum = 16
@@ -67,13 +68,12 @@ hipError_t launch_pass1(const VGrid& g, const LabelBuffers& b, int stride, bool 
 // Pass 2: every ASSIGNED vertex with a differently labelled All6 neighbour
 // (or flagged for re-tracing) goes to `out`.
 // Re-integration work queue: `items` (capacity entries, 0xFFFFFFFF until
-// written), claim/reserve counters and the count of unfinished items.
+// written) and its claim/reserve counters.
 struct WorkQueue {
     uint32_t* items;
     uint32_t capacity;
     uint32_t* head;         // next slot of the items queued during pass 3
     uint32_t* tail;         // slots reserved
-    uint32_t* pending;      // queued items not yet finished
     uint32_t* overflow;
     uint32_t* chunk_head;   // next chunk of the items pass 2 queued
     uint32_t* first_count;  // items pass 2 queued
@@ -94,8 +94,8 @@ hipError_t launch_certain_round(const VGrid& g, const float* values, const uint8
 // One pass-3 round over `in` (FP64 re-integration to the certain regions);
 // changed labels push their ASSIGNED neighbours to `out`; `clear` counts are zeroed.
 hipError_t launch_grad_field64(const VGrid& g, const LabelBuffers& b, hipStream_t s);
-// Persistent FP64 pass 3 over the queue filled by pass 2 (zero head, tail,
-// pending and chunk_head before pass 2).
+// FP64 pass 3 over the queue filled by pass 2 (zero head, tail and
+// chunk_head before pass 2).
 hipError_t launch_pass3(const VGrid& g, const LabelBuffers& b, const WorkQueue& q, float gthr, float ethr,
                         int max_iters, int sign, hipStream_t s);
 // Cleanup of isolated components.

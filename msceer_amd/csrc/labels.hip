@@ -143,12 +143,13 @@ int run_integration(msceer_ctx* ctx, const VGrid& g, const LabelBuffers& b, cons
         CHECK_HIP(launch_pass1(g, b, stride, stride == top, grad_thr, err_thr, max_iters, sign, s));
     // pass 2 fills the re-integration queue, pass 3 drains it in one launch
     const WorkQueue q{ctx->front0.as<uint32_t>(), (uint32_t)V, ctl + kCtlCursor, ctl + kCtlCursor + 1,
-                      ctl + kCtlCursor + 2, ctl + kCtlOverflow, ctl + kCtlCursor + 3, ctl + kCtlCursor + 4};
+                      ctl + kCtlOverflow, ctl + kCtlCursor + 3, ctl + kCtlCursor + 4};
     CHECK_HIP(launch_fill_u32(q.items, V, 0xFFFFFFFFu, s));
     CHECK_HIP(launch_zero(ctl + kCtlCursor, 5, s));
     CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
     CHECK_HIP(launch_pass2(g, b, q, s));
     CHECK_HIP(launch_grad_field64(g, b, s));
+    CHECK_HIP(launch_fill_i32(b.label2, V, -1, s));  // no vertex traced ahead yet
     CHECK_HIP(launch_pass3(g, b, q, grad_thr, err_thr, max_iters, sign, s));
     int rc = read_ctl(ctx, 0, kCtlWords);
     if (rc) return rc;
@@ -165,6 +166,7 @@ int run_integration(msceer_ctx* ctx, const VGrid& g, const LabelBuffers& b, cons
     };
     st.steps_fp32 = u64(kStatSteps1);
     st.steps_fp64 = u64(kStatSteps3);
+    st.ahead_fp64 = ctx->h_ctl[kCtlIntStats + kStatAhead3];
     return MSCEER_OK;
 }
 

@@ -86,6 +86,7 @@ class LabelStats(ctypes.Structure):
         ("ms_total", ctypes.c_float),
         ("steps_fp32", ctypes.c_int64),
         ("steps_fp64", ctypes.c_int64),
+        ("ahead_fp64", ctypes.c_int64),
     ]
 
     def as_dict(self):
