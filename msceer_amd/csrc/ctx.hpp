@@ -77,7 +77,7 @@ enum Ctl : int {
     kCtlTail = kCtlOverflow + 1,   // shadow of the grad array's last partial word
     kCtlFlag = kCtlOverflow + 2,   // change flag of iterative sweeps
     kCtlCount = kCtlOverflow + 3,  // a kernel-reported count
-    kCtlCursor = kCtlOverflow + 4,  // head, tail, (unused), chunk head, first count of the re-integration queue
+    kCtlCursor = kCtlOverflow + 4,  // head, tail, trace-ahead flag, chunk head, first count of the re-integration queue
     kCtlIntStats = kCtlOverflow + 12,  // sixteen integrator counters (8-byte aligned)
     kCtlWords = kCtlOverflow + 28
 };

@@ -409,15 +409,16 @@ pass2_kernel(VGrid g, LabelBuffers b, WorkQueue q) {
 // while the others keep advecting; the wave advects in bursts of up to kBurst
 // steps that end with the first streamline to end, so a wave left with one
 // long streamline does not visit the queue at every step.  A wave that holds
-// no streamline, no ring item and no claimed slot leaves as soon as no
-// reserved slot is unclaimed: every item queued later is queued by a wave
-// still running, which drains it before it leaves itself.  Waves thus do not
-// wait through the tail of a few long dependent streamlines, and kernels of
-// other streams get their compute units.
+// no streamline, no ring item and no claimed slot leaves once it has found
+// no unclaimed reserved slot for kLeaveTicks: every item queued later is
+// queued by a wave still running, which drains it before it leaves itself.
+// Waves thus do not wait through the tail of a few long dependent streamlines,
+// and kernels of other streams get their compute units.
 constexpr uint32_t kRing = 256;                // per-wave ring (a power of two)
 constexpr uint32_t kRingKeep = 192;            // children stay in the ring up to this fill
 constexpr uint32_t kChunk = 64;                // pass-2 items claimed at a time
 constexpr int kBurst = 32;                     // advection steps between visits to the queue
+constexpr uint64_t kLeaveTicks = 20000;        // 200 us idle (100 MHz real-time counter) before a wave leaves
 
 // Between pass 2 and pass 3: the pass-2 items are [0, tail); global slots
 // are claimed from tail on.
@@ -454,8 +455,9 @@ __constant__ uint16_t kShellOffsets[kShell] = {
     83,  90,  92,  99,  139, 146, 148, 153, 157, 162, 164, 171, 195, 202, 204, 209, 213, 216, 222,
     225, 229, 234, 236, 243, 267, 274, 276, 281, 285, 290, 292, 299, 339, 346, 348, 355, 411};
 
+// two waves per SIMD: the FP64 streamline state fills half the register file
 template <typename T>
-__global__ void __launch_bounds__(kThreads)
+__global__ void __launch_bounds__(kThreads, 2)
 pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr, T ethr, int max_iters) {
     __shared__ uint32_t ring_mem[kThreads / 64][kRing];
     uint32_t* ring = ring_mem[threadIdx.x >> 6];
@@ -468,8 +470,9 @@ pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr
     uint32_t rhead = 0, rtail = 0;  // ring cursors (the same in every lane)
     bool chunks = true;             // pass-2 chunks may remain (the same in every lane)
     bool active = false, claimed = false, speculative = false;
+    bool ahead_seen = false;        // the wave has seen tracing ahead (the same in every lane)
     uint32_t slot = 0;
-    uint64_t wait_since = 0;
+    uint64_t wait_since = 0, idle_since = 0;
     int64_t v = 0;
     Tracer<T> tr;
     // queue n items of every lane: in the ring while it has room, globally beyond
@@ -535,10 +538,11 @@ pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr
         want = __ballot(need);
         if (want && rhead == rtail && !chunks) {
             const int leader = __ffsll((unsigned long long)want) - 1;
-            uint32_t base = 0, n = 0;
+            uint32_t base = 0, n = 0, on = 0;
             if (lane == leader) {
                 uint32_t h = load_agent(q.head);
                 uint32_t t = load_agent(q.tail);
+                on = load_agent(q.ahead_on);
                 t = t < q.capacity ? t : q.capacity;
                 if (h < t) {
                     const uint32_t nw = (uint32_t)__popcll(want), k = t - h < nw ? t - h : nw;
@@ -551,6 +555,7 @@ pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr
             }
             base = __shfl(base, leader);
             n = __shfl(n, leader);
+            ahead_seen = ahead_seen || __shfl(on, leader) != 0u;
             const uint32_t rank = (uint32_t)__popcll(want & below);
             if (need && rank < n) {
                 slot = base + rank;
@@ -570,11 +575,14 @@ pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr
         if (item != kEmptySlot) {
             const bool ahead_item = (item & kSpecItem) != 0u;
             v = (int64_t)(item & ~kSpecItem);
+            // spec is read only once this wave has seen tracing ahead (a
+            // kSpecNone read where a trace ahead exists only costs a second trace)
+            const int32_t seen = ahead_seen ? spec[v] : kSpecNone;
             if (b.dest[v] == kAssigned) {
                 bool trace = true;
                 if (!ahead_item) {
                     traced++;
-                    const int32_t sp = atomicCAS(&spec[v], kSpecBusy, kSpecWanted);
+                    const int32_t sp = seen == kSpecNone ? kSpecNone : atomicCAS(&spec[v], kSpecBusy, kSpecWanted);
                     if (sp == kSpecBusy) {
                         trace = false;  // the lane tracing v ahead finishes it
                     } else if (sp >= 0) {
@@ -595,13 +603,21 @@ pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr
             if (rhead == rtail && !chunks) {
                 const uint64_t waiting = __ballot(claimed);
                 if (!waiting) {
-                    // nothing held: leave unless reserved slots are unclaimed
+                    // nothing held: leave after kLeaveTicks without a reserved
+                    // slot left unclaimed
                     uint32_t left = 0;
                     if (lane == 0) {
                         const uint32_t t = load_agent(q.tail);
                         left = load_agent(q.head) < (t < q.capacity ? t : q.capacity);
                     }
-                    if (!__shfl(left, 0)) break;
+                    if (!__shfl(left, 0)) {
+                        const uint64_t now = __builtin_amdgcn_s_memrealtime();
+                        if (idle_since == 0) idle_since = now;
+                        else if (now - idle_since > kLeaveTicks) break;
+                        __builtin_amdgcn_s_sleep(16);
+                    } else {
+                        idle_since = 0;
+                    }
                 } else {
                     // a claimed slot is written right after its reservation; a
                     // wave that waited kIdleTicks reports it and leaves, so the
@@ -618,7 +634,7 @@ pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr
             }
             continue;
         }
-        wait_since = 0;
+        wait_since = idle_since = 0;
         // advect in bursts of up to kBurst steps (none while a vertex traced
         // ahead waits to be finished); a burst ends when a streamline of the
         // wave ends (the lanes that ran the burst agree)
@@ -667,8 +683,10 @@ pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr
             }
         }
         push(kids, nk);
-        // trace ahead around each spawning lane's vertex, one shell vertex per lane
+        // trace ahead around each spawning lane's vertex, one shell vertex per
+        // lane, through the global part of the queue (any idle lane takes them)
         for (uint64_t sp = __ballot(spawn); sp; sp &= sp - 1) {
+            ahead_seen = true;
             const int from = __ffsll((unsigned long long)sp) - 1;
             const int64_t c = (int64_t)__shfl((uint32_t)v, from);
             uint32_t it = 0;
@@ -686,6 +704,7 @@ pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr
                     }
                 }
             }
+            if (__any(n > 0) && lane == 0) __hip_atomic_store(q.ahead_on, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             push(&it, n);
         }
     }

@@ -75,6 +75,7 @@ struct WorkQueue {
     uint32_t* head;         // next slot of the items queued during pass 3
     uint32_t* tail;         // slots reserved
     uint32_t* overflow;
+    uint32_t* ahead_on;     // set once pass 3 traces vertices ahead
     uint32_t* chunk_head;   // next chunk of the items pass 2 queued
     uint32_t* first_count;  // items pass 2 queued
 };

@@ -143,7 +143,7 @@ int run_integration(msceer_ctx* ctx, const VGrid& g, const LabelBuffers& b, cons
         CHECK_HIP(launch_pass1(g, b, stride, stride == top, grad_thr, err_thr, max_iters, sign, s));
     // pass 2 fills the re-integration queue, pass 3 drains it in one launch
     const WorkQueue q{ctx->front0.as<uint32_t>(), (uint32_t)V, ctl + kCtlCursor, ctl + kCtlCursor + 1,
-                      ctl + kCtlOverflow, ctl + kCtlCursor + 3, ctl + kCtlCursor + 4};
+                      ctl + kCtlOverflow, ctl + kCtlCursor + 2, ctl + kCtlCursor + 3, ctl + kCtlCursor + 4};
     CHECK_HIP(launch_fill_u32(q.items, V, 0xFFFFFFFFu, s));
     CHECK_HIP(launch_zero(ctl + kCtlCursor, 5, s));
     CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));

@@ -107,7 +107,6 @@ struct Tracer {
     int bx, by, bz;
     bool near;
     int iters;
-    int64_t checked;  // last vertex whose destination byte was read (no stop there)
 };
 
 // Corner gradients of the voxel of t.cur; (cx, cy, cz) is the vertex nearest
@@ -183,11 +182,7 @@ __device__ __forceinline__ int tracer_iterate(const GradSrc<T>& src, const VGrid
     const T grow = t.step * (T)1.8, cap = (T)kMaxStep / t.gm;
     t.step = grow < cap ? grow : cap;
     next = nearest(g, t.cur, t.near);
-    // The reference reads the destination at every step, but destinations
-    // change there only between streamlines, so a vertex already passed
-    // without a stop is not read again (same stops, one load fewer per step).
-    const bool fresh = next != t.checked;
-    const uint8_t d = fresh ? dest[next] : (uint8_t)kBackground;
+    const uint8_t d = dest[next];
     // the next iteration would leave the voxel: gather the next voxel's
     // corners while the destination byte is in flight
     const bool leave = t.iters > 0 &&
@@ -197,7 +192,6 @@ __device__ __forceinline__ int tracer_iterate(const GradSrc<T>& src, const VGrid
         tracer_load(src, g, t, cx, cy, cz);
     }
     if (d == kCertainTerminal || (stop_on_assigned && d == kAssigned)) return kHitPreassigned;
-    t.checked = next;
     if (leave) {
         t.iters--;  // the iteration that leaves the voxel
         tracer_begin(t);
@@ -221,7 +215,6 @@ __device__ __forceinline__ void tracer_start(const GradSrc<T>& src, const VGrid&
                                              int max_iters) {
     t.cur = {(T)x, (T)y, (T)z};
     t.iters = max_iters;
-    t.checked = -1;
     tracer_load(src, g, t, x, y, z);
     tracer_begin(t);
 }
