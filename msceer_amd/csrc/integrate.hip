@@ -23,6 +23,8 @@
 // arithmetic, so a streamline's positions follow the reference's operation by
 // operation.
 #include <hip/hip_runtime.h>
+
+#include <algorithm>
 #include <hipcub/hipcub.hpp>
 
 #include "integrate.hpp"
@@ -264,6 +266,18 @@ __device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t x, int lane) {
 // round r - 1 and appends those it releases to a flat list for round r + 1
 // (one atomic per wave).  Lists alternate between two buffers; three count
 // words rotate, and each round clears the one round r + 2 will append to.
+//
+// cnt[v] packs what a round needs of v, so that a round reads no values:
+// bits 0-7 count the unresolved earlier non-seed neighbours, bits 8-13 flag
+// the earlier neighbours and bits 16-21 the later non-seed neighbours, bit j
+// for the direction j of +x, -x, +y, -y, +z, -z.
+constexpr uint32_t kCntMask = 0xFFu;
+constexpr int kEarlierShift = 8, kLaterShift = 16;
+
+__device__ __forceinline__ int64_t dir_offset(const VGrid& g, int j) {
+    const int64_t o = j < 2 ? 1 : (j < 4 ? (int64_t)g.X : g.sxy);
+    return (j & 1) ? -o : o;
+}
 
 // The vertices with no earlier non-seed neighbour start the first list.
 __global__ void __launch_bounds__(kThreads)
@@ -282,12 +296,21 @@ certain_init_kernel(const float* __restrict__ f, VGrid g, const uint8_t* __restr
             cnt[v] = 0u;
             continue;
         }
-        int64_t nb[6];
-        const int k = all6(g, v, nb);
-        uint32_t c = 0;
-        for (int j = 0; j < k; j++)
-            if (before(f, nb[j], v) && !seedmask[nb[j]]) c++;
-        cnt[v] = c;
+        const int x = (int)(v % g.X), y = (int)((v / g.X) % g.Y), z = (int)(v / g.sxy);
+        const bool inside[6] = {x < g.X - 1, x > 0, y < g.Y - 1, y > 0, z < g.Z - 1, z > 0};
+        uint32_t c = 0, earlier = 0, later = 0;
+        for (int j = 0; j < 6; j++) {
+            if (!inside[j]) continue;
+            const int64_t u = v + dir_offset(g, j);
+            const bool seed = seedmask[u] != 0;
+            if (before(f, u, v)) {
+                earlier |= 1u << j;
+                if (!seed) c++;
+            } else if (!seed) {
+                later |= 1u << j;
+            }
+        }
+        cnt[v] = c | earlier << kEarlierShift | later << kLaterShift;
         if (c == 0) stage[atomicAdd(&s_n, 1u)] = (uint32_t)v;
     }
     __syncthreads();
@@ -303,11 +326,10 @@ certain_init_kernel(const float* __restrict__ f, VGrid g, const uint8_t* __restr
 
 // One round: resolve the listed vertices, release their later neighbours.
 __global__ void __launch_bounds__(kThreads)
-certain_round_kernel(const float* __restrict__ f, VGrid g, const uint8_t* __restrict__ seedmask,
-                     int32_t* __restrict__ label, uint32_t* __restrict__ cnt, const uint32_t* __restrict__ in,
-                     const uint32_t* __restrict__ in_count, uint32_t* __restrict__ out,
-                     uint32_t* __restrict__ out_count, uint32_t* __restrict__ clear_count, uint32_t capacity,
-                     uint32_t* __restrict__ overflow) {
+certain_round_kernel(VGrid g, int32_t* __restrict__ label, uint32_t* __restrict__ cnt,
+                     const uint32_t* __restrict__ in, const uint32_t* __restrict__ in_count,
+                     uint32_t* __restrict__ out, uint32_t* __restrict__ out_count, uint32_t* __restrict__ clear_count,
+                     uint32_t capacity, uint32_t* __restrict__ overflow) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *clear_count = 0u;
     const int lane = (int)(threadIdx.x & 63u);
     const uint32_t total = *in_count < capacity ? *in_count : capacity;
@@ -320,22 +342,23 @@ certain_round_kernel(const float* __restrict__ f, VGrid g, const uint8_t* __rest
         int nk = 0;
         if (it < total) {
             const int64_t v = (int64_t)in[it];
-            int64_t nb[6];
-            const int k = all6(g, v, nb);
+            const uint32_t meta = cnt[v];
+            int32_t l[6];
+            for (int j = 0; j < 6; j++)
+                l[j] = (meta >> (kEarlierShift + j)) & 1u ? label[v + dir_offset(g, j)] : -2;
             int32_t val = -1;
             bool has = false, ok = true;
-            for (int j = 0; j < k; j++) {
-                if (!before(f, nb[j], v)) continue;
-                const int32_t l = label[nb[j]];
-                if (l == -1) { ok = false; break; }
-                if (!has) { val = l; has = true; }
-                else if (val != l) { ok = false; break; }
+            for (int j = 0; j < 6; j++) {
+                if (l[j] == -2) continue;
+                if (l[j] == -1) ok = false;
+                else if (!has) { val = l[j]; has = true; }
+                else if (val != l[j]) ok = false;
             }
             if (ok && has) label[v] = val;
-            for (int j = 0; j < k; j++) {
-                const int64_t u = nb[j];
-                if (seedmask[u] || !before(f, v, u)) continue;
-                if (atomicSub(&cnt[u], 1u) == 1u) kids[nk++] = (uint32_t)u;
+            for (int j = 0; j < 6; j++) {
+                if (!((meta >> (kLaterShift + j)) & 1u)) continue;
+                const int64_t u = v + dir_offset(g, j);
+                if ((atomicSub(&cnt[u], 1u) & kCntMask) == 1u) kids[nk++] = (uint32_t)u;
             }
         }
         if (__any(nk > 0)) {
@@ -1175,13 +1198,17 @@ hipError_t launch_certain_init(const VGrid& g, const float* values, const uint8_
                        out, out_count, capacity, overflow);
     return hipGetLastError();
 }
-hipError_t launch_certain_round(const VGrid& g, const float* values, const uint8_t* seedmask, int32_t* label,
-                                uint32_t* cnt, const uint32_t* in, const uint32_t* in_count, uint32_t* out,
-                                uint32_t* out_count, uint32_t* clear_count, uint32_t capacity, uint32_t* overflow,
-                                hipStream_t s) {
-    static const uint32_t blocks = compute_units();  // a level holds thousands of vertices, not millions
-    hipLaunchKernelGGL(certain_round_kernel, dim3(blocks), dim3(kThreads), 0, s, values, g, seedmask, label,
-                       cnt, in, in_count, out, out_count, clear_count, capacity, overflow);
+hipError_t launch_certain_round(const VGrid& g, int32_t* label, uint32_t* cnt, const uint32_t* in,
+                                const uint32_t* in_count, uint32_t* out, uint32_t* out_count, uint32_t* clear_count,
+                                uint32_t capacity, uint32_t* overflow, uint64_t expected, hipStream_t s) {
+    // about two listed vertices per lane: one work-group per compute unit for
+    // the thousands of a smooth field's rounds, up to 32 for the millions of a
+    // noisy one's (the rounds are latency-bound)
+    static const uint32_t cus = compute_units();
+    const uint64_t want = (expected + 2 * kThreads - 1) / (2 * kThreads);
+    const uint32_t blocks = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, cus), 32ull * cus);
+    hipLaunchKernelGGL(certain_round_kernel, dim3(blocks), dim3(kThreads), 0, s, g, label, cnt, in, in_count, out,
+                       out_count, clear_count, capacity, overflow);
     return hipGetLastError();
 }
 hipError_t launch_dest_init(const VGrid& g, const LabelBuffers& b, hipStream_t s) {

@@ -327,18 +327,22 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
         CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
         CHECK_HIP(launch_certain_init(g, d_values, b.seedmask, b.stamp, lists[0], counts, (uint32_t)V,
                                       ctl + kCtlOverflow, s));
+        rc = read_ctl(ctx, 0, kCtlWords);
+        if (rc) return rc;
         int k = 0;
-        for (int guard = 0;; guard++) {
+        // the launches of a group are sized by the list length last read back
+        uint64_t expected = ctx->h_ctl[kCtlRound];
+        for (int guard = 0; expected > 0; guard++) {
             if (guard > (int)(V / 16 + 16)) return MSCEER_EHIP;  // a round resolves at least one vertex
             constexpr int kGroup = 16;
             for (int j = 0; j < kGroup; j++, k++)
-                CHECK_HIP(launch_certain_round(g, d_values, b.seedmask, b.label, b.stamp, lists[k % 2],
-                                               counts + k % 3, lists[(k + 1) % 2], counts + (k + 1) % 3,
-                                               counts + (k + 2) % 3, (uint32_t)V, ctl + kCtlOverflow, s));
+                CHECK_HIP(launch_certain_round(g, b.label, b.stamp, lists[k % 2], counts + k % 3,
+                                               lists[(k + 1) % 2], counts + (k + 1) % 3, counts + (k + 2) % 3,
+                                               (uint32_t)V, ctl + kCtlOverflow, expected, s));
             rc = read_ctl(ctx, 0, kCtlWords);
             if (rc) return rc;
             if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;
-            if (ctx->h_ctl[kCtlRound + k % 3] == 0) break;
+            expected = ctx->h_ctl[kCtlRound + k % 3];
         }
         sweeps = k;
     }
