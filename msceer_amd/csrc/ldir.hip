@@ -180,10 +180,9 @@ ldir_round_kernel(GradView v, CellGeom g, MList in, MList out, uint32_t* __restr
             const uint32_t pos[3] = {x, y, z};
             const uint32_t ext[3] = {g.Mx, g.My, g.Mz};
             const int64_t step_off[3] = {1, (int64_t)g.Mx, (int64_t)g.sxy};
-            // the item itself: set its ldir with non-returning atomics on its
-            // word (its other bits never change; the passes never read an
-            // item's own ldir, so the intermediate 0 of 2 -> 1 is harmless)
-            set_ldir(v, item, value);
+            // the item itself: its ldir is set below by a byte store (the
+            // tail word's bytes live in the shadow word and take atomics)
+            if (item >= v.tail_start) set_ldir(v, item, value);
 #pragma unroll
             for (int a = 0; a < 3; a++) {
                 if (pos[a] & 1u) continue;  // cofacets lie along even axes
@@ -195,11 +194,22 @@ ldir_round_kernel(GradView v, CellGeom g, MList in, MList out, uint32_t* __restr
                 }
             }
         }
+        uint32_t ib[kPerLane];
 #pragma unroll
-        for (int k = 0; k < kPerLane; k++)
+        for (int k = 0; k < kPerLane; k++) {
+            ib[k] = items[k] < v.tail_start ? v.grad[items[k]] : 0u;
 #pragma unroll
             for (int j = 0; j < 4; j++)
                 if (j < nt[k]) tb[k][j] = v.grad[tc[k][j]];
+        }
+        // Set the items' ldir with plain byte stores: within a pass every
+        // writer of a byte writes the same ldir and no writer changes its
+        // other bits, and the claims' atomics on the same words change other
+        // bytes only, so no atomic is needed (one per item fewer, and the
+        // device-scope atomics are performed at the memory, not in L2).
+#pragma unroll
+        for (int k = 0; k < kPerLane; k++)
+            if (items[k] < v.tail_start) v.grad[items[k]] = (uint8_t)((ib[k] & 0x1Fu) | (value << 5));
 #pragma unroll
         for (int k = 0; k < kPerLane; k++) {
             const uint32_t item = items[k];
