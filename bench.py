@@ -276,11 +276,12 @@ def bench_batch(args):
                                   "p99": round(float(np.percentile(ms, 99)), 3),
                                   "max": round(float(ms.max()), 3)}}
 
-    timed(False, 8)  # brick tuning for every x extent, buffers grown
-    out["grad"] = summary(*timed(False, 8), 8)
+    k = args.batch_streams
+    timed(False, k)  # brick tuning for every x extent, buffers grown
+    out["grad"] = summary(*timed(False, k), k)
     out["grad_one_stream"] = summary(*timed(False, 1), 1)
-    timed(True, 8)
-    out["labels"] = summary(*timed(True, 8), 8)
+    timed(True, k)
+    out["labels"] = summary(*timed(True, k), k)
     lib = native.lib()
     tuned = {}
     for x in sorted({int(v.shape[2]) for v in vols}):
@@ -385,12 +386,19 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--batch-volumes", type=int, default=512)
+    ap.add_argument("--batch-streams", type=int, default=16, help="worker streams of the batched workload")
+    ap.add_argument("--hw-queues", type=int, default=16,
+                    help="GPU_MAX_HW_QUEUES for this process (HIP maps the batch workers' streams onto this many "
+                         "hardware queues; 0 keeps the environment's value)")
     ap.add_argument("--learned-large", action="store_true",
                     help="only the learned-mask comparison on the 1024x768x512 volume")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB read + 1 GiB write streaming kernel (torch), the "
                          "reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE")
     args = ap.parse_args()
+    if args.hw_queues > 0:
+        # read by the HIP runtime at its first call, which has not happened yet
+        os.environ["GPU_MAX_HW_QUEUES"] = str(min(args.hw_queues, 32))
     if args.calibrate_copy:
         calibrate_copy()
         return

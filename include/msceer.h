@@ -164,9 +164,12 @@ int msceer_integrate_labels(msceer_ctx* ctx, const float* values, int64_t nx, in
  * processes one volume per program run): for each volume, the lower-star
  * .grad (msceer_lstar_grad) when labels_out is NULL, else the accurate
  * labelling and conforming .grad (msceer_ondemand_asc with the shared
- * parameters; grad_out may then be NULL).  `streams` worker threads (0: 8)
+ * parameters; grad_out may then be NULL).  `streams` worker threads (0: 16)
  * each drive a context of their own on the same GPU and take the volumes
- * largest first; the lower-star brick tuning is shared.  Host buffers only.
+ * largest first; the lower-star brick tuning is shared.  HIP maps streams
+ * onto GPU_MAX_HW_QUEUES hardware queues (4 unless set before the first HIP
+ * call); with fewer queues than workers, a long kernel of one worker delays
+ * the queued work of another (16 and 16 measured best).  Host buffers only.
  * Per volume, ms receives the wall time of its pipeline and status its code.
  */
 typedef struct msceer_volume {

@@ -27,7 +27,7 @@ extern "C" int msceer_batch(msceer_ctx* ctx, msceer_volume* vols, int32_t n, int
         if (rc) return rc;
     }
     if (n == 0) return MSCEER_OK;
-    if (streams <= 0) streams = 8;
+    if (streams <= 0) streams = 16;
     streams = std::min(std::min(streams, 32), n);
     // worker contexts persist in the parent context between calls
     while ((int)ctx->workers.size() < streams) {
