@@ -462,8 +462,9 @@ __device__ __forceinline__ uint32_t load_agent(const uint32_t* p) {
 // one after another: a re-integration whose label changed queues its
 // neighbours, which re-integrate and queue theirs.  A streamline depends on no
 // label (it stops only at certain terminals), so where such a chain runs, idle
-// lanes trace the vertices two and three All6 steps further ahead and keep
-// only the vertex each streamline ends at.  When the chain reaches such a
+// lanes trace the vertices two to five All6 steps further ahead that carry
+// another label than the one spreading, and keep only the vertex each
+// streamline ends at.  When the chain reaches such a
 // vertex, its re-integration takes that end vertex's label at that time, as a
 // trace of its own would.  spec[v] (the scratch labels): kSpecNone,
 // kSpecBusy (a trace ahead runs), kSpecWanted (v was reached meanwhile: the
@@ -471,12 +472,22 @@ __device__ __forceinline__ uint32_t load_agent(const uint32_t* p) {
 constexpr int32_t kSpecNone = -1, kSpecBusy = -2, kSpecWanted = -3;
 constexpr uint32_t kSpecItem = 0x80000000u;  // queue item flag: trace ahead (vertex ids < 2^31)
 constexpr int kSpecSteps = 512;              // only chains of streamlines this long are traced ahead
-constexpr int kShell = 56;                   // vertices at All6 distance 2 and 3
-// (dx + 3) | (dy + 3) << 3 | (dz + 3) << 6, distance 2 first
+constexpr int kShellR = 5;                   // traced ahead: All6 distance 2..kShellR (224 offsets)
+constexpr int kShell = 224;
+// (dx + 5) | (dy + 5) << 4 | (dz + 5) << 8, nearer distances first
 __constant__ uint16_t kShellOffsets[kShell] = {
-    91,  147, 154, 156, 163, 203, 210, 212, 217, 221, 226, 228, 235, 275, 282, 284, 291, 347, 27,
-    83,  90,  92,  99,  139, 146, 148, 153, 157, 162, 164, 171, 195, 202, 204, 209, 213, 216, 222,
-    225, 229, 234, 236, 243, 267, 274, 276, 281, 285, 290, 292, 299, 339, 346, 348, 355, 411};
+    853, 1093, 1108, 1110, 1125, 1333, 1348, 1350, 1363, 1367, 1380, 1382, 1397, 1605, 1620, 1622, 1637, 1877, 597,
+    837, 852, 854, 869, 1077, 1092, 1094, 1107, 1111, 1124, 1126, 1141, 1317, 1332, 1334, 1347, 1351, 1362, 1368,
+    1379, 1383, 1396, 1398, 1413, 1589, 1604, 1606, 1619, 1623, 1636, 1638, 1653, 1861, 1876, 1878, 1893, 2133, 341,
+    581, 596, 598, 613, 821, 836, 838, 851, 855, 868, 870, 885, 1061, 1076, 1078, 1091, 1095, 1106, 1112, 1123,
+    1127, 1140, 1142, 1157, 1301, 1316, 1318, 1331, 1335, 1346, 1352, 1361, 1369, 1378, 1384, 1395, 1399, 1412,
+    1414, 1429, 1573, 1588, 1590, 1603, 1607, 1618, 1624, 1635, 1639, 1652, 1654, 1669, 1845, 1860, 1862, 1875,
+    1879, 1892, 1894, 1909, 2117, 2132, 2134, 2149, 2389, 85, 325, 340, 342, 357, 565, 580, 582, 595, 599, 612, 614,
+    629, 805, 820, 822, 835, 839, 850, 856, 867, 871, 884, 886, 901, 1045, 1060, 1062, 1075, 1079, 1090, 1096, 1105,
+    1113, 1122, 1128, 1139, 1143, 1156, 1158, 1173, 1285, 1300, 1302, 1315, 1319, 1330, 1336, 1345, 1353, 1360,
+    1370, 1377, 1385, 1394, 1400, 1411, 1415, 1428, 1430, 1445, 1557, 1572, 1574, 1587, 1591, 1602, 1608, 1617,
+    1625, 1634, 1640, 1651, 1655, 1668, 1670, 1685, 1829, 1844, 1846, 1859, 1863, 1874, 1880, 1891, 1895, 1908,
+    1910, 1925, 2101, 2116, 2118, 2131, 2135, 2148, 2150, 2165, 2373, 2388, 2390, 2405, 2645};
 
 // two waves per SIMD: the FP64 streamline state fills half the register file
 template <typename T>
@@ -689,6 +700,7 @@ pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr
         uint32_t kids[6];
         int nk = 0;
         bool spawn = false;
+        int32_t spawn_label = 0;
         if (fin) {
             const int32_t init = b.label[v];
             const int32_t nl = b.label[end];
@@ -703,6 +715,7 @@ pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr
                         kids[nk++] = (uint32_t)nb[j];
                 }
                 spawn = chain && !chunks;
+                spawn_label = nl;
             }
         }
         push(kids, nk);
@@ -712,23 +725,28 @@ pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr
             ahead_seen = true;
             const int from = __ffsll((unsigned long long)sp) - 1;
             const int64_t c = (int64_t)__shfl((uint32_t)v, from);
-            uint32_t it = 0;
-            int n = 0;
-            if (lane < kShell) {
-                const uint32_t o = kShellOffsets[lane];
-                const int ux = (int)(c % g.X) + (int)(o & 7u) - 3, uy = (int)((c / g.X) % g.Y) + (int)((o >> 3) & 7u) - 3;
-                const int uz = (int)(c / g.sxy) + (int)(o >> 6) - 3;
-                if (ux >= 0 && uy >= 0 && uz >= 0 && ux < g.X && uy < g.Y && uz < g.Z) {
-                    const int64_t u = (int64_t)ux + (int64_t)g.X * ((int64_t)uy + (int64_t)g.Y * uz);
-                    if (b.dest[u] == kAssigned && b.stamp[u] == 0u && spec[u] == kSpecNone &&
-                        atomicCAS(&spec[u], kSpecNone, kSpecBusy) == kSpecNone) {
-                        it = (uint32_t)u | kSpecItem;
-                        n = 1;
+            const int32_t cl = __shfl(spawn_label, from);
+            for (int o0 = 0; o0 < kShell; o0 += 64) {
+                uint32_t it = 0;
+                int n = 0;
+                if (o0 + lane < kShell) {
+                    const uint32_t o = kShellOffsets[o0 + lane];
+                    const int ux = (int)(c % g.X) + (int)(o & 15u) - kShellR;
+                    const int uy = (int)((c / g.X) % g.Y) + (int)((o >> 4) & 15u) - kShellR;
+                    const int uz = (int)(c / g.sxy) + (int)(o >> 8) - kShellR;
+                    if (ux >= 0 && uy >= 0 && uz >= 0 && ux < g.X && uy < g.Y && uz < g.Z) {
+                        const int64_t u = (int64_t)ux + (int64_t)g.X * ((int64_t)uy + (int64_t)g.Y * uz);
+                        if (b.dest[u] == kAssigned && b.stamp[u] == 0u && spec[u] == kSpecNone && b.label[u] != cl &&
+                            atomicCAS(&spec[u], kSpecNone, kSpecBusy) == kSpecNone) {
+                            it = (uint32_t)u | kSpecItem;
+                            n = 1;
+                        }
                     }
                 }
+                if (__any(n > 0) && lane == 0)
+                    __hip_atomic_store(q.ahead_on, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                push(&it, n);
             }
-            if (__any(n > 0) && lane == 0) __hip_atomic_store(q.ahead_on, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            push(&it, n);
         }
     }
     if (traced) atomicAdd(&b.stats[kStatTraced3], traced);
