@@ -801,16 +801,80 @@ __device__ void uf_union(const float* f, int32_t* rep, int32_t a, int32_t b) {
     }
 }
 
-// Each edge once: +x, +y, +z neighbours with the same label.
+// Unions in two steps.  A work-group first joins the same-label edges inside
+// its 8x8x4 tile in LDS (hooking the later root under the earlier, as the
+// global unions do) and sets rep of each vertex to its tile-local root; then
+// only the edges that cross tile faces take the global unions, whose parent
+// reads and hooks are device-scope atomics performed at the memory.
+constexpr int kTx = 8, kTy = 8, kTz = 4;  // kTx * kTy * kTz == kThreads
+
+__device__ __forceinline__ int lds_find(int32_t* parent, int x) {
+    int p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    while (p != x) {
+        x = p;
+        p = __hip_atomic_load(&parent[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    return x;
+}
+
+__global__ void __launch_bounds__(kThreads)
+cc_union_tile_kernel(const float* __restrict__ f, VGrid g, LabelBuffers b) {
+    __shared__ int32_t parent[kThreads];
+    __shared__ int32_t lab[kThreads];
+    __shared__ float val[kThreads];
+    const int tx = (g.X + kTx - 1) / kTx, ty = (g.Y + kTy - 1) / kTy;
+    const int64_t tile = blockIdx.x;
+    const int ox = (int)(tile % tx) * kTx, oy = (int)((tile / tx) % ty) * kTy, oz = (int)(tile / ((int64_t)tx * ty)) * kTz;
+    const int t = (int)threadIdx.x, lx = t % kTx, ly = (t / kTx) % kTy, lz = t / (kTx * kTy);
+    const int x = ox + lx, y = oy + ly, z = oz + lz;
+    const bool inside = x < g.X && y < g.Y && z < g.Z;
+    const int64_t i = (int64_t)x + (int64_t)g.X * ((int64_t)y + (int64_t)g.Y * z);
+    parent[t] = t;
+    lab[t] = inside ? b.label2[i] : INT32_MIN;
+    val[t] = inside ? f[i] : 0.0f;
+    __syncthreads();
+    // the earlier of two tile vertices in the total order (value, vertex id);
+    // the local index order is the vertex id order within a tile
+    auto earlier = [&](int a, int c) { return val[a] < val[c] || (val[a] == val[c] && a < c); };
+    auto join = [&](int a, int c) {
+        while (true) {
+            a = lds_find(parent, a);
+            c = lds_find(parent, c);
+            if (a == c) return;
+            if (earlier(c, a)) {
+                const int s2 = a; a = c; c = s2;
+            }
+            if (atomicCAS(&parent[c], c, a) == c) return;  // hook the later root c under a
+        }
+    };
+    if (inside) {
+        const int32_t l = lab[t];
+        if (lx + 1 < kTx && x + 1 < g.X && lab[t + 1] == l) join(t, t + 1);
+        if (ly + 1 < kTy && y + 1 < g.Y && lab[t + kTx] == l) join(t, t + kTx);
+        if (lz + 1 < kTz && z + 1 < g.Z && lab[t + kTx * kTy] == l) join(t, t + kTx * kTy);
+    }
+    __syncthreads();
+    if (inside) {
+        const int r = lds_find(parent, t);
+        const int64_t ri = (int64_t)(ox + r % kTx) + (int64_t)g.X * ((int64_t)(oy + (r / kTx) % kTy) +
+                                                                       (int64_t)g.Y * (oz + r / (kTx * kTy)));
+        b.rep[i] = (int32_t)ri;
+    }
+}
+
+// The same-label edges across tile faces (+x, +y, +z).
 __global__ void __launch_bounds__(kThreads)
 cc_union_kernel(const float* __restrict__ f, VGrid g, LabelBuffers b) {
     const int64_t n = g.verts();
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-        const int32_t l = b.label2[i];
         const int x = (int)(i % g.X), y = (int)((i / g.X) % g.Y), z = (int)(i / g.sxy);
-        if (x < g.X - 1 && b.label2[i + 1] == l) uf_union(f, b.rep, (int32_t)i, (int32_t)(i + 1));
-        if (y < g.Y - 1 && b.label2[i + g.X] == l) uf_union(f, b.rep, (int32_t)i, (int32_t)(i + g.X));
-        if (z < g.Z - 1 && b.label2[i + g.sxy] == l) uf_union(f, b.rep, (int32_t)i, (int32_t)(i + g.sxy));
+        const bool fx = (x + 1) % kTx == 0 && x < g.X - 1, fy = (y + 1) % kTy == 0 && y < g.Y - 1;
+        const bool fz = (z + 1) % kTz == 0 && z < g.Z - 1;
+        if (!(fx || fy || fz)) continue;
+        const int32_t l = b.label2[i];
+        if (fx && b.label2[i + 1] == l) uf_union(f, b.rep, (int32_t)i, (int32_t)(i + 1));
+        if (fy && b.label2[i + g.X] == l) uf_union(f, b.rep, (int32_t)i, (int32_t)(i + g.X));
+        if (fz && b.label2[i + g.sxy] == l) uf_union(f, b.rep, (int32_t)i, (int32_t)(i + g.sxy));
     }
 }
 
@@ -1265,6 +1329,11 @@ hipError_t launch_cc_init(const VGrid& g, const LabelBuffers& b, hipStream_t s) 
     return hipGetLastError();
 }
 hipError_t launch_cc_union(const VGrid& g, const LabelBuffers& b, hipStream_t s) {
+    static_assert(kTx * kTy * kTz == kThreads, "one tile vertex per thread");
+    const int64_t tiles = (int64_t)((g.X + kTx - 1) / kTx) * ((g.Y + kTy - 1) / kTy) * ((g.Z + kTz - 1) / kTz);
+    if (tiles > 0x7FFFFFFF) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(cc_union_tile_kernel, dim3((uint32_t)tiles), dim3(kThreads), 0, s, b.values, g, b);
+    if (hipGetLastError() != hipSuccess) return hipGetLastError();
     hipLaunchKernelGGL(cc_union_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, b.values, g, b);
     if (hipGetLastError() != hipSuccess) return hipGetLastError();
     hipLaunchKernelGGL(cc_flatten_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, g, b);
