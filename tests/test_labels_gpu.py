@@ -91,6 +91,24 @@ def test_stage_entry_points_match_oracle(ctx, name):
     assert (g == ref["grad"]).all(), int((g != ref["grad"]).sum())
 
 
+@pytest.mark.parametrize("shape", [(23, 29, 37), (20, 40, 64), (9, 17, 5)])
+def test_clean_labels_matches_oracle_on_fragmented_labels(ctx, shape):
+    # many small same-label components, most of them crossing the 8x8x4
+    # tiles the GPU joins locally first, plus -1 (unlabelled) vertices and
+    # tied values (the total order then falls back to vertex ids)
+    rng = np.random.default_rng(sum(shape))
+    f = np.round(rng.standard_normal(shape) * 4.0).astype(np.float32)
+    Z, Y, X = shape
+    zz, yy, xx = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    lab = ((xx // 5) + 7 * (yy // 3) + 31 * (zz // 2)).astype(np.int32) % 11
+    flip = rng.random(shape) < 0.15
+    lab[flip] = rng.integers(-1, 11, size=int(flip.sum())).astype(np.int32)
+    _say(f"clean fragmented {shape}")
+    got = labels.clean_labels(f, lab, ctx=ctx).ravel()
+    expect = oracle_lib.clean(f, lab).ravel()
+    assert (got == expect).all(), int((got != expect).sum())
+
+
 @pytest.mark.parametrize("sign", [-1, 1])
 @pytest.mark.parametrize("name", ["labels_48", "smooth_34x33x32"])
 def test_integrate_labels_matches_oracle(ctx, name, sign):
