@@ -142,9 +142,13 @@ int run_integration(msceer_ctx* ctx, const VGrid& g, const LabelBuffers& b, cons
     for (int stride = top; stride >= 1; stride /= 2)
         CHECK_HIP(launch_pass1(g, b, stride, stride == top, grad_thr, err_thr, max_iters, sign, s));
     // pass 2 fills the re-integration queue, pass 3 drains it in one launch
-    const WorkQueue q{ctx->front0.as<uint32_t>(), (uint32_t)V, ctl + kCtlCursor, ctl + kCtlCursor + 1,
+    // each vertex is queued at most once as a re-integration and at most once
+    // to be traced ahead, so up to 2V items: the queue takes the whole
+    // frontier buffer (at least V words; items beyond it report overflow)
+    const uint64_t qcap = std::min<uint64_t>(std::min<uint64_t>(ctx->front0.bytes / 4, 2 * (uint64_t)V), 0xFFFFFFFFull);
+    const WorkQueue q{ctx->front0.as<uint32_t>(), (uint32_t)qcap, ctl + kCtlCursor, ctl + kCtlCursor + 1,
                       ctl + kCtlOverflow, ctl + kCtlCursor + 2, ctl + kCtlCursor + 3, ctl + kCtlCursor + 4};
-    CHECK_HIP(launch_fill_u32(q.items, V, 0xFFFFFFFFu, s));
+    CHECK_HIP(launch_fill_u32(q.items, (int64_t)qcap, 0xFFFFFFFFu, s));
     CHECK_HIP(launch_zero(ctl + kCtlCursor, 5, s));
     CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
     CHECK_HIP(launch_pass2(g, b, q, s));
