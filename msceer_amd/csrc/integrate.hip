@@ -279,12 +279,18 @@ __device__ __forceinline__ int64_t dir_offset(const VGrid& g, int j) {
     return (j & 1) ? -o : o;
 }
 
+// A list entry carries the vertex and its packed word (the count bits then
+// zero), so a round reads no counter of its own vertices.
+__device__ __forceinline__ uint64_t list_entry(uint32_t v, uint32_t meta) {
+    return (uint64_t)(meta & ~kCntMask) << 32 | v;
+}
+
 // The vertices with no earlier non-seed neighbour start the first list.
 __global__ void __launch_bounds__(kThreads)
 certain_init_kernel(const float* __restrict__ f, VGrid g, const uint8_t* __restrict__ seedmask,
-                    uint32_t* __restrict__ cnt, uint32_t* __restrict__ out, uint32_t* __restrict__ out_count,
+                    uint32_t* __restrict__ cnt, uint64_t* __restrict__ out, uint32_t* __restrict__ out_count,
                     uint32_t capacity, uint32_t* __restrict__ overflow) {
-    __shared__ uint32_t stage[kStage];
+    __shared__ uint64_t stage[kStage];
     __shared__ uint32_t s_n, s_base;
     if (threadIdx.x == 0) s_n = 0;
     __syncthreads();
@@ -310,8 +316,9 @@ certain_init_kernel(const float* __restrict__ f, VGrid g, const uint8_t* __restr
                 later |= 1u << j;
             }
         }
-        cnt[v] = c | earlier << kEarlierShift | later << kLaterShift;
-        if (c == 0) stage[atomicAdd(&s_n, 1u)] = (uint32_t)v;
+        const uint32_t meta = c | earlier << kEarlierShift | later << kLaterShift;
+        cnt[v] = meta;
+        if (c == 0) stage[atomicAdd(&s_n, 1u)] = list_entry((uint32_t)v, meta);
     }
     __syncthreads();
     const uint32_t m = s_n;
@@ -327,8 +334,8 @@ certain_init_kernel(const float* __restrict__ f, VGrid g, const uint8_t* __restr
 // One round: resolve the listed vertices, release their later neighbours.
 __global__ void __launch_bounds__(kThreads)
 certain_round_kernel(VGrid g, int32_t* __restrict__ label, uint32_t* __restrict__ cnt,
-                     const uint32_t* __restrict__ in, const uint32_t* __restrict__ in_count,
-                     uint32_t* __restrict__ out, uint32_t* __restrict__ out_count, uint32_t* __restrict__ clear_count,
+                     const uint64_t* __restrict__ in, const uint32_t* __restrict__ in_count,
+                     uint64_t* __restrict__ out, uint32_t* __restrict__ out_count, uint32_t* __restrict__ clear_count,
                      uint32_t capacity, uint32_t* __restrict__ overflow) {
     if (blockIdx.x == 0 && threadIdx.x == 0) *clear_count = 0u;
     const int lane = (int)(threadIdx.x & 63u);
@@ -338,11 +345,12 @@ certain_round_kernel(VGrid g, int32_t* __restrict__ label, uint32_t* __restrict_
     const uint32_t first = blockIdx.x * kThreads + (threadIdx.x & ~63u);
     for (uint32_t base = first; base < total; base += stride) {
         const uint32_t it = base + (uint32_t)lane;
-        uint32_t kids[6];
+        uint64_t kids[6];
         int nk = 0;
         if (it < total) {
-            const int64_t v = (int64_t)in[it];
-            const uint32_t meta = cnt[v];
+            const uint64_t e = in[it];
+            const int64_t v = (int64_t)(uint32_t)e;
+            const uint32_t meta = (uint32_t)(e >> 32);
             int32_t l[6];
             for (int j = 0; j < 6; j++)
                 l[j] = (meta >> (kEarlierShift + j)) & 1u ? label[v + dir_offset(g, j)] : -2;
@@ -358,7 +366,8 @@ certain_round_kernel(VGrid g, int32_t* __restrict__ label, uint32_t* __restrict_
             for (int j = 0; j < 6; j++) {
                 if (!((meta >> (kLaterShift + j)) & 1u)) continue;
                 const int64_t u = v + dir_offset(g, j);
-                if ((atomicSub(&cnt[u], 1u) & kCntMask) == 1u) kids[nk++] = (uint32_t)u;
+                const uint32_t old = atomicSub(&cnt[u], 1u);
+                if ((old & kCntMask) == 1u) kids[nk++] = list_entry((uint32_t)u, old);
             }
         }
         if (__any(nk > 0)) {
@@ -1273,15 +1282,15 @@ uint32_t resident_blocks(K kernel) {
 }
 }  // namespace
 hipError_t launch_certain_init(const VGrid& g, const float* values, const uint8_t* seedmask, uint32_t* cnt,
-                               uint32_t* out, uint32_t* out_count, uint32_t capacity, uint32_t* overflow,
+                               uint64_t* out, uint32_t* out_count, uint32_t capacity, uint32_t* overflow,
                                hipStream_t s) {
     const int64_t blocks = (g.verts() + kStage - 1) / kStage;
     hipLaunchKernelGGL(certain_init_kernel, dim3((uint32_t)blocks), dim3(kThreads), 0, s, values, g, seedmask, cnt,
                        out, out_count, capacity, overflow);
     return hipGetLastError();
 }
-hipError_t launch_certain_round(const VGrid& g, int32_t* label, uint32_t* cnt, const uint32_t* in,
-                                const uint32_t* in_count, uint32_t* out, uint32_t* out_count, uint32_t* clear_count,
+hipError_t launch_certain_round(const VGrid& g, int32_t* label, uint32_t* cnt, const uint64_t* in,
+                                const uint32_t* in_count, uint64_t* out, uint32_t* out_count, uint32_t* clear_count,
                                 uint32_t capacity, uint32_t* overflow, uint64_t expected, hipStream_t s) {
     // about two listed vertices per lane: one work-group per compute unit for
     // the thousands of a smooth field's rounds, up to 32 for the millions of a

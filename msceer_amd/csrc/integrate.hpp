@@ -82,15 +82,16 @@ struct WorkQueue {
 hipError_t launch_pass2(const VGrid& g, const LabelBuffers& b, const WorkQueue& q, hipStream_t s);
 // Certain regions in dependency order (label[v] becomes the label shared by
 // all earlier All6 neighbours of v when they all carry one; seeds are
-// labelled beforehand; cnt: one uint32 per vertex).  init lists the vertices
+// labelled beforehand; cnt: one uint32 per vertex; lists of 64-bit entries,
+// capacity entries each).  init lists the vertices
 // with no earlier non-seed neighbour; each round resolves the list `in` and
 // appends the vertices it releases to `out`, and zeroes *clear_count
 // (expected: about how many vertices `in` holds, which sizes the launch).
 hipError_t launch_certain_init(const VGrid& g, const float* values, const uint8_t* seedmask, uint32_t* cnt,
-                               uint32_t* out, uint32_t* out_count, uint32_t capacity, uint32_t* overflow,
+                               uint64_t* out, uint32_t* out_count, uint32_t capacity, uint32_t* overflow,
                                hipStream_t s);
-hipError_t launch_certain_round(const VGrid& g, int32_t* label, uint32_t* cnt, const uint32_t* in,
-                                const uint32_t* in_count, uint32_t* out, uint32_t* out_count, uint32_t* clear_count,
+hipError_t launch_certain_round(const VGrid& g, int32_t* label, uint32_t* cnt, const uint64_t* in,
+                                const uint32_t* in_count, uint64_t* out, uint32_t* out_count, uint32_t* clear_count,
                                 uint32_t capacity, uint32_t* overflow, uint64_t expected, hipStream_t s);
 // One pass-3 round over `in` (FP64 re-integration to the certain regions);
 // changed labels push their ASSIGNED neighbours to `out`; `clear` counts are zeroed.

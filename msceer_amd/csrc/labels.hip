@@ -325,11 +325,12 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
     // shows whether the next round's list is empty
     int sweeps = 0;
     {
-        uint32_t* lists[2] = {ctx->front0.as<uint32_t>(), ctx->front1.as<uint32_t>()};
+        uint64_t* lists[2] = {ctx->front0.as<uint64_t>(), ctx->front1.as<uint64_t>()};
+        const uint32_t lcap = (uint32_t)std::min<uint64_t>(std::min(ctx->front0.bytes, ctx->front1.bytes) / 8, V);
         uint32_t* counts = ctl + kCtlRound;  // three rotating count words
         CHECK_HIP(launch_zero(counts, 3, s));
         CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
-        CHECK_HIP(launch_certain_init(g, d_values, b.seedmask, b.stamp, lists[0], counts, (uint32_t)V,
+        CHECK_HIP(launch_certain_init(g, d_values, b.seedmask, b.stamp, lists[0], counts, lcap,
                                       ctl + kCtlOverflow, s));
         rc = read_ctl(ctx, 0, kCtlWords);
         if (rc) return rc;
@@ -342,7 +343,7 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
             for (int j = 0; j < kGroup; j++, k++)
                 CHECK_HIP(launch_certain_round(g, b.label, b.stamp, lists[k % 2], counts + k % 3,
                                                lists[(k + 1) % 2], counts + (k + 1) % 3, counts + (k + 2) % 3,
-                                               (uint32_t)V, ctl + kCtlOverflow, expected, s));
+                                               lcap, ctl + kCtlOverflow, expected, s));
             rc = read_ctl(ctx, 0, kCtlWords);
             if (rc) return rc;
             if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;
