@@ -73,11 +73,19 @@ __device__ __forceinline__ int clampi(int v, int n) { return v < 0 ? 0 : (v >= n
 // (Away from the boundary the position lies inside the grid and the final
 // index clamp changes nothing; it only keeps a corrupt position in bounds.)
 template <typename T>
-__device__ __forceinline__ int64_t nearest(const VGrid& g, const V3<T>& p, bool near) {
+__device__ __forceinline__ int64_t nearest_xyz(const VGrid& g, const V3<T>& p, bool near, int& x, int& y, int& z) {
     T qx = p.x + (T)0.5, qy = p.y + (T)0.5, qz = p.z + (T)0.5;
     if (near) { qx = clampc(qx, g.X); qy = clampc(qy, g.Y); qz = clampc(qz, g.Z); }
-    return (int64_t)clampi((int)qx, g.X) +
-           (int64_t)g.X * ((int64_t)clampi((int)qy, g.Y) + (int64_t)g.Y * (int64_t)clampi((int)qz, g.Z));
+    x = clampi((int)qx, g.X);
+    y = clampi((int)qy, g.Y);
+    z = clampi((int)qz, g.Z);
+    return (int64_t)x + (int64_t)g.X * ((int64_t)y + (int64_t)g.Y * (int64_t)z);
+}
+
+template <typename T>
+__device__ __forceinline__ int64_t nearest(const VGrid& g, const V3<T>& p, bool near) {
+    int x, y, z;
+    return nearest_xyz(g, p, near, x, y, z);
 }
 
 __device__ __forceinline__ int dist_to_boundary(const VGrid& g, int x, int y, int z) {
@@ -181,16 +189,14 @@ __device__ __forceinline__ int tracer_iterate(const GradSrc<T>& src, const VGrid
     if (t.near) { t.cur.x = clampc(t.cur.x, g.X); t.cur.y = clampc(t.cur.y, g.Y); t.cur.z = clampc(t.cur.z, g.Z); }
     const T grow = t.step * (T)1.8, cap = (T)kMaxStep / t.gm;
     t.step = grow < cap ? grow : cap;
-    next = nearest(g, t.cur, t.near);
+    int nx, ny, nz;
+    next = nearest_xyz(g, t.cur, t.near, nx, ny, nz);
     const uint8_t d = dest[next];
     // the next iteration would leave the voxel: gather the next voxel's
     // corners while the destination byte is in flight
     const bool leave = t.iters > 0 &&
                        ((int)t.cur.x != (int)t.old.x || (int)t.cur.y != (int)t.old.y || (int)t.cur.z != (int)t.old.z);
-    if (leave) {
-        const int cx = (int)(next % g.X), cy = (int)((next / g.X) % g.Y), cz = (int)(next / g.sxy);
-        tracer_load(src, g, t, cx, cy, cz);
-    }
+    if (leave) tracer_load(src, g, t, nx, ny, nz);
     if (d == kCertainTerminal || (stop_on_assigned && d == kAssigned)) return kHitPreassigned;
     if (leave) {
         t.iters--;  // the iteration that leaves the voxel
@@ -203,8 +209,9 @@ __device__ __forceinline__ int tracer_iterate(const GradSrc<T>& src, const VGrid
 // Enter the voxel of the position after kOutOfVoxel; returns the vertex nearest to it.
 template <typename T>
 __device__ __forceinline__ int64_t tracer_reenter(const GradSrc<T>& src, const VGrid& g, Tracer<T>& t) {
-    const int64_t next = nearest(g, t.cur, t.near);
-    tracer_load(src, g, t, (int)(next % g.X), (int)((next / g.X) % g.Y), (int)(next / g.sxy));
+    int x, y, z;
+    const int64_t next = nearest_xyz(g, t.cur, t.near, x, y, z);
+    tracer_load(src, g, t, x, y, z);
     tracer_begin(t);
     return next;
 }
