@@ -48,11 +48,23 @@ __device__ __forceinline__ bool before(const float* f, int64_t a, int64_t b) {
 
 // ---- gradient field ---------------------------------------------------------
 
+// Coordinates of vertex v in 32-bit arithmetic: every grid here has fewer
+// than 2^32 vertices (its cells number fewer than 2^32), and 64-bit integer
+// division is a long instruction sequence on the GPU.
+__device__ __forceinline__ void vertex_xyz(const VGrid& g, int64_t v, int& x, int& y, int& z) {
+    const uint32_t u = (uint32_t)v, X = (uint32_t)g.X, Y = (uint32_t)g.Y;
+    const uint32_t q = u / X, r = q / Y;
+    x = (int)(u - q * X);
+    y = (int)(q - r * Y);
+    z = (int)r;
+}
+
 __global__ void __launch_bounds__(kThreads)
 grad_field_kernel(const float* __restrict__ f, VGrid g, float4* __restrict__ out) {
     const int64_t n = g.verts();
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-        const int x = (int)(i % g.X), y = (int)((i / g.X) % g.Y), z = (int)(i / g.sxy);
+        int x, y, z;
+        vertex_xyz(g, i, x, y, z);
         float4 r;
         r.x = (x > 0 && x < g.X - 1) ? (float)(0.5 * ((double)f[i + 1] - (double)f[i - 1])) : 0.f;
         r.y = (y > 0 && y < g.Y - 1) ? (float)(0.5 * ((double)f[i + g.X] - (double)f[i - g.X])) : 0.f;
@@ -102,7 +114,8 @@ grad_field64_kernel(const float* __restrict__ f, VGrid g, const uint8_t* __restr
     for (int64_t v = (int64_t)blockIdx.x * kThreads + threadIdx.x; v < n; v += (int64_t)gridDim.x * kThreads) {
         double4 r = make_double4(0.0, 0.0, 0.0, 0.0);
         if (!seedmask[v]) {
-            const int x = (int)(v % g.X), y = (int)((v / g.X) % g.Y), z = (int)(v / g.sxy);
+            int x, y, z;
+            vertex_xyz(g, v, x, y, z);
             r.x = (x > 0 && x < g.X - 1) ? (0.0 + (double)(-0.5f * f[v - 1])) + (double)(0.5f * f[v + 1]) : 0.0;
             r.y = (y > 0 && y < g.Y - 1) ? (0.0 + (double)(-0.5f * f[v - g.X])) + (double)(0.5f * f[v + g.X]) : 0.0;
             r.z = (z > 0 && z < g.Z - 1) ? (0.0 + (double)(-0.5f * f[v - g.sxy])) + (double)(0.5f * f[v + g.sxy])
@@ -222,7 +235,8 @@ struct Outbox {
 };
 
 __device__ __forceinline__ int all6(const VGrid& g, int64_t i, int64_t nb[6]) {
-    const int x = (int)(i % g.X), y = (int)((i / g.X) % g.Y), z = (int)(i / g.sxy);
+    int x, y, z;
+    vertex_xyz(g, i, x, y, z);
     int k = 0;
     if (x < g.X - 1) nb[k++] = i + 1;
     if (x > 0) nb[k++] = i - 1;
@@ -302,7 +316,8 @@ certain_init_kernel(const float* __restrict__ f, VGrid g, const uint8_t* __restr
             cnt[v] = 0u;
             continue;
         }
-        const int x = (int)(v % g.X), y = (int)((v / g.X) % g.Y), z = (int)(v / g.sxy);
+        int x, y, z;
+        vertex_xyz(g, v, x, y, z);
         const bool inside[6] = {x < g.X - 1, x > 0, y < g.Y - 1, y > 0, z < g.Z - 1, z > 0};
         uint32_t c = 0, earlier = 0, later = 0;
         for (int j = 0; j < 6; j++) {
@@ -638,7 +653,9 @@ pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr
                     active = true;
                     speculative = ahead_item;
                     if (ahead_item) ahead++;
-                    tracer_start(src, g, tr, (int)(v % g.X), (int)((v / g.X) % g.Y), (int)(v / g.sxy), max_iters);
+                    int sx, sy, sz;
+                    vertex_xyz(g, v, sx, sy, sz);
+                    tracer_start(src, g, tr, sx, sy, sz, max_iters);
                 }
             }
         }
@@ -740,9 +757,11 @@ pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr
                 int n = 0;
                 if (o0 + lane < kShell) {
                     const uint32_t o = kShellOffsets[o0 + lane];
-                    const int ux = (int)(c % g.X) + (int)(o & 15u) - kShellR;
-                    const int uy = (int)((c / g.X) % g.Y) + (int)((o >> 4) & 15u) - kShellR;
-                    const int uz = (int)(c / g.sxy) + (int)(o >> 8) - kShellR;
+                    int cx, cy, cz;
+                    vertex_xyz(g, c, cx, cy, cz);
+                    const int ux = cx + (int)(o & 15u) - kShellR;
+                    const int uy = cy + (int)((o >> 4) & 15u) - kShellR;
+                    const int uz = cz + (int)(o >> 8) - kShellR;
                     if (ux >= 0 && uy >= 0 && uz >= 0 && ux < g.X && uy < g.Y && uz < g.Z) {
                         const int64_t u = (int64_t)ux + (int64_t)g.X * ((int64_t)uy + (int64_t)g.Y * uz);
                         if (b.dest[u] == kAssigned && b.stamp[u] == 0u && spec[u] == kSpecNone && b.label[u] != cl &&
@@ -876,7 +895,8 @@ __global__ void __launch_bounds__(kThreads)
 cc_union_kernel(const float* __restrict__ f, VGrid g, LabelBuffers b) {
     const int64_t n = g.verts();
     for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
-        const int x = (int)(i % g.X), y = (int)((i / g.X) % g.Y), z = (int)(i / g.sxy);
+        int x, y, z;
+        vertex_xyz(g, i, x, y, z);
         const bool fx = (x + 1) % kTx == 0 && x < g.X - 1, fy = (y + 1) % kTy == 0 && y < g.Y - 1;
         const bool fz = (z + 1) % kTz == 0 && z < g.Z - 1;
         if (!(fx || fy || fz)) continue;
@@ -1174,7 +1194,9 @@ collect_minima_kernel(VGrid g, const uint8_t* __restrict__ grad, MList out, uint
     const int64_t Mx = 2 * (int64_t)g.X - 1, My = 2 * (int64_t)g.Y - 1;
     const int64_t n = g.verts();
     for (int64_t v = (int64_t)blockIdx.x * kThreads + threadIdx.x; v < n; v += (int64_t)gridDim.x * kThreads) {
-        const int64_t x = v % g.X, y = (v / g.X) % g.Y, z = v / g.sxy;
+        int xi, yi, zi;
+        vertex_xyz(g, v, xi, yi, zi);
+        const int64_t x = xi, y = yi, z = zi;
         const int64_t c = 2 * x + Mx * (2 * y + My * 2 * z);
         if (((grad[c] >> 2) & 7u) == 7u) box.push((uint32_t)v);
     }
@@ -1189,7 +1211,9 @@ combinatorial_labels_kernel(VGrid g, const uint8_t* __restrict__ grad, const int
     const int64_t n = g.verts(), limit = n + 1, cells = cg.sxy * (2 * (int64_t)g.Z - 1);
     for (int64_t v = (int64_t)blockIdx.x * kThreads + threadIdx.x; v < n; v += (int64_t)gridDim.x * kThreads) {
         if (only_where_zero && only_where_zero[v]) continue;
-        const int64_t x = v % g.X, y = (v / g.X) % g.Y, z = v / g.sxy;
+        int xi, yi, zi;
+        vertex_xyz(g, v, xi, yi, zi);
+        const int64_t x = xi, y = yi, z = zi;
         const int64_t c = 2 * x + cg.Mx * (2 * y + cg.My * 2 * z);
         const int64_t m = trace_down_vertex(cg, grad, c, limit, cells);
         if (m < 0) { out[v] = -1; continue; }
