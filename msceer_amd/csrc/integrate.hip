@@ -59,6 +59,16 @@ __device__ __forceinline__ void vertex_xyz(const VGrid& g, int64_t v, int& x, in
     z = (int)r;
 }
 
+// Coordinates of cell c of an Mx x My x Mz cell grid (fewer than 2^32 cells)
+// in 32-bit arithmetic.
+__device__ __forceinline__ void cell_xyz(int64_t c, int64_t Mx, int64_t My, int64_t& x, int64_t& y, int64_t& z) {
+    const uint32_t u = (uint32_t)c, mx = (uint32_t)Mx, my = (uint32_t)My;
+    const uint32_t q = u / mx, r = q / my;
+    x = (int64_t)(u - q * mx);
+    y = (int64_t)(q - r * my);
+    z = (int64_t)r;
+}
+
 __global__ void __launch_bounds__(kThreads)
 grad_field_kernel(const float* __restrict__ f, VGrid g, float4* __restrict__ out) {
     const int64_t n = g.verts();
@@ -1031,7 +1041,8 @@ max_boundary_kernel(VGrid g, const float* __restrict__ f, const int32_t* __restr
     const int64_t Mx = 2 * (int64_t)g.X - 1, My = 2 * (int64_t)g.Y - 1, Mz = 2 * (int64_t)g.Z - 1;
     const int64_t sxy = Mx * My, C = sxy * Mz;
     for (int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x; c < C; c += (int64_t)gridDim.x * kThreads) {
-        const int64_t cx = c % Mx, cy = (c / Mx) % My, cz = c / sxy;
+        int64_t cx, cy, cz;
+        cell_xyz(c, Mx, My, cx, cy, cz);
         if (cx == 0 || cx == Mx - 1 || cy == 0 || cy == My - 1 || cz == 0 || cz == Mz - 1) continue;
         const int ox = (int)(cx & 1), oy = (int)(cy & 1), oz = (int)(cz & 1);
         const int dim = ox + oy + oz;
@@ -1066,7 +1077,8 @@ min_boundary_kernel(VGrid g, const int32_t* __restrict__ label, int8_t* __restri
     const int64_t Mx = 2 * (int64_t)g.X - 1, My = 2 * (int64_t)g.Y - 1, Mz = 2 * (int64_t)g.Z - 1;
     const int64_t C = Mx * My * Mz;
     for (int64_t c = (int64_t)blockIdx.x * kThreads + threadIdx.x; c < C; c += (int64_t)gridDim.x * kThreads) {
-        const int64_t cx = c % Mx, cy = (c / Mx) % My, cz = c / (Mx * My);
+        int64_t cx, cy, cz;
+        cell_xyz(c, Mx, My, cx, cy, cz);
         const int ox = (int)(cx & 1), oy = (int)(cy & 1), oz = (int)(cz & 1);
         int8_t res = 0;
         if (ox + oy + oz > 0) {
@@ -1131,8 +1143,9 @@ __device__ int64_t trace_down_vertex(const CellG& cg, const uint8_t* grad, int64
 }
 
 __device__ inline int64_t vertex_of_cell(const CellG& cg, int64_t c) {
-    const int64_t x = (c % cg.Mx) / 2, y = ((c / cg.Mx) % cg.My) / 2, z = (c / cg.sxy) / 2;
-    return x + ((cg.Mx + 1) / 2) * (y + ((cg.My + 1) / 2) * z);
+    int64_t x, y, z;
+    cell_xyz(c, cg.Mx, cg.My, x, y, z);
+    return x / 2 + ((cg.Mx + 1) / 2) * (y / 2 + ((cg.My + 1) / 2) * (z / 2));
 }
 
 // Arcs of the extremum graph below the simplification threshold.  The
@@ -1148,7 +1161,8 @@ saddle_arcs_kernel(VGrid g, const float* __restrict__ f, const uint8_t* __restri
     const int64_t limit = g.verts() + 1, cells = cg.sxy * (2 * (int64_t)g.Z - 1);
     for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < total; i += gridDim.x * kThreads) {
         const int64_t s = mlist_at(list, prefix, i);
-        const int64_t cx = s % cg.Mx, cy = (s / cg.Mx) % cg.My;
+        int64_t cx, cy, cz;
+        cell_xyz(s, cg.Mx, cg.My, cx, cy, cz);
         const int64_t step = (cx & 1) ? 1 : ((cy & 1) ? cg.Mx : cg.sxy);
         // facets of the edge in +axis, -axis order
         const int64_t m1 = trace_down_vertex(cg, grad, s + step, limit, cells);
@@ -1217,7 +1231,11 @@ combinatorial_labels_kernel(VGrid g, const uint8_t* __restrict__ grad, const int
         const int64_t c = 2 * x + cg.Mx * (2 * y + cg.My * 2 * z);
         const int64_t m = trace_down_vertex(cg, grad, c, limit, cells);
         if (m < 0) { out[v] = -1; continue; }
-        const int64_t mx = (m % cg.Mx) / 2, my = ((m / cg.Mx) % cg.My) / 2, mz = (m / cg.sxy) / 2;
+        int64_t mx, my, mz;
+        cell_xyz(m, cg.Mx, cg.My, mx, my, mz);
+        mx /= 2;
+        my /= 2;
+        mz /= 2;
         out[v] = min_label[mx + (int64_t)g.X * (my + (int64_t)g.Y * mz)];
     }
 }
