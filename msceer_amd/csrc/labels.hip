@@ -142,11 +142,18 @@ int run_integration(msceer_ctx* ctx, const VGrid& g, const LabelBuffers& b, cons
     // and 2, 1 take 47.6 ms at the same FP64-oracle mismatch (0.050 %), while
     // stride 1 alone raised it to 0.053 %
     constexpr int kTopStride = 2;
+    // Pass 1 is the FP32 first guess: a streamline that exhausts this budget is
+    // flagged and re-traced in FP64 by pass 3 with the full max_iters, as the
+    // ones exhausting max_iters are.  The few that run that long set each
+    // stride's length (256^3: 47.6 -> 43.2 ms for the whole call, FP64-oracle
+    // mismatch 0.050 % either way; 537 of 5.0 M streamlines hit it).
+    constexpr int kPass1Budget = 1024;
     const int max_dim = std::max(g.X, std::max(g.Y, g.Z));
     int top = 1;
     while (top * 2 <= kTopStride && top * 2 < max_dim) top *= 2;
     for (int stride = top; stride >= 1; stride /= 2)
-        CHECK_HIP(launch_pass1(g, b, stride, stride == top, grad_thr, err_thr, max_iters, sign, s));
+        CHECK_HIP(launch_pass1(g, b, stride, stride == top, grad_thr, err_thr, std::min(max_iters, kPass1Budget),
+                               sign, s));
     // pass 2 fills the re-integration queue, pass 3 drains it in one launch
     // each vertex is queued at most once as a re-integration and at most once
     // to be traced ahead, so up to 2V items: the queue takes the whole
