@@ -348,8 +348,15 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
         rc = read_ctl(ctx, 0, kCtlWords);
         if (rc) return rc;
         int k = 0;
-        // the launches of a group are sized by the list length last read back
+        // Groups are pipelined: group i+1 is enqueued before the host reads the
+        // list length that group i left (copied to pinned memory behind it), so
+        // the GPU does not wait for the host between groups; the group after
+        // the last non-empty list runs empty rounds.  The launches of a group
+        // are sized by the list length last read back.
         uint64_t expected = ctx->h_ctl[kCtlRound];
+        CHECK_HIP(ctx->host_stage.ensure(8 * sizeof(uint32_t)));
+        const volatile uint32_t* seen = ctx->host_stage.as<uint32_t>();  // two slots of the three counts
+        int done = 0, kprev = 0;
         for (int guard = 0; expected > 0; guard++) {
             if (guard > (int)(V / 16 + 16)) return MSCEER_EHIP;  // a round resolves at least one vertex
             constexpr int kGroup = 16;
@@ -357,12 +364,25 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
                 CHECK_HIP(launch_certain_round(g, b.label, b.stamp, lists[k % 2], counts + k % 3,
                                                lists[(k + 1) % 2], counts + (k + 1) % 3, counts + (k + 2) % 3,
                                                lcap, ctl + kCtlOverflow, expected, s));
-            rc = read_ctl(ctx, 0, kCtlWords);
-            if (rc) return rc;
-            if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;
-            expected = ctx->h_ctl[kCtlRound + k % 3];
+            const int slot = guard & 1;
+            CHECK_HIP(hipMemcpyAsync(ctx->host_stage.as<uint32_t>() + 4 * slot, counts, 3 * sizeof(uint32_t),
+                                     hipMemcpyDeviceToHost, s));
+            CHECK_HIP(hipEventRecord(ctx->ev[8 + slot], s));
+            if (guard > 0) {
+                CHECK_HIP(hipEventSynchronize(ctx->ev[8 + (slot ^ 1)]));
+                const uint32_t left = seen[4 * (slot ^ 1) + kprev % 3];
+                if (left == 0) {
+                    done = kprev;
+                    break;
+                }
+                expected = left;
+            }
+            kprev = k;
         }
-        sweeps = k;
+        rc = read_ctl(ctx, 0, kCtlWords);  // also waits for the last group
+        if (rc) return rc;
+        if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;
+        sweeps = done ? done : k;
     }
     st.certain_sweeps = sweeps;
     CHECK_HIP(hipEventRecord(ev[2], s));
