@@ -334,7 +334,7 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
         CHECK_HIP(hipMemcpyAsync(d_sl, seed_l.data(), sizeof(int32_t) * seed_l.size(), hipMemcpyHostToDevice, s));
         CHECK_HIP(launch_seed_scatter(d_sv, d_sl, (int)seed_v.size(), b, s));
     }
-    // rounds in dependency order, enqueued in groups; one read-back per group
+    // rounds in dependency order, enqueued in groups; a read-back per group
     // shows whether the next round's list is empty
     int sweeps = 0;
     {
