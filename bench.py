@@ -403,8 +403,8 @@ def main():
         calibrate_copy()
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); "
-              "the multi-GPU sharding is described in DESIGN.md", file=sys.stderr)
+        print(f"bench.py: this implementation runs on one GPU only (asked for --gpus {args.gpus}); "
+              "sharding across GPUs (z-slabs with halo exchange) is a design, not built", file=sys.stderr)
         sys.exit(2)
 
     if args.labels_only:

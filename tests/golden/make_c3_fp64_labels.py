@@ -3,7 +3,8 @@ ascending labels of the 256^3 labelling workload (200 Gaussians, sigma 4-16,
 plus 0.05 sin sin sin, seed 303; err 1e-3 (squared-error test), grad threshold
 1e-6, 4096 steps, no presimplification).
 
-This is the judge of the GPU labels at full size (label mismatch rate).  The
+These labels are what the GPU labels are measured against at full size (label
+mismatch rate).  The
 oracle's FP64 mode follows the reference pipeline (ondemandaccurate.cxx:728-1061)
 with every advection quantity in FP64; its reference-arithmetic mode is pinned
 bit-exact to reference runs by tests/test_oracle_ondemand.py.
