@@ -58,6 +58,24 @@ def test_labels_and_conforming_grad(ctx, name):
     gradcheck.check_consistency(grad, X, Y, Z)
 
 
+@pytest.mark.parametrize("shape", [(2, 2, 2), (5, 2, 3), (2, 7, 5), (4, 3, 9)])
+def test_tiny_volumes(ctx, shape):
+    # the smallest grids (two vertices along an axis, one tile, one stride):
+    # the pipeline runs, labels are minima of the field's own gradient or -1,
+    # and the conforming .grad is the oracle's for those labels
+    rng = np.random.default_rng(sum(shape) * 7)
+    f = rng.standard_normal(shape).astype(np.float32)
+    Z, Y, X = shape
+    _say(f"labels tiny {shape}")
+    grad, lab, _ = labels.ondemand_asc(f, ctx=ctx)
+    lab = lab.ravel()
+    assert ((lab == -1) | ((lab >= 0) & (lab < f.size))).all()
+    r = oracle_lib.min_boundary(lab, f.shape)
+    expect = oracle_lib.flags(oracle_lib.ldir(oracle_lib.lstar_pairs(f, r), f.shape), f.shape)
+    assert (grad == expect).all(), int((grad != expect).sum())
+    gradcheck.check_consistency(grad, X, Y, Z)
+
+
 def test_stages_are_monotone(ctx):
     # certain regions keep their labels through integration and cleanup
     f, (err, gthr, iters, presimp), ref = load_ondemand("smooth_34x33x32")
