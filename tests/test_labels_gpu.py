@@ -76,6 +76,23 @@ def test_tiny_volumes(ctx, shape):
     gradcheck.check_consistency(grad, X, Y, Z)
 
 
+@pytest.mark.parametrize("kind", ["constant", "ramp"])
+def test_degenerate_fields(ctx, kind):
+    # all ties (order by vertex id) and a linear ramp: the box's corners are
+    # critical in their own boundary strata; the labels equal the FP64
+    # oracle's exactly (every vertex is certain or flows straight down)
+    Z, Y, X = 11, 13, 17
+    zz, yy, xx = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    f = np.zeros((Z, Y, X), np.float32) if kind == "constant" else (xx + 2.0 * yy + 3.0 * zz).astype(np.float32)
+    _say(f"labels {kind} {f.shape}")
+    grad, lab, st = labels.ondemand_asc(f, ctx=ctx)
+    _, lab64, _, _ = oracle_lib.ondemand_asc(f, 1e-3, 1e-6, 4096, 0.0, mode=1)
+    assert (lab.ravel() == lab64).all(), int((lab.ravel() != lab64).sum())
+    r = oracle_lib.min_boundary(lab.ravel(), f.shape)
+    expect = oracle_lib.flags(oracle_lib.ldir(oracle_lib.lstar_pairs(f, r), f.shape), f.shape)
+    assert (grad == expect).all(), int((grad != expect).sum())
+
+
 def test_stages_are_monotone(ctx):
     # certain regions keep their labels through integration and cleanup
     f, (err, gthr, iters, presimp), ref = load_ondemand("smooth_34x33x32")
