@@ -109,9 +109,6 @@ struct msceer_ctx {
     // accurate labelling
     msceer::DevBuf gradf, gradd, scan, label, label2, dest, seedmask, redo, stamp, rep, rep2, restriction, host_io, mask;
     msceer::HostBuf host_stage;  // pinned staging of host-side merges (cleanup)
-    // host union-find of the cleanup's candidate roots, kept between calls so
-    // that its pages are not faulted in again for every volume
-    std::vector<int32_t> cc_parent, cc_term, cc_termlab;
     uint32_t* h_ctl = nullptr;  // pinned mirror of the control block
     hipEvent_t ev[12] = {};
     msceer_timing last{};

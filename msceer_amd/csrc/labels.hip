@@ -78,17 +78,8 @@ int run_cleanup(msceer_ctx* ctx, const VGrid& g, const LabelBuffers& b) {
     // candidates in vertex-id order; a set of candidates merged together has
     // a representative q, and term[q] >= 0 once the set joined a region that
     // no candidate represents (that root, with label termlab[q])
-    std::vector<int32_t>& parent = ctx->cc_parent;
-    std::vector<int32_t>& term = ctx->cc_term;
-    std::vector<int32_t>& termlab = ctx->cc_termlab;
-    if (parent.size() < R) {
-        parent.resize(R);
-        term.resize(R);
-        termlab.resize(R);
-    }
+    std::vector<int32_t> parent(R), term(R, -1), termlab(R, -1);
     for (uint32_t k = 0; k < R; k++) parent[k] = (int32_t)k;
-    std::fill(term.begin(), term.begin() + R, -1);
-    std::fill(termlab.begin(), termlab.begin() + R, -1);
     auto find = [&](int32_t x) {
         int32_t r = x;
         while (parent[r] != r) r = parent[r];
