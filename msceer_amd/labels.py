@@ -34,6 +34,8 @@ def ondemand_asc(values: np.ndarray, err_thr: float = 1e-3, grad_thr: float = 1e
         labels = np.empty(X * Y * Z, np.int32)
         grad = np.empty(num_cells(X, Y, Z), np.uint8) if want_grad and stop is None else None
         m = None if mask is None else np.ascontiguousarray(mask, np.uint8).ravel()
+        if m is not None and m.size != v.size:
+            raise ValueError(f"mask holds {m.size} entries, the volume {v.size} vertices")
         st = native.LabelStats()
         flags = {None: 0, "regions": native.STOP_REGIONS, "integrated": native.STOP_INTEGRATED}[stop]
         rc = native.lib().msceer_ondemand_asc(
@@ -73,8 +75,11 @@ def integrate_labels(values: np.ndarray, terminals: np.ndarray, zero_grad: np.nd
     v = np.ascontiguousarray(values, np.float32)
     Z, Y, X = v.shape
     t = np.ascontiguousarray(terminals, np.int32)
-    assert t.shape == v.shape
+    if t.size != v.size:
+        raise ValueError(f"terminals hold {t.size} entries, the volume {v.size} vertices")
     zg = None if zero_grad is None else np.ascontiguousarray(zero_grad, np.uint8)
+    if zg is not None and zg.size != v.size:
+        raise ValueError(f"zero_grad holds {zg.size} entries, the volume {v.size} vertices")
     out = np.empty(v.shape, np.int32)
     st = native.LabelStats()
     own = ctx is None
@@ -95,6 +100,8 @@ def clean_labels(values: np.ndarray, labels: np.ndarray, ctx: native.Context | N
     v = np.ascontiguousarray(values, np.float32)
     Z, Y, X = v.shape
     lab = np.ascontiguousarray(labels, np.int32).copy()
+    if lab.size != v.size:
+        raise ValueError(f"labels hold {lab.size} entries, the volume {v.size} vertices")
     own = ctx is None
     ctx = ctx or native.Context()
     try:
@@ -128,9 +135,11 @@ def restriction_add_max(values: np.ndarray, labels: np.ndarray, restriction: np.
     v = np.ascontiguousarray(values, np.float32)
     Z, Y, X = v.shape
     lab = np.ascontiguousarray(labels, np.int32)
-    assert lab.size == v.size
+    if lab.size != v.size:
+        raise ValueError(f"labels hold {lab.size} entries, the volume {v.size} vertices")
     r = np.ascontiguousarray(restriction, np.int8).ravel().copy()
-    assert r.size == num_cells(X, Y, Z)
+    if r.size != num_cells(X, Y, Z):
+        raise ValueError(f"restriction holds {r.size} entries, the cell complex {num_cells(X, Y, Z)} cells")
     own = ctx is None
     ctx = ctx or native.Context()
     try:
@@ -148,6 +157,8 @@ def constrained_grad(values: np.ndarray, restriction: np.ndarray, ctx: native.Co
     v = np.ascontiguousarray(values, np.float32)
     Z, Y, X = v.shape
     r = np.ascontiguousarray(restriction, np.int8)
+    if r.size != num_cells(X, Y, Z):
+        raise ValueError(f"restriction holds {r.size} entries, the cell complex {num_cells(X, Y, Z)} cells")
     out = np.empty(num_cells(X, Y, Z), np.uint8)
     own = ctx is None
     ctx = ctx or native.Context()
