@@ -135,18 +135,37 @@ grad_field64_kernel(const float* __restrict__ f, VGrid g, const uint8_t* __restr
     }
 }
 
-// Pass 1 at one level of the coarse-to-fine order: vertices whose coordinates
-// are multiples of `stride` (and not all multiples of 2*stride unless this is
-// the first level).  A streamline stops on a certain or already assigned
-// vertex; the first kPath vertices of its path take the label it reached.
+// Pass 1 at one level of the coarse-to-fine order: the seeds are the
+// unassigned vertices whose coordinates are multiples of `stride` (and not
+// all multiples of 2*stride unless this is the first level).
+//
+// Levels are synchronous, so that the labels do not depend on the order in
+// which streamlines run: a streamline stops on a certain terminal or on a
+// vertex assigned by an EARLIER level (dest becomes kAssigned only in
+// pass1_finish_kernel, between levels), never on a path of its own level.
+// The seed takes the label its streamline reaches, written at once (no
+// streamline of the level reads it).  Every other vertex among the first
+// kPath of the path claims it with seed id + 1 (atomic minimum on the claim
+// array), and pass1_finish_kernel gives it the label of its lowest claiming
+// seed.  The labels a level reads were all written by earlier launches.
 // Streamlines that run out of step budget are flagged for FP64 re-tracing.
+// (The reference labels paths as each streamline ends and stops on any
+// assigned vertex, gi_numeric_integrator_path_compressing.h:176-247, which
+// depends on the schedule of its threads.)
 constexpr int kPath = 32;
+constexpr uint32_t kNoClaim = 0xFFFFFFFFu;
+
+__device__ __forceinline__ bool pass1_seed(int x, int y, int z, int stride, int first_level) {
+    if (x % stride || y % stride || z % stride) return false;
+    return first_level || (x % (2 * stride)) || (y % (2 * stride)) || (z % (2 * stride));
+}
 
 __global__ void __launch_bounds__(kThreads)
 pass1_kernel(GradSrc<float> src, VGrid g, LabelBuffers b, int stride, int first_level, float dir, float gthr,
              float ethr, int max_iters) {
     __shared__ int32_t path_mem[kPath * kThreads];
     int32_t* path = path_mem + threadIdx.x;
+    uint32_t* const claim = b.stamp;
     const int nx = (g.X + stride - 1) / stride, ny = (g.Y + stride - 1) / stride, nz = (g.Z + stride - 1) / stride;
     const int64_t n = (int64_t)nx * ny * nz;
     uint32_t traced = 0, over = 0;
@@ -157,44 +176,20 @@ pass1_kernel(GradSrc<float> src, VGrid g, LabelBuffers b, int stride, int first_
     bool active = false;
     int np = 0;
     int64_t v0 = 0;
-    bool ended = false;
-    int64_t end_vertex = 0;
-    // The lanes of a wave trace a batch of vertices (the next vertex of each
-    // lane's grid-stride sequence; a lane whose vertex needs no tracing idles),
-    // one step per iteration.  When all have ended, every path of the batch
-    // takes the label its end vertex carries at that moment, and the next
-    // batch starts.  Streamlines of one batch thus never stop on each other's
-    // paths, which leaves fewer wrong path labels for pass 3 to correct than
-    // labelling each path as it ends.
+    // a lane whose streamline ended takes its next seed at once: no
+    // streamline waits for another of its level
     while (true) {
-        if (!__any(active)) {
-            if (ended) {
-                ended = false;
-                const int32_t lab = b.label[end_vertex];
-                for (int j = 0; j < np; j++) {
-                    const int32_t p = path[j * kThreads];
-                    if (b.dest[p] == kUnassigned) {
-                        b.label[p] = lab;
-                        b.dest[p] = kAssigned;
-                    }
-                }
-            }
-            while (!__any(active) && __any(t < n)) {
-                if (t < n) {
-                    const int x = (int)(t % nx) * stride, y = (int)((t / nx) % ny) * stride;
-                    const int z = (int)(t / ((int64_t)nx * ny)) * stride;
-                    t += t_step;
-                    v0 = (int64_t)x + (int64_t)g.X * ((int64_t)y + (int64_t)g.Y * z);
-                    const bool skip =
-                        !first_level && (x % (2 * stride)) == 0 && (y % (2 * stride)) == 0 && (z % (2 * stride)) == 0;
-                    if (!skip && b.dest[v0] == kUnassigned) {
-                        traced++;
-                        np = 0;
-                        path[(np++) * kThreads] = (int32_t)v0;
-                        tracer_start(src, g, tr, x, y, z, max_iters);
-                        active = true;
-                    }
-                }
+        while (!active && t < n) {
+            const int x = (int)(t % nx) * stride, y = (int)((t / nx) % ny) * stride;
+            const int z = (int)(t / ((int64_t)nx * ny)) * stride;
+            t += t_step;
+            v0 = (int64_t)x + (int64_t)g.X * ((int64_t)y + (int64_t)g.Y * z);
+            if (pass1_seed(x, y, z, stride, first_level) && b.dest[v0] == kUnassigned) {
+                traced++;
+                np = 0;
+                path[(np++) * kThreads] = (int32_t)v0;
+                tracer_start(src, g, tr, x, y, z, max_iters);
+                active = true;
             }
         }
         if (!__any(active)) break;
@@ -209,12 +204,19 @@ pass1_kernel(GradSrc<float> src, VGrid g, LabelBuffers b, int stride, int first_
         }
         if (moved && np < kPath) path[(np++) * kThreads] = (int32_t)next;
         if (ev == kNone) continue;
-        // the streamline ended; its path is labelled when the batch ends
         active = false;
-        ended = true;
         if (ev != kHitPreassigned) next = nearest(g, tr.cur, tr.near);
         if (np < kPath) path[(np++) * kThreads] = (int32_t)next;
-        end_vertex = next;
+        // the label reached: a certain terminal's or an earlier level's; a
+        // stop elsewhere (low gradient, step budget) at a vertex without one
+        // gives -1, the reference's label of an unassigned vertex
+        const uint8_t dn = b.dest[next];
+        b.label[v0] = (dn == kAssigned || dn == kCertainTerminal) ? b.label[next] : -1;
+        for (int j = 1; j < np; j++) {
+            const int32_t p = path[j * kThreads];
+            if (p != (int32_t)v0 && b.dest[p] == kUnassigned)
+                __hip_atomic_fetch_min(&claim[p], (uint32_t)v0 + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
         steps += (uint64_t)(max_iters - (tr.iters > 0 ? tr.iters : 0));
         if (ev == kOverMax) {
             over++;
@@ -224,6 +226,30 @@ pass1_kernel(GradSrc<float> src, VGrid g, LabelBuffers b, int stride, int first_
     if (traced) atomicAdd(&b.stats[kStatTraced1], traced);
     if (over) atomicAdd(&b.stats[kStatOver1], over);
     if (steps) atomicAdd(reinterpret_cast<unsigned long long*>(&b.stats[kStatSteps1]), (unsigned long long)steps);
+}
+
+// End of a pass-1 level: its seeds (labelled by their own streamlines) and
+// every claimed vertex become assigned; a claimed vertex takes the label of
+// its lowest claiming seed.  Claims are reset (to zero after the last level:
+// the array then serves as pass 2/3's queue stamps).
+__global__ void __launch_bounds__(kThreads)
+pass1_finish_kernel(VGrid g, LabelBuffers b, int stride, int first_level, int last) {
+    uint32_t* const claim = b.stamp;
+    const int64_t n = g.verts();
+    for (int64_t i = (int64_t)blockIdx.x * kThreads + threadIdx.x; i < n; i += (int64_t)gridDim.x * kThreads) {
+        const uint32_t c = claim[i];
+        if (b.dest[i] == kUnassigned) {
+            int x, y, z;
+            vertex_xyz(g, i, x, y, z);
+            if (pass1_seed(x, y, z, stride, first_level)) {
+                b.dest[i] = kAssigned;
+            } else if (c != kNoClaim) {
+                b.label[i] = b.label[c - 1u];
+                b.dest[i] = kAssigned;
+            }
+        }
+        claim[i] = last ? 0u : kNoClaim;
+    }
 }
 
 // Block-staged appends to a sub-list (as in the ldir rounds).
@@ -452,9 +478,17 @@ pass2_kernel(VGrid g, LabelBuffers b, WorkQueue q) {
 // Pass 3: re-integration to the original certain regions only, until the
 // queue drains.  A re-integrated vertex whose label changed queues its
 // assigned All6 neighbours that carry another label (each vertex is queued at
-// most once: after its re-integration it is no longer assigned).  The final
-// labels do not depend on the processing order, since every re-integration
-// ends on a certain terminal, whose label never changes.
+// most once: after its re-integration it is no longer assigned).  A
+// re-integration takes the label its end vertex carried when pass 3 began
+// (b.snap): a certain terminal's, which never changes, or, for a streamline
+// stopped elsewhere by the gradient threshold or the step budget, that
+// vertex's pass-1 label.  (The reference reads the label current at that
+// moment, gi_numeric_integrator_path_compressing.h:370, which depends on its
+// threads' schedule.)  So every re-integration's result depends on its vertex
+// alone; a vertex is re-integrated iff pass 2 queued it or a re-integrated
+// neighbour's new label differs from both that neighbour's and its own pass-1
+// labels, a closure that does not depend on the processing order either, and
+// the labels are the same from run to run.
 //
 // One launch of resident waves.  Each wave keeps a ring of queued vertices in
 // LDS: it refills an empty ring with a chunk of the items pass 2 queued (one
@@ -739,7 +773,7 @@ pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr
         int32_t spawn_label = 0;
         if (fin) {
             const int32_t init = b.label[v];
-            const int32_t nl = b.label[end];
+            const int32_t nl = b.snap[end];
             b.label[v] = nl;
             b.dest[v] = kCertainNonterminal;
             if (nl != init) {
@@ -1245,7 +1279,7 @@ __global__ void dest_init_kernel(VGrid g, LabelBuffers b) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
         b.dest[i] = b.label[i] == -1 ? kUnassigned : kCertainTerminal;
         b.redo[i] = 0;
-        b.stamp[i] = 0;
+        b.stamp[i] = kNoClaim;  // pass-1 claims
     }
 }
 
@@ -1354,6 +1388,12 @@ hipError_t launch_pass1(const VGrid& g, const LabelBuffers& b, int stride, bool 
     const int64_t n = (int64_t)((g.X + stride - 1) / stride) * ((g.Y + stride - 1) / stride) * ((g.Z + stride - 1) / stride);
     hipLaunchKernelGGL(pass1_kernel, dim3(grid_for(n, 8192)), dim3(kThreads), 0, s, src, g, b, stride,
                        first_level ? 1 : 0, (float)sign, gthr, ethr, max_iters);
+    return hipGetLastError();
+}
+hipError_t launch_pass1_finish(const VGrid& g, const LabelBuffers& b, int stride, bool first_level, bool last,
+                               hipStream_t s) {
+    hipLaunchKernelGGL(pass1_finish_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, g, b, stride,
+                       first_level ? 1 : 0, last ? 1 : 0);
     return hipGetLastError();
 }
 hipError_t launch_pass2(const VGrid& g, const LabelBuffers& b, const WorkQueue& q, hipStream_t s) {

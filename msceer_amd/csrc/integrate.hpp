@@ -32,6 +32,7 @@ struct LabelBuffers {
     uint8_t* seedmask;   // 1 at region seeds (their gradient is zero)
     uint8_t* redo;       // 1: re-trace in FP64 (pass 1 ran out of budget)
     uint32_t* stamp;     // work-list dedupe stamps
+    const int32_t* snap; // labels when pass 3 begins (read for re-integration results)
     int32_t* rep;        // component representatives (cleanup)
     int32_t* rep2;
     uint32_t* stats;     // device counters (see IntegStat)
@@ -59,12 +60,17 @@ hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s);
 // Seeds: label[seed_vert[i]] = seed_label[i]; zero gradient; seedmask = 1.
 hipError_t launch_seed_scatter(const int32_t* seed_vert, const int32_t* seed_label, int n, const LabelBuffers& b,
                                hipStream_t s);
-// dest = UNASSIGNED where label == -1 else CERTAIN_TERMINAL; redo = 0; stamp = 0.
+// dest = UNASSIGNED where label == -1 else CERTAIN_TERMINAL; redo = 0; stamp = no claim.
 hipError_t launch_dest_init(const VGrid& g, const LabelBuffers& b, hipStream_t s);
 // Pass 1 at one level of the coarse-to-fine order (sign -1: flow downhill to
 // minima, +1: uphill to maxima; the same for pass 3).
 hipError_t launch_pass1(const VGrid& g, const LabelBuffers& b, int stride, bool first_level, float gthr, float ethr,
                         int max_iters, int sign, hipStream_t s);
+// End of a pass-1 level: its seeds and claimed path vertices become assigned
+// (claims in b.stamp, all kNoClaim on entry; reset to zero after the last
+// level for passes 2 and 3).
+hipError_t launch_pass1_finish(const VGrid& g, const LabelBuffers& b, int stride, bool first_level, bool last,
+                               hipStream_t s);
 // Pass 2: every ASSIGNED vertex with a differently labelled All6 neighbour
 // (or flagged for re-tracing) goes to `out`.
 // Re-integration work queue: `items` (capacity entries, 0xFFFFFFFF until

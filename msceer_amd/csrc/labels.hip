@@ -138,22 +138,24 @@ int run_integration(msceer_ctx* ctx, const VGrid& g, const LabelBuffers& b, cons
     }
     // coarse-to-fine strides (the reference visits each thread's range in a
     // bisection order): 2, then 1.  Each stride's launch lasts as long as its
-    // longest streamline; at 256^3 strides 8..1 took 57 ms for the whole call
-    // and 2, 1 take 47.6 ms at the same FP64-oracle mismatch (0.050 %), while
-    // stride 1 alone raised it to 0.053 %
-    constexpr int kTopStride = 2;
+    // longest streamline.
     // Pass 1 is the FP32 first guess: a streamline that exhausts this budget is
     // flagged and re-traced in FP64 by pass 3 with the full max_iters, as the
     // ones exhausting max_iters are.  The few that run that long set each
     // stride's length (256^3: 47.6 -> 43.2 ms for the whole call, FP64-oracle
     // mismatch 0.050 % either way; 537 of 5.0 M streamlines hit it).
     constexpr int kPass1Budget = 1024;
+    constexpr int kPass1TopStride = 2;
     const int max_dim = std::max(g.X, std::max(g.Y, g.Z));
     int top = 1;
-    while (top * 2 <= kTopStride && top * 2 < max_dim) top *= 2;
-    for (int stride = top; stride >= 1; stride /= 2)
+    while (top * 2 <= kPass1TopStride && top * 2 < max_dim) top *= 2;
+    CHECK_HIP(hipEventRecord(ctx->ev[12], s));
+    for (int stride = top; stride >= 1; stride /= 2) {
         CHECK_HIP(launch_pass1(g, b, stride, stride == top, grad_thr, err_thr, std::min(max_iters, kPass1Budget),
                                sign, s));
+        CHECK_HIP(launch_pass1_finish(g, b, stride, stride == top, stride == 1, s));
+    }
+    CHECK_HIP(hipEventRecord(ctx->ev[13], s));
     // pass 2 fills the re-integration queue, pass 3 drains it in one launch
     // each vertex is queued at most once as a re-integration and at most once
     // to be traced ahead, so up to 2V items: the queue takes the whole
@@ -165,11 +167,17 @@ int run_integration(msceer_ctx* ctx, const VGrid& g, const LabelBuffers& b, cons
     CHECK_HIP(launch_zero(ctl + kCtlCursor, 5, s));
     CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
     CHECK_HIP(launch_pass2(g, b, q, s));
+    // the labels pass 3 reads for its results
+    CHECK_HIP(hipMemcpyAsync(const_cast<int32_t*>(b.snap), b.label, sizeof(int32_t) * V, hipMemcpyDeviceToDevice, s));
     CHECK_HIP(launch_grad_field64(g, b, s));
     CHECK_HIP(launch_fill_i32(b.label2, V, -1, s));  // no vertex traced ahead yet
+    CHECK_HIP(hipEventRecord(ctx->ev[14], s));
     CHECK_HIP(launch_pass3(g, b, q, grad_thr, err_thr, max_iters, sign, s));
+    CHECK_HIP(hipEventRecord(ctx->ev[15], s));
     int rc = read_ctl(ctx, 0, kCtlWords);
     if (rc) return rc;
+    st.ms_pass1 = elapsed(ctx->ev[12], ctx->ev[13]);
+    st.ms_pass3 = elapsed(ctx->ev[14], ctx->ev[15]);
     if (ctx->h_ctl[kCtlOverflow] & 2u) return MSCEER_EHIP;  // the queue stalled (never expected)
     if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;
     st.pass3_rounds = 1;
@@ -250,6 +258,7 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
     b.stamp = ctx->stamp.as<uint32_t>();
     b.rep = ctx->rep.as<int32_t>();
     b.rep2 = ctx->rep2.as<int32_t>();
+    b.snap = b.rep;  // free until the cleanup
     b.stats = ctl + kCtlIntStats;
     const uint8_t* d_mask = integrate_mask;
     if (integrate_mask && !dev) {
@@ -646,6 +655,7 @@ extern "C" int msceer_integrate_labels(msceer_ctx* ctx, const float* values, int
     CHECK_HIP(ctx->label.ensure(sizeof(int32_t) * V));
     CHECK_HIP(ctx->label2.ensure(sizeof(int32_t) * V));
     CHECK_HIP(ctx->stamp.ensure(sizeof(uint32_t) * V));
+    CHECK_HIP(ctx->rep.ensure(sizeof(int32_t) * V));
     CHECK_HIP(ctx->dest.ensure(V));
     CHECK_HIP(ctx->seedmask.ensure(V));
     CHECK_HIP(ctx->redo.ensure(V));
@@ -663,6 +673,7 @@ extern "C" int msceer_integrate_labels(msceer_ctx* ctx, const float* values, int
     b.seedmask = ctx->seedmask.as<uint8_t>();
     b.redo = ctx->redo.as<uint8_t>();
     b.stamp = ctx->stamp.as<uint32_t>();
+    b.snap = ctx->rep.as<int32_t>();
     b.stats = ctl + kCtlIntStats;
     CHECK_HIP(hipEventRecord(ev[0], s));
     CHECK_HIP(hipMemcpyAsync(b.label, terminals, sizeof(int32_t) * V,

@@ -10,7 +10,12 @@
 //          operation (FP32 step size, full-step displacement, magnitudes and
 //          error; FP64 positions, gradients and half steps);
 //   mode 1 "fp64": the same algorithm with every quantity in FP64 (the
-//          FP64 oracle the GPU labels are measured against).
+//          FP64 oracle the GPU labels are measured against);
+//   mode 2 "exact fp64": every vertex's own destination in FP64 -- each
+//          non-certain vertex integrated alone until it reaches a certain
+//          terminal, as pass 3 re-integrates (no path compression, which is
+//          the reference's approximation of exactly this), so the labels do
+//          not depend on any visiting order.
 // Run single-threaded it reproduces the reference run with one OpenMP
 // thread, whose first integration pass depends on the visiting order.
 //
@@ -270,6 +275,35 @@ idx_t trace(const Grid& g, const std::vector<V3d>& grad, const Checker& chk, flo
             return next;
         }
     }
+}
+
+// Every non-certain vertex integrated alone, in FP64, to a certain terminal
+// (the stop rule of pass 3, TerminateNearOriginalCertain,
+// gi_advection_checker.h:245-262); independent vertices, so OpenMP threads
+// share them in any order.
+void integrate_exact(const Grid& g, std::vector<V3d> grad, const std::vector<std::vector<idx_t>>& seeds, float ethr,
+                     float gthr, int max_iters, int* label, long long* stats, int mul) {
+    const idx_t n = g.X * g.Y * g.Z;
+    std::vector<Dest> dest((size_t)n);
+    for (idx_t i = 0; i < n; i++) dest[(size_t)i] = label[i] == -1 ? UNASSIGNED : CERTAIN_TERMINAL;
+    for (const auto& s : seeds)
+        for (idx_t id : s) grad[(size_t)id] = V3d{{0, 0, 0}};
+    const Checker chk{dest.data(), false, &g};
+    std::vector<int> out(label, label + n);
+    long long traced = 0, over = 0;
+#pragma omp parallel for schedule(dynamic, 256) reduction(+ : traced, over)
+    for (idx_t v = 0; v < n; v++) {
+        if (dest[(size_t)v] != UNASSIGNED) continue;
+        Event ev;
+        const idx_t end = trace<true>(g, grad, chk, gthr, ethr, max_iters, v, nullptr, &ev, mul);
+        traced++;
+        if (ev == OVER_MAX_ITERATIONS) over++;
+        // a terminal's label, or the label of a vertex the streamline stopped
+        // at otherwise (low gradient, step budget): a certain one or -1
+        out[(size_t)v] = label[end];
+    }
+    std::copy(out.begin(), out.end(), label);
+    if (stats) { stats[0] = traced; stats[1] = over; stats[2] = traced; }
 }
 
 // NumericIntegratorPathCompressingToTerminal::BeginIntegration with one thread
@@ -657,7 +691,8 @@ int oracle_integrate_signed(const float* f, long long X, long long Y, long long 
     std::vector<std::vector<idx_t>> sd(1);
     for (idx_t i = 0; i < X * Y * Z; i++) if (seeds && seeds[i]) sd[0].push_back(i);
     if (mode == 0) integrate<false>(g, f, gr, sd, ethr, gthr, max_iters, labels, stats, sign);
-    else integrate<true>(g, f, gr, sd, ethr, gthr, max_iters, labels, stats, sign);
+    else if (mode == 1) integrate<true>(g, f, gr, sd, ethr, gthr, max_iters, labels, stats, sign);
+    else integrate_exact(g, gr, sd, ethr, gthr, max_iters, labels, stats, sign);
     return 0;
 }
 
@@ -752,7 +787,7 @@ long long oracle_conforming_vertices(const float* f, long long X, long long Y, l
 }
 
 // The whole accurate ascending pipeline; returns the .grad and the cleaned
-// labels.  mode: 0 reference arithmetic, 1 FP64.
+// labels.  mode: 0 reference arithmetic, 1 FP64, 2 exact FP64 destinations.
 int oracle_ondemand_asc(const float* f, long long X, long long Y, long long Z, float ethr, float gthr, int max_iters,
                         float presimp, int mode, unsigned char* grad, int* labels, signed char* restriction_out,
                         long long* stats) {

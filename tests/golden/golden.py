@@ -64,10 +64,12 @@ C3_SHAPE = (256, 256, 256)
 C3_PARAMS = (1e-3, 1e-6, 4096, 0.0)  # err_thr, grad_thr, max_iters, presimp
 
 
-def load_c3_fp64():
-    """(labels (V,) int32 of the FP64 oracle on the 256^3 workload, input SHA-256)."""
+def load_c3_fp64(exact=False):
+    """(labels (V,) int32 of the FP64 oracle on the 256^3 workload, input SHA-256);
+    exact: every vertex's own FP64 destination (oracle mode 2) instead of the
+    path-compressing reference pipeline in FP64 (mode 1)."""
     import lzma
-    d = np.load(os.path.join(HERE, "c3", "labels_fp64_256.npz"))
+    d = np.load(os.path.join(HERE, "c3", "labels_exact_256.npz" if exact else "labels_fp64_256.npz"))
     assert tuple(d["shape"]) == C3_SHAPE
     idx = np.frombuffer(lzma.decompress(d["idx_lzma"].tobytes()), np.uint16)
     return d["table"][idx].astype(np.int32), str(d["input_sha256"])

@@ -1,11 +1,12 @@
 """GPU accurate labelling (msceer_ondemand_asc) against the reference goldens
 and the CPU oracle.
 
-Deterministic stages are bit-exact: certain regions, and the conforming .grad
-given the labels.  The integrated labels are compared as a mismatch rate with
-the FP64 oracle (pass 1 of the reference depends on its visiting order, and
-the GPU traces in FP32 first): tolerance 1% of vertices on these small
-volumes (the larger-volume target is reported by bench.py).
+Stages that follow the reference operation for operation are bit-exact:
+certain regions, and the conforming .grad given the labels.  The integrated
+labels are compared as a mismatch rate with the FP64 oracle, at most 0.1 % of
+the vertices (pass 1 of the reference depends on its threads' visiting
+order, the GPU's levels are synchronous, and the GPU traces in FP32 first).
+The GPU labels themselves are the same from run to run.
 """
 import sys
 
@@ -50,7 +51,7 @@ def test_labels_and_conforming_grad(ctx, name):
     _, lab64, _, _ = oracle_lib.ondemand_asc(f, err, gthr, iters, presimp, mode=1)
     mism = float((lab != lab64).mean())
     _say(f"  mismatch vs FP64 oracle {mism:.5f}, vs reference {float((lab != ref['cleaned']).mean()):.5f}, {st}")
-    assert mism < 0.01
+    assert mism <= 1e-3
     # the .grad is a pure function of the labels: bit-exact
     r = oracle_lib.min_boundary(lab, f.shape)
     expect = oracle_lib.flags(oracle_lib.ldir(oracle_lib.lstar_pairs(f, r), f.shape), f.shape)
@@ -159,7 +160,7 @@ def test_integrate_labels_matches_oracle(ctx, name, sign):
     ref, _ = oracle_lib.integrate_signed(f, term, seeds, err, gthr, iters, sign=sign, mode=1)
     mism = float((lab.ravel() != ref).mean())
     _say(f"  mismatch vs FP64 oracle {mism:.5f}, {st}")
-    assert mism < 0.01
+    assert mism <= 1e-3
     t = term.ravel()
     assert (lab.ravel()[t != -1] == t[t != -1]).all()  # terminals keep their labels
     assert float((lab == -1).mean()) < 0.01
@@ -172,8 +173,26 @@ def test_integrate_labels_uphill_is_downhill_of_the_negated_field(ctx):
     t, sd = term.reshape(f.shape), seeds.reshape(f.shape).astype(np.uint8)
     up, _ = labels.integrate_labels(f, t, sd, err, gthr, iters, 1, ctx=ctx)
     down, _ = labels.integrate_labels(neg, t, sd, err, gthr, iters, -1, ctx=ctx)
-    # identical arithmetic; only pass 1's visiting order may differ between runs
-    assert float((up != down).mean()) < 0.005
+    # every product and sum only changes sign, and the labels do not depend
+    # on the order streamlines run in: identical
+    assert (up == down).all(), int((up != down).sum())
+
+
+@pytest.mark.parametrize("name", ["gauss_44x36x40", "smooth_presimp_40"])
+def test_labels_are_the_same_from_run_to_run(ctx, name):
+    # pass 1's levels are synchronous and pass 3 reads labels fixed before it
+    # starts: every stage gives the same result on every call, and the
+    # integrator's work counts are the same too
+    f, (err, gthr, iters, presimp), _ = load_ondemand(name)
+    keys = ("traced_fp32", "reintegrated_fp64", "over_budget_fp32", "over_budget_fp64", "steps_fp32")
+    for stop in ("integrated", None):
+        runs = [labels.ondemand_asc(f, err, gthr, iters, presimp, ctx=ctx, stop=stop) for _ in range(3)]
+        _say(f"reproducible {name} stop={stop}: {[{k: r[2][k] for k in keys} for r in runs]}")
+        for g, lab, st in runs[1:]:
+            assert (lab == runs[0][1]).all(), int((lab != runs[0][1]).sum())
+            if stop is None:
+                assert (g == runs[0][0]).all()
+            assert {k: st[k] for k in keys} == {k: runs[0][2][k] for k in keys}
 
 
 def test_integrate_labels_rejects_bad_arguments(ctx):

@@ -112,3 +112,41 @@ def test_oracle_max_boundary_closed_form(field):
     assert (oracle_lib.max_boundary(f, lab, ones) == ones + expect).all()
     assert (oracle_lib.max_boundary(f, np.zeros_like(lab), ones) == ones).all()
     assert (oracle_lib.max_boundary(f, np.full_like(lab, -1), ones) == ones).all()
+
+
+def test_fp64_mode_equals_reference_arithmetic_where_every_quantity_is_exact():
+    # f = x: the central-difference gradient is (1, 0, 0) inside, the step is
+    # capped at 0.5 / |g| = 0.5 voxel from the start, positions stay on
+    # multiples of 0.25 and every interpolation weight is in {0, 1/4, 1/2, 3/4}:
+    # no quantity rounds in FP32 or FP64, so both arithmetic modes and the
+    # exact-destination mode give the same labels bit for bit
+    Z, Y, X = 7, 9, 23
+    f = np.broadcast_to(np.arange(X, dtype=np.float32)[None, None, :], (Z, Y, X)).copy()
+    labs = [oracle_lib.ondemand_asc(f, 1e-3, 1e-6, 4096, 0.0, mode=m)[1] for m in (0, 1, 2)]
+    assert (labs[0] == labs[1]).all() and (labs[1] == labs[2]).all()
+    assert (labs[0] >= 0).all()
+
+
+@pytest.mark.parametrize("name", ondemand_case_names())
+def test_exact_mode_keeps_certain_labels_and_stays_close_to_the_reference(name):
+    # mode 2 integrates every non-certain vertex alone to a certain terminal
+    # (the reference's pass-3 rule); certain regions keep their labels, and the
+    # result differs from the reference pipeline only where path compression
+    # labelled a vertex differently
+    f, (err, gthr, iters, presimp), ref = load_ondemand(name)
+    cert, seeds, _ = oracle_lib.certain_regions(f, oracle_lib.lstar_pairs(f), presimp)
+    exact, _ = oracle_lib.integrate(f, cert, seeds, err, gthr, iters, mode=2)
+    assert (exact[cert != -1] == cert[cert != -1]).all()
+    _, lab2, _, _ = oracle_lib.ondemand_asc(f, err, gthr, iters, presimp, mode=2)
+    mism = float((lab2 != ref["cleaned"]).mean())
+    assert mism < 0.01, mism
+
+
+def test_exact_mode_differs_from_path_compression_somewhere():
+    # a planted no-op would make mode 2 equal mode 1; on the golden whose
+    # streamlines run out of budget, path compression labels some vertices
+    # otherwise
+    f, (err, gthr, iters, presimp), _ = load_ondemand("budget_24x30x20")
+    _, l1, _, _ = oracle_lib.ondemand_asc(f, err, gthr, iters, presimp, mode=1)
+    _, l2, _, _ = oracle_lib.ondemand_asc(f, err, gthr, iters, presimp, mode=2)
+    assert 0 < int((l1 != l2).sum())

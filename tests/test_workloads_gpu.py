@@ -1,0 +1,84 @@
+"""The accurate-labelling workloads at their own sizes on the GPU.
+
+* 256^3 labelling workload (200 Gaussians sigma 4-16 + 0.05 sin sin sin, seed
+  303): labels within 0.1 % of both committed FP64-oracle labellings
+  (tests/golden/c3: the reference pipeline in FP64, and every vertex's exact
+  FP64 destination), the same on a second call.
+* The learned mask's evaluation family (N(0,1) noise smoothed with sigma 2.5,
+  seed 5999) on a 256x192x128 volume: plain and learned labels each within
+  0.1 % of every vertex's exact FP64 destination (oracle mode 2) on the same
+  volume.
+"""
+import ctypes
+import sys
+
+import numpy as np
+import pytest
+
+import oracle_lib
+from golden import C3_PARAMS, load_c3_fp64
+from msceer_amd import hipmem, labels, learned, native, synth
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = native.Context(0)
+    yield c
+    c.close()
+
+
+def test_c3_labels_against_the_fp64_fixture(ctx):
+    _say("c3: generating the 256^3 labelling field")
+    f = synth.labels_field(303, 256)
+    ref, sha = load_c3_fp64()
+    exact, sha2 = load_c3_fp64(exact=True)
+    assert synth.sha256(f) == sha == sha2, "labelling field generator drifted from the fixtures"
+    err, gthr, iters, presimp = C3_PARAMS
+    Z, Y, X = f.shape
+    V = X * Y * Z
+    # values resident in device memory, as in the benchmark
+    dv = hipmem.DeviceBuffer(f.nbytes)
+    dv.upload(f)
+    dl = hipmem.DeviceBuffer(4 * V)
+    st = native.LabelStats()
+    runs = []
+    for k in range(2):
+        _say(f"c3: labelling call {k}")
+        native.check(native.lib().msceer_ondemand_asc(ctx.handle, ctypes.c_void_p(dv.addr), X, Y, Z, err, gthr, iters,
+                                                      presimp, None, None, ctypes.c_void_p(dl.addr),
+                                                      ctypes.byref(st), native.DEVICE_PTRS), "msceer_ondemand_asc")
+        hipmem.synchronize()
+        runs.append((dl.download(np.int32, V), st.as_dict()))
+    lab, s0 = runs[0]
+    mism = float((lab != ref).mean())
+    mism_exact = float((lab != exact).mean())
+    _say(f"  mismatch vs FP64 oracle {mism:.6f}, vs exact FP64 destinations {mism_exact:.6f}; {s0}")
+    assert mism <= 1e-3
+    assert mism_exact <= 1e-3
+    assert (runs[1][0] == lab).all(), int((runs[1][0] != lab).sum())
+
+
+def test_learned_family_volume_against_fp64_oracle(ctx):
+    f = synth.smoothed_noise((128, 192, 256), 5999, 2.5)
+    _say(f"learned family volume {f.shape}")
+    _, plain, st = labels.ondemand_asc(f, want_grad=False, ctx=ctx)
+    net = learned.SeparatrixNet(ctx)
+    try:
+        net.load()
+        _, lab, _, frac = learned.learned_labels(f, net, ctx, want_grad=False)
+    finally:
+        net.close()
+    _, lab64, _, _ = oracle_lib.ondemand_asc(f, 1e-3, 1e-6, 4096, 0.0, mode=2)
+    m_plain = float((plain.ravel() != lab64).mean())
+    m_learned = float((lab.ravel() != lab64).mean())
+    _say(f"  mismatch vs exact FP64 destinations: plain {m_plain:.6f}, learned {m_learned:.6f} "
+         f"(mask fraction {frac:.3f}); {st}")
+    assert m_plain <= 1e-3
+    assert m_learned <= 1e-3
