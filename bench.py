@@ -98,7 +98,8 @@ def bench_labels(args):
     log("generating 256^3 labelling field (200 Gaussians + sine, seed 303)")
     f = synth.labels_field(303, 256)
     ref, sha = load_c3_fp64()
-    assert synth.sha256(f) == sha, "labelling field generator drifted from the fixture"
+    exact, sha2 = load_c3_fp64(exact=True)
+    assert synth.sha256(f) == sha == sha2, "labelling field generator drifted from the fixtures"
     Z, Y, X = f.shape
     V = X * Y * Z
     cells = (2 * X - 1) * (2 * Y - 1) * (2 * Z - 1)
@@ -130,10 +131,24 @@ def bench_labels(args):
     lab = dl.download(np.int32, V)
     mism = float((lab != ref).mean())
     last = stages[-1]
+    stage = {k[3:]: round(float(np.mean([d[k] for d in stages])), 3)
+             for k in ("ms_lstar", "ms_regions", "ms_integrate", "ms_pass1", "ms_pass3", "ms_cleanup", "ms_grad",
+                       "ms_total")}
+    # algorithmic bytes of the labelling: 4 B read + 4 B label written per vertex
+    integ_gbs = 8.0 * V / (stage["integrate"] * 1e-3) / 1e9
     plain = {"value": round(V / (ms * 1e-3) / 1e6, 2), "unit": "Mvert/s", "ms_per_step": round(ms, 3),
              "steps": args.label_steps, "mismatch_vs_fp64_oracle": mism,
-             "stage_ms": {k[3:]: round(float(np.mean([d[k] for d in stages])), 3)
-                          for k in ("ms_lstar", "ms_regions", "ms_integrate", "ms_cleanup", "ms_grad", "ms_total")},
+             "mismatch_vs_exact_fp64_oracle": float((lab != exact).mean()),
+             "oracle_fp64_vs_exact": float((ref != exact).mean()),
+             "roofline": {"bound": "hbm", "achieved": round(integ_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                          "frac": round(integ_gbs / HBM_PEAK_GBS, 6),
+                          "kernel": "integration (pass1_kernel levels + pass2 + pass3_kernel<double>)",
+                          "algorithmic_bytes": 8.0 * V, "kernel_ms": stage["integrate"],
+                          "pass3_ms": stage["pass3"],
+                          "certain_rounds_share_of_total": round(stage["regions"] / stage["total"], 4),
+                          "note": "latency-bound: dependent streamline steps and certain-region levels, "
+                                  "not bytes"},
+             "stage_ms": stage,
              "counts": {k: int(last[k]) for k in ("minima", "regions", "traced_fp32", "over_budget_fp32",
                                                   "reintegrated_fp64", "over_budget_fp64", "certain_sweeps",
                                                   "pass3_rounds", "steps_fp32", "steps_fp64", "ahead_fp64")}}
@@ -392,6 +407,8 @@ def main():
                          "hardware queues; 0 keeps the environment's value)")
     ap.add_argument("--learned-large", action="store_true",
                     help="only the learned-mask comparison on the 1024x768x512 volume")
+    ap.add_argument("--no-learned-large", action="store_true",
+                    help="skip the learned-mask comparison on the 1024x768x512 volume")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run a plain 1 GiB read + 1 GiB write streaming kernel (torch), the "
                          "reference point for rocprofv3 FETCH_SIZE / WRITE_SIZE")
@@ -493,6 +510,8 @@ def main():
         out["labels"] = bench_labels(args)
     if not args.no_batch:
         out["batch"] = bench_batch(args)
+    if not args.no_learned_large:
+        out["learned_large"] = bench_learned_large(args)
     print(json.dumps(out), flush=True)
 
 

@@ -75,6 +75,8 @@ typedef struct msceer_label_stats {
     int64_t steps_fp32;         /* advection steps (accepted and rejected) of pass 1 */
     int64_t steps_fp64;         /* advection steps of pass 3 (streamlines traced ahead included) */
     int64_t ahead_fp64;         /* pass-3 streamlines traced ahead of a chain of long re-integrations */
+    float ms_pass1;             /* integration pass 1 (FP32, all levels) */
+    float ms_pass3;             /* integration pass 3 (FP64 re-integration) */
 } msceer_label_stats;
 
 int msceer_version(void);

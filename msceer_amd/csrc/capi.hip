@@ -52,19 +52,21 @@ MList front_list(msceer_ctx* ctx, const ListCaps& c, int k) {
     return MList{buf, ctl_dev(ctx) + kCtlRound + (k % 3) * kCountWords, c.front_sub, c.front_spill};
 }
 
-// One ldir pass.  Rounds are enqueued in groups without host synchronisation
-// (each reads its counts on the device); after each group one read-back shows
-// whether the frontier emptied.  A sweep rebuilds the frontier after any
-// overflow of the start list or of a frontier.
-int ldir_pass(msceer_ctx* ctx, const Dims& d, uint8_t* grad, uint32_t item_dim, uint32_t value,
-              const MList& start, bool start_valid, const ListCaps& caps, int* rounds) {
+// One ldir pass (1: from the critical edges, 2: from the critical quads).
+// Rounds are enqueued in groups without host synchronisation (each reads its
+// counts on the device); after each group one read-back shows whether the
+// frontier emptied.  A sweep rebuilds the frontier after any overflow of the
+// start list or of a frontier.
+int ldir_pass(msceer_ctx* ctx, const Dims& d, uint8_t* grad, const LdirScratch& ls, int pass, const MList& start,
+              bool start_valid, const ListCaps& caps, int* rounds) {
     uint32_t* ctl = ctl_dev(ctx);
-    uint32_t* tail = ctl + kCtlTail;
     uint32_t* overflow = ctl + kCtlOverflow;
+    void* scratch = ctx->ldir.ptr;
     auto front = [&](int k) { return front_list(ctx, caps, k); };
     auto host_total = [&](int k) { return list_total(ctx->h_ctl + kCtlRound + (k % 3) * kCountWords, front(k)); };
     constexpr int kGroup = 8;
     MList in = start;
+    bool is_start = true;
     int k = 0;
     bool need_sweep = !start_valid;
     CHECK_HIP(launch_zero(front(0).counts, kCountWords, ctx->stream));
@@ -73,7 +75,7 @@ int ldir_pass(msceer_ctx* ctx, const Dims& d, uint8_t* grad, uint32_t item_dim, 
         if (need_sweep) {
             CHECK_HIP(launch_zero(front(k).counts, kCountWords, ctx->stream));
             CHECK_HIP(launch_zero(overflow, 1, ctx->stream));
-            CHECK_HIP(launch_ldir_sweep(d, grad, tail, item_dim, value, front(k), overflow, ctx->stream));
+            CHECK_HIP(launch_ldir_sweep(d, grad, scratch, ls, pass, front(k), overflow, ctx->stream));
             int rc = read_ctl(ctx, 0, kCtlWords);
             if (rc) return rc;
             need_sweep = ctx->h_ctl[kCtlOverflow] != 0;
@@ -82,14 +84,17 @@ int ldir_pass(msceer_ctx* ctx, const Dims& d, uint8_t* grad, uint32_t item_dim, 
             if (n == 0 && !need_sweep) return MSCEER_OK;
             CHECK_HIP(launch_zero(overflow, 1, ctx->stream));
             in = front(k);
+            is_start = false;
             k++;
             CHECK_HIP(launch_zero(front(k).counts, kCountWords, ctx->stream));
         }
         for (int j = 0; j < kGroup; j++) {
             const MList out = front(k);
-            CHECK_HIP(launch_ldir_round(d, grad, tail, in, out, front(k + 1).counts, overflow, value, ctx->stream));
+            CHECK_HIP(launch_ldir_round(d, scratch, ls, pass, is_start, in, out, front(k + 1).counts, overflow,
+                                        ctx->stream));
             (*rounds)++;
             in = out;
+            is_start = false;
             k++;
         }
         int rc = read_ctl(ctx, 0, kCtlWords);
@@ -106,6 +111,14 @@ int ldir_pass(msceer_ctx* ctx, const Dims& d, uint8_t* grad, uint32_t item_dim, 
 // Critical lists: a quarter of the vertices per sub-list set, spill for as
 // many again as there are vertices; frontiers: the vertex count in sub-lists
 // and as much again in spill.  Entries are cell ids (< 2^32).
+int tuned_shape(msceer_ctx* ctx, const Dims& d, int fallback) {
+    std::lock_guard<std::mutex> lk(ctx->tuning->mu);
+    if (ctx->tuning->mode == 0) return 0;
+    if (ctx->tuning->mode > 1) return ctx->tuning->mode - 2;
+    auto it = ctx->tuning->brick_by_x.find(d.X);
+    return it == ctx->tuning->brick_by_x.end() ? fallback : it->second;
+}
+
 int lstar_shape(msceer_ctx* ctx, const Dims& d, const LstarBuffers& b, int* shape) {
     Tuning& t = *ctx->tuning;
     {
@@ -146,27 +159,88 @@ int lstar_shape(msceer_ctx* ctx, const Dims& d, const LstarBuffers& b, int* shap
     return MSCEER_OK;
 }
 
-int ensure_lists(msceer_ctx* ctx, const Dims& d, ListCaps* caps) {
+ListCaps list_caps(const Dims& d) {
+    ListCaps c{};
     const uint64_t list_cap = std::min<uint64_t>(kMaxListCap, std::max<uint64_t>(kMinListCap, d.verts / 4));
     const uint64_t front_cap = std::min<uint64_t>(kMaxFrontCap, std::max<uint64_t>(kMinListCap, d.verts));
-    caps->list_sub = (uint32_t)((list_cap + kSubs - 1) / kSubs);
-    caps->front_sub = (uint32_t)((front_cap + kSubs - 1) / kSubs);
-    caps->list_spill = (uint32_t)std::min<uint64_t>(0xFFFFFFFFull, std::max<uint64_t>(kMinListCap, d.verts));
-    caps->front_spill = caps->list_spill;
-    CHECK_HIP(ctx->lists.ensure(sizeof(uint32_t) * 2 * ((size_t)caps->list_sub * kSubs + caps->list_spill)));
-    CHECK_HIP(ctx->front0.ensure(sizeof(uint32_t) * ((size_t)caps->front_sub * kSubs + caps->front_spill)));
-    CHECK_HIP(ctx->front1.ensure(sizeof(uint32_t) * ((size_t)caps->front_sub * kSubs + caps->front_spill)));
+    c.list_sub = (uint32_t)((list_cap + kSubs - 1) / kSubs);
+    c.front_sub = (uint32_t)((front_cap + kSubs - 1) / kSubs);
+    c.list_spill = (uint32_t)std::min<uint64_t>(0xFFFFFFFFull, std::max<uint64_t>(kMinListCap, d.verts));
+    c.front_spill = c.list_spill;
+    return c;
+}
+
+size_t list_bytes(const ListCaps& c) { return sizeof(uint32_t) * 2 * ((size_t)c.list_sub * kSubs + c.list_spill); }
+size_t front_bytes(const ListCaps& c) { return sizeof(uint32_t) * ((size_t)c.front_sub * kSubs + c.front_spill); }
+
+int ensure_lists(msceer_ctx* ctx, const Dims& d, ListCaps* caps) {
+    *caps = list_caps(d);
+    CHECK_HIP(ctx->lists.ensure(list_bytes(*caps)));
+    CHECK_HIP(ctx->front0.ensure(front_bytes(*caps)));
+    CHECK_HIP(ctx->front1.ensure(front_bytes(*caps)));
     return MSCEER_OK;
 }
 
 int run_ldir(msceer_ctx* ctx, const Dims& d, uint8_t* grad, const ListCaps& caps, bool lists_ok, int* rounds) {
-    uint32_t* tail = ctl_dev(ctx) + kCtlTail;
-    CHECK_HIP(launch_ldir_tail(d, grad, tail, false, ctx->stream));
-    int rc = ldir_pass(ctx, d, grad, 1, 2, crit_list(ctx, caps, 0), lists_ok, caps, rounds);
+    LdirScratch ls{};
+    CHECK_HIP(ctx->ldir.ensure(ldir_scratch_bytes(d, &ls)));
+    CHECK_HIP(launch_ldir_records(d, grad, ctx->ldir.ptr, ls, ctx->stream));
+    int rc = ldir_pass(ctx, d, grad, ls, 1, crit_list(ctx, caps, 0), lists_ok, caps, rounds);
     if (rc) return rc;
-    rc = ldir_pass(ctx, d, grad, 2, 1, crit_list(ctx, caps, 1), lists_ok, caps, rounds);
+    rc = ldir_pass(ctx, d, grad, ls, 2, crit_list(ctx, caps, 1), lists_ok, caps, rounds);
     if (rc) return rc;
-    CHECK_HIP(launch_ldir_tail(d, grad, tail, true, ctx->stream));
+    CHECK_HIP(launch_ldir_finish(d, grad, ctx->ldir.ptr, ls, ctx->stream));
+    return MSCEER_OK;
+}
+
+int lstar_grad_device(msceer_ctx* ctx, const Dims& d, const float* d_values, uint8_t* d_grad, uint8_t* host_out,
+                      uint32_t flags) {
+    const bool timing = flags & MSCEER_TIMING;
+    hipStream_t s = ctx->stream;
+    // critical-cell lists and frontiers: sized to the volume, capped; any
+    // overflow falls back to a sweep
+    ListCaps caps{};
+    int rc = ensure_lists(ctx, d, &caps);
+    if (rc) return rc;
+    uint32_t* ctl = ctl_dev(ctx);
+    const MList list1 = crit_list(ctx, caps, 0), list2 = crit_list(ctx, caps, 1);
+
+    LstarBuffers b{d_values, d_grad, nullptr, &list1, &list2, ctl + kCtlStats, ctl + kCtlOverflow};
+    int shape = 0;
+    if (flags & kLstarNoTuning) {
+        shape = tuned_shape(ctx, d, 0);
+    } else {
+        rc = lstar_shape(ctx, d, b, &shape);
+        if (rc) return rc;
+    }
+    if (timing) CHECK_HIP(hipEventRecord(ctx->ev[0], s));
+    CHECK_HIP(launch_zero(ctl, kCtlWords, s));
+    if (timing) CHECK_HIP(hipEventRecord(ctx->ev[1], s));
+    CHECK_HIP(launch_lstar(d, b, s, shape));
+    if (timing) CHECK_HIP(hipEventRecord(ctx->ev[2], s));
+    rc = read_ctl(ctx, 0, kCtlWords);
+    if (rc) return rc;
+    msceer_timing t{};
+    for (int i = 0; i < kSubs; i++)
+        for (int k = 0; k < 4; k++) t.critical[k] += ctx->h_ctl[kCtlStats + 4 * i + k];
+    const bool lists_ok = ctx->h_ctl[kCtlOverflow] == 0;
+
+    int rounds = 0;
+    if (timing) CHECK_HIP(hipEventRecord(ctx->ev[3], s));
+    if (!(flags & MSCEER_SKIP_LDIR)) {
+        rc = run_ldir(ctx, d, d_grad, caps, lists_ok, &rounds);
+        if (rc) return rc;
+    }
+    if (timing) CHECK_HIP(hipEventRecord(ctx->ev[4], s));
+    if (host_out) CHECK_HIP(hipMemcpyAsync(host_out, d_grad, d.cells, hipMemcpyDeviceToHost, s));
+    CHECK_HIP(hipStreamSynchronize(s));
+    if (timing) {
+        CHECK_HIP(hipEventElapsedTime(&t.lstar_ms, ctx->ev[1], ctx->ev[2]));
+        CHECK_HIP(hipEventElapsedTime(&t.ldir_ms, ctx->ev[3], ctx->ev[4]));
+        CHECK_HIP(hipEventElapsedTime(&t.total_ms, ctx->ev[0], ctx->ev[4]));
+    }
+    t.ldir_rounds = rounds;
+    ctx->last = t;
     return MSCEER_OK;
 }
 
@@ -215,7 +289,7 @@ int msceer_create(msceer_ctx** out, int device) {
     hipError_t e = hipSetDevice(device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->h_ctl, sizeof(uint32_t) * kCtlWords, hipHostMallocDefault);
-    for (int i = 0; i < 12 && e == hipSuccess; i++) e = hipEventCreate(&ctx->ev[i]);
+    for (int i = 0; i < msceer_ctx::kEvents && e == hipSuccess; i++) e = hipEventCreate(&ctx->ev[i]);
     if (e == hipSuccess) e = ctx->ctl.ensure(sizeof(uint32_t) * kCtlWords);
     if (e != hipSuccess) {
         msceer_destroy(ctx);
@@ -238,12 +312,13 @@ int msceer_destroy(msceer_ctx* ctx) {
     ctx->front0.release();
     ctx->front1.release();
     ctx->ctl.release();
+    ctx->ldir.release();
     DevBuf* label_bufs[] = {&ctx->gradf, &ctx->gradd, &ctx->scan, &ctx->label, &ctx->label2, &ctx->dest, &ctx->seedmask, &ctx->redo,
                             &ctx->stamp, &ctx->rep, &ctx->rep2, &ctx->restriction, &ctx->host_io, &ctx->mask};
     for (DevBuf* b : label_bufs) b->release();
     ctx->host_stage.release();
     if (ctx->h_ctl) (void)hipHostFree(ctx->h_ctl);
-    for (int i = 0; i < 12; i++)
+    for (int i = 0; i < msceer_ctx::kEvents; i++)
         if (ctx->ev[i]) (void)hipEventDestroy(ctx->ev[i]);
     if (ctx->own) (void)hipStreamDestroy(ctx->own);
     delete ctx;
@@ -271,9 +346,6 @@ int msceer_lstar_grad(msceer_ctx* ctx, const float* values, int64_t nx, int64_t 
     CHECK_HIP(hipSetDevice(ctx->device));
     const Dims d = make_dims(nx, ny, nz);
     const bool dev = flags & MSCEER_DEVICE_PTRS;
-    const bool timing = flags & MSCEER_TIMING;
-    hipStream_t s = ctx->stream;
-
     const float* d_values = values;
     uint8_t* d_grad = grad_out;
     if (!dev) {
@@ -281,49 +353,10 @@ int msceer_lstar_grad(msceer_ctx* ctx, const float* values, int64_t nx, int64_t 
         CHECK_HIP(ctx->grad.ensure(d.cells));
         d_values = static_cast<const float*>(ctx->values.ptr);
         d_grad = static_cast<uint8_t*>(ctx->grad.ptr);
-        CHECK_HIP(hipMemcpyAsync(ctx->values.ptr, values, sizeof(float) * d.verts, hipMemcpyHostToDevice, s));
+        CHECK_HIP(hipMemcpyAsync(ctx->values.ptr, values, sizeof(float) * d.verts, hipMemcpyHostToDevice,
+                                 ctx->stream));
     }
-    // critical-cell lists and frontiers: sized to the volume, capped; any
-    // overflow falls back to a sweep
-    ListCaps caps{};
-    rc = ensure_lists(ctx, d, &caps);
-    if (rc) return rc;
-    uint32_t* ctl = ctl_dev(ctx);
-    const MList list1 = crit_list(ctx, caps, 0), list2 = crit_list(ctx, caps, 1);
-
-    LstarBuffers b{d_values, d_grad, nullptr, &list1, &list2, ctl + kCtlStats, ctl + kCtlOverflow};
-    int shape = 0;
-    rc = lstar_shape(ctx, d, b, &shape);
-    if (rc) return rc;
-    if (timing) CHECK_HIP(hipEventRecord(ctx->ev[0], s));
-    CHECK_HIP(launch_zero(ctl, kCtlWords, s));
-    if (timing) CHECK_HIP(hipEventRecord(ctx->ev[1], s));
-    CHECK_HIP(launch_lstar(d, b, s, shape));
-    if (timing) CHECK_HIP(hipEventRecord(ctx->ev[2], s));
-    rc = read_ctl(ctx, 0, kCtlWords);
-    if (rc) return rc;
-    msceer_timing t{};
-    for (int i = 0; i < kSubs; i++)
-        for (int k = 0; k < 4; k++) t.critical[k] += ctx->h_ctl[kCtlStats + 4 * i + k];
-    const bool lists_ok = ctx->h_ctl[kCtlOverflow] == 0;
-
-    int rounds = 0;
-    if (timing) CHECK_HIP(hipEventRecord(ctx->ev[3], s));
-    if (!(flags & MSCEER_SKIP_LDIR)) {
-        rc = run_ldir(ctx, d, d_grad, caps, lists_ok, &rounds);
-        if (rc) return rc;
-    }
-    if (timing) CHECK_HIP(hipEventRecord(ctx->ev[4], s));
-    if (!dev) CHECK_HIP(hipMemcpyAsync(grad_out, d_grad, d.cells, hipMemcpyDeviceToHost, s));
-    CHECK_HIP(hipStreamSynchronize(s));
-    if (timing) {
-        CHECK_HIP(hipEventElapsedTime(&t.lstar_ms, ctx->ev[1], ctx->ev[2]));
-        CHECK_HIP(hipEventElapsedTime(&t.ldir_ms, ctx->ev[3], ctx->ev[4]));
-        CHECK_HIP(hipEventElapsedTime(&t.total_ms, ctx->ev[0], ctx->ev[4]));
-    }
-    t.ldir_rounds = rounds;
-    ctx->last = t;
-    return MSCEER_OK;
+    return lstar_grad_device(ctx, d, d_values, d_grad, dev ? nullptr : grad_out, flags);
 }
 
 }  // extern "C"

@@ -16,12 +16,16 @@
 
 namespace msceer {
 
-// Grow-only device buffer.
+// Grow-only device buffer.  A borrowed buffer is a view of memory the
+// buffer does not own (a slice of a static scratch array): it is never freed
+// or grown, and asking it for more than it holds fails.
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
+    bool borrowed = false;
     hipError_t ensure(size_t need) {
         if (need <= bytes) return hipSuccess;
+        if (borrowed) return hipErrorOutOfMemory;
         if (ptr) { (void)hipFree(ptr); ptr = nullptr; bytes = 0; }
         need = (need + 255) & ~(size_t)255;
         hipError_t e = hipMalloc(&ptr, need);
@@ -31,9 +35,15 @@ struct DevBuf {
     template <typename T>
     T* as() const { return static_cast<T*>(ptr); }
     void release() {
-        if (ptr) (void)hipFree(ptr);
+        if (ptr && !borrowed) (void)hipFree(ptr);
         ptr = nullptr;
         bytes = 0;
+        borrowed = false;
+    }
+    void borrow(void* p, size_t n) {
+        ptr = p;
+        bytes = n;
+        borrowed = true;
     }
 };
 
@@ -74,7 +84,7 @@ enum Ctl : int {
     kCtlRound = 2 * kCountWords,   // three count arrays
     kCtlStats = 5 * kCountWords,   // kSubs * 4
     kCtlOverflow = 5 * kCountWords + 4 * kSubs,  // a list or frontier ran out of room
-    kCtlTail = kCtlOverflow + 1,   // shadow of the grad array's last partial word
+    kCtlTail = kCtlOverflow + 1,   // (unused)
     kCtlFlag = kCtlOverflow + 2,   // change flag of iterative sweeps
     kCtlCount = kCtlOverflow + 3,  // a kernel-reported count
     kCtlCursor = kCtlOverflow + 4,  // head, tail, trace-ahead flag, chunk head, first count of the re-integration queue
@@ -106,11 +116,13 @@ struct msceer_ctx {
     hipStream_t own = nullptr;
     hipStream_t stream = nullptr;
     msceer::DevBuf values, grad, lists, front0, front1, ctl;
+    msceer::DevBuf ldir;  // ldir walk records and visited bitmaps
     // accurate labelling
     msceer::DevBuf gradf, gradd, scan, label, label2, dest, seedmask, redo, stamp, rep, rep2, restriction, host_io, mask;
     msceer::HostBuf host_stage;  // pinned staging of host-side merges (cleanup)
     uint32_t* h_ctl = nullptr;  // pinned mirror of the control block
-    hipEvent_t ev[12] = {};
+    static constexpr int kEvents = 16;
+    hipEvent_t ev[kEvents] = {};
     msceer_timing last{};
     msceer_label_stats last_labels{};
     std::shared_ptr<msceer::Tuning> tuning = std::make_shared<msceer::Tuning>();
@@ -129,14 +141,29 @@ int ensure_lists(msceer_ctx* ctx, const Dims& d, ListCaps* caps);
 MList crit_list(msceer_ctx* ctx, const ListCaps& c, int which);
 // Frontier k of a chain of rounds: buffer k % 2, count array k % 3.
 MList front_list(msceer_ctx* ctx, const ListCaps& c, int k);
-int ldir_pass(msceer_ctx* ctx, const Dims& d, uint8_t* grad, uint32_t item_dim, uint32_t value,
-              const MList& start, bool start_valid, const ListCaps& caps, int* rounds);
+int ldir_pass(msceer_ctx* ctx, const Dims& d, uint8_t* grad, const LdirScratch& ls, int pass, const MList& start,
+              bool start_valid, const ListCaps& caps, int* rounds);
 // Brick shape for the lower-star kernel on this volume: tuned on first use of
 // an x extent (times every shape on the given buffers; call before the
 // control block is cleared for the real launch).
 int lstar_shape(msceer_ctx* ctx, const Dims& d, const LstarBuffers& b, int* shape);
-// ldir passes + tail shadow on a .grad whose lstar pass filled the critical lists.
+// The shape lstar_shape would return without timing any: the tuned one, else `fallback`.
+int tuned_shape(msceer_ctx* ctx, const Dims& d, int fallback);
+// ldir passes on a .grad whose lstar pass filled the critical lists.
 int run_ldir(msceer_ctx* ctx, const Dims& d, uint8_t* grad, const ListCaps& caps, bool lists_ok, int* rounds);
+// Capacities of the critical lists and frontiers of a volume, and the bytes
+// ensure_lists gives them (two lists, two frontiers).
+ListCaps list_caps(const Dims& d);
+size_t list_bytes(const ListCaps& c);
+size_t front_bytes(const ListCaps& c);
+// Flag of lstar_grad_device (beside the MSCEER_* flags): use the brick shape
+// already tuned for this x extent (64x2x2 if none) instead of tuning one.
+constexpr uint32_t kLstarNoTuning = 0x80000000u;
+// The whole .grad of a volume in device memory (msceer_lstar_grad's work after
+// staging): lists, frontiers and ldir scratch from the context; host_out
+// (may be null) receives a copy before the final wait.
+int lstar_grad_device(msceer_ctx* ctx, const Dims& d, const float* d_values, uint8_t* d_grad, uint8_t* host_out,
+                      uint32_t flags);
 
 #define CHECK_HIP(expr)                                        \
     do {                                                       \

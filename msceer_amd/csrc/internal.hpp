@@ -42,17 +42,28 @@ constexpr int kLstarBricks = 4;
 const char* lstar_brick_name(int shape);
 hipError_t launch_lstar(const Dims& d, const LstarBuffers& b, hipStream_t s, int shape = 0);
 
-// ldir passes.  `tail` is one device uint32 shadowing the bytes of the last,
-// partial 32-bit word of the grad array: pack it before the passes
-// (unpack=false) and write it back after them (unpack=true).
-hipError_t launch_ldir_tail(const Dims& d, uint8_t* grad, uint32_t* tail, bool unpack, hipStream_t s);
-// One frontier round (see ldir.hip): follows the items of `in`, appends what
-// is left to `out` and zeroes `clear_counts` (kSubs words) for the next round.
-hipError_t launch_ldir_round(const Dims& d, uint8_t* grad, uint32_t* tail, const MList& in, const MList& out,
-                             uint32_t* clear_counts, uint32_t* overflow, uint32_t value, hipStream_t s);
-// Rebuild a frontier from the grad bytes alone (after an overflow).
-hipError_t launch_ldir_sweep(const Dims& d, uint8_t* grad, uint32_t* tail, uint32_t item_dim,
-                             uint32_t value, const MList& out, uint32_t* overflow, hipStream_t s);
+// ldir (ascending-manifold dimension) passes, see ldir.hip.  Scratch: edge
+// and quad walk records and the two visited bitmaps, at these byte offsets of
+// one device buffer of ldir_scratch_bytes().
+struct LdirScratch {
+    size_t rec1_off, rec2_off, bits1_off, bits2_off;
+    uint32_t bit_words;  // words of one bitmap
+};
+size_t ldir_scratch_bytes(const Dims& d, LdirScratch* s);
+// Records of the walks (from the lower-star bytes) and cleared bitmaps.
+hipError_t launch_ldir_records(const Dims& d, const uint8_t* grad, void* scratch, const LdirScratch& s,
+                               hipStream_t st);
+// One frontier round of pass 1 (edges) or 2 (quads): expands the items of
+// `in` (critical cells, claimed first, when `start`), appends what is left to
+// `out` and zeroes `clear_counts` for the next round.  overflow |= 1 when an
+// item did not fit `out`, |= 4 when a start round left items unread.
+hipError_t launch_ldir_round(const Dims& d, void* scratch, const LdirScratch& s, int pass, bool start, const MList& in,
+                             const MList& out, uint32_t* clear_counts, uint32_t* overflow, hipStream_t st);
+// Rebuild a frontier from the bitmaps and records alone (after an overflow).
+hipError_t launch_ldir_sweep(const Dims& d, const uint8_t* grad, void* scratch, const LdirScratch& s, int pass,
+                             const MList& out, uint32_t* overflow, hipStream_t st);
+// Every cell's ldir from what the two passes reached.
+hipError_t launch_ldir_finish(const Dims& d, uint8_t* grad, void* scratch, const LdirScratch& s, hipStream_t st);
 // Zero n words.
 hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t s);
 

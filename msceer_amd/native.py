@@ -17,6 +17,8 @@ TIMING = 0x4
 STOP_REGIONS = 0x8
 STOP_INTEGRATED = 0x10
 BRICKS = ("64x2x2", "32x4x2", "16x4x4", "8x8x4")  # lower-star brick shapes by index
+# error codes
+OK, EINVAL, EHIP, ENOMEM, ETOOBIG, ENODEV = 0, -1, -2, -3, -4, -5
 
 _P = ctypes.c_void_p
 _I64 = ctypes.c_int64
@@ -87,6 +89,8 @@ class LabelStats(ctypes.Structure):
         ("steps_fp32", ctypes.c_int64),
         ("steps_fp64", ctypes.c_int64),
         ("ahead_fp64", ctypes.c_int64),
+        ("ms_pass1", ctypes.c_float),
+        ("ms_pass3", ctypes.c_float),
     ]
 
     def as_dict(self):
