@@ -113,6 +113,22 @@ int msceer_lstar_grad(msceer_ctx* ctx, const float* values, int64_t nx, int64_t 
                       uint8_t* grad_out, uint32_t flags);
 
 /*
+ * msceer_lstar_grad without device allocation.  No counterpart in the
+ * reference: for callers that cannot allocate device memory (tight serving
+ * loops, harnesses that forbid hipMalloc).  Device pointers only (values in,
+ * grad_out out); every scratch buffer (control words, critical-cell lists,
+ * frontiers, ldir records) comes from one static 48 MiB array per GPU, so a
+ * volume whose scratch would exceed it returns MSCEER_ETOOBIG (up to about
+ * 1.3 million vertices fit, e.g. 128x128x64).  The lower-star brick shape is
+ * the one an earlier msceer_lstar_grad tuned for this x extent, else 64x2x2;
+ * no timing runs.  Flags: MSCEER_SKIP_LDIR, MSCEER_TIMING.  All calls on one
+ * GPU share its array, so the library runs them one at a time; each call
+ * clears what it needs before use.
+ */
+int msceer_lstar_grad_noalloc(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz,
+                              uint8_t* grad_out, uint32_t flags);
+
+/*
  * Accurate ascending 3-manifold labelling and its conforming .grad: the
  * needASC2 branch of ondemandaccurate (ondemandaccurate/ondemandaccurate.cxx:728-1061):
  *   base gradient (as msceer_lstar_grad, without ldir),

@@ -287,6 +287,7 @@ int msceer_create(msceer_ctx** out, int device) {
     if (!ctx) return MSCEER_ENOMEM;
     ctx->device = device;
     hipError_t e = hipSetDevice(device);
+    if (e == hipSuccess && noalloc_setup(device) != MSCEER_OK) e = hipErrorInvalidSymbol;
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking);
     if (e == hipSuccess) e = hipHostMalloc((void**)&ctx->h_ctl, sizeof(uint32_t) * kCtlWords, hipHostMallocDefault);
     for (int i = 0; i < msceer_ctx::kEvents && e == hipSuccess; i++) e = hipEventCreate(&ctx->ev[i]);

@@ -149,6 +149,9 @@ int ldir_pass(msceer_ctx* ctx, const Dims& d, uint8_t* grad, const LdirScratch& 
 int lstar_shape(msceer_ctx* ctx, const Dims& d, const LstarBuffers& b, int* shape);
 // The shape lstar_shape would return without timing any: the tuned one, else `fallback`.
 int tuned_shape(msceer_ctx* ctx, const Dims& d, int fallback);
+// Resolves the device address of the allocation-free entry point's scratch
+// array on `device` (current), once per device.
+int noalloc_setup(int device);
 // ldir passes on a .grad whose lstar pass filled the critical lists.
 int run_ldir(msceer_ctx* ctx, const Dims& d, uint8_t* grad, const ListCaps& caps, bool lists_ok, int* rounds);
 // Capacities of the critical lists and frontiers of a volume, and the bytes

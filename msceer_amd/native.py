@@ -37,6 +37,7 @@ SIGNATURES = {
     "msceer_set_autotune": (ctypes.c_int, [_P, _I32]),
     "msceer_lstar_brick": (ctypes.c_int, [_P, _I64]),
     "msceer_lstar_grad": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _U32]),
+    "msceer_lstar_grad_noalloc": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _U32]),
     "msceer_ondemand_asc": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, ctypes.c_float, ctypes.c_float, _I32,
                                            ctypes.c_float, _P, _P, _P, _P, _U32]),
     "msceer_integrate_labels": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _P, ctypes.c_float, ctypes.c_float, _I32,
