@@ -41,15 +41,47 @@ def read_traffic(name: str):
     return None
 
 
+REF_DIR = os.path.join(ROOT, "oracle", "_ref")
+
+
+def _run_reference(prog, args, cores):
+    """Run a reference driver (the reference's own sources built by
+    oracle/Makefile) and return its REFTIME fields, or None if it is absent."""
+    import subprocess
+    exe = os.path.join(REF_DIR, prog)
+    if not os.path.exists(exe):
+        return None
+    env = dict(os.environ, OMP_NUM_THREADS=str(cores))
+    r = subprocess.run([exe, *map(str, args), str(cores)], capture_output=True, text=True, env=env, timeout=600)
+    for line in r.stdout.splitlines():
+        if line.startswith("REFTIME"):
+            return {k: float(v) for k, v in (kv.split("=") for kv in line.split()[1:])}
+    return None
+
+
 def cpu_baseline(f: np.ndarray, slab: int):
-    """The CPU oracle (a C++/OpenMP restatement of the reference path) on a slab
-    of the same volume, all host cores."""
+    """The reference's own lower-star gradient (maxv + pairing + ldir, its
+    OpenMP code built from its sources) on a slab of the same volume, all host
+    cores; the oracle restatement where the reference build is absent."""
+    import tempfile
+    cores = _cores()
+    sample = np.ascontiguousarray(f[:slab])
+    Z, Y, X = sample.shape
+    with tempfile.TemporaryDirectory() as tmp:
+        raw = os.path.join(tmp, "slab.raw")
+        sample.tofile(raw)
+        t = _run_reference("ref_steepest", [X, Y, Z, raw, os.path.join(tmp, "out")], cores)
+    if t is not None:
+        dt = t["total_ms"] * 1e-3
+        return {"value": round(X * Y * Z / dt / 1e6, 3), "unit": "Mvert/s", "cores": cores, "kind": "reference",
+                "sample": f"{X}x{Y}x{Z} slab (first {Z} z-planes) of the benchmark volume; the reference's "
+                          f"RegularGridMaxMinVertexLabeling3D + SlidingWindowRobinsNoalloc + "
+                          f"setAscendingManifoldDimensions (OpenMP, built from its sources), {dt:.2f} s "
+                          f"(maxv {t['maxv_ms']:.0f} ms, pairing {t['pair_ms']:.0f} ms, ldir {t['ldir_ms']:.0f} ms)"}
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib  # noqa: E402  (checker / baseline only)
 
-    cores = _cores()
     oracle_lib.set_threads(cores)
-    sample = np.ascontiguousarray(f[:slab])
     oracle_lib.steepest_grad(sample[:8])  # warm the thread pool
     t0 = time.perf_counter()
     oracle_lib.steepest_grad(sample)
@@ -61,14 +93,29 @@ def cpu_baseline(f: np.ndarray, slab: int):
 
 
 def labels_cpu_baseline(f: np.ndarray, slab: int, params):
-    """The CPU oracle of the accurate labelling (reference arithmetic, all host
-    cores) on a z-slab of the labelling volume."""
+    """The reference's own accurate labelling (its OpenMP ondemandaccurate
+    needASC2 pipeline, built from its sources) on a z-slab of the labelling
+    volume, all host cores; the oracle restatement where the reference build
+    is absent."""
+    import tempfile
+    cores = _cores()
+    sample = np.ascontiguousarray(f[:slab])
+    Z, Y, X = sample.shape
+    err, gthr, iters, presimp = params
+    with tempfile.TemporaryDirectory() as tmp:
+        raw = os.path.join(tmp, "slab.raw")
+        sample.tofile(raw)
+        t = _run_reference("ref_ondemand", [X, Y, Z, raw, os.path.join(tmp, "out"), err, gthr, iters, presimp], cores)
+    if t is not None:
+        dt = t["total_ms"] * 1e-3
+        return {"value": round(X * Y * Z / dt / 1e6, 3), "unit": "Mvert/s", "cores": cores, "kind": "reference",
+                "sample": f"{X}x{Y}x{Z} slab (first {Z} z-planes) of the labelling volume; the reference's "
+                          f"needASC2 pipeline (OpenMP, built from its sources), {dt:.2f} s "
+                          f"(integration {t['integrate_ms']:.0f} ms)"}
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib  # noqa: E402  (checker / baseline only)
 
-    cores = _cores()
     oracle_lib.set_threads(cores)
-    sample = np.ascontiguousarray(f[:slab])
     t0 = time.perf_counter()
     oracle_lib.ondemand_asc(sample, *params, mode=0)
     dt = time.perf_counter() - t0

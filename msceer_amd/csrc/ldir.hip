@@ -31,8 +31,10 @@
 //     over rec2; a reached item is claimed in a bitmap (visited1 / visited2,
 //     one bit per record, atomic OR), so each item is expanded once.  The
 //     .grad bytes are not written during the walks.
-//  3. ldir_finish_kernel writes every cell's ldir from the closed form above,
-//     one streaming pass over the .grad (whole rows per wave).
+//  3. ldir_marks_kernel derives the marks of every quad from the pass-1 bits
+//     (marked1, and with which hex it is paired), ldir_finish_kernel then
+//     writes every cell's ldir from the closed form above, both streaming
+//     passes (whole rows per wave).
 //
 // Records and bits are indexed by 3 * (vertex id of the cell's lowest corner)
 // + axis, axis = the odd axis of an edge or the even axis of a quad.
@@ -80,24 +82,25 @@ __device__ __forceinline__ bool test_bit(const uint32_t* bits, uint32_t i) { ret
 // Claim bit i; true if this call set it.
 __device__ __forceinline__ bool claim_bit(uint32_t* bits, uint32_t i) {
     const uint32_t m = 1u << (i & 31u);
-    if (__hip_atomic_load(&bits[i >> 5], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) & m) return false;
     return !(atomicOr(&bits[i >> 5], m) & m);
 }
 
 // Pair direction code d (0..5 = +x,-x,+y,-y,+z,-z) applied to coordinates.
+// (selects rather than branches on the axis, so the coordinates stay in
+// registers when the axis is known only at run time)
 __device__ __forceinline__ void apply_dir(uint32_t d, uint32_t& x, uint32_t& y, uint32_t& z) {
     const uint32_t a = d >> 1;
     const uint32_t s = (d & 1u) ? 0xFFFFFFFFu : 1u;  // -1 or +1 (wrapping add)
-    if (a == 0) x += s;
-    else if (a == 1) y += s;
-    else z += s;
+    x += a == 0u ? s : 0u;
+    y += a == 1u ? s : 0u;
+    z += a == 2u ? s : 0u;
 }
 
 __device__ __forceinline__ void axis_step(uint32_t a, int sgn, uint32_t& x, uint32_t& y, uint32_t& z) {
     const uint32_t s = sgn > 0 ? 1u : 0xFFFFFFFFu;
-    if (a == 0) x += s;
-    else if (a == 1) y += s;
-    else z += s;
+    x += a == 0u ? s : 0u;
+    y += a == 1u ? s : 0u;
+    z += a == 2u ? s : 0u;
 }
 
 __device__ __forceinline__ uint32_t coord(uint32_t a, uint32_t x, uint32_t y, uint32_t z) {
@@ -119,68 +122,93 @@ __device__ __forceinline__ bool points_back(uint32_t d, uint32_t a, int sgn) {
 
 // ---- phase 1: records ------------------------------------------------------
 
+// 3 bits of vertex v in a bitmap of 3 bits per vertex (the bitmaps are padded
+// by one word, so the second load stays inside).
+__device__ __forceinline__ uint32_t bits3(const uint32_t* __restrict__ b, uint32_t v) {
+    const uint32_t i = 3u * v;
+    const uint64_t w = ((uint64_t)b[(i >> 5) + 1] << 32) | b[i >> 5];
+    return (uint32_t)(w >> (i & 31u)) & 7u;
+}
+
+// Record slot of the cofacet (byte b) reached from the item along axis a
+// with sign sgn; the item is an edge with odd axis o (kO < 3), or a quad
+// (kO == 3) whose pair partners are always quads.  8 | pair direction when
+// the walk goes on through it: assigned, not critical, the pair a facet
+// (for an edge item: along o or a) and not the item itself.
+template <int kO, int kA, int kSgn>
+__device__ __forceinline__ uint32_t slot_code(uint32_t b) {
+    const uint32_t d = (b >> 2) & 7u;
+    if (!(b & 1u) || d == 7u) return 0u;
+    const uint32_t ax = d >> 1;
+    if (kO < 3 && ax != (uint32_t)kO && ax != (uint32_t)kA) return 0u;
+    if (ax == (uint32_t)kA && ((d & 1u) != 0u) == (kSgn > 0)) return 0u;
+    return 8u | d;
+}
+
 // One thread per vertex (i, j, k): its three "+" edges and three "+" quads.
-// rec1 slot 2*m + (sgn < 0) is the cofacet of the edge along its m-th even
-// axis; rec2 slot (sgn < 0) the cofacet hex of the quad along its even axis.
-// A slot holds 8 | pair direction of the cofacet when the walk goes on
-// through it, else 0.
+// rec1[3v + o] (edge with odd axis o): slot 2m + (sgn < 0) is the cofacet
+// quad along the edge's m-th even axis; rec2[3v + a] (quad with even axis a):
+// slot (sgn < 0) is the cofacet hex along a.  Cells are addressed by 32-bit
+// offsets from the vertex's own cell (fewer than 2^32 cells).
 __global__ void __launch_bounds__(kThreads)
 ldir_records_kernel(const uint8_t* __restrict__ grad, LGeom g, uint16_t* __restrict__ rec1,
                     uint8_t* __restrict__ rec2, uint32_t* __restrict__ bits, uint32_t nbits_words) {
     for (uint32_t w = blockIdx.x * kThreads + threadIdx.x; w < nbits_words; w += gridDim.x * kThreads) bits[w] = 0u;
+    const uint32_t Zv = (g.Mz + 1) >> 1;
+    const uint32_t ox = 1u, oy = g.Mx, oz = g.sxy;
     for (uint32_t v = blockIdx.x * kThreads + threadIdx.x; v < g.V; v += gridDim.x * kThreads) {
         const uint32_t q = v / g.X, k = q / g.Y;
         const uint32_t i = v - q * g.X, j = q - k * g.Y;
-        const uint32_t vx = 2 * i, vy = 2 * j, vz = 2 * k;
-        // edges: odd axis o
-        for (uint32_t o = 0; o < 3; o++) {
-            uint32_t ex = vx, ey = vy, ez = vz;
-            axis_step(o, 1, ex, ey, ez);
-            if (coord(o, ex, ey, ez) >= extent(g, o)) continue;
-            uint32_t a1, a2;
-            other_axes(o, a1, a2);
+        const bool px = i + 1 < g.X, py = j + 1 < g.Y, pz = k + 1 < Zv;  // "+" neighbours exist
+        const bool mx = i > 0, my = j > 0, mz = k > 0;                     // "-" neighbours exist
+        const uint32_t c0 = 2u * i + oy * 2u * j + oz * 2u * k;
+        if (px) {  // x-edge: cofacets along y, z
+            const uint32_t e = c0 + ox;
             uint32_t r = 0;
-#pragma unroll
-            for (int slot = 0; slot < 4; slot++) {
-                const uint32_t a = slot < 2 ? a1 : a2;
-                const int sgn = (slot & 1) ? -1 : 1;
-                const uint32_t ca = coord(a, ex, ey, ez);
-                if (sgn > 0 ? ca + 1 >= extent(g, a) : ca == 0) continue;
-                uint32_t tx = ex, ty = ey, tz = ez;
-                axis_step(a, sgn, tx, ty, tz);
-                const uint32_t b = grad[cell_id(g, tx, ty, tz)];
-                const uint32_t d = (b >> 2) & 7u;
-                // on to an edge: the pair is a facet of t (along o or a), not the item
-                if (!(b & 1u) || d == 7u) continue;
-                if ((d >> 1) != o && (d >> 1) != a) continue;
-                if (points_back(d, a, sgn)) continue;
-                r |= (8u | d) << (4 * slot);
-            }
-            rec1[rec_index(g, ex, ey, ez, o)] = (uint16_t)r;
+            if (py) r |= slot_code<0, 1, 1>(grad[e + oy]);
+            if (my) r |= slot_code<0, 1, -1>(grad[e - oy]) << 4;
+            if (pz) r |= slot_code<0, 2, 1>(grad[e + oz]) << 8;
+            if (mz) r |= slot_code<0, 2, -1>(grad[e - oz]) << 12;
+            rec1[3u * v] = (uint16_t)r;
         }
-        // quads: even axis a
-        for (uint32_t a = 0; a < 3; a++) {
-            uint32_t qx = vx, qy = vy, qz = vz;
-            uint32_t a1, a2;
-            other_axes(a, a1, a2);
-            axis_step(a1, 1, qx, qy, qz);
-            axis_step(a2, 1, qx, qy, qz);
-            if (coord(a1, qx, qy, qz) >= extent(g, a1) || coord(a2, qx, qy, qz) >= extent(g, a2)) continue;
+        if (py) {  // y-edge: cofacets along x, z
+            const uint32_t e = c0 + oy;
             uint32_t r = 0;
-#pragma unroll
-            for (int slot = 0; slot < 2; slot++) {
-                const int sgn = slot ? -1 : 1;
-                const uint32_t ca = coord(a, qx, qy, qz);
-                if (sgn > 0 ? ca + 1 >= extent(g, a) : ca == 0) continue;
-                uint32_t hx = qx, hy = qy, hz = qz;
-                axis_step(a, sgn, hx, hy, hz);
-                const uint32_t b = grad[cell_id(g, hx, hy, hz)];
-                const uint32_t d = (b >> 2) & 7u;
-                if (!(b & 1u) || d == 7u) continue;
-                if (points_back(d, a, sgn)) continue;
-                r |= (8u | d) << (4 * slot);
-            }
-            rec2[rec_index(g, qx, qy, qz, a)] = (uint8_t)r;
+            if (px) r |= slot_code<1, 0, 1>(grad[e + ox]);
+            if (mx) r |= slot_code<1, 0, -1>(grad[e - ox]) << 4;
+            if (pz) r |= slot_code<1, 2, 1>(grad[e + oz]) << 8;
+            if (mz) r |= slot_code<1, 2, -1>(grad[e - oz]) << 12;
+            rec1[3u * v + 1u] = (uint16_t)r;
+        }
+        if (pz) {  // z-edge: cofacets along x, y
+            const uint32_t e = c0 + oz;
+            uint32_t r = 0;
+            if (px) r |= slot_code<2, 0, 1>(grad[e + ox]);
+            if (mx) r |= slot_code<2, 0, -1>(grad[e - ox]) << 4;
+            if (py) r |= slot_code<2, 1, 1>(grad[e + oy]) << 8;
+            if (my) r |= slot_code<2, 1, -1>(grad[e - oy]) << 12;
+            rec1[3u * v + 2u] = (uint16_t)r;
+        }
+        if (py && pz) {  // quad with even axis x: cofacet hexes along x
+            const uint32_t t = c0 + oy + oz;
+            uint32_t r = 0;
+            if (px) r |= slot_code<3, 0, 1>(grad[t + ox]);
+            if (mx) r |= slot_code<3, 0, -1>(grad[t - ox]) << 4;
+            rec2[3u * v] = (uint8_t)r;
+        }
+        if (px && pz) {  // even axis y
+            const uint32_t t = c0 + ox + oz;
+            uint32_t r = 0;
+            if (py) r |= slot_code<3, 1, 1>(grad[t + oy]);
+            if (my) r |= slot_code<3, 1, -1>(grad[t - oy]) << 4;
+            rec2[3u * v + 1u] = (uint8_t)r;
+        }
+        if (px && py) {  // even axis z
+            const uint32_t t = c0 + ox + oy;
+            uint32_t r = 0;
+            if (pz) r |= slot_code<3, 2, 1>(grad[t + oz]);
+            if (mz) r |= slot_code<3, 2, -1>(grad[t - oz]) << 4;
+            rec2[3u * v + 2u] = (uint8_t)r;
         }
     }
 }
@@ -194,50 +222,54 @@ struct Walk {
     int pass;        // 1: edges over rec1, 2: quads over rec2
 };
 
-// The items reached in one step from `item` (cell id): up to 4, as cell ids
-// whose bits this call claimed.  `start`: the item is a critical cell of the
-// start list, claimed here first (skipped if already reached).
-__device__ __forceinline__ int expand(const LGeom& g, const Walk& w, uint32_t item, bool start, uint32_t out[4]) {
+constexpr uint32_t kNoItem = 0xFFFFFFFFu;
+
+// Claim the bits of the (up to 4) items reached in one step from `item`
+// (cell id): out[slot] is the cell id of the item reached through record
+// slot `slot` when this call claimed it, else kNoItem.  `start`: the item is
+// a critical cell of the start list, claimed here first (skipped if already
+// reached).  The claims of the slots are independent atomics, issued
+// together.
+__device__ __forceinline__ void expand(const LGeom& g, const Walk& w, uint32_t item, bool start, uint32_t out[4]) {
+#pragma unroll
+    for (int s = 0; s < 4; s++) out[s] = kNoItem;
     uint32_t x, y, z;
     cell_coords(g, item, x, y, z);
-    int n = 0;
-    if (w.pass == 1) {
-        const uint32_t o = (x & 1u) ? 0u : ((y & 1u) ? 1u : 2u);
-        const uint32_t ri = rec_index(g, x, y, z, o);
-        if (start && !claim_bit(w.bits, ri)) return 0;
-        const uint32_t r = w.rec1[ri];
-        if (!r) return 0;
-        uint32_t a1, a2;
-        other_axes(o, a1, a2);
-#pragma unroll
-        for (int slot = 0; slot < 4; slot++) {
-            const uint32_t f = (r >> (4 * slot)) & 15u;
-            if (!(f & 8u)) continue;
-            uint32_t px = x, py = y, pz = z;
-            axis_step(slot < 2 ? a1 : a2, (slot & 1) ? -1 : 1, px, py, pz);
-            apply_dir(f & 7u, px, py, pz);
-            // p is an edge: its odd axis
-            const uint32_t po = (px & 1u) ? 0u : ((py & 1u) ? 1u : 2u);
-            if (claim_bit(w.bits, rec_index(g, px, py, pz, po))) out[n++] = cell_id(g, px, py, pz);
-        }
-    } else {
-        const uint32_t a = !(x & 1u) ? 0u : (!(y & 1u) ? 1u : 2u);
-        const uint32_t ri = rec_index(g, x, y, z, a);
-        if (start && !claim_bit(w.bits, ri)) return 0;
-        const uint32_t r = w.rec2[ri];
-        if (!r) return 0;
-#pragma unroll
-        for (int slot = 0; slot < 2; slot++) {
-            const uint32_t f = (r >> (4 * slot)) & 15u;
-            if (!(f & 8u)) continue;
-            uint32_t px = x, py = y, pz = z;
-            axis_step(a, slot ? -1 : 1, px, py, pz);
-            apply_dir(f & 7u, px, py, pz);
-            const uint32_t pa = !(px & 1u) ? 0u : (!(py & 1u) ? 1u : 2u);
-            if (claim_bit(w.bits, rec_index(g, px, py, pz, pa))) out[n++] = cell_id(g, px, py, pz);
-        }
+    const bool edges = w.pass == 1;
+    // an edge item: its odd axis; a quad item: its even axis
+    const uint32_t o = edges ? ((x & 1u) ? 0u : ((y & 1u) ? 1u : 2u)) : (!(x & 1u) ? 0u : (!(y & 1u) ? 1u : 2u));
+    const uint32_t ri = rec_index(g, x, y, z, o);
+    if (start) {
+        const uint32_t m = 1u << (ri & 31u);
+        if (atomicOr(&w.bits[ri >> 5], m) & m) return;
     }
-    return n;
+    const uint32_t r = edges ? (uint32_t)w.rec1[ri] : (uint32_t)w.rec2[ri];
+    if (!r) return;
+    uint32_t a1, a2;
+    other_axes(o, a1, a2);
+    uint32_t cand[4], word[4], mask[4];
+#pragma unroll
+    for (int s = 0; s < 4; s++) {
+        const uint32_t f = (r >> (4 * s)) & 15u;
+        cand[s] = kNoItem;
+        if (!(f & 8u) || (!edges && s >= 2)) continue;
+        uint32_t px = x, py = y, pz = z;
+        axis_step(edges ? (s < 2 ? a1 : a2) : o, (s & 1) ? -1 : 1, px, py, pz);
+        apply_dir(f & 7u, px, py, pz);
+        // the item reached: an edge (its odd axis) or a quad (its even axis)
+        const uint32_t pa = edges ? ((px & 1u) ? 0u : ((py & 1u) ? 1u : 2u))
+                                  : (!(px & 1u) ? 0u : (!(py & 1u) ? 1u : 2u));
+        const uint32_t pi = rec_index(g, px, py, pz, pa);
+        cand[s] = cell_id(g, px, py, pz);
+        word[s] = pi >> 5;
+        mask[s] = 1u << (pi & 31u);
+    }
+    uint32_t old[4];
+#pragma unroll
+    for (int s = 0; s < 4; s++) old[s] = cand[s] != kNoItem ? atomicOr(&w.bits[word[s]], mask[s]) : 0u;
+#pragma unroll
+    for (int s = 0; s < 4; s++)
+        if (cand[s] != kNoItem && !(old[s] & mask[s])) out[s] = cand[s];
 }
 
 // One round.  A block takes an even share of the incoming frontier into an
@@ -307,20 +339,45 @@ ldir_round_kernel(LGeom g, Walk w, MList in, MList out, uint32_t* __restrict__ c
             __syncthreads();
         }
         if (m == 0) break;
+        uint32_t nb[kPerLane][4];
 #pragma unroll
         for (int k = 0; k < kPerLane; k++) {
-            if (items[k] == 0xFFFFFFFFu) continue;
-            uint32_t nb[4];
-            const int c = expand(g, w, items[k], from_share[k], nb);
-            for (int j = 0; j < c; j++) {
-                const uint32_t slot = atomicAdd(&s_n, 1u);
-                if (slot < (uint32_t)kPool) {
-                    pool[slot] = nb[j];
-                } else {
-                    if (!mlist_push(out, sub, nb[j])) atomicOr(overflow, 1u);
-                }
+            if (items[k] == kNoItem) {
+#pragma unroll
+                for (int j = 0; j < 4; j++) nb[k][j] = kNoItem;
+            } else {
+                expand(g, w, items[k], from_share[k], nb[k]);
             }
         }
+        // one pool reservation per wave for everything its lanes reached
+        uint32_t cnt = 0;
+#pragma unroll
+        for (int k = 0; k < kPerLane; k++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) cnt += nb[k][j] != kNoItem ? 1u : 0u;
+        const uint32_t lane = tid & 63u;
+        uint32_t incl = cnt;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const uint32_t t = __shfl_up(incl, o);
+            if (lane >= (uint32_t)o) incl += t;
+        }
+        const uint32_t wave_total = __shfl(incl, 63);
+        uint32_t base = 0;
+        if (lane == 63 && wave_total) base = atomicAdd(&s_n, wave_total);
+        base = __shfl(base, 63) + incl - cnt;
+#pragma unroll
+        for (int k = 0; k < kPerLane; k++)
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                if (nb[k][j] == kNoItem) continue;
+                if (base < (uint32_t)kPool) {
+                    pool[base] = nb[k][j];
+                } else {
+                    if (!mlist_push(out, sub, nb[k][j])) atomicOr(overflow, 1u);
+                }
+                base++;
+            }
         __syncthreads();
     }
     // hand on what is left: the pool (claimed items), then the unread part of
@@ -377,8 +434,10 @@ ldir_sweep_kernel(const uint8_t* __restrict__ grad, LGeom g, Walk w, MList out, 
             const uint32_t ri = rec_index(g, cx, cy, cz, a);
             if (test_bit(w.bits, ri)) {
                 uint32_t nb[4];
-                const int n = expand(g, w, c, false, nb);
-                for (int t = 0; t < n; t++) push(nb[t]);
+                expand(g, w, c, false, nb);
+#pragma unroll
+                for (int t = 0; t < 4; t++)
+                    if (nb[t] != kNoItem) push(nb[t]);
             } else if (((grad[c] >> 2) & 7u) == 7u && (grad[c] & 1u)) {
                 if (claim_bit(w.bits, ri)) push(c);
             }
@@ -391,82 +450,112 @@ ldir_sweep_kernel(const uint8_t* __restrict__ grad, LGeom g, Walk w, MList out, 
 
 // ---- phase 3: the ldir bits -------------------------------------------------
 
-// marked1 of quad t at (x, y, z) with even axis a and pair direction d (7 if
-// critical): a facet edge other than its pair was reached in pass 1.
-__device__ __forceinline__ bool marked1(const LGeom& g, const uint32_t* v1, uint32_t x, uint32_t y, uint32_t z,
-                                        uint32_t a, uint32_t d) {
-    uint32_t u1, u2;
-    other_axes(a, u1, u2);
-#pragma unroll
-    for (int k = 0; k < 4; k++) {
-        const uint32_t u = k < 2 ? u1 : u2, other = k < 2 ? u2 : u1;
-        const int sgn = (k & 1) ? -1 : 1;
-        if (d != 7u && (d >> 1) == u && ((d & 1u) != 0u) == (sgn < 0)) continue;  // the pair itself
-        uint32_t ex = x, ey = y, ez = z;
-        axis_step(u, sgn, ex, ey, ez);
-        if (test_bit(v1, rec_index(g, ex, ey, ez, other))) return true;
-    }
-    return false;
-}
-
-__device__ __forceinline__ uint32_t final_ldir(const LGeom& g, const uint8_t* __restrict__ grad, const uint32_t* v1,
-                                               const uint32_t* v2, uint32_t x, uint32_t y, uint32_t z, uint32_t b) {
-    const uint32_t ox = x & 1u, oy = y & 1u, oz = z & 1u;
-    const uint32_t dim = ox + oy + oz;
+// Per vertex v, the marks of its three "+" quads (quad a: even axis a) that
+// the finish pass needs from its neighbours: bit a = marked1 (assigned, not
+// critical, a facet edge other than its pair reached in pass 1); bit 3 + a =
+// marked1 and paired with its "+" hex along a; bit 6 + a = marked1 and paired
+// with its "-" hex.  The facet edges of quad a lie at v (odd axis u) and at
+// v + e_u' (odd axis u) for its two odd axes u, u'.
+template <int kA>
+__device__ __forceinline__ uint32_t quad_marks(uint32_t b, uint32_t e_v, uint32_t e_n1, uint32_t e_n2) {
+    // odd axes u1 < u2 of the quad
+    constexpr int u1 = kA == 0 ? 1 : 0, u2 = kA == 2 ? 1 : 2;
     const uint32_t d = (b >> 2) & 7u;
-    const bool assigned = b & 1u;
-    if (dim == 1) {
-        const uint32_t o = ox ? 0u : (oy ? 1u : 2u);
-        return test_bit(v1, rec_index(g, x, y, z, o)) ? 2u : 3u;
-    }
-    if (dim == 2) {
-        const uint32_t a = !ox ? 0u : (!oy ? 1u : 2u);
-        if (test_bit(v2, rec_index(g, x, y, z, a))) return 1u;
-        if (assigned && d != 7u && marked1(g, v1, x, y, z, a, d)) return 2u;
-        return 3u;
-    }
-    if (dim == 3) {
-        if (d == 7u) return (b >> 5) & 7u;  // critical (0, preset by the lower-star pass), or unassigned
-        if (assigned) {
-#pragma unroll
-            for (uint32_t u = 0; u < 3; u++) {
-#pragma unroll
-                for (int s = 0; s < 2; s++) {
-                    const int sgn = s ? -1 : 1;
-                    if ((d >> 1) == u && ((d & 1u) != 0u) == (sgn < 0)) continue;  // the pair itself
-                    uint32_t qx = x, qy = y, qz = z;
-                    axis_step(u, sgn, qx, qy, qz);
-                    if (test_bit(v2, rec_index(g, qx, qy, qz, u))) return 1u;
-                }
-            }
-            // walked in pass 1 as the partner of a marked quad
-            uint32_t tx = x, ty = y, tz = z;
-            apply_dir(d, tx, ty, tz);
-            const uint32_t bt = grad[cell_id(g, tx, ty, tz)];
-            const uint32_t dt = (bt >> 2) & 7u;
-            if ((bt & 1u) && dt == (d ^ 1u) && marked1(g, v1, tx, ty, tz, d >> 1, dt)) return 2u;
-        }
-        return 3u;
-    }
-    return 3u;
+    if (!(b & 1u) || d == 7u) return 0u;
+    // facets: -u1: edge of v with odd axis u2; +u1: edge of v + e_u1, odd axis u2;
+    //         -u2: edge of v with odd axis u1; +u2: edge of v + e_u2, odd axis u1
+    bool m = false;
+    if (d != 2u * u1 + 1u && ((e_v >> u2) & 1u)) m = true;
+    if (d != 2u * u1 && ((e_n1 >> u2) & 1u)) m = true;
+    if (d != 2u * u2 + 1u && ((e_v >> u1) & 1u)) m = true;
+    if (d != 2u * u2 && ((e_n2 >> u1) & 1u)) m = true;
+    if (!m) return 0u;
+    return (1u << kA) | (d == 2u * kA ? 1u << (3 + kA) : 0u) | (d == 2u * kA + 1u ? 1u << (6 + kA) : 0u);
 }
 
-// One thread per vertex: the cells of its "+" octant (vertex bytes are
-// rewritten unchanged, so a wave writes whole row segments).
 __global__ void __launch_bounds__(kThreads)
-ldir_finish_kernel(uint8_t* __restrict__ grad, LGeom g, const uint32_t* __restrict__ v1,
-                   const uint32_t* __restrict__ v2) {
+ldir_marks_kernel(const uint8_t* __restrict__ grad, LGeom g, const uint32_t* __restrict__ v1,
+                  uint16_t* __restrict__ marks) {
+    const uint32_t Zv = (g.Mz + 1) >> 1;
+    const uint32_t ox = 1u, oy = g.Mx, oz = g.sxy;
+    const uint32_t sy = g.X, sz = g.X * g.Y;
     for (uint32_t v = blockIdx.x * kThreads + threadIdx.x; v < g.V; v += gridDim.x * kThreads) {
         const uint32_t q = v / g.X, k = q / g.Y;
         const uint32_t i = v - q * g.X, j = q - k * g.Y;
-#pragma unroll
-        for (uint32_t o = 0; o < 8; o++) {
-            const uint32_t x = 2 * i + (o & 1u), y = 2 * j + ((o >> 1) & 1u), z = 2 * k + (o >> 2);
-            if (x >= g.Mx || y >= g.My || z >= g.Mz) continue;
-            const uint32_t c = cell_id(g, x, y, z);
+        const bool px = i + 1 < g.X, py = j + 1 < g.Y, pz = k + 1 < Zv;
+        const uint32_t c0 = 2u * i + oy * 2u * j + oz * 2u * k;
+        const uint32_t e_v = bits3(v1, v);
+        const uint32_t e_x = px ? bits3(v1, v + 1u) : 0u, e_y = py ? bits3(v1, v + sy) : 0u;
+        const uint32_t e_z = pz ? bits3(v1, v + sz) : 0u;
+        uint32_t m = 0;
+        if (py && pz) m |= quad_marks<0>(grad[c0 + oy + oz], e_v, e_y, e_z);
+        if (px && pz) m |= quad_marks<1>(grad[c0 + ox + oz], e_v, e_x, e_z);
+        if (px && py) m |= quad_marks<2>(grad[c0 + ox + oy], e_v, e_x, e_y);
+        marks[v] = (uint16_t)m;
+    }
+}
+
+__device__ __forceinline__ uint8_t with_ldir(uint32_t b, uint32_t l) { return (uint8_t)((b & 0x1Fu) | (l << 5)); }
+
+// One thread per vertex: the cells of its "+" octant (the vertex byte is
+// rewritten unchanged, so a wave writes whole row segments).
+__global__ void __launch_bounds__(kThreads)
+ldir_finish_kernel(uint8_t* __restrict__ grad, LGeom g, const uint32_t* __restrict__ v1,
+                   const uint32_t* __restrict__ v2, const uint16_t* __restrict__ marks) {
+    const uint32_t Zv = (g.Mz + 1) >> 1;
+    const uint32_t ox = 1u, oy = g.Mx, oz = g.sxy;
+    const uint32_t sy = g.X, sz = g.X * g.Y;
+    for (uint32_t v = blockIdx.x * kThreads + threadIdx.x; v < g.V; v += gridDim.x * kThreads) {
+        const uint32_t q = v / g.X, k = q / g.Y;
+        const uint32_t i = v - q * g.X, j = q - k * g.Y;
+        const bool px = i + 1 < g.X, py = j + 1 < g.Y, pz = k + 1 < Zv;
+        const uint32_t c0 = 2u * i + oy * 2u * j + oz * 2u * k;
+        const uint32_t e_v = bits3(v1, v), q_v = bits3(v2, v);
+        const uint32_t m_v = marks[v];
+        grad[c0] = grad[c0];
+        if (px) { const uint32_t c = c0 + ox; grad[c] = with_ldir(grad[c], (e_v & 1u) ? 2u : 3u); }
+        if (py) { const uint32_t c = c0 + oy; grad[c] = with_ldir(grad[c], (e_v & 2u) ? 2u : 3u); }
+        if (pz) { const uint32_t c = c0 + oz; grad[c] = with_ldir(grad[c], (e_v & 4u) ? 2u : 3u); }
+        // quads: reached in pass 2 -> 1; marked in pass 1 -> 2
+        if (py && pz) {
+            const uint32_t c = c0 + oy + oz;
+            grad[c] = with_ldir(grad[c], (q_v & 1u) ? 1u : ((m_v & 1u) ? 2u : 3u));
+        }
+        if (px && pz) {
+            const uint32_t c = c0 + ox + oz;
+            grad[c] = with_ldir(grad[c], (q_v & 2u) ? 1u : ((m_v & 2u) ? 2u : 3u));
+        }
+        if (px && py) {
+            const uint32_t c = c0 + ox + oy;
+            grad[c] = with_ldir(grad[c], (q_v & 4u) ? 1u : ((m_v & 4u) ? 2u : 3u));
+        }
+        if (px && py && pz) {
+            const uint32_t c = c0 + ox + oy + oz;
             const uint32_t b = grad[c];
-            const uint32_t l = o == 0 ? (b >> 5) : final_ldir(g, grad, v1, v2, x, y, z, b);
-            grad[c] = (uint8_t)((b & 0x1Fu) | (l << 5));
+            const uint32_t d = (b >> 2) & 7u;
+            if (d != 7u) {
+                uint32_t l = 3u;
+                if (b & 1u) {
+                    // facet quads: -x, +x, -y, +y, -z, +z; the pair excluded
+                    const uint32_t q_x = bits3(v2, v + 1u), q_y = bits3(v2, v + sy), q_z = bits3(v2, v + sz);
+                    const bool f[6] = {(q_x & 1u) != 0u, (q_v & 1u) != 0u, (q_y & 2u) != 0u,
+                                       (q_v & 2u) != 0u, (q_z & 4u) != 0u, (q_v & 4u) != 0u};
+                    bool m2 = false;
+#pragma unroll
+                    for (uint32_t t = 0; t < 6; t++)
+                        if (t != d && f[t]) m2 = true;
+                    if (m2) {
+                        l = 1u;
+                    } else {
+                        // the pair quad, marked in pass 1 and paired back with this hex
+                        const uint32_t a = d >> 1;
+                        const uint32_t nb = (d & 1u) ? v : v + (a == 0 ? 1u : (a == 1 ? sy : sz));
+                        const uint32_t mq = (d & 1u) ? m_v : (uint32_t)marks[nb];
+                        if ((mq >> ((d & 1u) ? 3u + a : 6u + a)) & 1u) l = 2u;
+                    }
+                }
+                grad[c] = with_ldir(b, l);
+            }
         }
     }
 }
@@ -489,17 +578,19 @@ uint32_t grid_of(uint64_t n, uint32_t cap) {
 
 size_t ldir_scratch_bytes(const Dims& d, LdirScratch* s) {
     const uint64_t n = 3ull * d.verts;
-    const uint64_t words = (n + 31) / 32;
+    const uint64_t words = (n + 31) / 32 + 1;  // one word of padding: bits3 reads a word pair
     const size_t r1 = (2 * n + 255) & ~(size_t)255, r2 = (n + 255) & ~(size_t)255;
     const size_t bw = ((words * 4) + 255) & ~(size_t)255;
+    const size_t mk = (2 * d.verts + 255) & ~(size_t)255;
     if (s) {
         s->rec1_off = 0;
         s->rec2_off = r1;
         s->bits1_off = r1 + r2;
         s->bits2_off = r1 + r2 + bw;
+        s->marks_off = r1 + r2 + 2 * bw;
         s->bit_words = (uint32_t)words;
     }
-    return r1 + r2 + 2 * bw;
+    return r1 + r2 + 2 * bw + mk;
 }
 
 hipError_t launch_ldir_records(const Dims& d, const uint8_t* grad, void* scratch, const LdirScratch& s,
@@ -535,9 +626,13 @@ hipError_t launch_ldir_sweep(const Dims& d, const uint8_t* grad, void* scratch, 
 
 hipError_t launch_ldir_finish(const Dims& d, uint8_t* grad, void* scratch, const LdirScratch& s, hipStream_t st) {
     uint8_t* base = static_cast<uint8_t*>(scratch);
+    const uint32_t* v1 = reinterpret_cast<const uint32_t*>(base + s.bits1_off);
+    const uint32_t* v2 = reinterpret_cast<const uint32_t*>(base + s.bits2_off);
+    uint16_t* marks = reinterpret_cast<uint16_t*>(base + s.marks_off);
+    hipLaunchKernelGGL(ldir_marks_kernel, dim3(grid_of(d.verts, 16384)), dim3(kThreads), 0, st, grad, make_geom(d),
+                       v1, marks);
     hipLaunchKernelGGL(ldir_finish_kernel, dim3(grid_of(d.verts, 16384)), dim3(kThreads), 0, st, grad, make_geom(d),
-                       reinterpret_cast<const uint32_t*>(base + s.bits1_off),
-                       reinterpret_cast<const uint32_t*>(base + s.bits2_off));
+                       v1, v2, marks);
     return hipGetLastError();
 }
 

@@ -37,23 +37,39 @@ struct Brick {
     static constexpr int kTileN = kTX * kTY * kTZ;
 };
 
+// Work-groups are dealt to the XCDs round robin (work-group i runs on XCD
+// i % 8), each XCD with its own L2.  The brick of work-group i is chosen so
+// that every XCD takes a contiguous run of bricks in x-fastest order: bricks
+// that share halo values and partly written cache lines of the .grad then
+// meet in the same L2.  A bijection of [0, n).
+constexpr uint32_t kXcds = 8;
+__device__ __forceinline__ uint32_t xcd_brick(uint32_t b, uint32_t n) {
+    const uint32_t per = n / kXcds, rem = n % kXcds, xcd = b % kXcds, k = b / kXcds;
+    return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
+}
+
 template <bool kConstrained, int kBX, int kBY, int kBZ>
 __global__ void __launch_bounds__(kThreads)
 lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
              const int8_t* __restrict__ restriction, int X, int Y, int Z, uint32_t Mx, uint32_t My,
-             MList list1, MList list2, uint32_t* __restrict__ stats, uint32_t* __restrict__ overflow) {
+             MList list1, MList list2, uint32_t* __restrict__ stats, uint32_t* __restrict__ overflow, int nbx,
+             int nby, int nbz) {
     using B = Brick<kBX, kBY, kBZ>;
     constexpr int kTX = B::kTX, kTY = B::kTY, kTileN = B::kTileN;
     __shared__ uint64_t tile[kTileN];  // (order key << 32 | vertex id) of the halo'd brick
     __shared__ int s_off[27];  // tile offset of star vertex t
+    __shared__ int s_cofs[27];  // .grad offset of star cell t from the centre's cell
     __shared__ uint32_t s_cnt[4];  // per block: critical 1-cells, 2-cells, 0-cells, 3-cells
     __shared__ MSlot s_slot[2];
     __shared__ uint32_t s_mask[kThreads];     // lower star of each vertex of the brick
     __shared__ uint16_t s_order[kThreads];    // brick vertices by lower-star size
     __shared__ uint32_t s_bucket[32];
     __shared__ uint32_t s_res[4][kThreads];   // expansion result by brick vertex: pair-code bits, critical
-    const int x0 = blockIdx.x * kBX, y0 = blockIdx.y * kBY, z0 = blockIdx.z * kBZ;
-    const int flat = threadIdx.x + kBX * (threadIdx.y + kBY * threadIdx.z);
+    const uint32_t brick = xcd_brick(blockIdx.x, (uint32_t)nbx * nby * nbz);
+    const int x0 = (int)(brick % nbx) * kBX, y0 = (int)((brick / nbx) % nby) * kBY;
+    const int z0 = (int)(brick / ((uint32_t)nbx * nby)) * kBZ;
+    const int flat = threadIdx.x;
+    const int tx = flat % kBX, ty = (flat / kBX) % kBY, tz = flat / (kBX * kBY);
     for (int i = flat; i < kTileN; i += kThreads) {
         const int ix = i % kTX, iy = (i / kTX) % kTY, iz = i / (kTX * kTY);
         int gx = x0 - 1 + ix, gy = y0 - 1 + iy, gz = z0 - 1 + iz;
@@ -64,7 +80,10 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
         tile[i] = ((uint64_t)float_key(values[id]) << 32) | id;
     }
     if (flat < 4) s_cnt[flat] = 0;
-    if (flat < 27) s_off[flat] = tid_x(flat) + kTX * (tid_y(flat) + kTY * tid_z(flat));
+    if (flat < 27) {
+        s_off[flat] = tid_x(flat) + kTX * (tid_y(flat) + kTY * tid_z(flat));
+        s_cofs[flat] = tid_x(flat) + (int)Mx * (tid_y(flat) + (int)My * tid_z(flat));
+    }
     if (flat < 32) s_bucket[flat] = 0;
     __syncthreads();
 
@@ -75,10 +94,10 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
     // stars of similar size.  Any lane can expand any vertex of the brick,
     // since all keys are in the tile.
     {
-        const int ox = x0 + threadIdx.x, oy = y0 + threadIdx.y, oz = z0 + threadIdx.z;
+        const int ox = x0 + tx, oy = y0 + ty, oz = z0 + tz;
         uint32_t own = 0;
         if (ox < X && oy < Y && oz < Z) {
-            const int obase = (threadIdx.x + 1) + kTX * ((threadIdx.y + 1) + kTY * (threadIdx.z + 1));
+            const int obase = (tx + 1) + kTX * ((ty + 1) + kTY * (tz + 1));
             own = lower_star_mask(FixedKeys64<kTX, kTX * kTY>{tile, obase}, ox, oy, oz, X, Y, Z);
         }
         s_mask[flat] = own;
@@ -105,14 +124,14 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
     const uint32_t L = s_mask[me];
     const bool active = L != 0u;
     uint32_t crit = 0;
-    uint64_t vcell = 0;
+    uint32_t vcell = 0;
     if (active) {
     const int base = (mx + 1) + kTX * ((my + 1) + kTY * (mz + 1));
     const TableKeys64 keys{tile + base, s_off};
     uint32_t bx, by, bz;
     star_boundary(x, y, z, X, Y, Z, bx, by, bz);
 
-    vcell = 2ull * x + (uint64_t)Mx * (2ull * y + (uint64_t)My * 2ull * z);
+    vcell = 2u * x + Mx * (2u * y + My * 2u * z);  // cells < 2^32
     StarResult r;
     if (!kConstrained) {
         steepest_expand(keys, L, bx, by, bz, r);
@@ -135,8 +154,8 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
     // hold neighbouring vertices, so their byte stores share cache lines
     {
         const uint32_t Lh = s_mask[flat];
-        const int hx = x0 + threadIdx.x, hy = y0 + threadIdx.y, hz = z0 + threadIdx.z;
-        vcell = 2ull * hx + (uint64_t)Mx * (2ull * hy + (uint64_t)My * 2ull * hz);
+        const int hx = x0 + tx, hy = y0 + ty, hz = z0 + tz;
+        vcell = 2u * hx + Mx * (2u * hy + My * 2u * hz);
         if (Lh) {
             StarResult rh;
             rh.c0 = s_res[0][flat];
@@ -146,11 +165,9 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
             uint32_t hbx, hby, hbz;
             star_boundary(hx, hy, hz, X, Y, Z, hbx, hby, hbz);
             const uint32_t honb = hbx | hby | hbz;
-#pragma unroll
-            for (int t = 0; t < 27; t++) {
-                if (!((Lh >> t) & 1u)) continue;
-                const int64_t off = (int64_t)tid_x(t) + (int64_t)Mx * ((int64_t)tid_y(t) + (int64_t)My * tid_z(t));
-                grad[(int64_t)vcell + off] = star_cell_byte(rh, honb, t);
+            for (uint32_t m = Lh; m; m &= m - 1) {
+                const int t = __builtin_ctz(m);
+                grad[vcell + (uint32_t)s_cofs[t]] = star_cell_byte(rh, honb, t);
             }
             crit = rh.crit;
         }
@@ -167,7 +184,7 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
         if (crit & kDim3) atomicAdd(&s_cnt[3], (uint32_t)__builtin_popcount(crit & kDim3));
     }
     __syncthreads();
-    const int sub = (int)((blockIdx.x + gridDim.x * (blockIdx.y + gridDim.y * blockIdx.z)) % kSubs);
+    const int sub = (int)(blockIdx.x % kSubs);
     if (flat == 0) {
         const uint32_t c1 = s_cnt[0], c2 = s_cnt[1];
         s_slot[0] = c1 ? mlist_reserve(list1, sub, c1) : MSlot{0u, 0u, 0u};
@@ -183,8 +200,7 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
         bool lost = false;
         for (uint32_t m = crit & (kDim1 | kDim2); m; m &= m - 1) {
             const int t = __builtin_ctz(m);
-            const int64_t off = (int64_t)tid_x(t) + (int64_t)Mx * ((int64_t)tid_y(t) + (int64_t)My * tid_z(t));
-            const uint32_t cell = (uint32_t)((int64_t)vcell + off);
+            const uint32_t cell = vcell + (uint32_t)s_cofs[t];
             if ((kDim1 >> t) & 1u) lost |= !mlist_store(list1, sub, r1, j1++, cell);
             else lost |= !mlist_store(list2, sub, r2, j2++, cell);
         }
@@ -198,14 +214,14 @@ namespace {
 
 template <int kBX, int kBY, int kBZ>
 void launch_shape(const Dims& d, const LstarBuffers& b, hipStream_t s) {
-    const dim3 block(kBX, kBY, kBZ);
-    const dim3 grid((d.X + kBX - 1) / kBX, (d.Y + kBY - 1) / kBY, (d.Z + kBZ - 1) / kBZ);
+    const int nbx = (d.X + kBX - 1) / kBX, nby = (d.Y + kBY - 1) / kBY, nbz = (d.Z + kBZ - 1) / kBZ;
+    const dim3 grid((uint32_t)nbx * nby * nbz), block(kThreads);
     if (b.restriction)
         hipLaunchKernelGGL((lstar_kernel<true, kBX, kBY, kBZ>), grid, block, 0, s, b.values, b.grad, b.restriction,
-                           d.X, d.Y, d.Z, d.Mx, d.My, *b.list1, *b.list2, b.stats, b.overflow);
+                           d.X, d.Y, d.Z, d.Mx, d.My, *b.list1, *b.list2, b.stats, b.overflow, nbx, nby, nbz);
     else
         hipLaunchKernelGGL((lstar_kernel<false, kBX, kBY, kBZ>), grid, block, 0, s, b.values, b.grad, b.restriction,
-                           d.X, d.Y, d.Z, d.Mx, d.My, *b.list1, *b.list2, b.stats, b.overflow);
+                           d.X, d.Y, d.Z, d.Mx, d.My, *b.list1, *b.list2, b.stats, b.overflow, nbx, nby, nbz);
 }
 
 }  // namespace

@@ -238,6 +238,72 @@ MSC_HD void expand_subset(uint32_t S, const Keys& k, StarResult& r) {
 }
 
 
+// The relaxed expansion of the four most common interior lower stars, in
+// closed form (on smoothed fields about three quarters of all vertices):
+//   the vertex alone (a minimum): critical;
+//   the vertex and one edge: the pair;
+//   a quadrant (two perpendicular edges and their quad): the vertex pairs
+//     with the edge of the lower other vertex, the other edge with the quad;
+//   an octant (three edges, their three quads and the hex): the vertex pairs
+//     with the steepest edge e*; of the other two edges, the first in star
+//     order, a, has one candidate, q(e*, a); the second, b, then has two,
+//     q(e*, b) and q(a, b), and takes the one with the lower lowest vertex
+//     (equal: the earlier in cofacet order); the quad left pairs with the hex.
+// These are the steps expand_subset takes on those shapes (every edge of an
+// interior lower star has the centre as its highest vertex, so its lowest
+// vertex is its other one, and vertex keys never tie).  Returns false for
+// any other shape.
+template <class Keys>
+MSC_HD bool expand_common(uint32_t L, const Keys& k, StarResult& r) {
+    const uint32_t E = L & kDim1;
+    if (!E) {
+        if (L != kCenter) return false;
+        r.set_critical(kCenter);
+        return true;
+    }
+    const int ne = __builtin_popcount(E);
+    const int e1 = __builtin_ctz(E);
+    if (ne == 1) {
+        if (L != (kCenter | E)) return false;
+        r.set_pair(13, e1);
+        return true;
+    }
+    if (ne == 2) {
+        const int e2 = 31 - __builtin_clz(E);
+        const int q = e1 + e2 - 13;  // the quad spanned by the two edges
+        if (q == 13 || L != (kCenter | E | (1u << q))) return false;
+        const bool first = vkey(k, e1) < vkey(k, e2);
+        r.set_pair(13, first ? e1 : e2);
+        r.set_pair(first ? e2 : e1, q);
+        return true;
+    }
+    const uint32_t H = L & kDim3;
+    if (ne != 3 || !H || (H & (H - 1))) return false;
+    const int h = __builtin_ctz(H);
+    const int hx = comp_x(h), hy = comp_y(h), hz = comp_z(h);
+    const uint32_t oct = (kX0 | (hx > 0 ? kXP : kXM)) & (kY0 | (hy > 0 ? kYP : kYM)) & (kZ0 | (hz > 0 ? kZP : kZM));
+    if (L != oct) return false;
+    // the three edges in star order (ascending bits)
+    const int e2 = __builtin_ctz(E & (E - 1)), e3 = 31 - __builtin_clz(E);
+    const uint64_t k1 = vkey(k, e1), k2 = vkey(k, e2), k3 = vkey(k, e3);
+    int es, a, b;
+    if (k1 < k2 && k1 < k3) { es = e1; a = e2; b = e3; }
+    else if (k2 < k3) { es = e2; a = e1; b = e3; }
+    else { es = e3; a = e1; b = e2; }
+    r.set_pair(13, es);
+    r.set_pair(a, es + a - 13);
+    const int c1 = es + b - 13, c2 = a + b - 13;  // q(e*, b), q(a, b)
+    const uint64_t l1 = quad_lowest(k, c1), l2 = quad_lowest(k, c2);
+    auto rank = [b](int c) {
+        const int d = c - b;
+        return d == 1 ? 0 : (d == -1 ? 1 : (d == 3 ? 2 : (d == -3 ? 3 : (d == 9 ? 4 : 5))));
+    };
+    const bool take1 = l1 < l2 || (l1 == l2 && rank(c1) < rank(c2));
+    r.set_pair(b, take1 ? c1 : c2);
+    r.set_pair(take1 ? c2 : c1, h);
+    return true;
+}
+
 // Lower star of vertex (x,y,z) of an X*Y*Z grid as a star mask: the cells
 // whose every vertex other than the centre comes before it, restricted to
 // the domain (gi_fast_robins_noalloc.h:359-366, gi_modified_robins.h:1293-1297).
@@ -269,7 +335,7 @@ template <class Keys>
 MSC_HD void steepest_expand(const Keys& k, uint32_t L, uint32_t bx, uint32_t by, uint32_t bz, StarResult& r) {
     const uint32_t onb = bx | by | bz;
     if (!onb) {
-        expand_subset(L, k, r);
+        if (!expand_common(L, k, r)) expand_subset(L, k, r);
         return;
     }
     const uint32_t odd = bx ^ by ^ bz, two = (bx & by) | (bx & bz) | (by & bz);

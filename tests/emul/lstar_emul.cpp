@@ -50,3 +50,37 @@ extern "C" int emul_lstar_prepass(const float* f, long long X, long long Y, long
             }
     return 0;
 }
+
+// Every interior vertex: the closed-form expansion of the common lower-star
+// shapes (expand_common) against the general expansion (expand_subset).
+// counts[0]: vertices the closed form took, counts[1]: of those, results
+// that differ.
+extern "C" int emul_check_common(const float* f, long long X, long long Y, long long Z, long long* counts) {
+    const int64_t PX = X + 2, PY = Y + 2, PZ = Z + 2;
+    std::vector<uint32_t> keys((size_t)(PX * PY * PZ));
+    for (int64_t z = 0; z < PZ; z++)
+        for (int64_t y = 0; y < PY; y++)
+            for (int64_t x = 0; x < PX; x++) {
+                int64_t gx = x - 1, gy = y - 1, gz = z - 1;
+                gx = gx < 0 ? 0 : (gx >= X ? X - 1 : gx);
+                gy = gy < 0 ? 0 : (gy >= Y ? Y - 1 : gy);
+                gz = gz < 0 ? 0 : (gz >= Z ? Z - 1 : gz);
+                keys[(size_t)(x + PX * (y + PY * z))] = float_key(f[gx + X * (gy + Y * gz)]);
+            }
+    long long taken = 0, differ = 0;
+#pragma omp parallel for reduction(+ : taken, differ) schedule(dynamic, 1)
+    for (long long z = 1; z < Z - 1; z++)
+        for (long long y = 1; y < Y - 1; y++)
+            for (long long x = 1; x < X - 1; x++) {
+                StridedKeys k{keys.data(), (x + 1) + PX * ((y + 1) + PY * (z + 1)), PX, PX * PY};
+                const uint32_t L = lower_star_mask(k, (int)x, (int)y, (int)z, (int)X, (int)Y, (int)Z);
+                StarResult a, b;
+                if (!expand_common(L, k, a)) continue;
+                taken++;
+                expand_subset(L, k, b);
+                if (a.c0 != b.c0 || a.c1 != b.c1 || a.c2 != b.c2 || a.crit != b.crit) differ++;
+            }
+    counts[0] = taken;
+    counts[1] = differ;
+    return 0;
+}

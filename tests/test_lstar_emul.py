@@ -32,3 +32,23 @@ def test_emulated_constrained_pass_matches_oracle(seed):
     got = emul_lib.lstar_prepass(f, lab)
     expect = emul_lib.expected_prepass(ref, shape)
     assert (got == expect).all(), int((got != expect).sum())
+
+
+@pytest.mark.parametrize("kind", ["smooth", "noise", "ties", "plateau"])
+def test_closed_form_common_shapes_equal_general_expansion(kind):
+    # octants, quadrants, single edges and minima: the closed form must give
+    # exactly what the general loops give, ties between quads' lowest
+    # vertices included (plateaus and rounded noise make many)
+    shape = (40, 44, 48)
+    s = synth.Stream(77)
+    if kind == "smooth":
+        f = synth.smoothed_noise(shape, 77, 2.0)
+    elif kind == "noise":
+        f = s.normal(int(np.prod(shape))).reshape(shape).astype(np.float32)
+    elif kind == "ties":
+        f = np.floor(s.normal(int(np.prod(shape))).reshape(shape) * 2).astype(np.float32)
+    else:
+        f = np.floor(synth.smoothed_noise(shape, 78, 3.0) * 20).astype(np.float32)
+    taken, differ = emul_lib.check_common(f)
+    assert taken > 0.2 * (shape[0] - 2) * (shape[1] - 2) * (shape[2] - 2)
+    assert differ == 0, (taken, differ)
