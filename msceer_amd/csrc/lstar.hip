@@ -1,11 +1,18 @@
 // Lower-star discrete gradient on gfx950.
 //
-// One lane owns one vertex.  A work-group stages a halo'd brick of vertex
-// values in LDS as order-preserving 32-bit keys, every lane builds its lower
-// star as a 27-bit mask over its star (see star.hpp) and runs the relaxed
-// Robins homotopy expansion with mask algebra, then writes the bytes of the
-// cells it owns.  Cell ownership is by highest vertex, so each .grad byte is
-// written by exactly one lane and no pre-clear of the output is needed.
+// A work-group owns a brick of vertices.  It stages the halo'd brick's
+// values in LDS as 64-bit order keys, every lane builds the lower star of one
+// vertex as a 27-bit mask over its star (see star.hpp), and the brick's
+// vertices are dealt to the lanes by shape: product lower stars (a minimum,
+// an edge, a quadrant, an octant -- three in four vertices of a smooth
+// field) take a closed form that writes their bytes at once; the others run
+// the relaxed Robins expansion with mask algebra, grouped by size so that a
+// wave's loops run about as long as each of its lanes needs.  The bytes go to
+// an LDS image of the cells around the brick, which the work-group then
+// copies to the .grad with contiguous byte stores, collecting the critical
+// cells on the way.  Cell ownership is by highest vertex, so each .grad byte
+// is written by exactly one work-group and no pre-clear of the output is
+// needed.
 //
 // Semantics restated (reference file:line):
 //   order:      gi_regular_grid_trilinear_function.h:309-314 (value, then id)
@@ -30,11 +37,26 @@ constexpr int kThreads = 256;  // vertices per brick, any of the shapes below
 // Brick shapes (vertices along x, y, z); the autotuner picks one per volume
 // shape: wide bricks coalesce best, narrow ones waste fewer lanes when the
 // x extent is small or not a multiple of the brick width.
+//   tile:  the brick's vertices with a one-vertex halo (kTX * kTY * kTZ keys)
+//   stage: the cells whose highest vertex can be a brick vertex, cell
+//          coordinates 2 x0 - 1 .. 2 (x0 + kBX) - 1 along x and likewise for
+//          y and z; rows padded to whole dwords.  The outer layers are shared
+//          with the neighbouring bricks, the rest is the brick's alone.
 template <int kBX, int kBY, int kBZ>
 struct Brick {
     static_assert(kBX * kBY * kBZ == kThreads, "256 lanes");
     static constexpr int kTX = kBX + 2, kTY = kBY + 2, kTZ = kBZ + 2;
     static constexpr int kTileN = kTX * kTY * kTZ;
+    static constexpr int kCX = 2 * kBX + 1, kCY = 2 * kBY + 1, kCZ = 2 * kBZ + 1;
+    static constexpr int kRowW = (kCX + 3) / 4;  // dwords per staged row
+    static constexpr int kRows = kCY * kCZ;
+    static constexpr int kStageW = kRowW * kRows;
+    // cells of dimension 1 and 2 in the stage (odd coordinates: kB + 1 per
+    // axis, even: kB), the room their critical-cell lists need
+    static constexpr int kN1 = (kBX + 1) * kBY * kBZ + kBX * (kBY + 1) * kBZ + kBX * kBY * (kBZ + 1);
+    static constexpr int kN2 = (kBX + 1) * (kBY + 1) * kBZ + (kBX + 1) * kBY * (kBZ + 1) + kBX * (kBY + 1) * (kBZ + 1);
+    // the lists reuse the tile's LDS once the expansions are done
+    static constexpr int kSmem = kTileN > (kN1 + kN2 + 1) / 2 ? kTileN : (kN1 + kN2 + 1) / 2;
 };
 
 // Work-groups are dealt to the XCDs round robin (work-group i runs on XCD
@@ -48,6 +70,18 @@ __device__ __forceinline__ uint32_t xcd_brick(uint32_t b, uint32_t n) {
     return xcd < rem ? xcd * (per + 1) + k : rem * (per + 1) + (xcd - rem) * per + k;
 }
 
+// Keys and byte sink of one lane's product star: tile and stage offsets from
+// the lane's vertex and its cell.
+template <int kTX, int kTXY, int kRow, int kRowPlane>
+struct LaneStar {
+    const uint64_t* kc;
+    uint8_t* sc;
+    static constexpr int ts[3] = {1, kTX, kTXY};
+    static constexpr int cs[3] = {1, kRow, kRowPlane};
+    __device__ __forceinline__ uint64_t key(int o) const { return kc[o]; }
+    __device__ __forceinline__ void put(int o, uint32_t b) const { sc[o] = (uint8_t)b; }
+};
+
 template <bool kConstrained, int kBX, int kBY, int kBZ>
 __global__ void __launch_bounds__(kThreads)
 lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
@@ -55,56 +89,80 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
              MList list1, MList list2, uint32_t* __restrict__ stats, uint32_t* __restrict__ overflow, int nbx,
              int nby, int nbz) {
     using B = Brick<kBX, kBY, kBZ>;
-    constexpr int kTX = B::kTX, kTY = B::kTY, kTileN = B::kTileN;
-    __shared__ uint64_t tile[kTileN];  // (order key << 32 | vertex id) of the halo'd brick
-    __shared__ int s_off[27];  // tile offset of star vertex t
-    __shared__ int s_cofs[27];  // .grad offset of star cell t from the centre's cell
-    __shared__ uint32_t s_cnt[4];  // per block: critical 1-cells, 2-cells, 0-cells, 3-cells
-    __shared__ MSlot s_slot[2];
+    constexpr int kTX = B::kTX, kTY = B::kTY, kTXY = kTX * kTY, kTileN = B::kTileN;
+    constexpr int kCY = B::kCY, kRowW = B::kRowW, kRow = 4 * kRowW, kRows = B::kRows, kStageW = B::kStageW;
+    __shared__ uint64_t smem[B::kSmem];       // tile: (order key << 32 | vertex id); later the lists
+    __shared__ uint32_t stage[kStageW];       // .grad bytes of the cells around the brick
+    __shared__ uint32_t s_row[kRows];         // .grad index of stage row r at x cell 2 x0
+    __shared__ int s_off[27];                 // tile offset of star vertex t
+    __shared__ int s_soff[27];                // stage offset of star cell t
     __shared__ uint32_t s_mask[kThreads];     // lower star of each vertex of the brick
-    __shared__ uint16_t s_order[kThreads];    // brick vertices by lower-star size
+    __shared__ uint8_t s_shape[kThreads];     // its product shape, or kNotProduct
+    __shared__ uint16_t s_order[kThreads];    // brick vertices by shape and size
     __shared__ uint32_t s_bucket[32];
-    __shared__ uint32_t s_res[4][kThreads];   // expansion result by brick vertex: pair-code bits, critical
+    __shared__ uint32_t s_cnt[8];             // critical cells by dimension; list fills
+    __shared__ MSlot s_slot[2];
+    uint64_t* const tile = smem;
+    uint8_t* const st8 = reinterpret_cast<uint8_t*>(stage);
+
     const uint32_t brick = xcd_brick(blockIdx.x, (uint32_t)nbx * nby * nbz);
     const int x0 = (int)(brick % nbx) * kBX, y0 = (int)((brick / nbx) % nby) * kBY;
     const int z0 = (int)(brick / ((uint32_t)nbx * nby)) * kBZ;
     const int flat = threadIdx.x;
     const int tx = flat % kBX, ty = (flat / kBX) % kBY, tz = flat / (kBX * kBY);
+    // every vertex of the brick and its whole star inside the domain
+    const bool inner = x0 >= 1 && y0 >= 1 && z0 >= 1 && x0 + kBX < X && y0 + kBY < Y && z0 + kBZ < Z;
     for (int i = flat; i < kTileN; i += kThreads) {
-        const int ix = i % kTX, iy = (i / kTX) % kTY, iz = i / (kTX * kTY);
+        const int ix = i % kTX, iy = (i / kTX) % kTY, iz = i / kTXY;
         int gx = x0 - 1 + ix, gy = y0 - 1 + iy, gz = z0 - 1 + iz;
-        gx = gx < 0 ? 0 : (gx >= X ? X - 1 : gx);
-        gy = gy < 0 ? 0 : (gy >= Y ? Y - 1 : gy);
-        gz = gz < 0 ? 0 : (gz >= Z ? Z - 1 : gz);
+        if (!inner) {
+            gx = gx < 0 ? 0 : (gx >= X ? X - 1 : gx);
+            gy = gy < 0 ? 0 : (gy >= Y ? Y - 1 : gy);
+            gz = gz < 0 ? 0 : (gz >= Z ? Z - 1 : gz);
+        }
         const uint32_t id = (uint32_t)gx + (uint32_t)X * ((uint32_t)gy + (uint32_t)Y * (uint32_t)gz);
         tile[i] = ((uint64_t)float_key(values[id]) << 32) | id;
     }
-    if (flat < 4) s_cnt[flat] = 0;
+    for (int i = flat; i < kStageW; i += kThreads) stage[i] = 0u;
+    for (int r = flat; r < kRows; r += kThreads) {
+        const uint32_t gy = (uint32_t)(2 * y0 - 1 + r % kCY), gz = (uint32_t)(2 * z0 - 1 + r / kCY);
+        s_row[r] = (uint32_t)(2 * x0) + Mx * (gy + My * gz);  // wraps for rows outside the domain (never stored)
+    }
+    if (flat < 8) s_cnt[flat] = 0;
     if (flat < 27) {
         s_off[flat] = tid_x(flat) + kTX * (tid_y(flat) + kTY * tid_z(flat));
-        s_cofs[flat] = tid_x(flat) + (int)Mx * (tid_y(flat) + (int)My * tid_z(flat));
+        s_soff[flat] = tid_x(flat) + kRow * (tid_y(flat) + kCY * tid_z(flat));
     }
     if (flat < 32) s_bucket[flat] = 0;
     __syncthreads();
 
-    // Lower star of the lane's own vertex (0: outside the volume; a lower star
-    // always holds its vertex).  The expansion loops of a wave run as long as
-    // its largest star needs, so the brick's vertices are then dealt to the
-    // lanes in order of lower-star size (counting sort in LDS): a wave gets
-    // stars of similar size.  Any lane can expand any vertex of the brick,
-    // since all keys are in the tile.
+    // Lower star and shape of the lane's own vertex (0: outside the volume;
+    // a lower star always holds its vertex).  A wave runs each expansion path
+    // that any of its lanes takes, the general loops as long as its largest
+    // star needs, so the vertices are then dealt to the lanes in bucket order
+    // (counting sort in LDS): the four product shapes first, then the rest by
+    // size.  Any lane can expand any vertex of the brick, since all keys are
+    // in the tile.
     {
         const int ox = x0 + tx, oy = y0 + ty, oz = z0 + tz;
-        uint32_t own = 0;
-        if (ox < X && oy < Y && oz < Z) {
-            const int obase = (tx + 1) + kTX * ((ty + 1) + kTY * (tz + 1));
-            own = lower_star_mask(FixedKeys64<kTX, kTX * kTY>{tile, obase}, ox, oy, oz, X, Y, Z);
+        const int obase = (tx + 1) + kTX * ((ty + 1) + kTY * (tz + 1));
+        uint32_t own = 0, shape = kNotProduct;
+        if (inner) {
+            own = lower_star_from(lower_neighbours(FixedKeys64<kTX, kTXY>{tile, obase}), 0u);
+            if (!kConstrained) shape = product_shape(own);
+        } else if (ox < X && oy < Y && oz < Z) {
+            own = lower_star_from(lower_neighbours(FixedKeys64<kTX, kTXY>{tile, obase}),
+                                  star_outside(ox, oy, oz, X, Y, Z));
+            const bool onb = ox == 0 || ox == X - 1 || oy == 0 || oy == Y - 1 || oz == 0 || oz == Z - 1;
+            if (!kConstrained && !onb) shape = product_shape(own);
         }
         s_mask[flat] = own;
-        const uint32_t w = (uint32_t)__builtin_popcount(own);
+        s_shape[flat] = (uint8_t)shape;
+        const uint32_t w = shape != kNotProduct ? (uint32_t)__builtin_popcount(shape >> 3)
+                                                : 4u + (uint32_t)__builtin_popcount(own);
         const uint32_t rank = atomicAdd(&s_bucket[w], 1u);
         __syncthreads();
-        if (flat < 64) {  // exclusive prefix over the 28 sizes (one wave)
+        if (flat < 64) {  // exclusive prefix over the 32 buckets (one wave)
             const uint32_t c = flat < 32 ? s_bucket[flat] : 0u;
             uint32_t incl = c;
 #pragma unroll
@@ -118,92 +176,88 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
         s_order[s_bucket[w] + rank] = (uint16_t)flat;
         __syncthreads();
     }
-    const int me = s_order[flat];
-    const int mx = me % kBX, my = (me / kBX) % kBY, mz = me / (kBX * kBY);
-    const int x = x0 + mx, y = y0 + my, z = z0 + mz;
-    const uint32_t L = s_mask[me];
-    const bool active = L != 0u;
-    uint32_t crit = 0;
-    uint32_t vcell = 0;
-    if (active) {
-    const int base = (mx + 1) + kTX * ((my + 1) + kTY * (mz + 1));
-    const TableKeys64 keys{tile + base, s_off};
-    uint32_t bx, by, bz;
-    star_boundary(x, y, z, X, Y, Z, bx, by, bz);
-
-    vcell = 2u * x + Mx * (2u * y + My * 2u * z);  // cells < 2^32
-    StarResult r;
-    if (!kConstrained) {
-        steepest_expand(keys, L, bx, by, bz, r);
-    } else {
-        const int8_t* rv = restriction + vcell;
-        const int64_t sy = Mx, sz = (int64_t)Mx * My;
-        auto label = [rv, sy, sz](int t) -> int {
-            return rv[tid_x(t) + sy * tid_y(t) + sz * tid_z(t)];
-        };
-        constrained_expand(keys, L, bx, by, bz, label, r);
-    }
-
-    s_res[0][me] = r.c0;
-    s_res[1][me] = r.c1;
-    s_res[2][me] = r.c2;
-    s_res[3][me] = r.crit;
-    }  // active
-    __syncthreads();
-    // each lane writes the owned cells of its own vertex: the lanes of a wave
-    // hold neighbouring vertices, so their byte stores share cache lines
     {
-        const uint32_t Lh = s_mask[flat];
-        const int hx = x0 + tx, hy = y0 + ty, hz = z0 + tz;
-        vcell = 2u * hx + Mx * (2u * hy + My * 2u * hz);
-        if (Lh) {
-            StarResult rh;
-            rh.c0 = s_res[0][flat];
-            rh.c1 = s_res[1][flat];
-            rh.c2 = s_res[2][flat];
-            rh.crit = s_res[3][flat];
-            uint32_t hbx, hby, hbz;
-            star_boundary(hx, hy, hz, X, Y, Z, hbx, hby, hbz);
-            const uint32_t honb = hbx | hby | hbz;
-            for (uint32_t m = Lh; m; m &= m - 1) {
-                const int t = __builtin_ctz(m);
-                grad[vcell + (uint32_t)s_cofs[t]] = star_cell_byte(rh, honb, t);
+        const int me = s_order[flat];
+        const uint32_t L = s_mask[me];
+        if (L) {
+            const int mx = me % kBX, my = (me / kBX) % kBY, mz = me / (kBX * kBY);
+            const int base = (mx + 1) + kTX * ((my + 1) + kTY * (mz + 1));
+            const int cbase = (2 * mx + 1) + kRow * ((2 * my + 1) + kCY * (2 * mz + 1));
+            const uint32_t shape = s_shape[me];
+            if (shape != kNotProduct) {
+                expand_product(shape, LaneStar<kTX, kTXY, kRow, kRow * kCY>{tile + base, st8 + cbase});
+            } else {
+                const int x = x0 + mx, y = y0 + my, z = z0 + mz;
+                const TableKeys64 keys{tile + base, s_off};
+                uint32_t bx, by, bz;
+                star_boundary(x, y, z, X, Y, Z, bx, by, bz);
+                StarResult r;
+                if (!kConstrained) {
+                    steepest_expand(keys, L, bx, by, bz, r);
+                } else {
+                    const int8_t* rv = restriction + (2u * x + Mx * (2u * y + My * 2u * z));
+                    const int64_t sy = Mx, sz = (int64_t)Mx * My;
+                    auto label = [rv, sy, sz](int t) -> int {
+                        return rv[tid_x(t) + sy * tid_y(t) + sz * tid_z(t)];
+                    };
+                    constrained_expand(keys, L, bx, by, bz, label, r);
+                }
+                const uint32_t onb = bx | by | bz;
+                for (uint32_t m = L; m; m &= m - 1) {
+                    const int t = __builtin_ctz(m);
+                    st8[cbase + s_soff[t]] = star_cell_byte(r, onb, t);
+                }
             }
-            crit = rh.crit;
         }
     }
+    __syncthreads();
 
-    // critical cells: statistics and the ldir start lists, reserved once per
-    // block in sub-list blockIdx % kSubs
-    const uint32_t n1 = __builtin_popcount(crit & kDim1), n2 = __builtin_popcount(crit & kDim2);
-    uint32_t o1 = 0, o2 = 0;
-    if (crit) {
-        o1 = atomicAdd(&s_cnt[0], n1);
-        o2 = atomicAdd(&s_cnt[1], n2);
-        if (crit & kDim0) atomicAdd(&s_cnt[2], 1u);
-        if (crit & kDim3) atomicAdd(&s_cnt[3], (uint32_t)__builtin_popcount(crit & kDim3));
+    // Copy the stage out, four cells a lane at a time along the rows; a zero
+    // byte is a cell no brick vertex owns.  Critical cells (pair code 7) are
+    // counted by dimension and the 1- and 2-cells listed for the ldir passes.
+    // Cell parity: stage x offset 4 j + b is .grad x 2 x0 - 1 + 4 j + b (odd
+    // for even b), stage row (j, k) is y 2 y0 - 1 + j, z 2 z0 - 1 + k.
+    uint32_t* const lst = reinterpret_cast<uint32_t*>(smem);
+    for (int ch = flat; ch < kStageW; ch += kThreads) {
+        const uint32_t w = stage[ch];
+        if (!w) continue;
+        const int row = ch / kRowW, j4 = ch - row * kRowW;
+        uint8_t* const p = grad + (size_t)s_row[row] + 4 * j4;  // byte b of w is p[b - 1]
+        if (w & 0xffu) p[-1] = (uint8_t)w;
+        if (w & 0xff00u) p[0] = (uint8_t)(w >> 8);
+        if (w & 0xff0000u) p[1] = (uint8_t)(w >> 16);
+        if (w & 0xff000000u) p[2] = (uint8_t)(w >> 24);
+        const uint32_t z = ((w >> 2) & 0x07070707u) ^ 0x07070707u;  // zero byte: critical
+        if ((z - 0x01010101u) & ~z & 0x80808080u) {
+            const int j = row % kCY, k = row / kCY;
+            const int dyz = ((j & 1) ^ 1) + ((k & 1) ^ 1);
+#pragma unroll
+            for (int b = 0; b < 4; b++) {
+                const uint32_t v = (w >> (8 * b)) & 0xffu;
+                if (v == 0u || ((v >> 2) & 7u) != 7u) continue;
+                const int dim = dyz + ((b & 1) ^ 1);
+                atomicAdd(&s_cnt[dim], 1u);
+                const uint32_t cell = s_row[row] + (uint32_t)(4 * j4 + b - 1);
+                if (dim == 1) lst[atomicAdd(&s_cnt[4], 1u)] = cell;
+                else if (dim == 2) lst[B::kN1 + atomicAdd(&s_cnt[5], 1u)] = cell;
+            }
+        }
     }
     __syncthreads();
     const int sub = (int)(blockIdx.x % kSubs);
+    const uint32_t c1 = s_cnt[4], c2 = s_cnt[5];
     if (flat == 0) {
-        const uint32_t c1 = s_cnt[0], c2 = s_cnt[1];
         s_slot[0] = c1 ? mlist_reserve(list1, sub, c1) : MSlot{0u, 0u, 0u};
         s_slot[1] = c2 ? mlist_reserve(list2, sub, c2) : MSlot{0u, 0u, 0u};
-        const uint32_t st[4] = {s_cnt[2], c1, c2, s_cnt[3]};
         for (int k = 0; k < 4; k++)
-            if (st[k]) atomicAdd(&stats[sub * 4 + k], st[k]);
+            if (s_cnt[k]) atomicAdd(&stats[sub * 4 + k], s_cnt[k]);
     }
     __syncthreads();
-    if (crit & (kDim1 | kDim2)) {
+    if (c1 + c2) {
         const MSlot r1 = s_slot[0], r2 = s_slot[1];
-        uint32_t j1 = o1, j2 = o2;
         bool lost = false;
-        for (uint32_t m = crit & (kDim1 | kDim2); m; m &= m - 1) {
-            const int t = __builtin_ctz(m);
-            const uint32_t cell = vcell + (uint32_t)s_cofs[t];
-            if ((kDim1 >> t) & 1u) lost |= !mlist_store(list1, sub, r1, j1++, cell);
-            else lost |= !mlist_store(list2, sub, r2, j2++, cell);
-        }
+        for (uint32_t i = flat; i < c1; i += kThreads) lost |= !mlist_store(list1, sub, r1, i, lst[i]);
+        for (uint32_t i = flat; i < c2; i += kThreads) lost |= !mlist_store(list2, sub, r2, i, lst[B::kN1 + i]);
         if (lost) atomicOr(overflow, 1u);
     }
 }

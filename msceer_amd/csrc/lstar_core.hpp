@@ -238,70 +238,133 @@ MSC_HD void expand_subset(uint32_t S, const Keys& k, StarResult& r) {
 }
 
 
-// The relaxed expansion of the four most common interior lower stars, in
-// closed form (on smoothed fields about three quarters of all vertices):
-//   the vertex alone (a minimum): critical;
-//   the vertex and one edge: the pair;
-//   a quadrant (two perpendicular edges and their quad): the vertex pairs
-//     with the edge of the lower other vertex, the other edge with the quad;
-//   an octant (three edges, their three quads and the hex): the vertex pairs
-//     with the steepest edge e*; of the other two edges, the first in star
-//     order, a, has one candidate, q(e*, a); the second, b, then has two,
-//     q(e*, b) and q(a, b), and takes the one with the lower lowest vertex
-//     (equal: the earlier in cofacet order); the quad left pairs with the hex.
-// These are the steps expand_subset takes on those shapes (every edge of an
-// interior lower star has the centre as its highest vertex, so its lowest
-// vertex is its other one, and vertex keys never tie).  Returns false for
-// any other shape.
+// Product lower stars: one edge on each of n = 0..3 axes and every cell
+// those edges span -- the vertex alone (a minimum), an edge, a quadrant, an
+// octant; on smoothed fields about three in four interior vertices.
+// product_shape returns has << 3 | neg (bit a of has: axis a carries an
+// edge; bit a of neg: that edge points to -1) for those, kNotProduct for
+// any other lower star.
+constexpr uint32_t kNotProduct = 0xffu;
+constexpr uint32_t kEdgeXM = 1u << 12, kEdgeXP = 1u << 14, kEdgeYM = 1u << 10, kEdgeYP = 1u << 16;
+constexpr uint32_t kEdgeZM = 1u << 4, kEdgeZP = 1u << 22;
+
+MSC_HD uint32_t product_shape(uint32_t L) {
+    const uint32_t hx = (L & (kEdgeXM | kEdgeXP)) != 0u, hy = (L & (kEdgeYM | kEdgeYP)) != 0u;
+    const uint32_t hz = (L & (kEdgeZM | kEdgeZP)) != 0u;
+    const uint32_t nx = (L & kEdgeXP) == 0u, ny = (L & kEdgeYP) == 0u, nz = (L & kEdgeZP) == 0u;
+    const uint32_t p = (hx ? (kX0 | (nx ? kXM : kXP)) : kX0) & (hy ? (kY0 | (ny ? kYM : kYP)) : kY0) &
+                       (hz ? (kZ0 | (nz ? kZM : kZP)) : kZ0);
+    return L == p ? ((hx | hy << 1 | hz << 2) << 3) | ((hx & nx) | (hy & ny) << 1 | (hz & nz) << 2) : kNotProduct;
+}
+
+// .grad byte of a paired cell of an interior star (assigned, pair code d,
+// ldir preset 3), and of a critical vertex.
+MSC_HD uint32_t paired_byte(uint32_t d) { return 0x61u | (d << 2); }
+constexpr uint32_t kCriticalVertexByte = 0x7du;
+
+MSC_HD int sel3(int a, int v0, int v1, int v2) { return a == 0 ? v0 : (a == 1 ? v1 : v2); }
+MSC_HD uint64_t sel3(int a, uint64_t v0, uint64_t v1, uint64_t v2) { return a == 0 ? v0 : (a == 1 ? v1 : v2); }
+
+// The relaxed expansion (expand_subset) of a product lower star in closed
+// form, for a vertex whose star lies inside the domain:
+//   the vertex pairs with the steepest edge e*;
+//   quadrant: the other edge pairs with the quad;
+//   octant: of the other two edges, the first in star order, a, has one
+//     candidate, q(e*, a); the second, b, then has two, q(e*, b) and
+//     q(a, b), and takes the one whose lowest vertex is lower (they cannot
+//     tie: their only common vertices are the centre and b, and e* is below
+//     b); the quad left pairs with the hex.
+// Edge a comes before edge b in star order when a is on the lower axis and
+// the edge on the higher axis points to +1, or else the other way round.
+// g.key(o) is the order key of the vertex at tile offset o from the centre,
+// g.put(o, byte) stores the byte of the star cell at staging offset o, and
+// g.ts[a], g.cs[a] are the tile and staging strides of axis a.
+template <class G>
+MSC_HD void expand_product(uint32_t shape, const G& g) {
+    const uint32_t has = shape >> 3, neg = shape & 7u;
+    const int n = __builtin_popcount(has);
+    const int t0 = (neg & 1u) ? -g.ts[0] : g.ts[0], t1 = (neg & 2u) ? -g.ts[1] : g.ts[1];
+    const int t2 = (neg & 4u) ? -g.ts[2] : g.ts[2];
+    const int c0 = (neg & 1u) ? -g.cs[0] : g.cs[0], c1 = (neg & 2u) ? -g.cs[1] : g.cs[1];
+    const int c2 = (neg & 4u) ? -g.cs[2] : g.cs[2];
+    auto code = [neg](int a) { return 2u * (uint32_t)a + ((neg >> a) & 1u); };
+    if (n == 0) {
+        g.put(0, kCriticalVertexByte);
+    } else if (n == 1) {
+        const int a = __builtin_ctz(has);
+        const uint32_t d = code(a);
+        g.put(0, paired_byte(d));
+        g.put(sel3(a, c0, c1, c2), paired_byte(d ^ 1u));
+    } else if (n == 2) {
+        const int p = __builtin_ctz(has), q = 31 - __builtin_clz(has);
+        const int tp = sel3(p, t0, t1, t2), tq = sel3(q, t0, t1, t2);
+        const int cp = sel3(p, c0, c1, c2), cq = sel3(q, c0, c1, c2);
+        const bool low = g.key(tp) < g.key(tq);  // p's edge is the steepest
+        const uint32_t d = code(low ? p : q);
+        g.put(0, paired_byte(d));
+        g.put(low ? cp : cq, paired_byte(d ^ 1u));
+        g.put(low ? cq : cp, paired_byte(d));
+        g.put(cp + cq, paired_byte(d ^ 1u));
+    } else {
+        const uint64_t kx = g.key(t0), ky = g.key(t1), kz = g.key(t2);
+        const int es = (kx < ky && kx < kz) ? 0 : (ky < kz ? 1 : 2);
+        // the other two axes p < q; a is first in star order
+        const int p = es == 0 ? 1 : 0, q = es == 2 ? 1 : 2;
+        const bool pfirst = !((neg >> q) & 1u);
+        const int a = pfirst ? p : q, b = pfirst ? q : p;
+        const int te = sel3(es, t0, t1, t2), ta = sel3(a, t0, t1, t2), tb = sel3(b, t0, t1, t2);
+        const uint64_t ke = sel3(es, kx, ky, kz), ka = sel3(a, kx, ky, kz), kb = sel3(b, kx, ky, kz);
+        const uint64_t keb = g.key(te + tb), kab = g.key(ta + tb);
+        const uint64_t l1 = ke < keb ? ke : keb;
+        uint64_t l2 = ka < kb ? ka : kb;
+        l2 = kab < l2 ? kab : l2;
+        const bool take1 = l1 < l2;  // b pairs with q(e*, b)
+        const int ce = sel3(es, c0, c1, c2), ca = sel3(a, c0, c1, c2), cb = sel3(b, c0, c1, c2);
+        const uint32_t de = code(es), da = code(a);
+        g.put(0, paired_byte(de));
+        g.put(ce, paired_byte(de ^ 1u));
+        g.put(ca, paired_byte(de));
+        g.put(ce + ca, paired_byte(de ^ 1u));
+        g.put(cb, paired_byte(take1 ? de : da));
+        g.put(ce + cb, paired_byte(take1 ? de ^ 1u : da));
+        g.put(ca + cb, paired_byte(take1 ? de : da ^ 1u));
+        g.put(c0 + c1 + c2, paired_byte(take1 ? de ^ 1u : da ^ 1u));
+    }
+}
+
+// Neighbours of the centre that come before it, as a star mask (bit 13
+// clear).
+// 2 acc + (a < b).  On the GPU the comparison lands in VCC and is shifted in
+// as the carry of acc + acc: two instructions a neighbour.
+MSC_HD uint32_t shift_in_less(uint32_t acc, uint64_t a, uint64_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+    asm("v_cmp_lt_u64 vcc, %1, %2\n\tv_addc_co_u32 %0, vcc, %0, %0, vcc" : "+v"(acc) : "v"(a), "v"(b) : "vcc");
+    return acc;
+#else
+    return 2u * acc + (a < b ? 1u : 0u);
+#endif
+}
+
 template <class Keys>
-MSC_HD bool expand_common(uint32_t L, const Keys& k, StarResult& r) {
-    const uint32_t E = L & kDim1;
-    if (!E) {
-        if (L != kCenter) return false;
-        r.set_critical(kCenter);
-        return true;
-    }
-    const int ne = __builtin_popcount(E);
-    const int e1 = __builtin_ctz(E);
-    if (ne == 1) {
-        if (L != (kCenter | E)) return false;
-        r.set_pair(13, e1);
-        return true;
-    }
-    if (ne == 2) {
-        const int e2 = 31 - __builtin_clz(E);
-        const int q = e1 + e2 - 13;  // the quad spanned by the two edges
-        if (q == 13 || L != (kCenter | E | (1u << q))) return false;
-        const bool first = vkey(k, e1) < vkey(k, e2);
-        r.set_pair(13, first ? e1 : e2);
-        r.set_pair(first ? e2 : e1, q);
-        return true;
-    }
-    const uint32_t H = L & kDim3;
-    if (ne != 3 || !H || (H & (H - 1))) return false;
-    const int h = __builtin_ctz(H);
-    const int hx = comp_x(h), hy = comp_y(h), hz = comp_z(h);
-    const uint32_t oct = (kX0 | (hx > 0 ? kXP : kXM)) & (kY0 | (hy > 0 ? kYP : kYM)) & (kZ0 | (hz > 0 ? kZP : kZM));
-    if (L != oct) return false;
-    // the three edges in star order (ascending bits)
-    const int e2 = __builtin_ctz(E & (E - 1)), e3 = 31 - __builtin_clz(E);
-    const uint64_t k1 = vkey(k, e1), k2 = vkey(k, e2), k3 = vkey(k, e3);
-    int es, a, b;
-    if (k1 < k2 && k1 < k3) { es = e1; a = e2; b = e3; }
-    else if (k2 < k3) { es = e2; a = e1; b = e3; }
-    else { es = e3; a = e1; b = e2; }
-    r.set_pair(13, es);
-    r.set_pair(a, es + a - 13);
-    const int c1 = es + b - 13, c2 = a + b - 13;  // q(e*, b), q(a, b)
-    const uint64_t l1 = quad_lowest(k, c1), l2 = quad_lowest(k, c2);
-    auto rank = [b](int c) {
-        const int d = c - b;
-        return d == 1 ? 0 : (d == -1 ? 1 : (d == 3 ? 2 : (d == -3 ? 3 : (d == 9 ? 4 : 5))));
-    };
-    const bool take1 = l1 < l2 || (l1 == l2 && rank(c1) < rank(c2));
-    r.set_pair(b, take1 ? c1 : c2);
-    r.set_pair(take1 ? c2 : c1, h);
-    return true;
+MSC_HD uint32_t lower_neighbours(const Keys& k) {
+    const uint64_t kc = k.at64(13);
+    uint32_t lower = 0;  // built from bit 26 down
+#pragma unroll
+    for (int t = 26; t >= 0; t--) lower = t == 13 ? 2u * lower : shift_in_less(lower, k.at64(t), kc);
+    return lower;
+}
+
+// Lower star from the lower neighbours: the cells whose every vertex other
+// than the centre comes before it, less those touching `invalid` (star
+// vertices outside the domain).
+MSC_HD uint32_t lower_star_from(uint32_t lower, uint32_t invalid) {
+    return ~star_dilate(((~lower) & kAll & ~kCenter) | invalid) & kAll;
+}
+
+// Star vertices of vertex (x,y,z) that fall outside an X*Y*Z grid.
+MSC_HD uint32_t star_outside(int x, int y, int z, int X, int Y, int Z) {
+    return (x == 0 ? kXM : 0u) | (x == X - 1 ? kXP : 0u) | (y == 0 ? kYM : 0u) | (y == Y - 1 ? kYP : 0u) |
+           (z == 0 ? kZM : 0u) | (z == Z - 1 ? kZP : 0u);
 }
 
 // Lower star of vertex (x,y,z) of an X*Y*Z grid as a star mask: the cells
@@ -309,16 +372,7 @@ MSC_HD bool expand_common(uint32_t L, const Keys& k, StarResult& r) {
 // the domain (gi_fast_robins_noalloc.h:359-366, gi_modified_robins.h:1293-1297).
 template <class Keys>
 MSC_HD uint32_t lower_star_mask(const Keys& k, int x, int y, int z, int X, int Y, int Z) {
-    const uint64_t kc = k.at64(13);
-    uint32_t lower = 0;
-#pragma unroll
-    for (int t = 0; t < 27; t++) {
-        if (t == 13) continue;
-        lower |= (uint32_t)(k.at64(t) < kc) << t;
-    }
-    const uint32_t invalid = (x == 0 ? kXM : 0u) | (x == X - 1 ? kXP : 0u) | (y == 0 ? kYM : 0u) |
-                             (y == Y - 1 ? kYP : 0u) | (z == 0 ? kZM : 0u) | (z == Z - 1 ? kZP : 0u);
-    return ~star_dilate(((~lower) & kAll & ~kCenter) | invalid) & kAll;
+    return lower_star_from(lower_neighbours(k), star_outside(x, y, z, X, Y, Z));
 }
 
 // Boundary strata of the star cells of vertex (x,y,z): bit t of b[a] is set
@@ -335,7 +389,7 @@ template <class Keys>
 MSC_HD void steepest_expand(const Keys& k, uint32_t L, uint32_t bx, uint32_t by, uint32_t bz, StarResult& r) {
     const uint32_t onb = bx | by | bz;
     if (!onb) {
-        if (!expand_common(L, k, r)) expand_subset(L, k, r);
+        expand_subset(L, k, r);
         return;
     }
     const uint32_t odd = bx ^ by ^ bz, two = (bx & by) | (bx & bz) | (by & bz);

@@ -51,11 +51,26 @@ extern "C" int emul_lstar_prepass(const float* f, long long X, long long Y, long
     return 0;
 }
 
-// Every interior vertex: the closed-form expansion of the common lower-star
-// shapes (expand_common) against the general expansion (expand_subset).
-// counts[0]: vertices the closed form took, counts[1]: of those, results
-// that differ.
-extern "C" int emul_check_common(const float* f, long long X, long long Y, long long Z, long long* counts) {
+// Every interior vertex: the closed-form expansion of product lower stars
+// (expand_product, through star-bit offsets) against the general expansion
+// (expand_subset) and its bytes.  counts[0]: vertices the closed form took,
+// counts[1]: of those, stars whose bytes differ or whose written cells are
+// not exactly the lower star.
+namespace {
+struct StarBytes {
+    const StridedKeys* k;
+    uint8_t* out;     // 27 bytes by star bit
+    uint32_t* wrote;  // star bits written
+    int ts[3] = {1, 3, 9}, cs[3] = {1, 3, 9};
+    uint64_t key(int o) const { return k->at64(13 + o); }
+    void put(int o, uint32_t b) const {
+        out[13 + o] = (uint8_t)b;
+        *wrote |= 1u << (13 + o);
+    }
+};
+}  // namespace
+
+extern "C" int emul_check_product(const float* f, long long X, long long Y, long long Z, long long* counts) {
     const int64_t PX = X + 2, PY = Y + 2, PZ = Z + 2;
     std::vector<uint32_t> keys((size_t)(PX * PY * PZ));
     for (int64_t z = 0; z < PZ; z++)
@@ -74,13 +89,48 @@ extern "C" int emul_check_common(const float* f, long long X, long long Y, long 
             for (long long x = 1; x < X - 1; x++) {
                 StridedKeys k{keys.data(), (x + 1) + PX * ((y + 1) + PY * (z + 1)), PX, PX * PY};
                 const uint32_t L = lower_star_mask(k, (int)x, (int)y, (int)z, (int)X, (int)Y, (int)Z);
-                StarResult a, b;
-                if (!expand_common(L, k, a)) continue;
+                const uint32_t shape = product_shape(L);
+                if (shape == kNotProduct) continue;
                 taken++;
-                expand_subset(L, k, b);
-                if (a.c0 != b.c0 || a.c1 != b.c1 || a.c2 != b.c2 || a.crit != b.crit) differ++;
+                uint8_t out[27] = {0};
+                uint32_t wrote = 0;
+                expand_product(shape, StarBytes{&k, out, &wrote});
+                StarResult r;
+                expand_subset(L, k, r);
+                bool bad = wrote != L;
+                for (int t = 0; t < 27; t++)
+                    if ((L >> t) & 1u) bad |= out[t] != star_cell_byte(r, 0u, t);
+                differ += bad;
             }
     counts[0] = taken;
     counts[1] = differ;
+    return 0;
+}
+
+// Lower star of every vertex (star mask) and its product shape
+// (kNotProduct for vertices on the domain boundary), for shape statistics.
+extern "C" int emul_lower_stars(const float* f, long long X, long long Y, long long Z, uint32_t* L, uint8_t* shape) {
+    const int64_t PX = X + 2, PY = Y + 2, PZ = Z + 2;
+    std::vector<uint32_t> keys((size_t)(PX * PY * PZ));
+    for (int64_t z = 0; z < PZ; z++)
+        for (int64_t y = 0; y < PY; y++)
+            for (int64_t x = 0; x < PX; x++) {
+                int64_t gx = x - 1, gy = y - 1, gz = z - 1;
+                gx = gx < 0 ? 0 : (gx >= X ? X - 1 : gx);
+                gy = gy < 0 ? 0 : (gy >= Y ? Y - 1 : gy);
+                gz = gz < 0 ? 0 : (gz >= Z ? Z - 1 : gz);
+                keys[(size_t)(x + PX * (y + PY * z))] = float_key(f[gx + X * (gy + Y * gz)]);
+            }
+#pragma omp parallel for schedule(dynamic, 1)
+    for (long long z = 0; z < Z; z++)
+        for (long long y = 0; y < Y; y++)
+            for (long long x = 0; x < X; x++) {
+                StridedKeys k{keys.data(), (x + 1) + PX * ((y + 1) + PY * (z + 1)), PX, PX * PY};
+                const uint32_t l = lower_star_mask(k, (int)x, (int)y, (int)z, (int)X, (int)Y, (int)Z);
+                const bool onb = x == 0 || y == 0 || z == 0 || x == X - 1 || y == Y - 1 || z == Z - 1;
+                const size_t i = (size_t)(x + X * (y + Y * z));
+                L[i] = l;
+                shape[i] = (uint8_t)(onb ? kNotProduct : product_shape(l));
+            }
     return 0;
 }

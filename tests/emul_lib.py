@@ -18,7 +18,7 @@ def lib():
         h = ctypes.CDLL(SO)
         P, L = ctypes.c_void_p, ctypes.c_longlong
         h.emul_lstar_prepass.argtypes = [P, L, L, L, P, P]
-        h.emul_check_common.argtypes = [P, L, L, L, P]
+        h.emul_check_product.argtypes = [P, L, L, L, P]
         _lib = h
     return _lib
 
@@ -41,11 +41,11 @@ def expected_prepass(pairs_with_flags, shape_zyx):
     return pairs_with_flags | np.where(crit3, 0, 3 << 5).astype(np.uint8)
 
 
-def check_common(f):
-    """(interior vertices whose lower star the closed form expanded, of those
-    the number whose result differs from the general expansion)."""
+def check_product(f):
+    """(interior vertices whose product lower star the closed form expanded,
+    of those the number whose bytes differ from the general expansion's)."""
     f = np.ascontiguousarray(f, np.float32)
     Z, Y, X = f.shape
     c = np.zeros(2, np.int64)
-    lib().emul_check_common(f.ctypes.data_as(ctypes.c_void_p), X, Y, Z, c.ctypes.data_as(ctypes.c_void_p))
+    lib().emul_check_product(f.ctypes.data_as(ctypes.c_void_p), X, Y, Z, c.ctypes.data_as(ctypes.c_void_p))
     return int(c[0]), int(c[1])
