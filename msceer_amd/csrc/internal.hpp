@@ -46,7 +46,7 @@ hipError_t launch_lstar(const Dims& d, const LstarBuffers& b, hipStream_t s, int
 // and quad walk records and the two visited bitmaps, at these byte offsets of
 // one device buffer of ldir_scratch_bytes().
 struct LdirScratch {
-    size_t rec1_off, rec2_off, bits1_off, bits2_off, marks_off;
+    size_t rec1_off, rec2_off, bits1_off, bits2_off;
     uint32_t bit_words;  // words of one bitmap
 };
 size_t ldir_scratch_bytes(const Dims& d, LdirScratch* s);

@@ -22,19 +22,20 @@
 // Reachability does not depend on the visiting order.
 //
 // Three phases on the GPU:
-//  1. ldir_records_kernel, one streaming pass over the .grad: for every edge
-//     the pair directions of its cofacet quads that lead on to another edge
-//     (rec1, 4 bits per cofacet slot), for every quad those of its cofacet
-//     hexes that lead on to another quad (rec2).  A walk step then reads one
-//     record instead of four scattered .grad bytes.
+//  1. ldir_records_kernel, one streaming pass over the .grad (tiles of
+//     vertices, their cells staged in LDS by aligned dword loads): for every
+//     edge the pair directions of its cofacet quads that lead on to another
+//     edge (rec1, 4 bits per cofacet slot), for every quad those of its
+//     cofacet hexes that lead on to another quad (rec2).  A walk step then
+//     reads one record instead of four scattered .grad bytes.
 //  2. Frontier rounds (ldir_round_kernel) walk pass 1 over rec1 and pass 2
 //     over rec2; a reached item is claimed in a bitmap (visited1 / visited2,
 //     one bit per record, atomic OR), so each item is expanded once.  The
 //     .grad bytes are not written during the walks.
-//  3. ldir_marks_kernel derives the marks of every quad from the pass-1 bits
-//     (marked1, and with which hex it is paired), ldir_finish_kernel then
-//     writes every cell's ldir from the closed form above, both streaming
-//     passes (whole rows per wave).
+//  3. ldir_finish_kernel writes every cell's ldir from the closed form
+//     above, one streaming pass over the same tiles (the marks of a quad --
+//     marked1, and with which hex it is paired -- computed from the pass-1
+//     bits where a cell needs them).
 //
 // Records and bits are indexed by 3 * (vertex id of the cell's lowest corner)
 // + axis, axis = the odd axis of an edge or the even axis of a quad.
@@ -120,6 +121,118 @@ __device__ __forceinline__ bool points_back(uint32_t d, uint32_t a, int sgn) {
     return (d >> 1) == a && ((d & 1u) != 0u) == (sgn > 0);
 }
 
+// ---- LDS staging of .grad boxes and bitmap rows ----------------------------
+//
+// The streaming passes read a box of .grad cells around a tile of vertices.
+// Rows of the box are fetched with aligned dword loads (the .grad rows start
+// at any byte: Mx is odd) into LDS rows that keep the global alignment: row r
+// holds the dwords from the one containing the box's first cell, which lies
+// sh[r] bytes into the LDS row.  Only bytes inside the cell grid are
+// fetched; the rest of an LDS row is never read.  A dword that is not wholly
+// inside the .grad allocation's [0, C) (at most one at each end) is fetched
+// byte by byte.
+template <int kRX, int kRY, int kRZ>
+struct CellBox {
+    static constexpr int kRows = kRY * kRZ;
+    static constexpr int kDw = (kRX + 6) / 4;  // dwords one row can span
+    static constexpr int kPitch = 4 * kDw;      // LDS bytes per row
+};
+
+struct BoxRow {
+    int32_t dbase;   // aligned dword holding the box's first cell of the row
+    int32_t dfirst;  // first and last aligned dwords with cells of the grid
+    int32_t dlast;   // (dlast < dfirst: the row lies outside the grid)
+    int32_t off;     // LDS byte offset of the box's first cell: r * kPitch + sh
+};
+
+// Aligned dword index of .grad byte i: (align + i) >> 2, align = address & 3.
+template <int kRX, int kRY, int kRZ>
+__device__ __forceinline__ void box_rows(const LGeom& g, uint32_t align, int cx0, int cy0, int cz0, BoxRow* rows) {
+    using Bx = CellBox<kRX, kRY, kRZ>;
+    for (int r = threadIdx.x; r < Bx::kRows; r += blockDim.x) {
+        const int cy = cy0 + r % kRY, cz = cz0 + r / kRY;
+        BoxRow b;
+        b.dfirst = 1;
+        b.dlast = 0;
+        b.dbase = 0;
+        b.off = r * Bx::kPitch;
+        if (cy >= 0 && cz >= 0 && (uint32_t)cy < g.My && (uint32_t)cz < g.Mz) {
+            const int64_t rs = (int64_t)g.Mx * ((int64_t)cy + (int64_t)g.My * cz);
+            const int64_t gs = (int64_t)align + rs + cx0;  // aligned position of the box's first cell (maybe -1 + rs)
+            const int64_t lo = (int64_t)align + rs + (cx0 > 0 ? cx0 : 0);
+            const int64_t hi = (int64_t)align + rs + (cx0 + kRX < (int)g.Mx ? cx0 + kRX : (int)g.Mx);
+            b.dbase = (int32_t)(gs >> 2);
+            b.dfirst = (int32_t)(lo >> 2);
+            b.dlast = (int32_t)((hi - 1) >> 2);
+            b.off += (int32_t)(gs - 4 * (int64_t)b.dbase);
+        }
+        rows[r] = b;
+    }
+}
+
+template <int kRX, int kRY, int kRZ>
+__device__ __forceinline__ void box_load(const uint8_t* __restrict__ grad, uint64_t cells, uint32_t align,
+                                         const BoxRow* rows, uint32_t* stage) {
+    using Bx = CellBox<kRX, kRY, kRZ>;
+    const uint32_t* g32 = reinterpret_cast<const uint32_t*>(grad - align);
+    for (int d = threadIdx.x; d < Bx::kRows * Bx::kDw; d += blockDim.x) {
+        const int r = d / Bx::kDw, k = d - r * Bx::kDw;
+        const BoxRow b = rows[r];
+        const int32_t j = b.dfirst + k;
+        if (j > b.dlast) continue;
+        uint32_t w;
+        const int64_t first = 4 * (int64_t)j - (int64_t)align;  // .grad index of the dword's first byte
+        if (first >= 0 && first + 4 <= (int64_t)cells) {
+            w = g32[j];
+        } else {
+            w = 0;
+            for (int t = 0; t < 4; t++)
+                if (first + t >= 0 && first + t < (int64_t)cells) w |= (uint32_t)grad[first + t] << (8 * t);
+        }
+        stage[(r * Bx::kPitch) / 4 + (j - b.dbase)] = w;
+    }
+}
+
+// Rows of a bitmap of 3 bits per vertex: row r = (y0 + r % kVY, z0 + r / kVY)
+// from vertex x0 on, kVX vertices; word w of LDS row r is bitmap word
+// first[r] + w.
+template <int kVX>
+struct BitRows {
+    static constexpr int kWords = (3 * kVX + 31) / 32 + 2;  // with the word bits3 reads past the last
+};
+
+template <int kVX, int kVY, int kVZ>
+__device__ __forceinline__ void bits_load(const LGeom& g, const uint32_t* __restrict__ bits, uint32_t nwords, int x0,
+                                          int y0, int z0, uint32_t* lds, uint32_t* first) {
+    constexpr int kW = BitRows<kVX>::kWords, kRows = kVY * kVZ;
+    const uint32_t Zv = (g.Mz + 1) >> 1;
+    for (int r = threadIdx.x; r < kRows; r += blockDim.x) {
+        const uint32_t y = (uint32_t)(y0 + r % kVY), z = (uint32_t)(z0 + r / kVY);
+        first[r] = (3u * ((uint32_t)x0 + g.X * (y + g.Y * z))) >> 5;  // rows outside the grid: never read
+    }
+    __syncthreads();
+    for (int d = threadIdx.x; d < kRows * kW; d += blockDim.x) {
+        const int r = d / kW, w = d - r * kW;
+        const uint32_t y = (uint32_t)(y0 + r % kVY), z = (uint32_t)(z0 + r / kVY);
+        uint32_t v = 0;
+        if (y < g.Y && z < Zv) {
+            const uint32_t i = first[r] + (uint32_t)w;
+            if (i < nwords) v = bits[i];
+        }
+        lds[d] = v;
+    }
+}
+
+// 3 bits of vertex (x, y, z) from the staged rows (row r holds its vertex row).
+template <int kVX>
+__device__ __forceinline__ uint32_t bits3_lds(const uint32_t* lds, const uint32_t* first, int r, uint32_t vid) {
+    constexpr int kW = BitRows<kVX>::kWords;
+    const uint32_t i = 3u * vid - 32u * first[r];
+    const uint32_t* p = lds + r * kW + (i >> 5);
+    const uint64_t w = ((uint64_t)p[1] << 32) | p[0];
+    return (uint32_t)(w >> (i & 31u)) & 7u;
+}
+
 // ---- phase 1: records ------------------------------------------------------
 
 // 3 bits of vertex v in a bitmap of 3 bits per vertex (the bitmaps are padded
@@ -145,69 +258,87 @@ __device__ __forceinline__ uint32_t slot_code(uint32_t b) {
     return 8u | d;
 }
 
-// One thread per vertex (i, j, k): its three "+" edges and three "+" quads.
-// rec1[3v + o] (edge with odd axis o): slot 2m + (sgn < 0) is the cofacet
-// quad along the edge's m-th even axis; rec2[3v + a] (quad with even axis a):
-// slot (sgn < 0) is the cofacet hex along a.  Cells are addressed by 32-bit
-// offsets from the vertex's own cell (fewer than 2^32 cells).
+// Vertex tile of the streaming passes: 32 x 8 x 4 vertices, a lane per (x, y)
+// column of 4 vertices.
+constexpr int kTX = 32, kTY = 8, kTZ = 4;
+static_assert(kTX * kTY == kThreads, "one lane per tile column");
+using RecBox = CellBox<2 * kTX + 1, 2 * kTY + 1, 2 * kTZ + 1>;  // cells 2 x0 - 1 .. 2 (x0 + kTX) - 1
+using FinBox = CellBox<2 * kTX + 1, 2 * kTY + 1, 2 * kTZ + 1>;  // cells 2 x0 .. 2 (x0 + kTX)
+
+// Records of every vertex (i, j, k) of a tile: its three "+" edges and three
+// "+" quads.  rec1[3v + o] (edge with odd axis o): slot 2m + (sgn < 0) is the
+// cofacet quad along the edge's m-th even axis; rec2[3v + a] (quad with even
+// axis a): slot (sgn < 0) is the cofacet hex along a.  The cofacets lie
+// within one cell of the vertex's cell, all inside the tile's staged box.
 __global__ void __launch_bounds__(kThreads)
-ldir_records_kernel(const uint8_t* __restrict__ grad, LGeom g, uint16_t* __restrict__ rec1,
-                    uint8_t* __restrict__ rec2, uint32_t* __restrict__ bits, uint32_t nbits_words) {
+ldir_records_kernel(const uint8_t* __restrict__ grad, LGeom g, uint64_t cells, uint32_t align, int ntx, int nty,
+                    uint16_t* __restrict__ rec1, uint8_t* __restrict__ rec2, uint32_t* __restrict__ bits,
+                    uint32_t nbits_words) {
+    constexpr int kRY = 2 * kTY + 1;
+    __shared__ uint32_t stage[RecBox::kRows * RecBox::kDw];
+    __shared__ BoxRow rows[RecBox::kRows];
     for (uint32_t w = blockIdx.x * kThreads + threadIdx.x; w < nbits_words; w += gridDim.x * kThreads) bits[w] = 0u;
+    const int tile = (int)blockIdx.x;
+    const int x0 = (tile % ntx) * kTX, y0 = ((tile / ntx) % nty) * kTY, z0 = (tile / (ntx * nty)) * kTZ;
+    box_rows<2 * kTX + 1, kRY, 2 * kTZ + 1>(g, align, 2 * x0 - 1, 2 * y0 - 1, 2 * z0 - 1, rows);
+    __syncthreads();
+    box_load<2 * kTX + 1, kRY, 2 * kTZ + 1>(grad, cells, align, rows, stage);
+    __syncthreads();
+    const uint8_t* st8 = reinterpret_cast<const uint8_t*>(stage);
     const uint32_t Zv = (g.Mz + 1) >> 1;
-    const uint32_t ox = 1u, oy = g.Mx, oz = g.sxy;
-    for (uint32_t v = blockIdx.x * kThreads + threadIdx.x; v < g.V; v += gridDim.x * kThreads) {
-        const uint32_t q = v / g.X, k = q / g.Y;
-        const uint32_t i = v - q * g.X, j = q - k * g.Y;
+    const int lx = (int)(threadIdx.x % kTX), ly = (int)(threadIdx.x / kTX);
+    const uint32_t i = (uint32_t)(x0 + lx), j = (uint32_t)(y0 + ly);
+    if (i >= g.X || j >= g.Y) return;
+    for (int lz = 0; lz < kTZ; lz++) {
+        const uint32_t k = (uint32_t)(z0 + lz);
+        if (k >= Zv) break;
+        const uint32_t v = i + g.X * (j + g.Y * k);
+        // byte of the cell at offset (dx, dy, dz) from the vertex's cell
+        auto B = [&](int dx, int dy, int dz) -> uint32_t {
+            return st8[rows[(2 * ly + 1 + dy) + kRY * (2 * lz + 1 + dz)].off + (2 * lx + 1 + dx)];
+        };
         const bool px = i + 1 < g.X, py = j + 1 < g.Y, pz = k + 1 < Zv;  // "+" neighbours exist
         const bool mx = i > 0, my = j > 0, mz = k > 0;                     // "-" neighbours exist
-        const uint32_t c0 = 2u * i + oy * 2u * j + oz * 2u * k;
         if (px) {  // x-edge: cofacets along y, z
-            const uint32_t e = c0 + ox;
             uint32_t r = 0;
-            if (py) r |= slot_code<0, 1, 1>(grad[e + oy]);
-            if (my) r |= slot_code<0, 1, -1>(grad[e - oy]) << 4;
-            if (pz) r |= slot_code<0, 2, 1>(grad[e + oz]) << 8;
-            if (mz) r |= slot_code<0, 2, -1>(grad[e - oz]) << 12;
+            if (py) r |= slot_code<0, 1, 1>(B(1, 1, 0));
+            if (my) r |= slot_code<0, 1, -1>(B(1, -1, 0)) << 4;
+            if (pz) r |= slot_code<0, 2, 1>(B(1, 0, 1)) << 8;
+            if (mz) r |= slot_code<0, 2, -1>(B(1, 0, -1)) << 12;
             rec1[3u * v] = (uint16_t)r;
         }
         if (py) {  // y-edge: cofacets along x, z
-            const uint32_t e = c0 + oy;
             uint32_t r = 0;
-            if (px) r |= slot_code<1, 0, 1>(grad[e + ox]);
-            if (mx) r |= slot_code<1, 0, -1>(grad[e - ox]) << 4;
-            if (pz) r |= slot_code<1, 2, 1>(grad[e + oz]) << 8;
-            if (mz) r |= slot_code<1, 2, -1>(grad[e - oz]) << 12;
+            if (px) r |= slot_code<1, 0, 1>(B(1, 1, 0));
+            if (mx) r |= slot_code<1, 0, -1>(B(-1, 1, 0)) << 4;
+            if (pz) r |= slot_code<1, 2, 1>(B(0, 1, 1)) << 8;
+            if (mz) r |= slot_code<1, 2, -1>(B(0, 1, -1)) << 12;
             rec1[3u * v + 1u] = (uint16_t)r;
         }
         if (pz) {  // z-edge: cofacets along x, y
-            const uint32_t e = c0 + oz;
             uint32_t r = 0;
-            if (px) r |= slot_code<2, 0, 1>(grad[e + ox]);
-            if (mx) r |= slot_code<2, 0, -1>(grad[e - ox]) << 4;
-            if (py) r |= slot_code<2, 1, 1>(grad[e + oy]) << 8;
-            if (my) r |= slot_code<2, 1, -1>(grad[e - oy]) << 12;
+            if (px) r |= slot_code<2, 0, 1>(B(1, 0, 1));
+            if (mx) r |= slot_code<2, 0, -1>(B(-1, 0, 1)) << 4;
+            if (py) r |= slot_code<2, 1, 1>(B(0, 1, 1)) << 8;
+            if (my) r |= slot_code<2, 1, -1>(B(0, -1, 1)) << 12;
             rec1[3u * v + 2u] = (uint16_t)r;
         }
         if (py && pz) {  // quad with even axis x: cofacet hexes along x
-            const uint32_t t = c0 + oy + oz;
             uint32_t r = 0;
-            if (px) r |= slot_code<3, 0, 1>(grad[t + ox]);
-            if (mx) r |= slot_code<3, 0, -1>(grad[t - ox]) << 4;
+            if (px) r |= slot_code<3, 0, 1>(B(1, 1, 1));
+            if (mx) r |= slot_code<3, 0, -1>(B(-1, 1, 1)) << 4;
             rec2[3u * v] = (uint8_t)r;
         }
         if (px && pz) {  // even axis y
-            const uint32_t t = c0 + ox + oz;
             uint32_t r = 0;
-            if (py) r |= slot_code<3, 1, 1>(grad[t + oy]);
-            if (my) r |= slot_code<3, 1, -1>(grad[t - oy]) << 4;
+            if (py) r |= slot_code<3, 1, 1>(B(1, 1, 1));
+            if (my) r |= slot_code<3, 1, -1>(B(1, -1, 1)) << 4;
             rec2[3u * v + 1u] = (uint8_t)r;
         }
         if (px && py) {  // even axis z
-            const uint32_t t = c0 + ox + oy;
             uint32_t r = 0;
-            if (pz) r |= slot_code<3, 2, 1>(grad[t + oz]);
-            if (mz) r |= slot_code<3, 2, -1>(grad[t - oz]) << 4;
+            if (pz) r |= slot_code<3, 2, 1>(B(1, 1, 1));
+            if (mz) r |= slot_code<3, 2, -1>(B(1, 1, -1)) << 4;
             rec2[3u * v + 2u] = (uint8_t)r;
         }
     }
@@ -473,71 +604,75 @@ __device__ __forceinline__ uint32_t quad_marks(uint32_t b, uint32_t e_v, uint32_
     return (1u << kA) | (d == 2u * kA ? 1u << (3 + kA) : 0u) | (d == 2u * kA + 1u ? 1u << (6 + kA) : 0u);
 }
 
-__global__ void __launch_bounds__(kThreads)
-ldir_marks_kernel(const uint8_t* __restrict__ grad, LGeom g, const uint32_t* __restrict__ v1,
-                  uint16_t* __restrict__ marks) {
-    const uint32_t Zv = (g.Mz + 1) >> 1;
-    const uint32_t ox = 1u, oy = g.Mx, oz = g.sxy;
-    const uint32_t sy = g.X, sz = g.X * g.Y;
-    for (uint32_t v = blockIdx.x * kThreads + threadIdx.x; v < g.V; v += gridDim.x * kThreads) {
-        const uint32_t q = v / g.X, k = q / g.Y;
-        const uint32_t i = v - q * g.X, j = q - k * g.Y;
-        const bool px = i + 1 < g.X, py = j + 1 < g.Y, pz = k + 1 < Zv;
-        const uint32_t c0 = 2u * i + oy * 2u * j + oz * 2u * k;
-        const uint32_t e_v = bits3(v1, v);
-        const uint32_t e_x = px ? bits3(v1, v + 1u) : 0u, e_y = py ? bits3(v1, v + sy) : 0u;
-        const uint32_t e_z = pz ? bits3(v1, v + sz) : 0u;
-        uint32_t m = 0;
-        if (py && pz) m |= quad_marks<0>(grad[c0 + oy + oz], e_v, e_y, e_z);
-        if (px && pz) m |= quad_marks<1>(grad[c0 + ox + oz], e_v, e_x, e_z);
-        if (px && py) m |= quad_marks<2>(grad[c0 + ox + oy], e_v, e_x, e_y);
-        marks[v] = (uint16_t)m;
-    }
-}
-
 __device__ __forceinline__ uint8_t with_ldir(uint32_t b, uint32_t l) { return (uint8_t)((b & 0x1Fu) | (l << 5)); }
 
-// One thread per vertex: the cells of its "+" octant (the vertex byte is
-// rewritten unchanged, so a wave writes whole row segments).
+// The ldir bits of every cell of the tile's "+" octants (cells 2 x0 .. 2 (x0 +
+// kTX) - 1 along x, likewise y and z), written back row by row: aligned
+// dwords wholly inside the tile's cells as dwords, the rest of a row as
+// bytes.  The box staged holds one more cell layer on each "+" side (the
+// facet quads of the tile's last hexes) and the bitmaps one more vertex
+// layer; the marks of a quad (marked1, and with which hex it is paired) are
+// computed where needed.  The cells of the next tiles that are read here are
+// only read for their pair and assigned bits, which no tile changes.
 __global__ void __launch_bounds__(kThreads)
-ldir_finish_kernel(uint8_t* __restrict__ grad, LGeom g, const uint32_t* __restrict__ v1,
-                   const uint32_t* __restrict__ v2, const uint16_t* __restrict__ marks) {
+ldir_finish_kernel(uint8_t* __restrict__ grad, LGeom g, uint64_t cells, uint32_t align, int ntx, int nty,
+                   const uint32_t* __restrict__ v1, const uint32_t* __restrict__ v2, uint32_t nwords) {
+    constexpr int kRX = 2 * kTX + 1, kRY = 2 * kTY + 1, kRZ = 2 * kTZ + 1;
+    constexpr int kVX = kTX + 1, kVY = kTY + 1, kVZ = kTZ + 1, kW = BitRows<kVX>::kWords;
+    __shared__ uint32_t stage[FinBox::kRows * FinBox::kDw];
+    __shared__ BoxRow rows[FinBox::kRows];
+    __shared__ uint32_t b1[kVY * kVZ * kW], b2[kVY * kVZ * kW];
+    __shared__ uint32_t f1[kVY * kVZ], f2[kVY * kVZ];
+    const int tile = (int)blockIdx.x;
+    const int x0 = (tile % ntx) * kTX, y0 = ((tile / ntx) % nty) * kTY, z0 = (tile / (ntx * nty)) * kTZ;
+    box_rows<kRX, kRY, kRZ>(g, align, 2 * x0, 2 * y0, 2 * z0, rows);
+    bits_load<kVX, kVY, kVZ>(g, v1, nwords, x0, y0, z0, b1, f1);  // (synchronises)
+    bits_load<kVX, kVY, kVZ>(g, v2, nwords, x0, y0, z0, b2, f2);
+    box_load<kRX, kRY, kRZ>(grad, cells, align, rows, stage);
+    __syncthreads();
+    uint8_t* st8 = reinterpret_cast<uint8_t*>(stage);
     const uint32_t Zv = (g.Mz + 1) >> 1;
-    const uint32_t ox = 1u, oy = g.Mx, oz = g.sxy;
+    const int lx = (int)(threadIdx.x % kTX), ly = (int)(threadIdx.x / kTX);
+    const uint32_t i = (uint32_t)(x0 + lx), j = (uint32_t)(y0 + ly);
     const uint32_t sy = g.X, sz = g.X * g.Y;
-    for (uint32_t v = blockIdx.x * kThreads + threadIdx.x; v < g.V; v += gridDim.x * kThreads) {
-        const uint32_t q = v / g.X, k = q / g.Y;
-        const uint32_t i = v - q * g.X, j = q - k * g.Y;
+    for (int lz = 0; lz < kTZ; lz++) {
+        const uint32_t k = (uint32_t)(z0 + lz);
+        if (i >= g.X || j >= g.Y || k >= Zv) break;
+        const uint32_t v = i + g.X * (j + g.Y * k);
+        // cell at offset (dx, dy, dz) from the cell of vertex (lx + ex, ly + ey, lz + ez)
+        auto at = [&](int dx, int dy, int dz) -> uint8_t& {
+            return st8[rows[(2 * ly + dy) + kRY * (2 * lz + dz)].off + (2 * lx + dx)];
+        };
+        // bits of the vertex (ex, ey, ez) vertices further on
+        auto e1 = [&](int ex, int ey, int ez) {
+            return bits3_lds<kVX>(b1, f1, (ly + ey) + kVY * (lz + ez), v + (uint32_t)ex + sy * ey + sz * ez);
+        };
+        auto e2 = [&](int ex, int ey, int ez) {
+            return bits3_lds<kVX>(b2, f2, (ly + ey) + kVY * (lz + ez), v + (uint32_t)ex + sy * ey + sz * ez);
+        };
         const bool px = i + 1 < g.X, py = j + 1 < g.Y, pz = k + 1 < Zv;
-        const uint32_t c0 = 2u * i + oy * 2u * j + oz * 2u * k;
-        const uint32_t e_v = bits3(v1, v), q_v = bits3(v2, v);
-        const uint32_t m_v = marks[v];
-        grad[c0] = grad[c0];
-        if (px) { const uint32_t c = c0 + ox; grad[c] = with_ldir(grad[c], (e_v & 1u) ? 2u : 3u); }
-        if (py) { const uint32_t c = c0 + oy; grad[c] = with_ldir(grad[c], (e_v & 2u) ? 2u : 3u); }
-        if (pz) { const uint32_t c = c0 + oz; grad[c] = with_ldir(grad[c], (e_v & 4u) ? 2u : 3u); }
+        const uint32_t e_v = e1(0, 0, 0), q_v = e2(0, 0, 0);
+        const uint32_t e_x = px ? e1(1, 0, 0) : 0u, e_y = py ? e1(0, 1, 0) : 0u, e_z = pz ? e1(0, 0, 1) : 0u;
+        uint32_t m_v = 0;
+        if (py && pz) m_v |= quad_marks<0>(at(0, 1, 1), e_v, e_y, e_z);
+        if (px && pz) m_v |= quad_marks<1>(at(1, 0, 1), e_v, e_x, e_z);
+        if (px && py) m_v |= quad_marks<2>(at(1, 1, 0), e_v, e_x, e_y);
+        if (px) { uint8_t& c = at(1, 0, 0); c = with_ldir(c, (e_v & 1u) ? 2u : 3u); }
+        if (py) { uint8_t& c = at(0, 1, 0); c = with_ldir(c, (e_v & 2u) ? 2u : 3u); }
+        if (pz) { uint8_t& c = at(0, 0, 1); c = with_ldir(c, (e_v & 4u) ? 2u : 3u); }
         // quads: reached in pass 2 -> 1; marked in pass 1 -> 2
-        if (py && pz) {
-            const uint32_t c = c0 + oy + oz;
-            grad[c] = with_ldir(grad[c], (q_v & 1u) ? 1u : ((m_v & 1u) ? 2u : 3u));
-        }
-        if (px && pz) {
-            const uint32_t c = c0 + ox + oz;
-            grad[c] = with_ldir(grad[c], (q_v & 2u) ? 1u : ((m_v & 2u) ? 2u : 3u));
-        }
-        if (px && py) {
-            const uint32_t c = c0 + ox + oy;
-            grad[c] = with_ldir(grad[c], (q_v & 4u) ? 1u : ((m_v & 4u) ? 2u : 3u));
-        }
+        if (py && pz) { uint8_t& c = at(0, 1, 1); c = with_ldir(c, (q_v & 1u) ? 1u : ((m_v & 1u) ? 2u : 3u)); }
+        if (px && pz) { uint8_t& c = at(1, 0, 1); c = with_ldir(c, (q_v & 2u) ? 1u : ((m_v & 2u) ? 2u : 3u)); }
+        if (px && py) { uint8_t& c = at(1, 1, 0); c = with_ldir(c, (q_v & 4u) ? 1u : ((m_v & 4u) ? 2u : 3u)); }
         if (px && py && pz) {
-            const uint32_t c = c0 + ox + oy + oz;
-            const uint32_t b = grad[c];
+            uint8_t& c = at(1, 1, 1);
+            const uint32_t b = c;
             const uint32_t d = (b >> 2) & 7u;
             if (d != 7u) {
                 uint32_t l = 3u;
                 if (b & 1u) {
                     // facet quads: -x, +x, -y, +y, -z, +z; the pair excluded
-                    const uint32_t q_x = bits3(v2, v + 1u), q_y = bits3(v2, v + sy), q_z = bits3(v2, v + sz);
+                    const uint32_t q_x = e2(1, 0, 0), q_y = e2(0, 1, 0), q_z = e2(0, 0, 1);
                     const bool f[6] = {(q_x & 1u) != 0u, (q_v & 1u) != 0u, (q_y & 2u) != 0u,
                                        (q_v & 2u) != 0u, (q_z & 4u) != 0u, (q_v & 4u) != 0u};
                     bool m2 = false;
@@ -546,16 +681,44 @@ ldir_finish_kernel(uint8_t* __restrict__ grad, LGeom g, const uint32_t* __restri
                         if (t != d && f[t]) m2 = true;
                     if (m2) {
                         l = 1u;
+                    } else if (d & 1u) {
+                        // the pair is the vertex's own quad: marked in pass 1 and paired with its "+" hex
+                        if ((m_v >> (3u + (d >> 1))) & 1u) l = 2u;
                     } else {
-                        // the pair quad, marked in pass 1 and paired back with this hex
-                        const uint32_t a = d >> 1;
-                        const uint32_t nb = (d & 1u) ? v : v + (a == 0 ? 1u : (a == 1 ? sy : sz));
-                        const uint32_t mq = (d & 1u) ? m_v : (uint32_t)marks[nb];
-                        if ((mq >> ((d & 1u) ? 3u + a : 6u + a)) & 1u) l = 2u;
+                        // the pair is the next vertex's quad: marked in pass 1 and paired with its "-" hex
+                        uint32_t mq = 0;
+                        if (d == 0u) mq = quad_marks<0>(at(2, 1, 1), e_x, e1(1, 1, 0), e1(1, 0, 1));
+                        else if (d == 2u) mq = quad_marks<1>(at(1, 2, 1), e_y, e1(1, 1, 0), e1(0, 1, 1));
+                        else mq = quad_marks<2>(at(1, 1, 2), e_z, e1(1, 0, 1), e1(0, 1, 1));
+                        if ((mq >> (6u + (d >> 1))) & 1u) l = 2u;
                     }
                 }
-                grad[c] = with_ldir(b, l);
+                c = with_ldir(b, l);
             }
+        }
+    }
+    __syncthreads();
+    // write back the rows of the tile's cells
+    constexpr int kWY = 2 * kTY, kWZ = 2 * kTZ;  // rows written: cells 2 y0 .. 2 (y0 + kTY) - 1, likewise z
+    uint32_t* g32 = reinterpret_cast<uint32_t*>(grad - align);
+    for (int d = threadIdx.x; d < kWY * kWZ * FinBox::kDw; d += kThreads) {
+        const int wr = d / FinBox::kDw, kk = d - wr * FinBox::kDw;
+        const int r = (wr % kWY) + kRY * (wr / kWY);
+        const BoxRow br = rows[r];
+        const int32_t jj = br.dfirst + kk;
+        if (jj > br.dlast) continue;
+        // the tile's cells of the row: .grad bytes [lo, hi)
+        const int cy = 2 * y0 + wr % kWY, cz = 2 * z0 + wr / kWY;
+        const int64_t rs = (int64_t)g.Mx * ((int64_t)cy + (int64_t)g.My * cz);
+        const int64_t lo = rs + 2 * x0;
+        const int64_t hi = rs + (2 * x0 + 2 * kTX < (int)g.Mx ? 2 * x0 + 2 * kTX : (int)g.Mx);
+        const int64_t first = 4 * (int64_t)jj - (int64_t)align;
+        const uint32_t w = stage[(r * FinBox::kPitch) / 4 + (jj - br.dbase)];
+        if (first >= lo && first + 4 <= hi) {
+            g32[jj] = w;
+        } else {
+            for (int t = 0; t < 4; t++)
+                if (first + t >= lo && first + t < hi) grad[first + t] = (uint8_t)(w >> (8 * t));
         }
     }
 }
@@ -581,16 +744,23 @@ size_t ldir_scratch_bytes(const Dims& d, LdirScratch* s) {
     const uint64_t words = (n + 31) / 32 + 1;  // one word of padding: bits3 reads a word pair
     const size_t r1 = (2 * n + 255) & ~(size_t)255, r2 = (n + 255) & ~(size_t)255;
     const size_t bw = ((words * 4) + 255) & ~(size_t)255;
-    const size_t mk = (2 * d.verts + 255) & ~(size_t)255;
     if (s) {
         s->rec1_off = 0;
         s->rec2_off = r1;
         s->bits1_off = r1 + r2;
         s->bits2_off = r1 + r2 + bw;
-        s->marks_off = r1 + r2 + 2 * bw;
         s->bit_words = (uint32_t)words;
     }
-    return r1 + r2 + 2 * bw + mk;
+    return r1 + r2 + 2 * bw;
+}
+
+// Tiles of the streaming passes (fewer than 2^31: the volume has fewer than
+// 2^32 cells).
+static void tiles_of(const Dims& d, int& ntx, int& nty, uint32_t& n) {
+    ntx = (d.X + kTX - 1) / kTX;
+    nty = (d.Y + kTY - 1) / kTY;
+    const int ntz = (d.Z + kTZ - 1) / kTZ;
+    n = (uint32_t)ntx * (uint32_t)nty * (uint32_t)ntz;
 }
 
 hipError_t launch_ldir_records(const Dims& d, const uint8_t* grad, void* scratch, const LdirScratch& s,
@@ -599,8 +769,12 @@ hipError_t launch_ldir_records(const Dims& d, const uint8_t* grad, void* scratch
     uint32_t* bits1 = reinterpret_cast<uint32_t*>(base + s.bits1_off);
     // visited1 and visited2 are adjacent: one clear covers both
     const uint32_t words = (uint32_t)((s.bits2_off - s.bits1_off) / 4) + s.bit_words;
-    hipLaunchKernelGGL(ldir_records_kernel, dim3(grid_of(d.verts, 16384)), dim3(kThreads), 0, st, grad, make_geom(d),
-                       reinterpret_cast<uint16_t*>(base + s.rec1_off), base + s.rec2_off, bits1, words);
+    int ntx, nty;
+    uint32_t n;
+    tiles_of(d, ntx, nty, n);
+    const uint32_t align = (uint32_t)((uintptr_t)grad & 3u);
+    hipLaunchKernelGGL(ldir_records_kernel, dim3(n), dim3(kThreads), 0, st, grad, make_geom(d), d.cells, align, ntx,
+                       nty, reinterpret_cast<uint16_t*>(base + s.rec1_off), base + s.rec2_off, bits1, words);
     return hipGetLastError();
 }
 
@@ -628,11 +802,12 @@ hipError_t launch_ldir_finish(const Dims& d, uint8_t* grad, void* scratch, const
     uint8_t* base = static_cast<uint8_t*>(scratch);
     const uint32_t* v1 = reinterpret_cast<const uint32_t*>(base + s.bits1_off);
     const uint32_t* v2 = reinterpret_cast<const uint32_t*>(base + s.bits2_off);
-    uint16_t* marks = reinterpret_cast<uint16_t*>(base + s.marks_off);
-    hipLaunchKernelGGL(ldir_marks_kernel, dim3(grid_of(d.verts, 16384)), dim3(kThreads), 0, st, grad, make_geom(d),
-                       v1, marks);
-    hipLaunchKernelGGL(ldir_finish_kernel, dim3(grid_of(d.verts, 16384)), dim3(kThreads), 0, st, grad, make_geom(d),
-                       v1, v2, marks);
+    int ntx, nty;
+    uint32_t n;
+    tiles_of(d, ntx, nty, n);
+    const uint32_t align = (uint32_t)((uintptr_t)grad & 3u);
+    hipLaunchKernelGGL(ldir_finish_kernel, dim3(n), dim3(kThreads), 0, st, grad, make_geom(d), d.cells, align, ntx,
+                       nty, v1, v2, s.bit_words);
     return hipGetLastError();
 }
 

@@ -81,3 +81,33 @@ def test_prediction_is_the_sigmoid_of_the_loss_model():
     prob = oracle_lib.cnn_predict(p, f, quant=0)
     loss, _ = oracle_lib.cnn_gradient(p, f, np.zeros(f.shape, np.uint8), 1.0, quant=0)
     assert abs(loss - float(np.mean(-np.log1p(-prob)))) < 1e-9
+
+
+def test_forward_matches_torch_conv3d():
+    # an independent statement of the same network through torch's conv3d
+    # (CPU, float64): catches a transposed weight layout, a wrong tap order or
+    # a missing bias/ReLU in the oracle's own loops
+    import torch
+    import torch.nn.functional as F
+    f = _volume((6, 7, 9), 21)
+    p = _params(13).astype(np.float64)
+    prob = oracle_lib.cnn_predict(p.astype(np.float32), f, quant=0).reshape(f.shape)
+    n = torch.from_numpy(oracle_lib.cnn_normalize(f).astype(np.float64)).reshape(1, 1, *f.shape)
+
+    def weight(name, cout, cin):
+        # flat index (t * cout + co) * cin + ci, tap t = kx + 3 ky + 9 kz -> torch [co, ci, kz, ky, kx]
+        o, size = BLOCKS[name]
+        w = torch.from_numpy(p[o:o + size].copy()).reshape(3, 3, 3, cout, cin)
+        return w.permute(3, 4, 0, 1, 2).contiguous()
+
+    def bias(name):
+        o, size = BLOCKS[name]
+        return torch.from_numpy(p[o:o + size].copy())
+
+    a1 = F.relu(F.conv3d(n, weight("W1", 16, 1), bias("b1"), padding=1))
+    a2 = F.relu(F.conv3d(a1, weight("W2", 32, 16), bias("b2"), padding=1))
+    a3 = F.relu(F.conv3d(a2, weight("W3", 16, 32), bias("b3"), padding=1))
+    o, _ = BLOCKS["wh"]
+    logit = torch.einsum("c,bczyx->bzyx", torch.from_numpy(p[o:o + 16].copy()), a3) + float(p[28160])
+    expect = torch.sigmoid(logit)[0].numpy()
+    assert np.abs(prob - expect).max() < 1e-9, float(np.abs(prob - expect).max())

@@ -150,3 +150,30 @@ def test_exact_mode_differs_from_path_compression_somewhere():
     _, l1, _, _ = oracle_lib.ondemand_asc(f, err, gthr, iters, presimp, mode=1)
     _, l2, _, _ = oracle_lib.ondemand_asc(f, err, gthr, iters, presimp, mode=2)
     assert 0 < int((l1 != l2).sum())
+
+
+@pytest.mark.parametrize("mode", [1, 2])
+def test_two_basin_closed_form(mode):
+    # f = min(|p - a|^2, |p - b|^2): the flow runs straight to the nearer
+    # centre, so away from the bisector plane every vertex's FP64 label is the
+    # basin of the nearer centre (the path-compressing mode 1 and the exact
+    # mode 2 alike); a mistake in the step rule, the interpolation or the
+    # labelling of the end vertex moves vertices across
+    Z, Y, X = 24, 30, 40
+    z, y, x = np.meshgrid(np.arange(Z), np.arange(Y), np.arange(X), indexing="ij")
+    a, b = np.array([11.3, 14.6, 10.2]), np.array([10.8, 13.9, 29.7])  # (z, y, x)
+    da = (z - a[0]) ** 2 + (y - a[1]) ** 2 + (x - a[2]) ** 2
+    db = (z - b[0]) ** 2 + (y - b[1]) ** 2 + (x - b[2]) ** 2
+    f = np.minimum(da, db).astype(np.float32)
+    _, lab, _, _ = oracle_lib.ondemand_asc(f, 1e-3, 1e-6, 4096, 0.0, mode=mode)
+    lab = lab.reshape(Z, Y, X)
+    la = lab[int(round(a[0])), int(round(a[1])), int(round(a[2]))]
+    lb = lab[int(round(b[0])), int(round(b[1])), int(round(b[2]))]
+    assert la != lb
+    # signed distance to the bisector plane, in voxels
+    n = b - a
+    s = (np.stack([z, y, x], -1) - (a + b) / 2) @ n / np.linalg.norm(n)
+    far_a, far_b = s < -2.0, s > 2.0
+    assert far_a.sum() > 5000 and far_b.sum() > 5000
+    assert (lab[far_a] == la).all(), int((lab[far_a] != la).sum())
+    assert (lab[far_b] == lb).all(), int((lab[far_b] != lb).sum())
