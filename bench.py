@@ -189,7 +189,7 @@ def bench_labels(args):
              "oracle_fp64_vs_exact": float((ref != exact).mean()),
              "roofline": {"bound": "hbm", "achieved": round(integ_gbs, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                           "frac": round(integ_gbs / HBM_PEAK_GBS, 6),
-                          "kernel": "integration (pass1_kernel levels + pass2 + pass3_kernel<double>)",
+                          "kernel": "integration (pass1_kernel levels + pass2 + pass3_kernel<false>)",
                           "algorithmic_bytes": 8.0 * V, "kernel_ms": stage["integrate"],
                           "pass3_ms": stage["pass3"],
                           "certain_rounds_share_of_total": round(stage["regions"] / stage["total"], 4),
@@ -197,8 +197,33 @@ def bench_labels(args):
                                   "not bytes"},
              "stage_ms": stage,
              "counts": {k: int(last[k]) for k in ("minima", "regions", "traced_fp32", "over_budget_fp32",
-                                                  "reintegrated_fp64", "over_budget_fp64", "certain_sweeps",
-                                                  "pass3_rounds", "steps_fp32", "steps_fp64", "ahead_fp64")}}
+                                                  "reintegrated_pass3", "escalated_fp64", "reintegrated_fp64",
+                                                  "over_budget_fp64", "certain_sweeps", "pass3_rounds",
+                                                  "steps_fp32", "steps_fp64", "ahead_fp64")},
+             "fp64_share_of_steps": round(float(last["steps_fp64"]) /
+                                          max(1.0, float(last["steps_fp32"] + last["steps_fp64"])), 4)}
+    # the FP32-first precision policy on the same volume (MSCEER_FP32_FIRST)
+    run(extra=native.FP32_FIRST)
+    hipmem.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.label_steps):
+        run(extra=native.FP32_FIRST)
+    hipmem.synchronize()
+    ms32 = (time.perf_counter() - t0) * 1e3 / args.label_steps
+    lab32 = dl.download(np.int32, V)
+    s32 = st.as_dict()
+    plain["fp32_first"] = {
+        "ms_per_step": round(ms32, 3), "value": round(V / (ms32 * 1e-3) / 1e6, 2), "unit": "Mvert/s",
+        "policy": "pass 3 in FP32 (voxel-local coordinates); FP64 re-trace of the streamlines whose FP32 trace meets "
+                  "a near-tie decision (2^-16 voxel of a voxel face or half-integer; step error or gradient within "
+                  "2^-14 relative of its threshold) or uses up its step budget",
+        "mismatch_vs_fp64_oracle": float((lab32 != ref).mean()),
+        "mismatch_vs_exact_fp64_oracle": float((lab32 != exact).mean()),
+        "pass3_ms": round(float(s32["ms_pass3"]), 3), "escalated_fp64": int(s32["escalated_fp64"]),
+        "reintegrated_pass3": int(s32["reintegrated_pass3"]),
+        "steps_fp32": int(s32["steps_fp32"]), "steps_fp64": int(s32["steps_fp64"]),
+        "fp64_share_of_steps": round(float(s32["steps_fp64"]) /
+                                     max(1.0, float(s32["steps_fp32"] + s32["steps_fp64"])), 4)}
     if args.ideal_mask_probe:
         from scipy import ndimage
         zero = hipmem.DeviceBuffer(V)
@@ -406,8 +431,11 @@ def bench_learned_large(args):
     learned_s = time.perf_counter() - t0
     lab = dl.download(np.int32, V)
     net.close()
+    probe = None
+    if args.ideal_mask_probe:
+        probe = ideal_mask_probe(run, dl, dm, st, plain, (Z, Y, X))
     ctx.close()
-    return {"workload": f"{X}x{Y}x{Z} smoothed noise sigma 2.5 seed 5999; accurate labels + conforming .grad",
+    out = {"workload": f"{X}x{Y}x{Z} smoothed noise sigma 2.5 seed 5999; accurate labels + conforming .grad",
             "plain": {"seconds": round(plain_s, 3), "value": round(V / plain_s / 1e6, 2), "unit": "Mvert/s",
                       "stage_ms": {k[3:]: round(float(plain_stats[k]), 1) for k in
                                    ("ms_lstar", "ms_regions", "ms_integrate", "ms_cleanup", "ms_grad")}},
@@ -416,6 +444,49 @@ def bench_learned_large(args):
                         "threshold": net.threshold, "dilation": net.dilation,
                         "mismatch_vs_plain": float((lab != plain).mean()),
                         "speedup_vs_plain": round(plain_s / learned_s, 3)}}
+    if probe is not None:
+        out["ideal_mask_probe"] = probe
+    return out
+
+
+def ideal_mask_probe(run, dl, dm, st, plain, shape):
+    """The masked path's bound: labelling with the ideal mask (every vertex
+    next to a basin boundary of the plain labels or whose combinatorial label
+    is wrong, dilated by 1) instead of a learned one, stage by stage.  The
+    masks are built on the GPU with torch (plumbing)."""
+    import torch
+    import torch.nn.functional as F
+
+    Z, Y, X = shape
+    V = X * Y * Z
+    stages = ("ms_lstar", "ms_regions", "ms_integrate", "ms_pass1", "ms_pass3", "ms_cleanup", "ms_grad", "ms_total")
+    dm.upload(np.zeros(V, np.uint8))
+    run(dm.addr)
+    comb = torch.from_numpy(dl.download(np.int32, V)).cuda().view(Z, Y, X)
+    pl = torch.from_numpy(plain).cuda().view(Z, Y, X)
+    core = comb != pl
+    del comb
+    for ax in range(3):
+        n = pl.shape[ax]
+        d = pl.narrow(ax, 1, n - 1) != pl.narrow(ax, 0, n - 1)
+        core.narrow(ax, 0, n - 1).logical_or_(d)
+        core.narrow(ax, 1, n - 1).logical_or_(d)
+        del d
+    m = F.max_pool3d(core.to(torch.float16)[None, None], 3, stride=1, padding=1)[0, 0] > 0
+    frac_core, frac = float(core.float().mean()), float(m.float().mean())
+    dm.upload(m.to(torch.uint8).cpu().numpy().ravel())
+    del core, m, pl
+    torch.cuda.empty_cache()
+    run(dm.addr)  # warm-up
+    t0 = time.perf_counter()
+    run(dm.addr)
+    secs = time.perf_counter() - t0
+    labm = dl.download(np.int32, V)
+    return {"mask": "vertices next to a plain-label boundary or with a wrong combinatorial label, dilated by 1",
+            "core_fraction": frac_core, "mask_fraction": frac, "seconds": round(secs, 3),
+            "mismatch_vs_plain": float((labm != plain).mean()),
+            "stage_ms": {k[3:]: round(float(st.as_dict()[k]), 2) for k in stages},
+            "integrated_vertices": int(st.integrated_vertices)}
 
 
 def calibrate_copy():

@@ -41,6 +41,13 @@ extern "C" {
 #define MSCEER_TIMING 0x4u      /* record per-kernel HIP event timings */
 #define MSCEER_STOP_REGIONS 0x8u     /* msceer_ondemand_asc: return the certain-region labels */
 #define MSCEER_STOP_INTEGRATED 0x10u /* msceer_ondemand_asc: return labels before the cleanup */
+/* msceer_ondemand_asc, msceer_integrate_labels: re-integrate (pass 3) in FP32
+ * first and in FP64 only the streamlines whose FP32 trace meets a near-tie
+ * decision (within 2^-16 voxel of a voxel face or half-integer, or an error or
+ * gradient test within 2^-14 relative of its threshold) or uses up its step
+ * budget; default: pass 3 wholly in FP64.  No counterpart in the reference,
+ * whose advector has one fixed mixed precision. */
+#define MSCEER_FP32_FIRST 0x20u
 
 typedef struct msceer_ctx msceer_ctx;
 
@@ -61,22 +68,24 @@ typedef struct msceer_label_stats {
     int64_t regions;            /* regions after persistence simplification */
     int64_t traced_fp32;        /* pass-1 streamlines (FP32) */
     int64_t over_budget_fp32;   /* of those, stopped by the step budget */
-    int64_t reintegrated_fp64;  /* pass-3 re-integrations (FP64) */
+    int64_t reintegrated_fp64;  /* pass-3 re-integrations traced in FP64 (with MSCEER_FP32_FIRST: those escalated) */
     int64_t over_budget_fp64;   /* of those, stopped by the step budget */
     int64_t integrated_vertices;/* vertices the learned mask sent to integration (all if none) */
     int32_t certain_sweeps;     /* rounds of the certain-region resolution (dependency depth) */
-    int32_t pass3_rounds;       /* launches of the FP64 re-integration (one persistent launch) */
+    int32_t pass3_rounds;       /* launches of pass 3 (one persistent launch) */
     float ms_lstar;             /* base + constrained lower-star kernels */
     float ms_regions;           /* extremum graph + certain regions */
     float ms_integrate;         /* passes 1-3 */
     float ms_cleanup;           /* isolated-region cleanup + restriction labels */
     float ms_grad;              /* constrained pass + ldir */
     float ms_total;
-    int64_t steps_fp32;         /* advection steps (accepted and rejected) of pass 1 */
-    int64_t steps_fp64;         /* advection steps of pass 3 (streamlines traced ahead included) */
+    int64_t steps_fp32;         /* FP32 advection steps (accepted and rejected), passes 1 and 3 */
+    int64_t steps_fp64;         /* FP64 advection steps (pass 3) */
     int64_t ahead_fp64;         /* pass-3 streamlines traced ahead of a chain of long re-integrations */
     float ms_pass1;             /* integration pass 1 (FP32, all levels) */
-    float ms_pass3;             /* integration pass 3 (FP64 re-integration) */
+    float ms_pass3;             /* integration pass 3 */
+    int64_t reintegrated_pass3;  /* pass-3 re-integrations (all of them, whatever their precision) */
+    int64_t escalated_fp64;     /* with MSCEER_FP32_FIRST: of those, left in doubt by the FP32 trace and traced again in FP64 */
 } msceer_label_stats;
 
 int msceer_version(void);

@@ -73,7 +73,8 @@ int main(int argc, char** argv) {
         }
         printf("built minmaxgraph: %lld minima, %lld saddles -> %lld regions\n", (long long)st.minima,
                (long long)st.saddles, (long long)st.regions);
-        printf("re-integrated %lld vertices in FP64\n", (long long)st.reintegrated_fp64);
+        printf("re-integrated %lld vertices (%lld traced in FP64)\n", (long long)st.reintegrated_pass3,
+               (long long)st.reintegrated_fp64);
         if (dbg) {
             char lname[2048];
             snprintf(lname, sizeof(lname), "%s._int_%dx%dx%d.raw", filename.c_str(), X, Y, Z);

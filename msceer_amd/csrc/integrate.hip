@@ -538,6 +538,9 @@ __device__ __forceinline__ uint32_t load_agent(const uint32_t* p) {
 // kSpecBusy (a trace ahead runs), kSpecWanted (v was reached meanwhile: the
 // lane tracing ahead finishes it), else the end vertex.
 constexpr int32_t kSpecNone = -1, kSpecBusy = -2, kSpecWanted = -3;
+// flags kept beside the end vertex of a trace ahead (vertex ids < 2^29), for
+// the counts of the vertex that takes it: re-traced in FP64, out of budget
+constexpr int32_t kSpecFP64 = 0x40000000, kSpecOver = 0x20000000;
 constexpr uint32_t kSpecItem = 0x80000000u;  // queue item flag: trace ahead (vertex ids < 2^31)
 constexpr int kSpecSteps = 512;              // only chains of streamlines this long are traced ahead
 constexpr int kShellR = 5;                   // traced ahead: All6 distance 2..kShellR (224 offsets)
@@ -557,26 +560,55 @@ __constant__ uint16_t kShellOffsets[kShell] = {
     1625, 1634, 1640, 1651, 1655, 1668, 1670, 1685, 1829, 1844, 1846, 1859, 1863, 1874, 1880, 1891, 1895, 1908,
     1910, 1925, 2101, 2116, 2118, 2131, 2135, 2148, 2150, 2165, 2373, 2388, 2390, 2405, 2645};
 
-// two waves per SIMD: the FP64 streamline state fills half the register file
-template <typename T>
+// Precision of pass 3.  By default every re-integration is traced in FP64
+// with the gradient recomputed in FP64 from the values (the reference's
+// arithmetic).  With kFP32First a re-integration is traced in FP32 first
+// (the voxel-local tracer, TracerL); when that trace ends with a near-tie
+// decision or out of step budget (doubt), the same lane traces the same
+// vertex again in FP64 and only that result is used.  Either way a
+// re-integration's result is a function of its vertex alone, which keeps the
+// closure argument above, and the escalation happens inside the one launch,
+// so the chains of dependent re-integrations and the tracing ahead run on as
+// before.  With both states the corner gradients live in LDS (72 KB per
+// work-group), so that they fit the registers of two waves per SIMD.
+template <bool kFP32First>
+struct Pass3State;
+template <>
+struct Pass3State<false> {
+    Tracer<double> d;
+};
+template <>
+struct Pass3State<true> {
+    TracerL<LdsCorners<float, kThreads>> f;
+    Tracer<double, LdsCorners<double, kThreads>> d;
+};
+
+template <bool kFP32First>
 __global__ void __launch_bounds__(kThreads, 2)
-pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr, T ethr, int max_iters) {
+pass3_kernel(GradSrc<float> srcf, GradSrc<double> srcd, VGrid g, LabelBuffers b, WorkQueue q, float dir, float gthr,
+             float ethr, int max_iters) {
     __shared__ uint32_t ring_mem[kThreads / 64][kRing];
+    __shared__ double corner_mem[kFP32First ? 24 * kThreads : 1];
+    __shared__ float corner_mem32[kFP32First ? 24 * kThreads : 1];
     uint32_t* ring = ring_mem[threadIdx.x >> 6];
     int32_t* const spec = b.label2;
     const int lane = (int)(threadIdx.x & 63u);
     const uint64_t below = (1ull << lane) - 1ull;
     const uint32_t first = *q.first_count;
-    uint32_t traced = 0, over = 0, ahead = 0;
-    uint64_t steps = 0;
+    uint32_t traced = 0, over = 0, ahead = 0, escalated = 0;
+    uint64_t steps32 = 0, steps64 = 0;
     uint32_t rhead = 0, rtail = 0;  // ring cursors (the same in every lane)
     bool chunks = true;             // pass-2 chunks may remain (the same in every lane)
-    bool active = false, claimed = false, speculative = false;
+    bool active = false, claimed = false, speculative = false, fp64 = false;
     bool ahead_seen = false;        // the wave has seen tracing ahead (the same in every lane)
     uint32_t slot = 0;
     uint64_t wait_since = 0, idle_since = 0;
     int64_t v = 0;
-    Tracer<T> tr;
+    Pass3State<kFP32First> tr;
+    if constexpr (kFP32First) {
+        tr.d.c.p = corner_mem + threadIdx.x;
+        tr.f.c.p = corner_mem32 + threadIdx.x;
+    }
     // queue n items of every lane: in the ring while it has room, globally beyond
     auto push = [&](const uint32_t* items, int n) {
         if (!__any(n > 0)) return;
@@ -690,16 +722,20 @@ pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr
                     } else if (sp >= 0) {
                         trace = false;  // traced ahead
                         fin = chain = true;
-                        end = sp;
+                        end = sp & ~(kSpecFP64 | kSpecOver);
+                        escalated += (sp & kSpecFP64) ? 1u : 0u;
+                        over += (sp & kSpecOver) ? 1u : 0u;
                     }
                 }
                 if (trace) {
                     active = true;
+                    fp64 = !kFP32First;
                     speculative = ahead_item;
                     if (ahead_item) ahead++;
                     int sx, sy, sz;
                     vertex_xyz(g, v, sx, sy, sz);
-                    tracer_start(src, g, tr, sx, sy, sz, max_iters);
+                    if constexpr (kFP32First) tracer_start(srcf, g, tr.f, sx, sy, sz, max_iters);
+                    else tracer_start(srcd, g, tr.d, sx, sy, sz, max_iters);
                 }
             }
         }
@@ -746,25 +782,62 @@ pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr
         if (active) {
             int64_t next = 0;
             int ev = kNone;
+            const double dird = (double)dir, gthrd = (double)gthr, ethrd = (double)ethr;
             for (int k = 0; k < burst; k++) {
                 bool moved = false;
-                ev = tracer_iterate(src, g, b.dest, false, dir, gthr, ethr, tr, next, moved);
+                if constexpr (kFP32First) {
+                    if (!fp64) {
+                        ev = tracer_iterate(srcf, g, b.dest, false, dir, gthr, ethr, tr.f, next, moved);
+                        if (ev == kOutOfVoxel) {
+                            tracer_reenter(srcf, g, tr.f);
+                            ev = kNone;
+                        }
+                        if (__any(ev != kNone)) break;
+                        continue;
+                    }
+                }
+                ev = tracer_iterate(srcd, g, b.dest, false, dird, gthrd, ethrd, tr.d, next, moved);
                 if (ev == kOutOfVoxel) {
-                    tracer_reenter(src, g, tr);
+                    tracer_reenter(srcd, g, tr.d);
                     ev = kNone;
                 }
                 if (__any(ev != kNone)) break;
             }
             if (ev != kNone) {
+                if (!fp64) {
+                    if constexpr (kFP32First) {
+                        const int n_steps = max_iters - (tr.f.iters > 0 ? tr.f.iters : 0);
+                        steps32 += (uint64_t)n_steps;
+                        if (tr.f.doubt) {
+                            // trace v again in FP64 and use only that result
+                            fp64 = true;
+                            int sx, sy, sz;
+                            vertex_xyz(g, v, sx, sy, sz);
+                            tracer_start(srcd, g, tr.d, sx, sy, sz, max_iters);
+                            ev = kNone;
+                        } else {
+                            if (ev != kHitPreassigned) next = tracer_nearest(g, tr.f);
+                            chain = n_steps >= kSpecSteps;
+                        }
+                    }
+                } else {
+                    const int n_steps = max_iters - (tr.d.iters > 0 ? tr.d.iters : 0);
+                    steps64 += (uint64_t)n_steps;
+                    if (ev != kHitPreassigned) next = tracer_nearest(g, tr.d);
+                    chain = n_steps >= kSpecSteps;
+                }
+            }
+            if (ev != kNone) {
                 active = false;
-                if (ev != kHitPreassigned) next = nearest(g, tr.cur, tr.near);
-                const int n_steps = max_iters - (tr.iters > 0 ? tr.iters : 0);
-                steps += (uint64_t)n_steps;
                 end = next;
+                const bool ob = fp64 && ev == kOverMax, esc = kFP32First && fp64;
                 // a vertex traced ahead is finished here only if it was reached meanwhile
-                fin = !speculative || atomicExch(&spec[v], (int32_t)next) == kSpecWanted;
-                chain = n_steps >= kSpecSteps;
-                if (fin && ev == kOverMax) over++;
+                fin = !speculative || atomicExch(&spec[v], (int32_t)next | (esc ? kSpecFP64 : 0) |
+                                                               (ob ? kSpecOver : 0)) == kSpecWanted;
+                if (fin) {
+                    escalated += esc ? 1u : 0u;
+                    over += ob ? 1u : 0u;
+                }
             }
         }
         uint32_t kids[6];
@@ -821,10 +894,12 @@ pass3_kernel(GradSrc<T> src, VGrid g, LabelBuffers b, WorkQueue q, T dir, T gthr
             }
         }
     }
-    if (traced) atomicAdd(&b.stats[kStatTraced3], traced);
+    if (traced) atomicAdd(&b.stats[kStatTraced3f], traced);
     if (over) atomicAdd(&b.stats[kStatOver3], over);
     if (ahead) atomicAdd(&b.stats[kStatAhead3], ahead);
-    if (steps) atomicAdd(reinterpret_cast<unsigned long long*>(&b.stats[kStatSteps3]), (unsigned long long)steps);
+    if (escalated) atomicAdd(&b.stats[kStatDoubt], escalated);
+    if (steps32) atomicAdd(reinterpret_cast<unsigned long long*>(&b.stats[kStatSteps3f]), (unsigned long long)steps32);
+    if (steps64) atomicAdd(reinterpret_cast<unsigned long long*>(&b.stats[kStatSteps3]), (unsigned long long)steps64);
 }
 
 // ---- cleanup of isolated components ------------------------------------------
@@ -1406,13 +1481,20 @@ hipError_t launch_grad_field64(const VGrid& g, const LabelBuffers& b, hipStream_
                        b.gradd);
     return hipGetLastError();
 }
-hipError_t launch_pass3(const VGrid& g, const LabelBuffers& b, const WorkQueue& q, float gthr, float ethr,
-                        int max_iters, int sign, hipStream_t s) {
-    static uint32_t blocks = resident_blocks(pass3_kernel<double>);
-    GradSrc<double> src{b.gradd};
+hipError_t launch_pass3(const VGrid& g, const LabelBuffers& b, const WorkQueue& q, bool fp32_first, float gthr,
+                        float ethr, int max_iters, int sign, hipStream_t s) {
+    GradSrc<float> srcf{b.gradf, b.values, b.seedmask, g};
+    GradSrc<double> srcd{b.gradd};
     hipLaunchKernelGGL(pass3_prepare_kernel, dim3(1), dim3(64), 0, s, q);
-    hipLaunchKernelGGL(pass3_kernel<double>, dim3(blocks), dim3(kThreads), 0, s, src, g, b, q, (double)sign,
-                       (double)gthr, (double)ethr, max_iters);
+    if (fp32_first) {
+        static uint32_t blocks = resident_blocks(pass3_kernel<true>);
+        hipLaunchKernelGGL(pass3_kernel<true>, dim3(blocks), dim3(kThreads), 0, s, srcf, srcd, g, b, q, (float)sign,
+                           gthr, ethr, max_iters);
+    } else {
+        static uint32_t blocks = resident_blocks(pass3_kernel<false>);
+        hipLaunchKernelGGL(pass3_kernel<false>, dim3(blocks), dim3(kThreads), 0, s, srcf, srcd, g, b, q, (float)sign,
+                           gthr, ethr, max_iters);
+    }
     return hipGetLastError();
 }
 hipError_t launch_cc_init(const VGrid& g, const LabelBuffers& b, hipStream_t s) {

@@ -41,13 +41,16 @@ struct LabelBuffers {
 enum IntegStat : int {
     kStatTraced1 = 0,    // streamlines traced in pass 1
     kStatOver1 = 1,      // pass-1 streamlines that ran out of step budget
-    kStatTraced3 = 2,    // FP64 re-integrations in pass 3
-    kStatOver3 = 3,      // pass-3 streamlines that ran out of step budget
+    kStatTraced3 = 2,    // (unused)
+    kStatOver3 = 3,      // pass-3 FP64 streamlines that ran out of step budget
     kStatChanged = 4,    // sweep / round change flag
     kStatMasked = 5,     // unassigned vertices inside the integration mask
     kStatAhead3 = 6,     // pass-3 streamlines traced ahead of a chain
+    kStatTraced3f = 7,   // re-integrations in pass 3
+    kStatDoubt = 12,     // FP32-first: re-integrations traced again in FP64 (a near-tie or the step budget)
     kStatSteps1 = 8,     // uint64 (two words): pass-1 steps, accepted and rejected
-    kStatSteps3 = 10,    // uint64: pass-3 steps
+    kStatSteps3 = 10,    // uint64: pass-3 FP64 steps
+    kStatSteps3f = 14,   // uint64: pass-3 FP32 steps
     kStatNum = 16
 };
 
@@ -99,13 +102,12 @@ hipError_t launch_certain_init(const VGrid& g, const float* values, const uint8_
 hipError_t launch_certain_round(const VGrid& g, int32_t* label, uint32_t* cnt, const uint64_t* in,
                                 const uint32_t* in_count, uint64_t* out, uint32_t* out_count, uint32_t* clear_count,
                                 uint32_t capacity, uint32_t* overflow, uint64_t expected, hipStream_t s);
-// One pass-3 round over `in` (FP64 re-integration to the certain regions);
-// changed labels push their ASSIGNED neighbours to `out`; `clear` counts are zeroed.
 hipError_t launch_grad_field64(const VGrid& g, const LabelBuffers& b, hipStream_t s);
-// FP64 pass 3 over the queue filled by pass 2 (zero head, tail and
-// chunk_head before pass 2).
-hipError_t launch_pass3(const VGrid& g, const LabelBuffers& b, const WorkQueue& q, float gthr, float ethr,
-                        int max_iters, int sign, hipStream_t s);
+// Pass 3 over the queue filled by pass 2 (zero head, tail and chunk_head
+// before pass 2): one persistent launch, FP64 re-integration, or with
+// fp32_first FP32 re-integration and FP64 where the FP32 trace is in doubt.
+hipError_t launch_pass3(const VGrid& g, const LabelBuffers& b, const WorkQueue& q, bool fp32_first, float gthr,
+                        float ethr, int max_iters, int sign, hipStream_t s);
 // Cleanup of isolated components.
 hipError_t launch_cc_init(const VGrid& g, const LabelBuffers& b, hipStream_t s);
 // Same-label components (rep[] = earliest vertex of each component).

@@ -125,7 +125,7 @@ int run_cleanup(msceer_ctx* ctx, const VGrid& g, const LabelBuffers& b) {
 // mask take their combinatorial label from d_grad and count as assigned.
 // sign -1 integrates ascending manifolds (downhill), +1 descending (uphill).
 int run_integration(msceer_ctx* ctx, const VGrid& g, const LabelBuffers& b, const uint8_t* d_mask,
-                    const uint8_t* d_grad, float grad_thr, float err_thr, int32_t max_iters, int sign,
+                    const uint8_t* d_grad, float grad_thr, float err_thr, int32_t max_iters, int sign, bool fp32_first,
                     msceer_label_stats& st) {
     hipStream_t s = ctx->stream;
     uint32_t* ctl = ctl_dev(ctx);
@@ -172,7 +172,7 @@ int run_integration(msceer_ctx* ctx, const VGrid& g, const LabelBuffers& b, cons
     CHECK_HIP(launch_grad_field64(g, b, s));
     CHECK_HIP(launch_fill_i32(b.label2, V, -1, s));  // no vertex traced ahead yet
     CHECK_HIP(hipEventRecord(ctx->ev[14], s));
-    CHECK_HIP(launch_pass3(g, b, q, grad_thr, err_thr, max_iters, sign, s));
+    CHECK_HIP(launch_pass3(g, b, q, fp32_first, grad_thr, err_thr, max_iters, sign, s));
     CHECK_HIP(hipEventRecord(ctx->ev[15], s));
     int rc = read_ctl(ctx, 0, kCtlWords);
     if (rc) return rc;
@@ -183,13 +183,15 @@ int run_integration(msceer_ctx* ctx, const VGrid& g, const LabelBuffers& b, cons
     st.pass3_rounds = 1;
     st.traced_fp32 = ctx->h_ctl[kCtlIntStats + kStatTraced1];
     st.over_budget_fp32 = ctx->h_ctl[kCtlIntStats + kStatOver1];
-    st.reintegrated_fp64 = ctx->h_ctl[kCtlIntStats + kStatTraced3];
+    st.reintegrated_pass3 = ctx->h_ctl[kCtlIntStats + kStatTraced3f];
+    st.escalated_fp64 = ctx->h_ctl[kCtlIntStats + kStatDoubt];
+    st.reintegrated_fp64 = fp32_first ? st.escalated_fp64 : st.reintegrated_pass3;
     st.over_budget_fp64 = ctx->h_ctl[kCtlIntStats + kStatOver3];
     st.integrated_vertices = d_mask ? (int64_t)ctx->h_ctl[kCtlIntStats + kStatMasked] : V;
     auto u64 = [&](int k) {
         return (int64_t)((uint64_t)ctx->h_ctl[kCtlIntStats + k] | ((uint64_t)ctx->h_ctl[kCtlIntStats + k + 1] << 32));
     };
-    st.steps_fp32 = u64(kStatSteps1);
+    st.steps_fp32 = u64(kStatSteps1) + u64(kStatSteps3f);
     st.steps_fp64 = u64(kStatSteps3);
     st.ahead_fp64 = ctx->h_ctl[kCtlIntStats + kStatAhead3];
     return MSCEER_OK;
@@ -203,7 +205,8 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
                                    const uint8_t* integrate_mask, uint8_t* grad_out, int32_t* labels_out,
                                    msceer_label_stats* stats, uint32_t flags) {
     if (!ctx || !values) return MSCEER_EINVAL;
-    if (flags & ~(MSCEER_DEVICE_PTRS | MSCEER_TIMING | MSCEER_STOP_REGIONS | MSCEER_STOP_INTEGRATED))
+    if (flags & ~(MSCEER_DEVICE_PTRS | MSCEER_TIMING | MSCEER_STOP_REGIONS | MSCEER_STOP_INTEGRATED |
+                  MSCEER_FP32_FIRST))
         return MSCEER_EINVAL;
     if (!(err_thr > 0.f) || !(grad_thr >= 0.f) || max_iters < 1 || !(presimp >= 0.f)) return MSCEER_EINVAL;
     int rc = check_dims(nx, ny, nz);
@@ -407,7 +410,8 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
     if (flags & MSCEER_STOP_REGIONS) return finish_early();
 
     // ---- integration ----
-    rc = run_integration(ctx, g, b, d_mask, d_grad, grad_thr, err_thr, max_iters, -1, st);
+    rc = run_integration(ctx, g, b, d_mask, d_grad, grad_thr, err_thr, max_iters, -1,
+                         (flags & MSCEER_FP32_FIRST) != 0, st);
     if (rc) return rc;
     CHECK_HIP(hipEventRecord(ev[3], s));
     if (flags & MSCEER_STOP_INTEGRATED) return finish_early();
@@ -633,7 +637,7 @@ extern "C" int msceer_integrate_labels(msceer_ctx* ctx, const float* values, int
                                        float grad_thr, int32_t max_iters, int32_t sign, int32_t* labels_out,
                                        msceer_label_stats* stats, uint32_t flags) {
     if (!ctx || !values || !terminals || !labels_out) return MSCEER_EINVAL;
-    if (flags & ~(MSCEER_DEVICE_PTRS | MSCEER_TIMING)) return MSCEER_EINVAL;
+    if (flags & ~(MSCEER_DEVICE_PTRS | MSCEER_TIMING | MSCEER_FP32_FIRST)) return MSCEER_EINVAL;
     if (sign != -1 && sign != 1) return MSCEER_EINVAL;
     if (!(err_thr > 0.f) || !(grad_thr >= 0.f) || max_iters < 1) return MSCEER_EINVAL;
     int rc = check_dims(nx, ny, nz);
@@ -685,7 +689,8 @@ extern "C" int msceer_integrate_labels(msceer_ctx* ctx, const float* values, int
     CHECK_HIP(launch_grad_field(g, d_values, b.gradf, s));
     CHECK_HIP(launch_zero_seed_gradient(g, b, s));
     CHECK_HIP(launch_zero(ctl, kCtlWords, s));
-    rc = run_integration(ctx, g, b, nullptr, nullptr, grad_thr, err_thr, max_iters, sign, st);
+    rc = run_integration(ctx, g, b, nullptr, nullptr, grad_thr, err_thr, max_iters, sign,
+                         (flags & MSCEER_FP32_FIRST) != 0, st);
     if (rc) return rc;
     CHECK_HIP(hipMemcpyAsync(labels_out, b.label, sizeof(int32_t) * V,
                              dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));

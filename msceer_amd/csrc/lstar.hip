@@ -191,7 +191,8 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
                 const TableKeys64 keys{tile + base, s_off};
                 uint32_t bx, by, bz;
                 star_boundary(x, y, z, X, Y, Z, bx, by, bz);
-                StarResult r;
+                // the expansion writes each cell's byte as it pairs it
+                StageBytes r{st8 + cbase, s_soff, bx | by | bz};
                 if (!kConstrained) {
                     steepest_expand(keys, L, bx, by, bz, r);
                 } else {
@@ -201,11 +202,6 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
                         return rv[tid_x(t) + sy * tid_y(t) + sz * tid_z(t)];
                     };
                     constrained_expand(keys, L, bx, by, bz, label, r);
-                }
-                const uint32_t onb = bx | by | bz;
-                for (uint32_t m = L; m; m &= m - 1) {
-                    const int t = __builtin_ctz(m);
-                    st8[cbase + s_soff[t]] = star_cell_byte(r, onb, t);
                 }
             }
         }

@@ -129,6 +129,27 @@ struct StarResult {
     }
 };
 
+// The same results written straight as .grad bytes (star_cell_byte's) into a
+// byte image of the cells: the stage byte of star cell t is sc[soff[t]];
+// onb holds the cells on a domain face (their flag bit).
+struct StageBytes {
+    uint8_t* sc;
+    const int* soff;
+    uint32_t onb;
+    MSC_HD void set_pair(int a, int b) const {
+        const uint32_t ca = star_dir_code(a, b);
+        sc[soff[a]] = (uint8_t)(0x61u | (((onb >> a) & 1u) << 1) | (ca << 2));
+        sc[soff[b]] = (uint8_t)(0x61u | (((onb >> b) & 1u) << 1) | ((ca ^ 1u) << 2));
+    }
+    MSC_HD void set_critical(uint32_t m) const {
+        for (; m; m &= m - 1) {
+            const int t = __builtin_ctz(m);
+            const uint32_t ldir = ((kDim3 >> t) & 1u) ? 0u : 3u;
+            sc[soff[t]] = (uint8_t)(0x1du | (((onb >> t) & 1u) << 1) | (ldir << 5));
+        }
+    }
+};
+
 // First candidate, in cofacet order +x,-x,+y,-y,+z,-z, whose lowest vertex is
 // strictly smallest (gi_fast_robins_noalloc.h:124-137 over the cofacet
 // iteration order of :256-270).
@@ -157,8 +178,8 @@ MSC_HD int pick_candidate(const Keys& kk, int a, uint32_t cset) {
 //   it with the first lowest such cofacet; when none has one, the first
 //   unprocessed d-cell with the lowest lowest vertex becomes critical;
 //   leftover 3-cells are critical.
-template <class Keys>
-MSC_HD void expand_subset(uint32_t S, const Keys& k, StarResult& r) {
+template <class Keys, class Res>
+MSC_HD void expand_subset(uint32_t S, const Keys& k, Res& r) {
     uint32_t P = 0;  // processed (paired or critical)
     if (S & kCenter) {
         const uint32_t E = S & kDim1;
@@ -385,8 +406,8 @@ MSC_HD void star_boundary(int x, int y, int z, int X, int Y, int Z, uint32_t& bx
 }
 
 // Steepest pass: subsets keyed by boundaryValue 0..3 (slots of <5,4>).
-template <class Keys>
-MSC_HD void steepest_expand(const Keys& k, uint32_t L, uint32_t bx, uint32_t by, uint32_t bz, StarResult& r) {
+template <class Keys, class Res>
+MSC_HD void steepest_expand(const Keys& k, uint32_t L, uint32_t bx, uint32_t by, uint32_t bz, Res& r) {
     const uint32_t onb = bx | by | bz;
     if (!onb) {
         expand_subset(L, k, r);
@@ -404,9 +425,9 @@ MSC_HD void steepest_expand(const Keys& k, uint32_t L, uint32_t bx, uint32_t by,
 // clamped to 23 (gi_modified_robins.h:1299-1313); label(t) returns the
 // restriction label of star cell t.  Subsets are independent, so they are
 // expanded in any order.
-template <class Keys, class Labels>
+template <class Keys, class Labels, class Res>
 MSC_HD void constrained_expand(const Keys& k, uint32_t L, uint32_t bx, uint32_t by, uint32_t bz,
-                               const Labels& label, StarResult& r) {
+                               const Labels& label, Res& r) {
     uint32_t pending = L;
     while (pending) {
         int kmin = 1 << 30;

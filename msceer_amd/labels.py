@@ -19,12 +19,14 @@ from .lstar import num_cells
 
 def ondemand_asc(values: np.ndarray, err_thr: float = 1e-3, grad_thr: float = 1e-6, max_iters: int = 4096,
                  presimp: float = 0.0, mask: np.ndarray | None = None, want_grad: bool = True,
-                 ctx: native.Context | None = None, stop: str | None = None):
+                 ctx: native.Context | None = None, stop: str | None = None, fp32_first: bool = False):
     """(grad or None, labels (Z,Y,X) int32, stats dict) of a (Z,Y,X) float32 volume.
 
     mask: optional per-vertex uint8; only nonzero vertices are integrated,
     the others take the label of the minimum their discrete-gradient V-path
     reaches.  stop: None, "regions" or "integrated" (earlier stage labels).
+    fp32_first: re-integrate in FP32 and in FP64 only where the FP32 trace is
+    in doubt (MSCEER_FP32_FIRST).
     """
     v = np.ascontiguousarray(values, dtype=np.float32)
     Z, Y, X = v.shape
@@ -38,6 +40,7 @@ def ondemand_asc(values: np.ndarray, err_thr: float = 1e-3, grad_thr: float = 1e
             raise ValueError(f"mask holds {m.size} entries, the volume {v.size} vertices")
         st = native.LabelStats()
         flags = {None: 0, "regions": native.STOP_REGIONS, "integrated": native.STOP_INTEGRATED}[stop]
+        flags |= native.FP32_FIRST if fp32_first else 0
         rc = native.lib().msceer_ondemand_asc(
             ctx.handle, v.ctypes.data_as(ctypes.c_void_p), X, Y, Z, err_thr, grad_thr, max_iters, presimp,
             None if m is None else m.ctypes.data_as(ctypes.c_void_p),

@@ -16,6 +16,7 @@ SKIP_LDIR = 0x2
 TIMING = 0x4
 STOP_REGIONS = 0x8
 STOP_INTEGRATED = 0x10
+FP32_FIRST = 0x20
 BRICKS = ("64x2x2", "32x4x2", "16x4x4", "8x8x4")  # lower-star brick shapes by index
 # error codes
 OK, EINVAL, EHIP, ENOMEM, ETOOBIG, ENODEV = 0, -1, -2, -3, -4, -5
@@ -92,6 +93,8 @@ class LabelStats(ctypes.Structure):
         ("ahead_fp64", ctypes.c_int64),
         ("ms_pass1", ctypes.c_float),
         ("ms_pass3", ctypes.c_float),
+        ("reintegrated_pass3", ctypes.c_int64),
+        ("escalated_fp64", ctypes.c_int64),
     ]
 
     def as_dict(self):

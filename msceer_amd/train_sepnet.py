@@ -6,7 +6,11 @@ Training data: volumes 5000+i of the two families (smoothed noise and the
 Gaussian-mixture labelling family, `learned.training_volume`), labelled by
 the full GPU integrator; targets from `learned.separatrix_target`.  The
 threshold is the largest candidate whose learned labels differ from the full
-integrator's on at most `--budget` of the vertices of held-out volumes.
+integrator's on at most `--budget` of the vertices of held-out volumes and
+whose mask leaves at least 1 - `--max-fraction` of them to the combinatorial
+labels.  When no candidate does both, the mask would integrate (nearly) every
+vertex and cost more than the plain integrator: the script reports that and
+saves nothing (exit status 2).
 """
 from __future__ import annotations
 
@@ -33,6 +37,8 @@ def main(argv=None):
     ap.add_argument("--holdout", type=int, default=4)
     ap.add_argument("--budget", type=float, default=5e-4)
     ap.add_argument("--dilation", type=int, default=2)
+    ap.add_argument("--max-fraction", type=float, default=0.5,
+                    help="largest mask fraction (vertices integrated) worth using")
     ap.add_argument("--seed", type=int, default=1)
     ap.add_argument("--out", default=learned.WEIGHTS)
     args = ap.parse_args(argv)
@@ -78,10 +84,15 @@ def main(argv=None):
         table.append(row)
         log(f"threshold {thr}: mismatch mean {row[1]:.5f} max {row[2]:.5f}, mask fraction {row[3]:.3f}")
         if row[2] <= args.budget:
-            chosen = thr
-            break
+            if row[3] <= args.max_fraction:
+                chosen = thr
+            break  # lower thresholds only flag more vertices
     if chosen is None:
-        chosen = table[-1][0]
+        log(f"no threshold keeps the mismatch within {args.budget} with a mask fraction of at most "
+            f"{args.max_fraction}: the mask would not save work; nothing saved")
+        net.close()
+        ctx.close()
+        return 2
     net.threshold = chosen
     net.save(args.out, loss_history=np.array(hist, np.float32), calibration=np.array(table, np.float64),
              train_seconds=train_s, volumes=args.volumes, size=args.size, epochs=args.epochs)

@@ -24,6 +24,8 @@ extern "C" int emul_lstar_prepass(const float* f, long long X, long long Y, long
                 keys[(size_t)(x + PX * (y + PY * z))] = float_key(f[gx + X * (gy + Y * gz)]);
             }
     const int64_t Mx = 2 * X - 1, My = 2 * Y - 1;
+    int soff[27];  // .grad offset of star cell t (the kernel's stage offsets, here in the .grad itself)
+    for (int t = 0; t < 27; t++) soff[t] = (int)(tid_x(t) + Mx * (tid_y(t) + My * tid_z(t)));
 #pragma omp parallel for schedule(dynamic, 1)
     for (long long z = 0; z < Z; z++)
         for (long long y = 0; y < Y; y++)
@@ -33,7 +35,8 @@ extern "C" int emul_lstar_prepass(const float* f, long long X, long long Y, long
                 uint32_t bx, by, bz;
                 star_boundary((int)x, (int)y, (int)z, (int)X, (int)Y, (int)Z, bx, by, bz);
                 const int64_t vcell = 2 * x + Mx * (2 * y + My * 2 * z);
-                StarResult r;
+                // the kernel's byte sink: every cell of L written as it is paired
+                StageBytes r{grad + vcell, soff, bx | by | bz};
                 if (!restriction) {
                     steepest_expand(k, L, bx, by, bz, r);
                 } else {
@@ -42,10 +45,6 @@ extern "C" int emul_lstar_prepass(const float* f, long long X, long long Y, long
                         return rv[tid_x(t) + Mx * tid_y(t) + Mx * My * tid_z(t)];
                     };
                     constrained_expand(k, L, bx, by, bz, label, r);
-                }
-                for (int t = 0; t < 27; t++) {
-                    if (!((L >> t) & 1u)) continue;
-                    grad[vcell + tid_x(t) + Mx * (tid_y(t) + My * tid_z(t))] = star_cell_byte(r, bx | by | bz, t);
                 }
             }
     return 0;

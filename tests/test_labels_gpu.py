@@ -184,7 +184,8 @@ def test_labels_are_the_same_from_run_to_run(ctx, name):
     # starts: every stage gives the same result on every call, and the
     # integrator's work counts are the same too
     f, (err, gthr, iters, presimp), _ = load_ondemand(name)
-    keys = ("traced_fp32", "reintegrated_fp64", "over_budget_fp32", "over_budget_fp64", "steps_fp32")
+    keys = ("traced_fp32", "reintegrated_pass3", "escalated_fp64", "reintegrated_fp64", "over_budget_fp32",
+            "over_budget_fp64")
     for stop in ("integrated", None):
         runs = [labels.ondemand_asc(f, err, gthr, iters, presimp, ctx=ctx, stop=stop) for _ in range(3)]
         _say(f"reproducible {name} stop={stop}: {[{k: r[2][k] for k in keys} for r in runs]}")

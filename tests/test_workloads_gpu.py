@@ -4,6 +4,8 @@
   303): labels within 0.1 % of both committed FP64-oracle labellings
   (tests/golden/c3: the reference pipeline in FP64, and every vertex's exact
   FP64 destination), the same on a second call.
+* The same with the FP32-first precision policy (MSCEER_FP32_FIRST): within
+  0.1 % of both, with under a quarter of the advection steps in FP64.
 * The learned mask's evaluation family (N(0,1) noise smoothed with sigma 2.5,
   seed 5999) on a 256x192x128 volume: plain and learned labels each within
   0.1 % of every vertex's exact FP64 destination (oracle mode 2) on the same
@@ -63,6 +65,28 @@ def test_c3_labels_against_the_fp64_fixture(ctx):
     assert mism <= 1e-3
     assert mism_exact <= 1e-3
     assert (runs[1][0] == lab).all(), int((runs[1][0] != lab).sum())
+
+
+def test_c3_fp32_first_policy(ctx):
+    # MSCEER_FP32_FIRST: pass 3 in FP32, FP64 only for streamlines whose FP32
+    # trace meets a near-tie decision or its step budget; labels within 0.1 %
+    # of both FP64 fixtures, under a quarter of the advection steps in FP64,
+    # the same labels on a second call
+    f = synth.labels_field(303, 256)
+    ref, _ = load_c3_fp64()
+    exact, _ = load_c3_fp64(exact=True)
+    err, gthr, iters, presimp = C3_PARAMS
+    runs = [labels.ondemand_asc(f, err, gthr, iters, presimp, want_grad=False, ctx=ctx, fp32_first=True)
+            for _ in range(2)]
+    lab, st = runs[0][1].ravel(), runs[0][2]
+    share = st["steps_fp64"] / (st["steps_fp32"] + st["steps_fp64"])
+    _say(f"  FP32 first: mismatch vs FP64 oracle {(lab != ref).mean():.6f}, vs exact {(lab != exact).mean():.6f}, "
+         f"FP64 share {share:.4f}, escalated {st['escalated_fp64']} of {st['reintegrated_pass3']}")
+    assert (lab != ref).mean() <= 1e-3
+    assert (lab != exact).mean() <= 1e-3
+    assert share < 0.25
+    assert 0 < st["escalated_fp64"] < st["reintegrated_pass3"]
+    assert (runs[1][1] == runs[0][1]).all()
 
 
 def test_learned_family_volume_against_fp64_oracle(ctx):
