@@ -58,8 +58,9 @@ constexpr int kRoundBlocks = 2048;  // fixed grid of a round
 struct LGeom {
     uint32_t Mx, My, Mz;  // cell grid
     uint32_t sxy;         // Mx * My
-    uint32_t X, Y;        // vertex grid (record indices)
+    uint32_t X, Y;        // vertex grid
     uint32_t V;           // vertices
+    uint32_t nbx, nby;    // 4x4x4 vertex blocks of the record layout along x, y
 };
 
 __device__ __forceinline__ void cell_coords(const LGeom& g, uint32_t c, uint32_t& x, uint32_t& y, uint32_t& z) {
@@ -73,9 +74,17 @@ __device__ __forceinline__ uint32_t cell_id(const LGeom& g, uint32_t x, uint32_t
     return x + g.Mx * y + g.sxy * z;
 }
 
+// Records and visited bits are laid out by 4x4x4 blocks of vertices (64
+// vertices, 3 entries each: 384 record bytes of rec1, 192 bits): a walk
+// step moves to a neighbouring cell, whose entries then mostly share the
+// cache lines of the current one.  Slot of vertex (i, j, k):
+__device__ __forceinline__ uint32_t vslot(const LGeom& g, uint32_t i, uint32_t j, uint32_t k) {
+    return ((((k >> 2) * g.nby + (j >> 2)) * g.nbx + (i >> 2)) << 6) | ((k & 3u) << 4) | ((j & 3u) << 2) | (i & 3u);
+}
+
 // Record / bit index of the edge or quad at cell coordinates (x, y, z).
 __device__ __forceinline__ uint32_t rec_index(const LGeom& g, uint32_t x, uint32_t y, uint32_t z, uint32_t axis) {
-    return 3u * ((x >> 1) + g.X * ((y >> 1) + g.Y * (z >> 1))) + axis;
+    return 3u * vslot(g, x >> 1, y >> 1, z >> 1) + axis;
 }
 
 __device__ __forceinline__ bool test_bit(const uint32_t* bits, uint32_t i) { return (bits[i >> 5] >> (i & 31u)) & 1u; }
@@ -193,55 +202,43 @@ __device__ __forceinline__ void box_load(const uint8_t* __restrict__ grad, uint6
     }
 }
 
-// Rows of a bitmap of 3 bits per vertex: row r = (y0 + r % kVY, z0 + r / kVY)
-// from vertex x0 on, kVX vertices; word w of LDS row r is bitmap word
-// first[r] + w.
-template <int kVX>
-struct BitRows {
-    static constexpr int kWords = (3 * kVX + 31) / 32 + 2;  // with the word bits3 reads past the last
-};
-
-template <int kVX, int kVY, int kVZ>
-__device__ __forceinline__ void bits_load(const LGeom& g, const uint32_t* __restrict__ bits, uint32_t nwords, int x0,
-                                          int y0, int z0, uint32_t* lds, uint32_t* first) {
-    constexpr int kW = BitRows<kVX>::kWords, kRows = kVY * kVZ;
-    const uint32_t Zv = (g.Mz + 1) >> 1;
-    for (int r = threadIdx.x; r < kRows; r += blockDim.x) {
-        const uint32_t y = (uint32_t)(y0 + r % kVY), z = (uint32_t)(z0 + r / kVY);
-        first[r] = (3u * ((uint32_t)x0 + g.X * (y + g.Y * z))) >> 5;  // rows outside the grid: never read
-    }
-    __syncthreads();
-    for (int d = threadIdx.x; d < kRows * kW; d += blockDim.x) {
-        const int r = d / kW, w = d - r * kW;
-        const uint32_t y = (uint32_t)(y0 + r % kVY), z = (uint32_t)(z0 + r / kVY);
+// The visited bits of the 4x4x4 vertex blocks around a tile (kBX x kBY x
+// kBZ blocks from block (bx0, by0, bz0)): 6 words a block, as in the bitmap
+// (a block's 192 bits are word-aligned there), one more word of padding.
+template <int kBX, int kBY, int kBZ>
+__device__ __forceinline__ void bits_load(const LGeom& g, const uint32_t* __restrict__ bits, uint32_t nwords,
+                                          uint32_t bx0, uint32_t by0, uint32_t bz0, uint32_t* lds) {
+    constexpr int kN = kBX * kBY * kBZ * 6;
+    const uint32_t nbz = (((g.Mz + 1) >> 1) + 3u) >> 2;
+    for (int d = threadIdx.x; d <= kN; d += blockDim.x) {
         uint32_t v = 0;
-        if (y < g.Y && z < Zv) {
-            const uint32_t i = first[r] + (uint32_t)w;
-            if (i < nwords) v = bits[i];
+        if (d < kN) {
+            const int blk = d / 6, w = d - blk * 6;
+            const uint32_t bx = bx0 + (uint32_t)(blk % kBX), by = by0 + (uint32_t)((blk / kBX) % kBY);
+            const uint32_t bz = bz0 + (uint32_t)(blk / (kBX * kBY));
+            if (bx < g.nbx && by < g.nby && bz < nbz) {
+                const uint32_t i = 6u * ((bz * g.nby + by) * g.nbx + bx) + (uint32_t)w;
+                if (i < nwords) v = bits[i];
+            }
         }
         lds[d] = v;
     }
 }
 
-// 3 bits of vertex (x, y, z) from the staged rows (row r holds its vertex row).
-template <int kVX>
-__device__ __forceinline__ uint32_t bits3_lds(const uint32_t* lds, const uint32_t* first, int r, uint32_t vid) {
-    constexpr int kW = BitRows<kVX>::kWords;
-    const uint32_t i = 3u * vid - 32u * first[r];
-    const uint32_t* p = lds + r * kW + (i >> 5);
+// 3 bits of vertex (i, j, k) from the staged blocks (its block kBX x kBY x
+// kBZ-relative coordinates rb*).
+template <int kBX, int kBY>
+__device__ __forceinline__ uint32_t bits3_lds(const uint32_t* lds, uint32_t i, uint32_t j, uint32_t k, uint32_t rbx,
+                                              uint32_t rby, uint32_t rbz) {
+    const uint32_t inner = ((k & 3u) << 4) | ((j & 3u) << 2) | (i & 3u);
+    const uint32_t bit = 3u * inner;  // < 192: the 3 bits stay inside the block's 6 words
+    const uint32_t* p = lds + 6u * ((rbz * kBY + rby) * kBX + rbx) + (bit >> 5);
     const uint64_t w = ((uint64_t)p[1] << 32) | p[0];
-    return (uint32_t)(w >> (i & 31u)) & 7u;
+    return (uint32_t)(w >> (bit & 31u)) & 7u;
 }
 
 // ---- phase 1: records ------------------------------------------------------
 
-// 3 bits of vertex v in a bitmap of 3 bits per vertex (the bitmaps are padded
-// by one word, so the second load stays inside).
-__device__ __forceinline__ uint32_t bits3(const uint32_t* __restrict__ b, uint32_t v) {
-    const uint32_t i = 3u * v;
-    const uint64_t w = ((uint64_t)b[(i >> 5) + 1] << 32) | b[i >> 5];
-    return (uint32_t)(w >> (i & 31u)) & 7u;
-}
 
 // Record slot of the cofacet (byte b) reached from the item along axis a
 // with sign sgn; the item is an edge with odd axis o (kO < 3), or a quad
@@ -258,10 +255,11 @@ __device__ __forceinline__ uint32_t slot_code(uint32_t b) {
     return 8u | d;
 }
 
-// Vertex tile of the streaming passes: 32 x 8 x 4 vertices, a lane per (x, y)
-// column of 4 vertices.
+// Vertex tile of the streaming passes: 32 x 8 x 4 vertices (8 x 2 x 1 of the
+// 4x4x4 blocks of the record layout).
 constexpr int kTX = 32, kTY = 8, kTZ = 4;
 static_assert(kTX * kTY == kThreads, "one lane per tile column");
+static_assert(kTX % 4 == 0 && kTY % 4 == 0 && kTZ == 4, "tiles of whole 4x4x4 vertex blocks");
 using RecBox = CellBox<2 * kTX + 1, 2 * kTY + 1, 2 * kTZ + 1>;  // cells 2 x0 - 1 .. 2 (x0 + kTX) - 1
 using FinBox = CellBox<2 * kTX + 1, 2 * kTY + 1, 2 * kTZ + 1>;  // cells 2 x0 .. 2 (x0 + kTX)
 
@@ -286,13 +284,15 @@ ldir_records_kernel(const uint8_t* __restrict__ grad, LGeom g, uint64_t cells, u
     __syncthreads();
     const uint8_t* st8 = reinterpret_cast<const uint8_t*>(stage);
     const uint32_t Zv = (g.Mz + 1) >> 1;
-    const int lx = (int)(threadIdx.x % kTX), ly = (int)(threadIdx.x / kTX);
-    const uint32_t i = (uint32_t)(x0 + lx), j = (uint32_t)(y0 + ly);
-    if (i >= g.X || j >= g.Y) return;
-    for (int lz = 0; lz < kTZ; lz++) {
-        const uint32_t k = (uint32_t)(z0 + lz);
-        if (k >= Zv) break;
-        const uint32_t v = i + g.X * (j + g.Y * k);
+    // a wave per 4x4x4 vertex block of the tile (8 x 2 x 1 blocks), so that
+    // its records are written as one contiguous run
+    for (int it = 0; it < (kTX / 4) * (kTY / 4) * (kTZ / 4) / (kThreads / 64); it++) {
+        const int blk = it * (kThreads / 64) + (int)(threadIdx.x >> 6), inner = (int)(threadIdx.x & 63u);
+        const int lx = (blk % (kTX / 4)) * 4 + (inner & 3), ly = (blk / (kTX / 4)) * 4 + ((inner >> 2) & 3);
+        const int lz = inner >> 4;
+        const uint32_t i = (uint32_t)(x0 + lx), j = (uint32_t)(y0 + ly), k = (uint32_t)(z0 + lz);
+        if (i >= g.X || j >= g.Y || k >= Zv) continue;
+        const uint32_t v = vslot(g, i, j, k);
         // byte of the cell at offset (dx, dy, dz) from the vertex's cell
         auto B = [&](int dx, int dy, int dz) -> uint32_t {
             return st8[rows[(2 * ly + 1 + dy) + kRY * (2 * lz + 1 + dz)].off + (2 * lx + 1 + dx)];
@@ -618,37 +618,39 @@ __global__ void __launch_bounds__(kThreads)
 ldir_finish_kernel(uint8_t* __restrict__ grad, LGeom g, uint64_t cells, uint32_t align, int ntx, int nty,
                    const uint32_t* __restrict__ v1, const uint32_t* __restrict__ v2, uint32_t nwords) {
     constexpr int kRX = 2 * kTX + 1, kRY = 2 * kTY + 1, kRZ = 2 * kTZ + 1;
-    constexpr int kVX = kTX + 1, kVY = kTY + 1, kVZ = kTZ + 1, kW = BitRows<kVX>::kWords;
+    // vertex blocks of the tile and its "+" halo layer
+    constexpr int kBX = kTX / 4 + 1, kBY = kTY / 4 + 1, kBZ = kTZ / 4 + 1, kBW = kBX * kBY * kBZ * 6 + 1;
     __shared__ uint32_t stage[FinBox::kRows * FinBox::kDw];
     __shared__ BoxRow rows[FinBox::kRows];
-    __shared__ uint32_t b1[kVY * kVZ * kW], b2[kVY * kVZ * kW];
-    __shared__ uint32_t f1[kVY * kVZ], f2[kVY * kVZ];
+    __shared__ uint32_t b1[kBW], b2[kBW];
     const int tile = (int)blockIdx.x;
     const int x0 = (tile % ntx) * kTX, y0 = ((tile / ntx) % nty) * kTY, z0 = (tile / (ntx * nty)) * kTZ;
+    const uint32_t bx0 = (uint32_t)x0 >> 2, by0 = (uint32_t)y0 >> 2, bz0 = (uint32_t)z0 >> 2;
     box_rows<kRX, kRY, kRZ>(g, align, 2 * x0, 2 * y0, 2 * z0, rows);
-    bits_load<kVX, kVY, kVZ>(g, v1, nwords, x0, y0, z0, b1, f1);  // (synchronises)
-    bits_load<kVX, kVY, kVZ>(g, v2, nwords, x0, y0, z0, b2, f2);
+    bits_load<kBX, kBY, kBZ>(g, v1, nwords, bx0, by0, bz0, b1);
+    bits_load<kBX, kBY, kBZ>(g, v2, nwords, bx0, by0, bz0, b2);
+    __syncthreads();
     box_load<kRX, kRY, kRZ>(grad, cells, align, rows, stage);
     __syncthreads();
     uint8_t* st8 = reinterpret_cast<uint8_t*>(stage);
     const uint32_t Zv = (g.Mz + 1) >> 1;
     const int lx = (int)(threadIdx.x % kTX), ly = (int)(threadIdx.x / kTX);
     const uint32_t i = (uint32_t)(x0 + lx), j = (uint32_t)(y0 + ly);
-    const uint32_t sy = g.X, sz = g.X * g.Y;
     for (int lz = 0; lz < kTZ; lz++) {
         const uint32_t k = (uint32_t)(z0 + lz);
         if (i >= g.X || j >= g.Y || k >= Zv) break;
-        const uint32_t v = i + g.X * (j + g.Y * k);
         // cell at offset (dx, dy, dz) from the cell of vertex (lx + ex, ly + ey, lz + ez)
         auto at = [&](int dx, int dy, int dz) -> uint8_t& {
             return st8[rows[(2 * ly + dy) + kRY * (2 * lz + dz)].off + (2 * lx + dx)];
         };
         // bits of the vertex (ex, ey, ez) vertices further on
         auto e1 = [&](int ex, int ey, int ez) {
-            return bits3_lds<kVX>(b1, f1, (ly + ey) + kVY * (lz + ez), v + (uint32_t)ex + sy * ey + sz * ez);
+            const uint32_t a = i + (uint32_t)ex, b = j + (uint32_t)ey, c = k + (uint32_t)ez;
+            return bits3_lds<kBX, kBY>(b1, a, b, c, (a >> 2) - bx0, (b >> 2) - by0, (c >> 2) - bz0);
         };
         auto e2 = [&](int ex, int ey, int ez) {
-            return bits3_lds<kVX>(b2, f2, (ly + ey) + kVY * (lz + ez), v + (uint32_t)ex + sy * ey + sz * ez);
+            const uint32_t a = i + (uint32_t)ex, b = j + (uint32_t)ey, c = k + (uint32_t)ez;
+            return bits3_lds<kBX, kBY>(b2, a, b, c, (a >> 2) - bx0, (b >> 2) - by0, (c >> 2) - bz0);
         };
         const bool px = i + 1 < g.X, py = j + 1 < g.Y, pz = k + 1 < Zv;
         const uint32_t e_v = e1(0, 0, 0), q_v = e2(0, 0, 0);
@@ -728,7 +730,13 @@ __global__ void zero_kernel(uint32_t* p, uint32_t n) {
 }
 
 LGeom make_geom(const Dims& d) {
-    return LGeom{d.Mx, d.My, d.Mz, d.Mx * d.My, (uint32_t)d.X, (uint32_t)d.Y, (uint32_t)d.verts};
+    return LGeom{d.Mx, d.My, d.Mz, d.Mx * d.My, (uint32_t)d.X, (uint32_t)d.Y, (uint32_t)d.verts,
+                 (uint32_t)((d.X + 3) / 4), (uint32_t)((d.Y + 3) / 4)};
+}
+
+// Entries of the block layout: 3 per vertex slot, 64 slots per 4x4x4 block.
+uint64_t layout_entries(const Dims& d) {
+    return 3ull * 64ull * (uint64_t)((d.X + 3) / 4) * (uint64_t)((d.Y + 3) / 4) * (uint64_t)((d.Z + 3) / 4);
 }
 
 uint32_t grid_of(uint64_t n, uint32_t cap) {
@@ -740,7 +748,7 @@ uint32_t grid_of(uint64_t n, uint32_t cap) {
 }  // namespace
 
 size_t ldir_scratch_bytes(const Dims& d, LdirScratch* s) {
-    const uint64_t n = 3ull * d.verts;
+    const uint64_t n = layout_entries(d);
     const uint64_t words = (n + 31) / 32 + 1;  // one word of padding: bits3 reads a word pair
     const size_t r1 = (2 * n + 255) & ~(size_t)255, r2 = (n + 255) & ~(size_t)255;
     const size_t bw = ((words * 4) + 255) & ~(size_t)255;
