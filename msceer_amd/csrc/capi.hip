@@ -55,7 +55,8 @@ MList front_list(msceer_ctx* ctx, const ListCaps& c, int k) {
 // One ldir pass (1: from the critical edges, 2: from the critical quads).
 // Rounds are enqueued in groups without host synchronisation (each reads its
 // counts on the device); after each group one read-back shows whether the
-// frontier emptied.  A sweep rebuilds the frontier after any overflow of the
+// frontier emptied.  The first group is short: the start round's work-groups
+// usually finish their whole manifolds, and the round after it the rest.  A sweep rebuilds the frontier after any overflow of the
 // start list or of a frontier.
 int ldir_pass(msceer_ctx* ctx, const Dims& d, uint8_t* grad, const LdirScratch& ls, int pass, const MList& start,
               bool start_valid, const ListCaps& caps, int* rounds) {
@@ -64,7 +65,8 @@ int ldir_pass(msceer_ctx* ctx, const Dims& d, uint8_t* grad, const LdirScratch& 
     void* scratch = ctx->ldir.ptr;
     auto front = [&](int k) { return front_list(ctx, caps, k); };
     auto host_total = [&](int k) { return list_total(ctx->h_ctl + kCtlRound + (k % 3) * kCountWords, front(k)); };
-    constexpr int kGroup = 8;
+    constexpr int kFirstGroup = 2, kGroup = 8;
+    int group = kFirstGroup;
     MList in = start;
     bool is_start = true;
     int k = 0;
@@ -88,7 +90,7 @@ int ldir_pass(msceer_ctx* ctx, const Dims& d, uint8_t* grad, const LdirScratch& 
             k++;
             CHECK_HIP(launch_zero(front(k).counts, kCountWords, ctx->stream));
         }
-        for (int j = 0; j < kGroup; j++) {
+        for (int j = 0; j < group; j++) {
             const MList out = front(k);
             CHECK_HIP(launch_ldir_round(d, scratch, ls, pass, is_start, in, out, front(k + 1).counts, overflow,
                                         ctx->stream));
@@ -97,6 +99,7 @@ int ldir_pass(msceer_ctx* ctx, const Dims& d, uint8_t* grad, const LdirScratch& 
             is_start = false;
             k++;
         }
+        group = kGroup;
         int rc = read_ctl(ctx, 0, kCtlWords);
         if (rc) return rc;
         if (ctx->h_ctl[kCtlOverflow]) {

@@ -152,10 +152,15 @@ struct BoxRow {
     int32_t dfirst;  // first and last aligned dwords with cells of the grid
     int32_t dlast;   // (dlast < dfirst: the row lies outside the grid)
     int32_t off;     // LDS byte offset of the box's first cell: r * kPitch + sh
+    uint32_t lo;     // .grad bytes [lo, hi) of the row's cells from the box's first one
+    uint32_t hi;     // up to kWX of them (the cells a finish tile writes)
+    int32_t wfirst;  // aligned dwords wholly inside [lo, hi)
+    int32_t wlast;
 };
 
 // Aligned dword index of .grad byte i: (align + i) >> 2, align = address & 3.
-template <int kRX, int kRY, int kRZ>
+// kWX: the cells of a row a finish tile writes, from the box's first one.
+template <int kRX, int kRY, int kRZ, int kWX = 0>
 __device__ __forceinline__ void box_rows(const LGeom& g, uint32_t align, int cx0, int cy0, int cz0, BoxRow* rows) {
     using Bx = CellBox<kRX, kRY, kRZ>;
     for (int r = threadIdx.x; r < Bx::kRows; r += blockDim.x) {
@@ -165,6 +170,9 @@ __device__ __forceinline__ void box_rows(const LGeom& g, uint32_t align, int cx0
         b.dlast = 0;
         b.dbase = 0;
         b.off = r * Bx::kPitch;
+        b.lo = b.hi = 0u;
+        b.wfirst = 1;
+        b.wlast = 0;
         if (cy >= 0 && cz >= 0 && (uint32_t)cy < g.My && (uint32_t)cz < g.Mz) {
             const int64_t rs = (int64_t)g.Mx * ((int64_t)cy + (int64_t)g.My * cz);
             const int64_t gs = (int64_t)align + rs + cx0;  // aligned position of the box's first cell (maybe -1 + rs)
@@ -174,31 +182,39 @@ __device__ __forceinline__ void box_rows(const LGeom& g, uint32_t align, int cx0
             b.dfirst = (int32_t)(lo >> 2);
             b.dlast = (int32_t)((hi - 1) >> 2);
             b.off += (int32_t)(gs - 4 * (int64_t)b.dbase);
+            if (kWX > 0 && cx0 >= 0) {
+                const int64_t wl = rs + cx0, wh = rs + (cx0 + kWX < (int)g.Mx ? cx0 + kWX : (int)g.Mx);
+                b.lo = (uint32_t)wl;
+                b.hi = (uint32_t)wh;
+                b.wfirst = (int32_t)((wl + align + 3) >> 2);
+                b.wlast = (int32_t)((wh + align) >> 2) - 1;
+            }
         }
         rows[r] = b;
     }
 }
 
+// jf, jl: the aligned dwords wholly inside the .grad allocation's [0, C).
 template <int kRX, int kRY, int kRZ>
 __device__ __forceinline__ void box_load(const uint8_t* __restrict__ grad, uint64_t cells, uint32_t align,
                                          const BoxRow* rows, uint32_t* stage) {
     using Bx = CellBox<kRX, kRY, kRZ>;
     const uint32_t* g32 = reinterpret_cast<const uint32_t*>(grad - align);
+    const int32_t jf = (int32_t)((align + 3u) >> 2), jl = (int32_t)(((uint64_t)align + cells) >> 2) - 1;
     for (int d = threadIdx.x; d < Bx::kRows * Bx::kDw; d += blockDim.x) {
         const int r = d / Bx::kDw, k = d - r * Bx::kDw;
-        const BoxRow b = rows[r];
-        const int32_t j = b.dfirst + k;
-        if (j > b.dlast) continue;
+        const int32_t j = rows[r].dfirst + k;
+        if (j > rows[r].dlast) continue;
         uint32_t w;
-        const int64_t first = 4 * (int64_t)j - (int64_t)align;  // .grad index of the dword's first byte
-        if (first >= 0 && first + 4 <= (int64_t)cells) {
+        if (j >= jf && j <= jl) {
             w = g32[j];
-        } else {
+        } else {  // the allocation's first or last dword: byte by byte
             w = 0;
+            const int64_t first = 4 * (int64_t)j - (int64_t)align;  // .grad index of the dword's first byte
             for (int t = 0; t < 4; t++)
                 if (first + t >= 0 && first + t < (int64_t)cells) w |= (uint32_t)grad[first + t] << (8 * t);
         }
-        stage[(r * Bx::kPitch) / 4 + (j - b.dbase)] = w;
+        stage[(r * Bx::kPitch) / 4 + (j - rows[r].dbase)] = w;
     }
 }
 
@@ -622,15 +638,25 @@ ldir_finish_kernel(uint8_t* __restrict__ grad, LGeom g, uint64_t cells, uint32_t
     constexpr int kBX = kTX / 4 + 1, kBY = kTY / 4 + 1, kBZ = kTZ / 4 + 1, kBW = kBX * kBY * kBZ * 6 + 1;
     __shared__ uint32_t stage[FinBox::kRows * FinBox::kDw];
     __shared__ BoxRow rows[FinBox::kRows];
+    // per vertex of the tile and its halo: visited1 bits | visited2 bits << 3
+    constexpr int kVX = kTX + 1, kVY = kTY + 1, kVZ = kTZ + 1;
     __shared__ uint32_t b1[kBW], b2[kBW];
+    __shared__ uint8_t vbits[kVX * kVY * kVZ];
     const int tile = (int)blockIdx.x;
     const int x0 = (tile % ntx) * kTX, y0 = ((tile / ntx) % nty) * kTY, z0 = (tile / (ntx * nty)) * kTZ;
     const uint32_t bx0 = (uint32_t)x0 >> 2, by0 = (uint32_t)y0 >> 2, bz0 = (uint32_t)z0 >> 2;
-    box_rows<kRX, kRY, kRZ>(g, align, 2 * x0, 2 * y0, 2 * z0, rows);
+    box_rows<kRX, kRY, kRZ, 2 * kTX>(g, align, 2 * x0, 2 * y0, 2 * z0, rows);
     bits_load<kBX, kBY, kBZ>(g, v1, nwords, bx0, by0, bz0, b1);
     bits_load<kBX, kBY, kBZ>(g, v2, nwords, bx0, by0, bz0, b2);
     __syncthreads();
     box_load<kRX, kRY, kRZ>(grad, cells, align, rows, stage);
+    for (int t = threadIdx.x; t < kVX * kVY * kVZ; t += kThreads) {
+        const uint32_t a = (uint32_t)(x0 + t % kVX), b = (uint32_t)(y0 + (t / kVX) % kVY);
+        const uint32_t c = (uint32_t)(z0 + t / (kVX * kVY));
+        const uint32_t rbx = (a >> 2) - bx0, rby = (b >> 2) - by0, rbz = (c >> 2) - bz0;
+        vbits[t] = (uint8_t)(bits3_lds<kBX, kBY>(b1, a, b, c, rbx, rby, rbz) |
+                             (bits3_lds<kBX, kBY>(b2, a, b, c, rbx, rby, rbz) << 3));
+    }
     __syncthreads();
     uint8_t* st8 = reinterpret_cast<uint8_t*>(stage);
     const uint32_t Zv = (g.Mz + 1) >> 1;
@@ -644,13 +670,11 @@ ldir_finish_kernel(uint8_t* __restrict__ grad, LGeom g, uint64_t cells, uint32_t
             return st8[rows[(2 * ly + dy) + kRY * (2 * lz + dz)].off + (2 * lx + dx)];
         };
         // bits of the vertex (ex, ey, ez) vertices further on
-        auto e1 = [&](int ex, int ey, int ez) {
-            const uint32_t a = i + (uint32_t)ex, b = j + (uint32_t)ey, c = k + (uint32_t)ez;
-            return bits3_lds<kBX, kBY>(b1, a, b, c, (a >> 2) - bx0, (b >> 2) - by0, (c >> 2) - bz0);
+        auto e1 = [&](int ex, int ey, int ez) -> uint32_t {
+            return vbits[(lx + ex) + kVX * ((ly + ey) + kVY * (lz + ez))] & 7u;
         };
-        auto e2 = [&](int ex, int ey, int ez) {
-            const uint32_t a = i + (uint32_t)ex, b = j + (uint32_t)ey, c = k + (uint32_t)ez;
-            return bits3_lds<kBX, kBY>(b2, a, b, c, (a >> 2) - bx0, (b >> 2) - by0, (c >> 2) - bz0);
+        auto e2 = [&](int ex, int ey, int ez) -> uint32_t {
+            return (uint32_t)vbits[(lx + ex) + kVX * ((ly + ey) + kVY * (lz + ez))] >> 3;
         };
         const bool px = i + 1 < g.X, py = j + 1 < g.Y, pz = k + 1 < Zv;
         const uint32_t e_v = e1(0, 0, 0), q_v = e2(0, 0, 0);
@@ -706,21 +730,16 @@ ldir_finish_kernel(uint8_t* __restrict__ grad, LGeom g, uint64_t cells, uint32_t
     for (int d = threadIdx.x; d < kWY * kWZ * FinBox::kDw; d += kThreads) {
         const int wr = d / FinBox::kDw, kk = d - wr * FinBox::kDw;
         const int r = (wr % kWY) + kRY * (wr / kWY);
-        const BoxRow br = rows[r];
-        const int32_t jj = br.dfirst + kk;
-        if (jj > br.dlast) continue;
-        // the tile's cells of the row: .grad bytes [lo, hi)
-        const int cy = 2 * y0 + wr % kWY, cz = 2 * z0 + wr / kWY;
-        const int64_t rs = (int64_t)g.Mx * ((int64_t)cy + (int64_t)g.My * cz);
-        const int64_t lo = rs + 2 * x0;
-        const int64_t hi = rs + (2 * x0 + 2 * kTX < (int)g.Mx ? 2 * x0 + 2 * kTX : (int)g.Mx);
-        const int64_t first = 4 * (int64_t)jj - (int64_t)align;
-        const uint32_t w = stage[(r * FinBox::kPitch) / 4 + (jj - br.dbase)];
-        if (first >= lo && first + 4 <= hi) {
+        const int32_t jj = rows[r].dfirst + kk;
+        if (jj > rows[r].dlast) continue;
+        const uint32_t w = stage[(r * FinBox::kPitch) / 4 + (jj - rows[r].dbase)];
+        if (jj >= rows[r].wfirst && jj <= rows[r].wlast) {
             g32[jj] = w;
-        } else {
+        } else {  // a dword at an end of the tile's cells of the row: its own bytes only
+            const uint32_t lo = rows[r].lo, hi = rows[r].hi;
+            const int64_t first = 4 * (int64_t)jj - (int64_t)align;
             for (int t = 0; t < 4; t++)
-                if (first + t >= lo && first + t < hi) grad[first + t] = (uint8_t)(w >> (8 * t));
+                if (first + t >= (int64_t)lo && first + t < (int64_t)hi) grad[first + t] = (uint8_t)(w >> (8 * t));
         }
     }
 }

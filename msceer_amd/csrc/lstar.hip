@@ -112,16 +112,28 @@ lstar_kernel(const float* __restrict__ values, uint8_t* __restrict__ grad,
     const int tx = flat % kBX, ty = (flat / kBX) % kBY, tz = flat / (kBX * kBY);
     // every vertex of the brick and its whole star inside the domain
     const bool inner = x0 >= 1 && y0 >= 1 && z0 >= 1 && x0 + kBX < X && y0 + kBY < Y && z0 + kBZ < Z;
-    for (int i = flat; i < kTileN; i += kThreads) {
-        const int ix = i % kTX, iy = (i / kTX) % kTY, iz = i / kTXY;
-        int gx = x0 - 1 + ix, gy = y0 - 1 + iy, gz = z0 - 1 + iz;
-        if (!inner) {
+    if (inner) {
+        // tile row (iy, iz) starts at vertex id corner + X iy + X Y iz
+        __shared__ uint32_t s_trow[kTY * B::kTZ];
+        if (flat < kTY * B::kTZ)
+            s_trow[flat] = (uint32_t)X * ((uint32_t)(flat % kTY) + (uint32_t)Y * (uint32_t)(flat / kTY));
+        __syncthreads();
+        const uint32_t corner = (uint32_t)(x0 - 1) + (uint32_t)X * ((uint32_t)(y0 - 1) + (uint32_t)Y * (uint32_t)(z0 - 1));
+        for (int i = flat; i < kTileN; i += kThreads) {
+            const int row = i / kTX, ix = i - row * kTX;
+            const uint32_t id = corner + (uint32_t)ix + s_trow[row];
+            tile[i] = ((uint64_t)float_key(values[id]) << 32) | id;
+        }
+    } else {
+        for (int i = flat; i < kTileN; i += kThreads) {
+            const int ix = i % kTX, iy = (i / kTX) % kTY, iz = i / kTXY;
+            int gx = x0 - 1 + ix, gy = y0 - 1 + iy, gz = z0 - 1 + iz;
             gx = gx < 0 ? 0 : (gx >= X ? X - 1 : gx);
             gy = gy < 0 ? 0 : (gy >= Y ? Y - 1 : gy);
             gz = gz < 0 ? 0 : (gz >= Z ? Z - 1 : gz);
+            const uint32_t id = (uint32_t)gx + (uint32_t)X * ((uint32_t)gy + (uint32_t)Y * (uint32_t)gz);
+            tile[i] = ((uint64_t)float_key(values[id]) << 32) | id;
         }
-        const uint32_t id = (uint32_t)gx + (uint32_t)X * ((uint32_t)gy + (uint32_t)Y * (uint32_t)gz);
-        tile[i] = ((uint64_t)float_key(values[id]) << 32) | id;
     }
     for (int i = flat; i < kStageW; i += kThreads) stage[i] = 0u;
     for (int r = flat; r < kRows; r += kThreads) {
