@@ -271,9 +271,9 @@ __device__ __forceinline__ uint32_t slot_code(uint32_t b) {
     return 8u | d;
 }
 
-// Vertex tile of the streaming passes: 32 x 8 x 4 vertices (8 x 2 x 1 of the
+// Vertex tile of the streaming passes: 64 x 4 x 4 vertices (16 x 1 x 1 of the
 // 4x4x4 blocks of the record layout).
-constexpr int kTX = 32, kTY = 8, kTZ = 4;
+constexpr int kTX = 64, kTY = 4, kTZ = 4;
 static_assert(kTX * kTY == kThreads, "one lane per tile column");
 static_assert(kTX % 4 == 0 && kTY % 4 == 0 && kTZ == 4, "tiles of whole 4x4x4 vertex blocks");
 using RecBox = CellBox<2 * kTX + 1, 2 * kTY + 1, 2 * kTZ + 1>;  // cells 2 x0 - 1 .. 2 (x0 + kTX) - 1
