@@ -261,14 +261,16 @@ __device__ __forceinline__ uint32_t bits3_lds(const uint32_t* lds, uint32_t i, u
 // (kO == 3) whose pair partners are always quads.  8 | pair direction when
 // the walk goes on through it: assigned, not critical, the pair a facet
 // (for an edge item: along o or a) and not the item itself.
+// The pair directions that qualify form a constant set per slot: for an
+// edge item the two directions of axes o and a, for a quad item all six,
+// less the one pointing straight back along a (code 7, critical, never; nor
+// 6, which no .grad byte holds).
 template <int kO, int kA, int kSgn>
 __device__ __forceinline__ uint32_t slot_code(uint32_t b) {
+    constexpr uint32_t kBack = 2u * (uint32_t)kA + (kSgn > 0 ? 1u : 0u);
+    constexpr uint32_t kOk = (kO < 3 ? ((3u << (2 * kO)) | (3u << (2 * kA))) : 0x3Fu) & ~(1u << kBack);
     const uint32_t d = (b >> 2) & 7u;
-    if (!(b & 1u) || d == 7u) return 0u;
-    const uint32_t ax = d >> 1;
-    if (kO < 3 && ax != (uint32_t)kO && ax != (uint32_t)kA) return 0u;
-    if (ax == (uint32_t)kA && ((d & 1u) != 0u) == (kSgn > 0)) return 0u;
-    return 8u | d;
+    return ((b & 1u) && ((kOk >> d) & 1u)) ? (8u | d) : 0u;
 }
 
 // Vertex tile of the streaming passes: 64 x 4 x 4 vertices (16 x 1 x 1 of the
