@@ -86,6 +86,12 @@ typedef struct msceer_label_stats {
     float ms_pass3;             /* integration pass 3 */
     int64_t reintegrated_pass3;  /* pass-3 re-integrations (all of them, whatever their precision) */
     int64_t escalated_fp64;     /* with MSCEER_FP32_FIRST: of those, left in doubt by the FP32 trace and traced again in FP64 */
+    int64_t lines_asc;          /* msceer_ondemand, needASC1: streamlines traced from the 2-saddles (speculation included) */
+    int64_t lines_dsc;          /* needDSC1: streamlines traced from the 1-saddles */
+    int64_t lines_replayed;     /* of the 1-manifold streamlines, re-traced in order inside a component */
+    int32_t line_windows;       /* speculation windows of the 1-manifold stage */
+    int32_t line_cuts;          /* windows cut short, and lines traced alone */
+    float ms_lines;             /* the 1-manifold stage */
 } msceer_label_stats;
 
 int msceer_version(void);
@@ -166,6 +172,37 @@ int msceer_lstar_grad_noalloc(msceer_ctx* ctx, const float* values, int64_t nx, 
 int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz, float err_thr,
                         float grad_thr, int32_t max_iters, float presimp, const uint8_t* integrate_mask,
                         uint8_t* grad_out, int32_t* labels_out, msceer_label_stats* stats, uint32_t flags);
+
+/* Accuracy branches of OnDemandAccurate (argv needASC1 needDSC1 needASC2,
+ * ondemandaccurate.cxx:158-185): bits of `need` for msceer_ondemand.
+ * needDSC2 (descending 3-manifolds) is not provided. */
+#define MSCEER_NEED_ASC1 0x1u /* ascending 1-manifolds: ReIntegrateUpFrom2Saddles */
+#define MSCEER_NEED_DSC1 0x2u /* descending 1-manifolds: ReIntegrateDownFrom1Saddles */
+#define MSCEER_NEED_ASC2 0x4u /* ascending 3-manifolds: msceer_ondemand_asc's labelling */
+
+/*
+ * OnDemandAccurate with any nonzero combination of needASC1, needDSC1 and
+ * needASC2 (ondemandaccurate.cxx:704-1157).  Replaces, beside what
+ * msceer_ondemand_asc replaces:
+ *   DigitizingNumericStreamlineIntegrator3dASC / 3dDSC
+ *   (gi_numeric_streamline_integrator.h:526-1072) driven by
+ *   ReIntegrateUpFrom2Saddles / ReIntegrateDownFrom1Saddles
+ *   (ondemandaccurate.cxx:254-500, 1075-1091): from every critical 2-cell
+ *   (critical 1-cell), in the order one reference thread takes them,
+ *   streamlines uphill (downhill) from the seed cells of a short V-path
+ *   search, digitized into the restriction labels as 4 (5), each stopping on
+ *   a cell an earlier one labelled.  The GPU keeps that order exactly.
+ * The streamlines run in FP64 on the gradient field, zeroed at the region
+ * extrema when needASC2 ran first (as the reference's shared field is).
+ * restriction_out (cells int8, may be NULL): the restriction labels the
+ * conforming gradient was computed under (MIN boundary 1 when needASC2,
+ * 4 / 5 on the digitized 1-manifolds).  labels_out, integrate_mask and the
+ * STOP flags need needASC2.  Other arguments as msceer_ondemand_asc.
+ */
+int msceer_ondemand(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz, float err_thr,
+                    float grad_thr, int32_t max_iters, float presimp, uint32_t need, const uint8_t* integrate_mask,
+                    uint8_t* grad_out, int32_t* labels_out, int8_t* restriction_out, msceer_label_stats* stats,
+                    uint32_t flags);
 
 /*
  * The numeric integrator alone on given terminals: replaces

@@ -1,14 +1,17 @@
 // OnDemandAccurate X Y Z filename error_threshold grad_threshold maxnumiter
 //                  needASC1 needDSC1 needASC2 needDSC2 PresimpThesh [parallelism] [outputdebug]
 //
-// Drop-in for the volume path of ondemandaccurate/ondemandaccurate.cxx:704-1063
-// with needASC2 (accurate ascending 3-manifolds): writes <filename>.grad, the
+// Drop-in for the volume path of ondemandaccurate/ondemandaccurate.cxx:704-1157
+// with any of needASC1 / needDSC1 (accurate 1-manifolds from the saddles) and
+// needASC2 (accurate ascending 3-manifolds): writes <filename>.grad, the
 // timing file <filename>.<parallelism:%03d>.gtime.txt and, with outputdebug
 // set, the per-vertex int32 labels <filename>._int_XxYxZ.raw (the
-// accurate3dmorselabeling output format).  Without any accuracy flag it
-// writes the plain lower-star .grad, as the reference does.  The 1-manifold
-// (needASC1/needDSC1) and descending (needDSC2) branches are not built here
-// and are rejected with a message.
+// accurate3dmorselabeling output format, needASC2) and the int8 restriction
+// labels boundaries_<Mx>x<My>x<Mz>.raw in the working directory (the name
+// ondemandaccurate.cxx:1006-1011 gives them).  Without any accuracy flag it
+// writes the plain lower-star .grad, as the reference does.  The descending
+// 3-manifold branch (needDSC2) is not built here and is rejected with a
+// message.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -34,8 +37,8 @@ int main(int argc, char** argv) {
     printf("dims=(%d,%d,%d)\nfile=%s\nintegration parameters: e=%f, gt=%f, il=%d\n"
            "ondemandacc: a1=%d, d1=%d, a2=%d, d2=%d, ps=%f\npar=%d\n",
            X, Y, Z, filename.c_str(), err, gthr, iters, a1, d1, a2, d2, presimp, par);
-    if (a1 || d1 || d2) {
-        fprintf(stderr, "only the ascending 3-manifold branch (needASC2) is available\n");
+    if (d2) {
+        fprintf(stderr, "the descending 3-manifold branch (needDSC2) is not available\n");
         return 2;
     }
     const size_t nv = (size_t)X * Y * Z;
@@ -57,28 +60,42 @@ int main(int argc, char** argv) {
     char tname[2048];
     snprintf(tname, sizeof(tname), "%s.%03d.gtime.txt", filename.c_str(), par);
     FILE* ft = fopen(tname, "w");
-    if (a2) {
-        rc = msceer_ondemand_asc(ctx, values.data(), X, Y, Z, err, gthr, iters, presimp, nullptr, grad.data(),
-                                 labels.data(), &st, MSCEER_TIMING);
-        if (rc) { fprintf(stderr, "msceer_ondemand_asc: %s\n", msceer_strerror(rc)); msceer_destroy(ctx); return 2; }
+    if (a1 || d1 || a2) {
+        const uint32_t need = (a1 ? MSCEER_NEED_ASC1 : 0u) | (d1 ? MSCEER_NEED_DSC1 : 0u) | (a2 ? MSCEER_NEED_ASC2 : 0u);
+        std::vector<signed char> restr(dbg ? nc : 0);
+        rc = msceer_ondemand(ctx, values.data(), X, Y, Z, err, gthr, iters, presimp, need, nullptr, grad.data(),
+                             a2 ? labels.data() : nullptr, dbg ? reinterpret_cast<int8_t*>(restr.data()) : nullptr,
+                             &st, MSCEER_TIMING);
+        if (rc) { fprintf(stderr, "msceer_ondemand: %s\n", msceer_strerror(rc)); msceer_destroy(ctx); return 2; }
         // task lines "name start end duration" in ms, as StartTask/EndTaskAndRecord write them
-        const float parts[5] = {st.ms_lstar, st.ms_regions, st.ms_integrate, st.ms_cleanup, st.ms_grad};
-        const char* names[5] = {"Topological Robins0", "Topological SimExtRBASC", "NumericalTracing NewIntegrationASC",
-                                "Removed CleanerASC", "Topological ConformingGrad"};
+        const float parts[6] = {st.ms_lstar, st.ms_regions, st.ms_integrate, st.ms_cleanup, st.ms_lines, st.ms_grad};
+        const char* names[6] = {"Topological Robins0", "Topological SimExtRBASC", "NumericalTracing NewIntegrationASC",
+                                "Removed CleanerASC", "Numerical SaddleInt", "Topological ConformingGrad"};
         float t = 0.f;
-        for (int i = 0; i < 5; i++) {
+        for (int i = 0; i < 6; i++) {
+            if ((i >= 1 && i <= 3 && !a2) || (i == 4 && !(a1 || d1))) continue;
             if (ft) fprintf(ft, "%s %d %d %d\n", names[i], (int)t, (int)(t + parts[i]), (int)parts[i]);
             printf("TIMING: %s %d %d %d\n", names[i], (int)t, (int)(t + parts[i]), (int)parts[i]);
             t += parts[i];
         }
-        printf("built minmaxgraph: %lld minima, %lld saddles -> %lld regions\n", (long long)st.minima,
-               (long long)st.saddles, (long long)st.regions);
-        printf("re-integrated %lld vertices (%lld traced in FP64)\n", (long long)st.reintegrated_pass3,
-               (long long)st.reintegrated_fp64);
-        if (dbg) {
+        if (a2) {
+            printf("built minmaxgraph: %lld minima, %lld saddles -> %lld regions\n", (long long)st.minima,
+                   (long long)st.saddles, (long long)st.regions);
+            printf("re-integrated %lld vertices (%lld traced in FP64)\n", (long long)st.reintegrated_pass3,
+                   (long long)st.reintegrated_fp64);
+        }
+        if (a1 || d1)
+            printf("1-manifold streamlines: %lld from 2-saddles, %lld from 1-saddles (%lld re-traced in order)\n",
+                   (long long)st.lines_asc, (long long)st.lines_dsc, (long long)st.lines_replayed);
+        if (dbg && a2) {
             char lname[2048];
             snprintf(lname, sizeof(lname), "%s._int_%dx%dx%d.raw", filename.c_str(), X, Y, Z);
             if (FILE* fl = fopen(lname, "wb")) { fwrite(labels.data(), sizeof(int), nv, fl); fclose(fl); }
+        }
+        if (dbg) {
+            char bname[2048];
+            snprintf(bname, sizeof(bname), "boundaries_%dx%dx%d.raw", 2 * X - 1, 2 * Y - 1, 2 * Z - 1);
+            if (FILE* fb = fopen(bname, "wb")) { fwrite(restr.data(), 1, nc, fb); fclose(fb); }
         }
     } else {
         printf("no accuraccy needed, outputting\n");

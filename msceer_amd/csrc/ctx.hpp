@@ -119,6 +119,11 @@ struct msceer_ctx {
     msceer::DevBuf ldir;  // ldir walk records and visited bitmaps
     // accurate labelling
     msceer::DevBuf gradf, gradd, scan, label, label2, dest, seedmask, redo, stamp, rep, rep2, restriction, host_io, mask;
+    // accurate 1-manifolds: per-cell line stamps (two words, kept cleared
+    // between calls) and the saddle / window scratch
+    msceer::DevBuf line_stamps, lines;
+    bool line_stamps_clear = false;
+    uint64_t line_stamps_cells = 0;
     msceer::HostBuf host_stage;  // pinned staging of host-side merges (cleanup)
     uint32_t* h_ctl = nullptr;  // pinned mirror of the control block
     static constexpr int kEvents = 16;

@@ -8,6 +8,7 @@
 #include "ctx.hpp"
 #include "extgraph.hpp"
 #include "integrate.hpp"
+#include "lines.hpp"
 
 using namespace msceer;
 
@@ -200,13 +201,23 @@ int run_integration(msceer_ctx* ctx, const VGrid& g, const LabelBuffers& b, cons
 }  // namespace
 }  // namespace msceer
 
-extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz,
-                                   float err_thr, float grad_thr, int32_t max_iters, float presimp,
-                                   const uint8_t* integrate_mask, uint8_t* grad_out, int32_t* labels_out,
-                                   msceer_label_stats* stats, uint32_t flags) {
+namespace {
+
+// OnDemandAccurate's accuracy branches: the steepest pairing; the accurate
+// ascending labelling and its MIN boundary (needASC2); the digitized
+// 1-manifolds (needASC1, needDSC1); the conforming lower stars and ldir.
+int ondemand_impl(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz, float err_thr,
+                  float grad_thr, int32_t max_iters, float presimp, uint32_t need, const uint8_t* integrate_mask,
+                  uint8_t* grad_out, int32_t* labels_out, int8_t* restriction_out, msceer_label_stats* stats,
+                  uint32_t flags) {
     if (!ctx || !values) return MSCEER_EINVAL;
     if (flags & ~(MSCEER_DEVICE_PTRS | MSCEER_TIMING | MSCEER_STOP_REGIONS | MSCEER_STOP_INTEGRATED |
                   MSCEER_FP32_FIRST))
+        return MSCEER_EINVAL;
+    if (!need || (need & ~(MSCEER_NEED_ASC1 | MSCEER_NEED_DSC1 | MSCEER_NEED_ASC2))) return MSCEER_EINVAL;
+    const bool asc2 = need & MSCEER_NEED_ASC2;
+    const bool lines = need & (MSCEER_NEED_ASC1 | MSCEER_NEED_DSC1);
+    if (!asc2 && (integrate_mask || labels_out || (flags & (MSCEER_STOP_REGIONS | MSCEER_STOP_INTEGRATED))))
         return MSCEER_EINVAL;
     if (!(err_thr > 0.f) || !(grad_thr >= 0.f) || max_iters < 1 || !(presimp >= 0.f)) return MSCEER_EINVAL;
     int rc = check_dims(nx, ny, nz);
@@ -280,152 +291,191 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
     CHECK_HIP(launch_lstar(d, lb, s, shape));
     CHECK_HIP(hipEventRecord(ev[1], s));
 
-    // ---- extremum graph (critical vertices and 1-saddles) ----
-    // The minima stay on the device; only the arcs that the simplification can
-    // cancel (persistence at most presimp) come to the host.
-    const MList minl = front_list(ctx, caps, 0);
-    CHECK_HIP(launch_zero(minl.counts, kCountWords, s));
-    CHECK_HIP(launch_collect_minima(g, d_grad, minl, ctl + kCtlOverflow, s));
-    rc = read_ctl(ctx, 0, kCtlWords);
-    if (rc) return rc;
-    if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;  // more critical cells than list room
-    const uint64_t n_min = list_total(ctx->h_ctl + kCtlRound, minl);
-    const uint64_t n_sad = list_total(ctx->h_ctl + kCtlList1, list1);
-    std::vector<ArcRecord> rec;
-    if (n_sad) {
-        CHECK_HIP(ctx->host_io.ensure(sizeof(ArcRecord) * n_sad + 64));
-        ArcRecord* d_arcs = ctx->host_io.as<ArcRecord>();
-        CHECK_HIP(launch_zero(ctl + kCtlCount, 1, s));
-        CHECK_HIP(launch_saddle_arcs(g, d_values, d_grad, list1, presimp, d_arcs, (uint32_t)n_sad, ctl + kCtlCount, s));
-        rc = read_ctl(ctx, kCtlCount, 1);
-        if (rc) return rc;
-        rec.resize(std::min<uint64_t>(ctx->h_ctl[kCtlCount], n_sad));
-        if (!rec.empty()) {
-            CHECK_HIP(hipMemcpyAsync(rec.data(), d_arcs, sizeof(ArcRecord) * rec.size(), hipMemcpyDeviceToHost, s));
-            CHECK_HIP(hipStreamSynchronize(s));
-        }
-    }
-    // nodes: the minima the cancellable arcs touch, in ascending cell order
-    std::vector<MinNode> nodes;
-    nodes.reserve(2 * rec.size());
-    std::vector<SaddleArc> arcs(rec.size());
-    for (size_t i = 0; i < rec.size(); i++) {
-        nodes.push_back(MinNode{rec[i].min1, rec[i].v1});
-        nodes.push_back(MinNode{rec[i].min2, rec[i].v2});
-        arcs[i] = SaddleArc{rec[i].saddle, rec[i].min1, rec[i].min2, rec[i].sval};
-    }
-    std::sort(nodes.begin(), nodes.end(), [](const MinNode& x, const MinNode& y) { return x.cell < y.cell; });
-    nodes.erase(std::unique(nodes.begin(), nodes.end(),
-                            [](const MinNode& x, const MinNode& y) { return x.cell == y.cell; }),
-                nodes.end());
-    const std::vector<int> rep = simplify_minima(nodes, arcs, presimp);
-    // seeds: every minimum labelled by itself, then the cancelled ones by
-    // their representative's vertex
-    const int64_t Mx = 2 * nx - 1, My = 2 * ny - 1;
-    auto vnum = [&](int64_t c) { return (c % Mx) / 2 + nx * (((c / Mx) % My) / 2 + ny * ((c / (Mx * My)) / 2)); };
-    std::vector<int32_t> seed_v, seed_l;
-    for (size_t i = 0; i < nodes.size(); i++) {
-        if (rep[i] == (int)i) continue;
-        seed_v.push_back((int32_t)vnum(nodes[i].cell));
-        seed_l.push_back((int32_t)vnum(nodes[rep[i]].cell));
-    }
-    st.minima = (int64_t)n_min;
-    st.saddles = (int64_t)n_sad;
-    st.regions = (int64_t)n_min - (int64_t)seed_v.size();
-
-    // ---- certain regions ----
-    CHECK_HIP(launch_fill_i32(b.label, V, -1, s));
-    CHECK_HIP(launch_fill_u8(b.seedmask, V, 0, s));
-    CHECK_HIP(launch_grad_field(g, d_values, b.gradf, s));
-    CHECK_HIP(launch_seed_minima(minl, b, s));
-    if (!seed_v.empty()) {
-        CHECK_HIP(ctx->host_io.ensure(2 * sizeof(int32_t) * seed_v.size() + 64));
-        int32_t* d_sv = ctx->host_io.as<int32_t>();
-        int32_t* d_sl = d_sv + seed_v.size();
-        CHECK_HIP(hipMemcpyAsync(d_sv, seed_v.data(), sizeof(int32_t) * seed_v.size(), hipMemcpyHostToDevice, s));
-        CHECK_HIP(hipMemcpyAsync(d_sl, seed_l.data(), sizeof(int32_t) * seed_l.size(), hipMemcpyHostToDevice, s));
-        CHECK_HIP(launch_seed_scatter(d_sv, d_sl, (int)seed_v.size(), b, s));
-    }
-    // rounds in dependency order, enqueued in groups; a read-back per group
-    // shows whether the next round's list is empty
-    int sweeps = 0;
-    {
-        uint64_t* lists[2] = {ctx->front0.as<uint64_t>(), ctx->front1.as<uint64_t>()};
-        const uint32_t lcap = (uint32_t)std::min<uint64_t>(std::min(ctx->front0.bytes, ctx->front1.bytes) / 8, V);
-        uint32_t* counts = ctl + kCtlRound;  // three rotating count words
-        CHECK_HIP(launch_zero(counts, 3, s));
-        CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
-        CHECK_HIP(launch_certain_init(g, d_values, b.seedmask, b.stamp, lists[0], counts, lcap,
-                                      ctl + kCtlOverflow, s));
+    if (asc2) {
+        // ---- extremum graph (critical vertices and 1-saddles) ----
+        // The minima stay on the device; only the arcs that the simplification can
+        // cancel (persistence at most presimp) come to the host.
+        const MList minl = front_list(ctx, caps, 0);
+        CHECK_HIP(launch_zero(minl.counts, kCountWords, s));
+        CHECK_HIP(launch_collect_minima(g, d_grad, minl, ctl + kCtlOverflow, s));
         rc = read_ctl(ctx, 0, kCtlWords);
         if (rc) return rc;
-        int k = 0;
-        // Groups are pipelined: group i+1 is enqueued before the host reads the
-        // list length that group i left (copied to pinned memory behind it), so
-        // the GPU does not wait for the host between groups; the group after
-        // the last non-empty list runs empty rounds.  The launches of a group
-        // are sized by the list length last read back.
-        uint64_t expected = ctx->h_ctl[kCtlRound];
-        CHECK_HIP(ctx->host_stage.ensure(8 * sizeof(uint32_t)));
-        const volatile uint32_t* seen = ctx->host_stage.as<uint32_t>();  // two slots of the three counts
-        int done = 0, kprev = 0;
-        for (int guard = 0; expected > 0; guard++) {
-            if (guard > (int)(V / 16 + 16)) return MSCEER_EHIP;  // a round resolves at least one vertex
-            constexpr int kGroup = 16;
-            for (int j = 0; j < kGroup; j++, k++)
-                CHECK_HIP(launch_certain_round(g, b.label, b.stamp, lists[k % 2], counts + k % 3,
-                                               lists[(k + 1) % 2], counts + (k + 1) % 3, counts + (k + 2) % 3,
-                                               lcap, ctl + kCtlOverflow, expected, s));
-            const int slot = guard & 1;
-            CHECK_HIP(hipMemcpyAsync(ctx->host_stage.as<uint32_t>() + 4 * slot, counts, 3 * sizeof(uint32_t),
-                                     hipMemcpyDeviceToHost, s));
-            CHECK_HIP(hipEventRecord(ctx->ev[8 + slot], s));
-            if (guard > 0) {
-                CHECK_HIP(hipEventSynchronize(ctx->ev[8 + (slot ^ 1)]));
-                const uint32_t left = seen[4 * (slot ^ 1) + kprev % 3];
-                if (left == 0) {
-                    done = kprev;
-                    break;
-                }
-                expected = left;
+        if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;  // more critical cells than list room
+        const uint64_t n_min = list_total(ctx->h_ctl + kCtlRound, minl);
+        const uint64_t n_sad = list_total(ctx->h_ctl + kCtlList1, list1);
+        std::vector<ArcRecord> rec;
+        if (n_sad) {
+            CHECK_HIP(ctx->host_io.ensure(sizeof(ArcRecord) * n_sad + 64));
+            ArcRecord* d_arcs = ctx->host_io.as<ArcRecord>();
+            CHECK_HIP(launch_zero(ctl + kCtlCount, 1, s));
+            CHECK_HIP(launch_saddle_arcs(g, d_values, d_grad, list1, presimp, d_arcs, (uint32_t)n_sad, ctl + kCtlCount, s));
+            rc = read_ctl(ctx, kCtlCount, 1);
+            if (rc) return rc;
+            rec.resize(std::min<uint64_t>(ctx->h_ctl[kCtlCount], n_sad));
+            if (!rec.empty()) {
+                CHECK_HIP(hipMemcpyAsync(rec.data(), d_arcs, sizeof(ArcRecord) * rec.size(), hipMemcpyDeviceToHost, s));
+                CHECK_HIP(hipStreamSynchronize(s));
             }
-            kprev = k;
         }
-        rc = read_ctl(ctx, 0, kCtlWords);  // also waits for the last group
+        // nodes: the minima the cancellable arcs touch, in ascending cell order
+        std::vector<MinNode> nodes;
+        nodes.reserve(2 * rec.size());
+        std::vector<SaddleArc> arcs(rec.size());
+        for (size_t i = 0; i < rec.size(); i++) {
+            nodes.push_back(MinNode{rec[i].min1, rec[i].v1});
+            nodes.push_back(MinNode{rec[i].min2, rec[i].v2});
+            arcs[i] = SaddleArc{rec[i].saddle, rec[i].min1, rec[i].min2, rec[i].sval};
+        }
+        std::sort(nodes.begin(), nodes.end(), [](const MinNode& x, const MinNode& y) { return x.cell < y.cell; });
+        nodes.erase(std::unique(nodes.begin(), nodes.end(),
+                                [](const MinNode& x, const MinNode& y) { return x.cell == y.cell; }),
+                    nodes.end());
+        const std::vector<int> rep = simplify_minima(nodes, arcs, presimp);
+        // seeds: every minimum labelled by itself, then the cancelled ones by
+        // their representative's vertex
+        const int64_t Mx = 2 * nx - 1, My = 2 * ny - 1;
+        auto vnum = [&](int64_t c) { return (c % Mx) / 2 + nx * (((c / Mx) % My) / 2 + ny * ((c / (Mx * My)) / 2)); };
+        std::vector<int32_t> seed_v, seed_l;
+        for (size_t i = 0; i < nodes.size(); i++) {
+            if (rep[i] == (int)i) continue;
+            seed_v.push_back((int32_t)vnum(nodes[i].cell));
+            seed_l.push_back((int32_t)vnum(nodes[rep[i]].cell));
+        }
+        st.minima = (int64_t)n_min;
+        st.saddles = (int64_t)n_sad;
+        st.regions = (int64_t)n_min - (int64_t)seed_v.size();
+
+        // ---- certain regions ----
+        CHECK_HIP(launch_fill_i32(b.label, V, -1, s));
+        CHECK_HIP(launch_fill_u8(b.seedmask, V, 0, s));
+        CHECK_HIP(launch_grad_field(g, d_values, b.gradf, s));
+        CHECK_HIP(launch_seed_minima(minl, b, s));
+        if (!seed_v.empty()) {
+            CHECK_HIP(ctx->host_io.ensure(2 * sizeof(int32_t) * seed_v.size() + 64));
+            int32_t* d_sv = ctx->host_io.as<int32_t>();
+            int32_t* d_sl = d_sv + seed_v.size();
+            CHECK_HIP(hipMemcpyAsync(d_sv, seed_v.data(), sizeof(int32_t) * seed_v.size(), hipMemcpyHostToDevice, s));
+            CHECK_HIP(hipMemcpyAsync(d_sl, seed_l.data(), sizeof(int32_t) * seed_l.size(), hipMemcpyHostToDevice, s));
+            CHECK_HIP(launch_seed_scatter(d_sv, d_sl, (int)seed_v.size(), b, s));
+        }
+        // rounds in dependency order, enqueued in groups; a read-back per group
+        // shows whether the next round's list is empty
+        int sweeps = 0;
+        {
+            uint64_t* lists[2] = {ctx->front0.as<uint64_t>(), ctx->front1.as<uint64_t>()};
+            const uint32_t lcap = (uint32_t)std::min<uint64_t>(std::min(ctx->front0.bytes, ctx->front1.bytes) / 8, V);
+            uint32_t* counts = ctl + kCtlRound;  // three rotating count words
+            CHECK_HIP(launch_zero(counts, 3, s));
+            CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
+            CHECK_HIP(launch_certain_init(g, d_values, b.seedmask, b.stamp, lists[0], counts, lcap,
+                                          ctl + kCtlOverflow, s));
+            rc = read_ctl(ctx, 0, kCtlWords);
+            if (rc) return rc;
+            int k = 0;
+            // Groups are pipelined: group i+1 is enqueued before the host reads the
+            // list length that group i left (copied to pinned memory behind it), so
+            // the GPU does not wait for the host between groups; the group after
+            // the last non-empty list runs empty rounds.  The launches of a group
+            // are sized by the list length last read back.
+            uint64_t expected = ctx->h_ctl[kCtlRound];
+            CHECK_HIP(ctx->host_stage.ensure(8 * sizeof(uint32_t)));
+            const volatile uint32_t* seen = ctx->host_stage.as<uint32_t>();  // two slots of the three counts
+            int done = 0, kprev = 0;
+            for (int guard = 0; expected > 0; guard++) {
+                if (guard > (int)(V / 16 + 16)) return MSCEER_EHIP;  // a round resolves at least one vertex
+                constexpr int kGroup = 16;
+                for (int j = 0; j < kGroup; j++, k++)
+                    CHECK_HIP(launch_certain_round(g, b.label, b.stamp, lists[k % 2], counts + k % 3,
+                                                   lists[(k + 1) % 2], counts + (k + 1) % 3, counts + (k + 2) % 3,
+                                                   lcap, ctl + kCtlOverflow, expected, s));
+                const int slot = guard & 1;
+                CHECK_HIP(hipMemcpyAsync(ctx->host_stage.as<uint32_t>() + 4 * slot, counts, 3 * sizeof(uint32_t),
+                                         hipMemcpyDeviceToHost, s));
+                CHECK_HIP(hipEventRecord(ctx->ev[8 + slot], s));
+                if (guard > 0) {
+                    CHECK_HIP(hipEventSynchronize(ctx->ev[8 + (slot ^ 1)]));
+                    const uint32_t left = seen[4 * (slot ^ 1) + kprev % 3];
+                    if (left == 0) {
+                        done = kprev;
+                        break;
+                    }
+                    expected = left;
+                }
+                kprev = k;
+            }
+            rc = read_ctl(ctx, 0, kCtlWords);  // also waits for the last group
+            if (rc) return rc;
+            if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;
+            sweeps = done ? done : k;
+        }
+        st.certain_sweeps = sweeps;
+        CHECK_HIP(hipEventRecord(ev[2], s));
+        auto finish_early = [&]() -> int {
+            if (labels_out)
+                CHECK_HIP(hipMemcpyAsync(labels_out, b.label, sizeof(int32_t) * V,
+                                         dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+            CHECK_HIP(hipStreamSynchronize(s));
+            ctx->last_labels = st;
+            if (stats) *stats = st;
+            return MSCEER_OK;
+        };
+        if (flags & MSCEER_STOP_REGIONS) return finish_early();
+
+        // ---- integration ----
+        rc = run_integration(ctx, g, b, d_mask, d_grad, grad_thr, err_thr, max_iters, -1,
+                             (flags & MSCEER_FP32_FIRST) != 0, st);
         if (rc) return rc;
-        if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;
-        sweeps = done ? done : k;
+        CHECK_HIP(hipEventRecord(ev[3], s));
+        if (flags & MSCEER_STOP_INTEGRATED) return finish_early();
+
+        // ---- cleanup of isolated components ----
+        rc = run_cleanup(ctx, g, b);
+        if (rc) return rc;
+        CHECK_HIP(hipEventRecord(ev[4], s));
+    } else {
+        CHECK_HIP(hipEventRecord(ev[2], s));
+        CHECK_HIP(hipEventRecord(ev[3], s));
+        CHECK_HIP(hipEventRecord(ev[4], s));
     }
-    st.certain_sweeps = sweeps;
-    CHECK_HIP(hipEventRecord(ev[2], s));
-    auto finish_early = [&]() -> int {
-        if (labels_out)
-            CHECK_HIP(hipMemcpyAsync(labels_out, b.label, sizeof(int32_t) * V,
-                                     dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
-        CHECK_HIP(hipStreamSynchronize(s));
-        ctx->last_labels = st;
-        if (stats) *stats = st;
-        return MSCEER_OK;
-    };
-    if (flags & MSCEER_STOP_REGIONS) return finish_early();
 
-    // ---- integration ----
-    rc = run_integration(ctx, g, b, d_mask, d_grad, grad_thr, err_thr, max_iters, -1,
-                         (flags & MSCEER_FP32_FIRST) != 0, st);
-    if (rc) return rc;
-    CHECK_HIP(hipEventRecord(ev[3], s));
-    if (flags & MSCEER_STOP_INTEGRATED) return finish_early();
-
-    // ---- cleanup of isolated components ----
-    rc = run_cleanup(ctx, g, b);
-    if (rc) return rc;
-    CHECK_HIP(hipEventRecord(ev[4], s));
-
-    // ---- restriction labels, conforming gradient, ldir ----
+    // ---- restriction labels, 1-manifolds, conforming gradient, ldir ----
+    if (grad_out || restriction_out) {
+        int8_t* restr = ctx->restriction.as<int8_t>();
+        if (asc2) CHECK_HIP(launch_min_boundary(g, b.label, restr, s));
+        else CHECK_HIP(launch_fill_u8(reinterpret_cast<uint8_t*>(restr), C, 0, s));
+        CHECK_HIP(hipEventRecord(ev[5], s));
+        if (lines) {
+            // the streamlines follow the FP64 field, which the labelling has
+            // zeroed at its region extrema (as the reference's shared field is)
+            if (!asc2) {
+                CHECK_HIP(launch_fill_u8(b.seedmask, V, 0, s));
+                CHECK_HIP(launch_grad_field64(g, b, s));
+            }
+            LineStats la, ld;
+            if (need & MSCEER_NEED_ASC1) {
+                rc = run_lines(ctx, d, d_values, d_grad, b.gradd, restr, true, grad_thr, err_thr, max_iters, &la);
+                if (rc) return rc;
+            }
+            if (need & MSCEER_NEED_DSC1) {
+                rc = run_lines(ctx, d, d_values, d_grad, b.gradd, restr, false, grad_thr, err_thr, max_iters, &ld);
+                if (rc) return rc;
+            }
+            st.lines_asc = (int64_t)la.traced;
+            st.lines_dsc = (int64_t)ld.traced;
+            st.lines_replayed = (int64_t)(la.replayed + ld.replayed);
+            st.line_windows = (int32_t)(la.windows + ld.windows);
+            st.line_cuts = (int32_t)(la.cuts + ld.cuts + la.direct + ld.direct);
+        }
+        CHECK_HIP(hipEventRecord(ev[10], s));
+        if (restriction_out)
+            CHECK_HIP(hipMemcpyAsync(restriction_out, restr, C, dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost,
+                                     s));
+    } else {
+        CHECK_HIP(hipEventRecord(ev[5], s));
+        CHECK_HIP(hipEventRecord(ev[10], s));
+    }
     if (grad_out) {
         int8_t* restr = ctx->restriction.as<int8_t>();
-        CHECK_HIP(launch_min_boundary(g, b.label, restr, s));
-        CHECK_HIP(hipEventRecord(ev[5], s));
         CHECK_HIP(launch_zero(ctl + kCtlList1, 2 * kCountWords, s));
         CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
         LstarBuffers cb{d_values, d_grad, restr, &list1, &list2, ctl + kCtlStats, ctl + kCtlOverflow};
@@ -438,7 +488,6 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
         if (rc) return rc;
         if (!dev) CHECK_HIP(hipMemcpyAsync(grad_out, d_grad, C, hipMemcpyDeviceToHost, s));
     } else {
-        CHECK_HIP(hipEventRecord(ev[5], s));
         CHECK_HIP(hipEventRecord(ev[6], s));
     }
     CHECK_HIP(hipEventRecord(ev[7], s));
@@ -447,15 +496,34 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
                                  dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     }
     CHECK_HIP(hipStreamSynchronize(s));
-    st.ms_lstar = elapsed(ev[0], ev[1]) + elapsed(ev[5], ev[6]);
+    st.ms_lstar = elapsed(ev[0], ev[1]) + elapsed(ev[10], ev[6]);
     st.ms_regions = elapsed(ev[1], ev[2]);
     st.ms_integrate = elapsed(ev[2], ev[3]);
     st.ms_cleanup = elapsed(ev[3], ev[4]) + elapsed(ev[4], ev[5]);
-    st.ms_grad = elapsed(ev[5], ev[7]);
+    st.ms_lines = elapsed(ev[5], ev[10]);
+    st.ms_grad = elapsed(ev[10], ev[7]);
     st.ms_total = elapsed(ev[0], ev[7]);
     ctx->last_labels = st;
     if (stats) *stats = st;
     return MSCEER_OK;
+}
+
+}  // namespace
+
+extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz,
+                                   float err_thr, float grad_thr, int32_t max_iters, float presimp,
+                                   const uint8_t* integrate_mask, uint8_t* grad_out, int32_t* labels_out,
+                                   msceer_label_stats* stats, uint32_t flags) {
+    return ondemand_impl(ctx, values, nx, ny, nz, err_thr, grad_thr, max_iters, presimp, MSCEER_NEED_ASC2,
+                         integrate_mask, grad_out, labels_out, nullptr, stats, flags);
+}
+
+extern "C" int msceer_ondemand(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz,
+                               float err_thr, float grad_thr, int32_t max_iters, float presimp, uint32_t need,
+                               const uint8_t* integrate_mask, uint8_t* grad_out, int32_t* labels_out,
+                               int8_t* restriction_out, msceer_label_stats* stats, uint32_t flags) {
+    return ondemand_impl(ctx, values, nx, ny, nz, err_thr, grad_thr, max_iters, presimp, need, integrate_mask,
+                         grad_out, labels_out, restriction_out, stats, flags);
 }
 
 namespace {

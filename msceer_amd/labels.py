@@ -53,6 +53,35 @@ def ondemand_asc(values: np.ndarray, err_thr: float = 1e-3, grad_thr: float = 1e
     return grad, labels.reshape(Z, Y, X), st.as_dict()
 
 
+def ondemand(values: np.ndarray, need_asc1: bool = False, need_dsc1: bool = False, need_asc2: bool = True,
+             err_thr: float = 1e-3, grad_thr: float = 1e-6, max_iters: int = 4096, presimp: float = 0.0,
+             ctx: native.Context | None = None):
+    """OnDemandAccurate's accuracy branches (needASC1, needDSC1, needASC2):
+    (conforming grad, labels (Z,Y,X) int32 or None without needASC2,
+    restriction labels (cells int8), stats dict)."""
+    v = np.ascontiguousarray(values, dtype=np.float32)
+    Z, Y, X = v.shape
+    need = ((native.NEED_ASC1 if need_asc1 else 0) | (native.NEED_DSC1 if need_dsc1 else 0) |
+            (native.NEED_ASC2 if need_asc2 else 0))
+    own = ctx is None
+    ctx = ctx or native.Context()
+    try:
+        labels = np.empty(X * Y * Z, np.int32) if need_asc2 else None
+        grad = np.empty(num_cells(X, Y, Z), np.uint8)
+        restr = np.empty(num_cells(X, Y, Z), np.int8)
+        st = native.LabelStats()
+        rc = native.lib().msceer_ondemand(
+            ctx.handle, v.ctypes.data_as(ctypes.c_void_p), X, Y, Z, err_thr, grad_thr, max_iters, presimp, need,
+            None, grad.ctypes.data_as(ctypes.c_void_p),
+            None if labels is None else labels.ctypes.data_as(ctypes.c_void_p),
+            restr.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st), 0)
+        native.check(rc, "msceer_ondemand")
+    finally:
+        if own:
+            ctx.close()
+    return grad, None if labels is None else labels.reshape(Z, Y, X), restr, st.as_dict()
+
+
 def label_file_name(base: str, X: int, Y: int, Z: int) -> str:
     return f"{base}._int_{X}x{Y}x{Z}.raw"
 

@@ -17,6 +17,10 @@ TIMING = 0x4
 STOP_REGIONS = 0x8
 STOP_INTEGRATED = 0x10
 FP32_FIRST = 0x20
+# accuracy branches of msceer_ondemand
+NEED_ASC1 = 0x1
+NEED_DSC1 = 0x2
+NEED_ASC2 = 0x4
 BRICKS = ("64x2x2", "32x4x2", "16x4x4", "8x8x4")  # lower-star brick shapes by index
 # error codes
 OK, EINVAL, EHIP, ENOMEM, ETOOBIG, ENODEV = 0, -1, -2, -3, -4, -5
@@ -41,6 +45,8 @@ SIGNATURES = {
     "msceer_lstar_grad_noalloc": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _U32]),
     "msceer_ondemand_asc": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, ctypes.c_float, ctypes.c_float, _I32,
                                            ctypes.c_float, _P, _P, _P, _P, _U32]),
+    "msceer_ondemand": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, ctypes.c_float, ctypes.c_float, _I32,
+                                       ctypes.c_float, _U32, _P, _P, _P, _P, _P, _U32]),
     "msceer_integrate_labels": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _P, ctypes.c_float, ctypes.c_float, _I32,
                                                _I32, _P, _P, _U32]),
     "msceer_clean_labels": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _U32]),
@@ -95,6 +101,12 @@ class LabelStats(ctypes.Structure):
         ("ms_pass3", ctypes.c_float),
         ("reintegrated_pass3", ctypes.c_int64),
         ("escalated_fp64", ctypes.c_int64),
+        ("lines_asc", ctypes.c_int64),
+        ("lines_dsc", ctypes.c_int64),
+        ("lines_replayed", ctypes.c_int64),
+        ("line_windows", ctypes.c_int32),
+        ("line_cuts", ctypes.c_int32),
+        ("ms_lines", ctypes.c_float),
     ]
 
     def as_dict(self):

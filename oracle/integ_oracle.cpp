@@ -186,12 +186,13 @@ inline double mag(const V3d& a) { return std::sqrt(a.v[0] * a.v[0] + a.v[1] * a.
 
 // AdaptiveEulerAdvector3D<F,-1>::AdvectThroughVoxel{NoCheck,NearBoundary}
 // (gi_adaptive_euler_advector.h:51-305).  kFp64 selects all-FP64 arithmetic.
-template <bool kFp64>
-Event advect_voxel(const Grid& g, const std::vector<V3d>& grad, const Checker& chk, float gthr, float ethr,
+template <bool kFp64, class Chk>
+Event advect_voxel(const Grid& g, const std::vector<V3d>& grad, const Chk& chk, float gthr, float ethr,
                    bool near_boundary, V3d& cur, int& iters, int kMul) {
     typedef typename std::conditional<kFp64, double, float>::type F;
     const double kMaxStep = 0.5;  // MAX_STEP_IN_GRID_CELLS
     // kMul: -1 for ascending manifolds (flow downhill), +1 for descending
+    // (and for the ascending 1-manifolds traced up from 2-saddles)
     V3d vals[8];
     V3d old = cur;
     const V3l base = near_boundary ? trunc_l(g.inbounds(cur)) : trunc_l(cur);
@@ -623,6 +624,185 @@ void min_boundary(const CellMesh& m, const int* lab, signed char* r) {
         }
 }
 
+
+// ---- accurate 1-manifolds: digitized streamlines from the saddles ---------
+//
+// DigitizingNumericStreamlineIntegrator3dASC / 3dDSC
+// (gi_numeric_streamline_integrator.h:526-990) driven by
+// ReIntegrateUpFrom2Saddles / ReIntegrateDownFrom1Saddles
+// (ondemandaccurate.cxx:254-500), as one OpenMP thread runs them: saddles in
+// decreasing (value, id) order (ASC) or increasing value, decreasing id
+// (DSC: keys (-value, id) taken from the largest), each saddle's seed cells
+// gathered by a short V-path search, one streamline per seed that labels the
+// cells it crosses with the target value (4 ASC, 5 DSC) in the restriction
+// array and stops near a cell already carrying that value; the saddle takes
+// the target after its streamlines.
+//
+// ASC: cells are hexes (voxels) and the quads between them; the flow is
+// uphill (advector sign +1).  DSC: the same on the dual grid (vertices and
+// the edges between them, positions shifted by half a voxel); downhill.
+
+struct LineChecker {
+    // TerminateNearAssignedHex / TerminateNearAssignedVert
+    // (gi_advection_checker.h:59-100)
+    const signed char* lab;
+    const CellMesh* m;
+    int target;
+    bool asc;
+    Event check(const V3d& p, bool near_boundary) const {
+        if (near_boundary) return HIT_PREASSIGNED;
+        idx_t c[3];
+        for (int a = 0; a < 3; a++)
+            c[a] = asc ? 2 * (idx_t)std::floor(p.v[a]) + 1 : 2 * (idx_t)std::floor(p.v[a] + 0.5);
+        return lab[c[0] + m->M[0] * (c[1] + m->M[1] * c[2])] == target ? HIT_PREASSIGNED : NONE;
+    }
+};
+
+// DigitizeSegmentInternal (gi_numeric_streamline_integrator.h:566-700 ASC,
+// 824-985 DSC): the cell faces the segment a -> b crosses, in the order of
+// their crossing parameter; for each crossing whose face bounds the current
+// cell, label the current cell and the face (stop at the first that already
+// carries the target) and move to the cell on the other side.
+void digitize(const CellMesh& m, V3d a, V3d b, signed char* lab, int target, bool asc, long long* nlab) {
+    if (!asc)
+        for (int d = 0; d < 3; d++) { a.v[d] += 0.5; b.v[d] += 0.5; }
+    const V3d ab{{b.v[0] - a.v[0], b.v[1] - a.v[1], b.v[2] - a.v[2]}};
+    idx_t face[3];
+    double tc[3];
+    int n = 0;
+    for (int d = 0; d < 3; d++) {
+        const int va = (int)std::floor(a.v[d]), vb = (int)std::floor(b.v[d]);
+        if (va == vb) continue;
+        const int plane = va > vb ? va : vb;
+        const double t = ((double)plane - a.v[d]) / (b.v[d] - a.v[d]);
+        tc[n] = t;
+        if (t < 0 || t > 1) return;
+        idx_t fc[3];
+        for (int k = 0; k < 3; k++) fc[k] = 2 * (idx_t)std::floor(ab.v[k] * t + a.v[k]) + (k == d ? 0 : 1);
+        if (!asc)
+            for (int k = 0; k < 3; k++) fc[k] -= 1;
+        face[n++] = fc[0] + m.M[0] * (fc[1] + m.M[1] * fc[2]);
+    }
+    if (n == 0) return;
+    for (int i = 0; i < n; i++)
+        for (int j = i; j < n; j++)
+            if (tc[i] > tc[j]) {
+                std::swap(face[i], face[j]);
+                std::swap(tc[i], tc[j]);
+            }
+    idx_t sc[3];
+    for (int k = 0; k < 3; k++) sc[k] = 2 * (idx_t)std::floor(a.v[k]) + (asc ? 1 : 0);
+    idx_t cur = sc[0] + m.M[0] * (sc[1] + m.M[1] * sc[2]);
+    for (int i = 0; i < n; i++) {
+        // the two cells the face separates: cofacets (hexes) of a quad for
+        // ASC, facets (vertices) of an edge for DSC, along the face's
+        // crossing axis, the domain-clipped ones only
+        idx_t p[3];
+        m.coords(face[i], p);
+        bool has = false;
+        idx_t next = cur;
+        for (int d = 0; d < 3; d++) {
+            if (asc ? (p[d] & 1) != 0 : (p[d] & 1) == 0) continue;
+            if (p[d] < m.M[d] - 1) { const idx_t c = face[i] + m.off(d); if (c == cur) has = true; else next = c; }
+            if (p[d] > 0) { const idx_t c = face[i] - m.off(d); if (c == cur) has = true; else next = c; }
+        }
+        if (!has) continue;
+        if (lab[cur] == target) return;
+        lab[cur] = (signed char)target;
+        if (nlab) (*nlab)++;
+        if (lab[face[i]] == target) return;
+        lab[face[i]] = (signed char)target;
+        if (nlab) (*nlab)++;
+        cur = next;
+    }
+}
+
+// IntegrateStreamline (gi_numeric_streamline_integrator.h:734-788, 1018-1072).
+template <bool kFp64>
+Event trace_line(const Grid& g, const CellMesh& m, const std::vector<V3d>& grad, const LineChecker& chk, float gthr,
+                 float ethr, int max_iters, V3d seed, signed char* lab, long long* nvox, long long* nlab) {
+    V3l c = trunc_l(V3d{{seed.v[0] + 0.5, seed.v[1] + 0.5, seed.v[2] + 0.5}});
+    V3d cur = seed;
+    int iters = max_iters;
+    const int mul = chk.asc ? 1 : -1;
+    while (true) {
+        const V3d old = cur;
+        Event e;
+        if (g.dist_to_boundary(c) <= 1) {
+            e = advect_voxel<kFp64>(g, grad, chk, gthr, ethr, true, cur, iters, mul);
+            c = trunc_l(g.inbounds(V3d{{cur.v[0] + 0.5, cur.v[1] + 0.5, cur.v[2] + 0.5}}));
+        } else {
+            e = advect_voxel<kFp64>(g, grad, chk, gthr, ethr, false, cur, iters, mul);
+            c = trunc_l(V3d{{cur.v[0] + 0.5, cur.v[1] + 0.5, cur.v[2] + 0.5}});
+            digitize(m, old, cur, lab, chk.target, chk.asc, nlab);
+        }
+        if (nvox) (*nvox)++;
+        if (e == LOW_GRADIENT || e == HIT_EXTREMUM || e == HIT_PREASSIGNED || e == OVER_MAX_ITERATIONS) return e;
+    }
+}
+
+// ReIntegrateUpFrom2Saddles / ReIntegrateDownFrom1Saddles with one thread.
+// stats (7): saddles, streamlines, voxel traversals, cells labelled,
+//            stops on labelled cells or the boundary, low-gradient stops, budget stops
+template <bool kFp64>
+void saddle_lines(const Grid& g, const CellMesh& m, const float* f, const unsigned char* grad,
+                  const std::vector<V3d>& field, float ethr, float gthr, int max_iters, bool asc, signed char* lab,
+                  long long* stats) {
+    const idx_t C = m.M[0] * m.M[1] * m.M[2];
+    std::vector<unsigned char> maxv((size_t)C), minv((size_t)C);
+    oracle_maxmin(f, g.X, g.Y, g.Z, maxv.data(), minv.data());
+    const CellFunc cf{&m, f, maxv.data()};
+    const int sdim = asc ? 2 : 1, target = asc ? 4 : 5;
+    std::vector<std::pair<float, idx_t>> sad;
+    for (idx_t c = 0; c < C; c++)
+        if (m.dim(c) == sdim && m.critical(grad, c)) sad.push_back({asc ? cf.value(c) : -1 * cf.value(c), c});
+    std::sort(sad.begin(), sad.end());
+    const LineChecker chk{lab, &m, target, asc};
+    long long st[7] = {(long long)sad.size(), 0, 0, 0, 0, 0, 0};
+    for (idx_t k = (idx_t)sad.size() - 1; k >= 0; k--) {
+        const idx_t s = sad[(size_t)k].second;
+        // a few V-path steps on either side of the saddle
+        // (ondemandaccurate.cxx:319-351 ASC, 448-480 DSC)
+        std::vector<idx_t> result, queue{s};
+        std::set<idx_t> seen;
+        size_t qh = 0;
+        for (int counter = 4; qh < queue.size() && counter >= 0; counter--) {
+            const idx_t cur = queue[qh++];
+            seen.insert(cur);
+            idx_t p[3];
+            m.coords(cur, p);
+            for (int d = 0; d < 3; d++) {
+                if (asc ? (p[d] & 1) != 0 : (p[d] & 1) == 0) continue;  // cofacets (ASC) / facets (DSC) along d
+                for (int sg = 0; sg < 2; sg++) {
+                    if (asc && (sg == 0 ? p[d] >= m.M[d] - 1 : p[d] <= 0)) continue;
+                    const idx_t t = cur + (sg == 0 ? m.off(d) : -m.off(d));
+                    if (seen.count(t)) continue;
+                    result.push_back(t);
+                    seen.insert(t);
+                    if (!m.critical(grad, t)) {
+                        const idx_t pr = m.pair(grad, t);
+                        seen.insert(pr);
+                        queue.push_back(pr);
+                    }
+                }
+            }
+        }
+        for (idx_t t : result) {
+            if (m.dim(t) != (asc ? 3 : 0)) continue;
+            idx_t p[3];
+            m.coords(t, p);
+            const V3d seed{{(double)p[0] * 0.5, (double)p[1] * 0.5, (double)p[2] * 0.5}};
+            const Event e = trace_line<kFp64>(g, m, field, chk, gthr, ethr, max_iters, seed, lab, &st[2], &st[3]);
+            st[1]++;
+            if (e == HIT_PREASSIGNED) st[4]++;
+            else if (e == LOW_GRADIENT) st[5]++;
+            else if (e == OVER_MAX_ITERATIONS) st[6]++;
+        }
+        lab[s] = (signed char)target;
+    }
+    if (stats) std::copy(st, st + 7, stats);
+}
+
 }  // namespace
 
 extern "C" {
@@ -811,6 +991,69 @@ int oracle_ondemand_asc(const float* f, long long X, long long Y, long long Z, f
     std::copy(base.begin(), base.end(), grad);
     if (restriction_out) std::copy(r.begin(), r.end(), restriction_out);
     if (stats) stats[3] = nv;
+    return 0;
+}
+
+// The digitized 1-manifolds of one family added to restriction (in/out):
+// asc 1 = from the 2-saddles up (value 4), asc 0 = from the 1-saddles down
+// (value 5).  grad: a .grad whose pairs are the steepest lower-star pairing;
+// seeds (may be null): vertices whose gradient is zeroed first, as the
+// accurate 3-manifold labelling leaves the shared gradient field
+// (gi_numeric_integrator_path_compressing.h:118-123).  mode 0: the
+// reference's mixed arithmetic, else FP64.  stats: 7 entries (saddle_lines).
+int oracle_saddle_lines(const float* f, long long X, long long Y, long long Z, const unsigned char* grad,
+                        const signed char* seeds, float ethr, float gthr, int max_iters, int asc, int mode,
+                        signed char* restriction, long long* stats) {
+    Grid g(X, Y, Z);
+    CellMesh m(X, Y, Z);
+    std::vector<V3d> field;
+    gradient_field(g, f, field);
+    if (seeds)
+        for (idx_t i = 0; i < X * Y * Z; i++) if (seeds[i]) field[(size_t)i] = V3d{{0, 0, 0}};
+    if (mode == 0) saddle_lines<false>(g, m, f, grad, field, ethr, gthr, max_iters, asc != 0, restriction, stats);
+    else saddle_lines<true>(g, m, f, grad, field, ethr, gthr, max_iters, asc != 0, restriction, stats);
+    return 0;
+}
+
+// OnDemandAccurate with any of needASC1, needDSC1, needASC2 (needDSC2 is not
+// restated), ondemandaccurate.cxx:728-1157: the steepest pairing; the
+// accurate ascending labelling and its MIN boundary when need_asc2; the
+// digitized 1-manifolds (ASC then DSC); the conforming lower stars; ldir and
+// flags.  labels (V) are written only when need_asc2 (else left as they are);
+// stats: [0..3] the labelling's, [4..10] ASC lines, [11..17] DSC lines.
+int oracle_ondemand(const float* f, long long X, long long Y, long long Z, float ethr, float gthr, int max_iters,
+                    float presimp, int need_asc1, int need_dsc1, int need_asc2, int mode, unsigned char* grad,
+                    int* labels, signed char* restriction_out, long long* stats) {
+    CellMesh m(X, Y, Z);
+    const idx_t C = m.M[0] * m.M[1] * m.M[2];
+    const idx_t V = X * Y * Z;
+    std::vector<unsigned char> base((size_t)C);
+    oracle_lstar_pairs(f, X, Y, Z, nullptr, base.data());
+    std::vector<signed char> r((size_t)C, 0), seeds;
+    long long st[18] = {0};
+    if (need_asc2) {
+        seeds.assign((size_t)V, 0);
+        oracle_certain_regions(f, X, Y, Z, base.data(), presimp, labels, seeds.data());
+        oracle_integrate(f, X, Y, Z, seeds.data(), ethr, gthr, max_iters, mode, labels, st);
+        oracle_clean(f, X, Y, Z, labels);
+        oracle_min_boundary(labels, X, Y, Z, r.data());
+    }
+    const int lmode = mode == 0 ? 0 : 1;
+    const signed char* zs = need_asc2 ? seeds.data() : nullptr;
+    if (need_asc1) oracle_saddle_lines(f, X, Y, Z, base.data(), zs, ethr, gthr, max_iters, 1, lmode, r.data(), st + 4);
+    if (need_dsc1) oracle_saddle_lines(f, X, Y, Z, base.data(), zs, ethr, gthr, max_iters, 0, lmode, r.data(), st + 11);
+    if (need_asc2 || need_asc1 || need_dsc1) {
+        const long long nv = oracle_conforming_vertices(f, X, Y, Z, r.data(), nullptr);
+        std::vector<long long> vs((size_t)nv);
+        oracle_conforming_vertices(f, X, Y, Z, r.data(), vs.data());
+        oracle_lstar_vertices(f, X, Y, Z, r.data(), vs.data(), nv, base.data());
+        st[3] = nv;
+    }
+    oracle_ldir(base.data(), X, Y, Z);
+    oracle_flags(base.data(), X, Y, Z);
+    std::copy(base.begin(), base.end(), grad);
+    if (restriction_out) std::copy(r.begin(), r.end(), restriction_out);
+    if (stats) std::copy(st, st + 18, stats);
     return 0;
 }
 

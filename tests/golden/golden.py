@@ -45,6 +45,29 @@ def load_ondemand(name):
     return f, (err, gthr, iters, presimp), out
 
 
+LINE_FLAGS = [(1, 0, 0), (0, 1, 0), (1, 1, 0), (1, 1, 1)]
+
+
+def lines_case_names():
+    """(case, flags) pairs of the 1-manifold golden vectors."""
+    from cases import ONDEMAND_CASES
+    return [(n, fl) for n in ONDEMAND_CASES for fl in LINE_FLAGS]
+
+
+def load_lines(name, flags):
+    """(f, params, restriction (cells int8), SHA-256 of the conforming .grad)
+    of the reference's run with needASC1, needDSC1, needASC2 = flags."""
+    from cases import ONDEMAND_CASES
+    shape, gen, *_ = ONDEMAND_CASES[name]
+    z = np.load(os.path.join(HERE, "lines", "%s_%d%d%d.npz" % ((name,) + tuple(flags))), allow_pickle=False)
+    f = np.ascontiguousarray(gen(), np.float32)
+    assert synth.sha256(f) == str(z["input_sha256"]), "input generator drifted from the fixture"
+    assert tuple(int(v) for v in z["flags"]) == tuple(flags)
+    err, gthr, iters, presimp = (float(v) for v in z["params"])
+    res = np.frombuffer(zlib.decompress(z["restriction_z"].tobytes()), np.int8)
+    return f, (err, gthr, int(iters), presimp), res, str(z["grad_sha256"])
+
+
 def canonical(lab, f):
     """Rename each label by its lowest vertex (value, then vertex id)."""
     flat = f.ravel()

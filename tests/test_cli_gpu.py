@@ -74,5 +74,27 @@ def test_ondemandaccurate_cli_rejects_unbuilt_branches(tmp_path):
     raw = tmp_path / "vol.raw"
     np.zeros(8, np.float32).tofile(raw)
     r = subprocess.run([_exe("ondemandaccurate"), "2", "2", "2", str(raw), "1e-3", "1e-6", "10",
-                        "1", "0", "0", "0", "0"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
-    assert r.returncode == 2 and "needASC2" in r.stderr
+                        "0", "0", "0", "1", "0"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "needDSC2" in r.stderr
+
+
+@pytest.mark.parametrize("flags", [(1, 0, 0), (1, 1, 1)])
+def test_ondemandaccurate_cli_one_manifolds_match_library(tmp_path, flags):
+    # needASC1 / needDSC1 argv positions (ondemandaccurate.cxx:167-168): the
+    # .grad, the restriction labels (boundaries_MxMyMz.raw with outputdebug)
+    # and the task lines of the branches that ran
+    from golden import load_lines
+    f, (err, gthr, iters, presimp), ref_r, _ = load_lines("gauss_44x36x40", flags)
+    Z, Y, X = f.shape
+    raw = tmp_path / "vol.raw"
+    f.tofile(raw)
+    out = _run([_exe("ondemandaccurate"), str(X), str(Y), str(Z), str(raw), repr(err), repr(gthr), str(iters),
+                str(flags[0]), str(flags[1]), str(flags[2]), "0", repr(presimp), "8", "1"], tmp_path)
+    assert "1-manifold streamlines" in out
+    g = np.fromfile(str(raw) + ".grad", np.uint8)
+    r = np.fromfile(tmp_path / f"boundaries_{2 * X - 1}x{2 * Y - 1}x{2 * Z - 1}.raw", np.int8)
+    grad, _, restr, _ = labels.ondemand(f, *map(bool, flags), err, gthr, iters, presimp)
+    assert (g == grad).all() and (r == restr).all()
+    tasks = open(str(raw) + ".008.gtime.txt").read().splitlines()
+    assert len(tasks) == (6 if flags[2] else 3)
+    assert any(t.startswith("Numerical SaddleInt") for t in tasks)

@@ -1,0 +1,105 @@
+"""GPU accurate 1-manifolds (msceer_ondemand with needASC1 / needDSC1)
+against the CPU oracle and the reference's runs.
+
+The streamlines run in FP64 on the GPU, as in the oracle's FP64 mode, and the
+GPU keeps the reference's one-thread saddle order exactly (speculation with
+in-order replay), so the digitized cells (4, 5) equal the FP64 oracle's bit
+for bit, and so do the whole restriction and the conforming .grad without
+needASC2 (with it, the MIN boundary follows labels that may differ by the
+pass-1 visiting order; the .grad is then checked as the oracle's conforming
+gradient of the GPU's restriction).  Against the reference's own runs (mixed FP32/FP64
+arithmetic) the restriction labels differ on at most 0.1 % of the labelled
+cells.  Small speculation windows (MSCEER_LINE_WINDOW) give the same bytes as
+one window.
+"""
+import os
+import sys
+
+import numpy as np
+import pytest
+
+import gradcheck
+import oracle_lib
+from golden import lines_case_names, load_lines
+from msceer_amd import labels, native, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = native.Context(0)
+    yield c
+    c.close()
+
+
+def _say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def _labelled(r):
+    return (r == 4) | (r == 5)
+
+
+@pytest.mark.parametrize("name,flags", lines_case_names())
+def test_lines_match_fp64_oracle(ctx, name, flags):
+    f, (err, gthr, iters, presimp), ref_r, _ = load_lines(name, flags)
+    Z, Y, X = f.shape
+    _say(f"lines {name} {flags}")
+    g, lab, r, st = labels.ondemand(f, *map(bool, flags), err, gthr, iters, presimp, ctx=ctx)
+    og, olab, o_r, ost = oracle_lib.ondemand(f, *flags, err, gthr, iters, presimp, mode=1)
+    # the digitized cells equal the FP64 oracle's; with needASC2 the MIN
+    # boundary (1) follows the 3-manifold labels, which may differ from the
+    # oracle's by the pass-1 visiting order (at most 0.1 % of the vertices)
+    for t in (4, 5):
+        assert ((r == t) == (o_r == t)).all(), (t, int(((r == t) != (o_r == t)).sum()))
+    if flags[2]:
+        assert float((lab.ravel() != olab).mean()) <= 1e-3
+    else:
+        assert (r == o_r).all(), int((r != o_r).sum())
+        assert (g == og).all(), int((g != og).sum())
+    # the .grad is a pure function of the restriction labels: bit-exact
+    expect = oracle_lib.flags(oracle_lib.ldir(oracle_lib.lstar_pairs(f, r), f.shape), f.shape)
+    assert (g == expect).all(), int((g != expect).sum())
+    gradcheck.check_consistency(g, X, Y, Z)
+    # against the reference (its mixed-precision arithmetic)
+    diff = int((_labelled(r) != _labelled(ref_r)).sum())
+    assert diff <= max(1, 1e-3 * int(_labelled(ref_r).sum())), diff
+    if flags[0]:
+        assert st["lines_asc"] >= ost[5]
+    if flags[1]:
+        assert st["lines_dsc"] >= ost[12]
+
+
+@pytest.mark.parametrize("window", [8, 61])
+def test_small_windows_give_the_same_bytes(ctx, window):
+    f = synth.smoothed_noise((40, 44, 48), 77, 2.0)
+    _say(f"lines windows {window}")
+    g0, _, r0, s0 = labels.ondemand(f, True, True, False, ctx=ctx)
+    os.environ["MSCEER_LINE_WINDOW"] = str(window)
+    try:
+        g1, _, r1, s1 = labels.ondemand(f, True, True, False, ctx=ctx)
+    finally:
+        del os.environ["MSCEER_LINE_WINDOW"]
+    assert s1["line_windows"] > s0["line_windows"]
+    assert (r1 == r0).all(), int((r1 != r0).sum())
+    assert (g1 == g0).all()
+
+
+def test_larger_volume_matches_fp64_oracle_and_repeats(ctx):
+    f = synth.smoothed_noise((96, 112, 128), 303, 2.0)
+    _say(f"lines {f.shape}")
+    runs = [labels.ondemand(f, True, True, True, ctx=ctx) for _ in range(2)]
+    g, lab, r, st = runs[0]
+    _say(f"  stats {st}")
+    assert (runs[1][2] == r).all() and (runs[1][0] == g).all()
+    og, olab, o_r, ost = oracle_lib.ondemand(f, 1, 1, 1, mode=1)
+    # the 3-manifold labels may differ slightly from the oracle's (pass-1
+    # order), which moves MIN-boundary cells (1) only: the streamlines depend
+    # on the steepest pairs, the field zeroed at the region extrema (the same
+    # on both) and the cells already carrying 4 or 5, so 4 and 5 match exactly
+    for t in (4, 5):
+        d = int(((r == t) != (o_r == t)).sum())
+        _say(f"  cells {t}: {int((r == t).sum())}, differing {d}")
+        assert d == 0
