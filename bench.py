@@ -259,6 +259,7 @@ def bench_labels(args):
                 "stage_ms": {k[3:]: round(float(st.as_dict()[k]), 3) for k in
                              ("ms_lstar", "ms_regions", "ms_integrate", "ms_cleanup", "ms_grad")},
                 "fp64": int(st.reintegrated_fp64)}
+    lines_out = None if args.no_lines else bench_lines(args, ctx, lib, dv, dg, f, C3_PARAMS)
     learned_out = None
     if not args.no_learned:
         learned_out = bench_learned(args, ctx, lib, dv, dg, dl, st, X, Y, Z, ref, plain, run)
@@ -270,10 +271,78 @@ def bench_labels(args):
     return {"workload": f"ondemandaccurate needASC2 + conforming .grad, {X}x{Y}x{Z} float32, 200 Gaussians "
                         f"(sigma 4-16) + 0.05 sin sin sin, seed 303; err 1e-3 (squared error), grad 1e-6, "
                         f"4096 steps, presimp 0; values resident in HBM",
-            "plain": plain, "learned": learned_out, "cpu_baseline": base}
+            "plain": plain, "lines": lines_out, "learned": learned_out, "cpu_baseline": base}
 
 
 CNN_FLOP_PER_VOXEL = 2 * (27 * 16 + 27 * 16 * 32 + 27 * 32 * 16 + 16)  # 56 288
+
+
+def bench_lines(args, ctx, lib, dv, dg, f, params):
+    """Accurate 1-manifolds on the labelling volume: OnDemandAccurate with
+    needASC1 + needDSC1 (and with needASC2 as well), values resident in HBM;
+    the 1-manifold stage timed apart.  CPU: the oracle's restatement of the
+    reference's saddle loops (one thread, as the reference's order is one
+    thread's; its own build cannot be optimised, see DESIGN), the loop alone."""
+    from msceer_amd import hipmem, native
+
+    Z, Y, X = f.shape
+    V = X * Y * Z
+    cells = (2 * X - 1) * (2 * Y - 1) * (2 * Z - 1)
+    err, gthr, iters, presimp = params
+    dr = hipmem.DeviceBuffer(cells)
+    dl = hipmem.DeviceBuffer(4 * V)
+    st = native.LabelStats()
+
+    def run(need):
+        native.check(lib.msceer_ondemand(ctx.handle, ctypes.c_void_p(dv.addr), X, Y, Z, err, gthr, iters, presimp,
+                                         need, None, ctypes.c_void_p(dg.addr),
+                                         ctypes.c_void_p(dl.addr) if need & native.NEED_ASC2 else None,
+                                         ctypes.c_void_p(dr.addr), ctypes.byref(st), native.DEVICE_PTRS),
+                     "msceer_ondemand")
+
+    out = {}
+    for key, need in (("asc1_dsc1", native.NEED_ASC1 | native.NEED_DSC1),
+                      ("asc1_dsc1_asc2", native.NEED_ASC1 | native.NEED_DSC1 | native.NEED_ASC2)):
+        run(need)
+        hipmem.synchronize()
+        t0 = time.perf_counter()
+        stats = []
+        for _ in range(args.label_steps):
+            run(need)
+            stats.append(st.as_dict())
+        hipmem.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.label_steps
+        d = stats[-1]
+        out[key] = {"value": round(V / (ms * 1e-3) / 1e6, 2), "unit": "Mvert/s", "ms_per_step": round(ms, 3),
+                    "lines_ms": round(float(np.mean([x["ms_lines"] for x in stats])), 3),
+                    "streamlines": {"asc": int(d["lines_asc"]), "dsc": int(d["lines_dsc"]),
+                                    "replayed": int(d["lines_replayed"]), "windows": int(d["line_windows"]),
+                                    "replay_cuts": int(d["line_cuts"]), "overflow_cuts": int(d["line_overflows"]),
+                                    "traced_alone": int(d["line_alone"])}}
+    r = dr.download(np.int8, cells)
+    out["cells_4"], out["cells_5"] = int((r == 4).sum()), int((r == 5).sum())
+    out["roofline"] = {"bound": "hbm", "achieved": None, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": None,
+                       "note": "latency-bound: streamlines of a few voxels each, in the saddles' order; "
+                               "the stage is speculation, in-order replays of interacting lines and host "
+                               "read-backs per window"}
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib  # noqa: E402  (checker / baseline only)
+
+        log("timing the 1-manifold CPU baseline (oracle saddle loops, one thread)")
+        oracle_lib.set_threads(_cores())
+        base = oracle_lib.lstar_pairs(f)
+        us = 0
+        for asc in (1, 0):
+            _, ost = oracle_lib.saddle_lines(f, base, asc, err=err, gthr=gthr, iters=iters, mode=0)
+            us += int(ost[7])
+        out["cpu_baseline"] = {"value": round(V / (us * 1e-6) / 1e6, 3), "unit": "Mvert/s", "cores": 1,
+                               "kind": "port", "seconds": round(us * 1e-6, 3),
+                               "sample": f"the whole {X}x{Y}x{Z} volume: saddle gathering, sorting and the "
+                                         f"digitizing streamlines of both families (ASC, DSC), reference "
+                                         f"arithmetic, given the steepest pairs and the gradient field"}
+        out["gpu_lines_vs_cpu_loop"] = round(us * 1e-3 / out["asc1_dsc1"]["lines_ms"], 2)
+    return out
 
 
 def bench_learned(args, ctx, lib, dv, dg, dl, st, X, Y, Z, ref, plain, run):
@@ -526,6 +595,7 @@ def main():
     ap.add_argument("--label-cpu-slab", type=int, default=48)
     ap.add_argument("--ideal-mask-probe", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-lines", action="store_true", help="skip the accurate 1-manifold measurement")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--batch-volumes", type=int, default=512)

@@ -90,8 +90,10 @@ typedef struct msceer_label_stats {
     int64_t lines_dsc;          /* needDSC1: streamlines traced from the 1-saddles */
     int64_t lines_replayed;     /* of the 1-manifold streamlines, re-traced in order inside a component */
     int32_t line_windows;       /* speculation windows of the 1-manifold stage */
-    int32_t line_cuts;          /* windows cut short, and lines traced alone */
+    int32_t line_cuts;          /* windows cut short by a replay that left its component */
     float ms_lines;             /* the 1-manifold stage */
+    int32_t line_alone;         /* 1-manifold streamlines traced alone, in order */
+    int32_t line_overflows;     /* windows cut short where the footprint pool ran out */
 } msceer_label_stats;
 
 int msceer_version(void);

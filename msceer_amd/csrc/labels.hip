@@ -464,7 +464,9 @@ int ondemand_impl(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, 
             st.lines_dsc = (int64_t)ld.traced;
             st.lines_replayed = (int64_t)(la.replayed + ld.replayed);
             st.line_windows = (int32_t)(la.windows + ld.windows);
-            st.line_cuts = (int32_t)(la.cuts + ld.cuts + la.direct + ld.direct);
+            st.line_cuts = (int32_t)(la.cuts + ld.cuts);
+            st.line_alone = (int32_t)(la.direct + ld.direct);
+            st.line_overflows = (int32_t)(la.overflows + ld.overflows);
         }
         CHECK_HIP(hipEventRecord(ev[10], s));
         if (restriction_out)

@@ -24,21 +24,23 @@
 // On the GPU the order is kept exactly by speculation over windows of
 // streamlines ("lines"; the saddle's own label is a line without a trace):
 //  1. every line of the window is traced at once against the labels committed
-//     before the window (plus its own), recording its footprint: the cells it
-//     read (tested for the target) and the cells it labelled;
-//  2. lines are joined (union-find) when one labels a cell another's footprint
-//     holds; a line alone in its component saw exactly the labels it would see
-//     in order, so its labels stand;
-//  3. each component of several lines is replayed by one lane, its lines in
-//     order, re-tracing each against the committed labels and those of its
-//     earlier members.  A replayed line that reads a cell outside its own
-//     footprint, or labels one it did not label before, may depend on or
-//     affect lines of other components: the window is cut before it;
+//     before the window plus its own, recording its footprint: the cells it
+//     tested for the target and the cells it labelled (a cell labelled by
+//     lines of the window belongs to the lowest of them, wstamp);
+//  2. lines are joined (union-find) when one labels a cell in another's
+//     footprint.  A line whose footprint no earlier line of the window
+//     labels saw exactly the labels it would see in order: its labels stand;
+//  3. each component is replayed by one lane, its lines in order: those
+//     depending on an earlier line are re-traced against the committed labels
+//     and the labels of their component's earlier lines.  A replayed line
+//     that tests a cell of another component's labels, or labels a cell its
+//     component did not label in speculation, went beyond what the
+//     speculation saw: the window is cut before it;
 //  4. the labels of the lines before the cut are committed; the next window
-//     starts at the cut.  The first line of a window never fails (nothing
-//     earlier in the window), so every window commits at least one line.
-// A line whose footprint outgrows its record is traced alone, in order, when
-// it is first in its window.
+//     starts at the cut.  The first line of a window depends on nothing, so
+//     every window commits at least one line.
+// A line whose footprint does not fit the pool ends its window; first in a
+// window, it is traced alone, in order.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -48,7 +50,7 @@
 #include "ctx.hpp"
 #include "integrate.hpp"
 #include "lines.hpp"
-#include "tracer.hpp"
+#include "lines_core.hpp"
 
 #pragma clang fp contract(off)
 
@@ -56,10 +58,6 @@ namespace msceer {
 namespace {
 
 constexpr int kThreads = 256;
-constexpr uint32_t kNoLine = 0xFFFFFFFFu;
-constexpr int kFoot = 128;           // footprint entries recorded per line
-constexpr uint8_t kRead = 1, kWrote = 2;
-constexpr int kAbort = 100;          // trace event: the footprint rule stopped it
 
 __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter) {
     const uint64_t m = __ballot(want);
@@ -71,41 +69,6 @@ __device__ __forceinline__ uint32_t wave_append(bool want, uint32_t* counter) {
     base = __shfl(base, leader);
     return base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull));
 }
-
-struct CellGeom {
-    int64_t Mx, My, Mz, sxy;
-    __device__ uint32_t id(int64_t x, int64_t y, int64_t z) const { return (uint32_t)(x + Mx * (y + My * z)); }
-    __device__ void xyz(uint32_t c, int64_t& x, int64_t& y, int64_t& z) const {
-        x = (int64_t)c % Mx;
-        y = ((int64_t)c / Mx) % My;
-        z = (int64_t)c / sxy;
-    }
-    __device__ int64_t off(int a) const { return a == 0 ? 1 : (a == 1 ? Mx : sxy); }
-    __device__ int64_t ext(int a) const { return a == 0 ? Mx : (a == 1 ? My : Mz); }
-    // Cell coordinates inside the complex.  A streamline away from the
-    // boundary stays inside; this only keeps a step that a huge error
-    // threshold let run far from addressing outside the arrays.
-    __device__ bool inside(int64_t x, int64_t y, int64_t z) const {
-        return x >= 0 && y >= 0 && z >= 0 && x < Mx && y < My && z < Mz;
-    }
-};
-
-// Order key of a float: monotone, -0 and +0 equal (as float comparison has them).
-__device__ __forceinline__ uint32_t value_key(float f) {
-    uint32_t u = __builtin_bit_cast(uint32_t, f);
-    if ((u & 0x7fffffffu) == 0u) u = 0u;
-    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
-}
-
-__device__ __forceinline__ bool critical(const uint8_t* grad, uint32_t c) { return ((grad[c] >> 2) & 7u) == 7u; }
-
-__device__ __forceinline__ uint32_t pair_of(const CellGeom& m, const uint8_t* grad, uint32_t c) {
-    const uint32_t code = (grad[c] >> 2) & 7u;
-    const int64_t o = m.off((int)(code >> 1));
-    return (uint32_t)((int64_t)c + ((code & 1u) ? -o : o));
-}
-
-__device__ __forceinline__ double comp(const V3<double>& v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
 
 // ---- the saddles and their seeds -------------------------------------------
 
@@ -144,283 +107,13 @@ line_saddles_kernel(const uint8_t* __restrict__ grad, CellGeom m, int X, int Y, 
     }
 }
 
-// Per saddle (rank k in processing order): the seed cells of the short
-// V-path search (ondemandaccurate.cxx:319-351 ASC, 448-480 DSC), at most six,
-// in slots 7k .. 7k+5 (kNoLine where absent), and the saddle in slot 7k+6.
-// The search pops at most five cells: the saddle, whose two cofacets (ASC) /
-// facets (DSC) are new, then cells that each add one new neighbour.
+// Per saddle (rank k in processing order): its seeds in slots 7k .. 7k+5 and
+// the saddle in slot 7k+6.
 __global__ void __launch_bounds__(kThreads)
 line_seeds_kernel(const uint8_t* __restrict__ grad, CellGeom m, const uint64_t* __restrict__ keys, uint32_t n,
                   int asc, uint32_t* __restrict__ slots) {
     const uint32_t k = blockIdx.x * kThreads + threadIdx.x;
-    if (k >= n) return;
-    const uint32_t s = (uint32_t)keys[k];
-    uint32_t result[6], queue[6], seen[16];
-    int nres = 0, qh = 0, qt = 0, ns = 0;
-    auto is_seen = [&](uint32_t c) {
-        for (int i = 0; i < ns; i++)
-            if (seen[i] == c) return true;
-        return false;
-    };
-    queue[qt++] = s;
-    for (int counter = 4; qh < qt && counter >= 0; counter--) {
-        const uint32_t cur = queue[qh++];
-        if (!is_seen(cur) && ns < 16) seen[ns++] = cur;
-        int64_t p[3];
-        m.xyz(cur, p[0], p[1], p[2]);
-        for (int a = 0; a < 3; a++) {
-            if (asc ? (p[a] & 1) != 0 : (p[a] & 1) == 0) continue;
-            for (int sg = 0; sg < 2; sg++) {
-                if (sg == 0 ? p[a] >= m.ext(a) - 1 : p[a] <= 0) continue;
-                const uint32_t t = (uint32_t)((int64_t)cur + (sg == 0 ? m.off(a) : -m.off(a)));
-                if (is_seen(t)) continue;
-                if (nres < 6) result[nres++] = t;
-                if (ns < 16) seen[ns++] = t;
-                if (!critical(grad, t)) {
-                    const uint32_t pr = pair_of(m, grad, t);
-                    if (ns < 16) seen[ns++] = pr;
-                    if (qt < 6) queue[qt++] = pr;
-                }
-            }
-        }
-    }
-    uint32_t* out = slots + 7 * (uint64_t)k;
-    for (int j = 0; j < 6; j++) out[j] = j < nres ? result[j] : kNoLine;
-    out[6] = s;
-}
-
-// ---- one streamline ----------------------------------------------------------
-
-// Label policies of a trace: test(c) -- does cell c carry the target (a
-// checker or digitizing test); write(c) -- give it the target; ok() -- false
-// once the trace must stop for a footprint rule.
-
-// Alone and in order: the restriction labels themselves.
-struct DirectLabels {
-    int8_t* restr;
-    int8_t target;
-    __device__ bool test(uint32_t c) { return restr[c] == target; }
-    __device__ void write(uint32_t c) { restr[c] = target; }
-    __device__ bool ok() const { return true; }
-};
-
-// Speculation: committed labels plus the line's own; every cell tested or
-// labelled goes to the footprint (cells, flags), labelled ones also claim
-// wstamp (lowest line of the window that labels the cell).
-struct SpecLabels {
-    const int8_t* restr;
-    int8_t target;
-    uint32_t* wstamp;
-    uint32_t line;
-    uint32_t* cells;
-    uint8_t* flags;
-    int n;
-    bool over;
-    __device__ int find(uint32_t c) const {
-        for (int i = 0; i < n; i++)
-            if (cells[i] == c) return i;
-        return -1;
-    }
-    __device__ int add(uint32_t c) {
-        int i = find(c);
-        if (i >= 0) return i;
-        if (n >= kFoot) {
-            over = true;
-            return -1;
-        }
-        cells[n] = c;
-        flags[n] = 0;
-        return n++;
-    }
-    __device__ bool test(uint32_t c) {
-        const int i = add(c);
-        if (i < 0) return false;
-        flags[i] |= kRead;
-        return (flags[i] & kWrote) || restr[c] == target;
-    }
-    __device__ void write(uint32_t c) {
-        const int i = add(c);
-        if (i < 0) return;
-        flags[i] |= kWrote;
-        atomicMin(&wstamp[c], line);
-    }
-    __device__ bool ok() const { return !over; }
-};
-
-// Replay: committed labels plus those of the component's earlier lines
-// (lstamp); every access must stay inside the line's speculative footprint
-// and every label inside its speculative labels, else the window is cut here.
-struct ReplayLabels {
-    const int8_t* restr;
-    int8_t target;
-    uint32_t* lstamp;
-    uint32_t line;
-    const uint32_t* cells;
-    const uint8_t* flags;
-    int n;
-    uint32_t* fail;
-    bool bad;
-    __device__ int find(uint32_t c) const {
-        for (int i = 0; i < n; i++)
-            if (cells[i] == c) return i;
-        return -1;
-    }
-    __device__ bool test(uint32_t c) {
-        if (find(c) < 0) {
-            bad = true;
-            return false;
-        }
-        return restr[c] == target || lstamp[c] != kNoLine;
-    }
-    __device__ void write(uint32_t c) {
-        const int i = find(c);
-        if (i < 0 || !(flags[i] & kWrote)) {
-            bad = true;
-            return;
-        }
-        lstamp[c] = line;
-    }
-    __device__ bool ok() const { return !bad; }
-};
-
-// DigitizeSegmentInternal: the faces the segment a -> b crosses, by their
-// crossing parameter; for each face that bounds the current cell, label the
-// current cell and the face (stop at the first already labelled) and move to
-// the cell beyond.  ASC: voxel faces, cells = hexes; DSC: the dual grid
-// (positions shifted by half a voxel), faces = edges, cells = vertices.
-template <class L>
-__device__ void digitize(const CellGeom& m, V3<double> a, V3<double> b, bool asc, L& lab) {
-    if (!asc) {
-        a = {a.x + 0.5, a.y + 0.5, a.z + 0.5};
-        b = {b.x + 0.5, b.y + 0.5, b.z + 0.5};
-    }
-    const V3<double> ab{b.x - a.x, b.y - a.y, b.z - a.z};
-    uint32_t face[3];
-    double tc[3];
-    int n = 0;
-    for (int d = 0; d < 3; d++) {
-        const double av = comp(a, d), bv = comp(b, d);
-        const int va = (int)floor(av), vb = (int)floor(bv);
-        if (va == vb) continue;
-        const int plane = va > vb ? va : vb;
-        const double t = ((double)plane - av) / (bv - av);
-        tc[n] = t;
-        if (t < 0 || t > 1) return;
-        int64_t fc[3];
-        for (int k = 0; k < 3; k++)
-            fc[k] = 2 * (int64_t)floor(comp(ab, k) * t + comp(a, k)) + (k == d ? 0 : 1) - (asc ? 0 : 1);
-        if (!m.inside(fc[0], fc[1], fc[2])) return;
-        face[n++] = m.id(fc[0], fc[1], fc[2]);
-    }
-    if (n == 0) return;
-    for (int i = 0; i < n; i++)
-        for (int j = i; j < n; j++)
-            if (tc[i] > tc[j]) {
-                const uint32_t tf = face[i]; face[i] = face[j]; face[j] = tf;
-                const double tt = tc[i]; tc[i] = tc[j]; tc[j] = tt;
-            }
-    const int64_t s0 = 2 * (int64_t)floor(a.x) + (asc ? 1 : 0), s1 = 2 * (int64_t)floor(a.y) + (asc ? 1 : 0),
-                  s2 = 2 * (int64_t)floor(a.z) + (asc ? 1 : 0);
-    if (!m.inside(s0, s1, s2)) return;
-    uint32_t cur = m.id(s0, s1, s2);
-    for (int i = 0; i < n; i++) {
-        int64_t p[3];
-        m.xyz(face[i], p[0], p[1], p[2]);
-        bool has = false;
-        uint32_t next = cur;
-        for (int d = 0; d < 3; d++) {
-            if (asc ? (p[d] & 1) != 0 : (p[d] & 1) == 0) continue;
-            if (p[d] < m.ext(d) - 1) {
-                const uint32_t c = (uint32_t)((int64_t)face[i] + m.off(d));
-                if (c == cur) has = true; else next = c;
-            }
-            if (p[d] > 0) {
-                const uint32_t c = (uint32_t)((int64_t)face[i] - m.off(d));
-                if (c == cur) has = true; else next = c;
-            }
-        }
-        if (!has) continue;
-        if (lab.test(cur)) return;
-        lab.write(cur);
-        if (!lab.ok()) return;
-        if (lab.test(face[i])) return;
-        lab.write(face[i]);
-        if (!lab.ok()) return;
-        cur = next;
-    }
-}
-
-// AdvectThroughVoxel{NoCheck,NearBoundary} in FP64 with the line checker:
-// near the boundary the first accepted step ends the line; elsewhere a step
-// into a cell carrying the target does.
-template <class L>
-__device__ int advect_voxel(const VGrid& g, const CellGeom& m, Tracer<double>& t, double dir, double gthr,
-                            double ethr, bool asc, L& lab) {
-    while (true) {
-        if (!(t.iters-- > 0)) return kOverMax;
-        if (!(t.old.x == t.cur.x && t.old.y == t.cur.y && t.old.z == t.cur.z)) {
-            if ((int)t.cur.x != (int)t.old.x || (int)t.cur.y != (int)t.old.y || (int)t.cur.z != (int)t.old.z)
-                return kOutOfVoxel;
-            t.gr = trilin(t.c, t.cur.x - (double)t.bx, t.cur.y - (double)t.by, t.cur.z - (double)t.bz);
-            t.gm = sqrt(t.gr.x * t.gr.x + t.gr.y * t.gr.y + t.gr.z * t.gr.z);
-            if (t.gm * t.step > kMaxStep) t.step = kMaxStep / t.gm;
-            t.old = t.cur;
-        }
-        if (t.gm < gthr) return kLowGradient;
-        const double s = dir * t.step, hs = (dir * 0.5) * t.step;
-        const V3<double> full{t.gr.x * s, t.gr.y * s, t.gr.z * s};
-        const V3<double> half{t.gr.x * hs, t.gr.y * hs, t.gr.z * hs};
-        const V3<double> hg = trilin(t.c, t.cur.x + half.x - (double)t.bx, t.cur.y + half.y - (double)t.by,
-                                     t.cur.z + half.z - (double)t.bz);
-        const V3<double> two{hg.x * hs + half.x, hg.y * hs + half.y, hg.z * hs + half.z};
-        const double dx = two.x - full.x, dy = two.y - full.y, dz = two.z - full.z;
-        const double err = dx * dx + dy * dy + dz * dz;
-        if (!(err < ethr)) {
-            t.step = t.step * 0.5;
-            continue;
-        }
-        t.cur = {t.cur.x + two.x, t.cur.y + two.y, t.cur.z + two.z};
-        if (t.near) { t.cur.x = clampc(t.cur.x, g.X); t.cur.y = clampc(t.cur.y, g.Y); t.cur.z = clampc(t.cur.z, g.Z); }
-        const double grow = t.step * 1.8, cap = kMaxStep / t.gm;
-        t.step = grow < cap ? grow : cap;
-        if (t.near) return kHitPreassigned;
-        const int64_t cx = asc ? 2 * (int64_t)floor(t.cur.x) + 1 : 2 * (int64_t)floor(t.cur.x + 0.5);
-        const int64_t cy = asc ? 2 * (int64_t)floor(t.cur.y) + 1 : 2 * (int64_t)floor(t.cur.y + 0.5);
-        const int64_t cz = asc ? 2 * (int64_t)floor(t.cur.z) + 1 : 2 * (int64_t)floor(t.cur.z + 0.5);
-        if (!m.inside(cx, cy, cz)) continue;
-        const bool hit = lab.test(m.id(cx, cy, cz));
-        if (!lab.ok()) return kAbort;
-        if (hit) return kHitPreassigned;
-    }
-}
-
-// IntegrateStreamline from the centre of seed cell `seed` (cell coordinates / 2).
-template <class L>
-__device__ int trace_line(const GradSrc<double>& src, const VGrid& g, const CellGeom& m, uint32_t seed, bool asc,
-                          double gthr, double ethr, int max_iters, L& lab, uint32_t* voxels) {
-    int64_t sx, sy, sz;
-    m.xyz(seed, sx, sy, sz);
-    Tracer<double> t;
-    t.cur = {(double)sx * 0.5, (double)sy * 0.5, (double)sz * 0.5};
-    t.iters = max_iters;
-    tracer_load(src, g, t, (int)(t.cur.x + 0.5), (int)(t.cur.y + 0.5), (int)(t.cur.z + 0.5));
-    tracer_begin(t);
-    const double dir = asc ? 1.0 : -1.0;
-    while (true) {
-        const V3<double> start = t.cur;
-        const int ev = advect_voxel(g, m, t, dir, gthr, ethr, asc, lab);
-        (*voxels)++;
-        if (ev == kAbort) return kAbort;
-        if (!t.near) {
-            digitize(m, start, t.cur, asc, lab);
-            if (!lab.ok()) return kAbort;
-        }
-        if (ev != kOutOfVoxel) return ev;
-        int x, y, z;
-        nearest_xyz(g, t.cur, t.near, x, y, z);
-        tracer_load(src, g, t, x, y, z);
-        tracer_begin(t);
-    }
+    if (k < n) saddle_seeds(grad, m, (uint32_t)keys[k], asc != 0, slots + 7 * (uint64_t)k);
 }
 
 struct LineArgs {
@@ -434,29 +127,36 @@ struct LineArgs {
     double gthr, ethr;
     int max_iters;
     uint32_t* wstamp;  // per cell, kNoLine outside a window
+    uint32_t* pstamp;  // per cell, kNoLine outside a window
     uint32_t* lstamp;  // per cell, kNoLine outside a window
-    uint32_t* foot;    // kFoot cells per line of the window
-    uint8_t* fflags;   // kFoot flags per line of the window
-    uint32_t* fcount;  // entries per line of the window
-    uint32_t* ctl;     // kLineCtl words
+    uint32_t* own;     // per cell, kNoLine outside a window
+    Pool pool;
+    uint32_t* fhead;   // first footprint chunk per line of the window (kNoLine: none)
+    uint32_t* whead;   // first chunk of the cells each line labelled
+    uint32_t* mhead;   // first chunk of the cells each line's stops could label
+    uint32_t* touched; // cells labelled speculatively in the window
+    uint32_t* ctl;     // kLcWords words
 };
 
 enum LineCtl : int {
-    kLcOverflow = 0,  // lowest line whose footprint outgrew its record
-    kLcFail = 1,      // lowest line whose replay left its footprint
+    kLcOverflow = 0,  // lowest line whose footprint did not fit the pool
+    kLcFail = 1,      // lowest line whose replay left its component
     kLcComps = 2,     // components of the window
     kLcTraced = 3,    // streamlines traced (all phases)
     kLcReplayed = 4,  // streamlines re-traced in replays
     kLcVoxels = 5,    // voxel traversals (all phases)
-    kLcWords = 8
+    kLcTouched = 6,   // cells labelled speculatively in the window
+    kLcPoolTop = 7,   // next free word of the footprint pool
+    kLcWhyTest = 8,   // replays stopped by a test of another component's cell
+    kLcWhyLabel = 9,  // replays stopped by a label the speculation did not give
+    kLcWords = 12
 };
 
 // 1. Speculation over lines [w0, w1).
 __global__ void __launch_bounds__(kThreads) line_spec_kernel(LineArgs a, uint32_t w0, uint32_t w1) {
     const uint32_t i = w0 + blockIdx.x * kThreads + threadIdx.x;
     if (i >= w1) return;
-    const uint64_t r = i - w0;
-    SpecLabels lab{a.restr, a.target, a.wstamp, i, a.foot + r * kFoot, a.fflags + r * kFoot, 0, false};
+    SpecLabels lab{a.restr, a.target, a.wstamp, a.pstamp, a.own, i, a.pool, a.touched, &a.ctl[kLcTouched]};
     const uint32_t c = a.slots[i];
     if (c != kNoLine) {
         if (i % 7 == 6) {
@@ -468,54 +168,40 @@ __global__ void __launch_bounds__(kThreads) line_spec_kernel(LineArgs a, uint32_
             atomicAdd(&a.ctl[kLcVoxels], vox);
         }
     }
-    a.fcount[r] = (uint32_t)lab.n;
+    lab.finish();
+    a.fhead[i - w0] = lab.over ? kNoLine : lab.all.head;
+    a.whead[i - w0] = lab.over ? kNoLine : lab.wrote.head;
+    a.mhead[i - w0] = lab.over ? kNoLine : lab.maybe.head;
     if (lab.over) atomicMin(&a.ctl[kLcOverflow], i);
 }
 
-__device__ __forceinline__ uint32_t lf_parent(uint32_t* p, uint32_t x) {
-    return __hip_atomic_load(&p[x], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ uint32_t lf_find(uint32_t* p, uint32_t x) {
-    uint32_t q = lf_parent(p, x);
-    while (q != x) {
-        const uint32_t gq = lf_parent(p, q);
-        if (gq != q) __hip_atomic_store(&p[x], gq, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        x = q;
-        q = gq;
-    }
-    return x;
-}
-// Hook the later root under the earlier one, so a component's root is its
-// first line.
-__device__ void lf_union(uint32_t* p, uint32_t a, uint32_t b) {
-    while (true) {
-        a = lf_find(p, a);
-        b = lf_find(p, b);
-        if (a == b) return;
-        if (a > b) { const uint32_t t = a; a = b; b = t; }
-        uint32_t expected = b;
-        if (__hip_atomic_compare_exchange_strong(&p[b], &expected, a, __ATOMIC_RELAXED, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT))
-            return;
-    }
-}
-
-// 2. Components: a line joins the lowest line of [w0, w1) that labels any
-// cell of its footprint (every line that labels a cell also tests it, so all
-// lines touching a labelled cell meet in one component).
+// 2. Components: a line joins the lowest line of [w0, wc) that labels any
+// cell of its footprint (every line that labels a cell also has it in its
+// footprint, so all lines touching a labelled cell meet in one component).
 __global__ void __launch_bounds__(kThreads) line_init_parent_kernel(uint32_t* parent, uint32_t n) {
     const uint32_t r = blockIdx.x * kThreads + threadIdx.x;
     if (r < n) parent[r] = r;
 }
-__global__ void __launch_bounds__(kThreads) line_union_kernel(LineArgs a, uint32_t w0, uint32_t w1, uint32_t* parent) {
+// Also flags the lines that depend on an earlier line of the window (an
+// earlier line labels a cell of their footprint): only those are re-traced.
+__global__ void __launch_bounds__(kThreads)
+line_union_kernel(LineArgs a, uint32_t w0, uint32_t wc, uint32_t* parent, uint8_t* dep) {
     const uint32_t i = w0 + blockIdx.x * kThreads + threadIdx.x;
-    if (i >= w1) return;
-    const uint64_t r = i - w0;
-    const uint32_t n = a.fcount[r];
-    for (uint32_t j = 0; j < n; j++) {
-        const uint32_t w = a.wstamp[a.foot[r * kFoot + j]];
-        if (w != i && w >= w0 && w < w1) lf_union(parent, (uint32_t)r, w - w0);
+    if (i >= wc) return;
+    bool d = false;
+    for (uint32_t ch = a.fhead[i - w0]; ch != kNoLine; ch = a.pool.cells[ch]) {
+        for (uint32_t j = 1; j < kChunk; j++) {
+            const uint32_t c = a.pool.cells[ch + j];
+            if (c == kNoLine) break;
+            const uint32_t owners[2] = {a.wstamp[c], a.pstamp[c]};
+            for (uint32_t w : owners)
+                if (w != i && w >= w0 && w < wc) {
+                    lf_union(parent, i - w0, w - w0);
+                    d = d || w < i;
+                }
+        }
     }
+    dep[i - w0] = d ? 1 : 0;
 }
 __global__ void __launch_bounds__(kThreads) line_keys_kernel(uint32_t* parent, uint32_t n, uint64_t* keys) {
     const uint32_t r = blockIdx.x * kThreads + threadIdx.x;
@@ -533,29 +219,37 @@ line_starts_kernel(const uint64_t* keys, uint32_t n, uint32_t* starts, uint32_t*
     }
 }
 
-// 3. Replay: one lane per component.  A lone line keeps its speculative
-// labels; the lines of a larger component are re-traced in order.
+// 3. Replay: one lane per component, its lines in order.  A line no earlier
+// line of the window depends on keeps its speculative labels (the cells of
+// its footprint it claimed in wstamp: no earlier line labels any cell it
+// reads, so it saw what it would see in order); the others are re-traced.
+__device__ void keep_speculation(const LineArgs& a, uint32_t w0, uint32_t i) {
+    for (uint32_t ch = a.fhead[i - w0]; ch != kNoLine; ch = a.pool.cells[ch])
+        for (uint32_t j = 1; j < kChunk; j++) {
+            const uint32_t c = a.pool.cells[ch + j];
+            if (c == kNoLine) break;
+            if (a.wstamp[c] == i) a.lstamp[c] = i;
+        }
+}
+
 __global__ void __launch_bounds__(kThreads)
-line_replay_kernel(LineArgs a, uint32_t w0, uint32_t n, const uint64_t* keys, const uint32_t* starts) {
+line_replay_kernel(LineArgs a, uint32_t w0, uint32_t wc, const uint64_t* keys, const uint32_t* starts,
+                   uint32_t* parent, const uint8_t* dep) {
     const uint32_t k = blockIdx.x * kThreads + threadIdx.x;
     if (k >= a.ctl[kLcComps]) return;
+    const uint32_t n = wc - w0;
     const uint32_t s0 = starts[k];
     const uint32_t root = (uint32_t)(keys[s0] >> 32);
-    uint32_t e = s0 + 1;
-    while (e < n && (uint32_t)(keys[e] >> 32) == root) e++;
-    if (e - s0 == 1) {
-        const uint64_t r = (uint32_t)keys[s0];
-        const uint32_t cnt = a.fcount[r];
-        for (uint32_t j = 0; j < cnt; j++)
-            if (a.fflags[r * kFoot + j] & kWrote) a.lstamp[a.foot[r * kFoot + j]] = w0 + (uint32_t)r;
-        return;
-    }
     uint32_t replayed = 0, vox = 0;
-    for (uint32_t q = s0; q < e; q++) {
-        const uint64_t r = (uint32_t)keys[q];
-        const uint32_t i = w0 + (uint32_t)r;
-        ReplayLabels lab{a.restr, a.target, a.lstamp, i, a.foot + r * kFoot, a.fflags + r * kFoot,
-                         (int)a.fcount[r], &a.ctl[kLcFail], false};
+    for (uint32_t q = s0; q < n && (uint32_t)(keys[q] >> 32) == root; q++) {
+        const uint32_t i = w0 + (uint32_t)keys[q];
+        if (!dep[i - w0]) {
+            keep_speculation(a, w0, i);
+            continue;
+        }
+        ReplayLabels lab{a.restr, a.target,      a.wstamp,       a.pstamp, a.lstamp, a.own,
+                         a.pool.cells, a.whead[i - w0], a.mhead[i - w0], parent, w0, wc, root, i,
+                         &a.ctl[kLcWhyTest], false};
         const uint32_t c = a.slots[i];
         if (c != kNoLine) {
             if (i % 7 == 6) {
@@ -575,30 +269,26 @@ line_replay_kernel(LineArgs a, uint32_t w0, uint32_t n, const uint64_t* keys, co
 }
 
 // 4. Commit the labels of the lines before `cut`, then clear the stamps of
-// every footprint of the window.
-__global__ void __launch_bounds__(kThreads) line_commit_kernel(LineArgs a, uint32_t w0, uint32_t w1, uint32_t cut) {
-    const uint32_t i = w0 + blockIdx.x * kThreads + threadIdx.x;
-    if (i >= w1) return;
-    const uint64_t r = i - w0;
-    const uint32_t n = a.fcount[r];
-    for (uint32_t j = 0; j < n; j++) {
-        const uint32_t c = a.foot[r * kFoot + j];
+// every cell labelled speculatively in the window.
+__global__ void __launch_bounds__(kThreads) line_commit_kernel(LineArgs a, uint32_t cut) {
+    const uint32_t n = min(a.ctl[kLcTouched], a.pool.capacity);
+    for (uint32_t t = blockIdx.x * kThreads + threadIdx.x; t < n; t += gridDim.x * kThreads) {
+        const uint32_t c = a.touched[t];
         if (a.lstamp[c] < cut) a.restr[c] = a.target;
     }
 }
-__global__ void __launch_bounds__(kThreads) line_clear_kernel(LineArgs a, uint32_t w0, uint32_t w1) {
-    const uint32_t i = w0 + blockIdx.x * kThreads + threadIdx.x;
-    if (i >= w1) return;
-    const uint64_t r = i - w0;
-    const uint32_t n = a.fcount[r];
-    for (uint32_t j = 0; j < n; j++) {
-        const uint32_t c = a.foot[r * kFoot + j];
+__global__ void __launch_bounds__(kThreads) line_clear_kernel(LineArgs a) {
+    const uint32_t n = min(a.ctl[kLcTouched], a.pool.capacity);
+    for (uint32_t t = blockIdx.x * kThreads + threadIdx.x; t < n; t += gridDim.x * kThreads) {
+        const uint32_t c = a.touched[t];
         a.wstamp[c] = kNoLine;
+        a.pstamp[c] = kNoLine;
         a.lstamp[c] = kNoLine;
+        a.own[c] = kNoLine;
     }
 }
 
-// A line traced alone, in order (first of its window, footprint too large).
+// A line traced alone, in order (first of its window, footprint beyond the pool).
 __global__ void line_direct_kernel(LineArgs a, uint32_t i) {
     if (blockIdx.x != 0 || threadIdx.x != 0) return;
     DirectLabels lab{a.restr, a.target};
@@ -619,10 +309,14 @@ __global__ void line_ctl_init_kernel(uint32_t* ctl, bool all) {
         ctl[kLcOverflow] = kNoLine;
         ctl[kLcFail] = kNoLine;
         ctl[kLcComps] = 0;
+        ctl[kLcTouched] = 0;
+        ctl[kLcPoolTop] = 0;
         if (all) {
             ctl[kLcTraced] = 0;
             ctl[kLcReplayed] = 0;
             ctl[kLcVoxels] = 0;
+            ctl[kLcWhyTest] = 0;
+            ctl[kLcWhyLabel] = 0;
         }
     }
 }
@@ -658,16 +352,19 @@ int run_lines(msceer_ctx* ctx, const Dims& d, const float* values, const uint8_t
     uint32_t* ctl = ctl_dev(ctx);
     uint32_t* count = ctl + kCtlCount;
 
-    // stamps: one word each per cell, kNoLine between windows
-    CHECK_HIP(ctx->line_stamps.ensure(8 * d.cells));
+    // per-cell stamps (lowest speculative labeller, lowest possible labeller,
+    // replay labeller, last speculative labeller), kNoLine between windows
+    CHECK_HIP(ctx->line_stamps.ensure(16 * d.cells));
     uint32_t* wstamp = ctx->line_stamps.as<uint32_t>();
-    uint32_t* lstamp = wstamp + d.cells;
+    uint32_t* pstamp = wstamp + d.cells;
+    uint32_t* lstamp = pstamp + d.cells;
+    uint32_t* own = lstamp + d.cells;
     if (!ctx->line_stamps_clear || ctx->line_stamps_cells != d.cells) {
-        hipLaunchKernelGGL(line_fill_kernel, dim3(2048), dim3(kThreads), 0, s, wstamp, 2 * d.cells, kNoLine);
+        hipLaunchKernelGGL(line_fill_kernel, dim3(2048), dim3(kThreads), 0, s, wstamp, 4 * d.cells, kNoLine);
         CHECK_HIP(hipGetLastError());
         ctx->line_stamps_cells = d.cells;
     }
-    ctx->line_stamps_clear = false;  // until every window has cleared its footprints
+    ctx->line_stamps_clear = false;  // until every window has cleared its cells
 
     // saddles: count first, then keys sorted into processing order
     CHECK_HIP(launch_zero(count, 1, s));
@@ -685,6 +382,11 @@ int run_lines(msceer_ctx* ctx, const Dims& d, const float* values, const uint8_t
     if ((uint64_t)S * 7 >= 0xFFFFFFFFull) return MSCEER_ETOOBIG;
     const uint64_t nslots = 7ull * S;
     const uint32_t W = (uint32_t)std::min<uint64_t>(nslots, line_window());
+    // footprint pool: 256 cells per line of a window on average, at least 2^22
+    // words (a speculative trace runs until committed or its own labels stop
+    // it, so early in a volume it runs to the extremum)
+    const uint32_t pool_cap = (uint32_t)std::min<uint64_t>(
+        std::max<uint64_t>((uint64_t)W * 256, 1u << 22) / kChunk * kChunk, 0x7FFFFFFFull / kChunk * kChunk);
 
     // scratch layout
     size_t sort_tmp = 0, sort_tmp2 = 0;
@@ -698,10 +400,13 @@ int run_lines(msceer_ctx* ctx, const Dims& d, const float* values, const uint8_t
     const size_t o_keys = off; off += al(8 * (size_t)S);
     const size_t o_sorted = off; off += al(8 * (size_t)S);
     const size_t o_slots = off; off += al(4 * nslots);
-    const size_t o_foot = off; off += al((size_t)W * 4 * kFoot);
-    const size_t o_fflags = off; off += al((size_t)W * kFoot);
-    const size_t o_fcount = off; off += al((size_t)W * 4);
+    const size_t o_pool = off; off += al(4 * (size_t)pool_cap);
+    const size_t o_touched = off; off += al(4 * (size_t)pool_cap);
+    const size_t o_fhead = off; off += al((size_t)W * 4);
+    const size_t o_whead = off; off += al((size_t)W * 4);
+    const size_t o_mhead = off; off += al((size_t)W * 4);
     const size_t o_parent = off; off += al((size_t)W * 4);
+    const size_t o_dep = off; off += al((size_t)W);
     const size_t o_wkeys = off; off += al((size_t)W * 8);
     const size_t o_wsorted = off; off += al((size_t)W * 8);
     const size_t o_starts = off; off += al((size_t)W * 4);
@@ -713,6 +418,7 @@ int run_lines(msceer_ctx* ctx, const Dims& d, const float* values, const uint8_t
     uint64_t* sorted = (uint64_t*)(base + o_sorted);
     uint32_t* slots = (uint32_t*)(base + o_slots);
     uint32_t* parent = (uint32_t*)(base + o_parent);
+    uint8_t* dep = (uint8_t*)(base + o_dep);
     uint64_t* wkeys = (uint64_t*)(base + o_wkeys);
     uint64_t* wsorted = (uint64_t*)(base + o_wsorted);
     uint32_t* starts = (uint32_t*)(base + o_starts);
@@ -741,10 +447,14 @@ int run_lines(msceer_ctx* ctx, const Dims& d, const float* values, const uint8_t
     a.ethr = (double)ethr;
     a.max_iters = max_iters;
     a.wstamp = wstamp;
+    a.pstamp = pstamp;
     a.lstamp = lstamp;
-    a.foot = (uint32_t*)(base + o_foot);
-    a.fflags = (uint8_t*)(base + o_fflags);
-    a.fcount = (uint32_t*)(base + o_fcount);
+    a.own = own;
+    a.pool = Pool{(uint32_t*)(base + o_pool), pool_cap, lctl + kLcPoolTop};
+    a.fhead = (uint32_t*)(base + o_fhead);
+    a.whead = (uint32_t*)(base + o_whead);
+    a.mhead = (uint32_t*)(base + o_mhead);
+    a.touched = (uint32_t*)(base + o_touched);
     a.ctl = lctl;
     hipLaunchKernelGGL(line_ctl_init_kernel, dim3(1), dim3(64), 0, s, lctl, true);
     CHECK_HIP(hipGetLastError());
@@ -762,8 +472,8 @@ int run_lines(msceer_ctx* ctx, const Dims& d, const float* values, const uint8_t
         CHECK_HIP(hipStreamSynchronize(s));
         out->windows++;
         if (h[kLcOverflow] == a0) {
-            // the first line's footprint outgrew its record: it alone, in order
-            hipLaunchKernelGGL(line_clear_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, s, a, a0, w1);
+            // the first line's footprint did not fit the pool: it alone, in order
+            hipLaunchKernelGGL(line_clear_kernel, dim3(256), dim3(kThreads), 0, s, a);
             hipLaunchKernelGGL(line_direct_kernel, dim3(1), dim3(64), 0, s, a, a0);
             CHECK_HIP(hipGetLastError());
             out->direct++;
@@ -771,24 +481,35 @@ int run_lines(msceer_ctx* ctx, const Dims& d, const float* values, const uint8_t
             continue;
         }
         const uint32_t wc = h[kLcOverflow] < w1 ? h[kLcOverflow] : w1;  // lines of this window
+        if (wc < w1) out->overflows++;
         const uint32_t nc = wc - a0;
         hipLaunchKernelGGL(line_init_parent_kernel, dim3(blocks_for(nc)), dim3(kThreads), 0, s, parent, nc);
-        hipLaunchKernelGGL(line_union_kernel, dim3(blocks_for(nc)), dim3(kThreads), 0, s, a, a0, wc, parent);
+        hipLaunchKernelGGL(line_union_kernel, dim3(blocks_for(nc)), dim3(kThreads), 0, s, a, a0, wc, parent, dep);
         hipLaunchKernelGGL(line_keys_kernel, dim3(blocks_for(nc)), dim3(kThreads), 0, s, parent, nc, wkeys);
         CHECK_HIP(hipGetLastError());
         tb = tmp_bytes;
         CHECK_HIP(hipcub::DeviceRadixSort::SortKeys(tmp, tb, wkeys, wsorted, (int)nc, 0, 64, s));
         hipLaunchKernelGGL(line_starts_kernel, dim3(std::min<uint32_t>(blocks_for(nc), 1024)), dim3(kThreads), 0, s,
                            wsorted, nc, starts, lctl + kLcComps);
-        hipLaunchKernelGGL(line_replay_kernel, dim3(blocks_for(nc)), dim3(kThreads), 0, s, a, a0, nc, wsorted,
-                           starts);
+        hipLaunchKernelGGL(line_replay_kernel, dim3(blocks_for(nc)), dim3(kThreads), 0, s, a, a0, wc, wsorted, starts,
+                           parent, dep);
         CHECK_HIP(hipGetLastError());
         CHECK_HIP(hipMemcpyAsync(h, lctl, sizeof(h), hipMemcpyDeviceToHost, s));
         CHECK_HIP(hipStreamSynchronize(s));
         const uint32_t cut = h[kLcFail] < wc ? h[kLcFail] : wc;
-        if (cut == a0) return MSCEER_EHIP;  // the first line of a window cannot fail
-        hipLaunchKernelGGL(line_commit_kernel, dim3(blocks_for(nc)), dim3(kThreads), 0, s, a, a0, wc, cut);
-        hipLaunchKernelGGL(line_clear_kernel, dim3(blocks_for(n)), dim3(kThreads), 0, s, a, a0, w1);
+        if (cut == a0) {
+            // (the first line's speculation is exact, so its replay is too;
+            // should it fail all the same, it is traced alone, in order)
+            hipLaunchKernelGGL(line_clear_kernel, dim3(256), dim3(kThreads), 0, s, a);
+            hipLaunchKernelGGL(line_direct_kernel, dim3(1), dim3(64), 0, s, a, a0);
+            CHECK_HIP(hipGetLastError());
+            out->direct++;
+            w0 = a0 + 1;
+            continue;
+        }
+        const uint32_t tb_blocks = std::min<uint32_t>(blocks_for(h[kLcTouched]), 4096);
+        hipLaunchKernelGGL(line_commit_kernel, dim3(tb_blocks), dim3(kThreads), 0, s, a, cut);
+        hipLaunchKernelGGL(line_clear_kernel, dim3(tb_blocks), dim3(kThreads), 0, s, a);
         CHECK_HIP(hipGetLastError());
         out->components += h[kLcComps];
         if (cut < wc) out->cuts++;
@@ -800,6 +521,8 @@ int run_lines(msceer_ctx* ctx, const Dims& d, const float* values, const uint8_t
     out->traced = h[kLcTraced];
     out->replayed = h[kLcReplayed];
     out->voxels = h[kLcVoxels];
+    out->fail_tests = h[kLcWhyTest];
+    out->fail_labels = h[kLcWhyLabel];
     return MSCEER_OK;
 }
 

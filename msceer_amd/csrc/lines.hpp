@@ -17,7 +17,10 @@ struct LineStats {
     uint64_t windows = 0;     // speculation windows
     uint64_t components = 0;  // components replayed (lone lines included)
     uint64_t cuts = 0;        // windows cut short by a replay that left its footprint
-    uint64_t direct = 0;      // lines traced alone (footprint beyond its record)
+    uint64_t direct = 0;      // lines traced alone (first of a window and beyond the footprint pool)
+    uint64_t overflows = 0;   // windows cut short where the footprint pool ran out
+    uint64_t fail_tests = 0;  // replays stopped by a test of another component's cell
+    uint64_t fail_labels = 0; // replays stopped by a label their speculation did not give
 };
 
 // The digitized 1-manifolds of one family added onto `restr` (device, one

@@ -107,6 +107,8 @@ class LabelStats(ctypes.Structure):
         ("line_windows", ctypes.c_int32),
         ("line_cuts", ctypes.c_int32),
         ("ms_lines", ctypes.c_float),
+        ("line_alone", ctypes.c_int32),
+        ("line_overflows", ctypes.c_int32),
     ]
 
     def as_dict(self):

@@ -49,3 +49,40 @@ def check_product(f):
     c = np.zeros(2, np.int64)
     lib().emul_check_product(f.ctypes.data_as(ctypes.c_void_p), X, Y, Z, c.ctypes.data_as(ctypes.c_void_p))
     return int(c[0]), int(c[1])
+
+
+# ---- accurate 1-manifolds (msceer_amd/csrc/lines_core.hpp) ------------------
+LINES_SO = os.path.join(HERE, "emul", "liblines_emul.so")
+_lines = None
+
+
+def lines_lib():
+    global _lines
+    if _lines is None:
+        subprocess.run(["make", "-s", "-C", os.path.join(HERE, "emul")], check=True)
+        h = ctypes.CDLL(LINES_SO)
+        P, L = ctypes.c_void_p, ctypes.c_longlong
+        h.emul_lines.argtypes = [P, L, L, L, P, P, P, ctypes.c_int, ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                 L, ctypes.c_int, P]
+        _lines = h
+    return _lines
+
+
+def lines(f, grad, field, asc, window=1 << 20, order=0, err=1e-3, gthr=1e-6, iters=4096):
+    """The GPU's window algorithm for one 1-manifold family, run on the host:
+    (restriction labels, stats[8]: saddles, windows, cuts, lines traced alone,
+    speculative traces, replayed traces, replays stopped on a foreign test /
+    a foreign label).  field: the FP64 gradient (Z, Y, X, 3); order: the
+    order in which a launch's lanes run (0 ascending, 1 descending, 2 random)."""
+    f = np.ascontiguousarray(f, np.float32)
+    Z, Y, X = f.shape
+    fld = np.ascontiguousarray(field, np.float64)
+    g = np.ascontiguousarray(grad, np.uint8)
+    r = np.zeros((2 * X - 1) * (2 * Y - 1) * (2 * Z - 1), np.int8)
+    st = np.zeros(8, np.int64)
+    rc = lines_lib().emul_lines(f.ctypes.data_as(ctypes.c_void_p), X, Y, Z, g.ctypes.data_as(ctypes.c_void_p),
+                                fld.ctypes.data_as(ctypes.c_void_p), r.ctypes.data_as(ctypes.c_void_p), int(asc),
+                                err, gthr, iters, window, order, st.ctypes.data_as(ctypes.c_void_p))
+    if rc:
+        raise RuntimeError(f"emul_lines returned {rc}")
+    return r, st

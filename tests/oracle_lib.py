@@ -198,14 +198,14 @@ def ondemand_asc(f, err=1e-3, gthr=1e-6, iters=4096, presimp=0.0, mode=0):
 
 def ondemand(f, need_asc1, need_dsc1, need_asc2, err=1e-3, gthr=1e-6, iters=4096, presimp=0.0, mode=0):
     """OnDemandAccurate with the needASC1 / needDSC1 / needASC2 flags:
-    (grad, labels (all -1 unless need_asc2), restriction, stats[18]: [0..3]
-    labelling, [4..10] ASC lines, [11..17] DSC lines)."""
+    (grad, labels (all -1 unless need_asc2), restriction, stats[20]: [0..3]
+    labelling, [4..11] ASC lines, [12..19] DSC lines)."""
     f = np.ascontiguousarray(f, np.float32)
     Z, Y, X = f.shape
     g = np.empty(ncells(X, Y, Z), np.uint8)
     lab = np.full(X * Y * Z, -1, np.int32)
     r = np.empty(ncells(X, Y, Z), np.int8)
-    st = np.zeros(18, np.int64)
+    st = np.zeros(20, np.int64)
     _integ_sigs().oracle_ondemand(_p(f), X, Y, Z, err, gthr, iters, presimp, int(need_asc1), int(need_dsc1),
                                   int(need_asc2), mode, _p(g), _p(lab), _p(r), _p(st))
     return g, lab, r, st
@@ -213,12 +213,14 @@ def ondemand(f, need_asc1, need_dsc1, need_asc2, err=1e-3, gthr=1e-6, iters=4096
 
 def saddle_lines(f, grad, asc, restriction=None, seeds=None, err=1e-3, gthr=1e-6, iters=4096, mode=0):
     """The digitized 1-manifolds of one family added onto restriction
-    (a copy): (restriction, stats[7])."""
+    (a copy): (restriction, stats[8]: saddles, streamlines, voxel traversals,
+    cells labelled, stops on labels or the boundary, low-gradient stops,
+    budget stops, microseconds of the saddle loop)."""
     f = np.ascontiguousarray(f, np.float32)
     Z, Y, X = f.shape
     r = np.zeros(ncells(X, Y, Z), np.int8) if restriction is None else np.array(restriction, np.int8)
     sd = None if seeds is None else np.ascontiguousarray(seeds, np.int8)
-    st = np.zeros(7, np.int64)
+    st = np.zeros(8, np.int64)
     _integ_sigs().oracle_saddle_lines(_p(f), X, Y, Z, _p(np.ascontiguousarray(grad, np.uint8)),
                                       _p(sd) if sd is not None else None, err, gthr, iters, int(asc), mode, _p(r),
                                       _p(st))

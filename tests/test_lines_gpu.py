@@ -69,7 +69,7 @@ def test_lines_match_fp64_oracle(ctx, name, flags):
     if flags[0]:
         assert st["lines_asc"] >= ost[5]
     if flags[1]:
-        assert st["lines_dsc"] >= ost[12]
+        assert st["lines_dsc"] >= ost[13]
 
 
 @pytest.mark.parametrize("window", [8, 61])

@@ -32,7 +32,7 @@ def test_lines_label_only_their_cell_types():
     M = (2 * Z - 1, 2 * Y - 1, 2 * X - 1)
     dim = sum(np.indices(M)[a] % 2 for a in range(3)).ravel()
     assert set(np.unique(dim[r == 4])) <= {2, 3} and set(np.unique(dim[r == 5])) <= {0, 1}
-    assert st[4] > 0 and st[11] > 0 and (r == 4).sum() > 0 and (r == 5).sum() > 0
+    assert st[4] > 0 and st[12] > 0 and (r == 4).sum() > 0 and (r == 5).sum() > 0
 
 
 def test_saddles_labelled_and_faces_next_to_their_cells():
