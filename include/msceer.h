@@ -258,6 +258,15 @@ int msceer_clean_labels(msceer_ctx* ctx, const float* values, int64_t nx, int64_
  * int8 restriction label per cell (0 or 1) from X*Y*Z int32 labels. */
 int msceer_restriction_labels(msceer_ctx* ctx, const int32_t* labels, int64_t nx, int64_t ny, int64_t nz,
                               int8_t* restriction_out, uint32_t flags);
+/* VertexLabelingToBoundaryLabeling::ComputeRegionBoundaryKind
+ * (gi_labeling_to_bounary_labeling.h:262-393), the `_edges_` file of
+ * Accurate3DMorseLabeling (accurate3dmorselabeling.cxx:337-343): int8 per
+ * cell from X*Y*Z int32 labels -- edges 1 where their vertices' valid labels
+ * differ; quads 0, 1, or 2 where more than two regions meet; hexes up to 3.
+ * Vertices, and edges with an unlabelled vertex, which the reference leaves
+ * unwritten, are 0. */
+int msceer_region_boundary_kind(msceer_ctx* ctx, const int32_t* labels, int64_t nx, int64_t ny, int64_t nz,
+                                int8_t* kind_out, uint32_t flags);
 /*
  * VertexLabelingToBoundaryLabeling::ComputeMAXBoundary
  * (gi_labeling_to_bounary_labeling.h:118-195) added onto restriction_inout

@@ -135,6 +135,9 @@ hipError_t launch_min_boundary(const VGrid& g, const int32_t* label, int8_t* res
 // such an edge.
 hipError_t launch_max_boundary(const VGrid& g, const float* values, const int32_t* label, int8_t* restriction,
                                hipStream_t s);
+// Region boundary kinds (regions.hip): per cell, 0..3 by how many regions
+// of `label` meet there (ComputeRegionBoundaryKind).
+hipError_t launch_region_kind(const VGrid& g, const int32_t* label, int8_t* out, hipStream_t s);
 // One arc of the extremum graph: a critical 1-cell (saddle), the critical
 // vertices its two V-paths end at, the value of its highest vertex and the
 // values at the two minima.

@@ -611,6 +611,34 @@ extern "C" int msceer_restriction_labels(msceer_ctx* ctx, const int32_t* labels,
     return MSCEER_OK;
 }
 
+// VertexLabelingToBoundaryLabeling::ComputeRegionBoundaryKind
+// (gi_labeling_to_bounary_labeling.h:262-393), as Accurate3DMorseLabeling
+// writes it (accurate3dmorselabeling.cxx:337-343).
+extern "C" int msceer_region_boundary_kind(msceer_ctx* ctx, const int32_t* labels, int64_t nx, int64_t ny, int64_t nz,
+                                           int8_t* kind_out, uint32_t flags) {
+    if (!ctx || !labels || !kind_out) return MSCEER_EINVAL;
+    if (flags & ~MSCEER_DEVICE_PTRS) return MSCEER_EINVAL;
+    int rc = check_dims(nx, ny, nz);
+    if (rc) return rc;
+    CHECK_HIP(hipSetDevice(ctx->device));
+    const bool dev = flags & MSCEER_DEVICE_PTRS;
+    const Dims d = make_dims(nx, ny, nz);
+    const VGrid g = make_vgrid(nx, ny, nz);
+    hipStream_t s = ctx->stream;
+    const int32_t* d_labels = nullptr;
+    rc = stage_in(ctx, ctx->label, labels, (size_t)d.verts, dev, &d_labels);
+    if (rc) return rc;
+    int8_t* d_k = kind_out;
+    if (!dev) {
+        CHECK_HIP(ctx->restriction.ensure(d.cells));
+        d_k = ctx->restriction.as<int8_t>();
+    }
+    CHECK_HIP(launch_region_kind(g, d_labels, d_k, s));
+    if (!dev) CHECK_HIP(hipMemcpyAsync(kind_out, d_k, d.cells, hipMemcpyDeviceToHost, s));
+    CHECK_HIP(hipStreamSynchronize(s));
+    return MSCEER_OK;
+}
+
 // VertexLabelingToBoundaryLabeling::ComputeMAXBoundary (gi_labeling_to_bounary_labeling.h:118-195)
 // added onto an existing restriction labelling (in/out), as ondemandaccurate
 // does after the MIN boundary of the ascending labels (ondemandaccurate.cxx:988).

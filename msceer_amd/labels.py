@@ -161,6 +161,23 @@ def restriction_labels(labels: np.ndarray, ctx: native.Context | None = None) ->
     return out.reshape(2 * Z - 1, 2 * Y - 1, 2 * X - 1)
 
 
+def region_boundary_kind(labels: np.ndarray, ctx: native.Context | None = None) -> np.ndarray:
+    """ComputeRegionBoundaryKind (Accurate3DMorseLabeling's `_edges_` file):
+    int8 per cell, shape (2Z-1, 2Y-1, 2X-1)."""
+    lab = np.ascontiguousarray(labels, np.int32)
+    Z, Y, X = lab.shape
+    out = np.empty(num_cells(X, Y, Z), np.int8)
+    own = ctx is None
+    ctx = ctx or native.Context()
+    try:
+        native.check(native.lib().msceer_region_boundary_kind(ctx.handle, _ptr(lab), X, Y, Z, _ptr(out), 0),
+                     "msceer_region_boundary_kind")
+    finally:
+        if own:
+            ctx.close()
+    return out.reshape(2 * Z - 1, 2 * Y - 1, 2 * X - 1)
+
+
 def restriction_add_max(values: np.ndarray, labels: np.ndarray, restriction: np.ndarray,
                         ctx: native.Context | None = None) -> np.ndarray:
     """ComputeMAXBoundary of `labels` added onto a copy of `restriction` (int8 per cell)."""

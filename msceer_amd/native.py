@@ -51,6 +51,7 @@ SIGNATURES = {
                                                _I32, _P, _P, _U32]),
     "msceer_clean_labels": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _U32]),
     "msceer_restriction_labels": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _U32]),
+    "msceer_region_boundary_kind": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _U32]),
     "msceer_restriction_add_max": (ctypes.c_int, [_P, _P, _P, _I64, _I64, _I64, _P, _U32]),
     "msceer_constrained_lstar": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _P, _U32]),
     "msceer_batch": (ctypes.c_int, [_P, _P, _I32, _I32, ctypes.c_float, ctypes.c_float, _I32, ctypes.c_float, _U32]),

@@ -21,6 +21,7 @@
 //
 // Labels are canonical: a region is named by the vertex id of its lowest
 // vertex (its representative minimum), -1 = unassigned.
+#include <chrono>
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -742,8 +743,9 @@ Event trace_line(const Grid& g, const CellMesh& m, const std::vector<V3d>& grad,
 }
 
 // ReIntegrateUpFrom2Saddles / ReIntegrateDownFrom1Saddles with one thread.
-// stats (7): saddles, streamlines, voxel traversals, cells labelled,
-//            stops on labelled cells or the boundary, low-gradient stops, budget stops
+// stats (8): saddles, streamlines, voxel traversals, cells labelled,
+//            stops on labelled cells or the boundary, low-gradient stops, budget stops,
+//            microseconds of the saddle gathering, sorting and streamlines
 template <bool kFp64>
 void saddle_lines(const Grid& g, const CellMesh& m, const float* f, const unsigned char* grad,
                   const std::vector<V3d>& field, float ethr, float gthr, int max_iters, bool asc, signed char* lab,
@@ -752,13 +754,14 @@ void saddle_lines(const Grid& g, const CellMesh& m, const float* f, const unsign
     std::vector<unsigned char> maxv((size_t)C), minv((size_t)C);
     oracle_maxmin(f, g.X, g.Y, g.Z, maxv.data(), minv.data());
     const CellFunc cf{&m, f, maxv.data()};
+    const auto t0 = std::chrono::steady_clock::now();
     const int sdim = asc ? 2 : 1, target = asc ? 4 : 5;
     std::vector<std::pair<float, idx_t>> sad;
     for (idx_t c = 0; c < C; c++)
         if (m.dim(c) == sdim && m.critical(grad, c)) sad.push_back({asc ? cf.value(c) : -1 * cf.value(c), c});
     std::sort(sad.begin(), sad.end());
     const LineChecker chk{lab, &m, target, asc};
-    long long st[7] = {(long long)sad.size(), 0, 0, 0, 0, 0, 0};
+    long long st[8] = {(long long)sad.size(), 0, 0, 0, 0, 0, 0, 0};
     for (idx_t k = (idx_t)sad.size() - 1; k >= 0; k--) {
         const idx_t s = sad[(size_t)k].second;
         // a few V-path steps on either side of the saddle
@@ -800,7 +803,9 @@ void saddle_lines(const Grid& g, const CellMesh& m, const float* f, const unsign
         }
         lab[s] = (signed char)target;
     }
-    if (stats) std::copy(st, st + 7, stats);
+    st[7] = (long long)std::chrono::duration_cast<std::chrono::microseconds>(std::chrono::steady_clock::now() - t0)
+                .count();
+    if (stats) std::copy(st, st + 8, stats);
 }
 
 }  // namespace
@@ -932,6 +937,56 @@ int oracle_max_boundary(const float* f, const int* labels, long long X, long lon
     return 0;
 }
 
+// VertexLabelingToBoundaryLabeling::ComputeRegionBoundaryKind
+// (gi_labeling_to_bounary_labeling.h:262-393): edges, then quads, then hexes,
+// each from the previous dimension's kinds; cells the reference leaves
+// unwritten (vertices, edges with an unlabelled vertex) are 0.
+int oracle_region_boundary_kind(const int* labels, long long X, long long Y, long long Z, signed char* out) {
+    CellMesh m(X, Y, Z);
+    const idx_t C = m.M[0] * m.M[1] * m.M[2];
+    std::fill(out, out + C, (signed char)0);
+    auto verts = [&](idx_t c, std::vector<int>& l) {
+        l.clear();
+        idx_t p[3];
+        m.coords(c, p);
+        for (int k = 0; k < 8; k++) {
+            if (((k & 1) && !(p[0] & 1)) || ((k & 2) && !(p[1] & 1)) || ((k & 4) && !(p[2] & 1))) continue;
+            const idx_t v = (p[0] / 2 + (k & 1)) + X * ((p[1] / 2 + ((k >> 1) & 1)) + Y * (p[2] / 2 + (k >> 2)));
+            l.push_back(labels[v]);
+        }
+    };
+    auto unique_beyond = [](const std::vector<int>& l, int limit) {
+        std::vector<int> u;
+        for (int x : l) {
+            if (std::find(u.begin(), u.end(), x) == u.end()) u.push_back(x);
+            if ((int)u.size() > limit) break;
+        }
+        return (int)u.size() > limit;
+    };
+    std::vector<int> l;
+    for (int d = 1; d <= 3; d++)
+        for (idx_t c = 0; c < C; c++) {
+            if (m.dim(c) != d) continue;
+            verts(c, l);
+            if (d == 1) {
+                if (l[0] == -1 || l[1] == -1) continue;
+                out[c] = l[0] != l[1] ? 1 : 0;
+                continue;
+            }
+            idx_t fc[6];
+            const int k = m.facets(c, fc);
+            int sum = 0, maxv = 0;
+            for (int j = 0; j < k; j++) {
+                const int v = out[fc[j]];
+                if (d == 2) sum += v == 1;
+                else { maxv = std::max(maxv, v); sum += v == 2; }
+            }
+            if (d == 2) out[c] = (signed char)(sum == 0 ? 0 : (sum <= 2 ? 1 : (unique_beyond(l, 2) ? 2 : 1)));
+            else out[c] = (signed char)(sum > 2 ? (unique_beyond(l, 3) ? 3 : 2) : maxv);
+        }
+    return 0;
+}
+
 int oracle_min_boundary(const int* labels, long long X, long long Y, long long Z, signed char* restriction) {
     CellMesh m(X, Y, Z);
     const idx_t C = m.M[0] * m.M[1] * m.M[2];
@@ -1000,7 +1055,7 @@ int oracle_ondemand_asc(const float* f, long long X, long long Y, long long Z, f
 // seeds (may be null): vertices whose gradient is zeroed first, as the
 // accurate 3-manifold labelling leaves the shared gradient field
 // (gi_numeric_integrator_path_compressing.h:118-123).  mode 0: the
-// reference's mixed arithmetic, else FP64.  stats: 7 entries (saddle_lines).
+// reference's mixed arithmetic, else FP64.  stats: 8 entries (saddle_lines).
 int oracle_saddle_lines(const float* f, long long X, long long Y, long long Z, const unsigned char* grad,
                         const signed char* seeds, float ethr, float gthr, int max_iters, int asc, int mode,
                         signed char* restriction, long long* stats) {
@@ -1020,7 +1075,7 @@ int oracle_saddle_lines(const float* f, long long X, long long Y, long long Z, c
 // accurate ascending labelling and its MIN boundary when need_asc2; the
 // digitized 1-manifolds (ASC then DSC); the conforming lower stars; ldir and
 // flags.  labels (V) are written only when need_asc2 (else left as they are);
-// stats: [0..3] the labelling's, [4..10] ASC lines, [11..17] DSC lines.
+// stats: [0..3] the labelling's, [4..11] ASC lines, [12..19] DSC lines.
 int oracle_ondemand(const float* f, long long X, long long Y, long long Z, float ethr, float gthr, int max_iters,
                     float presimp, int need_asc1, int need_dsc1, int need_asc2, int mode, unsigned char* grad,
                     int* labels, signed char* restriction_out, long long* stats) {
@@ -1030,7 +1085,7 @@ int oracle_ondemand(const float* f, long long X, long long Y, long long Z, float
     std::vector<unsigned char> base((size_t)C);
     oracle_lstar_pairs(f, X, Y, Z, nullptr, base.data());
     std::vector<signed char> r((size_t)C, 0), seeds;
-    long long st[18] = {0};
+    long long st[20] = {0};
     if (need_asc2) {
         seeds.assign((size_t)V, 0);
         oracle_certain_regions(f, X, Y, Z, base.data(), presimp, labels, seeds.data());
@@ -1041,7 +1096,7 @@ int oracle_ondemand(const float* f, long long X, long long Y, long long Z, float
     const int lmode = mode == 0 ? 0 : 1;
     const signed char* zs = need_asc2 ? seeds.data() : nullptr;
     if (need_asc1) oracle_saddle_lines(f, X, Y, Z, base.data(), zs, ethr, gthr, max_iters, 1, lmode, r.data(), st + 4);
-    if (need_dsc1) oracle_saddle_lines(f, X, Y, Z, base.data(), zs, ethr, gthr, max_iters, 0, lmode, r.data(), st + 11);
+    if (need_dsc1) oracle_saddle_lines(f, X, Y, Z, base.data(), zs, ethr, gthr, max_iters, 0, lmode, r.data(), st + 12);
     if (need_asc2 || need_asc1 || need_dsc1) {
         const long long nv = oracle_conforming_vertices(f, X, Y, Z, r.data(), nullptr);
         std::vector<long long> vs((size_t)nv);
@@ -1053,7 +1108,7 @@ int oracle_ondemand(const float* f, long long X, long long Y, long long Z, float
     oracle_flags(base.data(), X, Y, Z);
     std::copy(base.begin(), base.end(), grad);
     if (restriction_out) std::copy(r.begin(), r.end(), restriction_out);
-    if (stats) std::copy(st, st + 18, stats);
+    if (stats) std::copy(st, st + 20, stats);
     return 0;
 }
 

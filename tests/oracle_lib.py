@@ -101,6 +101,7 @@ def _integ_sigs():
     h.oracle_conforming_vertices.argtypes = [_P, _L, _L, _L, _P, _P]
     h.oracle_conforming_vertices.restype = ctypes.c_longlong
     h.oracle_ondemand_asc.argtypes = [_P, _L, _L, _L, _F, _F, _I, _F, _I, _P, _P, _P, _P]
+    h.oracle_region_boundary_kind.argtypes = [_P, _L, _L, _L, _P]
     h.oracle_saddle_lines.argtypes = [_P, _L, _L, _L, _P, _P, _F, _F, _I, _I, _I, _P, _P]
     h.oracle_ondemand.argtypes = [_P, _L, _L, _L, _F, _F, _I, _F, _I, _I, _I, _I, _P, _P, _P, _P]
     h._integ_ready = True
@@ -209,6 +210,15 @@ def ondemand(f, need_asc1, need_dsc1, need_asc2, err=1e-3, gthr=1e-6, iters=4096
     _integ_sigs().oracle_ondemand(_p(f), X, Y, Z, err, gthr, iters, presimp, int(need_asc1), int(need_dsc1),
                                   int(need_asc2), mode, _p(g), _p(lab), _p(r), _p(st))
     return g, lab, r, st
+
+
+def region_boundary_kind(labels, shape_zyx):
+    """ComputeRegionBoundaryKind of X*Y*Z int32 labels: int8 per cell."""
+    Z, Y, X = shape_zyx
+    lab = np.ascontiguousarray(labels, np.int32).ravel()
+    out = np.empty(ncells(X, Y, Z), np.int8)
+    _integ_sigs().oracle_region_boundary_kind(_p(lab), X, Y, Z, _p(out))
+    return out
 
 
 def saddle_lines(f, grad, asc, restriction=None, seeds=None, err=1e-3, gthr=1e-6, iters=4096, mode=0):
