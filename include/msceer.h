@@ -249,6 +249,41 @@ typedef struct msceer_volume {
 int msceer_batch(msceer_ctx* ctx, msceer_volume* vols, int32_t n, int32_t streams, float err_thr, float grad_thr,
                  int32_t max_iters, float presimp, uint32_t flags);
 
+/*
+ * Batched lower-star .grad (no counterpart in the reference, which processes
+ * one volume per program run): for every volume the .grad of
+ * msceer_lstar_grad (labels_out must be NULL), bit for bit.  Consecutive
+ * volumes form groups of fewer than 2^32 - 1 cells in all and, if
+ * max_group_verts > 0, at most that many vertices (a single larger volume is
+ * a group of its own).  Each group runs as one launch per stage for all its
+ * volumes (lower star, ldir records, the two ldir passes' frontier rounds on
+ * two streams, finish), captured once in a hipGraph and replayed on later
+ * calls with the same shapes (and, with device pointers, the same
+ * addresses); the lower-star launch (brick shape; a work-group per brick, or
+ * persistent work-groups claiming bricks from per-XCD queues in chunks) is
+ * tuned on a group's first call unless msceer_set_autotune fixed the shape.
+ * One host synchronisation per group.  Flags: MSCEER_DEVICE_PTRS (values and
+ * grad_out of every volume in device memory; else host memory, staged
+ * through the context's device buffers), MSCEER_TIMING (per-volume device
+ * clock: ms = from the first brick of the volume's group to the volume's
+ * last finished tile; without it ms is the group's time).  status: 0 once
+ * the volume's group has run.
+ */
+int msceer_batch_grad(msceer_ctx* ctx, msceer_volume* vols, int32_t n, int64_t max_group_verts, uint32_t flags);
+typedef struct msceer_batch_stats {
+    int32_t groups;        /* groups the last call ran */
+    int32_t replays;       /* graph launches */
+    int32_t captures;      /* graphs captured (first calls, or after a plan changed) */
+    int32_t eager_rounds;  /* frontier rounds and sweeps run after a replay left work */
+    int32_t shape;         /* lower-star brick shape (last group), as msceer_lstar_brick */
+    int32_t blocks;        /* persistent work-groups, 0: a work-group per brick */
+    int32_t chunk;         /* bricks per queue claim */
+    int32_t rounds;        /* frontier rounds per pass in the graph */
+    int64_t critical[4];   /* critical cells by dimension, all volumes */
+    float ms_gpu;          /* device time of the groups (events around each replay and what followed) */
+} msceer_batch_stats;
+int msceer_last_batch_stats(const msceer_ctx* ctx, msceer_batch_stats* out);
+
 /* Stages of the accurate pipeline as separate calls (host or device pointers):
  * IsolatedCCRegionRemoverNEW::ComputeOutput (gi_isolated_region_remover.h:155-192),
  * in place on X*Y*Z int32 labels. */

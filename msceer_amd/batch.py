@@ -1,4 +1,4 @@
-"""Batches of independent volumes (msceer_batch)."""
+"""Batches of independent volumes (msceer_batch, msceer_batch_grad)."""
 from __future__ import annotations
 
 import ctypes
@@ -40,3 +40,36 @@ def run_batch(volumes, labels: bool = False, streams: int = 0, err: float = 1e-3
             ctx.close()
     shaped = [None if l is None else l.reshape(v.shape) for l, v in zip(labs, vols)]
     return grads, shaped, np.array([arr[i].ms for i in range(n)], np.float64)
+
+
+def volume_table(shapes, values_ptrs, grad_ptrs):
+    """A msceer_volume array for .grad-only batches."""
+    n = len(shapes)
+    arr = (native.Volume * n)()
+    for i, ((Z, Y, X), vp, gp) in enumerate(zip(shapes, values_ptrs, grad_ptrs)):
+        arr[i].values = vp
+        arr[i].nx, arr[i].ny, arr[i].nz = X, Y, Z
+        arr[i].grad_out = gp
+        arr[i].labels_out = None
+    return arr
+
+
+def run_batch_grad(volumes, max_group_verts: int = 0, timing: bool = False, ctx: native.Context | None = None):
+    """Lower-star .grad of each (Z,Y,X) float32 volume through msceer_batch_grad
+    (host buffers).  Returns (grads, ms, stats): the .grad arrays in input
+    order, per-volume milliseconds (device clock with `timing`, else the
+    group's time) and the call's native.BatchStats."""
+    vols = [np.ascontiguousarray(v, np.float32) for v in volumes]
+    grads = [np.empty(num_cells(v.shape[2], v.shape[1], v.shape[0]), np.uint8) for v in vols]
+    arr = volume_table([v.shape for v in vols], [v.ctypes.data for v in vols], [g.ctypes.data for g in grads])
+    own = ctx is None
+    ctx = ctx or native.Context()
+    st = native.BatchStats()
+    try:
+        native.check(native.lib().msceer_batch_grad(ctx.handle, arr, len(vols), max_group_verts,
+                                                    native.TIMING if timing else 0), "msceer_batch_grad")
+        native.check(native.lib().msceer_last_batch_stats(ctx.handle, ctypes.byref(st)), "msceer_last_batch_stats")
+    finally:
+        if own:
+            ctx.close()
+    return grads, np.array([arr[i].ms for i in range(len(vols))], np.float64), st

@@ -310,6 +310,7 @@ int msceer_destroy(msceer_ctx* ctx) {
     ctx->workers.clear();
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    batch_state_free(ctx);
     ctx->values.release();
     ctx->grad.release();
     ctx->lists.release();

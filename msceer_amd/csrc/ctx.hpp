@@ -16,6 +16,10 @@
 
 namespace msceer {
 
+struct BatchState;
+// Releases the batched .grad state of a context (graphs, buffers, streams).
+void batch_state_free(msceer_ctx* ctx);
+
 // Grow-only device buffer.  A borrowed buffer is a view of memory the
 // buffer does not own (a slice of a static scratch array): it is never freed
 // or grown, and asking it for more than it holds fails.
@@ -132,6 +136,7 @@ struct msceer_ctx {
     msceer_label_stats last_labels{};
     std::shared_ptr<msceer::Tuning> tuning = std::make_shared<msceer::Tuning>();
     std::vector<msceer_ctx*> workers;  // contexts of msceer_batch's worker threads
+    msceer::BatchState* gbatch = nullptr;  // msceer_batch_grad: buffers, plans and graphs
 };
 
 namespace msceer {

@@ -41,6 +41,18 @@ struct LstarBuffers {
 constexpr int kLstarBricks = 4;
 const char* lstar_brick_name(int shape);
 hipError_t launch_lstar(const Dims& d, const LstarBuffers& b, hipStream_t s, int shape = 0);
+void lstar_brick_dims(int shape, int* bx, int* by, int* bz);
+
+// Batched launches over a group of volumes (gbatch.hpp): the unit tables,
+// queues and volume table in g.
+struct GBatch;
+// Lower-star bricks of every volume of the group (list entries: each volume's
+// cells from its first frontier item on; restriction unused): a work-group
+// per brick (blocks == 0) or `blocks` persistent work-groups claiming bricks
+// (queue cleared beforehand).
+hipError_t launch_lstar_batch(const GBatch& g, const LstarBuffers& b, uint32_t blocks, hipStream_t s, int shape);
+// Persistent work-groups of the batched lower-star kernel one CU holds at once.
+int lstar_batch_occupancy(int shape);
 
 // ldir (ascending-manifold dimension) passes, see ldir.hip.  Scratch: edge
 // and quad walk records and the two visited bitmaps, at these byte offsets of
@@ -60,10 +72,21 @@ hipError_t launch_ldir_records(const Dims& d, const uint8_t* grad, void* scratch
 hipError_t launch_ldir_round(const Dims& d, void* scratch, const LdirScratch& s, int pass, bool start, const MList& in,
                              const MList& out, uint32_t* clear_counts, uint32_t* overflow, hipStream_t st);
 // Rebuild a frontier from the bitmaps and records alone (after an overflow).
+// (The frontier entries are the cell ids plus item0.)
 hipError_t launch_ldir_sweep(const Dims& d, const uint8_t* grad, void* scratch, const LdirScratch& s, int pass,
-                             const MList& out, uint32_t* overflow, hipStream_t st);
+                             const MList& out, uint32_t* overflow, hipStream_t st, uint32_t item0 = 0);
 // Every cell's ldir from what the two passes reached.
 hipError_t launch_ldir_finish(const Dims& d, uint8_t* grad, void* scratch, const LdirScratch& s, hipStream_t st);
+// Tiles of the ldir streaming passes of a volume (along x, y, in all).
+void ldir_tiles(const Dims& d, uint32_t* ntx, uint32_t* nty, uint32_t* tiles);
+// The same passes over a group of volumes (one launch each; records and
+// bitmaps of every volume in its table entry): records, clearing bit_words
+// words of the group's bitmaps from `bits`; one round of pass 1 or 2 over
+// frontier items of the group; the ldir bits of every volume.
+hipError_t launch_ldir_records_group(const GBatch& b, uint32_t* bits, uint64_t bit_words, hipStream_t st);
+hipError_t launch_ldir_round_group(const GBatch& b, int pass, bool start, const MList& in, const MList& out,
+                                   uint32_t* clear_counts, uint32_t* overflow, hipStream_t st);
+hipError_t launch_ldir_finish_group(const GBatch& b, hipStream_t st);
 // Zero n words.
 hipError_t launch_zero(uint32_t* p, uint32_t n, hipStream_t s);
 

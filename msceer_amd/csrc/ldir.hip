@@ -42,6 +42,7 @@
 #include <hip/hip_runtime.h>
 
 #include "internal.hpp"
+#include "gbatch.hpp"
 #include "mlist.hpp"
 
 namespace msceer {
@@ -286,15 +287,60 @@ using FinBox = CellBox<2 * kTX + 1, 2 * kTY + 1, 2 * kTZ + 1>;  // cells 2 x0 ..
 // cofacet quad along the edge's m-th even axis; rec2[3v + a] (quad with even
 // axis a): slot (sgn < 0) is the cofacet hex along a.  The cofacets lie
 // within one cell of the vertex's cell, all inside the tile's staged box.
-__global__ void __launch_bounds__(kThreads)
-ldir_records_kernel(const uint8_t* __restrict__ grad, LGeom g, uint64_t cells, uint32_t align, int ntx, int nty,
-                    uint16_t* __restrict__ rec1, uint8_t* __restrict__ rec2, uint32_t* __restrict__ bits,
-                    uint32_t nbits_words) {
+// Arguments of the streaming passes: one volume, or (kGroup) the tiles of
+// every volume of a group, a work-group per tile.
+struct TileArgs {
+    uint8_t* grad;
+    LGeom g;
+    uint64_t cells;
+    int ntx, nty;
+    uint16_t* rec1;
+    uint8_t* rec2;
+    const uint32_t* v1;
+    const uint32_t* v2;
+    uint32_t* bits;      // records: bitmaps to clear (both, or every volume's of a group)
+    uint32_t bit_words;  // records: words to clear; finish: words of one bitmap
+    GBatch batch;
+};
+
+// The volume of the work-group's tile: from the arguments, or from the
+// group's volume table.
+template <bool kGroup>
+__device__ __forceinline__ int tile_of(TileArgs& a, uint32_t& vol) {
+    vol = 0;
+    if (!kGroup) return (int)blockIdx.x;
+    const uint32_t unit = blockIdx.x;
+    vol = unit_volume(a.batch.tiles, unit);
+    const GVol& gv = a.batch.vols[vol];
+    a.grad = gv.grad;
+    a.g = LGeom{gv.Mx, gv.My, gv.Mz, gv.Mx * gv.My, (uint32_t)gv.X, (uint32_t)gv.Y,
+                (uint32_t)gv.X * (uint32_t)gv.Y * (uint32_t)gv.Z, ((uint32_t)gv.X + 3u) / 4u, ((uint32_t)gv.Y + 3u) / 4u};
+    a.cells = gv.cells;
+    a.ntx = (int)gv.ntx;
+    a.nty = (int)gv.nty;
+    a.rec1 = gv.rec1;
+    a.rec2 = gv.rec2;
+    a.v1 = gv.bits1;
+    a.v2 = gv.bits2;
+    a.bit_words = gv.bit_words;
+    return (int)(unit - a.batch.tiles.first[vol]);
+}
+
+template <bool kGroup>
+__global__ void __launch_bounds__(kThreads) ldir_records_kernel(TileArgs a) {
     constexpr int kRY = 2 * kTY + 1;
     __shared__ uint32_t stage[RecBox::kRows * RecBox::kDw];
     __shared__ BoxRow rows[RecBox::kRows];
-    for (uint32_t w = blockIdx.x * kThreads + threadIdx.x; w < nbits_words; w += gridDim.x * kThreads) bits[w] = 0u;
-    const int tile = (int)blockIdx.x;
+    for (uint32_t w = blockIdx.x * kThreads + threadIdx.x; w < a.bit_words; w += gridDim.x * kThreads) a.bits[w] = 0u;
+    uint32_t vol;
+    const int tile = tile_of<kGroup>(a, vol);
+    const uint8_t* __restrict__ grad = a.grad;
+    const LGeom g = a.g;
+    const uint64_t cells = a.cells;
+    const uint32_t align = (uint32_t)((uintptr_t)grad & 3u);
+    const int ntx = a.ntx, nty = a.nty;
+    uint16_t* __restrict__ rec1 = a.rec1;
+    uint8_t* __restrict__ rec2 = a.rec2;
     const int x0 = (tile % ntx) * kTX, y0 = ((tile / ntx) % nty) * kTY, z0 = (tile / (ntx * nty)) * kTZ;
     box_rows<2 * kTX + 1, kRY, 2 * kTZ + 1>(g, align, 2 * x0 - 1, 2 * y0 - 1, 2 * z0 - 1, rows);
     __syncthreads();
@@ -429,8 +475,37 @@ __device__ __forceinline__ void expand(const LGeom& g, const Walk& w, uint32_t i
 // frontier, which the next round shares out again.  Counts are read on the
 // device, so rounds are enqueued back to back; `clear` (the counts the next
 // round appends to) is zeroed here.
+// Where the items of a round live: one volume (items are its cell ids), or
+// a group of volumes (items are cell ids from each volume's first item on).
+struct OneSpace {
+    LGeom g;
+    Walk w;
+    __device__ __forceinline__ void expand_item(uint32_t item, bool start, uint32_t out[4]) const {
+        expand(g, w, item, start, out);
+    }
+};
+
+struct GroupSpace {
+    GBatch b;
+    int pass;
+    __device__ __forceinline__ void expand_item(uint32_t item, bool start, uint32_t out[4]) const {
+        const uint32_t vol = unit_volume(b.items, item);
+        const GVol& gv = b.vols[vol];
+        const LGeom g{gv.Mx, gv.My, gv.Mz, gv.Mx * gv.My, (uint32_t)gv.X, (uint32_t)gv.Y,
+                      (uint32_t)gv.X * (uint32_t)gv.Y * (uint32_t)gv.Z, ((uint32_t)gv.X + 3u) / 4u,
+                      ((uint32_t)gv.Y + 3u) / 4u};
+        const Walk w{gv.rec1, gv.rec2, pass == 1 ? gv.bits1 : gv.bits2, pass};
+        const uint32_t base = b.items.first[vol];
+        expand(g, w, item - base, start, out);
+#pragma unroll
+        for (int s = 0; s < 4; s++)
+            if (out[s] != kNoItem) out[s] += base;
+    }
+};
+
+template <class Space>
 __global__ void __launch_bounds__(kThreads)
-ldir_round_kernel(LGeom g, Walk w, MList in, MList out, uint32_t* __restrict__ clear, uint32_t* __restrict__ overflow,
+ldir_round_kernel(Space sp, MList in, MList out, uint32_t* __restrict__ clear, uint32_t* __restrict__ overflow,
                   int start) {
     __shared__ uint32_t pool[kPool];
     __shared__ uint32_t prefix[kSubs + 2];
@@ -495,7 +570,7 @@ ldir_round_kernel(LGeom g, Walk w, MList in, MList out, uint32_t* __restrict__ c
 #pragma unroll
                 for (int j = 0; j < 4; j++) nb[k][j] = kNoItem;
             } else {
-                expand(g, w, items[k], from_share[k], nb[k]);
+                sp.expand_item(items[k], from_share[k], nb[k]);
             }
         }
         // one pool reservation per wave for everything its lanes reached
@@ -552,7 +627,8 @@ ldir_round_kernel(LGeom g, Walk w, MList in, MList out, uint32_t* __restrict__ c
 // unreached critical cells of the pass (claimed here), and every unreached
 // cell one step from a reached one (claimed here).
 __global__ void __launch_bounds__(kThreads)
-ldir_sweep_kernel(const uint8_t* __restrict__ grad, LGeom g, Walk w, MList out, uint32_t* __restrict__ overflow) {
+ldir_sweep_kernel(const uint8_t* __restrict__ grad, LGeom g, Walk w, MList out, uint32_t* __restrict__ overflow,
+                  uint32_t item0) {
     __shared__ uint32_t stage[kStage];
     __shared__ uint32_t s_n;
     __shared__ MSlot s_base;
@@ -561,8 +637,8 @@ ldir_sweep_kernel(const uint8_t* __restrict__ grad, LGeom g, Walk w, MList out, 
     const int sub = (int)(blockIdx.x % kSubs);
     auto push = [&](uint32_t item) {
         const uint32_t slot = atomicAdd(&s_n, 1u);
-        if (slot < (uint32_t)kStage) stage[slot] = item;
-        else if (!mlist_push(out, sub, item)) atomicOr(overflow, 1u);
+        if (slot < (uint32_t)kStage) stage[slot] = item0 + item;
+        else if (!mlist_push(out, sub, item0 + item)) atomicOr(overflow, 1u);
     };
     for (uint32_t v = blockIdx.x * kThreads + threadIdx.x; v < g.V; v += gridDim.x * kThreads) {
         const uint32_t q = v / g.X, k = q / g.Y;
@@ -632,9 +708,8 @@ __device__ __forceinline__ uint8_t with_ldir(uint32_t b, uint32_t l) { return (u
 // layer; the marks of a quad (marked1, and with which hex it is paired) are
 // computed where needed.  The cells of the next tiles that are read here are
 // only read for their pair and assigned bits, which no tile changes.
-__global__ void __launch_bounds__(kThreads)
-ldir_finish_kernel(uint8_t* __restrict__ grad, LGeom g, uint64_t cells, uint32_t align, int ntx, int nty,
-                   const uint32_t* __restrict__ v1, const uint32_t* __restrict__ v2, uint32_t nwords) {
+template <bool kGroup>
+__global__ void __launch_bounds__(kThreads) ldir_finish_kernel(TileArgs a) {
     constexpr int kRX = 2 * kTX + 1, kRY = 2 * kTY + 1, kRZ = 2 * kTZ + 1;
     // vertex blocks of the tile and its "+" halo layer
     constexpr int kBX = kTX / 4 + 1, kBY = kTY / 4 + 1, kBZ = kTZ / 4 + 1, kBW = kBX * kBY * kBZ * 6 + 1;
@@ -644,7 +719,16 @@ ldir_finish_kernel(uint8_t* __restrict__ grad, LGeom g, uint64_t cells, uint32_t
     constexpr int kVX = kTX + 1, kVY = kTY + 1, kVZ = kTZ + 1;
     __shared__ uint32_t b1[kBW], b2[kBW];
     __shared__ uint8_t vbits[kVX * kVY * kVZ];
-    const int tile = (int)blockIdx.x;
+    uint32_t vol;
+    const int tile = tile_of<kGroup>(a, vol);
+    uint8_t* __restrict__ grad = a.grad;
+    const LGeom g = a.g;
+    const uint64_t cells = a.cells;
+    const uint32_t align = (uint32_t)((uintptr_t)grad & 3u);
+    const int ntx = a.ntx, nty = a.nty;
+    const uint32_t* __restrict__ v1 = a.v1;
+    const uint32_t* __restrict__ v2 = a.v2;
+    const uint32_t nwords = a.bit_words;
     const int x0 = (tile % ntx) * kTX, y0 = ((tile / ntx) % nty) * kTY, z0 = (tile / (ntx * nty)) * kTZ;
     const uint32_t bx0 = (uint32_t)x0 >> 2, by0 = (uint32_t)y0 >> 2, bz0 = (uint32_t)z0 >> 2;
     box_rows<kRX, kRY, kRZ, 2 * kTX>(g, align, 2 * x0, 2 * y0, 2 * z0, rows);
@@ -744,6 +828,10 @@ ldir_finish_kernel(uint8_t* __restrict__ grad, LGeom g, uint64_t cells, uint32_t
                 if (first + t >= (int64_t)lo && first + t < (int64_t)hi) grad[first + t] = (uint8_t)(w >> (8 * t));
         }
     }
+    if (kGroup && a.batch.times) {
+        __syncthreads();
+        if (threadIdx.x == 0) atomicMax(&a.batch.times[2 * vol + 1], batch_clock());
+    }
 }
 
 __global__ void zero_kernel(uint32_t* p, uint32_t n) {
@@ -795,15 +883,21 @@ static void tiles_of(const Dims& d, int& ntx, int& nty, uint32_t& n) {
 hipError_t launch_ldir_records(const Dims& d, const uint8_t* grad, void* scratch, const LdirScratch& s,
                                hipStream_t st) {
     uint8_t* base = static_cast<uint8_t*>(scratch);
-    uint32_t* bits1 = reinterpret_cast<uint32_t*>(base + s.bits1_off);
-    // visited1 and visited2 are adjacent: one clear covers both
-    const uint32_t words = (uint32_t)((s.bits2_off - s.bits1_off) / 4) + s.bit_words;
     int ntx, nty;
     uint32_t n;
     tiles_of(d, ntx, nty, n);
-    const uint32_t align = (uint32_t)((uintptr_t)grad & 3u);
-    hipLaunchKernelGGL(ldir_records_kernel, dim3(n), dim3(kThreads), 0, st, grad, make_geom(d), d.cells, align, ntx,
-                       nty, reinterpret_cast<uint16_t*>(base + s.rec1_off), base + s.rec2_off, bits1, words);
+    TileArgs a{};
+    a.grad = const_cast<uint8_t*>(grad);  // read only here
+    a.g = make_geom(d);
+    a.cells = d.cells;
+    a.ntx = ntx;
+    a.nty = nty;
+    a.rec1 = reinterpret_cast<uint16_t*>(base + s.rec1_off);
+    a.rec2 = base + s.rec2_off;
+    // visited1 and visited2 are adjacent: one clear covers both
+    a.bits = reinterpret_cast<uint32_t*>(base + s.bits1_off);
+    a.bit_words = (uint32_t)((s.bits2_off - s.bits1_off) / 4) + s.bit_words;
+    hipLaunchKernelGGL(ldir_records_kernel<false>, dim3(n), dim3(kThreads), 0, st, a);
     return hipGetLastError();
 }
 
@@ -815,28 +909,64 @@ static Walk make_walk(void* scratch, const LdirScratch& s, int pass) {
 
 hipError_t launch_ldir_round(const Dims& d, void* scratch, const LdirScratch& s, int pass, bool start, const MList& in,
                              const MList& out, uint32_t* clear_counts, uint32_t* overflow, hipStream_t st) {
-    hipLaunchKernelGGL(ldir_round_kernel, dim3(kRoundBlocks), dim3(kThreads), 0, st, make_geom(d),
-                       make_walk(scratch, s, pass), in, out, clear_counts, overflow, start ? 1 : 0);
+    hipLaunchKernelGGL(ldir_round_kernel<OneSpace>, dim3(kRoundBlocks), dim3(kThreads), 0, st,
+                       OneSpace{make_geom(d), make_walk(scratch, s, pass)}, in, out, clear_counts, overflow,
+                       start ? 1 : 0);
     return hipGetLastError();
 }
 
 hipError_t launch_ldir_sweep(const Dims& d, const uint8_t* grad, void* scratch, const LdirScratch& s, int pass,
-                             const MList& out, uint32_t* overflow, hipStream_t st) {
+                             const MList& out, uint32_t* overflow, hipStream_t st, uint32_t item0) {
     hipLaunchKernelGGL(ldir_sweep_kernel, dim3(grid_of(d.verts, 8192)), dim3(kThreads), 0, st, grad, make_geom(d),
-                       make_walk(scratch, s, pass), out, overflow);
+                       make_walk(scratch, s, pass), out, overflow, item0);
     return hipGetLastError();
 }
 
 hipError_t launch_ldir_finish(const Dims& d, uint8_t* grad, void* scratch, const LdirScratch& s, hipStream_t st) {
     uint8_t* base = static_cast<uint8_t*>(scratch);
-    const uint32_t* v1 = reinterpret_cast<const uint32_t*>(base + s.bits1_off);
-    const uint32_t* v2 = reinterpret_cast<const uint32_t*>(base + s.bits2_off);
     int ntx, nty;
     uint32_t n;
     tiles_of(d, ntx, nty, n);
-    const uint32_t align = (uint32_t)((uintptr_t)grad & 3u);
-    hipLaunchKernelGGL(ldir_finish_kernel, dim3(n), dim3(kThreads), 0, st, grad, make_geom(d), d.cells, align, ntx,
-                       nty, v1, v2, s.bit_words);
+    TileArgs a{};
+    a.grad = grad;
+    a.g = make_geom(d);
+    a.cells = d.cells;
+    a.ntx = ntx;
+    a.nty = nty;
+    a.v1 = reinterpret_cast<const uint32_t*>(base + s.bits1_off);
+    a.v2 = reinterpret_cast<const uint32_t*>(base + s.bits2_off);
+    a.bit_words = s.bit_words;
+    hipLaunchKernelGGL(ldir_finish_kernel<false>, dim3(n), dim3(kThreads), 0, st, a);
+    return hipGetLastError();
+}
+
+void ldir_tiles(const Dims& d, uint32_t* ntx, uint32_t* nty, uint32_t* tiles) {
+    int x, y;
+    tiles_of(d, x, y, *tiles);
+    *ntx = (uint32_t)x;
+    *nty = (uint32_t)y;
+}
+
+hipError_t launch_ldir_records_group(const GBatch& b, uint32_t* bits, uint64_t bit_words, hipStream_t st) {
+    TileArgs a{};
+    a.batch = b;
+    a.bits = bits;
+    a.bit_words = (uint32_t)bit_words;
+    hipLaunchKernelGGL(ldir_records_kernel<true>, dim3(b.tiles.total), dim3(kThreads), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_ldir_round_group(const GBatch& b, int pass, bool start, const MList& in, const MList& out,
+                                   uint32_t* clear_counts, uint32_t* overflow, hipStream_t st) {
+    hipLaunchKernelGGL(ldir_round_kernel<GroupSpace>, dim3(kRoundBlocks), dim3(kThreads), 0, st, GroupSpace{b, pass},
+                       in, out, clear_counts, overflow, start ? 1 : 0);
+    return hipGetLastError();
+}
+
+hipError_t launch_ldir_finish_group(const GBatch& b, hipStream_t st) {
+    TileArgs a{};
+    a.batch = b;
+    hipLaunchKernelGGL(ldir_finish_kernel<true>, dim3(b.tiles.total), dim3(kThreads), 0, st, a);
     return hipGetLastError();
 }
 

@@ -55,6 +55,8 @@ SIGNATURES = {
     "msceer_restriction_add_max": (ctypes.c_int, [_P, _P, _P, _I64, _I64, _I64, _P, _U32]),
     "msceer_constrained_lstar": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _P, _U32]),
     "msceer_batch": (ctypes.c_int, [_P, _P, _I32, _I32, ctypes.c_float, ctypes.c_float, _I32, ctypes.c_float, _U32]),
+    "msceer_batch_grad": (ctypes.c_int, [_P, _P, _I32, _I64, _U32]),
+    "msceer_last_batch_stats": (ctypes.c_int, [_P, _P]),
     "msceer_cnn_num_params": (ctypes.c_int, []),
     "msceer_cnn_create": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(_P)]),
     "msceer_cnn_destroy": (ctypes.c_int, [_P]),
@@ -126,6 +128,21 @@ class Volume(ctypes.Structure):
         ("labels_out", ctypes.c_void_p),
         ("ms", ctypes.c_float),
         ("status", ctypes.c_int32),
+    ]
+
+
+class BatchStats(ctypes.Structure):
+    _fields_ = [
+        ("groups", ctypes.c_int32),
+        ("replays", ctypes.c_int32),
+        ("captures", ctypes.c_int32),
+        ("eager_rounds", ctypes.c_int32),
+        ("shape", ctypes.c_int32),
+        ("blocks", ctypes.c_int32),
+        ("chunk", ctypes.c_int32),
+        ("rounds", ctypes.c_int32),
+        ("critical", ctypes.c_int64 * 4),
+        ("ms_gpu", ctypes.c_float),
     ]
 
 
