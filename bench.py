@@ -399,14 +399,23 @@ def _batch_volume(i):
     return synth.batch_volume(i).astype(np.float32)
 
 
+def _percentiles(ms):
+    ms = np.asarray(ms, np.float64)
+    return {"p50": round(float(np.percentile(ms, 50)), 3), "p90": round(float(np.percentile(ms, 90)), 3),
+            "p99": round(float(np.percentile(ms, 99)), 3), "max": round(float(ms.max()), 3)}
+
+
 def bench_batch(args):
     """512 independent volumes (sides 32..160, smoothed noise / Gaussian
-    mixtures): lower-star .grad for all, then accurate labels + conforming
-    .grad for all, through msceer_batch; aggregate vertices/s and per-volume
+    mixtures): the lower-star .grad of all of them through msceer_batch_grad
+    (one launch per stage per group of volumes, replayed from a hipGraph;
+    values resident in HBM, and with host buffers in and out), through the
+    worker-thread batch (msceer_batch), and the accurate labels + conforming
+    .grad of all through msceer_batch; aggregate vertices/s and per-volume
     time percentiles."""
     from concurrent.futures import ProcessPoolExecutor
 
-    from msceer_amd import batch, native
+    from msceer_amd import batch, hipmem, native
 
     n = args.batch_volumes
     log(f"generating {n} batch volumes")
@@ -414,10 +423,72 @@ def bench_batch(args):
         vols = list(ex.map(_batch_volume, range(n), chunksize=8))
     V = int(sum(v.size for v in vols))
     ctx = native.Context(0)
+    lib = native.lib()
     err, gthr, iters, presimp = 1e-3, 1e-6, 4096, 0.0
     out = {"workload": f"{n} volumes, sides uniform 32..160 per axis (seed 4000+i), 50/50 smoothed noise "
-                       f"(sigma 1.5-3) / Gaussian mixtures (sigma 4-16, sine term); host buffers in and out",
+                       f"(sigma 1.5-3) / Gaussian mixtures (sigma 4-16, sine term)",
            "vertices": V}
+
+    # batched graph path, device-resident values and .grad
+    al = lambda b: (int(b) + 255) & ~255  # noqa: E731
+    voff, goff, vo, go = [], [], 0, 0
+    for v in vols:
+        Z, Y, X = v.shape
+        voff.append(vo)
+        goff.append(go)
+        vo += al(v.nbytes)
+        go += al((2 * X - 1) * (2 * Y - 1) * (2 * Z - 1))
+    dv, dg = hipmem.DeviceBuffer(vo), hipmem.DeviceBuffer(go)
+    for v, o in zip(vols, voff):
+        dv.upload_at(o, v)
+    arr = batch.volume_table([v.shape for v in vols], [dv.addr + o for o in voff], [dg.addr + o for o in goff])
+
+    def graph_call(flags, cap=0):
+        native.check(lib.msceer_batch_grad(ctx.handle, arr, n, cap, native.DEVICE_PTRS | flags), "msceer_batch_grad")
+        st = native.BatchStats()
+        native.check(lib.msceer_last_batch_stats(ctx.handle, ctypes.byref(st)), "msceer_last_batch_stats")
+        return st
+
+    def graph_leg(cap, reps):
+        t0 = time.perf_counter()
+        st0 = graph_call(0, cap)  # tuning, capture
+        first = time.perf_counter() - t0
+        graph_call(0, cap)
+        hipmem.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            st = graph_call(0, cap)
+        hipmem.synchronize()
+        wall = (time.perf_counter() - t0) / reps
+        graph_call(native.TIMING, cap)  # capture of the timed variant
+        stt = graph_call(native.TIMING, cap)
+        ms = np.array([arr[i].ms for i in range(n)], np.float64)
+        return {"value": round(V / wall / 1e6, 2), "unit": "Mvert/s", "ms_per_call": round(wall * 1e3, 3),
+                "calls": reps, "first_call_s": round(first, 3), "groups": st.groups,
+                "max_group_verts": cap, "replays_per_call": st.replays, "captures_in_timed_calls": st.captures,
+                "eager_rounds_first_call": st0.eager_rounds, "eager_rounds": st.eager_rounds,
+                "rounds_per_pass": st.rounds, "gpu_ms": round(st.ms_gpu, 3),
+                "launch": {"brick": native.BRICKS[st.shape], "persistent_blocks": st.blocks, "chunk": st.chunk},
+                "per_volume_ms": _percentiles(ms),
+                "per_volume_ms_is": "device clock, from the first brick of the volume's group to its last "
+                                    "finished tile (timed call)",
+                "gpu_ms_timed_call": round(stt.ms_gpu, 3)}
+
+    log("batched .grad, one group, graph replays")
+    out["grad_graph"] = graph_leg(0, 5)
+    log("batched .grad, groups of at most 2^25 vertices")
+    out["grad_graph_groups"] = graph_leg(1 << 25, 3)
+    dv.free()
+    dg.free()
+
+    log("batched .grad, host buffers")
+    batch.run_batch_grad(vols, ctx=ctx)
+    t0 = time.perf_counter()
+    _, _, st = batch.run_batch_grad(vols, ctx=ctx)
+    wall = time.perf_counter() - t0
+    out["grad_graph_host_buffers"] = {"value": round(V / wall / 1e6, 2), "unit": "Mvert/s",
+                                      "seconds": round(wall, 3), "gpu_ms": round(st.ms_gpu, 3),
+                                      "note": "PCIe-inclusive: values copied in and .grad out per call"}
 
     def timed(labels, streams):
         t0 = time.perf_counter()
@@ -427,24 +498,16 @@ def bench_batch(args):
 
     def summary(wall, ms, streams):
         return {"value": round(V / wall / 1e6, 2), "unit": "Mvert/s", "seconds": round(wall, 3), "streams": streams,
-                "per_volume_ms": {"p50": round(float(np.percentile(ms, 50)), 3),
-                                  "p90": round(float(np.percentile(ms, 90)), 3),
-                                  "p99": round(float(np.percentile(ms, 99)), 3),
-                                  "max": round(float(ms.max()), 3)}}
+                "note": "host buffers in and out", "per_volume_ms": _percentiles(ms)}
 
     k = args.batch_streams
+    log("worker-thread batch (.grad)")
     timed(False, k)  # brick tuning for every x extent, buffers grown
-    out["grad"] = summary(*timed(False, k), k)
+    out["grad_threads"] = summary(*timed(False, k), k)
     out["grad_one_stream"] = summary(*timed(False, 1), 1)
+    log("worker-thread batch (labels)")
     timed(True, k)
     out["labels"] = summary(*timed(True, k), k)
-    lib = native.lib()
-    tuned = {}
-    for x in sorted({int(v.shape[2]) for v in vols}):
-        sh = lib.msceer_lstar_brick(ctx.handle, x)
-        tuned[native.BRICKS[sh] if 0 <= sh < len(native.BRICKS) else "untuned"] = \
-            tuned.get(native.BRICKS[sh] if 0 <= sh < len(native.BRICKS) else "untuned", 0) + 1
-    out["brick_choice_by_x_extent"] = tuned
     ctx.close()
     return out
 

@@ -312,7 +312,9 @@ hipError_t lstar_stage(BatchState& s, GroupPlan& p, const ListCaps& caps, const 
 
 // Measure the lower-star launches of a group on its own data: every brick
 // shape a work-group per brick, then the best shape persistent, with claim
-// chunks of 1, 4 and 16 bricks.
+// chunks of 1, 4 and 16 bricks, as many work-groups as the CUs hold and half
+// as many.  (A brick is always four wavefronts, 256 vertices: the tile, the
+// stage and the dealing of vertices to lanes by shape are laid out for it.)
 int tune_group(msceer_ctx* ctx, BatchState& s, GroupPlan& p, const ListCaps& caps, LaunchTune* out) {
     // a fixed launch "shape:blocks_per_cu:chunk" (blocks_per_cu 0: a
     // work-group per brick), for tests of each launch form
@@ -360,17 +362,28 @@ int tune_group(msceer_ctx* ctx, BatchState& s, GroupPlan& p, const ListCaps& cap
         if (rc) return rc;
         if (t.ms < best.ms) best = t;
     }
+    // persistent: as many work-groups as the CUs hold, then half as many
     const int occ = lstar_batch_occupancy(best.shape);
     const uint32_t chunks[3] = {1, 4, 16};
     const LaunchTune grid = best;
+    LaunchTune pbest;
+    pbest.ms = 1e30f;
     for (uint32_t c : chunks) {
         LaunchTune t = grid;
         t.blocks = (uint32_t)(s.cus * occ);
         t.chunk = c;
         int rc = time_one(t);
         if (rc) return rc;
-        if (t.ms < best.ms) best = t;
+        if (t.ms < pbest.ms) pbest = t;
     }
+    if (occ > 1) {
+        LaunchTune t = pbest;
+        t.blocks = (uint32_t)(s.cus * (occ / 2));
+        int rc = time_one(t);
+        if (rc) return rc;
+        if (t.ms < pbest.ms) pbest = t;
+    }
+    if (pbest.ms < best.ms) best = pbest;
     *out = best;
     return MSCEER_OK;
 }
@@ -582,7 +595,7 @@ int run_group(msceer_ctx* ctx, BatchState& s, msceer_volume* vols, GroupLayout& 
     int rc = read_gctl(ctx, s);
     if (rc) return rc;
     const bool lists_lost = s.h_ctl[kGOverflow] != 0;
-    int extra = 0;
+    int extra = 0, grow_by = 0;
     bool redo = false;
     for (int pass = 1; pass <= 2; pass++) {
         const int R = p.rounds;
@@ -595,6 +608,8 @@ int run_group(msceer_ctx* ctx, BatchState& s, msceer_volume* vols, GroupLayout& 
         rc = continue_pass(ctx, s, p, g, vols, dev, caps, pass, R, sweep, &used);
         if (rc) return rc;
         extra += used;
+        // more captured rounds help a walk that was still going, not an overflow
+        if (!sweep || R == 0) grow_by = std::max(grow_by, used);
         redo = true;
     }
     if (redo) {
@@ -603,9 +618,11 @@ int run_group(msceer_ctx* ctx, BatchState& s, msceer_volume* vols, GroupLayout& 
         b.times = timing ? s.times.as<unsigned long long>() : nullptr;
         CHECK_HIP(launch_ldir_finish_group(b, ctx->stream));
         // the next capture of this plan takes the rounds this call needed
-        if (p.exec) (void)hipGraphExecDestroy(p.exec);
-        p.exec = nullptr;
-        p.rounds = std::min(p.rounds + extra, 64);
+        if (grow_by > 0) {
+            if (p.exec) (void)hipGraphExecDestroy(p.exec);
+            p.exec = nullptr;
+            p.rounds = std::min(p.rounds + grow_by, 64);
+        }
         st.eager_rounds += extra;
     }
     CHECK_HIP(hipEventRecord(s.t1, ctx->stream));

@@ -67,8 +67,9 @@ hipError_t launch_ldir_records(const Dims& d, const uint8_t* grad, void* scratch
                                hipStream_t st);
 // One frontier round of pass 1 (edges) or 2 (quads): expands the items of
 // `in` (critical cells, claimed first, when `start`), appends what is left to
-// `out` and zeroes `clear_counts` for the next round.  overflow |= 1 when an
-// item did not fit `out`, |= 4 when a start round left items unread.
+// `out` (start items it did not reach claimed first) and zeroes
+// `clear_counts` for the next round.  overflow |= 1 when an item did not fit
+// `out`.
 hipError_t launch_ldir_round(const Dims& d, void* scratch, const LdirScratch& s, int pass, bool start, const MList& in,
                              const MList& out, uint32_t* clear_counts, uint32_t* overflow, hipStream_t st);
 // Rebuild a frontier from the bitmaps and records alone (after an overflow).

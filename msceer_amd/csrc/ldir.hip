@@ -425,15 +425,27 @@ constexpr uint32_t kNoItem = 0xFFFFFFFFu;
 // a critical cell of the start list, claimed here first (skipped if already
 // reached).  The claims of the slots are independent atomics, issued
 // together.
+// Coordinates, axis (an edge item: its odd axis; a quad item: its even axis)
+// and record index of an item.
+__device__ __forceinline__ uint32_t item_record(const LGeom& g, bool edges, uint32_t item, uint32_t& x, uint32_t& y,
+                                                uint32_t& z, uint32_t& o) {
+    cell_coords(g, item, x, y, z);
+    o = edges ? ((x & 1u) ? 0u : ((y & 1u) ? 1u : 2u)) : (!(x & 1u) ? 0u : (!(y & 1u) ? 1u : 2u));
+    return rec_index(g, x, y, z, o);
+}
+
+// Claim an item's bit; true if this call set it.
+__device__ __forceinline__ bool claim_item(const LGeom& g, const Walk& w, uint32_t item) {
+    uint32_t x, y, z, o;
+    return claim_bit(w.bits, item_record(g, w.pass == 1, item, x, y, z, o));
+}
+
 __device__ __forceinline__ void expand(const LGeom& g, const Walk& w, uint32_t item, bool start, uint32_t out[4]) {
 #pragma unroll
     for (int s = 0; s < 4; s++) out[s] = kNoItem;
-    uint32_t x, y, z;
-    cell_coords(g, item, x, y, z);
+    uint32_t x, y, z, o;
     const bool edges = w.pass == 1;
-    // an edge item: its odd axis; a quad item: its even axis
-    const uint32_t o = edges ? ((x & 1u) ? 0u : ((y & 1u) ? 1u : 2u)) : (!(x & 1u) ? 0u : (!(y & 1u) ? 1u : 2u));
-    const uint32_t ri = rec_index(g, x, y, z, o);
+    const uint32_t ri = item_record(g, edges, item, x, y, z, o);
     if (start) {
         const uint32_t m = 1u << (ri & 31u);
         if (atomicOr(&w.bits[ri >> 5], m) & m) return;
@@ -483,23 +495,36 @@ struct OneSpace {
     __device__ __forceinline__ void expand_item(uint32_t item, bool start, uint32_t out[4]) const {
         expand(g, w, item, start, out);
     }
+    __device__ __forceinline__ bool claim(uint32_t item) const { return claim_item(g, w, item); }
 };
 
 struct GroupSpace {
     GBatch b;
     int pass;
-    __device__ __forceinline__ void expand_item(uint32_t item, bool start, uint32_t out[4]) const {
+    // the item's volume: its geometry, walk and first item
+    __device__ __forceinline__ uint32_t locate(uint32_t item, LGeom& g, Walk& w) const {
         const uint32_t vol = unit_volume(b.items, item);
         const GVol& gv = b.vols[vol];
-        const LGeom g{gv.Mx, gv.My, gv.Mz, gv.Mx * gv.My, (uint32_t)gv.X, (uint32_t)gv.Y,
-                      (uint32_t)gv.X * (uint32_t)gv.Y * (uint32_t)gv.Z, ((uint32_t)gv.X + 3u) / 4u,
-                      ((uint32_t)gv.Y + 3u) / 4u};
-        const Walk w{gv.rec1, gv.rec2, pass == 1 ? gv.bits1 : gv.bits2, pass};
-        const uint32_t base = b.items.first[vol];
+        g = LGeom{gv.Mx, gv.My, gv.Mz, gv.Mx * gv.My, (uint32_t)gv.X, (uint32_t)gv.Y,
+                  (uint32_t)gv.X * (uint32_t)gv.Y * (uint32_t)gv.Z, ((uint32_t)gv.X + 3u) / 4u,
+                  ((uint32_t)gv.Y + 3u) / 4u};
+        w = Walk{gv.rec1, gv.rec2, pass == 1 ? gv.bits1 : gv.bits2, pass};
+        return b.items.first[vol];
+    }
+    __device__ __forceinline__ void expand_item(uint32_t item, bool start, uint32_t out[4]) const {
+        LGeom g;
+        Walk w;
+        const uint32_t base = locate(item, g, w);
         expand(g, w, item - base, start, out);
 #pragma unroll
         for (int s = 0; s < 4; s++)
             if (out[s] != kNoItem) out[s] += base;
+    }
+    __device__ __forceinline__ bool claim(uint32_t item) const {
+        LGeom g;
+        Walk w;
+        const uint32_t base = locate(item, g, w);
+        return claim_item(g, w, item - base);
     }
 };
 
@@ -605,21 +630,29 @@ ldir_round_kernel(Space sp, MList in, MList out, uint32_t* __restrict__ clear, u
         __syncthreads();
     }
     // hand on what is left: the pool (claimed items), then the unread part of
-    // the share (start items stay start items only in the first round, so a
-    // start round hands on nothing unread: see launch_ldir_round)
+    // the share -- in a start round claimed here first, as their expansion
+    // would have claimed them, and dropped if already reached (the rounds
+    // after a start round hold claimed items only)
     __syncthreads();
     const uint32_t n = min(s_n, (uint32_t)kPool);
     mlist_flush_block(out, sub, pool, n, &s_base, overflow);
-    if (!start) {
-        for (uint32_t nx = s_next; nx < hi; nx += kPool) {
-            const uint32_t take = min(hi - nx, (uint32_t)kPool);
-            __syncthreads();
+    for (uint32_t nx = s_next; nx < hi; nx += kPool) {
+        const uint32_t take = min(hi - nx, (uint32_t)kPool);
+        __syncthreads();
+        if (!start) {
             for (uint32_t j = tid; j < take; j += kThreads) pool[j] = mlist_at(in, prefix, nx + j);
             __syncthreads();
             mlist_flush_block(out, sub, pool, take, &s_base, overflow);
+        } else {
+            if (tid == 0) s_n = 0;
+            __syncthreads();
+            for (uint32_t j = tid; j < take; j += kThreads) {
+                const uint32_t item = mlist_at(in, prefix, nx + j);
+                if (sp.claim(item)) pool[atomicAdd(&s_n, 1u)] = item;
+            }
+            __syncthreads();
+            mlist_flush_block(out, sub, pool, s_n, &s_base, overflow);
         }
-    } else if (s_next < hi && tid == 0) {
-        atomicOr(overflow, 4u);  // start items left unread: the caller sweeps
     }
 }
 

@@ -52,6 +52,19 @@ class DeviceBuffer:
         assert a.nbytes <= self.nbytes
         _ok(hip().hipMemcpy(self.ptr, a.ctypes.data_as(ctypes.c_void_p), a.nbytes, _H2D), "hipMemcpy H2D")
 
+    def upload_at(self, offset: int, a: np.ndarray):
+        a = np.ascontiguousarray(a)
+        assert offset + a.nbytes <= self.nbytes
+        _ok(hip().hipMemcpy(ctypes.c_void_p(self.ptr.value + offset), a.ctypes.data_as(ctypes.c_void_p), a.nbytes,
+                            _H2D), "hipMemcpy H2D")
+
+    def download_at(self, offset: int, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        assert offset + out.nbytes <= self.nbytes
+        _ok(hip().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), ctypes.c_void_p(self.ptr.value + offset), out.nbytes,
+                            _D2H), "hipMemcpy D2H")
+        return out
+
     def download(self, dtype, count: int) -> np.ndarray:
         out = np.empty(count, dtype=dtype)
         _ok(hip().hipMemcpy(out.ctypes.data_as(ctypes.c_void_p), self.ptr, out.nbytes, _D2H), "hipMemcpy D2H")
