@@ -475,7 +475,10 @@ def bench_batch(args):
                 "gpu_ms_timed_call": round(stt.ms_gpu, 3)}
 
     log("batched .grad, one group, graph replays")
-    out["grad_graph"] = graph_leg(0, 5)
+    out["grad_graph"] = graph_leg(0, args.batch_calls)
+    if args.batch_graph_only:
+        ctx.close()
+        return out
     log("batched .grad, groups of at most 2^25 vertices")
     out["grad_graph_groups"] = graph_leg(1 << 25, 3)
     dv.free()
@@ -662,6 +665,9 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--batch-volumes", type=int, default=512)
+    ap.add_argument("--batch-calls", type=int, default=5, help="timed calls of the batched .grad (one group)")
+    ap.add_argument("--batch-graph-only", action="store_true",
+                    help="with --batch-only: only the batched .grad as one group (for profiling)")
     ap.add_argument("--batch-streams", type=int, default=16, help="worker streams of the batched workload")
     ap.add_argument("--hw-queues", type=int, default=16,
                     help="GPU_MAX_HW_QUEUES for this process (HIP maps the batch workers' streams onto this many "
