@@ -284,6 +284,25 @@ typedef struct msceer_batch_stats {
 } msceer_batch_stats;
 int msceer_last_batch_stats(const msceer_ctx* ctx, msceer_batch_stats* out);
 
+/*
+ * The .grad consumer of extractmsc before any simplification
+ * (extractmsc/extractmsc.cxx:467-556): Morse-Smale complex nodes = critical
+ * cells in cell-id order (MorseSmaleComplexBasic::ComputeFromGrad,
+ * gi_morse_smale_complex_basic.h:676-690); basins_out (X*Y*Z int32, may be
+ * NULL): per vertex the node id of the minimum its V-path descends to
+ * (FillAscendingIndexVolumeByVPaths, extractmsc.cxx:371-440; TESTBASINS.raw);
+ * mounts_out (X*Y*Z int32, may be NULL): per vertex the node id of the
+ * maximum whose descending manifold holds the hex at that vertex
+ * (fillGeometry of the maxima, gi_morse_smale_complex_basic.h:1181-1198;
+ * TESTMOUNTS.raw), -1 where there is none (the last vertex layers, hexes
+ * paired with a boundary quad).  counts_out (4 int64, may be NULL): critical
+ * cells by dimension.  Needs fewer than 2^31 vertices.  Flags:
+ * MSCEER_DEVICE_PTRS.  Arcs and the persistence hierarchy
+ * (ComputeHierarchy) are not computed.
+ */
+int msceer_msc_base(msceer_ctx* ctx, const uint8_t* grad, int64_t nx, int64_t ny, int64_t nz, int32_t* basins_out,
+                    int32_t* mounts_out, int64_t* counts_out, uint32_t flags);
+
 /* Stages of the accurate pipeline as separate calls (host or device pointers):
  * IsolatedCCRegionRemoverNEW::ComputeOutput (gi_isolated_region_remover.h:155-192),
  * in place on X*Y*Z int32 labels. */

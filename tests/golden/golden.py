@@ -96,3 +96,18 @@ def load_c3_fp64(exact=False):
     assert tuple(d["shape"]) == C3_SHAPE
     idx = np.frombuffer(lzma.decompress(d["idx_lzma"].tobytes()), np.uint16)
     return d["table"][idx].astype(np.int32), str(d["input_sha256"])
+
+
+def msc_case_names():
+    return sorted(os.path.splitext(f)[0] for f in os.listdir(os.path.join(HERE, "msc")) if f.endswith(".npz"))
+
+
+def load_msc(name):
+    """(values, reference .grad, basins, mounts, node counts by dimension) of
+    the reference's Morse-Smale complex read of a lower-star golden .grad."""
+    f, grad = load_lstar(name)
+    d = np.load(os.path.join(HERE, "msc", name + ".npz"))
+    assert tuple(d["shape"]) == f.shape and str(d["input_sha256"]) == synth.sha256(f)
+    basins = np.frombuffer(zlib.decompress(d["basins_z"].tobytes()), np.int32).reshape(f.shape)
+    mounts = np.frombuffer(zlib.decompress(d["mounts_z"].tobytes()), np.int32).reshape(f.shape)
+    return f, grad, basins, mounts, d["counts"]

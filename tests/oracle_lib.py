@@ -285,3 +285,19 @@ def cnn_gradient(params, f, target, pos_w=1.0, quant=1):
     grad = np.zeros(cnn_num_params(), np.float64)
     _cnn_sigs().oracle_cnn_gradient(_p(p), _p(f), X, Y, Z, _p(t), float(pos_w), quant, _p(loss), _p(grad))
     return float(loss[0]), grad
+
+
+def msc_base(grad, shape_zyx):
+    """extractmsc's base reading of a .grad: (basins, mounts, counts) -- int32
+    node ids per vertex (nodes = critical cells in cell-id order) and the
+    critical cells by dimension."""
+    h = lib()
+    h.oracle_msc_base.argtypes = [_P, _L, _L, _L, _P, _P, _P]
+    h.oracle_msc_base.restype = ctypes.c_longlong
+    g = np.ascontiguousarray(grad, np.uint8).ravel()
+    Z, Y, X = shape_zyx
+    basins = np.empty(X * Y * Z, np.int32)
+    mounts = np.empty(X * Y * Z, np.int32)
+    counts = np.zeros(4, np.int64)
+    h.oracle_msc_base(_p(g), X, Y, Z, _p(basins), _p(mounts), _p(counts))
+    return basins.reshape(shape_zyx), mounts.reshape(shape_zyx), counts
