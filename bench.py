@@ -745,6 +745,30 @@ def main():
                 "kernel": "lstar_kernel", "kernel_ms": round(k_ms, 4),
                 "algorithmic_bytes": algo_bytes}
 
+    # extractmsc's reading of the same .grad (nodes, base basins, base mountains)
+    log("Morse-Smale nodes, basins and mountains of the .grad")
+    db, dm = hipmem.DeviceBuffer(4 * verts), hipmem.DeviceBuffer(4 * verts)
+    counts = np.zeros(4, np.int64)
+
+    def msc_call():
+        native.check(lib.msceer_msc_base(ctx.handle, ctypes.c_void_p(dg.addr), X, Y, Z, ctypes.c_void_p(db.addr),
+                                         ctypes.c_void_p(dm.addr), counts.ctypes.data_as(ctypes.c_void_p),
+                                         native.DEVICE_PTRS), "msceer_msc_base")
+
+    msc_call()
+    hipmem.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        msc_call()
+    hipmem.synchronize()
+    msc_ms = (time.perf_counter() - t0) * 1e3 / 3
+    db.free()
+    dm.free()
+    msc_leg = {"ms": round(msc_ms, 3), "value": round(verts / (msc_ms * 1e-3) / 1e6, 2), "unit": "Mvert/s",
+               "nodes_by_dim": counts.tolist(),
+               "what": "msceer_msc_base on the .grad above: node numbering, V-path basins, descending-manifold "
+                       "mountains (extractmsc before simplification), device buffers"}
+
     base = None
     if not args.no_cpu:
         log("timing the CPU baseline")
@@ -768,6 +792,7 @@ def main():
                    "parallelism": "single GPU"},
         "roofline": roofline,
         "cpu_baseline": base,
+        "msc_base": msc_leg,
         "detail": {"lstar_kernel_ms_mean": round(k_ms, 4),
                    "ldir_ms_mean": round(float(np.mean(ldir_ms)), 4),
                    "ldir_rounds": int(np.max(rounds)), "critical_cells_by_dim": crit},
