@@ -303,6 +303,27 @@ int msceer_last_batch_stats(const msceer_ctx* ctx, msceer_batch_stats* out);
 int msceer_msc_base(msceer_ctx* ctx, const uint8_t* grad, int64_t nx, int64_t ny, int64_t nz, int32_t* basins_out,
                     int32_t* mounts_out, int64_t* counts_out, uint32_t flags);
 
+/*
+ * NumericIntegratorExpandingRegionStop<AdaptiveEulerAdvector3D<F, -1>,
+ * IndexCompareLessThan<F>>::BeginIntegration
+ * (gi_numeric_integrator_expanding_region_stop.h:204-440; no reference
+ * program instantiates it): the ascending-manifold labels of the minima of
+ * the SameBdry6 rule (no neighbour before the vertex along the axes where it
+ * is interior).  Certain regions grow from each minimum over the vertices
+ * whose earlier neighbours all lie in it; every other vertex is integrated
+ * (budget max_iters) until it nears a certain vertex; vertices whose
+ * neighbours all share their label become certain, and the rest are
+ * integrated again with 500 iterations.  labels_out (X*Y*Z int32): the vertex
+ * id of the minimum, or -1.  The streamlines are traced in FP64, every vertex
+ * alone (the reference compresses pass-1 paths in its threads' visiting
+ * order, which makes its own result schedule-dependent).  stats_out (5 int64,
+ * may be NULL): minima, certain vertices, pass-1 streamlines, unambiguous
+ * vertices, re-integrated vertices.  Flags: MSCEER_DEVICE_PTRS.
+ */
+int msceer_expanding_region_stop(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz,
+                                 float err_thr, float grad_thr, int32_t max_iters, int32_t* labels_out,
+                                 int64_t* stats_out, uint32_t flags);
+
 /* Stages of the accurate pipeline as separate calls (host or device pointers):
  * IsolatedCCRegionRemoverNEW::ComputeOutput (gi_isolated_region_remover.h:155-192),
  * in place on X*Y*Z int32 labels. */

@@ -123,7 +123,7 @@ grad_field64_kernel(const float* __restrict__ f, VGrid g, const uint8_t* __restr
     const int64_t n = g.verts();
     for (int64_t v = (int64_t)blockIdx.x * kThreads + threadIdx.x; v < n; v += (int64_t)gridDim.x * kThreads) {
         double4 r = make_double4(0.0, 0.0, 0.0, 0.0);
-        if (!seedmask[v]) {
+        if (!seedmask || !seedmask[v]) {
             int x, y, z;
             vertex_xyz(g, v, x, y, z);
             r.x = (x > 0 && x < g.X - 1) ? (0.0 + (double)(-0.5f * f[v - 1])) + (double)(0.5f * f[v + 1]) : 0.0;
@@ -302,6 +302,118 @@ __device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t x, int lane) {
     return incl - x;
 }
 
+// ---- NumericIntegratorExpandingRegionStop --------------------------------
+// (gi_numeric_integrator_expanding_region_stop.h:63-440; minima, downhill)
+
+// SameBdry6 neighbours of (x, y, z): the axes along which it is interior.
+__device__ __forceinline__ int same_bdry6(const VGrid& g, int64_t v, int x, int y, int z, int64_t out[6]) {
+    int k = 0;
+    if (x > 0 && x < g.X - 1) { out[k++] = v + 1; out[k++] = v - 1; }
+    if (y > 0 && y < g.Y - 1) { out[k++] = v + g.X; out[k++] = v - g.X; }
+    if (z > 0 && z < g.Z - 1) { out[k++] = v + g.sxy; out[k++] = v - g.sxy; }
+    return k;
+}
+
+// Extrema (:94-107, :218-233): no SameBdry6 neighbour before the vertex; each
+// labelled by its own id, the rest -1.  counts[0]: extrema.
+__global__ void __launch_bounds__(kThreads)
+ers_seed_kernel(const float* __restrict__ f, VGrid g, uint8_t* __restrict__ seedmask, int32_t* __restrict__ label,
+                uint32_t* __restrict__ counts) {
+    uint32_t found = 0;
+    for (int64_t v = (int64_t)blockIdx.x * kThreads + threadIdx.x; v < g.verts(); v += (int64_t)gridDim.x * kThreads) {
+        int x, y, z;
+        vertex_xyz(g, v, x, y, z);
+        int64_t nb[6];
+        const int k = same_bdry6(g, v, x, y, z, nb);
+        bool ext = true;
+        for (int j = 0; j < k; j++)
+            if (before(f, nb[j], v)) ext = false;
+        seedmask[v] = ext ? 1 : 0;
+        label[v] = ext ? (int32_t)v : -1;
+        found += ext ? 1u : 0u;
+    }
+    if (found) atomicAdd(&counts[0], found);
+}
+
+// Stop states of the streamlines: certain vertices are terminals
+// (TerminateNearCertain, gi_advection_checker.h:183-208).  counts[1]: certain.
+__global__ void __launch_bounds__(kThreads)
+ers_state_kernel(VGrid g, const int32_t* __restrict__ label, uint8_t* __restrict__ dest, uint32_t* __restrict__ counts) {
+    uint32_t n = 0;
+    for (int64_t v = (int64_t)blockIdx.x * kThreads + threadIdx.x; v < g.verts(); v += (int64_t)gridDim.x * kThreads) {
+        const bool c = label[v] != -1;
+        dest[v] = c ? kCertainTerminal : kUnassigned;
+        n += c ? 1u : 0u;
+    }
+    if (n) atomicAdd(&counts[1], n);
+}
+
+// Every vertex still unassigned traces its own streamline in FP64 (budget
+// `budget` iterations) to a certain terminal, or wherever it stops, and takes
+// lab_in of that vertex.  counts[slot]: streamlines.
+__global__ void __launch_bounds__(kThreads)
+ers_trace_kernel(GradSrc<double> src, VGrid g, const uint8_t* __restrict__ dest, const int32_t* __restrict__ lab_in,
+                 int32_t* __restrict__ lab_out, double gthr, double ethr, int budget, uint32_t* __restrict__ counts,
+                 int slot) {
+    uint32_t traced = 0;
+    int64_t t = (int64_t)blockIdx.x * kThreads + threadIdx.x;
+    const int64_t t_step = (int64_t)gridDim.x * kThreads;
+    Tracer<double> tr;
+    bool active = false;
+    int64_t v0 = 0;
+    // a lane whose streamline ended takes its next vertex at once
+    while (true) {
+        while (!active && t < g.verts()) {
+            v0 = t;
+            t += t_step;
+            if (dest[v0] == kUnassigned) {
+                int x, y, z;
+                vertex_xyz(g, v0, x, y, z);
+                tracer_start(src, g, tr, x, y, z, budget);
+                traced++;
+                active = true;
+            }
+        }
+        if (!__any(active)) break;
+        if (!active) continue;
+        int64_t next = 0;
+        bool moved = false;
+        int ev = tracer_iterate(src, g, dest, false, -1.0, gthr, ethr, tr, next, moved);
+        if (ev == kOutOfVoxel) {
+            next = tracer_reenter(src, g, tr);
+            ev = kNone;
+        }
+        if (ev == kNone) continue;
+        active = false;
+        if (ev != kHitPreassigned) next = nearest(g, tr.cur, tr.near);
+        lab_out[v0] = lab_in[next];
+    }
+    if (traced) atomicAdd(&counts[slot], traced);
+}
+
+// Unambiguous vertices (:348-373): every SameBdry6 neighbour has the vertex's
+// label; they become terminals.  counts[3]: marked.
+__global__ void __launch_bounds__(kThreads)
+ers_mark_kernel(VGrid g, const int32_t* __restrict__ label, uint8_t* __restrict__ dest, uint32_t* __restrict__ counts) {
+    uint32_t n = 0;
+    for (int64_t v = (int64_t)blockIdx.x * kThreads + threadIdx.x; v < g.verts(); v += (int64_t)gridDim.x * kThreads) {
+        if (dest[v] != kUnassigned) continue;
+        int x, y, z;
+        vertex_xyz(g, v, x, y, z);
+        int64_t nb[6];
+        const int k = same_bdry6(g, v, x, y, z, nb);
+        const int32_t l = label[v];
+        bool same = true;
+        for (int j = 0; j < k; j++)
+            if (label[nb[j]] != l) same = false;
+        if (same) {
+            dest[v] = kCertainTerminal;
+            n++;
+        }
+    }
+    if (n) atomicAdd(&counts[3], n);
+}
+
 // ---- certain regions (GridSimplifiedExtremalRegionBuilder::BeginIntegration) ----
 //
 // A vertex takes the label shared by all of its earlier All6 neighbours if
@@ -323,6 +435,14 @@ __device__ __forceinline__ uint32_t wave_exclusive_scan(uint32_t x, int lane) {
 // for the direction j of +x, -x, +y, -y, +z, -z.
 constexpr uint32_t kCntMask = 0xFFu;
 constexpr int kEarlierShift = 8, kLaterShift = 16;
+// SameBdry6 (not symmetric: a vertex on a face has no neighbour across it,
+// while the interior vertex beside it counts the face vertex): bits 22-27
+// flag the earlier vertices that count this one as their neighbour, and bit
+// 28 says the test below applies -- the priority flood reaches a vertex only
+// from such a vertex of its region (Enqueue_Later_Neighbors,
+// gi_numeric_integrator_expanding_region_stop.h:108-123).
+constexpr int kReachShift = 22;
+constexpr uint32_t kAsym = 1u << 28;
 
 __device__ __forceinline__ int64_t dir_offset(const VGrid& g, int j) {
     const int64_t o = j < 2 ? 1 : (j < 4 ? (int64_t)g.X : g.sxy);
@@ -336,10 +456,12 @@ __device__ __forceinline__ uint64_t list_entry(uint32_t v, uint32_t meta) {
 }
 
 // The vertices with no earlier non-seed neighbour start the first list.
+// same_bdry: SameBdry6 neighbours (both along an axis, and only where the
+// vertex is interior along it: gi_regular_grid.h:232-265) instead of All6.
 __global__ void __launch_bounds__(kThreads)
 certain_init_kernel(const float* __restrict__ f, VGrid g, const uint8_t* __restrict__ seedmask,
                     uint32_t* __restrict__ cnt, uint64_t* __restrict__ out, uint32_t* __restrict__ out_count,
-                    uint32_t capacity, uint32_t* __restrict__ overflow) {
+                    uint32_t capacity, uint32_t* __restrict__ overflow, int same_bdry) {
     __shared__ uint64_t stage[kStage];
     __shared__ uint32_t s_n, s_base;
     if (threadIdx.x == 0) s_n = 0;
@@ -354,20 +476,28 @@ certain_init_kernel(const float* __restrict__ f, VGrid g, const uint8_t* __restr
         }
         int x, y, z;
         vertex_xyz(g, v, x, y, z);
-        const bool inside[6] = {x < g.X - 1, x > 0, y < g.Y - 1, y > 0, z < g.Z - 1, z > 0};
-        uint32_t c = 0, earlier = 0, later = 0;
+        const bool exists[6] = {x < g.X - 1, x > 0, y < g.Y - 1, y > 0, z < g.Z - 1, z > 0};
+        // along axis a: the vertex's own SameBdry6 neighbours (interior along a),
+        // and whether the neighbour in direction j counts this vertex
+        const int c3[3] = {x, y, z}, n3[3] = {g.X, g.Y, g.Z};
+        uint32_t c = 0, earlier = 0, later = 0, reach = 0;
         for (int j = 0; j < 6; j++) {
-            if (!inside[j]) continue;
+            if (!exists[j]) continue;
+            const int a = j >> 1, cu = c3[a] + ((j & 1) ? -1 : 1);
+            const bool mine = !same_bdry || (c3[a] > 0 && c3[a] < n3[a] - 1);   // u is my neighbour
+            const bool theirs = !same_bdry || (cu > 0 && cu < n3[a] - 1);       // I am u's neighbour
             const int64_t u = v + dir_offset(g, j);
             const bool seed = seedmask[u] != 0;
             if (before(f, u, v)) {
-                earlier |= 1u << j;
-                if (!seed) c++;
-            } else if (!seed) {
+                if (mine) earlier |= 1u << j;
+                if (theirs) reach |= 1u << j;
+                if (!seed && (mine || theirs)) c++;
+            } else if (!seed && (mine || theirs)) {
                 later |= 1u << j;
             }
         }
-        const uint32_t meta = c | earlier << kEarlierShift | later << kLaterShift;
+        uint32_t meta = c | earlier << kEarlierShift | later << kLaterShift;
+        if (same_bdry) meta |= reach << kReachShift | kAsym;
         cnt[v] = meta;
         if (c == 0) stage[atomicAdd(&s_n, 1u)] = list_entry((uint32_t)v, meta);
     }
@@ -412,6 +542,12 @@ certain_round_kernel(VGrid g, int32_t* __restrict__ label, uint32_t* __restrict_
                 if (l[j] == -1) ok = false;
                 else if (!has) { val = l[j]; has = true; }
                 else if (val != l[j]) ok = false;
+            }
+            if (ok && has && (meta & kAsym)) {
+                bool reached = false;
+                for (int j = 0; j < 6; j++)
+                    if ((meta >> (kReachShift + j)) & 1u) reached = reached || label[v + dir_offset(g, j)] == val;
+                ok = reached;
             }
             if (ok && has) label[v] = val;
             for (int j = 0; j < 6; j++) {
@@ -1434,10 +1570,10 @@ uint32_t resident_blocks(K kernel) {
 }  // namespace
 hipError_t launch_certain_init(const VGrid& g, const float* values, const uint8_t* seedmask, uint32_t* cnt,
                                uint64_t* out, uint32_t* out_count, uint32_t capacity, uint32_t* overflow,
-                               hipStream_t s) {
+                               hipStream_t s, bool same_bdry) {
     const int64_t blocks = (g.verts() + kStage - 1) / kStage;
     hipLaunchKernelGGL(certain_init_kernel, dim3((uint32_t)blocks), dim3(kThreads), 0, s, values, g, seedmask, cnt,
-                       out, out_count, capacity, overflow);
+                       out, out_count, capacity, overflow, same_bdry ? 1 : 0);
     return hipGetLastError();
 }
 hipError_t launch_certain_round(const VGrid& g, int32_t* label, uint32_t* cnt, const uint64_t* in,
@@ -1451,6 +1587,29 @@ hipError_t launch_certain_round(const VGrid& g, int32_t* label, uint32_t* cnt, c
     const uint32_t blocks = (uint32_t)std::min<uint64_t>(std::max<uint64_t>(want, cus), 32ull * cus);
     hipLaunchKernelGGL(certain_round_kernel, dim3(blocks), dim3(kThreads), 0, s, g, label, cnt, in, in_count, out,
                        out_count, clear_count, capacity, overflow);
+    return hipGetLastError();
+}
+hipError_t launch_ers_seeds(const VGrid& g, const float* values, uint8_t* seedmask, int32_t* label, uint32_t* counts,
+                            hipStream_t s) {
+    hipLaunchKernelGGL(ers_seed_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, values, g, seedmask, label,
+                       counts);
+    return hipGetLastError();
+}
+hipError_t launch_ers_state(const VGrid& g, const int32_t* label, uint8_t* dest, uint32_t* counts, hipStream_t s) {
+    hipLaunchKernelGGL(ers_state_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, g, label, dest, counts);
+    return hipGetLastError();
+}
+hipError_t launch_ers_trace(const VGrid& g, const double4* gradd, const uint8_t* dest, const int32_t* lab_in,
+                            int32_t* lab_out, float gthr, float ethr, int budget, uint32_t* counts, int slot,
+                            hipStream_t s) {
+    static const uint32_t blocks = resident_blocks(ers_trace_kernel);
+    GradSrc<double> src{gradd};
+    hipLaunchKernelGGL(ers_trace_kernel, dim3(blocks), dim3(kThreads), 0, s, src, g, dest, lab_in, lab_out,
+                       (double)gthr, (double)ethr, budget, counts, slot);
+    return hipGetLastError();
+}
+hipError_t launch_ers_mark(const VGrid& g, const int32_t* label, uint8_t* dest, uint32_t* counts, hipStream_t s) {
+    hipLaunchKernelGGL(ers_mark_kernel, dim3(grid_for(g.verts())), dim3(kThreads), 0, s, g, label, dest, counts);
     return hipGetLastError();
 }
 hipError_t launch_dest_init(const VGrid& g, const LabelBuffers& b, hipStream_t s) {

@@ -98,7 +98,21 @@ hipError_t launch_pass2(const VGrid& g, const LabelBuffers& b, const WorkQueue& 
 // (expected: about how many vertices `in` holds, which sizes the launch).
 hipError_t launch_certain_init(const VGrid& g, const float* values, const uint8_t* seedmask, uint32_t* cnt,
                                uint64_t* out, uint32_t* out_count, uint32_t capacity, uint32_t* overflow,
-                               hipStream_t s);
+                               hipStream_t s, bool same_bdry = false);
+// NumericIntegratorExpandingRegionStop (minima, downhill): extrema (no
+// SameBdry6 neighbour before the vertex) labelled by their own ids, the rest
+// -1, counts[0] += extrema; stop states from labels (certain: terminal),
+// counts[1] += certain; every unassigned vertex traced alone in FP64 with
+// `budget` iterations, lab_out[v] = lab_in[where it stops], counts[slot] +=
+// streamlines; unambiguous vertices (all SameBdry6 neighbours share the
+// label) made terminals, counts[3] += them.
+hipError_t launch_ers_seeds(const VGrid& g, const float* values, uint8_t* seedmask, int32_t* label, uint32_t* counts,
+                            hipStream_t s);
+hipError_t launch_ers_state(const VGrid& g, const int32_t* label, uint8_t* dest, uint32_t* counts, hipStream_t s);
+hipError_t launch_ers_trace(const VGrid& g, const double4* gradd, const uint8_t* dest, const int32_t* lab_in,
+                            int32_t* lab_out, float gthr, float ethr, int budget, uint32_t* counts, int slot,
+                            hipStream_t s);
+hipError_t launch_ers_mark(const VGrid& g, const int32_t* label, uint8_t* dest, uint32_t* counts, hipStream_t s);
 hipError_t launch_certain_round(const VGrid& g, int32_t* label, uint32_t* cnt, const uint64_t* in,
                                 const uint32_t* in_count, uint64_t* out, uint32_t* out_count, uint32_t* clear_count,
                                 uint32_t capacity, uint32_t* overflow, uint64_t expected, hipStream_t s);

@@ -206,6 +206,64 @@ namespace {
 // OnDemandAccurate's accuracy branches: the steepest pairing; the accurate
 // ascending labelling and its MIN boundary (needASC2); the digitized
 // 1-manifolds (needASC1, needDSC1); the conforming lower stars and ldir.
+// Certain regions from the labelled seeds (b.label, b.seedmask): rounds in
+// dependency order (certain_init / certain_round), enqueued in groups; a
+// read-back per group shows whether the next round's list is empty.  Uses
+// the frontier buffers (lists) and b.stamp; same_bdry: SameBdry6 neighbours.
+int run_certain_rounds(msceer_ctx* ctx, const VGrid& g, const float* d_values, const LabelBuffers& b, bool same_bdry,
+                       int* sweeps_out) {
+    hipStream_t s = ctx->stream;
+    uint32_t* ctl = ctl_dev(ctx);
+    const int64_t V = g.verts();
+    int rc;
+    uint64_t* lists[2] = {ctx->front0.as<uint64_t>(), ctx->front1.as<uint64_t>()};
+    const uint32_t lcap = (uint32_t)std::min<uint64_t>(std::min(ctx->front0.bytes, ctx->front1.bytes) / 8, V);
+    uint32_t* counts = ctl + kCtlRound;  // three rotating count words
+    CHECK_HIP(launch_zero(counts, 3, s));
+    CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
+    CHECK_HIP(launch_certain_init(g, d_values, b.seedmask, b.stamp, lists[0], counts, lcap, ctl + kCtlOverflow, s,
+                                  same_bdry));
+    rc = read_ctl(ctx, 0, kCtlWords);
+    if (rc) return rc;
+    int k = 0;
+    // Groups are pipelined: group i+1 is enqueued before the host reads the
+    // list length that group i left (copied to pinned memory behind it), so
+    // the GPU does not wait for the host between groups; the group after
+    // the last non-empty list runs empty rounds.  The launches of a group
+    // are sized by the list length last read back.
+    uint64_t expected = ctx->h_ctl[kCtlRound];
+    CHECK_HIP(ctx->host_stage.ensure(8 * sizeof(uint32_t)));
+    const volatile uint32_t* seen = ctx->host_stage.as<uint32_t>();  // two slots of the three counts
+    int done = 0, kprev = 0;
+    for (int guard = 0; expected > 0; guard++) {
+        if (guard > (int)(V / 16 + 16)) return MSCEER_EHIP;  // a round resolves at least one vertex
+        constexpr int kGroup = 16;
+        for (int j = 0; j < kGroup; j++, k++)
+            CHECK_HIP(launch_certain_round(g, b.label, b.stamp, lists[k % 2], counts + k % 3, lists[(k + 1) % 2],
+                                           counts + (k + 1) % 3, counts + (k + 2) % 3, lcap, ctl + kCtlOverflow,
+                                           expected, s));
+        const int slot = guard & 1;
+        CHECK_HIP(hipMemcpyAsync(ctx->host_stage.as<uint32_t>() + 4 * slot, counts, 3 * sizeof(uint32_t),
+                                 hipMemcpyDeviceToHost, s));
+        CHECK_HIP(hipEventRecord(ctx->ev[8 + slot], s));
+        if (guard > 0) {
+            CHECK_HIP(hipEventSynchronize(ctx->ev[8 + (slot ^ 1)]));
+            const uint32_t left = seen[4 * (slot ^ 1) + kprev % 3];
+            if (left == 0) {
+                done = kprev;
+                break;
+            }
+            expected = left;
+        }
+        kprev = k;
+    }
+    rc = read_ctl(ctx, 0, kCtlWords);  // also waits for the last group
+    if (rc) return rc;
+    if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;
+    *sweeps_out = done ? done : k;
+    return MSCEER_OK;
+}
+
 int ondemand_impl(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz, float err_thr,
                   float grad_thr, int32_t max_iters, float presimp, uint32_t need, const uint8_t* integrate_mask,
                   uint8_t* grad_out, int32_t* labels_out, int8_t* restriction_out, msceer_label_stats* stats,
@@ -358,56 +416,10 @@ int ondemand_impl(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, 
             CHECK_HIP(hipMemcpyAsync(d_sl, seed_l.data(), sizeof(int32_t) * seed_l.size(), hipMemcpyHostToDevice, s));
             CHECK_HIP(launch_seed_scatter(d_sv, d_sl, (int)seed_v.size(), b, s));
         }
-        // rounds in dependency order, enqueued in groups; a read-back per group
-        // shows whether the next round's list is empty
+        // rounds in dependency order
         int sweeps = 0;
-        {
-            uint64_t* lists[2] = {ctx->front0.as<uint64_t>(), ctx->front1.as<uint64_t>()};
-            const uint32_t lcap = (uint32_t)std::min<uint64_t>(std::min(ctx->front0.bytes, ctx->front1.bytes) / 8, V);
-            uint32_t* counts = ctl + kCtlRound;  // three rotating count words
-            CHECK_HIP(launch_zero(counts, 3, s));
-            CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
-            CHECK_HIP(launch_certain_init(g, d_values, b.seedmask, b.stamp, lists[0], counts, lcap,
-                                          ctl + kCtlOverflow, s));
-            rc = read_ctl(ctx, 0, kCtlWords);
-            if (rc) return rc;
-            int k = 0;
-            // Groups are pipelined: group i+1 is enqueued before the host reads the
-            // list length that group i left (copied to pinned memory behind it), so
-            // the GPU does not wait for the host between groups; the group after
-            // the last non-empty list runs empty rounds.  The launches of a group
-            // are sized by the list length last read back.
-            uint64_t expected = ctx->h_ctl[kCtlRound];
-            CHECK_HIP(ctx->host_stage.ensure(8 * sizeof(uint32_t)));
-            const volatile uint32_t* seen = ctx->host_stage.as<uint32_t>();  // two slots of the three counts
-            int done = 0, kprev = 0;
-            for (int guard = 0; expected > 0; guard++) {
-                if (guard > (int)(V / 16 + 16)) return MSCEER_EHIP;  // a round resolves at least one vertex
-                constexpr int kGroup = 16;
-                for (int j = 0; j < kGroup; j++, k++)
-                    CHECK_HIP(launch_certain_round(g, b.label, b.stamp, lists[k % 2], counts + k % 3,
-                                                   lists[(k + 1) % 2], counts + (k + 1) % 3, counts + (k + 2) % 3,
-                                                   lcap, ctl + kCtlOverflow, expected, s));
-                const int slot = guard & 1;
-                CHECK_HIP(hipMemcpyAsync(ctx->host_stage.as<uint32_t>() + 4 * slot, counts, 3 * sizeof(uint32_t),
-                                         hipMemcpyDeviceToHost, s));
-                CHECK_HIP(hipEventRecord(ctx->ev[8 + slot], s));
-                if (guard > 0) {
-                    CHECK_HIP(hipEventSynchronize(ctx->ev[8 + (slot ^ 1)]));
-                    const uint32_t left = seen[4 * (slot ^ 1) + kprev % 3];
-                    if (left == 0) {
-                        done = kprev;
-                        break;
-                    }
-                    expected = left;
-                }
-                kprev = k;
-            }
-            rc = read_ctl(ctx, 0, kCtlWords);  // also waits for the last group
-            if (rc) return rc;
-            if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;
-            sweeps = done ? done : k;
-        }
+        rc = run_certain_rounds(ctx, g, d_values, b, false, &sweeps);
+        if (rc) return rc;
         st.certain_sweeps = sweeps;
         CHECK_HIP(hipEventRecord(ev[2], s));
         auto finish_early = [&]() -> int {
@@ -797,5 +809,78 @@ extern "C" int msceer_integrate_labels(msceer_ctx* ctx, const float* values, int
     st.ms_integrate = elapsed(ev[0], ev[1]);
     st.ms_total = st.ms_integrate;
     if (stats) *stats = st;
+    return MSCEER_OK;
+}
+
+// NumericIntegratorExpandingRegionStop<AdaptiveEulerAdvector3D<F, -1>,
+// IndexCompareLessThan<F>>::BeginIntegration
+// (gi_numeric_integrator_expanding_region_stop.h:204-440): extrema by the
+// SameBdry6 rule, certain regions grown from each (the priority flood of
+// Expand_Lower_Neighborhood, :153-180, has one solution, computed here in
+// dependency order), pass 1 to the certain regions, unambiguous vertices made
+// terminals, and a re-integration of the rest with 500 iterations.  The
+// streamlines are traced in FP64, each vertex alone in both passes (the
+// reference's pass 1 compresses paths in its threads' visiting order, and
+// its re-integration reads labels that earlier re-integrations wrote; here
+// the re-integration reads pass 1's labels).
+extern "C" int msceer_expanding_region_stop(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz,
+                                            float err_thr, float grad_thr, int32_t max_iters, int32_t* labels_out,
+                                            int64_t* stats_out, uint32_t flags) {
+    if (!ctx || !values || !labels_out || max_iters < 0) return MSCEER_EINVAL;
+    if (flags & ~MSCEER_DEVICE_PTRS) return MSCEER_EINVAL;
+    int rc = check_dims(nx, ny, nz);
+    if (rc) return rc;
+    if (nx * ny * nz >= (int64_t)1 << 31) return MSCEER_ETOOBIG;
+    CHECK_HIP(hipSetDevice(ctx->device));
+    const bool dev = flags & MSCEER_DEVICE_PTRS;
+    const VGrid g = make_vgrid(nx, ny, nz);
+    const Dims d = make_dims(nx, ny, nz);
+    const int64_t V = g.verts();
+    hipStream_t s = ctx->stream;
+    const float* d_values = nullptr;
+    rc = stage_in(ctx, ctx->values, values, (size_t)V, dev, &d_values);
+    if (rc) return rc;
+    CHECK_HIP(ctx->gradd.ensure(sizeof(double4) * V));
+    CHECK_HIP(ctx->label.ensure(sizeof(int32_t) * V));
+    CHECK_HIP(ctx->label2.ensure(sizeof(int32_t) * V));
+    CHECK_HIP(ctx->stamp.ensure(sizeof(uint32_t) * V));
+    CHECK_HIP(ctx->dest.ensure(V));
+    CHECK_HIP(ctx->seedmask.ensure(V));
+    ListCaps caps{};
+    rc = ensure_lists(ctx, d, &caps);
+    if (rc) return rc;
+    uint32_t* ctl = ctl_dev(ctx);
+    uint32_t* counts = ctl + kCtlIntStats;  // extrema, certain, pass-1 streamlines, unambiguous, re-integrated
+    CHECK_HIP(launch_zero(counts, 5, s));
+    LabelBuffers b{};
+    b.values = d_values;
+    b.gradd = ctx->gradd.as<double4>();
+    b.label = ctx->label.as<int32_t>();
+    b.label2 = ctx->label2.as<int32_t>();
+    b.dest = ctx->dest.as<uint8_t>();
+    b.seedmask = ctx->seedmask.as<uint8_t>();
+    b.stamp = ctx->stamp.as<uint32_t>();
+    // the gradient of the image, zeroed nowhere
+    LabelBuffers nb = b;
+    nb.seedmask = nullptr;
+    CHECK_HIP(launch_grad_field64(g, nb, s));
+    CHECK_HIP(launch_ers_seeds(g, d_values, b.seedmask, b.label, counts, s));
+    int sweeps = 0;
+    rc = run_certain_rounds(ctx, g, d_values, b, true, &sweeps);
+    if (rc) return rc;
+    CHECK_HIP(launch_ers_state(g, b.label, b.dest, counts, s));
+    // pass 1: label2 = the certain labels, and the label each streamline reaches
+    CHECK_HIP(hipMemcpyAsync(b.label2, b.label, sizeof(int32_t) * V, hipMemcpyDeviceToDevice, s));
+    CHECK_HIP(launch_ers_trace(g, b.gradd, b.dest, b.label, b.label2, grad_thr, err_thr, max_iters, counts, 2, s));
+    CHECK_HIP(launch_ers_mark(g, b.label2, b.dest, counts, s));
+    // re-integration of the rest: label = pass 1's labels, then the label each streamline reaches
+    CHECK_HIP(hipMemcpyAsync(b.label, b.label2, sizeof(int32_t) * V, hipMemcpyDeviceToDevice, s));
+    CHECK_HIP(launch_ers_trace(g, b.gradd, b.dest, b.label2, b.label, grad_thr, err_thr, 500, counts, 4, s));
+    CHECK_HIP(hipMemcpyAsync(labels_out, b.label, sizeof(int32_t) * V,
+                             dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    rc = read_ctl(ctx, 0, kCtlWords);
+    if (rc) return rc;
+    if (stats_out)
+        for (int k = 0; k < 5; k++) stats_out[k] = (int64_t)ctx->h_ctl[kCtlIntStats + k];
     return MSCEER_OK;
 }

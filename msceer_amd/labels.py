@@ -219,3 +219,27 @@ def constrained_grad(values: np.ndarray, restriction: np.ndarray, ctx: native.Co
         if own:
             ctx.close()
     return out.reshape(2 * Z - 1, 2 * Y - 1, 2 * X - 1)
+
+
+def expanding_region_stop(values: np.ndarray, err: float = 1e-3, gthr: float = 1e-6, iters: int = 4096,
+                          ctx: native.Context | None = None):
+    """NumericIntegratorExpandingRegionStop (minima) of a (Z, Y, X) float32
+    volume on the GPU: (labels int32 (Z, Y, X): vertex id of the minimum or
+    -1, stats: minima, certain vertices, pass-1 streamlines, unambiguous
+    vertices, re-integrated vertices)."""
+    v = np.ascontiguousarray(values, np.float32)
+    Z, Y, X = v.shape
+    lab = np.empty(v.shape, np.int32)
+    st = np.zeros(5, np.int64)
+    own = ctx is None
+    ctx = ctx or native.Context()
+    try:
+        native.check(native.lib().msceer_expanding_region_stop(ctx.handle, v.ctypes.data_as(ctypes.c_void_p), X, Y, Z,
+                                                               err, gthr, iters,
+                                                               lab.ctypes.data_as(ctypes.c_void_p),
+                                                               st.ctypes.data_as(ctypes.c_void_p), 0),
+                     "msceer_expanding_region_stop")
+    finally:
+        if own:
+            ctx.close()
+    return lab, st

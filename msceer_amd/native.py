@@ -57,6 +57,8 @@ SIGNATURES = {
     "msceer_batch": (ctypes.c_int, [_P, _P, _I32, _I32, ctypes.c_float, ctypes.c_float, _I32, ctypes.c_float, _U32]),
     "msceer_batch_grad": (ctypes.c_int, [_P, _P, _I32, _I64, _U32]),
     "msceer_msc_base": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _P, _P, _U32]),
+    "msceer_expanding_region_stop": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, ctypes.c_float, ctypes.c_float, _I32,
+                                                    _P, _P, _U32]),
     "msceer_last_batch_stats": (ctypes.c_int, [_P, _P]),
     "msceer_cnn_num_params": (ctypes.c_int, []),
     "msceer_cnn_create": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.POINTER(_P)]),

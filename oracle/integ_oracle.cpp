@@ -808,6 +808,165 @@ void saddle_lines(const Grid& g, const CellMesh& m, const float* f, const unsign
     if (stats) std::copy(st, st + 8, stats);
 }
 
+
+// ---- NumericIntegratorExpandingRegionStop ---------------------------------
+//
+// gi_numeric_integrator_expanding_region_stop.h:63-440 with
+// AdaptiveEulerAdvector3D<F, -1> and IndexCompareLessThan (minima, flow
+// downhill), run with one thread.  SameBdry6 neighbours: both neighbours
+// along an axis, and only where the vertex is interior along it
+// (gi_regular_grid.h:232-265).
+int same_bdry6(const Grid& g, idx_t id, idx_t out[6]) {
+    const V3l c = g.coords(id);
+    const idx_t st[3] = {1, g.X, g.X * g.Y};
+    int k = 0;
+    for (int a = 0; a < 3; a++)
+        if (c.v[a] > 0 && c.v[a] < g.n[a] - 1) {
+            out[k++] = id + st[a];
+            out[k++] = id - st[a];
+        }
+    return k;
+}
+
+// mode 0: the reference's mixed arithmetic, one thread's order (pass 1 with
+// path compression: every path vertex takes the label the path ends on,
+// later paths overwrite earlier ones; the re-integration reads the labels as
+// it leaves them).  mode 1: the same in FP64.  mode 2 (FP64, no visiting
+// order): every non-certain vertex traced alone in pass 1, and the
+// re-integration reads pass 1's labels.  labels: the minimum's vertex id or
+// -1.  stats: extrema, certain vertices, pass-1 streamlines, unambiguous
+// vertices, re-integrated vertices.
+void expanding_region_stop(const Grid& g, const float* f, float ethr, float gthr, int max_iters, int mode,
+                           int* label, long long* stats) {
+    const idx_t n = g.X * g.Y * g.Z;
+    std::vector<V3d> grad;
+    gradient_field(g, f, grad);  // ComputeGradFromImage(1); no gradient is zeroed
+    std::vector<int> certain((size_t)n, -1);
+    std::vector<idx_t> dest((size_t)n, -1);
+    // extrema: no SameBdry6 neighbour before the vertex (:94-107, :218-233)
+    std::vector<idx_t> extrema;
+    for (idx_t i = 0; i < n; i++) {
+        idx_t nb[6];
+        const int k = same_bdry6(g, i, nb);
+        bool ext = true;
+        for (int j = 0; j < k && ext; j++)
+            if (before(f, nb[j], i)) ext = false;
+        if (ext) {
+            dest[(size_t)i] = i;
+            extrema.push_back(i);
+        }
+    }
+    // certain regions, one priority flood per extremum in order (:153-180, :235-251)
+    struct After {
+        const float* f;
+        bool operator()(idx_t a, idx_t b) const { return is_greater(f, a, b); }  // lowest on top
+    };
+    long long ncertain = 0;
+    for (size_t m = 0; m < extrema.size(); m++) {
+        const idx_t start = extrema[m];
+        certain[(size_t)start] = (int)m;
+        std::set<idx_t> seen;
+        std::priority_queue<idx_t, std::vector<idx_t>, After> front(After{f});
+        seen.insert(start);
+        auto enqueue_later = [&](idx_t t) {
+            idx_t nb[6];
+            const int k = same_bdry6(g, t, nb);
+            for (int j = 0; j < k; j++) {
+                const idx_t u = nb[j];
+                if (certain[(size_t)u] == -1 && before(f, t, u) && !seen.count(u)) {
+                    seen.insert(u);
+                    front.push(u);
+                }
+            }
+        };
+        enqueue_later(start);
+        while (!front.empty()) {
+            const idx_t cur = front.top();
+            front.pop();
+            // Inspect_Higher_Certains (:113-151): the earlier neighbours all certain, with one label
+            idx_t nb[6];
+            const int k = same_bdry6(g, cur, nb);
+            int lab = -1;
+            bool has = false, ok = true;
+            for (int j = 0; j < k && ok; j++) {
+                if (!before(f, nb[j], cur)) continue;
+                const int c = certain[(size_t)nb[j]];
+                if (c < 0) ok = false;
+                else if (!has) { lab = c; has = true; }
+                else if (lab != c) ok = false;
+            }
+            if (ok && has) {
+                certain[(size_t)cur] = lab;
+                dest[(size_t)cur] = start;
+                enqueue_later(cur);
+            }
+        }
+    }
+    for (idx_t i = 0; i < n; i++) ncertain += certain[(size_t)i] != -1;
+    auto run = [&](auto fp64) {
+        constexpr bool kFp64 = decltype(fp64)::value;
+        std::vector<Dest> stop((size_t)n);
+        for (idx_t i = 0; i < n; i++) stop[(size_t)i] = certain[(size_t)i] != -1 ? CERTAIN_TERMINAL : UNASSIGNED;
+        const Checker chk{stop.data(), false, &g};  // TerminateNearCertain (gi_advection_checker.h:183-208)
+        std::vector<idx_t> path;
+        long long traced = 0;
+        // pass 1 (:252-337)
+        const std::vector<idx_t> before_pass1 = dest;
+        for (idx_t i = 0; i < n; i++) {
+            if (certain[(size_t)i] != -1) continue;
+            if (mode != 2 && dest[(size_t)i] != -1) continue;
+            path.clear();
+            path.push_back(i);
+            Event ev;
+            const idx_t end = trace<kFp64>(g, grad, chk, gthr, ethr, max_iters, i, &path, &ev);
+            traced++;
+            if (mode == 2) {
+                dest[(size_t)i] = before_pass1[(size_t)end];
+            } else {
+                const idx_t lab = dest[(size_t)end];
+                for (idx_t p : path) dest[(size_t)p] = lab;
+            }
+        }
+        // unambiguous vertices become certain (:348-373)
+        long long unamb = 0;
+        for (idx_t i = 0; i < n; i++) {
+            if (certain[(size_t)i] != -1) continue;
+            idx_t nb[6];
+            const int k = same_bdry6(g, i, nb);
+            bool same = true;
+            for (int j = 0; j < k && same; j++)
+                if (dest[(size_t)nb[j]] != dest[(size_t)i]) same = false;
+            if (same) {
+                certain[(size_t)i] = 1;
+                stop[(size_t)i] = CERTAIN_TERMINAL;
+                unamb++;
+            }
+        }
+        // re-integration with a budget of 500 iterations (:380-420)
+        const std::vector<idx_t> after_pass1 = dest;
+        long long re = 0;
+        for (idx_t i = 0; i < n; i++) {
+            if (certain[(size_t)i] != -1) continue;
+            Event ev;
+            const idx_t end = trace<kFp64>(g, grad, chk, gthr, ethr, 500, i, nullptr, &ev);
+            dest[(size_t)i] = mode == 2 ? after_pass1[(size_t)end] : dest[(size_t)end];
+            re++;
+        }
+        if (stats) {
+            stats[2] = traced;
+            stats[3] = unamb;
+            stats[4] = re;
+        }
+    };
+    if (mode == 0) run(std::false_type{});
+    else run(std::true_type{});
+    for (idx_t i = 0; i < n; i++) label[i] = (int)dest[(size_t)i];
+    if (stats) {
+        stats[0] = (long long)extrema.size();
+        stats[1] = ncertain;
+    }
+}
+
 }  // namespace
 
 extern "C" {
@@ -1023,6 +1182,15 @@ long long oracle_conforming_vertices(const float* f, long long X, long long Y, l
 
 // The whole accurate ascending pipeline; returns the .grad and the cleaned
 // labels.  mode: 0 reference arithmetic, 1 FP64, 2 exact FP64 destinations.
+// NumericIntegratorExpandingRegionStop (see expanding_region_stop above).
+// labels: X*Y*Z int32; stats: 5 entries (may be null).
+int oracle_expanding_region_stop(const float* f, long long X, long long Y, long long Z, float ethr, float gthr,
+                                 int max_iters, int mode, int* labels, long long* stats) {
+    const Grid g(X, Y, Z);
+    expanding_region_stop(g, f, ethr, gthr, max_iters, mode, labels, stats);
+    return 0;
+}
+
 int oracle_ondemand_asc(const float* f, long long X, long long Y, long long Z, float ethr, float gthr, int max_iters,
                         float presimp, int mode, unsigned char* grad, int* labels, signed char* restriction_out,
                         long long* stats) {

@@ -111,3 +111,16 @@ def load_msc(name):
     basins = np.frombuffer(zlib.decompress(d["basins_z"].tobytes()), np.int32).reshape(f.shape)
     mounts = np.frombuffer(zlib.decompress(d["mounts_z"].tobytes()), np.int32).reshape(f.shape)
     return f, grad, basins, mounts, d["counts"]
+
+
+def load_ers(name):
+    """(values, (err, gthr, iters), one-thread reference labels, extrema,
+    8-thread mismatch fractions) of NumericIntegratorExpandingRegionStop on a
+    labelling case."""
+    from cases import ONDEMAND_CASES
+    shape, gen, err, gthr, iters, presimp = ONDEMAND_CASES[name]
+    f = np.ascontiguousarray(gen(), dtype=np.float32)
+    d = np.load(os.path.join(HERE, "ers", name + ".npz"))
+    assert tuple(d["shape"]) == f.shape and str(d["input_sha256"]) == synth.sha256(f)
+    lab = np.frombuffer(zlib.decompress(d["labels_z"].tobytes()), np.int32).reshape(f.shape)
+    return f, (err, gthr, iters), lab, int(d["extrema"]), d["threads8_mismatch"]

@@ -301,3 +301,18 @@ def msc_base(grad, shape_zyx):
     counts = np.zeros(4, np.int64)
     h.oracle_msc_base(_p(g), X, Y, Z, _p(basins), _p(mounts), _p(counts))
     return basins.reshape(shape_zyx), mounts.reshape(shape_zyx), counts
+
+
+def expanding_region_stop(f, err, gthr, iters, mode=0):
+    """NumericIntegratorExpandingRegionStop (minima): (labels int32 (Z,Y,X),
+    stats[5]).  mode 0: reference arithmetic and one thread's order; 1: the
+    same in FP64; 2: FP64, every vertex traced alone, the re-integration
+    reading pass 1's labels."""
+    h = lib()
+    h.oracle_expanding_region_stop.argtypes = [_P, _L, _L, _L, _F, _F, _I, _I, _P, _P]
+    f = np.ascontiguousarray(f, np.float32)
+    Z, Y, X = f.shape
+    lab = np.empty(f.shape, np.int32)
+    st = np.zeros(5, np.int64)
+    h.oracle_expanding_region_stop(_p(f), X, Y, Z, float(err), float(gthr), int(iters), int(mode), _p(lab), _p(st))
+    return lab, st
