@@ -429,6 +429,26 @@ idx_t trace_down(const CellMesh& m, const unsigned char* g, idx_t c) {
     }
 }
 
+// rec_tu (gi_extrema_region_builder.h:330-352): follow the V-path up from a
+// hex (its paired quad's other hex) to a critical hex; -1 where a hex is
+// paired with a quad on the domain boundary (the reference reads past its one
+// cofacet there).
+idx_t trace_up(const CellMesh& m, const unsigned char* g, idx_t c) {
+    while (true) {
+        if (m.critical(g, c)) return c;
+        const idx_t p = m.pair(g, c);
+        if (m.critical(g, p)) return p;
+        idx_t q[3];
+        m.coords(p, q);
+        int a = 0;
+        while (a < 3 && (q[a] & 1)) a++;  // the quad's even axis: its cofacets lie along it
+        const idx_t nxt = (c == p + m.off(a)) ? p - m.off(a) : p + m.off(a);
+        const idx_t along = (c == p + m.off(a)) ? q[a] - 1 : q[a] + 1;
+        if (along < 0 || along >= m.M[a]) return -1;
+        c = nxt;
+    }
+}
+
 // cellValue / lessThan (gi_topological_max_vertex_mesh_function.h:45-62):
 // value of the highest vertex; ties by highest-vertex id then cell id.
 struct CellFunc {
@@ -464,6 +484,7 @@ struct MinGraph {
         }
     };
     const CellFunc* fn;
+    bool min_hier = true;  // MINIMAL (true) or MAXIMAL: which extremum of a merge stays representative
     std::vector<Ext> ext;
     std::unordered_map<idx_t, int> id;
     std::priority_queue<Arc, std::vector<Arc>, ArcOrder> arcs;
@@ -490,7 +511,7 @@ struct MinGraph {
     void merge(int l1, int l2) {
         Ext& e1 = ext[l1];
         Ext& e2 = ext[l2];
-        if (fn->less(e1.cell, e2.cell) ^ true) e1.rep = e2.rep;
+        if (fn->less(e1.cell, e2.cell) ^ min_hier) e1.rep = e2.rep;
         else e2.rep = e1.rep;
     }
     void cancel(Arc m) {
@@ -527,11 +548,15 @@ struct MinGraph {
 // neighbours carry that region's label (GridSimplifiedExtremalRegionBuilder,
 // gi_extrema_region_builder.h:806-1042).  The fixed point does not depend on
 // the order regions are grown in.
+// desc: the descending order of ComparerDSC (IndexCompareGreaterThan,
+// gi_numeric_integrator_expanding_region_stop.h:41-59): a vertex comes
+// before another when it is greater (value, then id); flooded from the top.
 void certain_regions(const Grid& g, const float* f, const std::vector<std::vector<idx_t>>& seeds,
-                     const std::vector<int>& seed_label, int* lab) {
+                     const std::vector<int>& seed_label, int* lab, bool desc = false) {
     const idx_t n = g.X * g.Y * g.Z;
     for (idx_t i = 0; i < n; i++) lab[i] = -1;
-    auto cmp = [f](idx_t a, idx_t b) { return is_greater(f, a, b); };  // lowest on top
+    auto before = [desc](const float* ff, idx_t a, idx_t b) { return desc ? is_greater(ff, a, b) : ::before(ff, a, b); };
+    auto cmp = [f, desc](idx_t a, idx_t b) { return desc ? is_greater(f, b, a) : is_greater(f, a, b); };  // earliest on top
     for (size_t r = 0; r < seeds.size(); r++) {
         std::set<idx_t> enq;
         std::priority_queue<idx_t, std::vector<idx_t>, decltype(cmp)> pq(cmp);
@@ -564,7 +589,8 @@ void certain_regions(const Grid& g, const float* f, const std::vector<std::vecto
 }
 
 // IsolatedCCRegionRemoverNEW::ComputeOutput (gi_isolated_region_remover.h:155-192).
-void clean_regions(const Grid& g, const float* f, int* lab) {
+void clean_regions(const Grid& g, const float* f, int* lab, bool desc = false) {
+    auto before = [desc](const float* ff, idx_t a, idx_t b) { return desc ? is_greater(ff, a, b) : ::before(ff, a, b); };
     const idx_t n = g.X * g.Y * g.Z;
     std::vector<idx_t> dst((size_t)n);
     for (idx_t i = 0; i < n; i++) dst[(size_t)i] = i;
@@ -1023,6 +1049,62 @@ int oracle_certain_regions(const float* f, long long X, long long Y, long long Z
     return (int)seeds.size();
 }
 
+// The descending side (ondemandaccurate.cxx:913-940): maxima (critical hexes)
+// joined by interior 2-saddles (critical quads off the boundary, traced up
+// through both cofacet hexes, + side first: trace_up_2saddle,
+// gi_extrema_region_builder.h:354-372), UFMergeGraph MAXIMAL +
+// SimplifyToThreshold, seeds at each maximum's highest vertex grouped by
+// representative (a region is named by its representative's highest vertex),
+// then the certain regions in the descending order.
+int oracle_certain_regions_dsc(const float* f, long long X, long long Y, long long Z, const unsigned char* grad,
+                               float presimp, int* labels, signed char* seeds_out) {
+    CellMesh m(X, Y, Z);
+    Grid g(X, Y, Z);
+    const idx_t C = m.M[0] * m.M[1] * m.M[2];
+    std::vector<unsigned char> maxv((size_t)C), minv((size_t)C);
+    oracle_maxmin(f, X, Y, Z, maxv.data(), minv.data());
+    CellFunc fn{&m, f, maxv.data()};
+    MinGraph mg;
+    mg.fn = &fn;
+    mg.min_hier = false;
+    std::vector<idx_t> sad2;
+    for (idx_t c = 0; c < C; c++) {
+        if (!m.critical(grad, c)) continue;
+        const int d = m.dim(c);
+        if (d == 3) mg.add_node(c);
+        else if (d == 2 && m.bv(c) == 0) sad2.push_back(c);
+    }
+    for (idx_t s : sad2) {
+        idx_t q[3];
+        m.coords(s, q);
+        int a = 0;
+        while (a < 3 && (q[a] & 1)) a++;
+        const idx_t h1 = trace_up(m, grad, s + m.off(a)), h2 = trace_up(m, grad, s - m.off(a));
+        if (h1 != h2 && h1 >= 0 && h2 >= 0) mg.add_arc(h1, h2, s);
+    }
+    mg.simplify(presimp);
+    std::map<int, std::vector<idx_t>> groups;
+    for (int l = 0; l < (int)mg.ext.size(); l++) groups[mg.ext[l].rep].push_back(fn.highest(mg.ext[l].cell));
+    std::vector<std::vector<idx_t>> seeds;
+    std::vector<int> seed_label;
+    for (auto& kv : groups) {
+        seeds.push_back(kv.second);
+        seed_label.push_back((int)fn.highest(mg.ext[kv.first].cell));
+    }
+    certain_regions(g, f, seeds, seed_label, labels, true);
+    if (seeds_out) {
+        for (idx_t i = 0; i < X * Y * Z; i++) seeds_out[i] = 0;
+        for (auto& sd : seeds) for (idx_t v : sd) seeds_out[v] = 1;
+    }
+    return (int)seeds.size();
+}
+
+int oracle_clean_dsc(const float* f, long long X, long long Y, long long Z, int* labels) {
+    Grid g(X, Y, Z);
+    clean_regions(g, f, labels, true);
+    return 0;
+}
+
 // Path-compressing integration from certain regions (labels in/out).
 // seeds: per-vertex flag of region seeds (their gradient is zeroed).
 // sign -1 integrates ascending manifolds (downhill), +1 descending (uphill).
@@ -1244,28 +1326,40 @@ int oracle_saddle_lines(const float* f, long long X, long long Y, long long Z, c
 // digitized 1-manifolds (ASC then DSC); the conforming lower stars; ldir and
 // flags.  labels (V) are written only when need_asc2 (else left as they are);
 // stats: [0..3] the labelling's, [4..11] ASC lines, [12..19] DSC lines.
-int oracle_ondemand(const float* f, long long X, long long Y, long long Z, float ethr, float gthr, int max_iters,
-                    float presimp, int need_asc1, int need_dsc1, int need_asc2, int mode, unsigned char* grad,
-                    int* labels, signed char* restriction_out, long long* stats) {
+// OnDemandAccurate with any of needASC1, needDSC1, needASC2, needDSC2
+// (ondemandaccurate.cxx:728-1157).  labels_dsc (may be null when dsc2 is 0):
+// the descending labels.  The gradient field is shared: the descending
+// integration and the 1-manifolds see it zeroed at every seed so far.
+int oracle_ondemand2(const float* f, long long X, long long Y, long long Z, float ethr, float gthr, int max_iters,
+                     float presimp, int need_asc1, int need_dsc1, int need_asc2, int need_dsc2, int mode,
+                     unsigned char* grad, int* labels, int* labels_dsc, signed char* restriction_out,
+                     long long* stats) {
     CellMesh m(X, Y, Z);
     const idx_t C = m.M[0] * m.M[1] * m.M[2];
     const idx_t V = X * Y * Z;
     std::vector<unsigned char> base((size_t)C);
     oracle_lstar_pairs(f, X, Y, Z, nullptr, base.data());
-    std::vector<signed char> r((size_t)C, 0), seeds;
+    std::vector<signed char> r((size_t)C, 0), seeds((size_t)V, 0);
     long long st[20] = {0};
     if (need_asc2) {
-        seeds.assign((size_t)V, 0);
         oracle_certain_regions(f, X, Y, Z, base.data(), presimp, labels, seeds.data());
         oracle_integrate(f, X, Y, Z, seeds.data(), ethr, gthr, max_iters, mode, labels, st);
         oracle_clean(f, X, Y, Z, labels);
         oracle_min_boundary(labels, X, Y, Z, r.data());
     }
+    if (need_dsc2) {
+        std::vector<signed char> ds((size_t)V, 0);
+        oracle_certain_regions_dsc(f, X, Y, Z, base.data(), presimp, labels_dsc, ds.data());
+        for (idx_t i = 0; i < V; i++) seeds[(size_t)i] |= ds[(size_t)i];
+        oracle_integrate_signed(f, X, Y, Z, seeds.data(), ethr, gthr, max_iters, 1, mode, labels_dsc, st + 16);
+        oracle_clean_dsc(f, X, Y, Z, labels_dsc);
+        oracle_max_boundary(f, labels_dsc, X, Y, Z, r.data());
+    }
     const int lmode = mode == 0 ? 0 : 1;
-    const signed char* zs = need_asc2 ? seeds.data() : nullptr;
+    const signed char* zs = (need_asc2 || need_dsc2) ? seeds.data() : nullptr;
     if (need_asc1) oracle_saddle_lines(f, X, Y, Z, base.data(), zs, ethr, gthr, max_iters, 1, lmode, r.data(), st + 4);
     if (need_dsc1) oracle_saddle_lines(f, X, Y, Z, base.data(), zs, ethr, gthr, max_iters, 0, lmode, r.data(), st + 12);
-    if (need_asc2 || need_asc1 || need_dsc1) {
+    if (need_asc2 || need_dsc2 || need_asc1 || need_dsc1) {
         const long long nv = oracle_conforming_vertices(f, X, Y, Z, r.data(), nullptr);
         std::vector<long long> vs((size_t)nv);
         oracle_conforming_vertices(f, X, Y, Z, r.data(), vs.data());
@@ -1278,6 +1372,13 @@ int oracle_ondemand(const float* f, long long X, long long Y, long long Z, float
     if (restriction_out) std::copy(r.begin(), r.end(), restriction_out);
     if (stats) std::copy(st, st + 20, stats);
     return 0;
+}
+
+int oracle_ondemand(const float* f, long long X, long long Y, long long Z, float ethr, float gthr, int max_iters,
+                    float presimp, int need_asc1, int need_dsc1, int need_asc2, int mode, unsigned char* grad,
+                    int* labels, signed char* restriction_out, long long* stats) {
+    return oracle_ondemand2(f, X, Y, Z, ethr, gthr, max_iters, presimp, need_asc1, need_dsc1, need_asc2, 0, mode,
+                            grad, labels, nullptr, restriction_out, stats);
 }
 
 }  // extern "C"

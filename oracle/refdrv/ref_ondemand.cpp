@@ -14,8 +14,10 @@
 // The same RefMesh subclass as ref_steepest.cpp is used for the dead loop at
 // include/gi_modified_robins.h:1005-1013.
 //
-// usage: ref_ondemand X Y Z in.raw out_prefix err gthr iters presimp [threads [asc1 dsc1 asc2]]
+// usage: ref_ondemand X Y Z in.raw out_prefix err gthr iters presimp [threads [asc1 dsc1 asc2 [dsc2]]]
 // writes (int32 per vertex) out_prefix.certain, .integrated, .cleaned (when asc2);
+//        .certain_dsc, .integrated_dsc, .cleaned_dsc (when dsc2: the needDSC2
+//        branch, ondemandaccurate.cxx:913-991, after the needASC2 one);
 //        (int8 per cell) out_prefix.restriction; out_prefix.grad.
 //
 // Built with -DREF_LINES (target ref_lines) it also runs the needASC1 /
@@ -63,6 +65,8 @@ typedef RegularGridTrilinearFunction GridFuncType;
 typedef RefMesh MeshType;
 typedef IndexCompareLessThan<GridFuncType> ComparerASC;
 typedef NumericIntegratorPathCompressingToTerminal<AdaptiveEulerAdvector3D<GridFuncType, -1>, GridFuncType> IntegratorTypeASC;
+typedef IndexCompareGreaterThan<GridFuncType> ComparerDSC;
+typedef NumericIntegratorPathCompressingToTerminal<AdaptiveEulerAdvector3D<GridFuncType, 1>, GridFuncType> IntegratorTypeDSC;
 typedef DiscreteGradientLabeling<MeshType> GradType;
 typedef RegularGridMaxMinVertexLabeling3D<MeshType, GridFuncType> MaxVLType;
 typedef MyRobinsNoalloc<MeshType, MaxVLType, GradType, 4, 6> RobinsType;
@@ -144,6 +148,7 @@ int main(int argc, char** argv) {
     int threads = argc > 10 ? atoi(argv[10]) : 1;
     const int need_asc1 = argc > 13 ? atoi(argv[11]) : 0, need_dsc1 = argc > 13 ? atoi(argv[12]) : 0;
     const int need_asc2 = argc > 13 ? atoi(argv[13]) : 1;
+    const int need_dsc2 = argc > 14 ? atoi(argv[14]) : 0;
 #ifndef REF_LINES
     if (need_asc1 || need_dsc1) {
         fprintf(stderr, "1-manifolds need the ref_lines build\n");
@@ -173,15 +178,19 @@ int main(int argc, char** argv) {
     RobinsType* constrained = new RobinsType(mesh, maxv, restriction, base_grad);
 
     double t_integ = 0;
+    SimplifiedExtremumGraph<MeshType, MeshFuncType, GradType>* ext_graph = nullptr;
+    VertexLabelingToBoundaryLabeling<int, MaxVLType>* edge_map = nullptr;
+    if (need_asc2 || need_dsc2) {
+        // ondemandaccurate.cxx:806-827
+        ext_graph = new SimplifiedExtremumGraph<MeshType, MeshFuncType, GradType>(mesh, topo_func, base_grad);
+        typedef SimplifiedExtremumGraph<MeshType, MeshFuncType, GradType> EG;
+        ext_graph->SetMode(need_asc2 && need_dsc2 ? EG::EXTGRAPHMODE::BOTH
+                                                  : (need_asc2 ? EG::EXTGRAPHMODE::MINS : EG::EXTGRAPHMODE::MAXS));
+        ext_graph->ComputeMinMapFromGradient(presimp);
+        edge_map = new VertexLabelingToBoundaryLabeling<int, MaxVLType>(mesh, restriction);
+        edge_map->InitializeFirst();
+    }
     if (need_asc2) {
-    SimplifiedExtremumGraph<MeshType, MeshFuncType, GradType>* ext_graph =
-        new SimplifiedExtremumGraph<MeshType, MeshFuncType, GradType>(mesh, topo_func, base_grad);
-    ext_graph->SetMode(SimplifiedExtremumGraph<MeshType, MeshFuncType, GradType>::EXTGRAPHMODE::MINS);
-    ext_graph->ComputeMinMapFromGradient(presimp);
-
-    VertexLabelingToBoundaryLabeling<int, MaxVLType>* edge_map =
-        new VertexLabelingToBoundaryLabeling<int, MaxVLType>(mesh, restriction);
-    edge_map->InitializeFirst();
 
     std::unordered_map<INDEX_TYPE, INT_TYPE> extmap;
     for (auto p : ext_graph->mMinGraph->mCellIndexToListIdMap)
@@ -203,6 +212,26 @@ int main(int argc, char** argv) {
     dump_int(out + ".cleaned", integrator->GetOutputLabels());
 
     edge_map->ComputeMINBoundary(integrator->GetOutputLabels());
+    }
+    if (need_dsc2) {
+        // ondemandaccurate.cxx:913-991
+        std::unordered_map<INDEX_TYPE, INT_TYPE> extmap;
+        for (auto p : ext_graph->mMaxGraph->mCellIndexToListIdMap)
+            extmap[maxv->Cell2HighestVertex(p.first)] = ext_graph->mMaxGraph->Representative(p.second);
+        GridSimplifiedExtremalRegionBuilder<ComparerDSC, GridFuncType, MeshType>* builder =
+            new GridSimplifiedExtremalRegionBuilder<ComparerDSC, GridFuncType, MeshType>(func, grid, mesh);
+        builder->BeginIntegration(extmap);
+        dump_int(out + ".certain_dsc", builder->GetOutputLabels());
+        auto ti = std::chrono::steady_clock::now();
+        IntegratorTypeDSC* integrator = new IntegratorTypeDSC(func, grid, err, gthr, iters);
+        integrator->BeginIntegration(builder->GetIdMap(), builder->GetOutputLabels(), true);
+        t_integ += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - ti).count();
+        dump_int(out + ".integrated_dsc", integrator->GetOutputLabels());
+        IsolatedCCRegionRemoverNEW<ComparerDSC, GridFuncType>* cleaner =
+            new IsolatedCCRegionRemoverNEW<ComparerDSC, GridFuncType>(func, integrator->GetOutputLabels());
+        cleaner->ComputeOutput();
+        dump_int(out + ".cleaned_dsc", integrator->GetOutputLabels());
+        edge_map->ComputeMAXBoundary(integrator->GetOutputLabels(), maxv);
     }
     long long lines_asc = 0, lines_dsc = 0;
 #ifdef REF_LINES

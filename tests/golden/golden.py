@@ -124,3 +124,31 @@ def load_ers(name):
     assert tuple(d["shape"]) == f.shape and str(d["input_sha256"]) == synth.sha256(f)
     lab = np.frombuffer(zlib.decompress(d["labels_z"].tobytes()), np.int32).reshape(f.shape)
     return f, (err, gthr, iters), lab, int(d["extrema"]), d["threads8_mismatch"]
+
+
+def dsc2_case_names():
+    return sorted(os.path.splitext(f)[0] for f in os.listdir(os.path.join(HERE, "dsc2")) if f.endswith(".npz"))
+
+
+def load_dsc2(tag):
+    """(values, (err, gthr, iters, presimp), flags (asc1, dsc1, asc2, dsc2), dict of
+    the reference's descending certain / integrated / cleaned labels and
+    restriction, .grad SHA-256) of a needDSC2 case."""
+    d = np.load(os.path.join(HERE, "dsc2", tag + ".npz"))
+    from cases import ONDEMAND_CASES
+    name = tag.rsplit("_", 1)[0]
+    shape, gen, err, gthr, iters, presimp = ONDEMAND_CASES[name]
+    f = np.ascontiguousarray(gen(), dtype=np.float32)
+    assert tuple(d["shape"]) == f.shape and str(d["input_sha256"]) == synth.sha256(f)
+    out = {k: np.frombuffer(zlib.decompress(d[k + "_z"].tobytes()), np.int32).reshape(f.shape)
+           for k in ("certain", "integrated", "cleaned")}
+    out["restriction"] = np.frombuffer(zlib.decompress(d["restriction_z"].tobytes()), np.int8)
+    return f, (err, gthr, iters, presimp), tuple(int(x) for x in d["flags"]), out, str(d["grad_sha256"])
+
+
+def same_partition(a, b):
+    """Do two label arrays split the vertices the same way (ids may differ)?"""
+    a = np.asarray(a).ravel().astype(np.int64)
+    b = np.asarray(b).ravel().astype(np.int64)
+    pairs = np.unique(np.stack([a, b], 1), axis=0)
+    return len(pairs) == len(np.unique(a)) == len(np.unique(b))

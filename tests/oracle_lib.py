@@ -316,3 +316,20 @@ def expanding_region_stop(f, err, gthr, iters, mode=0):
     st = np.zeros(5, np.int64)
     h.oracle_expanding_region_stop(_p(f), X, Y, Z, float(err), float(gthr), int(iters), int(mode), _p(lab), _p(st))
     return lab, st
+
+
+def ondemand2(f, err, gthr, iters, presimp, asc1=0, dsc1=0, asc2=1, dsc2=0, mode=0):
+    """OnDemandAccurate with any accuracy branches: (grad, labels_asc,
+    labels_dsc, restriction, stats[20])."""
+    h = lib()
+    h.oracle_ondemand2.argtypes = [_P, _L, _L, _L, _F, _F, _I, _F, _I, _I, _I, _I, _I, _P, _P, _P, _P, _P]
+    f = np.ascontiguousarray(f, np.float32)
+    Z, Y, X = f.shape
+    grad = np.empty(ncells(X, Y, Z), np.uint8)
+    la = np.full(f.size, -1, np.int32)
+    ld = np.full(f.size, -1, np.int32)
+    r = np.empty(ncells(X, Y, Z), np.int8)
+    st = np.zeros(20, np.int64)
+    h.oracle_ondemand2(_p(f), X, Y, Z, float(err), float(gthr), int(iters), float(presimp), int(asc1), int(dsc1),
+                       int(asc2), int(dsc2), int(mode), _p(grad), _p(la), _p(ld), _p(r), _p(st))
+    return grad, la.reshape(f.shape), ld.reshape(f.shape), r, st
