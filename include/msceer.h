@@ -94,6 +94,11 @@ typedef struct msceer_label_stats {
     float ms_lines;             /* the 1-manifold stage */
     int32_t line_alone;         /* 1-manifold streamlines traced alone, in order */
     int32_t line_overflows;     /* windows cut short where the footprint pool ran out */
+    int64_t maxima;             /* needDSC2: critical hexes of the steepest pairing */
+    int64_t saddles2;           /* needDSC2: critical quads */
+    int64_t regions_dsc;        /* needDSC2: descending certain regions after simplification */
+    int32_t certain_sweeps_dsc; /* needDSC2: certain-region rounds */
+    float ms_dsc;               /* needDSC2: the whole descending branch */
 } msceer_label_stats;
 
 int msceer_version(void);
@@ -181,6 +186,7 @@ int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t nx, int64_
 #define MSCEER_NEED_ASC1 0x1u /* ascending 1-manifolds: ReIntegrateUpFrom2Saddles */
 #define MSCEER_NEED_DSC1 0x2u /* descending 1-manifolds: ReIntegrateDownFrom1Saddles */
 #define MSCEER_NEED_ASC2 0x4u /* ascending 3-manifolds: msceer_ondemand_asc's labelling */
+#define MSCEER_NEED_DSC2 0x8u /* descending 3-manifolds: the max-side labelling and its MAX boundary */
 
 /*
  * OnDemandAccurate with any nonzero combination of needASC1, needDSC1 and
@@ -198,13 +204,32 @@ int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t nx, int64_
  * extrema when needASC2 ran first (as the reference's shared field is).
  * restriction_out (cells int8, may be NULL): the restriction labels the
  * conforming gradient was computed under (MIN boundary 1 when needASC2,
- * 4 / 5 on the digitized 1-manifolds).  labels_out, integrate_mask and the
- * STOP flags need needASC2.  Other arguments as msceer_ondemand_asc.
+ * 4 / 5 on the digitized 1-manifolds).  labels_out and integrate_mask need
+ * needASC2; the STOP flags need needASC2 or needDSC2 (msceer_ondemand2), end
+ * every labelling asked for at that stage and return its labels alone.
+ * Other arguments as msceer_ondemand_asc.
  */
 int msceer_ondemand(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz, float err_thr,
                     float grad_thr, int32_t max_iters, float presimp, uint32_t need, const uint8_t* integrate_mask,
                     uint8_t* grad_out, int32_t* labels_out, int8_t* restriction_out, msceer_label_stats* stats,
                     uint32_t flags);
+/*
+ * msceer_ondemand with the descending labels too: labels_dsc_out (X*Y*Z
+ * int32, may be NULL; needs MSCEER_NEED_DSC2) receives the needDSC2 labels
+ * (ondemandaccurate.cxx:913-991: the max extremum graph of the steepest
+ * pairing, GridSimplifiedExtremalRegionBuilder<ComparerDSC>,
+ * NumericIntegratorPathCompressingToTerminal<AdaptiveEulerAdvector3D<F, 1>>,
+ * IsolatedCCRegionRemoverNEW<ComparerDSC>), named by the highest vertex of
+ * each region; their MAX boundary (ComputeMAXBoundary) enters the
+ * restriction.  The gradient field is shared as in the reference: after
+ * needASC2 the descending integration sees it zeroed at the ascending seeds
+ * too.  The descending labels follow the ascending ones' arithmetic: certain
+ * regions bit-exact, the streamlines FP32 first and FP64 where in doubt.
+ */
+int msceer_ondemand2(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz, float err_thr,
+                     float grad_thr, int32_t max_iters, float presimp, uint32_t need, const uint8_t* integrate_mask,
+                     uint8_t* grad_out, int32_t* labels_out, int32_t* labels_dsc_out, int8_t* restriction_out,
+                     msceer_label_stats* stats, uint32_t flags);
 
 /*
  * The numeric integrator alone on given terminals: replaces

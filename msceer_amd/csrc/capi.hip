@@ -319,7 +319,8 @@ int msceer_destroy(msceer_ctx* ctx) {
     ctx->ctl.release();
     ctx->ldir.release();
     DevBuf* label_bufs[] = {&ctx->gradf, &ctx->gradd, &ctx->scan, &ctx->label, &ctx->label2, &ctx->dest, &ctx->seedmask, &ctx->redo,
-                            &ctx->stamp, &ctx->rep, &ctx->rep2, &ctx->restriction, &ctx->host_io, &ctx->mask};
+                            &ctx->stamp, &ctx->rep, &ctx->rep2, &ctx->restriction, &ctx->host_io, &ctx->mask,
+                            &ctx->label_dsc, &ctx->seedmask2, &ctx->line_stamps, &ctx->lines};
     for (DevBuf* b : label_bufs) b->release();
     ctx->host_stage.release();
     if (ctx->h_ctl) (void)hipHostFree(ctx->h_ctl);

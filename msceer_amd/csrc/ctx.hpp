@@ -126,6 +126,8 @@ struct msceer_ctx {
     // accurate 1-manifolds: per-cell line stamps (two words, kept cleared
     // between calls) and the saddle / window scratch
     msceer::DevBuf line_stamps, lines;
+    // descending 3-manifolds: their labels and seed mask
+    msceer::DevBuf label_dsc, seedmask2;
     bool line_stamps_clear = false;
     uint64_t line_stamps_cells = 0;
     msceer::HostBuf host_stage;  // pinned staging of host-side merges (cleanup)

@@ -1,5 +1,6 @@
 // Host side of the extremum graph: persistence simplification of the minima
-// joined by 1-saddles (UFMergeGraph MINIMAL + SimplifyToThreshold,
+// joined by 1-saddles, or of the maxima joined by 2-saddles (UFMergeGraph
+// MINIMAL / MAXIMAL + SimplifyToThreshold,
 // gi_extrema_region_builder.h:28-268).  Inputs are small (one node per
 // critical vertex, one arc per 1-saddle) and the cancellation order is a
 // global priority order, so this runs on the host between GPU stages.
@@ -35,13 +36,16 @@ struct Graph {
     std::vector<int> rep;
     std::priority_queue<Arc, std::vector<Arc>, ArcLater> arcs;
     float thr = 0.f;
+    bool minimal = true;  // MINIMAL: the lower extremum of a merge stays representative; MAXIMAL: the higher
 
-    // cellValue then cell id (for vertex cells the highest vertex is the cell)
+    // lessThan (gi_topological_max_vertex_mesh_function.h:45-62): value of
+    // the highest vertex, then that vertex's id, then the cell id
     bool less(int a, int b) const {
         const MinNode& x = (*nodes)[a];
         const MinNode& y = (*nodes)[b];
         if (x.value < y.value) return true;
         if (y.value < x.value) return false;
+        if (x.vertex != y.vertex) return x.vertex < y.vertex;
         return x.cell < y.cell;
     }
     int find(int l) {
@@ -50,9 +54,10 @@ struct Graph {
         while (rep[l] != r) { const int n = rep[l]; rep[l] = r; l = n; }
         return r;
     }
-    // the later extremum joins the earlier one's set
+    // MergeByVal (gi_extrema_region_builder.h:84-94): the later extremum
+    // joins the earlier one's set (lower for minima, higher for maxima)
     void merge(int a, int b) {
-        if (!less(a, b)) rep[a] = rep[b];
+        if (less(a, b) ^ minimal) rep[a] = rep[b];
         else rep[b] = rep[a];
     }
     void cancel(Arc m) {
@@ -79,10 +84,13 @@ struct Graph {
 
 }  // namespace
 
-std::vector<int> simplify_minima(const std::vector<MinNode>& nodes, const std::vector<SaddleArc>& saddles,
-                                 float threshold) {
+namespace {
+
+std::vector<int> simplify(const std::vector<MinNode>& nodes, const std::vector<SaddleArc>& saddles, float threshold,
+                          bool minimal) {
     Graph g;
     g.nodes = &nodes;
+    g.minimal = minimal;
     g.rep.resize(nodes.size());
     for (size_t i = 0; i < nodes.size(); i++) g.rep[i] = (int)i;
     // node index of a cell: nodes come in ascending cell order
@@ -131,6 +139,18 @@ std::vector<int> simplify_minima(const std::vector<MinNode>& nodes, const std::v
     std::vector<int> out(nodes.size());
     for (size_t i = 0; i < nodes.size(); i++) out[i] = g.find((int)i);
     return out;
+}
+
+}  // namespace
+
+std::vector<int> simplify_minima(const std::vector<MinNode>& nodes, const std::vector<SaddleArc>& saddles,
+                                 float threshold) {
+    return simplify(nodes, saddles, threshold, true);
+}
+
+std::vector<int> simplify_maxima(const std::vector<MinNode>& nodes, const std::vector<SaddleArc>& saddles,
+                                 float threshold) {
+    return simplify(nodes, saddles, threshold, false);
 }
 
 }  // namespace msceer

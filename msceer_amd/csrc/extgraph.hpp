@@ -7,8 +7,9 @@
 namespace msceer {
 
 struct MinNode {
-    int64_t cell;  // critical vertex cell id
-    float value;   // value at that vertex
+    int64_t cell;    // critical cell id (a vertex cell; a hex for the max graph)
+    float value;     // value of its highest vertex
+    int64_t vertex;  // that vertex (order ties: value, then vertex, then cell)
 };
 
 struct SaddleArc {
@@ -20,6 +21,10 @@ struct SaddleArc {
 // Representative node index per node after cancelling every arc whose
 // persistence does not exceed `threshold`.
 std::vector<int> simplify_minima(const std::vector<MinNode>& nodes, const std::vector<SaddleArc>& saddles,
+                                 float threshold);
+// The max graph (UFMergeGraph MAXIMAL): the higher extremum of a merge stays
+// representative; arcs join maxima (min1/min2 hold their cells).
+std::vector<int> simplify_maxima(const std::vector<MinNode>& nodes, const std::vector<SaddleArc>& saddles,
                                  float threshold);
 
 }  // namespace msceer

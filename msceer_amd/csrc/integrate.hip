@@ -46,6 +46,13 @@ __device__ __forceinline__ bool before(const float* f, int64_t a, int64_t b) {
     return b > a;
 }
 
+// The order of a descending labelling (IndexCompareGreaterThan,
+// gi_numeric_integrator_expanding_region_stop.h:41-59): a vertex comes first
+// when it is greater.  desc false: before().
+__device__ __forceinline__ bool ord_before(const float* f, int64_t a, int64_t b, int desc) {
+    return desc ? before(f, b, a) : before(f, a, b);
+}
+
 // ---- gradient field ---------------------------------------------------------
 
 // Coordinates of vertex v in 32-bit arithmetic: every grid here has fewer
@@ -103,6 +110,10 @@ __global__ void zero_seed_gradient_kernel(VGrid g, const uint8_t* __restrict__ s
 
 __global__ void fill_i32_kernel(int32_t* p, int64_t n, int32_t v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
+}
+__global__ void or_u8_kernel(uint8_t* dst, const uint8_t* src, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        dst[i] |= src[i];
 }
 __global__ void fill_u8_kernel(uint8_t* p, int64_t n, uint8_t v) {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) p[i] = v;
@@ -461,7 +472,7 @@ __device__ __forceinline__ uint64_t list_entry(uint32_t v, uint32_t meta) {
 __global__ void __launch_bounds__(kThreads)
 certain_init_kernel(const float* __restrict__ f, VGrid g, const uint8_t* __restrict__ seedmask,
                     uint32_t* __restrict__ cnt, uint64_t* __restrict__ out, uint32_t* __restrict__ out_count,
-                    uint32_t capacity, uint32_t* __restrict__ overflow, int same_bdry) {
+                    uint32_t capacity, uint32_t* __restrict__ overflow, int same_bdry, int desc) {
     __shared__ uint64_t stage[kStage];
     __shared__ uint32_t s_n, s_base;
     if (threadIdx.x == 0) s_n = 0;
@@ -488,7 +499,7 @@ certain_init_kernel(const float* __restrict__ f, VGrid g, const uint8_t* __restr
             const bool theirs = !same_bdry || (cu > 0 && cu < n3[a] - 1);       // I am u's neighbour
             const int64_t u = v + dir_offset(g, j);
             const bool seed = seedmask[u] != 0;
-            if (before(f, u, v)) {
+            if (ord_before(f, u, v, desc)) {
                 if (mine) earlier |= 1u << j;
                 if (theirs) reach |= 1u << j;
                 if (!seed && (mine || theirs)) c++;
@@ -1068,12 +1079,12 @@ __device__ int32_t uf_find(int32_t* rep, int32_t x) {
     return x;
 }
 
-__device__ void uf_union(const float* f, int32_t* rep, int32_t a, int32_t b) {
+__device__ void uf_union(const float* f, int32_t* rep, int32_t a, int32_t b, int desc) {
     while (true) {
         a = uf_find(rep, a);
         b = uf_find(rep, b);
         if (a == b) return;
-        if (before(f, b, a)) {  // hook the later root a under b
+        if (ord_before(f, b, a, desc)) {  // hook the later root a under b
             const int32_t t = a; a = b; b = t;
         }
         // b is the later root now
@@ -1118,7 +1129,10 @@ cc_union_tile_kernel(const float* __restrict__ f, VGrid g, LabelBuffers b) {
     __syncthreads();
     // the earlier of two tile vertices in the total order (value, vertex id);
     // the local index order is the vertex id order within a tile
-    auto earlier = [&](int a, int c) { return val[a] < val[c] || (val[a] == val[c] && a < c); };
+    auto earlier = [&](int a, int c) {
+        return b.desc ? (val[a] > val[c] || (val[a] == val[c] && a > c))
+                      : (val[a] < val[c] || (val[a] == val[c] && a < c));
+    };
     auto join = [&](int a, int c) {
         while (true) {
             a = lds_find(parent, a);
@@ -1156,9 +1170,9 @@ cc_union_kernel(const float* __restrict__ f, VGrid g, LabelBuffers b) {
         const bool fz = (z + 1) % kTz == 0 && z < g.Z - 1;
         if (!(fx || fy || fz)) continue;
         const int32_t l = b.label2[i];
-        if (fx && b.label2[i + 1] == l) uf_union(f, b.rep, (int32_t)i, (int32_t)(i + 1));
-        if (fy && b.label2[i + g.X] == l) uf_union(f, b.rep, (int32_t)i, (int32_t)(i + g.X));
-        if (fz && b.label2[i + g.sxy] == l) uf_union(f, b.rep, (int32_t)i, (int32_t)(i + g.sxy));
+        if (fx && b.label2[i + 1] == l) uf_union(f, b.rep, (int32_t)i, (int32_t)(i + 1), b.desc);
+        if (fy && b.label2[i + g.X] == l) uf_union(f, b.rep, (int32_t)i, (int32_t)(i + g.X), b.desc);
+        if (fz && b.label2[i + g.sxy] == l) uf_union(f, b.rep, (int32_t)i, (int32_t)(i + g.sxy), b.desc);
     }
 }
 
@@ -1197,7 +1211,7 @@ cc_candidates_kernel(const float* __restrict__ f, VGrid g, LabelBuffers b, uint3
         const int32_t l = b.label2[i];
         bool c = false;
         for (int j = 0; j < k; j++)
-            if (before(f, nb[j], i) && b.label2[nb[j]] != l) c = true;
+            if (ord_before(f, nb[j], i, b.desc) && b.label2[nb[j]] != l) c = true;
         if (c) {
             cand[i] = 1u;
             mine++;
@@ -1223,9 +1237,9 @@ cc_collect_kernel(const float* __restrict__ f, VGrid g, LabelBuffers b, const ui
         int64_t lower[6];
         int m = 0;
         for (int j = 0; j < k; j++) {
-            if (!before(f, nb[j], i)) continue;
+            if (!ord_before(f, nb[j], i, b.desc)) continue;
             int p = m++;
-            while (p > 0 && before(f, nb[j], lower[p - 1])) {
+            while (p > 0 && ord_before(f, nb[j], lower[p - 1], b.desc)) {
                 lower[p] = lower[p - 1];
                 p--;
             }
@@ -1429,6 +1443,102 @@ saddle_arcs_kernel(VGrid g, const float* __restrict__ f, const uint8_t* __restri
     }
 }
 
+// From a hex, follow hex -> paired quad -> the quad's other hex until a
+// critical hex (rec_tu, gi_extrema_region_builder.h:330-352); -1 where a hex
+// is paired with a quad on the domain boundary (no other hex).
+__device__ int64_t trace_up_hex(const CellG& cg, int64_t Mz, const uint8_t* grad, int64_t c, int64_t limit) {
+    for (int64_t s = 0; s < limit; s++) {
+        const uint32_t d = (grad[c] >> 2) & 7u;
+        if (d == 7u) return c;
+        if (d > 5u) return -1;  // only a corrupt byte
+        const int64_t q = grad_pair(cg, grad, c);
+        if (((grad[q] >> 2) & 7u) == 7u) return q;
+        int64_t x, y, z;
+        cell_xyz(q, cg.Mx, cg.My, x, y, z);
+        const int a = (int)(d >> 1);
+        const int64_t along = a == 0 ? x : (a == 1 ? y : z), ext = a == 0 ? cg.Mx : (a == 1 ? cg.My : Mz);
+        if (along == 0 || along == ext - 1) return -1;
+        c = 2 * q - c;
+    }
+    return -1;
+}
+
+// Highest vertex (value, then vertex id) of the cell at (x, y, z): its
+// corners are the vertices (x +- (x odd)) / 2 ... (the max vertex labelling,
+// gi_max_vertex_labeling.h).
+__device__ int64_t highest_vertex(const VGrid& g, const float* __restrict__ f, int64_t x, int64_t y, int64_t z) {
+    int64_t best = -1;
+    for (int k = 0; k < 8; k++) {
+        const int64_t vx = (x + ((x & 1) ? ((k & 1) ? 1 : -1) : 0)) / 2;
+        const int64_t vy = (y + ((y & 1) ? ((k & 2) ? 1 : -1) : 0)) / 2;
+        const int64_t vz = (z + ((z & 1) ? ((k & 4) ? 1 : -1) : 0)) / 2;
+        const int64_t v = vx + (int64_t)g.X * (vy + (int64_t)g.Y * vz);
+        if (best < 0 || before(f, best, v)) best = v;
+    }
+    return best;
+}
+
+// Arcs of the max extremum graph (trace_up_2saddle, gi_extrema_region_builder.h:354-372):
+// every critical quad off the domain boundary traced up through its two
+// hexes (+ side first) to maxima; arcs joining two different maxima with
+// persistence not above `thr` are appended.
+__global__ void __launch_bounds__(kThreads)
+saddle2_arcs_kernel(VGrid g, const float* __restrict__ f, const uint8_t* __restrict__ grad, MList list, float thr,
+                    MaxArcRecord* __restrict__ out, uint32_t cap, uint32_t* __restrict__ count) {
+    __shared__ uint32_t prefix[kSubs + 2];
+    const uint32_t total = mlist_prefix(list, prefix);
+    const CellG cg{2 * (int64_t)g.X - 1, 2 * (int64_t)g.Y - 1, (2 * (int64_t)g.X - 1) * (2 * (int64_t)g.Y - 1)};
+    const int64_t Mz = 2 * (int64_t)g.Z - 1, limit = g.verts() + 1;
+    for (uint32_t i = blockIdx.x * kThreads + threadIdx.x; i < total; i += gridDim.x * kThreads) {
+        const int64_t s = mlist_at(list, prefix, i);
+        int64_t cx, cy, cz;
+        cell_xyz(s, cg.Mx, cg.My, cx, cy, cz);
+        // the quad's even axis: its hexes lie along it; on the boundary it has one
+        const int a = !(cx & 1) ? 0 : (!(cy & 1) ? 1 : 2);
+        const int64_t along = a == 0 ? cx : (a == 1 ? cy : cz), ext = a == 0 ? cg.Mx : (a == 1 ? cg.My : Mz);
+        if (along == 0 || along == ext - 1) continue;
+        const int64_t step = a == 0 ? 1 : (a == 1 ? cg.Mx : cg.sxy);
+        const int64_t m1 = trace_up_hex(cg, Mz, grad, s + step, limit);
+        const int64_t m2 = trace_up_hex(cg, Mz, grad, s - step, limit);
+        if (m1 == m2 || m1 < 0 || m2 < 0) continue;
+        const float sval = f[highest_vertex(g, f, cx, cy, cz)];
+        int64_t x, y, z;
+        cell_xyz(m1, cg.Mx, cg.My, x, y, z);
+        const int64_t h1 = highest_vertex(g, f, x, y, z);
+        cell_xyz(m2, cg.Mx, cg.My, x, y, z);
+        const int64_t h2 = highest_vertex(g, f, x, y, z);
+        const float v1 = f[h1], v2 = f[h2];
+        float d1 = v1 - sval, d2 = v2 - sval;
+        if (d1 < 0) d1 *= -1;
+        if (d2 < 0) d2 *= -1;
+        const float pers = d1 < d2 ? d1 : d2;
+        if (pers > thr) continue;
+        const uint32_t slot = atomicAdd(count, 1u);
+        if (slot < cap) out[slot] = MaxArcRecord{s, m1, m2, h1, h2, sval, v1, v2, 0.f};
+    }
+}
+
+// The highest vertex of every critical hex (the maxima of the max graph).
+__global__ void __launch_bounds__(kThreads)
+collect_maxima_kernel(VGrid g, const float* __restrict__ f, const uint8_t* __restrict__ grad, MList out,
+                      uint32_t* __restrict__ overflow) {
+    __shared__ uint32_t stage[kStage];
+    __shared__ uint32_t s_n;
+    __shared__ MSlot s_base;
+    if (threadIdx.x == 0) s_n = 0;
+    __syncthreads();
+    const Outbox box{stage, &s_n, out, (int)(blockIdx.x % kSubs), overflow};
+    const int64_t Mx = 2 * (int64_t)g.X - 1, My = 2 * (int64_t)g.Y - 1;
+    const int64_t HX = g.X - 1, HY = g.Y - 1, n = HX * HY * (int64_t)(g.Z - 1);
+    for (int64_t h = (int64_t)blockIdx.x * kThreads + threadIdx.x; h < n; h += (int64_t)gridDim.x * kThreads) {
+        const int64_t x = 2 * (h % HX) + 1, y = 2 * ((h / HX) % HY) + 1, z = 2 * (h / (HX * HY)) + 1;
+        const int64_t c = x + Mx * (y + My * z);
+        if (((grad[c] >> 2) & 7u) == 7u) box.push((uint32_t)highest_vertex(g, f, x, y, z));
+    }
+    __syncthreads();
+    mlist_flush_block(out, box.sub, stage, s_n < (uint32_t)kStage ? s_n : (uint32_t)kStage, &s_base, overflow);
+}
+
 __global__ void __launch_bounds__(kThreads)
 seed_minima_kernel(MList minima, int32_t* __restrict__ label, float4* __restrict__ gradf,
                    uint8_t* __restrict__ seedmask) {
@@ -1540,6 +1650,10 @@ hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s) {
     hipLaunchKernelGGL(fill_u8_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, p, n, v);
     return hipGetLastError();
 }
+hipError_t launch_or_u8(uint8_t* dst, const uint8_t* src, int64_t n, hipStream_t s) {
+    hipLaunchKernelGGL(or_u8_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, dst, src, n);
+    return hipGetLastError();
+}
 hipError_t launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t s) {
     hipLaunchKernelGGL(fill_u32_kernel, dim3(grid_for(n)), dim3(kThreads), 0, s, p, n, v);
     return hipGetLastError();
@@ -1570,10 +1684,10 @@ uint32_t resident_blocks(K kernel) {
 }  // namespace
 hipError_t launch_certain_init(const VGrid& g, const float* values, const uint8_t* seedmask, uint32_t* cnt,
                                uint64_t* out, uint32_t* out_count, uint32_t capacity, uint32_t* overflow,
-                               hipStream_t s, bool same_bdry) {
+                               hipStream_t s, bool same_bdry, bool desc) {
     const int64_t blocks = (g.verts() + kStage - 1) / kStage;
     hipLaunchKernelGGL(certain_init_kernel, dim3((uint32_t)blocks), dim3(kThreads), 0, s, values, g, seedmask, cnt,
-                       out, out_count, capacity, overflow, same_bdry ? 1 : 0);
+                       out, out_count, capacity, overflow, same_bdry ? 1 : 0, desc ? 1 : 0);
     return hipGetLastError();
 }
 hipError_t launch_certain_round(const VGrid& g, int32_t* label, uint32_t* cnt, const uint64_t* in,
@@ -1713,6 +1827,19 @@ hipError_t launch_saddle_arcs(const VGrid& g, const float* values, const uint8_t
                               float threshold, ArcRecord* out, uint32_t cap, uint32_t* count, hipStream_t s) {
     hipLaunchKernelGGL(saddle_arcs_kernel, dim3(1024), dim3(kThreads), 0, s, g, values, grad, list, threshold, out,
                        cap, count);
+    return hipGetLastError();
+}
+hipError_t launch_saddle2_arcs(const VGrid& g, const float* values, const uint8_t* grad, const MList& list,
+                               float threshold, MaxArcRecord* out, uint32_t cap, uint32_t* count, hipStream_t s) {
+    hipLaunchKernelGGL(saddle2_arcs_kernel, dim3(1024), dim3(kThreads), 0, s, g, values, grad, list, threshold, out,
+                       cap, count);
+    return hipGetLastError();
+}
+hipError_t launch_collect_maxima(const VGrid& g, const float* values, const uint8_t* grad, const MList& out,
+                                 uint32_t* overflow, hipStream_t s) {
+    const int64_t n = (int64_t)(g.X - 1) * (g.Y - 1) * (g.Z - 1);
+    hipLaunchKernelGGL(collect_maxima_kernel, dim3(grid_for(n > 0 ? n : 1, 8192)), dim3(kThreads), 0, s, g, values,
+                       grad, out, overflow);
     return hipGetLastError();
 }
 hipError_t launch_seed_minima(const MList& minima, const LabelBuffers& b, hipStream_t s) {

@@ -36,6 +36,7 @@ struct LabelBuffers {
     int32_t* rep;        // component representatives (cleanup)
     int32_t* rep2;
     uint32_t* stats;     // device counters (see IntegStat)
+    int32_t desc;        // cleanup order: 0 ascending labels, 1 descending (greater vertex first)
 };
 
 enum IntegStat : int {
@@ -56,6 +57,8 @@ enum IntegStat : int {
 
 hipError_t launch_grad_field(const VGrid& g, const float* values, float4* gradf, hipStream_t s);
 hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
+// dst[i] |= src[i] for n bytes.
+hipError_t launch_or_u8(uint8_t* dst, const uint8_t* src, int64_t n, hipStream_t s);
 // b.gradf = 0 where b.seedmask is set (the region extrema).
 hipError_t launch_zero_seed_gradient(const VGrid& g, const LabelBuffers& b, hipStream_t s);
 hipError_t launch_fill_u8(uint8_t* p, int64_t n, uint8_t v, hipStream_t s);
@@ -98,7 +101,7 @@ hipError_t launch_pass2(const VGrid& g, const LabelBuffers& b, const WorkQueue& 
 // (expected: about how many vertices `in` holds, which sizes the launch).
 hipError_t launch_certain_init(const VGrid& g, const float* values, const uint8_t* seedmask, uint32_t* cnt,
                                uint64_t* out, uint32_t* out_count, uint32_t capacity, uint32_t* overflow,
-                               hipStream_t s, bool same_bdry = false);
+                               hipStream_t s, bool same_bdry = false, bool desc = false);
 // NumericIntegratorExpandingRegionStop (minima, downhill): extrema (no
 // SameBdry6 neighbour before the vertex) labelled by their own ids, the rest
 // -1, counts[0] += extrema; stop states from labels (certain: terminal),
@@ -165,6 +168,18 @@ struct ArcRecord {
 // out[0..cap) (count: appended arcs, also those beyond cap; order arbitrary).
 hipError_t launch_saddle_arcs(const VGrid& g, const float* values, const uint8_t* grad, const MList& list,
                               float threshold, ArcRecord* out, uint32_t cap, uint32_t* count, hipStream_t s);
+// The same for the max extremum graph: critical quads off the domain
+// boundary traced up through their two hexes to critical hexes (maxima);
+// h1, h2: the maxima's highest vertices.
+struct MaxArcRecord {
+    int64_t saddle, max1, max2, h1, h2;
+    float sval, v1, v2, pad;
+};
+hipError_t launch_saddle2_arcs(const VGrid& g, const float* values, const uint8_t* grad, const MList& list,
+                               float threshold, MaxArcRecord* out, uint32_t cap, uint32_t* count, hipStream_t s);
+// The highest vertex of every critical hex, appended to `out`.
+hipError_t launch_collect_maxima(const VGrid& g, const float* values, const uint8_t* grad, const MList& out,
+                                 uint32_t* overflow, hipStream_t s);
 // Critical vertices of a .grad (pair code 7 at even cells) appended to `out`.
 hipError_t launch_collect_minima(const VGrid& g, const uint8_t* grad, const MList& out, uint32_t* overflow,
                                  hipStream_t s);

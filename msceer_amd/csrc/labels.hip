@@ -211,7 +211,7 @@ namespace {
 // read-back per group shows whether the next round's list is empty.  Uses
 // the frontier buffers (lists) and b.stamp; same_bdry: SameBdry6 neighbours.
 int run_certain_rounds(msceer_ctx* ctx, const VGrid& g, const float* d_values, const LabelBuffers& b, bool same_bdry,
-                       int* sweeps_out) {
+                       int* sweeps_out, bool desc = false) {
     hipStream_t s = ctx->stream;
     uint32_t* ctl = ctl_dev(ctx);
     const int64_t V = g.verts();
@@ -222,7 +222,7 @@ int run_certain_rounds(msceer_ctx* ctx, const VGrid& g, const float* d_values, c
     CHECK_HIP(launch_zero(counts, 3, s));
     CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
     CHECK_HIP(launch_certain_init(g, d_values, b.seedmask, b.stamp, lists[0], counts, lcap, ctl + kCtlOverflow, s,
-                                  same_bdry));
+                                  same_bdry, desc));
     rc = read_ctl(ctx, 0, kCtlWords);
     if (rc) return rc;
     int k = 0;
@@ -266,17 +266,20 @@ int run_certain_rounds(msceer_ctx* ctx, const VGrid& g, const float* d_values, c
 
 int ondemand_impl(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz, float err_thr,
                   float grad_thr, int32_t max_iters, float presimp, uint32_t need, const uint8_t* integrate_mask,
-                  uint8_t* grad_out, int32_t* labels_out, int8_t* restriction_out, msceer_label_stats* stats,
-                  uint32_t flags) {
+                  uint8_t* grad_out, int32_t* labels_out, int32_t* labels_dsc_out, int8_t* restriction_out,
+                  msceer_label_stats* stats, uint32_t flags) {
     if (!ctx || !values) return MSCEER_EINVAL;
     if (flags & ~(MSCEER_DEVICE_PTRS | MSCEER_TIMING | MSCEER_STOP_REGIONS | MSCEER_STOP_INTEGRATED |
                   MSCEER_FP32_FIRST))
         return MSCEER_EINVAL;
-    if (!need || (need & ~(MSCEER_NEED_ASC1 | MSCEER_NEED_DSC1 | MSCEER_NEED_ASC2))) return MSCEER_EINVAL;
-    const bool asc2 = need & MSCEER_NEED_ASC2;
-    const bool lines = need & (MSCEER_NEED_ASC1 | MSCEER_NEED_DSC1);
-    if (!asc2 && (integrate_mask || labels_out || (flags & (MSCEER_STOP_REGIONS | MSCEER_STOP_INTEGRATED))))
+    if (!need || (need & ~(MSCEER_NEED_ASC1 | MSCEER_NEED_DSC1 | MSCEER_NEED_ASC2 | MSCEER_NEED_DSC2)))
         return MSCEER_EINVAL;
+    const bool asc2 = need & MSCEER_NEED_ASC2, dsc2 = need & MSCEER_NEED_DSC2;
+    if (labels_dsc_out && !dsc2) return MSCEER_EINVAL;
+    const bool lines = need & (MSCEER_NEED_ASC1 | MSCEER_NEED_DSC1);
+    const bool stop_r = flags & MSCEER_STOP_REGIONS, stop_i = flags & MSCEER_STOP_INTEGRATED;
+    if (!asc2 && (integrate_mask || labels_out)) return MSCEER_EINVAL;
+    if (!asc2 && !dsc2 && (stop_r || stop_i)) return MSCEER_EINVAL;
     if (!(err_thr > 0.f) || !(grad_thr >= 0.f) || max_iters < 1 || !(presimp >= 0.f)) return MSCEER_EINVAL;
     int rc = check_dims(nx, ny, nz);
     if (rc) return rc;
@@ -349,6 +352,21 @@ int ondemand_impl(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, 
     CHECK_HIP(launch_lstar(d, lb, s, shape));
     CHECK_HIP(hipEventRecord(ev[1], s));
 
+    // The STOP flags end every labelling asked for at that stage and return
+    // its labels alone.
+    LabelBuffers bd = b;
+    auto finish_early = [&]() -> int {
+        if (labels_out)
+            CHECK_HIP(hipMemcpyAsync(labels_out, b.label, sizeof(int32_t) * V,
+                                     dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+        if (labels_dsc_out)
+            CHECK_HIP(hipMemcpyAsync(labels_dsc_out, bd.label, sizeof(int32_t) * V,
+                                     dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+        CHECK_HIP(hipStreamSynchronize(s));
+        ctx->last_labels = st;
+        if (stats) *stats = st;
+        return MSCEER_OK;
+    };
     if (asc2) {
         // ---- extremum graph (critical vertices and 1-saddles) ----
         // The minima stay on the device; only the arcs that the simplification can
@@ -379,9 +397,13 @@ int ondemand_impl(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, 
         std::vector<MinNode> nodes;
         nodes.reserve(2 * rec.size());
         std::vector<SaddleArc> arcs(rec.size());
+        const int64_t Mx0 = 2 * nx - 1, My0 = 2 * ny - 1;
+        auto vnum0 = [&](int64_t c) {
+            return (c % Mx0) / 2 + nx * (((c / Mx0) % My0) / 2 + ny * ((c / (Mx0 * My0)) / 2));
+        };
         for (size_t i = 0; i < rec.size(); i++) {
-            nodes.push_back(MinNode{rec[i].min1, rec[i].v1});
-            nodes.push_back(MinNode{rec[i].min2, rec[i].v2});
+            nodes.push_back(MinNode{rec[i].min1, rec[i].v1, vnum0(rec[i].min1)});
+            nodes.push_back(MinNode{rec[i].min2, rec[i].v2, vnum0(rec[i].min2)});
             arcs[i] = SaddleArc{rec[i].saddle, rec[i].min1, rec[i].min2, rec[i].sval};
         }
         std::sort(nodes.begin(), nodes.end(), [](const MinNode& x, const MinNode& y) { return x.cell < y.cell; });
@@ -422,27 +444,22 @@ int ondemand_impl(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, 
         if (rc) return rc;
         st.certain_sweeps = sweeps;
         CHECK_HIP(hipEventRecord(ev[2], s));
-        auto finish_early = [&]() -> int {
-            if (labels_out)
-                CHECK_HIP(hipMemcpyAsync(labels_out, b.label, sizeof(int32_t) * V,
-                                         dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
-            CHECK_HIP(hipStreamSynchronize(s));
-            ctx->last_labels = st;
-            if (stats) *stats = st;
-            return MSCEER_OK;
-        };
-        if (flags & MSCEER_STOP_REGIONS) return finish_early();
+        if (stop_r && !dsc2) return finish_early();
 
         // ---- integration ----
-        rc = run_integration(ctx, g, b, d_mask, d_grad, grad_thr, err_thr, max_iters, -1,
-                             (flags & MSCEER_FP32_FIRST) != 0, st);
-        if (rc) return rc;
+        if (!stop_r) {
+            rc = run_integration(ctx, g, b, d_mask, d_grad, grad_thr, err_thr, max_iters, -1,
+                                 (flags & MSCEER_FP32_FIRST) != 0, st);
+            if (rc) return rc;
+        }
         CHECK_HIP(hipEventRecord(ev[3], s));
-        if (flags & MSCEER_STOP_INTEGRATED) return finish_early();
+        if (stop_i && !dsc2) return finish_early();
 
         // ---- cleanup of isolated components ----
-        rc = run_cleanup(ctx, g, b);
-        if (rc) return rc;
+        if (!stop_r && !stop_i) {
+            rc = run_cleanup(ctx, g, b);
+            if (rc) return rc;
+        }
         CHECK_HIP(hipEventRecord(ev[4], s));
     } else {
         CHECK_HIP(hipEventRecord(ev[2], s));
@@ -450,16 +467,113 @@ int ondemand_impl(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, 
         CHECK_HIP(hipEventRecord(ev[4], s));
     }
 
+    // ---- descending 3-manifolds (needDSC2, ondemandaccurate.cxx:913-991) ----
+    // The max extremum graph of the steepest pairing (critical hexes joined by
+    // interior critical quads), certain regions in the descending order,
+    // uphill integration and cleanup in that order.  The gradient field is
+    // the one the ascending branch left (zeroed at its seeds), zeroed at the
+    // maxima too, as the reference's shared field is.
+    if (dsc2) {
+        CHECK_HIP(hipEventRecord(ev[8], s));
+        CHECK_HIP(ctx->label_dsc.ensure(sizeof(int32_t) * V));
+        CHECK_HIP(ctx->seedmask2.ensure(V));
+        bd.label = ctx->label_dsc.as<int32_t>();
+        bd.seedmask = ctx->seedmask2.as<uint8_t>();
+        bd.desc = 1;
+        const MList maxl = front_list(ctx, caps, 0);
+        CHECK_HIP(launch_zero(maxl.counts, kCountWords, s));
+        CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
+        CHECK_HIP(launch_collect_maxima(g, d_values, d_grad, maxl, ctl + kCtlOverflow, s));
+        rc = read_ctl(ctx, 0, kCtlWords);
+        if (rc) return rc;
+        if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;
+        const uint64_t n_max = list_total(ctx->h_ctl + kCtlRound, maxl);
+        const uint64_t n_sad = list_total(ctx->h_ctl + kCtlList2, list2);
+        std::vector<MaxArcRecord> rec;
+        if (n_sad) {
+            CHECK_HIP(ctx->host_io.ensure(sizeof(MaxArcRecord) * n_sad + 64));
+            MaxArcRecord* d_arcs = ctx->host_io.as<MaxArcRecord>();
+            CHECK_HIP(launch_zero(ctl + kCtlCount, 1, s));
+            CHECK_HIP(launch_saddle2_arcs(g, d_values, d_grad, list2, presimp, d_arcs, (uint32_t)n_sad, ctl + kCtlCount,
+                                          s));
+            rc = read_ctl(ctx, kCtlCount, 1);
+            if (rc) return rc;
+            rec.resize(std::min<uint64_t>(ctx->h_ctl[kCtlCount], n_sad));
+            if (!rec.empty()) {
+                CHECK_HIP(hipMemcpyAsync(rec.data(), d_arcs, sizeof(MaxArcRecord) * rec.size(), hipMemcpyDeviceToHost,
+                                         s));
+                CHECK_HIP(hipStreamSynchronize(s));
+            }
+        }
+        std::vector<MinNode> nodes;
+        nodes.reserve(2 * rec.size());
+        std::vector<SaddleArc> arcs(rec.size());
+        for (size_t i = 0; i < rec.size(); i++) {
+            nodes.push_back(MinNode{rec[i].max1, rec[i].v1, rec[i].h1});
+            nodes.push_back(MinNode{rec[i].max2, rec[i].v2, rec[i].h2});
+            arcs[i] = SaddleArc{rec[i].saddle, rec[i].max1, rec[i].max2, rec[i].sval};
+        }
+        std::sort(nodes.begin(), nodes.end(), [](const MinNode& x, const MinNode& y) { return x.cell < y.cell; });
+        nodes.erase(std::unique(nodes.begin(), nodes.end(),
+                                [](const MinNode& x, const MinNode& y) { return x.cell == y.cell; }),
+                    nodes.end());
+        const std::vector<int> rep = simplify_maxima(nodes, arcs, presimp);
+        std::vector<int32_t> seed_v, seed_l;
+        for (size_t i = 0; i < nodes.size(); i++) {
+            if (rep[i] == (int)i) continue;
+            seed_v.push_back((int32_t)nodes[i].vertex);
+            seed_l.push_back((int32_t)nodes[rep[i]].vertex);
+        }
+        st.maxima = (int64_t)n_max;
+        st.saddles2 = (int64_t)n_sad;
+        st.regions_dsc = (int64_t)n_max - (int64_t)seed_v.size();
+        CHECK_HIP(launch_fill_i32(bd.label, V, -1, s));
+        CHECK_HIP(launch_fill_u8(bd.seedmask, V, 0, s));
+        if (!asc2) CHECK_HIP(launch_grad_field(g, d_values, b.gradf, s));
+        CHECK_HIP(launch_seed_minima(maxl, bd, s));  // every maximum's highest vertex: its own label
+        if (!seed_v.empty()) {
+            CHECK_HIP(ctx->host_io.ensure(2 * sizeof(int32_t) * seed_v.size() + 64));
+            int32_t* d_sv = ctx->host_io.as<int32_t>();
+            int32_t* d_sl = d_sv + seed_v.size();
+            CHECK_HIP(hipMemcpyAsync(d_sv, seed_v.data(), sizeof(int32_t) * seed_v.size(), hipMemcpyHostToDevice, s));
+            CHECK_HIP(hipMemcpyAsync(d_sl, seed_l.data(), sizeof(int32_t) * seed_l.size(), hipMemcpyHostToDevice, s));
+            CHECK_HIP(launch_seed_scatter(d_sv, d_sl, (int)seed_v.size(), bd, s));
+            CHECK_HIP(hipStreamSynchronize(s));  // host vectors go out of scope
+        }
+        int sweeps = 0;
+        rc = run_certain_rounds(ctx, g, d_values, bd, false, &sweeps, true);
+        if (rc) return rc;
+        st.certain_sweeps_dsc = sweeps;
+        if (!stop_r) {
+            // the FP64 field of pass 3 is zeroed at every seed so far
+            if (asc2) CHECK_HIP(launch_or_u8(bd.seedmask, b.seedmask, V, s));
+            CHECK_HIP(launch_zero(ctl + kCtlIntStats, kStatNum, s));
+            msceer_label_stats sd{};
+            rc = run_integration(ctx, g, bd, nullptr, nullptr, grad_thr, err_thr, max_iters, 1,
+                                 (flags & MSCEER_FP32_FIRST) != 0, sd);
+            if (rc) return rc;
+            if (!stop_i) {
+                rc = run_cleanup(ctx, g, bd);
+                if (rc) return rc;
+            }
+        }
+        CHECK_HIP(hipEventRecord(ev[9], s));
+        CHECK_HIP(hipEventSynchronize(ev[9]));
+        st.ms_dsc = elapsed(ev[8], ev[9]);
+    }
+    if (stop_r || stop_i) return finish_early();
+
     // ---- restriction labels, 1-manifolds, conforming gradient, ldir ----
     if (grad_out || restriction_out) {
         int8_t* restr = ctx->restriction.as<int8_t>();
         if (asc2) CHECK_HIP(launch_min_boundary(g, b.label, restr, s));
         else CHECK_HIP(launch_fill_u8(reinterpret_cast<uint8_t*>(restr), C, 0, s));
+        if (dsc2) CHECK_HIP(launch_max_boundary(g, d_values, bd.label, restr, s));
         CHECK_HIP(hipEventRecord(ev[5], s));
         if (lines) {
             // the streamlines follow the FP64 field, which the labelling has
             // zeroed at its region extrema (as the reference's shared field is)
-            if (!asc2) {
+            if (!asc2 && !dsc2) {
                 CHECK_HIP(launch_fill_u8(b.seedmask, V, 0, s));
                 CHECK_HIP(launch_grad_field64(g, b, s));
             }
@@ -509,11 +623,15 @@ int ondemand_impl(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, 
         CHECK_HIP(hipMemcpyAsync(labels_out, b.label, sizeof(int32_t) * V,
                                  dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
     }
+    if (labels_dsc_out) {
+        CHECK_HIP(hipMemcpyAsync(labels_dsc_out, bd.label, sizeof(int32_t) * V,
+                                 dev ? hipMemcpyDeviceToDevice : hipMemcpyDeviceToHost, s));
+    }
     CHECK_HIP(hipStreamSynchronize(s));
     st.ms_lstar = elapsed(ev[0], ev[1]) + elapsed(ev[10], ev[6]);
     st.ms_regions = elapsed(ev[1], ev[2]);
     st.ms_integrate = elapsed(ev[2], ev[3]);
-    st.ms_cleanup = elapsed(ev[3], ev[4]) + elapsed(ev[4], ev[5]);
+    st.ms_cleanup = elapsed(ev[3], ev[4]) + elapsed(ev[4], ev[5]) - st.ms_dsc;  // the restriction labels
     st.ms_lines = elapsed(ev[5], ev[10]);
     st.ms_grad = elapsed(ev[10], ev[7]);
     st.ms_total = elapsed(ev[0], ev[7]);
@@ -529,7 +647,7 @@ extern "C" int msceer_ondemand_asc(msceer_ctx* ctx, const float* values, int64_t
                                    const uint8_t* integrate_mask, uint8_t* grad_out, int32_t* labels_out,
                                    msceer_label_stats* stats, uint32_t flags) {
     return ondemand_impl(ctx, values, nx, ny, nz, err_thr, grad_thr, max_iters, presimp, MSCEER_NEED_ASC2,
-                         integrate_mask, grad_out, labels_out, nullptr, stats, flags);
+                         integrate_mask, grad_out, labels_out, nullptr, nullptr, stats, flags);
 }
 
 extern "C" int msceer_ondemand(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz,
@@ -537,7 +655,16 @@ extern "C" int msceer_ondemand(msceer_ctx* ctx, const float* values, int64_t nx,
                                const uint8_t* integrate_mask, uint8_t* grad_out, int32_t* labels_out,
                                int8_t* restriction_out, msceer_label_stats* stats, uint32_t flags) {
     return ondemand_impl(ctx, values, nx, ny, nz, err_thr, grad_thr, max_iters, presimp, need, integrate_mask,
-                         grad_out, labels_out, restriction_out, stats, flags);
+                         grad_out, labels_out, nullptr, restriction_out, stats, flags);
+}
+
+extern "C" int msceer_ondemand2(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz,
+                                float err_thr, float grad_thr, int32_t max_iters, float presimp, uint32_t need,
+                                const uint8_t* integrate_mask, uint8_t* grad_out, int32_t* labels_out,
+                                int32_t* labels_dsc_out, int8_t* restriction_out, msceer_label_stats* stats,
+                                uint32_t flags) {
+    return ondemand_impl(ctx, values, nx, ny, nz, err_thr, grad_thr, max_iters, presimp, need, integrate_mask,
+                         grad_out, labels_out, labels_dsc_out, restriction_out, stats, flags);
 }
 
 namespace {

@@ -55,31 +55,37 @@ def ondemand_asc(values: np.ndarray, err_thr: float = 1e-3, grad_thr: float = 1e
 
 def ondemand(values: np.ndarray, need_asc1: bool = False, need_dsc1: bool = False, need_asc2: bool = True,
              err_thr: float = 1e-3, grad_thr: float = 1e-6, max_iters: int = 4096, presimp: float = 0.0,
-             ctx: native.Context | None = None):
-    """OnDemandAccurate's accuracy branches (needASC1, needDSC1, needASC2):
-    (conforming grad, labels (Z,Y,X) int32 or None without needASC2,
-    restriction labels (cells int8), stats dict)."""
+             ctx: native.Context | None = None, need_dsc2: bool = False, with_dsc: bool = False):
+    """OnDemandAccurate's accuracy branches (needASC1, needDSC1, needASC2,
+    needDSC2): (conforming grad, labels (Z,Y,X) int32 or None without
+    needASC2, restriction labels (cells int8), stats dict); with_dsc=True
+    appends the descending labels (Z,Y,X) int32, or None without needDSC2."""
     v = np.ascontiguousarray(values, dtype=np.float32)
     Z, Y, X = v.shape
     need = ((native.NEED_ASC1 if need_asc1 else 0) | (native.NEED_DSC1 if need_dsc1 else 0) |
-            (native.NEED_ASC2 if need_asc2 else 0))
+            (native.NEED_ASC2 if need_asc2 else 0) | (native.NEED_DSC2 if need_dsc2 else 0))
     own = ctx is None
     ctx = ctx or native.Context()
     try:
         labels = np.empty(X * Y * Z, np.int32) if need_asc2 else None
+        labels_dsc = np.empty(X * Y * Z, np.int32) if need_dsc2 else None
         grad = np.empty(num_cells(X, Y, Z), np.uint8)
         restr = np.empty(num_cells(X, Y, Z), np.int8)
         st = native.LabelStats()
-        rc = native.lib().msceer_ondemand(
+        rc = native.lib().msceer_ondemand2(
             ctx.handle, v.ctypes.data_as(ctypes.c_void_p), X, Y, Z, err_thr, grad_thr, max_iters, presimp, need,
             None, grad.ctypes.data_as(ctypes.c_void_p),
             None if labels is None else labels.ctypes.data_as(ctypes.c_void_p),
+            None if labels_dsc is None else labels_dsc.ctypes.data_as(ctypes.c_void_p),
             restr.ctypes.data_as(ctypes.c_void_p), ctypes.byref(st), 0)
-        native.check(rc, "msceer_ondemand")
+        native.check(rc, "msceer_ondemand2")
     finally:
         if own:
             ctx.close()
-    return grad, None if labels is None else labels.reshape(Z, Y, X), restr, st.as_dict()
+    out = (grad, None if labels is None else labels.reshape(Z, Y, X), restr, st.as_dict())
+    if with_dsc:
+        out += (None if labels_dsc is None else labels_dsc.reshape(Z, Y, X),)
+    return out
 
 
 def label_file_name(base: str, X: int, Y: int, Z: int) -> str:

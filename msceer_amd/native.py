@@ -21,6 +21,7 @@ FP32_FIRST = 0x20
 NEED_ASC1 = 0x1
 NEED_DSC1 = 0x2
 NEED_ASC2 = 0x4
+NEED_DSC2 = 0x8
 BRICKS = ("64x2x2", "32x4x2", "16x4x4", "8x8x4")  # lower-star brick shapes by index
 # error codes
 OK, EINVAL, EHIP, ENOMEM, ETOOBIG, ENODEV = 0, -1, -2, -3, -4, -5
@@ -47,6 +48,8 @@ SIGNATURES = {
                                            ctypes.c_float, _P, _P, _P, _P, _U32]),
     "msceer_ondemand": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, ctypes.c_float, ctypes.c_float, _I32,
                                        ctypes.c_float, _U32, _P, _P, _P, _P, _P, _U32]),
+    "msceer_ondemand2": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, ctypes.c_float, ctypes.c_float, _I32,
+                                        ctypes.c_float, _U32, _P, _P, _P, _P, _P, _P, _U32]),
     "msceer_integrate_labels": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _P, ctypes.c_float, ctypes.c_float, _I32,
                                                _I32, _P, _P, _U32]),
     "msceer_clean_labels": (ctypes.c_int, [_P, _P, _I64, _I64, _I64, _P, _U32]),
@@ -115,6 +118,11 @@ class LabelStats(ctypes.Structure):
         ("ms_lines", ctypes.c_float),
         ("line_alone", ctypes.c_int32),
         ("line_overflows", ctypes.c_int32),
+        ("maxima", ctypes.c_int64),
+        ("saddles2", ctypes.c_int64),
+        ("regions_dsc", ctypes.c_int64),
+        ("certain_sweeps_dsc", ctypes.c_int32),
+        ("ms_dsc", ctypes.c_float),
     ]
 
     def as_dict(self):
