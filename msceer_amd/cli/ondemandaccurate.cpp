@@ -2,16 +2,18 @@
 //                  needASC1 needDSC1 needASC2 needDSC2 PresimpThesh [parallelism] [outputdebug]
 //
 // Drop-in for the volume path of ondemandaccurate/ondemandaccurate.cxx:704-1157
-// with any of needASC1 / needDSC1 (accurate 1-manifolds from the saddles) and
-// needASC2 (accurate ascending 3-manifolds): writes <filename>.grad, the
-// timing file <filename>.<parallelism:%03d>.gtime.txt and, with outputdebug
-// set, the per-vertex int32 labels <filename>._int_XxYxZ.raw (the
-// accurate3dmorselabeling output format, needASC2) and the int8 restriction
-// labels boundaries_<Mx>x<My>x<Mz>.raw in the working directory (the name
-// ondemandaccurate.cxx:1006-1011 gives them).  Without any accuracy flag it
-// writes the plain lower-star .grad, as the reference does.  The descending
-// 3-manifold branch (needDSC2) is not built here and is rejected with a
-// message.
+// with any of needASC1 / needDSC1 (accurate 1-manifolds from the saddles),
+// needASC2 and needDSC2 (accurate ascending / descending 3-manifolds):
+// writes <filename>.grad, the timing file
+// <filename>.<parallelism:%03d>.gtime.txt and, with outputdebug set, the
+// int8 restriction labels boundaries_<Mx>x<My>x<Mz>.raw in the working
+// directory (the name ondemandaccurate.cxx:1006-1011 gives them) and the
+// per-vertex int32 labels in the accurate3dmorselabeling output format:
+// <filename>._int_XxYxZ.raw (needASC2) and <filename>.dsc._int_XxYxZ.raw
+// (needDSC2; the reference keeps these in memory only).  Without any
+// accuracy flag it writes the plain lower-star .grad, as the reference does.
+// The timing line of the descending branch covers its region builder,
+// integration and cleanup together.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -37,10 +39,6 @@ int main(int argc, char** argv) {
     printf("dims=(%d,%d,%d)\nfile=%s\nintegration parameters: e=%f, gt=%f, il=%d\n"
            "ondemandacc: a1=%d, d1=%d, a2=%d, d2=%d, ps=%f\npar=%d\n",
            X, Y, Z, filename.c_str(), err, gthr, iters, a1, d1, a2, d2, presimp, par);
-    if (d2) {
-        fprintf(stderr, "the descending 3-manifold branch (needDSC2) is not available\n");
-        return 2;
-    }
     const size_t nv = (size_t)X * Y * Z;
     std::vector<float> values(nv);
     FILE* fin = fopen(filename.c_str(), "rb");
@@ -55,25 +53,29 @@ int main(int argc, char** argv) {
     if (rc) { fprintf(stderr, "msceer_create: %s\n", msceer_strerror(rc)); return 2; }
     const size_t nc = (size_t)(2 * X - 1) * (2 * Y - 1) * (2 * Z - 1);
     std::vector<unsigned char> grad(nc);
-    std::vector<int> labels(nv);
+    std::vector<int> labels(nv), labels_dsc(d2 ? nv : 0);
     msceer_label_stats st{};
     char tname[2048];
     snprintf(tname, sizeof(tname), "%s.%03d.gtime.txt", filename.c_str(), par);
     FILE* ft = fopen(tname, "w");
-    if (a1 || d1 || a2) {
-        const uint32_t need = (a1 ? MSCEER_NEED_ASC1 : 0u) | (d1 ? MSCEER_NEED_DSC1 : 0u) | (a2 ? MSCEER_NEED_ASC2 : 0u);
+    if (a1 || d1 || a2 || d2) {
+        const uint32_t need = (a1 ? MSCEER_NEED_ASC1 : 0u) | (d1 ? MSCEER_NEED_DSC1 : 0u) |
+                              (a2 ? MSCEER_NEED_ASC2 : 0u) | (d2 ? MSCEER_NEED_DSC2 : 0u);
         std::vector<signed char> restr(dbg ? nc : 0);
-        rc = msceer_ondemand(ctx, values.data(), X, Y, Z, err, gthr, iters, presimp, need, nullptr, grad.data(),
-                             a2 ? labels.data() : nullptr, dbg ? reinterpret_cast<int8_t*>(restr.data()) : nullptr,
-                             &st, MSCEER_TIMING);
-        if (rc) { fprintf(stderr, "msceer_ondemand: %s\n", msceer_strerror(rc)); msceer_destroy(ctx); return 2; }
+        rc = msceer_ondemand2(ctx, values.data(), X, Y, Z, err, gthr, iters, presimp, need, nullptr, grad.data(),
+                              a2 ? labels.data() : nullptr, d2 ? labels_dsc.data() : nullptr,
+                              dbg ? reinterpret_cast<int8_t*>(restr.data()) : nullptr, &st, MSCEER_TIMING);
+        if (rc) { fprintf(stderr, "msceer_ondemand2: %s\n", msceer_strerror(rc)); msceer_destroy(ctx); return 2; }
         // task lines "name start end duration" in ms, as StartTask/EndTaskAndRecord write them
-        const float parts[6] = {st.ms_lstar, st.ms_regions, st.ms_integrate, st.ms_cleanup, st.ms_lines, st.ms_grad};
-        const char* names[6] = {"Topological Robins0", "Topological SimExtRBASC", "NumericalTracing NewIntegrationASC",
-                                "Removed CleanerASC", "Numerical SaddleInt", "Topological ConformingGrad"};
+        const float parts[7] = {st.ms_lstar,   st.ms_regions, st.ms_integrate, st.ms_cleanup,
+                                st.ms_dsc,     st.ms_lines,   st.ms_grad};
+        const char* names[7] = {"Topological Robins0",      "Topological SimExtRBASC",
+                                "NumericalTracing NewIntegrationASC", "Removed CleanerASC",
+                                "NumericalTracing NewIntegrationDSC", "Numerical SaddleInt",
+                                "Topological ConformingGrad"};
         float t = 0.f;
-        for (int i = 0; i < 6; i++) {
-            if ((i >= 1 && i <= 3 && !a2) || (i == 4 && !(a1 || d1))) continue;
+        for (int i = 0; i < 7; i++) {
+            if ((i >= 1 && i <= 3 && !a2) || (i == 4 && !d2) || (i == 5 && !(a1 || d1))) continue;
             if (ft) fprintf(ft, "%s %d %d %d\n", names[i], (int)t, (int)(t + parts[i]), (int)parts[i]);
             printf("TIMING: %s %d %d %d\n", names[i], (int)t, (int)(t + parts[i]), (int)parts[i]);
             t += parts[i];
@@ -84,6 +86,9 @@ int main(int argc, char** argv) {
             printf("re-integrated %lld vertices (%lld traced in FP64)\n", (long long)st.reintegrated_pass3,
                    (long long)st.reintegrated_fp64);
         }
+        if (d2)
+            printf("built maxgraph: %lld maxima, %lld 2-saddles -> %lld descending regions\n", (long long)st.maxima,
+                   (long long)st.saddles2, (long long)st.regions_dsc);
         if (a1 || d1)
             printf("1-manifold streamlines: %lld from 2-saddles, %lld from 1-saddles (%lld re-traced in order)\n",
                    (long long)st.lines_asc, (long long)st.lines_dsc, (long long)st.lines_replayed);
@@ -91,6 +96,11 @@ int main(int argc, char** argv) {
             char lname[2048];
             snprintf(lname, sizeof(lname), "%s._int_%dx%dx%d.raw", filename.c_str(), X, Y, Z);
             if (FILE* fl = fopen(lname, "wb")) { fwrite(labels.data(), sizeof(int), nv, fl); fclose(fl); }
+        }
+        if (dbg && d2) {
+            char lname[2048];
+            snprintf(lname, sizeof(lname), "%s.dsc._int_%dx%dx%d.raw", filename.c_str(), X, Y, Z);
+            if (FILE* fl = fopen(lname, "wb")) { fwrite(labels_dsc.data(), sizeof(int), nv, fl); fclose(fl); }
         }
         if (dbg) {
             char bname[2048];

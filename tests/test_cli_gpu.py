@@ -70,12 +70,33 @@ def test_ondemandaccurate_cli_without_accuracy_writes_lstar_grad(tmp_path):
     assert (g == ref).all()
 
 
-def test_ondemandaccurate_cli_rejects_unbuilt_branches(tmp_path):
+@pytest.mark.parametrize("tag", ["smooth_34x33x32_0001", "gauss_44x36x40_0011"])
+def test_ondemandaccurate_cli_descending_matches_library_and_reference(tmp_path, tag):
+    # needDSC2 (argv[11], ondemandaccurate.cxx:170): the .grad and the
+    # restriction labels equal the library's; the descending labels split the
+    # vertices as the reference's run does, up to the integration's
+    # visiting-order mismatch
+    import hashlib
+    from golden import load_dsc2, same_partition
+    f, (err, gthr, iters, presimp), flags, ref, grad_sha = load_dsc2(tag)
+    Z, Y, X = f.shape
     raw = tmp_path / "vol.raw"
-    np.zeros(8, np.float32).tofile(raw)
-    r = subprocess.run([_exe("ondemandaccurate"), "2", "2", "2", str(raw), "1e-3", "1e-6", "10",
-                        "0", "0", "0", "1", "0"], cwd=tmp_path, capture_output=True, text=True, timeout=60)
-    assert r.returncode == 2 and "needDSC2" in r.stderr
+    f.tofile(raw)
+    out = _run([_exe("ondemandaccurate"), str(X), str(Y), str(Z), str(raw), repr(err), repr(gthr), str(iters),
+                *map(str, flags), repr(presimp), "8", "1"], tmp_path)
+    assert "built maxgraph" in out
+    g = np.fromfile(str(raw) + ".grad", np.uint8)
+    r = np.fromfile(tmp_path / f"boundaries_{2 * X - 1}x{2 * Y - 1}x{2 * Z - 1}.raw", np.int8)
+    ld = np.fromfile(f"{raw}.dsc._int_{X}x{Y}x{Z}.raw", np.int32)
+    grad, _, restr, _, ld_api = labels.ondemand(f, *map(bool, flags[:3]), err, gthr, iters, presimp,
+                                                need_dsc2=True, with_dsc=True)
+    assert (g == grad).all() and (r == restr).all() and (ld == ld_api.ravel()).all()
+    if tag.startswith("smooth_34"):  # no visiting-order mismatch on this case
+        assert same_partition(ld, ref["cleaned"])
+        assert (r == ref["restriction"]).all()
+        assert hashlib.sha256(g.tobytes()).hexdigest() == grad_sha
+    tasks = open(str(raw) + ".008.gtime.txt").read().splitlines()
+    assert any(t.startswith("NumericalTracing NewIntegrationDSC") for t in tasks)
 
 
 @pytest.mark.parametrize("flags", [(1, 0, 0), (1, 1, 1)])
