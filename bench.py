@@ -260,6 +260,7 @@ def bench_labels(args):
                              ("ms_lstar", "ms_regions", "ms_integrate", "ms_cleanup", "ms_grad")},
                 "fp64": int(st.reintegrated_fp64)}
     lines_out = None if args.no_lines else bench_lines(args, ctx, lib, dv, dg, f, C3_PARAMS)
+    dsc_out = None if args.no_dsc else bench_dsc(args, ctx, lib, dv, dg, f, C3_PARAMS)
     learned_out = None
     if not args.no_learned:
         learned_out = bench_learned(args, ctx, lib, dv, dg, dl, st, X, Y, Z, ref, plain, run)
@@ -271,7 +272,7 @@ def bench_labels(args):
     return {"workload": f"ondemandaccurate needASC2 + conforming .grad, {X}x{Y}x{Z} float32, 200 Gaussians "
                         f"(sigma 4-16) + 0.05 sin sin sin, seed 303; err 1e-3 (squared error), grad 1e-6, "
                         f"4096 steps, presimp 0; values resident in HBM",
-            "plain": plain, "lines": lines_out, "learned": learned_out, "cpu_baseline": base}
+            "plain": plain, "lines": lines_out, "descending": dsc_out, "learned": learned_out, "cpu_baseline": base}
 
 
 CNN_FLOP_PER_VOXEL = 2 * (27 * 16 + 27 * 16 * 32 + 27 * 32 * 16 + 16)  # 56 288
@@ -342,6 +343,74 @@ def bench_lines(args, ctx, lib, dv, dg, f, params):
                                          f"digitizing streamlines of both families (ASC, DSC), reference "
                                          f"arithmetic, given the steepest pairs and the gradient field"}
         out["gpu_lines_vs_cpu_loop"] = round(us * 1e-3 / out["asc1_dsc1"]["lines_ms"], 2)
+    return out
+
+
+def bench_dsc(args, ctx, lib, dv, dg, f, params):
+    """Descending 3-manifolds (needDSC2) on the labelling volume, alone and
+    with needASC2 (the reference's BOTH mode: one extremum graph build, the
+    gradient field shared), values resident in HBM; the descending branch
+    timed apart.  CPU: the reference's own needASC2 + needDSC2 pipeline
+    (OpenMP, built from its sources) on a z-slab."""
+    from msceer_amd import hipmem, native
+
+    Z, Y, X = f.shape
+    V = X * Y * Z
+    cells = (2 * X - 1) * (2 * Y - 1) * (2 * Z - 1)
+    err, gthr, iters, presimp = params
+    dr = hipmem.DeviceBuffer(cells)
+    dla = hipmem.DeviceBuffer(4 * V)
+    dld = hipmem.DeviceBuffer(4 * V)
+    st = native.LabelStats()
+
+    def run(need):
+        native.check(lib.msceer_ondemand2(ctx.handle, ctypes.c_void_p(dv.addr), X, Y, Z, err, gthr, iters, presimp,
+                                          need, None, ctypes.c_void_p(dg.addr),
+                                          ctypes.c_void_p(dla.addr) if need & native.NEED_ASC2 else None,
+                                          ctypes.c_void_p(dld.addr), ctypes.c_void_p(dr.addr), ctypes.byref(st),
+                                          native.DEVICE_PTRS), "msceer_ondemand2")
+
+    out = {}
+    for key, need in (("dsc2", native.NEED_DSC2), ("asc2_dsc2", native.NEED_ASC2 | native.NEED_DSC2)):
+        run(need)
+        hipmem.synchronize()
+        t0 = time.perf_counter()
+        stats = []
+        for _ in range(args.label_steps):
+            run(need)
+            stats.append(st.as_dict())
+        hipmem.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / args.label_steps
+        d = stats[-1]
+        out[key] = {"value": round(V / (ms * 1e-3) / 1e6, 2), "unit": "Mvert/s", "ms_per_step": round(ms, 3),
+                    "dsc_ms": round(float(np.mean([x["ms_dsc"] for x in stats])), 3),
+                    "maxima": int(d["maxima"]), "saddles2": int(d["saddles2"]),
+                    "regions_dsc": int(d["regions_dsc"]), "certain_sweeps_dsc": int(d["certain_sweeps_dsc"]),
+                    "stage_ms": {k[3:]: round(float(np.mean([x[k] for x in stats])), 3) for k in
+                                 ("ms_lstar", "ms_regions", "ms_integrate", "ms_cleanup", "ms_dsc", "ms_grad",
+                                  "ms_total")}}
+    ld = dld.download(np.int32, V)
+    out["unlabelled_fraction"] = float((ld < 0).mean())
+    if not args.no_cpu:
+        import tempfile
+        cores = _cores()
+        slab = max(8, args.label_cpu_slab // 2)
+        sample = np.ascontiguousarray(f[:slab])
+        sz, sy, sx = sample.shape
+        log("timing the needASC2 + needDSC2 CPU baseline (reference pipeline on a slab)")
+        with tempfile.TemporaryDirectory() as tmp:
+            raw = os.path.join(tmp, "slab.raw")
+            sample.tofile(raw)
+            t = _run_reference("ref_ondemand", [sx, sy, sz, raw, os.path.join(tmp, "out"), err, gthr, iters, presimp,
+                                                cores, 0, 0, 1, 1], cores)
+        if t is not None:
+            dt = t["total_ms"] * 1e-3
+            out["cpu_baseline"] = {
+                "value": round(sx * sy * sz / dt / 1e6, 3), "unit": "Mvert/s", "cores": cores, "kind": "reference",
+                "sample": f"{sx}x{sy}x{sz} slab (first {sz} z-planes) of the labelling volume; the reference's "
+                          f"needASC2 + needDSC2 pipeline (OpenMP, built from its sources), {dt:.2f} s "
+                          f"(integrations {t['integrate_ms']:.0f} ms)"}
+            out["gpu_vs_cpu"] = round(out["asc2_dsc2"]["value"] / out["cpu_baseline"]["value"], 1)
     return out
 
 
@@ -662,6 +731,7 @@ def main():
     ap.add_argument("--ideal-mask-probe", action="store_true")
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-lines", action="store_true", help="skip the accurate 1-manifold measurement")
+    ap.add_argument("--no-dsc", action="store_true", help="skip the descending 3-manifold measurement")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-only", action="store_true")
     ap.add_argument("--batch-volumes", type=int, default=512)

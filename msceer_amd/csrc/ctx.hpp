@@ -132,7 +132,7 @@ struct msceer_ctx {
     uint64_t line_stamps_cells = 0;
     msceer::HostBuf host_stage;  // pinned staging of host-side merges (cleanup)
     uint32_t* h_ctl = nullptr;  // pinned mirror of the control block
-    static constexpr int kEvents = 16;
+    static constexpr int kEvents = 18;  // 8, 9: certain-round groups; 12-15: integration passes; 16, 17: needDSC2
     hipEvent_t ev[kEvents] = {};
     msceer_timing last{};
     msceer_label_stats last_labels{};

@@ -474,7 +474,7 @@ int ondemand_impl(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, 
     // the one the ascending branch left (zeroed at its seeds), zeroed at the
     // maxima too, as the reference's shared field is.
     if (dsc2) {
-        CHECK_HIP(hipEventRecord(ev[8], s));
+        CHECK_HIP(hipEventRecord(ev[16], s));
         CHECK_HIP(ctx->label_dsc.ensure(sizeof(int32_t) * V));
         CHECK_HIP(ctx->seedmask2.ensure(V));
         bd.label = ctx->label_dsc.as<int32_t>();
@@ -557,9 +557,9 @@ int ondemand_impl(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, 
                 if (rc) return rc;
             }
         }
-        CHECK_HIP(hipEventRecord(ev[9], s));
-        CHECK_HIP(hipEventSynchronize(ev[9]));
-        st.ms_dsc = elapsed(ev[8], ev[9]);
+        CHECK_HIP(hipEventRecord(ev[17], s));
+        CHECK_HIP(hipEventSynchronize(ev[17]));
+        st.ms_dsc = elapsed(ev[16], ev[17]);
     }
     if (stop_r || stop_i) return finish_early();
 
