@@ -387,8 +387,8 @@ def bench_dsc(args, ctx, lib, dv, dg, f, params):
                     "maxima": int(d["maxima"]), "saddles2": int(d["saddles2"]),
                     "regions_dsc": int(d["regions_dsc"]), "certain_sweeps_dsc": int(d["certain_sweeps_dsc"]),
                     "stage_ms": {k[3:]: round(float(np.mean([x[k] for x in stats])), 3) for k in
-                                 ("ms_lstar", "ms_regions", "ms_integrate", "ms_cleanup", "ms_dsc", "ms_grad",
-                                  "ms_total")}}
+                                 ("ms_lstar", "ms_regions", "ms_integrate", "ms_cleanup", "ms_dsc", "ms_dsc_wait",
+                                  "ms_grad", "ms_total")}}
     ld = dld.download(np.int32, V)
     out["unlabelled_fraction"] = float((ld < 0).mean())
     if not args.no_cpu:

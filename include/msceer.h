@@ -98,7 +98,8 @@ typedef struct msceer_label_stats {
     int64_t saddles2;           /* needDSC2: critical quads */
     int64_t regions_dsc;        /* needDSC2: descending certain regions after simplification */
     int32_t certain_sweeps_dsc; /* needDSC2: certain-region rounds */
-    float ms_dsc;               /* needDSC2: the whole descending branch */
+    float ms_dsc;               /* needDSC2: the whole descending branch (beside the ascending one with needASC2) */
+    float ms_dsc_wait;          /* needDSC2: time its end lies past the ascending branch's */
 } msceer_label_stats;
 
 int msceer_version(void);

@@ -308,6 +308,8 @@ int msceer_destroy(msceer_ctx* ctx) {
     if (!ctx) return MSCEER_EINVAL;
     for (msceer_ctx* w : ctx->workers) msceer_destroy(w);
     ctx->workers.clear();
+    if (ctx->aux) msceer_destroy(ctx->aux);
+    ctx->aux = nullptr;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
     batch_state_free(ctx);

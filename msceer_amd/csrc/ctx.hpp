@@ -132,12 +132,15 @@ struct msceer_ctx {
     uint64_t line_stamps_cells = 0;
     msceer::HostBuf host_stage;  // pinned staging of host-side merges (cleanup)
     uint32_t* h_ctl = nullptr;  // pinned mirror of the control block
-    static constexpr int kEvents = 18;  // 8, 9: certain-round groups; 12-15: integration passes; 16, 17: needDSC2
+    // 8, 9: certain-round groups; 12-15: integration passes; 16, 17: the
+    // descending branch (on the auxiliary context); 18: its start
+    static constexpr int kEvents = 19;
     hipEvent_t ev[kEvents] = {};
     msceer_timing last{};
     msceer_label_stats last_labels{};
     std::shared_ptr<msceer::Tuning> tuning = std::make_shared<msceer::Tuning>();
     std::vector<msceer_ctx*> workers;  // contexts of msceer_batch's worker threads
+    msceer_ctx* aux = nullptr;         // the descending labelling's own stream and buffers
     msceer::BatchState* gbatch = nullptr;  // msceer_batch_grad: buffers, plans and graphs
 };
 

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <thread>
 #include <vector>
 
 #include "ctx.hpp"
@@ -264,6 +265,152 @@ int run_certain_rounds(msceer_ctx* ctx, const VGrid& g, const float* d_values, c
     return MSCEER_OK;
 }
 
+// Inputs of the descending branch.  The values and the base .grad are read
+// only by both branches; list2 holds the base pass's critical quads (n_sad
+// of them, counted in the calling context); asc_gradf / asc_seedmask are the
+// ascending branch's FP32 field and seed mask after its seeding (null
+// without needASC2), ready once `ready` has completed.
+struct DscIn {
+    VGrid g;
+    Dims d;
+    const float* values;
+    const uint8_t* grad;
+    MList list2;
+    uint64_t n_sad;
+    const float4* asc_gradf;
+    const uint8_t* asc_seedmask;
+    hipEvent_t ready;
+    float presimp, grad_thr, err_thr;
+    int32_t max_iters;
+    bool fp32_first, stop_r, stop_i;
+};
+
+// Descending 3-manifolds (needDSC2, ondemandaccurate.cxx:913-991) on the
+// context `cx` (its own stream and buffers, so that it runs beside the
+// ascending branch): the max extremum graph of the steepest pairing
+// (critical hexes joined by interior critical quads), certain regions in the
+// descending order, uphill integration and cleanup in that order.  The
+// gradient field is the one the ascending branch left (zeroed at its seeds),
+// zeroed at the maxima too, as the reference's shared field is.  Labels in
+// cx->label; cx->ev[17] completes with the branch.
+int run_dsc(msceer_ctx* cx, const DscIn& in, msceer_label_stats& st) {
+    CHECK_HIP(hipSetDevice(cx->device));
+    hipStream_t s = cx->stream;
+    const VGrid& g = in.g;
+    const int64_t V = g.verts();
+    CHECK_HIP(hipStreamWaitEvent(s, in.ready, 0));
+    // frontiers only: the critical lists stay the calling context's
+    const ListCaps caps = list_caps(in.d);
+    CHECK_HIP(cx->front0.ensure(front_bytes(caps)));
+    CHECK_HIP(cx->front1.ensure(front_bytes(caps)));
+    int rc = MSCEER_OK;
+    CHECK_HIP(cx->gradf.ensure(sizeof(float4) * V));
+    CHECK_HIP(cx->gradd.ensure(sizeof(double4) * V));
+    DevBuf* i32[] = {&cx->label, &cx->label2, &cx->stamp, &cx->rep, &cx->rep2};
+    for (DevBuf* b : i32) CHECK_HIP(b->ensure(sizeof(int32_t) * V));
+    DevBuf* u8[] = {&cx->dest, &cx->seedmask, &cx->redo};
+    for (DevBuf* b : u8) CHECK_HIP(b->ensure(V));
+    uint32_t* ctl = ctl_dev(cx);
+    LabelBuffers bd{};
+    bd.values = in.values;
+    bd.gradf = cx->gradf.as<float4>();
+    bd.gradd = cx->gradd.as<double4>();
+    bd.label = cx->label.as<int32_t>();
+    bd.label2 = cx->label2.as<int32_t>();
+    bd.dest = cx->dest.as<uint8_t>();
+    bd.seedmask = cx->seedmask.as<uint8_t>();
+    bd.redo = cx->redo.as<uint8_t>();
+    bd.stamp = cx->stamp.as<uint32_t>();
+    bd.rep = cx->rep.as<int32_t>();
+    bd.rep2 = cx->rep2.as<int32_t>();
+    bd.snap = bd.rep;  // free until the cleanup
+    bd.stats = ctl + kCtlIntStats;
+    bd.desc = 1;
+    CHECK_HIP(hipEventRecord(cx->ev[16], s));
+    CHECK_HIP(launch_zero(ctl, kCtlWords, s));
+    const MList maxl = front_list(cx, caps, 0);
+    CHECK_HIP(launch_collect_maxima(g, in.values, in.grad, maxl, ctl + kCtlOverflow, s));
+    std::vector<MaxArcRecord> rec;
+    if (in.n_sad) {
+        CHECK_HIP(cx->host_io.ensure(sizeof(MaxArcRecord) * in.n_sad + 64));
+        MaxArcRecord* d_arcs = cx->host_io.as<MaxArcRecord>();
+        CHECK_HIP(launch_saddle2_arcs(g, in.values, in.grad, in.list2, in.presimp, d_arcs, (uint32_t)in.n_sad,
+                                      ctl + kCtlCount, s));
+        rc = read_ctl(cx, 0, kCtlWords);
+        if (rc) return rc;
+        rec.resize(std::min<uint64_t>(cx->h_ctl[kCtlCount], in.n_sad));
+        if (!rec.empty()) {
+            CHECK_HIP(hipMemcpyAsync(rec.data(), d_arcs, sizeof(MaxArcRecord) * rec.size(), hipMemcpyDeviceToHost, s));
+            CHECK_HIP(hipStreamSynchronize(s));
+        }
+    } else {
+        rc = read_ctl(cx, 0, kCtlWords);
+        if (rc) return rc;
+    }
+    if (cx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;
+    const uint64_t n_max = list_total(cx->h_ctl + kCtlRound, maxl);
+    std::vector<MinNode> nodes;
+    nodes.reserve(2 * rec.size());
+    std::vector<SaddleArc> arcs(rec.size());
+    for (size_t i = 0; i < rec.size(); i++) {
+        nodes.push_back(MinNode{rec[i].max1, rec[i].v1, rec[i].h1});
+        nodes.push_back(MinNode{rec[i].max2, rec[i].v2, rec[i].h2});
+        arcs[i] = SaddleArc{rec[i].saddle, rec[i].max1, rec[i].max2, rec[i].sval};
+    }
+    std::sort(nodes.begin(), nodes.end(), [](const MinNode& x, const MinNode& y) { return x.cell < y.cell; });
+    nodes.erase(std::unique(nodes.begin(), nodes.end(),
+                            [](const MinNode& x, const MinNode& y) { return x.cell == y.cell; }),
+                nodes.end());
+    const std::vector<int> rep = simplify_maxima(nodes, arcs, in.presimp);
+    std::vector<int32_t> seed_v, seed_l;
+    for (size_t i = 0; i < nodes.size(); i++) {
+        if (rep[i] == (int)i) continue;
+        seed_v.push_back((int32_t)nodes[i].vertex);
+        seed_l.push_back((int32_t)nodes[rep[i]].vertex);
+    }
+    st.maxima = (int64_t)n_max;
+    st.saddles2 = (int64_t)in.n_sad;
+    st.regions_dsc = (int64_t)n_max - (int64_t)seed_v.size();
+    CHECK_HIP(launch_fill_i32(bd.label, V, -1, s));
+    CHECK_HIP(launch_fill_u8(bd.seedmask, V, 0, s));
+    if (in.asc_gradf)
+        CHECK_HIP(hipMemcpyAsync(bd.gradf, in.asc_gradf, sizeof(float4) * V, hipMemcpyDeviceToDevice, s));
+    else
+        CHECK_HIP(launch_grad_field(g, in.values, bd.gradf, s));
+    CHECK_HIP(launch_seed_minima(maxl, bd, s));  // every maximum's highest vertex: its own label
+    if (!seed_v.empty()) {
+        CHECK_HIP(cx->host_io.ensure(2 * sizeof(int32_t) * seed_v.size() + 64));
+        int32_t* d_sv = cx->host_io.as<int32_t>();
+        int32_t* d_sl = d_sv + seed_v.size();
+        CHECK_HIP(hipMemcpyAsync(d_sv, seed_v.data(), sizeof(int32_t) * seed_v.size(), hipMemcpyHostToDevice, s));
+        CHECK_HIP(hipMemcpyAsync(d_sl, seed_l.data(), sizeof(int32_t) * seed_l.size(), hipMemcpyHostToDevice, s));
+        CHECK_HIP(launch_seed_scatter(d_sv, d_sl, (int)seed_v.size(), bd, s));
+        CHECK_HIP(hipStreamSynchronize(s));  // host vectors go out of scope
+    }
+    int sweeps = 0;
+    rc = run_certain_rounds(cx, g, in.values, bd, false, &sweeps, true);
+    if (rc) return rc;
+    st.certain_sweeps_dsc = sweeps;
+    if (!in.stop_r) {
+        // the FP64 field of pass 3 is zeroed at every seed so far
+        if (in.asc_seedmask) CHECK_HIP(launch_or_u8(bd.seedmask, in.asc_seedmask, V, s));
+        CHECK_HIP(launch_zero(ctl + kCtlIntStats, kStatNum, s));
+        msceer_label_stats sd{};
+        rc = run_integration(cx, g, bd, nullptr, nullptr, in.grad_thr, in.err_thr, in.max_iters, 1, in.fp32_first, sd);
+        if (rc) return rc;
+        if (!in.stop_i) {
+            rc = run_cleanup(cx, g, bd);
+            if (rc) return rc;
+        }
+    }
+    CHECK_HIP(hipEventRecord(cx->ev[17], s));
+    CHECK_HIP(hipEventSynchronize(cx->ev[17]));
+    float ms = 0.f;
+    CHECK_HIP(hipEventElapsedTime(&ms, cx->ev[16], cx->ev[17]));
+    st.ms_dsc = ms;
+    return MSCEER_OK;
+}
+
 int ondemand_impl(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz, float err_thr,
                   float grad_thr, int32_t max_iters, float presimp, uint32_t need, const uint8_t* integrate_mask,
                   uint8_t* grad_out, int32_t* labels_out, int32_t* labels_dsc_out, int8_t* restriction_out,
@@ -352,9 +499,82 @@ int ondemand_impl(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, 
     CHECK_HIP(launch_lstar(d, lb, s, shape));
     CHECK_HIP(hipEventRecord(ev[1], s));
 
+    // The descending branch runs on an auxiliary context (own stream and
+    // buffers) on a host thread of its own, beside the ascending branch once
+    // that has seeded its regions (its gradient field, zeroed at its seeds,
+    // is the one the descending integration starts from); joined before the
+    // restriction labels.  Without needASC2 it runs in this thread, where the
+    // ascending branch would.
+    LabelBuffers bd = b;
+    DscIn din{};
+    msceer_label_stats st_dsc{};
+    int rc_dsc = MSCEER_OK;
+    std::thread dsc_thread;
+    struct Joiner {
+        std::thread& t;
+        ~Joiner() {
+            if (t.joinable()) t.join();
+        }
+    } joiner{dsc_thread};
+    bool dsc_started = false;
+    if (dsc2) {
+        if (!ctx->aux) {
+            rc = msceer_create(&ctx->aux, ctx->device);
+            if (rc) return rc;
+            ctx->aux->tuning = ctx->tuning;
+        }
+        rc = read_ctl(ctx, 0, kCtlWords);  // the critical quads of the base pass
+        if (rc) return rc;
+        if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;
+        din.g = g;
+        din.d = d;
+        din.values = d_values;
+        din.grad = d_grad;
+        din.list2 = list2;
+        din.n_sad = list_total(ctx->h_ctl + kCtlList2, list2);
+        din.ready = ev[18];
+        din.presimp = presimp;
+        din.grad_thr = grad_thr;
+        din.err_thr = err_thr;
+        din.max_iters = max_iters;
+        din.fp32_first = (flags & MSCEER_FP32_FIRST) != 0;
+        din.stop_r = stop_r;
+        din.stop_i = stop_i;
+    }
+    auto start_dsc = [&](bool threaded) -> int {
+        CHECK_HIP(hipEventRecord(ev[18], s));
+        dsc_started = true;
+        if (!threaded) {
+            rc_dsc = run_dsc(ctx->aux, din, st_dsc);
+            return MSCEER_OK;
+        }
+        try {
+            dsc_thread = std::thread([&]() { rc_dsc = run_dsc(ctx->aux, din, st_dsc); });
+        } catch (...) {
+            dsc_started = false;
+            return MSCEER_ENOMEM;
+        }
+        return MSCEER_OK;
+    };
+    // waits for the descending branch; its labels then read on this stream
+    auto join_dsc = [&]() -> int {
+        if (!dsc_started) return MSCEER_OK;
+        if (dsc_thread.joinable()) dsc_thread.join();
+        dsc_started = false;
+        if (rc_dsc) return rc_dsc;
+        CHECK_HIP(hipStreamWaitEvent(s, ctx->aux->ev[17], 0));
+        CHECK_HIP(hipEventRecord(ev[11], s));
+        bd.label = ctx->aux->label.as<int32_t>();
+        st.maxima = st_dsc.maxima;
+        st.saddles2 = st_dsc.saddles2;
+        st.regions_dsc = st_dsc.regions_dsc;
+        st.certain_sweeps_dsc = st_dsc.certain_sweeps_dsc;
+        st.ms_dsc = st_dsc.ms_dsc;
+        return MSCEER_OK;
+    };
+
     // The STOP flags end every labelling asked for at that stage and return
     // its labels alone.
-    LabelBuffers bd = b;
     auto finish_early = [&]() -> int {
         if (labels_out)
             CHECK_HIP(hipMemcpyAsync(labels_out, b.label, sizeof(int32_t) * V,
@@ -438,13 +658,18 @@ int ondemand_impl(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, 
             CHECK_HIP(hipMemcpyAsync(d_sl, seed_l.data(), sizeof(int32_t) * seed_l.size(), hipMemcpyHostToDevice, s));
             CHECK_HIP(launch_seed_scatter(d_sv, d_sl, (int)seed_v.size(), b, s));
         }
+        if (dsc2) {
+            din.asc_gradf = b.gradf;
+            din.asc_seedmask = b.seedmask;
+            rc = start_dsc(true);
+            if (rc) return rc;
+        }
         // rounds in dependency order
         int sweeps = 0;
         rc = run_certain_rounds(ctx, g, d_values, b, false, &sweeps);
         if (rc) return rc;
         st.certain_sweeps = sweeps;
         CHECK_HIP(hipEventRecord(ev[2], s));
-        if (stop_r && !dsc2) return finish_early();
 
         // ---- integration ----
         if (!stop_r) {
@@ -453,7 +678,6 @@ int ondemand_impl(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, 
             if (rc) return rc;
         }
         CHECK_HIP(hipEventRecord(ev[3], s));
-        if (stop_i && !dsc2) return finish_early();
 
         // ---- cleanup of isolated components ----
         if (!stop_r && !stop_i) {
@@ -465,102 +689,16 @@ int ondemand_impl(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, 
         CHECK_HIP(hipEventRecord(ev[2], s));
         CHECK_HIP(hipEventRecord(ev[3], s));
         CHECK_HIP(hipEventRecord(ev[4], s));
+        if (dsc2) {
+            rc = start_dsc(false);
+            if (rc) return rc;
+        }
     }
 
-    // ---- descending 3-manifolds (needDSC2, ondemandaccurate.cxx:913-991) ----
-    // The max extremum graph of the steepest pairing (critical hexes joined by
-    // interior critical quads), certain regions in the descending order,
-    // uphill integration and cleanup in that order.  The gradient field is
-    // the one the ascending branch left (zeroed at its seeds), zeroed at the
-    // maxima too, as the reference's shared field is.
-    if (dsc2) {
-        CHECK_HIP(hipEventRecord(ev[16], s));
-        CHECK_HIP(ctx->label_dsc.ensure(sizeof(int32_t) * V));
-        CHECK_HIP(ctx->seedmask2.ensure(V));
-        bd.label = ctx->label_dsc.as<int32_t>();
-        bd.seedmask = ctx->seedmask2.as<uint8_t>();
-        bd.desc = 1;
-        const MList maxl = front_list(ctx, caps, 0);
-        CHECK_HIP(launch_zero(maxl.counts, kCountWords, s));
-        CHECK_HIP(launch_zero(ctl + kCtlOverflow, 1, s));
-        CHECK_HIP(launch_collect_maxima(g, d_values, d_grad, maxl, ctl + kCtlOverflow, s));
-        rc = read_ctl(ctx, 0, kCtlWords);
-        if (rc) return rc;
-        if (ctx->h_ctl[kCtlOverflow]) return MSCEER_ETOOBIG;
-        const uint64_t n_max = list_total(ctx->h_ctl + kCtlRound, maxl);
-        const uint64_t n_sad = list_total(ctx->h_ctl + kCtlList2, list2);
-        std::vector<MaxArcRecord> rec;
-        if (n_sad) {
-            CHECK_HIP(ctx->host_io.ensure(sizeof(MaxArcRecord) * n_sad + 64));
-            MaxArcRecord* d_arcs = ctx->host_io.as<MaxArcRecord>();
-            CHECK_HIP(launch_zero(ctl + kCtlCount, 1, s));
-            CHECK_HIP(launch_saddle2_arcs(g, d_values, d_grad, list2, presimp, d_arcs, (uint32_t)n_sad, ctl + kCtlCount,
-                                          s));
-            rc = read_ctl(ctx, kCtlCount, 1);
-            if (rc) return rc;
-            rec.resize(std::min<uint64_t>(ctx->h_ctl[kCtlCount], n_sad));
-            if (!rec.empty()) {
-                CHECK_HIP(hipMemcpyAsync(rec.data(), d_arcs, sizeof(MaxArcRecord) * rec.size(), hipMemcpyDeviceToHost,
-                                         s));
-                CHECK_HIP(hipStreamSynchronize(s));
-            }
-        }
-        std::vector<MinNode> nodes;
-        nodes.reserve(2 * rec.size());
-        std::vector<SaddleArc> arcs(rec.size());
-        for (size_t i = 0; i < rec.size(); i++) {
-            nodes.push_back(MinNode{rec[i].max1, rec[i].v1, rec[i].h1});
-            nodes.push_back(MinNode{rec[i].max2, rec[i].v2, rec[i].h2});
-            arcs[i] = SaddleArc{rec[i].saddle, rec[i].max1, rec[i].max2, rec[i].sval};
-        }
-        std::sort(nodes.begin(), nodes.end(), [](const MinNode& x, const MinNode& y) { return x.cell < y.cell; });
-        nodes.erase(std::unique(nodes.begin(), nodes.end(),
-                                [](const MinNode& x, const MinNode& y) { return x.cell == y.cell; }),
-                    nodes.end());
-        const std::vector<int> rep = simplify_maxima(nodes, arcs, presimp);
-        std::vector<int32_t> seed_v, seed_l;
-        for (size_t i = 0; i < nodes.size(); i++) {
-            if (rep[i] == (int)i) continue;
-            seed_v.push_back((int32_t)nodes[i].vertex);
-            seed_l.push_back((int32_t)nodes[rep[i]].vertex);
-        }
-        st.maxima = (int64_t)n_max;
-        st.saddles2 = (int64_t)n_sad;
-        st.regions_dsc = (int64_t)n_max - (int64_t)seed_v.size();
-        CHECK_HIP(launch_fill_i32(bd.label, V, -1, s));
-        CHECK_HIP(launch_fill_u8(bd.seedmask, V, 0, s));
-        if (!asc2) CHECK_HIP(launch_grad_field(g, d_values, b.gradf, s));
-        CHECK_HIP(launch_seed_minima(maxl, bd, s));  // every maximum's highest vertex: its own label
-        if (!seed_v.empty()) {
-            CHECK_HIP(ctx->host_io.ensure(2 * sizeof(int32_t) * seed_v.size() + 64));
-            int32_t* d_sv = ctx->host_io.as<int32_t>();
-            int32_t* d_sl = d_sv + seed_v.size();
-            CHECK_HIP(hipMemcpyAsync(d_sv, seed_v.data(), sizeof(int32_t) * seed_v.size(), hipMemcpyHostToDevice, s));
-            CHECK_HIP(hipMemcpyAsync(d_sl, seed_l.data(), sizeof(int32_t) * seed_l.size(), hipMemcpyHostToDevice, s));
-            CHECK_HIP(launch_seed_scatter(d_sv, d_sl, (int)seed_v.size(), bd, s));
-            CHECK_HIP(hipStreamSynchronize(s));  // host vectors go out of scope
-        }
-        int sweeps = 0;
-        rc = run_certain_rounds(ctx, g, d_values, bd, false, &sweeps, true);
-        if (rc) return rc;
-        st.certain_sweeps_dsc = sweeps;
-        if (!stop_r) {
-            // the FP64 field of pass 3 is zeroed at every seed so far
-            if (asc2) CHECK_HIP(launch_or_u8(bd.seedmask, b.seedmask, V, s));
-            CHECK_HIP(launch_zero(ctl + kCtlIntStats, kStatNum, s));
-            msceer_label_stats sd{};
-            rc = run_integration(ctx, g, bd, nullptr, nullptr, grad_thr, err_thr, max_iters, 1,
-                                 (flags & MSCEER_FP32_FIRST) != 0, sd);
-            if (rc) return rc;
-            if (!stop_i) {
-                rc = run_cleanup(ctx, g, bd);
-                if (rc) return rc;
-            }
-        }
-        CHECK_HIP(hipEventRecord(ev[17], s));
-        CHECK_HIP(hipEventSynchronize(ev[17]));
-        st.ms_dsc = elapsed(ev[16], ev[17]);
-    }
+    // ---- join the descending branch ----
+    CHECK_HIP(hipEventRecord(ev[11], s));
+    rc = join_dsc();  // records ev[11] again after the wait
+    if (rc) return rc;
     if (stop_r || stop_i) return finish_early();
 
     // ---- restriction labels, 1-manifolds, conforming gradient, ldir ----
@@ -577,13 +715,15 @@ int ondemand_impl(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, 
                 CHECK_HIP(launch_fill_u8(b.seedmask, V, 0, s));
                 CHECK_HIP(launch_grad_field64(g, b, s));
             }
+            // after needDSC2 the descending branch's field, zeroed at both families' seeds
+            const double4* field = dsc2 ? ctx->aux->gradd.as<double4>() : b.gradd;
             LineStats la, ld;
             if (need & MSCEER_NEED_ASC1) {
-                rc = run_lines(ctx, d, d_values, d_grad, b.gradd, restr, true, grad_thr, err_thr, max_iters, &la);
+                rc = run_lines(ctx, d, d_values, d_grad, field, restr, true, grad_thr, err_thr, max_iters, &la);
                 if (rc) return rc;
             }
             if (need & MSCEER_NEED_DSC1) {
-                rc = run_lines(ctx, d, d_values, d_grad, b.gradd, restr, false, grad_thr, err_thr, max_iters, &ld);
+                rc = run_lines(ctx, d, d_values, d_grad, field, restr, false, grad_thr, err_thr, max_iters, &ld);
                 if (rc) return rc;
             }
             st.lines_asc = (int64_t)la.traced;
@@ -631,7 +771,8 @@ int ondemand_impl(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, 
     st.ms_lstar = elapsed(ev[0], ev[1]) + elapsed(ev[10], ev[6]);
     st.ms_regions = elapsed(ev[1], ev[2]);
     st.ms_integrate = elapsed(ev[2], ev[3]);
-    st.ms_cleanup = elapsed(ev[3], ev[4]) + elapsed(ev[4], ev[5]) - st.ms_dsc;  // the restriction labels
+    st.ms_cleanup = elapsed(ev[3], ev[4]) + elapsed(ev[11], ev[5]);  // and the restriction labels
+    st.ms_dsc_wait = elapsed(ev[4], ev[11]);
     st.ms_lines = elapsed(ev[5], ev[10]);
     st.ms_grad = elapsed(ev[10], ev[7]);
     st.ms_total = elapsed(ev[0], ev[7]);

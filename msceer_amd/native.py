@@ -123,6 +123,7 @@ class LabelStats(ctypes.Structure):
         ("regions_dsc", ctypes.c_int64),
         ("certain_sweeps_dsc", ctypes.c_int32),
         ("ms_dsc", ctypes.c_float),
+        ("ms_dsc_wait", ctypes.c_float),
     ]
 
     def as_dict(self):

@@ -189,3 +189,24 @@ def test_larger_volume_against_the_oracle(ctx):
     assert (restr == r).all(), int((restr != r).sum())
     expect = oracle_lib.flags(oracle_lib.ldir(oracle_lib.lstar_pairs(f, r), f.shape), f.shape)
     assert (grad == expect).all(), int((grad != expect).sum())
+
+
+@pytest.mark.parametrize("flags", [(1, 1, 0, 1), (1, 1, 1, 1)])
+def test_one_manifolds_after_the_descending_labels(ctx, flags):
+    """needASC1 / needDSC1 with needDSC2: the digitized 1-manifolds follow
+    the FP64 field zeroed at the seeds of every labelling that ran (the
+    descending branch's field): cells 4 and 5 equal the FP64 oracle's."""
+    f, params, _, _, _ = load_dsc2("smooth_34x33x32_0001")
+    Z, Y, X = f.shape
+    _say(f"dsc2 with 1-manifolds {flags}")
+    grad, la, ld, restr, st = _run(ctx, f, params, flags)
+    _, ola, old, o_r, ost = oracle_lib.ondemand2(f, *params, *flags, mode=1)
+    for t in (4, 5):
+        assert ((restr == t) == (o_r == t)).all(), (t, int(((restr == t) != (o_r == t)).sum()))
+    assert float((ld != old.ravel()).mean()) <= 1e-3
+    if flags[2]:
+        assert float((la != ola.ravel()).mean()) <= 1e-3
+    assert st["lines_asc"] >= ost[5] and st["lines_dsc"] >= ost[13]
+    expect = oracle_lib.flags(oracle_lib.ldir(oracle_lib.lstar_pairs(f, restr), f.shape), f.shape)
+    assert (grad == expect).all(), int((grad != expect).sum())
+    gradcheck.check_consistency(grad, X, Y, Z)
