@@ -48,6 +48,11 @@ extern "C" {
  * budget; default: pass 3 wholly in FP64.  No counterpart in the reference,
  * whose advector has one fixed mixed precision. */
 #define MSCEER_FP32_FIRST 0x20u
+/* msceer_lstar_grad: the .grad file of the tiled program
+ * (save_discrete_gradient_to_file1, steepest_lstar_tiled/
+ * localized_morse_complex.h:1063-1133): pairs and the boundary flag, the
+ * ascending-manifold dimension (bits 5-7) left 0; ldir is not run */
+#define MSCEER_TILED_LAYOUT 0x40u
 
 typedef struct msceer_ctx msceer_ctx;
 

@@ -230,7 +230,9 @@ int lstar_grad_device(msceer_ctx* ctx, const Dims& d, const float* d_values, uin
 
     int rounds = 0;
     if (timing) CHECK_HIP(hipEventRecord(ctx->ev[3], s));
-    if (!(flags & MSCEER_SKIP_LDIR)) {
+    if (flags & MSCEER_TILED_LAYOUT) {
+        CHECK_HIP(launch_clear_ldir_bits(d_grad, (int64_t)d.cells, s));
+    } else if (!(flags & MSCEER_SKIP_LDIR)) {
         rc = run_ldir(ctx, d, d_grad, caps, lists_ok, &rounds);
         if (rc) return rc;
     }
@@ -348,7 +350,7 @@ int msceer_last_timing(const msceer_ctx* ctx, msceer_timing* out) {
 int msceer_lstar_grad(msceer_ctx* ctx, const float* values, int64_t nx, int64_t ny, int64_t nz,
                       uint8_t* grad_out, uint32_t flags) {
     if (!ctx || !values || !grad_out) return MSCEER_EINVAL;
-    if (flags & ~(MSCEER_DEVICE_PTRS | MSCEER_SKIP_LDIR | MSCEER_TIMING)) return MSCEER_EINVAL;
+    if (flags & ~(MSCEER_DEVICE_PTRS | MSCEER_SKIP_LDIR | MSCEER_TIMING | MSCEER_TILED_LAYOUT)) return MSCEER_EINVAL;
     int rc = check_dims(nx, ny, nz);
     if (rc) return rc;
     CHECK_HIP(hipSetDevice(ctx->device));

@@ -62,6 +62,8 @@ struct LdirScratch {
     uint32_t bit_words;  // words of one bitmap
 };
 size_t ldir_scratch_bytes(const Dims& d, LdirScratch* s);
+// Clears the ascending-manifold dimension bits (5-7) of n .grad bytes.
+hipError_t launch_clear_ldir_bits(uint8_t* grad, int64_t n, hipStream_t st);
 // Records of the walks (from the lower-star bytes) and cleared bitmaps.
 hipError_t launch_ldir_records(const Dims& d, const uint8_t* grad, void* scratch, const LdirScratch& s,
                                hipStream_t st);

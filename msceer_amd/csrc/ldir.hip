@@ -41,6 +41,9 @@
 // + axis, axis = the odd axis of an edge or the even axis of a quad.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cstdint>
+
 #include "internal.hpp"
 #include "gbatch.hpp"
 #include "mlist.hpp"
@@ -911,6 +914,30 @@ static void tiles_of(const Dims& d, int& ntx, int& nty, uint32_t& n) {
     nty = (d.Y + kTY - 1) / kTY;
     const int ntz = (d.Z + kTZ - 1) / kTZ;
     n = (uint32_t)ntx * (uint32_t)nty * (uint32_t)ntz;
+}
+
+__global__ void __launch_bounds__(256) clear_ldir_words_kernel(uint32_t* __restrict__ w, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) w[i] &= 0x1F1F1F1Fu;
+}
+
+__global__ void __launch_bounds__(256) clear_ldir_bytes_kernel(uint8_t* __restrict__ g, int64_t n) {
+    for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (int64_t)gridDim.x * 256) g[i] &= 0x1Fu;
+}
+
+hipError_t launch_clear_ldir_bits(uint8_t* grad, int64_t n, hipStream_t st) {
+    if (n <= 0) return hipSuccess;
+    // 4-byte words where the buffer is aligned, bytes for the rest
+    const int64_t lead = (int64_t)((4 - (reinterpret_cast<uintptr_t>(grad) & 3)) & 3);
+    const int64_t head = lead < n ? lead : n;
+    const int64_t words = (n - head) / 4;
+    const int64_t tail = n - head - 4 * words;
+    auto blocks = [](int64_t m) { return (unsigned)std::min<int64_t>((m + 255) / 256, 8192); };
+    if (head) hipLaunchKernelGGL(clear_ldir_bytes_kernel, dim3(1), dim3(256), 0, st, grad, head);
+    if (words)
+        hipLaunchKernelGGL(clear_ldir_words_kernel, dim3(blocks(words)), dim3(256), 0, st,
+                           reinterpret_cast<uint32_t*>(grad + head), words);
+    if (tail) hipLaunchKernelGGL(clear_ldir_bytes_kernel, dim3(1), dim3(256), 0, st, grad + head + 4 * words, tail);
+    return hipGetLastError();
 }
 
 hipError_t launch_ldir_records(const Dims& d, const uint8_t* grad, void* scratch, const LdirScratch& s,

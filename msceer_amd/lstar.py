@@ -21,8 +21,11 @@ def num_cells(X: int, Y: int, Z: int) -> int:
 
 
 def steepest_lstar_grad(values: np.ndarray, ctx: native.Context | None = None,
-                        skip_ldir: bool = False, timing: bool = False) -> np.ndarray:
-    """.grad bytes of a (Z, Y, X) float32 volume, shape (2Z-1, 2Y-1, 2X-1)."""
+                        skip_ldir: bool = False, timing: bool = False, tiled_layout: bool = False) -> np.ndarray:
+    """.grad bytes of a (Z, Y, X) float32 volume, shape (2Z-1, 2Y-1, 2X-1).
+
+    tiled_layout: the tiled program's file (pairs and boundary flag, the
+    ascending-manifold dimension bits 0)."""
     v = np.ascontiguousarray(values, dtype=np.float32)
     if v.ndim != 3:
         raise ValueError("values must be a (Z, Y, X) array")
@@ -31,7 +34,8 @@ def steepest_lstar_grad(values: np.ndarray, ctx: native.Context | None = None,
     ctx = ctx or native.Context()
     try:
         out = np.empty(num_cells(X, Y, Z), dtype=np.uint8)
-        flags = (native.SKIP_LDIR if skip_ldir else 0) | (native.TIMING if timing else 0)
+        flags = ((native.SKIP_LDIR if skip_ldir else 0) | (native.TIMING if timing else 0) |
+                 (native.TILED_LAYOUT if tiled_layout else 0))
         rc = native.lib().msceer_lstar_grad(ctx.handle, v.ctypes.data_as(ctypes.c_void_p), X, Y, Z,
                                             out.ctypes.data_as(ctypes.c_void_p), flags)
         native.check(rc, "msceer_lstar_grad")

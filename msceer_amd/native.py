@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(PKG, "libmsceer_hip.so")
 
 DEVICE_PTRS = 0x1
 SKIP_LDIR = 0x2
+TILED_LAYOUT = 0x40
 TIMING = 0x4
 STOP_REGIONS = 0x8
 STOP_INTEGRATED = 0x10
