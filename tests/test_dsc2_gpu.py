@@ -210,3 +210,39 @@ def test_one_manifolds_after_the_descending_labels(ctx, flags):
     expect = oracle_lib.flags(oracle_lib.ldir(oracle_lib.lstar_pairs(f, restr), f.shape), f.shape)
     assert (grad == expect).all(), int((grad != expect).sum())
     gradcheck.check_consistency(grad, X, Y, Z)
+
+
+def test_labelling_volume_256(ctx):
+    """The 256^3 labelling volume (200 Gaussians + sine, seed 303), needASC2 +
+    needDSC2 beside each other: the descending certain regions bit-exact
+    against the oracle's at full size, kept through the integration; the
+    restriction labels and the .grad a pure function of the labels."""
+    import ctypes as C
+    from golden import C3_PARAMS
+    from msceer_amd import synth
+    f = synth.labels_field(303, 256).astype(np.float32)
+    Z, Y, X = f.shape
+    flags = (0, 0, 1, 1)
+    _say("dsc2 256^3: certain regions")
+    _, _, l0, _, _ = _run(ctx, f, C3_PARAMS, flags, stop="regions")
+    base = oracle_lib.lstar_pairs(f)
+    lab = np.full(f.size, -1, np.int32)
+    seeds = np.zeros(f.size, np.int8)
+    h = oracle_lib.lib()
+    h.oracle_certain_regions_dsc.argtypes = [C.c_void_p, C.c_longlong, C.c_longlong, C.c_longlong, C.c_void_p,
+                                             C.c_float, C.c_void_p, C.c_void_p]
+    p = lambda a: a.ctypes.data_as(C.c_void_p)
+    h.oracle_certain_regions_dsc(p(f), X, Y, Z, p(base), float(C3_PARAMS[3]), p(lab), p(seeds))
+    assert (l0 == lab).all(), int((l0 != lab).sum())
+    _say("dsc2 256^3: integrated")
+    _, _, l1, _, _ = _run(ctx, f, C3_PARAMS, flags, stop="integrated")
+    c = lab >= 0
+    assert (l1[c] == lab[c]).all()
+    _say("dsc2 256^3: full")
+    grad, la, ld, restr, st = _run(ctx, f, C3_PARAMS, flags)
+    _say(f"  maxima {st['maxima']} regions {st['regions_dsc']} ms_dsc {st['ms_dsc']:.3f} "
+         f"ms_total {st['ms_total']:.3f} unlabelled {float((ld < 0).mean()):.4f}")
+    r = oracle_lib.max_boundary(f, ld, oracle_lib.min_boundary(la, f.shape))
+    assert (restr == r).all(), int((restr != r).sum())
+    expect = oracle_lib.flags(oracle_lib.ldir(oracle_lib.lstar_pairs(f, r), f.shape), f.shape)
+    assert (grad == expect).all(), int((grad != expect).sum())
